@@ -1,0 +1,190 @@
+// detmath.h - deterministic double-precision log, sin and cos.
+//
+// The transport path needs log (flight distances, exponential sampling) and
+// sin/cos (azimuthal angles) to give the same bits on the x86-64 host and on
+// gfx950.  Vendor libms differ by an ulp here and there, so both sides use
+// these routines, built only from IEEE +,-,*,/ and integer bit operations.
+// Every translation unit that includes this header must be compiled with
+// -ffp-contract=off so that no multiply-add is fused.
+//
+// Methods (published, re-derived here; coefficients are plain Taylor terms):
+//  * log: argument reduction x = 2^k * m, m in [sqrt(2)/2, sqrt(2)); with
+//    f = m - 1 and s = f / (2 + f), log(m) = 2 atanh(s) evaluated as
+//    f - hfsq + s (hfsq + R(s^2)), hfsq = f^2/2, R(z) = sum_{n>=1} 2 z^n/(2n+1)
+//    (the Sun fdlibm / Tang formulation of the atanh series).  ln 2 is split
+//    into a 32-bit head and a tail so that k * ln2_hi is exact.
+//  * sin/cos: Cody-Waite reduction by pi/2 with a three-term split of pi/2,
+//    then Taylor polynomials on |r| <= pi/4 (through r^19 for sin and r^20
+//    for cos) with the compensated 1 - r^2/2 step for cos.
+// Accuracy is checked against glibc by tests/test_detmath.py (<= 1 ulp over
+// the ranges the transport path uses).
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define DM_HD __host__ __device__ __forceinline__
+#else
+#define DM_HD static inline
+#endif
+
+DM_HD uint64_t dm_bits(double x) {
+  union {
+    double d;
+    uint64_t u;
+  } v;
+  v.d = x;
+  return v.u;
+}
+
+DM_HD double dm_from_bits(uint64_t u) {
+  union {
+    double d;
+    uint64_t u;
+  } v;
+  v.u = u;
+  return v.d;
+}
+
+// Natural logarithm.  Returns -inf for +-0, NaN for negative or NaN input and
+// +inf for +inf.
+DM_HD double dm_log(double x) {
+  const double ln2_hi = 6.93147180369123816490e-01; /* 0x3fe62e42fee00000 */
+  const double ln2_lo = 1.90821492927058770002e-10; /* 0x3dea39ef35793c76 */
+  const double two54 = 18014398509481984.0;         /* 2^54 */
+
+  uint64_t b = dm_bits(x);
+  if (x != x) return x;
+  if (x <= 0.0) {
+    if (x == 0.0) return -1.0 / 0.0;
+    return (x - x) / 0.0;
+  }
+  if (b == 0x7ff0000000000000ULL) return x;
+
+  int k = 0;
+  if ((b >> 52) == 0) {  // subnormal: scale into the normal range
+    x *= two54;
+    b = dm_bits(x);
+    k -= 54;
+  }
+  k += (int)((b >> 52) & 0x7ff) - 1023;
+  // mantissa in [1, 2)
+  uint64_t mb = (b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL;
+  // sqrt(2) = 0x3ff6a09e667f3bcd
+  if (mb > 0x3ff6a09e667f3bcdULL) {
+    mb -= 0x0010000000000000ULL;  // m /= 2
+    k += 1;
+  }
+  const double m = dm_from_bits(mb);
+  const double f = m - 1.0;
+  const double s = f / (2.0 + f);
+  const double z = s * s;
+  // R(z) = 2z/3 + 2z^2/5 + ... + 2z^12/25, Horner form.
+  double R = 2.0 / 25.0;
+  R = 2.0 / 23.0 + z * R;
+  R = 2.0 / 21.0 + z * R;
+  R = 2.0 / 19.0 + z * R;
+  R = 2.0 / 17.0 + z * R;
+  R = 2.0 / 15.0 + z * R;
+  R = 2.0 / 13.0 + z * R;
+  R = 2.0 / 11.0 + z * R;
+  R = 2.0 / 9.0 + z * R;
+  R = 2.0 / 7.0 + z * R;
+  R = 2.0 / 5.0 + z * R;
+  R = 2.0 / 3.0 + z * R;
+  R = z * R;
+  const double hfsq = 0.5 * f * f;
+  const double dk = (double)k;
+  return dk * ln2_hi - ((hfsq - (s * (hfsq + R) + dk * ln2_lo)) - f);
+}
+
+// sin and cos of r, |r| <= pi/4 (Taylor kernels).
+DM_HD double dm_ksin(double r) {
+  const double z = r * r;
+  double p = -1.0 / 121645100408832000.0;  // -1/19!
+  p = 1.0 / 355687428096000.0 + z * p;     //  1/17!
+  p = -1.0 / 1307674368000.0 + z * p;      // -1/15!
+  p = 1.0 / 6227020800.0 + z * p;          //  1/13!
+  p = -1.0 / 39916800.0 + z * p;           // -1/11!
+  p = 1.0 / 362880.0 + z * p;              //  1/9!
+  p = -1.0 / 5040.0 + z * p;               // -1/7!
+  p = 1.0 / 120.0 + z * p;                 //  1/5!
+  p = -1.0 / 6.0 + z * p;                  // -1/3!
+  return r + (r * z) * p;
+}
+
+DM_HD double dm_kcos(double r) {
+  const double z = r * r;
+  // C(z) = 1/4! - z/6! + ... + z^8/20!
+  double p = 1.0 / 2432902008176640000.0;  //  1/20!
+  p = -1.0 / 6402373705728000.0 + z * p;   // -1/18!
+  p = 1.0 / 20922789888000.0 + z * p;      //  1/16!
+  p = -1.0 / 87178291200.0 + z * p;        // -1/14!
+  p = 1.0 / 479001600.0 + z * p;           //  1/12!
+  p = -1.0 / 3628800.0 + z * p;            // -1/10!
+  p = 1.0 / 40320.0 + z * p;               //  1/8!
+  p = -1.0 / 720.0 + z * p;                // -1/6!
+  p = 1.0 / 24.0 + z * p;                  //  1/4!
+  const double hz = 0.5 * z;
+  const double w = 1.0 - hz;
+  return w + (((1.0 - w) - hz) + (z * z) * p);
+}
+
+// sin and cos together.  Accurate for |x| up to about 1e5 (the reduction
+// uses a three-part Cody-Waite split of pi/2); the transport path only
+// calls it on [0, 2 pi].
+DM_HD void dm_sincos(double x, double* sn, double* cs) {
+  const double invpio2 = 6.36619772367581382433e-01; /* 2/pi */
+  const double pio2_1 = 1.57079632673412561417e+00;  /* first 33 bits */
+  const double pio2_2 = 6.07710050630396597660e-11;  /* next 33 bits */
+  const double pio2_3 = 2.02226624879595063154e-21;  /* next 33 bits */
+  const double pio4 = 7.85398163397448278999e-01;
+  double ax = x < 0.0 ? -x : x;
+  if (!(ax == ax) || ax == 1.0 / 0.0) {  // NaN or inf
+    *sn = x - x;
+    *cs = x - x;
+    return;
+  }
+  if (ax <= pio4) {
+    *sn = dm_ksin(x);
+    *cs = dm_kcos(x);
+    return;
+  }
+  // n = nearest integer to x * 2/pi (ties away from zero)
+  double t = x * invpio2;
+  double fn = t >= 0.0 ? (double)(int64_t)(t + 0.5) : -(double)(int64_t)(0.5 - t);
+  const double r = ((x - fn * pio2_1) - fn * pio2_2) - fn * pio2_3;
+  const int64_t n = (int64_t)fn;
+  const double s = dm_ksin(r);
+  const double c = dm_kcos(r);
+  switch ((int)(n & 3)) {
+    case 0:
+      *sn = s;
+      *cs = c;
+      break;
+    case 1:
+      *sn = c;
+      *cs = -s;
+      break;
+    case 2:
+      *sn = -s;
+      *cs = -c;
+      break;
+    default:
+      *sn = -c;
+      *cs = s;
+      break;
+  }
+}
+
+DM_HD double dm_sin(double x) {
+  double s, c;
+  dm_sincos(x, &s, &c);
+  return s;
+}
+
+DM_HD double dm_cos(double x) {
+  double s, c;
+  dm_sincos(x, &s, &c);
+  return c;
+}

@@ -1,0 +1,114 @@
+// devtypes.h - flattened, HBM-resident problem tables.
+//
+// Built once by abx_create() from abx_problem_desc and uploaded as one
+// allocation; kernels get a DevProblem by value (pointers into that block).
+// Layout choices for gfx950:
+//  * surfaces are 48-byte records (4 doubles + type/bc) so one 16-byte-aligned
+//    burst per surface; cells keep their RPN tokens contiguous;
+//  * per-material cross sections are [mat][G] rows, scatter / chi CDFs
+//    [mat][G][G] (row = incoming group), small enough (< 40 KB at G = 33 with
+//    a handful of materials) to stay resident in L2 / LDS;
+//  * mesh tallies are FP64 arrays [Ne][Nx][Ny][Nz] like MeshTally::tally_gen.
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define ABX_HD __host__ __device__ __forceinline__
+#else
+#define ABX_HD static inline
+#endif
+
+#define ABX_MAXPADS 10   // lily pads per tracker (C3 needs 6)
+#define ABX_MAXLVL 4     // coordinate frames (root + 3 nested lattices)
+#define ABX_SEC_CAP 64   // secondary stack per history slot
+#define ABX_INF 1.7976931348623157e308
+#define ABX_PI 3.14159265358979323846264338327950288
+#define ABX_SURFACE_COINCIDENT 1E-12
+#define ABX_BOUNDRY_TOL (500. * 1E-12)
+
+struct DSurf {
+  double p[4];
+  int32_t type;
+  int32_t bc;
+  int32_t pad_[2];
+};
+
+struct DCell {
+  int32_t rpn_off, rpn_len;
+  int32_t simple;
+  int32_t fill_type, fill;
+  int32_t has_bc;
+  int32_t redescend;  // universe index used when re-descending from this cell's pad
+  uint32_t id;
+};
+
+struct DUniv {
+  int32_t type;
+  int32_t cells_off, ncells;
+  int32_t has_bc;
+  int32_t Nx, Ny, Nz;
+  int32_t elems_off;
+  int32_t outer;
+  int32_t pad_;
+  double Px, Py, Pz, Pxi, Pyi, Pzi, Xl, Yl, Zl;
+};
+
+struct DTally {
+  int32_t estimator, quantity;
+  int32_t Nx, Ny, Nz, Ne;
+  int32_t eb_off;   // into DevProblem.tally_eb
+  int32_t pad_;
+  int64_t data_off; // into tally data array (doubles)
+  double lx, ly, lz, hx, hy, hz;
+  double dx, dy, dz, dxi, dyi, dzi;
+  double net_weight;
+};
+
+struct DevProblem {
+  const DSurf* surf;
+  const DCell* cell;
+  const int32_t* rpn;
+  const DUniv* univ;
+  const int32_t* ucells;
+  const int32_t* elems;
+  int32_t root;
+  int32_t nmat;
+  int32_t G;
+  int32_t mode;
+  int32_t tracking;
+  int32_t ntally;
+  int32_t n_dgrp_max;
+  int32_t precision;
+  double wgt_cutoff, wgt_survival, wgt_split;
+  double min_energy, max_energy;
+  uint64_t seed, stride;
+  // energy groups
+  const double* ebounds;   // [G+1]
+  const double* emid;      // [G] 0.5 * (b_g + b_{g+1})
+  const double* majorant;  // [G]
+  const double* sampling;  // [G] carter sampling xs
+  // per material [nmat][G]
+  const double* Et;
+  const double* Eabs;    // fission + (Ea - Ef), the micro absorption
+  const double* Ef;
+  const double* nu_tot;
+  const double* nu_d;
+  const double* Eel;     // scatter row sums
+  const double* vEf;     // nu_total * fission
+  const int32_t* fissile;  // [nmat]
+  const double* scat_cdf;  // [nmat][G][G]
+  const double* chi_cdf;   // [nmat][G][G]
+  const int32_t* ang_off;  // [nmat][G][G] + 1 into ang_mu/pdf/cdf
+  const double* ang_mu;
+  const double* ang_pdf;
+  const double* ang_cdf;
+  const int32_t* dgrp_n;   // [nmat] delayed groups
+  const double* dgrp_cdf;  // [nmat][n_dgrp_max]
+  // tallies
+  const DTally* tally;
+  const double* tally_eb;
+  double* tally_data;
+  // per-group tally energy bin of the group midpoint, [ntally][G]
+  const int32_t* tally_gbin;
+};

@@ -1,0 +1,558 @@
+// geometry.h - CSG navigation on the flattened tables (device code).
+//
+// Restates, on SoA tables and a fixed-size per-particle cursor:
+//  * Surface::{sign,distance,norm}: src/xplane.cpp:32-56 (y/z alike),
+//    src/plane.cpp:33-60, src/{x,y,z}cylinder.cpp (e.g. src/zcylinder.cpp:33-80),
+//    src/sphere.cpp:33-80 - same operand order, same 1e-12 tolerances;
+//  * Cell::is_inside / distance_to_boundary(_condition): src/cell.cpp:71-193;
+//  * RectLattice::get_tile / distance_to_tile_boundary / tile_center:
+//    src/rect_lattice.cpp:209-310;
+//  * CellUniverse / RectLattice ::get_cell with the lily-pad stack and
+//    Tracker::{move,get_current,get_boundary_condition,get_nearest_boundary}:
+//    include/simulation/tracker.hpp:41-372, src/cell_universe.cpp:72-150,
+//    src/lattice.cpp:77-91.
+//
+// The reference's vector-of-pads becomes a bounded cursor: pads (type,
+// index, frame, tile) and one coordinate frame per nesting level.  Pads of
+// one level share a frame because the reference moves every pad by the same
+// d*u (tracker.hpp:76-85), so their r_local stay bit-identical; local
+// positions are advanced incrementally exactly like the reference.
+#pragma once
+
+#include <math.h>
+
+#include "devtypes.h"
+
+struct V3 {
+  double x, y, z;
+};
+
+ABX_HD V3 vadd(const V3& a, const V3& b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+ABX_HD V3 vsub(const V3& a, const V3& b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+ABX_HD V3 vscale(const V3& v, double d) { return V3{v.x * d, v.y * d, v.z * d}; }
+ABX_HD double vdot(const V3& a, const V3& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+ABX_HD double vnorm(const V3& a) { return sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+// Direction(x, y, z): normalising constructor (include/utils/direction.hpp:37-42)
+ABX_HD V3 vdir(double x, double y, double z) {
+  const double m = sqrt(x * x + y * y + z * z);
+  return V3{x / m, y / m, z / m};
+}
+ABX_HD double dabs(double x) { return fabs(x); }
+
+// ---------------------------------------------------------------- surfaces
+ABX_HD V3 surf_norm(const DSurf& s, const V3& r) {
+  switch (s.type) {
+    case 0: return vdir(1., 0., 0.);
+    case 1: return vdir(0., 1., 0.);
+    case 2: return vdir(0., 0., 1.);
+    case 3: return vdir(s.p[0], s.p[1], s.p[2]);
+    case 4: return vdir(0., r.y - s.p[0], r.z - s.p[1]);
+    case 5: return vdir(r.x - s.p[0], 0., r.z - s.p[1]);
+    case 6: return vdir(r.x - s.p[0], r.y - s.p[1], 0.);
+    default: return vdir(r.x - s.p[0], r.y - s.p[1], r.z - s.p[2]);
+  }
+}
+
+ABX_HD int surf_sign(const DSurf& s, const V3& r, const V3& u) {
+  double eval;
+  switch (s.type) {
+    case 0: eval = r.x - s.p[0]; break;
+    case 1: eval = r.y - s.p[0]; break;
+    case 2: eval = r.z - s.p[0]; break;
+    case 3: eval = s.p[0] * r.x + s.p[1] * r.y + s.p[2] * r.z - s.p[3]; break;
+    case 4: {
+      const double y = r.y - s.p[0], z = r.z - s.p[1];
+      eval = y * y + z * z - s.p[2] * s.p[2];
+      break;
+    }
+    case 5: {
+      const double x = r.x - s.p[0], z = r.z - s.p[1];
+      eval = x * x + z * z - s.p[2] * s.p[2];
+      break;
+    }
+    case 6: {
+      const double x = r.x - s.p[0], y = r.y - s.p[1];
+      eval = y * y + x * x - s.p[2] * s.p[2];
+      break;
+    }
+    default: {
+      const double x = r.x - s.p[0], y = r.y - s.p[1], z = r.z - s.p[2];
+      eval = (x * x) + (y * y) + (z * z) - s.p[3] * s.p[3];
+      break;
+    }
+  }
+  if (eval > ABX_SURFACE_COINCIDENT) return 1;
+  if (eval < -ABX_SURFACE_COINCIDENT) return -1;
+  return vdot(u, surf_norm(s, r)) > 0. ? 1 : -1;
+}
+
+ABX_HD double plane_axis_distance(double x0, double r0, double u0, bool on) {
+  const double diff = x0 - r0;
+  if (on || dabs(diff) < ABX_SURFACE_COINCIDENT || u0 == 0.) return ABX_INF;
+  if (diff / u0 < 0.) return ABX_INF;
+  return diff / u0;
+}
+
+ABX_HD double quad_distance(double a, double k, double c, bool on) {
+  const double quad = k * k - a * c;
+  if (quad < 0.) return ABX_INF;
+  if (on || dabs(c) < ABX_SURFACE_COINCIDENT) {
+    if (k >= 0.) return ABX_INF;
+    return (-k + sqrt(quad)) / a;
+  }
+  if (c < 0.) return (-k + sqrt(quad)) / a;
+  const double d = (-k - sqrt(quad)) / a;
+  return d < 0. ? ABX_INF : d;
+}
+
+ABX_HD double surf_distance(const DSurf& s, const V3& r, const V3& u, bool on) {
+  switch (s.type) {
+    case 0: return plane_axis_distance(s.p[0], r.x, u.x, on);
+    case 1: return plane_axis_distance(s.p[0], r.y, u.y, on);
+    case 2: return plane_axis_distance(s.p[0], r.z, u.z, on);
+    case 3: {
+      const double num = s.p[3] - s.p[0] * r.x - s.p[1] * r.y - s.p[2] * r.z;
+      const double denom = s.p[0] * u.x + s.p[1] * u.y + s.p[2] * u.z;
+      const double d = num / denom;
+      if (on || dabs(d) < ABX_SURFACE_COINCIDENT || denom == 0.) return ABX_INF;
+      return d < 0. ? ABX_INF : d;
+    }
+    case 4: {
+      const double a = u.y * u.y + u.z * u.z;
+      if (a == 0.) return ABX_INF;
+      const double y = r.y - s.p[0], z = r.z - s.p[1];
+      return quad_distance(a, y * u.y + z * u.z, y * y + z * z - s.p[2] * s.p[2], on);
+    }
+    case 5: {
+      const double a = u.x * u.x + u.z * u.z;
+      if (a == 0.) return ABX_INF;
+      const double x = r.x - s.p[0], z = r.z - s.p[1];
+      return quad_distance(a, x * u.x + z * u.z, x * x + z * z - s.p[2] * s.p[2], on);
+    }
+    case 6: {
+      const double a = u.y * u.y + u.x * u.x;
+      if (a == 0.) return ABX_INF;
+      const double x = r.x - s.p[0], y = r.y - s.p[1];
+      return quad_distance(a, y * u.y + x * u.x, y * y + x * x - s.p[2] * s.p[2], on);
+    }
+    default: {
+      const double x = r.x - s.p[0], y = r.y - s.p[1], z = r.z - s.p[2];
+      const double k = x * u.x + y * u.y + z * u.z;
+      const double c = x * x + y * y + z * z - s.p[3] * s.p[3];
+      const double quad = k * k - c;
+      if (quad < 0.) return ABX_INF;
+      if (on || dabs(c) < ABX_SURFACE_COINCIDENT) {
+        if (k >= 0.) return ABX_INF;
+        return -k + sqrt(quad);
+      }
+      if (c < 0.) return -k + sqrt(quad);
+      const double d = -k - sqrt(quad);
+      return d < 0. ? ABX_INF : d;
+    }
+  }
+}
+
+// ------------------------------------------------------------------- cells
+ABX_HD int iabs(int x) { return x < 0 ? -x : x; }
+
+ABX_HD bool cell_inside(const DevProblem& P, int ci, const V3& r, const V3& u,
+                        int32_t on_surf) {
+  const DCell& c = P.cell[ci];
+  const int32_t* rpn = P.rpn + c.rpn_off;
+  if (c.simple) {
+    for (int k = 0; k < c.rpn_len; k++) {
+      const int32_t token = rpn[k];
+      if (token == on_surf) continue;
+      if (-token == on_surf) return false;
+      const int sign = surf_sign(P.surf[iabs(token) - 1], r, u);
+      if ((sign > 0 && token < 0) || (sign < 0 && token > 0)) return false;
+    }
+    return true;
+  }
+  // general region: boolean stack as a bit-vector (depth <= 64)
+  uint64_t st = 0;
+  int i = -1;
+  for (int k = 0; k < c.rpn_len; k++) {
+    const int32_t token = rpn[k];
+    if (token == 2147483647 - 4) {  // union
+      const uint64_t a = (st >> (i - 1)) & 1u, b = (st >> i) & 1u;
+      st = (st & ~(1ull << (i - 1))) | ((a | b) << (i - 1));
+      i--;
+    } else if (token == 2147483647 - 3) {  // intersection
+      const uint64_t a = (st >> (i - 1)) & 1u, b = (st >> i) & 1u;
+      st = (st & ~(1ull << (i - 1))) | ((a & b) << (i - 1));
+      i--;
+    } else if (token == 2147483647 - 2) {  // complement
+      st ^= (1ull << i);
+    } else {
+      i++;
+      bool v;
+      if (token == on_surf)
+        v = true;
+      else if (-token == on_surf)
+        v = false;
+      else {
+        const int sign = surf_sign(P.surf[iabs(token) - 1], r, u);
+        v = (sign > 0 && token > 0) || (sign < 0 && token < 0);
+      }
+      st = (st & ~(1ull << i)) | ((uint64_t)(v ? 1u : 0u) << i);
+    }
+  }
+  if (i == 0) return (st & 1u) != 0;
+  return true;
+}
+
+struct DistTok {
+  double d;
+  int32_t tok;
+};
+
+// Cell::distance_to_boundary / distance_to_boundary_condition
+ABX_HD DistTok cell_distance(const DevProblem& P, int ci, const V3& r, const V3& u,
+                             int32_t on_surf, bool bc_only) {
+  const DCell& c = P.cell[ci];
+  DistTok res{ABX_INF, 0};
+  if (bc_only && !c.has_bc) return res;
+  const int32_t* rpn = P.rpn + c.rpn_off;
+  for (int k = 0; k < c.rpn_len; k++) {
+    const int32_t token = rpn[k];
+    if (token >= 2147483647 - 4) continue;
+    const bool coincident = iabs(token) == iabs(on_surf);
+    const DSurf& s = P.surf[iabs(token) - 1];
+    if (bc_only && s.bc == 2) continue;
+    const double d = surf_distance(s, r, u, coincident);
+    if (d < res.d) {
+      if (dabs(d - res.d) / res.d >= 1e-14) {
+        res.d = d;
+        res.tok = -token;
+      }
+    }
+  }
+  return res;
+}
+
+// ---------------------------------------------------------------- lattices
+ABX_HD V3 tile_center(const DUniv& L, int nx, int ny, int nz) {
+  return V3{((double)nx + 0.5) * L.Px + L.Xl, ((double)ny + 0.5) * L.Py + L.Yl,
+            ((double)nz + 0.5) * L.Pz + L.Zl};
+}
+
+ABX_HD void get_tile(const DUniv& L, const V3& r, const V3& u, int t[3]) {
+  int nx = (int)floor((r.x - L.Xl) * L.Pxi);
+  int ny = (int)floor((r.y - L.Yl) * L.Pyi);
+  int nz = (int)floor((r.z - L.Zl) * L.Pzi);
+  const V3 c = tile_center(L, nx, ny, nz);
+  const double xl = c.x - L.Px * 0.5;
+  if (dabs(xl - r.x) < ABX_SURFACE_COINCIDENT && u.x < 0.) nx--;
+  const double xh = c.x + L.Px * 0.5;
+  if (dabs(xh - r.x) < ABX_SURFACE_COINCIDENT && u.x >= 0.) nx++;
+  const double yl = c.y - L.Py * 0.5;
+  if (dabs(yl - r.y) < ABX_SURFACE_COINCIDENT && u.y < 0.) ny--;
+  const double yh = c.y + L.Py * 0.5;
+  if (dabs(yh - r.y) < ABX_SURFACE_COINCIDENT && u.y >= 0.) ny++;
+  const double zl = c.z - L.Pz * 0.5;
+  if (dabs(zl - r.z) < ABX_SURFACE_COINCIDENT && u.z < 0.) nz--;
+  const double zh = c.z + L.Pz * 0.5;
+  if (dabs(zh - r.z) < ABX_SURFACE_COINCIDENT && u.z >= 0.) nz++;
+  t[0] = nx;
+  t[1] = ny;
+  t[2] = nz;
+}
+
+ABX_HD double tile_distance(const DUniv& L, const V3& rl, const V3& u, const int32_t t[3]) {
+  const V3 c = tile_center(L, t[0], t[1], t[2]);
+  const V3 rt = vsub(rl, c);
+  double dist = ABX_INF;
+  const double diff_xl = -L.Px * 0.5 - rt.x;
+  const double diff_xh = L.Px * 0.5 - rt.x;
+  const double diff_yl = -L.Py * 0.5 - rt.y;
+  const double diff_yh = L.Py * 0.5 - rt.y;
+  const double diff_zl = -L.Pz * 0.5 - rt.z;
+  const double diff_zh = L.Pz * 0.5 - rt.z;
+  const double ux_inv = 1. / u.x;
+  const double uy_inv = 1. / u.y;
+  const double uz_inv = 1. / u.z;
+  const double d_xl = diff_xl * ux_inv;
+  const double d_xh = diff_xh * ux_inv;
+  const double d_yl = diff_yl * uy_inv;
+  const double d_yh = diff_yh * uy_inv;
+  const double d_zl = diff_zl * uz_inv;
+  const double d_zh = diff_zh * uz_inv;
+  const double tol = 100 * ABX_SURFACE_COINCIDENT;
+  if (d_xl > 0. && d_xl < dist && dabs(diff_xl) > tol) dist = d_xl;
+  if (d_xh > 0. && d_xh < dist && dabs(diff_xh) > tol) dist = d_xh;
+  if (d_yl > 0. && d_yl < dist && dabs(diff_yl) > tol) dist = d_yl;
+  if (d_yh > 0. && d_yh < dist && dabs(diff_yh) > tol) dist = d_yh;
+  if (d_zl > 0. && d_zl < dist && dabs(diff_zl) > tol) dist = d_zl;
+  if (d_zh > 0. && d_zh < dist && dabs(diff_zh) > tol) dist = d_zh;
+  return dist;
+}
+
+// ------------------------------------------------------------------ cursor
+enum { PAD_U = 0, PAD_L = 1, PAD_C = 2 };
+
+// pad word: type (2 bits) << 30 | level (3 bits) << 27 | index (27 bits)
+ABX_HD int32_t pad_word(int type, int lvl, int idx) {
+  return (int32_t)(((uint32_t)type << 30) | ((uint32_t)lvl << 27) | (uint32_t)idx);
+}
+ABX_HD int pad_type(int32_t w) { return (int)(((uint32_t)w) >> 30); }
+ABX_HD int pad_lvl(int32_t w) { return (int)((((uint32_t)w) >> 27) & 7u); }
+ABX_HD int pad_idx(int32_t w) { return (int)(((uint32_t)w) & 0x07ffffffu); }
+ABX_HD bool tile_eq(const int32_t a[3], const int b[3]) {
+  return a[0] == b[0] && a[1] == b[1] && a[2] == b[2];
+}
+
+struct Trk {
+  int32_t np;                  // pads in use
+  int32_t cell;                // material cell index, -1 = lost
+  int32_t token;               // surface token the particle sits on
+  int32_t err;                 // ABX_E_* raised inside navigation
+  int32_t pad[ABX_MAXPADS];
+  int32_t tile[ABX_MAXPADS][3];  // lattice pads only
+  V3 lv[ABX_MAXLVL];           // frame positions; lv[0] is the global position
+};
+
+ABX_HD bool tile_in_range(const DUniv& L, const int t[3]) {
+  return !(t[0] < 0 || t[0] >= L.Nx || t[1] < 0 || t[1] >= L.Ny || t[2] < 0 ||
+           t[2] >= L.Nz);
+}
+
+ABX_HD bool trk_push(Trk& t, int32_t w, const int* tile) {
+  if (t.np >= ABX_MAXPADS) {
+    t.err = -3;
+    return false;
+  }
+  t.pad[t.np] = w;
+  if (tile) {
+    t.tile[t.np][0] = tile[0];
+    t.tile[t.np][1] = tile[1];
+    t.tile[t.np][2] = tile[2];
+  }
+  t.np++;
+  return true;
+}
+
+// Universe::get_cell with the stack, iteratively; r is frame `lvl`.
+ABX_HD int trk_descend(const DevProblem& P, Trk& t, int ui, int lvl, V3 r, const V3& u) {
+  for (int guard = 0; guard < 4 * ABX_MAXPADS; guard++) {
+    const DUniv& U = P.univ[ui];
+    if (U.type == 0) {
+      if (!trk_push(t, pad_word(PAD_U, lvl, ui), nullptr)) return -1;
+      int found = -1;
+      for (int k = 0; k < U.ncells; k++) {
+        const int ci = P.ucells[U.cells_off + k];
+        if (cell_inside(P, ci, r, u, t.token)) {
+          found = ci;
+          break;
+        }
+      }
+      if (found < 0) return -1;
+      if (!trk_push(t, pad_word(PAD_C, lvl, found), nullptr)) return -1;
+      const DCell& c = P.cell[found];
+      if (c.fill_type == 0) return found;
+      ui = c.fill;
+      continue;
+    }
+    int tl[3];
+    get_tile(U, r, u, tl);
+    int elem = -1;
+    if (tile_in_range(U, tl))
+      elem = P.elems[U.elems_off + tl[2] * (U.Nx * U.Ny) + tl[0] * U.Ny + tl[1]];
+    if (!trk_push(t, pad_word(PAD_L, lvl, ui), tl)) return -1;
+    if (elem >= 0) {
+      if (lvl + 1 >= ABX_MAXLVL) {
+        t.err = -3;
+        return -1;
+      }
+      r = vsub(r, tile_center(U, tl[0], tl[1], tl[2]));
+      lvl++;
+      t.lv[lvl] = r;
+      ui = elem;
+      continue;
+    }
+    if (U.outer >= 0) {
+      ui = U.outer;
+      continue;
+    }
+    return -1;
+  }
+  t.err = -3;
+  return -1;
+}
+
+// Tracker::restart_get_current from the global position lv[0].
+ABX_HD void trk_restart(const DevProblem& P, Trk& t, const V3& u) {
+  t.np = 0;
+  t.cell = trk_descend(P, t, P.root, 0, t.lv[0], u);
+}
+
+ABX_HD void trk_init(const DevProblem& P, Trk& t, const V3& r, const V3& u, int32_t token) {
+  t.err = 0;
+  t.token = token;
+  t.lv[0] = r;
+  trk_restart(P, t, u);
+}
+
+// Re-descend after pad `first_bad` (>= 1) went stale (tracker.hpp:276-305).
+ABX_HD void trk_redescend(const DevProblem& P, Trk& t, const V3& u, int first_bad) {
+  t.np = first_bad;
+  const int32_t back = t.pad[t.np - 1];
+  const int lvl = pad_lvl(back);
+  const int ui = pad_type(back) == PAD_C ? P.cell[pad_idx(back)].redescend : pad_idx(back);
+  t.np--;
+  t.cell = trk_descend(P, t, ui, lvl, t.lv[lvl], u);
+  if (t.cell < 0) trk_restart(P, t, u);
+}
+
+// Tracker::get_current after a move (tree consistent with lv[0]).
+ABX_HD void trk_get_current(const DevProblem& P, Trk& t, const V3& u) {
+  int first_bad = t.np;
+  for (int i = 0; i < t.np; i++) {
+    const int32_t w = t.pad[i];
+    const int ty = pad_type(w);
+    if (ty == PAD_C) {
+      if (!cell_inside(P, pad_idx(w), t.lv[pad_lvl(w)], u, t.token)) {
+        first_bad = i;
+        break;
+      }
+    } else if (ty == PAD_L) {
+      int tl[3];
+      get_tile(P.univ[pad_idx(w)], t.lv[pad_lvl(w)], u, tl);
+      if (!tile_eq(t.tile[i], tl)) {
+        first_bad = i;
+        break;
+      }
+    }
+  }
+  if (first_bad == t.np) return;
+  if (first_bad == 0) {
+    trk_restart(P, t, u);
+    return;
+  }
+  trk_redescend(P, t, u, first_bad);
+}
+
+// Tracker::set_r(r) followed by get_current(): the stale tree forces a
+// restart, after which the reference still compares against the old tree
+// size (tracker.hpp:237-278).
+ABX_HD void trk_reset_position(const DevProblem& P, Trk& t, const V3& r, const V3& u) {
+  const int old_np = t.np;
+  t.token = 0;
+  t.lv[0] = r;
+  trk_restart(P, t, u);
+  if (old_np == t.np) return;
+  if (old_np > t.np) {
+    t.err = -3;  // reference would resize into default pads
+    return;
+  }
+  trk_redescend(P, t, u, old_np);
+}
+
+ABX_HD void trk_move(Trk& t, const V3& u, double d) {
+  const V3 du = vscale(u, d);
+  int maxl = 0;
+  for (int i = 0; i < t.np; i++) {
+    const int l = pad_lvl(t.pad[i]);
+    maxl = l > maxl ? l : maxl;
+  }
+  for (int l = 0; l <= maxl && l < ABX_MAXLVL; l++) t.lv[l] = vadd(t.lv[l], du);
+  t.token = 0;
+}
+
+struct Bound {
+  double d;
+  int32_t sidx;
+  int32_t btype;
+  int32_t tok;
+};
+
+// Universe::get_boundary_condition (cell universe / lattice via outer)
+ABX_HD Bound univ_bc(const DevProblem& P, int ui, const V3& r, const V3& u, int32_t on) {
+  Bound b{ABX_INF, -1, 0, 0};
+  for (int guard = 0; guard < 16; guard++) {
+    const DUniv& U = P.univ[ui];
+    if (U.type == 1) {
+      if (!U.has_bc) return b;
+      ui = U.outer;
+      continue;
+    }
+    if (U.has_bc) {
+      for (int k = 0; k < U.ncells; k++) {
+        const int ci = P.ucells[U.cells_off + k];
+        if (!P.cell[ci].has_bc) continue;
+        const DistTok dt = cell_distance(P, ci, r, u, on, true);
+        if (dt.d < b.d && dabs(dt.d - b.d) > ABX_BOUNDRY_TOL) {
+          const int32_t tt = iabs(dt.tok);
+          if (!tt) continue;
+          b.tok = tt;
+          b.d = dt.d;
+          b.sidx = tt - 1;
+          b.btype = P.surf[b.sidx].bc;
+          if (surf_sign(P.surf[b.sidx], r, u) < 0) b.tok = -b.tok;
+        }
+      }
+    }
+    return b;
+  }
+  return b;
+}
+
+// Tracker::get_boundary_condition
+ABX_HD Bound trk_bc(const DevProblem& P, const Trk& t, const V3& u) {
+  if (t.cell < 0) return univ_bc(P, P.root, t.lv[0], u, t.token);
+  Bound b{ABX_INF, -1, 0, 0};
+  for (int i = 0; i < t.np; i++) {
+    const int32_t w = t.pad[i];
+    const V3& rl = t.lv[pad_lvl(w)];
+    if (pad_type(w) == PAD_C) {
+      const int ci = pad_idx(w);
+      if (!P.cell[ci].has_bc) continue;
+      const DistTok dt = cell_distance(P, ci, rl, u, t.token, true);
+      if (dt.d < b.d && dabs(dt.d - b.d) > ABX_BOUNDRY_TOL) {
+        const int32_t tt = iabs(dt.tok);
+        if (!tt) continue;
+        b.d = dt.d;
+        b.tok = tt;
+        b.sidx = tt - 1;
+        b.btype = P.surf[b.sidx].bc;
+        if (surf_sign(P.surf[b.sidx], rl, u) < 0) b.tok = -b.tok;
+      }
+    } else {
+      const int ui = pad_idx(w);
+      if (P.univ[ui].has_bc) {
+        const Bound ub = univ_bc(P, ui, rl, u, t.token);
+        if (ub.d < b.d && dabs(ub.d - b.d) > ABX_BOUNDRY_TOL) b = ub;
+      }
+    }
+  }
+  return b;
+}
+
+// Tracker::get_nearest_boundary
+ABX_HD Bound trk_nearest(const DevProblem& P, const Trk& t, const V3& u) {
+  Bound b = trk_bc(P, t, u);
+  for (int i = 0; i < t.np; i++) {
+    const int32_t w = t.pad[i];
+    const V3& rl = t.lv[pad_lvl(w)];
+    const int ty = pad_type(w);
+    if (ty == PAD_L) {
+      const double d = tile_distance(P.univ[pad_idx(w)], rl, u, t.tile[i]);
+      if (d < b.d && dabs(d - b.d) > ABX_BOUNDRY_TOL) {
+        b.d = d;
+        b.btype = 2;
+        b.sidx = -1;
+        b.tok = 0;
+      }
+    } else if (ty == PAD_C) {
+      const DistTok dt = cell_distance(P, pad_idx(w), rl, u, t.token, false);
+      if (dt.d < b.d && dabs(dt.d - b.d) > ABX_BOUNDRY_TOL) {
+        b.d = dt.d;
+        b.tok = iabs(dt.tok);
+        b.sidx = b.tok ? b.tok - 1 : -1;
+        b.btype = b.sidx >= 0 ? P.surf[b.sidx].bc : 2;
+        if (b.sidx >= 0 && surf_sign(P.surf[b.sidx], rl, u) < 0) b.tok = -b.tok;
+      }
+    }
+  }
+  return b;
+}

@@ -1,0 +1,576 @@
+// physics.h - one history of the multigroup transport loop (device code).
+//
+// Restates, per particle and with the reference's draw order:
+//  * DeltaTracker::transport loop body      src/delta_tracker.cpp:86-229
+//  * CarterTracker::transport loop body     src/carter_tracker.cpp:104-262
+//  * SurfaceTracker::transport loop body    src/surface_tracker.cpp:70-200
+//  * Transporter::collision / branching_collision / make_fission_neutrons /
+//    do_scatter / russian_roulette          src/transporter.cpp:35-93,269-487
+//  * MGNuclide::sample_scatter / sample_fission, MGAngleDistribution::sample_mu
+//                                           src/mg_nuclide.cpp:442-544,
+//                                           include/materials/mg_angle_distribution.hpp:45-60
+//  * rotate_direction / Direction(mu, phi)  include/utils/direction.hpp:43-62,130-151
+//  * Tracker::do_reflection                 include/simulation/tracker.hpp:314-359
+//  * CollisionMeshTally::score_collision    src/collision_mesh_tally.cpp:33-110
+//  * TrackLengthMeshTally::score_flight     src/track_length_mesh_tally.cpp:79-393
+//
+// Everything a kernel needs to do with global memory (tally atomics, the
+// secondary stack, fission-bank appends, traces) goes through the `Sink`
+// template parameter, so the same code runs in the history kernel and in
+// the event kernels.
+#pragma once
+
+#include "detmath.h"
+#include "geometry.h"
+#include "rng.h"
+
+struct Part {
+  V3 r, u;
+  V3 rb;  // r_birth
+  double E, w, w2;
+  uint64_t rng;
+  uint64_t hid, fam;
+  uint32_t calls;      // 32-bit draws since the history entered transport
+  uint32_t daughters;  // Particle::daughter_counter
+  int32_t sec_id;
+  int32_t g;           // energy group of E
+  int32_t alive;
+};
+
+struct SecRec {
+  double r[3], u[3], E, w, w2;
+};
+
+// RNG with draw counting (for traces / stride checks)
+ABX_HD double p_rand(Part& p) {
+  p.calls += 2;
+  return rng_rand(p.rng);
+}
+ABX_HD double p_exponential(Part& p, double lambda) {
+  if (lambda == 0.0) return ABX_INF;
+  p.calls += 2;
+  return rng_exponential(p.rng, lambda);
+}
+ABX_HD int p_discrete(Part& p, const double* cdf, int n) {
+  if (n < 2) return 0;
+  p.calls += 2;
+  return rng_discrete(p.rng, cdf, n);
+}
+
+ABX_HD int energy_index(const DevProblem& P, double E) {
+  int i = 0;
+  for (i = 0; i < P.G; i++)
+    if (P.ebounds[i] <= E && E < P.ebounds[i + 1]) break;
+  return i;
+}
+
+// Direction(mu, phi) with clamping, then normalisation
+ABX_HD V3 dir_mu_phi(double mu, double phi) {
+  if (mu < -1.)
+    mu = -1.;
+  else if (mu > 1.)
+    mu = 1.;
+  if (phi < 0.)
+    phi = 0.;
+  else if (phi > 2 * ABX_PI)
+    phi = 2 * ABX_PI;
+  double sn, cs;
+  dm_sincos(phi, &sn, &cs);
+  const double x = sqrt(1. - mu * mu) * cs;
+  const double y = sqrt(1. - mu * mu) * sn;
+  return vdir(x, y, mu);
+}
+
+ABX_HD V3 rotate_dir(const V3& u, double mu, double phi) {
+  double sn, cs;
+  dm_sincos(phi, &sn, &cs);
+  const double sqrt_mu = sqrt(1. - mu * mu);
+  const double sqrt_w = sqrt(1. - u.z * u.z);
+  double ux, uy, uz;
+  if (sqrt_w > 1.E-10) {
+    ux = mu * u.x + sqrt_mu * (u.x * u.z * cs - u.y * sn) / sqrt_w;
+    uy = mu * u.y + sqrt_mu * (u.y * u.z * cs + u.x * sn) / sqrt_w;
+    uz = mu * u.z - sqrt_mu * sqrt_w * cs;
+  } else {
+    const double sqrt_v = sqrt(1. - u.y * u.y);
+    ux = mu * u.x + sqrt_mu * (u.x * u.y * cs + u.z * sn) / sqrt_v;
+    uy = mu * u.y - sqrt_mu * sqrt_v * cs;
+    uz = mu * u.z + sqrt_mu * (u.y * u.z * cs - u.x * sn) / sqrt_v;
+  }
+  return vdir(ux, uy, uz);
+}
+
+ABX_HD double sample_mu(const DevProblem& P, Part& p, int m, int gi, int go) {
+  const int k = (m * P.G + gi) * P.G + go;
+  const int b0 = P.ang_off[k], n = P.ang_off[k + 1] - b0;
+  const double* mu = P.ang_mu + b0;
+  const double* pdf = P.ang_pdf + b0;
+  const double* cdf = P.ang_cdf + b0;
+  const double xi = p_rand(p);
+  int l = cdf_lower_bound(cdf, n, xi);
+  if (xi == cdf[l]) return mu[l];
+  l--;
+  if (pdf[l] == pdf[l + 1]) return mu[l] + ((xi - cdf[l]) / pdf[l]);
+  const double mm = (pdf[l + 1] - pdf[l]) / (mu[l + 1] - mu[l]);
+  return mu[l] + (1. / mm) * (sqrt(pdf[l] * pdf[l] + 2. * mm * (xi - cdf[l])) - pdf[l]);
+}
+
+ABX_HD int tally_ebin(const DevProblem& P, const DTally& T, int ti, const Part& p) {
+  if (p.g < P.G && p.E == P.emid[p.g]) return P.tally_gbin[ti * P.G + p.g];
+  const double* eb = P.tally_eb + T.eb_off;
+  for (int e = 0; e < T.Ne; e++)
+    if (eb[e] <= p.E && p.E <= eb[e + 1]) return e;
+  return -1;
+}
+
+ABX_HD double quantity_factor(const DevProblem& P, const DTally& T, const Part& p, int mg) {
+  switch (T.quantity) {
+    case 1: return p.w * P.Eel[mg];
+    case 2: return p.w * P.Eabs[mg];
+    case 3: return p.w * P.Ef[mg];
+    case 4: return p.w * P.Et[mg];
+    case 7: return p.w2;
+    default: return p.w;
+  }
+}
+
+template <class Sink>
+ABX_HD void score_collision_tallies(const DevProblem& P, Sink& s, const Part& p, int m) {
+  const int mg = m * P.G + p.g;
+  for (int ti = 0; ti < P.ntally; ti++) {
+    const DTally& T = P.tally[ti];
+    if (T.estimator != 0) continue;
+    const double Et = P.Et[mg];
+    double scr = 1. / (Et * T.net_weight);
+    const int i = (int)floor((p.r.x - T.lx) * T.dxi);
+    const int j = (int)floor((p.r.y - T.ly) * T.dyi);
+    const int k = (int)floor((p.r.z - T.lz) * T.dzi);
+    const int l = tally_ebin(P, T, ti, p);
+    if (l == -1) continue;
+    if (i >= 0 && i < T.Nx && j >= 0 && j < T.Ny && k >= 0 && k < T.Nz) {
+      scr *= quantity_factor(P, T, p, mg);
+      const int64_t idx = (((int64_t)l * T.Nx + i) * T.Ny + j) * T.Nz + k;
+      s.tally_add(P.tally_data + T.data_off + idx, scr);
+    }
+  }
+}
+
+// TrackLengthMeshTally::initialize_indices
+ABX_HD void tle_init(const DTally& T, const V3& r, const V3& u, int& i, int& j, int& k,
+                     int on[3]) {
+  i = (int)floor((r.x - T.lx) * T.dxi);
+  j = (int)floor((r.y - T.ly) * T.dyi);
+  k = (int)floor((r.z - T.lz) * T.dzi);
+  on[0] = on[1] = on[2] = 0;
+  const double xc = T.lx + i * T.dx + 0.5 * T.dx;
+  const double yc = T.ly + j * T.dy + 0.5 * T.dy;
+  const double zc = T.lz + k * T.dz + 0.5 * T.dz;
+  const double xl = xc - 0.5 * T.dx, xh = xc + 0.5 * T.dx;
+  const double yl = yc - 0.5 * T.dy, yh = yc + 0.5 * T.dy;
+  const double zl = zc - 0.5 * T.dz, zh = zc + 0.5 * T.dz;
+  if (dabs(xl - r.x) < ABX_SURFACE_COINCIDENT) {
+    if (u.x < 0.) { i--; on[0] = 1; } else { on[0] = -1; }
+  } else if (dabs(xh - r.x) < ABX_SURFACE_COINCIDENT) {
+    if (u.x < 0.) { on[0] = 1; } else { i++; on[0] = -1; }
+  }
+  if (dabs(yl - r.y) < ABX_SURFACE_COINCIDENT) {
+    if (u.y < 0.) { j--; on[1] = 1; } else { on[1] = -1; }
+  } else if (dabs(yh - r.y) < ABX_SURFACE_COINCIDENT) {
+    if (u.y < 0.) { on[1] = 1; } else { j++; on[1] = -1; }
+  }
+  if (dabs(zl - r.z) < ABX_SURFACE_COINCIDENT) {
+    if (u.z < 0.) { k--; on[2] = 1; } else { on[2] = -1; }
+  } else if (dabs(zh - r.z) < ABX_SURFACE_COINCIDENT) {
+    if (u.z < 0.) { on[2] = 1; } else { k++; on[2] = -1; }
+  }
+}
+
+// TrackLengthMeshTally::find_entry_point (slab method)
+ABX_HD bool tle_entry(const DTally& T, V3& r, const V3& u, double& d) {
+  const double ux_inv = 1. / u.x, uy_inv = 1. / u.y, uz_inv = 1. / u.z;
+  double d_min = (T.lx - r.x) * ux_inv, d_max = (T.hx - r.x) * ux_inv;
+  double t;
+  if (d_min > d_max) { t = d_min; d_min = d_max; d_max = t; }
+  double y0 = (T.ly - r.y) * uy_inv, y1 = (T.hy - r.y) * uy_inv;
+  if (y0 > y1) { t = y0; y0 = y1; y1 = t; }
+  if ((d_min > y1) || (y0 > d_max)) return false;
+  if (y0 > d_min) d_min = y0;
+  if (y1 < d_max) d_max = y1;
+  double z0 = (T.lz - r.z) * uz_inv, z1 = (T.hz - r.z) * uz_inv;
+  if (z0 > z1) { t = z0; z0 = z1; z1 = t; }
+  if ((d_min > z1) || (z0 > d_max)) return false;
+  if (z0 > d_min) d_min = z0;
+  if (z1 < d_max) d_max = z1;
+  if (d_max < d_min) { t = d_max; d_max = d_min; d_min = t; }
+  if ((d_max < 0.) && (d_min < 0.)) return false;
+  if (d_min < 0.) return false;
+  r = vadd(r, vscale(u, d_min));
+  d -= d_min;
+  return true;
+}
+
+template <class Sink>
+ABX_HD void score_flight_tallies(const DevProblem& P, Sink& s, const Part& p, double d,
+                                 int m) {
+  const int mg = m * P.G + p.g;
+  for (int ti = 0; ti < P.ntally; ti++) {
+    const DTally& T = P.tally[ti];
+    if (T.estimator != 1) continue;
+    V3 r = p.r;
+    const V3 u = p.u;
+    double dd = d;
+    int i, j, k, on[3];
+    tle_init(T, r, u, i, j, k, on);
+    const bool inside = i >= 0 && i < T.Nx && j >= 0 && j < T.Ny && k >= 0 && k < T.Nz;
+    if (!inside) {
+      if (!tle_entry(T, r, u, dd)) continue;
+      tle_init(T, r, u, i, j, k, on);
+    }
+    double base = 1. / T.net_weight;
+    base *= quantity_factor(P, T, p, mg);
+    const int l = tally_ebin(P, T, ti, p);
+    if (l == -1) continue;
+    double remaining = dd;
+    double* data = P.tally_data + T.data_off + (int64_t)l * T.Nx * T.Ny * T.Nz;
+    while (remaining > 0.) {
+      const double xc = T.lx + i * T.dx + 0.5 * T.dx;
+      const double yc = T.ly + j * T.dy + 0.5 * T.dy;
+      const double zc = T.lz + k * T.dz + 0.5 * T.dz;
+      const double rtx = r.x - xc, rty = r.y - yc, rtz = r.z - zc;
+      double dist = ABX_INF;
+      int key = 0;
+      const double diff_xl = -T.dx * 0.5 - rtx, diff_xh = T.dx * 0.5 - rtx;
+      const double diff_yl = -T.dy * 0.5 - rty, diff_yh = T.dy * 0.5 - rty;
+      const double diff_zl = -T.dz * 0.5 - rtz, diff_zh = T.dz * 0.5 - rtz;
+      const double ux_inv = 1. / u.x, uy_inv = 1. / u.y, uz_inv = 1. / u.z;
+      const double d_xl = diff_xl * ux_inv, d_xh = diff_xh * ux_inv;
+      const double d_yl = diff_yl * uy_inv, d_yh = diff_yh * uy_inv;
+      const double d_zl = diff_zl * uz_inv, d_zh = diff_zh * uz_inv;
+      if (d_xl > 0. && d_xl < dist && on[0] != -1) { dist = d_xl; key = -1; }
+      if (d_xh > 0. && d_xh < dist && on[0] != 1) { dist = d_xh; key = 1; }
+      if (d_yl > 0. && d_yl < dist && on[1] != -1) { dist = d_yl; key = -2; }
+      if (d_yh > 0. && d_yh < dist && on[1] != 1) { dist = d_yh; key = 2; }
+      if (d_zl > 0. && d_zl < dist && on[2] != -1) { dist = d_zl; key = -3; }
+      if (d_zh > 0. && d_zh < dist && on[2] != 1) { dist = d_zh; key = 3; }
+      if (dist == ABX_INF) break;
+      const double d_tile = dist < remaining ? dist : remaining;
+      if (i >= 0 && i < T.Nx && j >= 0 && j < T.Ny && k >= 0 && k < T.Nz) {
+        s.tally_add(data + ((int64_t)i * T.Ny + j) * T.Nz + k, d_tile * base);
+      } else {
+        break;
+      }
+      remaining -= d_tile;
+      if (remaining <= 0.) break;
+      r = vadd(r, vscale(u, d_tile));
+      on[0] = on[1] = on[2] = 0;
+      switch (key) {
+        case -1: i--; on[0] = 1; break;
+        case 1: i++; on[0] = -1; break;
+        case -2: j--; on[1] = 1; break;
+        case 2: j++; on[1] = -1; break;
+        case -3: k--; on[2] = 1; break;
+        case 3: k++; on[2] = -1; break;
+        default: break;
+      }
+    }
+  }
+}
+
+struct Scores6 {
+  double k_col, k_abs, k_trk, k_tot, leak, mig;
+};
+
+// Transporter::collision + branching_collision.  `kcol` is Tallies::kcol().
+template <class Sink>
+ABX_HD void collide(const DevProblem& P, Sink& s, Part& p, int m, double kcol,
+                    bool converged, Scores6& sc) {
+  const int G = P.G;
+  const int mg = m * G + p.g;
+  if (converged) score_collision_tallies(P, s, p, m);
+  {
+    const double k_col_scr = p.w * P.vEf[mg] / P.Et[mg];
+    const V3 dr = vsub(p.r, p.rb);
+    const double mig_dist = vnorm(dr);
+    const double mig_area_scr = p.w * P.Eabs[mg] / P.Et[mg] * mig_dist * mig_dist;
+    sc.k_col += k_col_scr;
+    sc.mig += mig_area_scr;
+  }
+  (void)p_rand(p);  // MaterialHelper::sample_nuclide (single component)
+  const double total = P.Et[mg], fission = P.Ef[mg], absorption = P.Eabs[mg];
+  const double nu_total = P.nu_tot[mg], nu_delayed = P.nu_d[mg];
+  sc.k_abs += p.w * nu_total * fission / total;
+  // make_fission_neutrons
+  {
+    const double k_abs_scr = p.w * nu_total * fission / total;
+    int n_new;
+    if (P.mode == 0)
+      n_new = (int)floor(dabs(k_abs_scr) / kcol + p_rand(p));
+    else
+      n_new = (int)floor(dabs(k_abs_scr) + p_rand(p));
+    const double P_delayed = nu_delayed / nu_total;
+    const double* chi = P.chi_cdf + (size_t)mg * G;
+    for (int i = 0; i < n_new; i++) {
+      const int ei = p_discrete(p, chi, G);
+      const double E_out = P.emid[ei];
+      const double mu = 2. * p_rand(p) - 1.;
+      const double phi = 2. * ABX_PI * p_rand(p);
+      const V3 uout = rotate_dir(p.u, mu, phi);
+      if (p_rand(p) < P_delayed) {
+        (void)p_discrete(p, P.dgrp_cdf + (size_t)m * P.n_dgrp_max, P.dgrp_n[m]);
+      }
+      const double wgt = p.w > 0. ? 1. : -1.;
+      if (P.mode == 1) {
+        SecRec q{{p.r.x, p.r.y, p.r.z}, {uout.x, uout.y, uout.z}, E_out, wgt, 0.};
+        if (!s.sec_push(q)) {
+          s.error(-5, p.hid);
+          p.alive = 0;
+          return;
+        }
+      } else {
+        s.bank_fission(p, uout, E_out, wgt);
+        p.daughters++;
+      }
+    }
+  }
+  p.w = p.w * (1. - (absorption + 0.) / total);
+  p.w2 = p.w2 * (1. - (absorption + 0.) / total);
+  // russian_roulette
+  if (dabs(p.w) < P.wgt_cutoff) {
+    const double P_kill = 1.0 - (dabs(p.w) / P.wgt_survival);
+    if (p_rand(p) < P_kill)
+      p.w = 0.;
+    else
+      p.w = copysign(P.wgt_survival, p.w);
+  }
+  if (dabs(p.w2) < P.wgt_cutoff) {
+    const double P_kill = 1.0 - (dabs(p.w2) / P.wgt_survival);
+    if (p_rand(p) < P_kill)
+      p.w2 = 0.;
+    else
+      p.w2 = copysign(P.wgt_survival, p.w2);
+  }
+  if (p.w == 0. && p.w2 == 0.) p.alive = 0;
+  if (p.alive) {
+    // do_scatter -> MGNuclide::sample_scatter
+    const int ei = p_discrete(p, P.scat_cdf + (size_t)mg * G, G);
+    const double E_out = P.emid[ei];
+    const double mu = sample_mu(P, p, m, p.g, ei);
+    const double phi = 2. * ABX_PI * p_rand(p);
+    p.u = rotate_dir(p.u, mu, phi);
+    p.E = E_out;
+    p.g = energy_index(P, E_out);
+    p.w = p.w * 1.;
+    p.w2 = p.w2 * 1.;
+    if (p.E < P.min_energy) p.alive = 0;
+  }
+}
+
+template <class Sink>
+ABX_HD void leak(Part& p, const Bound& b, Scores6& sc) {
+  p.alive = 0;
+  sc.leak += p.w;
+  const V3 r_leak = vadd(p.r, vscale(p.u, b.d));
+  const V3 dr = vsub(r_leak, p.rb);
+  sc.mig += vdot(vscale(dr, p.w), dr);
+}
+
+// Tracker::do_reflection; the particle's position before the reflection
+// only feeds Particle::previous_r, which this path does not keep.
+ABX_HD void reflect(const DevProblem& P, Trk& t, Part& p, const Bound& b) {
+  if (b.sidx < 0) {
+    t.err = -3;
+    return;
+  }
+  const DSurf& S = P.surf[b.sidx];
+  const V3 r_on = vadd(p.r, vscale(p.u, b.d));
+  const V3 n = surf_norm(S, r_on);
+  const V3 nd = vsub(p.u, vscale(n, 2. * vdot(p.u, n)));
+  p.r = r_on;
+  p.u = vdir(nd.x, nd.y, nd.z);
+  t.token = 0;
+  t.lv[0] = p.r;
+  trk_restart(P, t, p.u);
+}
+
+template <class Sink>
+ABX_HD bool try_resurrect(const DevProblem& P, Sink& s, Part& p, Trk& t, int& m) {
+  SecRec q;
+  if (!s.sec_pop(q)) return false;
+  p.alive = 1;
+  p.r = V3{q.r[0], q.r[1], q.r[2]};
+  p.u = V3{q.u[0], q.u[1], q.u[2]};
+  p.E = q.E;
+  p.g = energy_index(P, p.E);
+  p.w = q.w;
+  p.w2 = q.w2;
+  p.sec_id++;
+  t.token = 0;
+  t.lv[0] = p.r;
+  trk_restart(P, t, p.u);
+  if (t.cell < 0 || t.err) {
+    s.error(-3, p.hid);
+    p.alive = 0;
+    return false;
+  }
+  m = P.cell[t.cell].fill;
+  s.count_secondary();
+  s.trace(p, t, 6);
+  return true;
+}
+
+// Carter-tracking weight split (src/carter_tracker.cpp:219-223)
+template <class Sink>
+ABX_HD void split(Sink& s, Part& p, const DevProblem& P) {
+  if (p.alive && dabs(p.w) >= P.wgt_split) {
+    const int n_new = (int)ceil(dabs(p.w));
+    if (n_new > 1) {
+      p.w = p.w / (double)n_new;
+      p.w2 = p.w2 / (double)n_new;
+      for (int np = 0; np < n_new - 1; np++) {
+        SecRec q{{p.r.x, p.r.y, p.r.z}, {p.u.x, p.u.y, p.u.z}, p.E, p.w, p.w2};
+        if (!s.sec_push(q)) {
+          s.error(-5, p.hid);
+          p.alive = 0;
+          return;
+        }
+      }
+    }
+  }
+}
+
+// One whole history (plus its secondaries).
+template <class Sink>
+ABX_HD void run_history(const DevProblem& P, Sink& s, Part& p, double kcol, bool converged,
+                        Scores6& sc) {
+  Trk t;
+  trk_init(P, t, p.r, p.u, 0);
+  if (t.cell < 0) {
+    p.alive = 0;
+    s.count_lost();
+  }
+  if (t.err) {
+    s.error(-3, p.hid);
+    return;
+  }
+  int m = t.cell >= 0 ? P.cell[t.cell].fill : -1;
+  const int trk = P.tracking;
+  for (int guard = 0; p.alive; guard++) {
+    if (guard > 100000000) {
+      s.error(-3, p.hid);
+      return;
+    }
+    int ev;
+    bool had_collision = false;
+    if (trk == 0) {
+      // ---------------- surface tracking
+      const double d_coll = p_exponential(p, P.Et[m * P.G + p.g]);
+      s.count_flight();
+      const Bound b = trk_nearest(P, t, p.u);
+      const double dmin = d_coll < b.d ? d_coll : b.d;
+      if (converged) score_flight_tallies(P, s, p, dmin, m);
+      sc.k_trk += p.w * dmin * P.vEf[m * P.G + p.g];
+      ev = 2;
+      if (b.d < d_coll || dabs(b.d - d_coll) < ABX_BOUNDRY_TOL) {
+        s.count_crossing();
+        if (b.btype == 0) {
+          leak<Sink>(p, b, sc);
+          ev = 4;
+        } else if (b.btype == 1) {
+          reflect(P, t, p, b);
+          if (t.cell < 0 || t.err) {
+            s.error(-3, p.hid);
+            return;
+          }
+          ev = 3;
+        } else {
+          trk_move(t, p.u, b.d);
+          t.token = -b.tok;
+          trk_get_current(P, t, p.u);
+          p.r = vadd(p.r, vscale(p.u, b.d));
+          if (t.cell < 0 || t.err) {
+            s.error(-3, p.hid);
+            return;
+          }
+          m = P.cell[t.cell].fill;
+        }
+      } else {
+        p.r = vadd(p.r, vscale(p.u, d_coll));
+        trk_move(t, p.u, d_coll);
+        had_collision = true;
+      }
+    } else {
+      // ---------------- delta / carter tracking
+      const bool carter = trk == 2;
+      const double Emaj = (carter ? P.sampling[p.g] : P.majorant[p.g]) + 0.;
+      const double d_coll = p_exponential(p, Emaj);
+      s.count_flight();
+      Bound b{ABX_INF, -1, 2, 0};
+      bool crossed = false;
+      trk_move(t, p.u, d_coll);
+      trk_get_current(P, t, p.u);
+      if (t.err) {
+        s.error(-3, p.hid);
+        return;
+      }
+      if (t.cell < 0) {
+        trk_reset_position(P, t, p.r, p.u);
+        if (t.err) {
+          s.error(-3, p.hid);
+          return;
+        }
+        b = trk_bc(P, t, p.u);
+        crossed = true;
+      }
+      if (converged) score_flight_tallies(P, s, p, d_coll < b.d ? d_coll : b.d, m);
+      ev = 0;
+      if (crossed) {
+        s.count_crossing();
+        if (b.btype == 0) {
+          leak<Sink>(p, b, sc);
+          ev = 4;
+        } else if (b.btype == 1) {
+          reflect(P, t, p, b);
+          if (t.cell < 0 || t.err) {
+            s.error(-3, p.hid);
+            return;
+          }
+          ev = 3;
+        } else {
+          s.error(-3, p.hid);
+          return;
+        }
+      } else {
+        p.r = vadd(p.r, vscale(p.u, d_coll));
+        m = P.cell[t.cell].fill;
+        const double Et = P.Et[m * P.G + p.g];
+        if (!carter) {
+          if (Et - Emaj > 1.E-10) {
+            s.error(-4, p.hid);
+            return;
+          }
+          if (p_rand(p) < (Et / Emaj)) had_collision = true;
+        } else if (Emaj >= Et) {
+          if (p_rand(p) < (Et / Emaj)) had_collision = true;
+        } else {
+          const double D = Et / (2. * Et - Emaj);
+          const double F = Et / (D * Emaj);
+          if ((D - p_rand(p)) > 0.) {
+            p.w = p.w * F;
+            had_collision = true;
+          } else {
+            p.w = -p.w * F;
+          }
+        }
+        if (!had_collision) s.count_virtual();
+      }
+    }
+    if (p.alive && had_collision) {
+      s.count_collision();
+      collide(P, s, p, m, kcol, converged, sc);
+      ev = p.alive ? 1 : 5;
+    }
+    if (trk == 2) split(s, p, P);
+    s.trace(p, t, ev);
+    if (!p.alive) try_resurrect(P, s, p, t, m);
+  }
+}

@@ -1,0 +1,256 @@
+"""Problem builder: the multigroup subset of an Abeille input deck as Python
+objects, flattened into the C description of include/abx.h.
+
+The object model follows the deck (src/parser.cpp:77-1006): surfaces with
+ids and boundary conditions, cells with region strings, cell universes and
+rectilinear lattices referring to each other by id, multigroup materials,
+sources, mesh tallies and settings.  `Model.desc()` resolves ids to indices
+and returns an `abx_problem_desc` (ctypes) that stays valid while the model
+object is alive.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+
+
+class Model:
+    def __init__(self):
+        self.surfaces = []      # (type, params, boundary, id)
+        self.cells = []         # dict(id, region, material|universe)
+        self.universes = []     # dict
+        self.materials = []     # dict
+        self.sources = []       # dict
+        self.tallies = []       # dict
+        self.root = None
+        self.settings = dict(mode=abi.K_EIGENVALUE, tracking=abi.SURFACE_TRACKING,
+                             seed=abi.DEFAULT_SEED, stride=abi.DEFAULT_STRIDE,
+                             wgt_cutoff=0.25, wgt_survival=1.0, wgt_split=2.0,
+                             nparticles=100000, energy_bounds=None,
+                             precision=abi.FP64, sample_xs_ratio=None)
+        self._keep = []
+
+    # ---- deck-like API -------------------------------------------------
+    def surface(self, kind, params, boundary="normal", id=None):
+        sid = id if id is not None else len(self.surfaces) + 1
+        bc = {"vacuum": abi.VACUUM, "reflective": abi.REFLECTIVE,
+              "normal": abi.NORMAL}[boundary]
+        self.surfaces.append((abi.SURFACE_TYPES[kind], list(params), bc, sid))
+        return sid
+
+    def cell(self, region, material=None, universe=None, id=None):
+        cid = id if id is not None else len(self.cells) + 1
+        if (material is None) == (universe is None):
+            raise ValueError("cell needs exactly one of material / universe")
+        self.cells.append(dict(id=cid, region=region, material=material,
+                               universe=universe))
+        return cid
+
+    def cell_universe(self, cells, id=None):
+        uid = id if id is not None else len(self.universes) + 1
+        self.universes.append(dict(id=uid, type=abi.CELL_UNIVERSE, cells=list(cells)))
+        return uid
+
+    def rect_lattice(self, shape, pitch, origin, universes, outer=None, id=None):
+        """`universes` lists universe ids (or -1) in the deck's order:
+        index = nz*Nx*Ny + nx*Ny + ny (src/rect_lattice.cpp:300)."""
+        uid = id if id is not None else len(self.universes) + 1
+        if len(universes) != shape[0] * shape[1] * shape[2]:
+            raise ValueError("Lattice instance has improper number of universes.")
+        self.universes.append(dict(id=uid, type=abi.RECT_LATTICE, shape=list(shape),
+                                   pitch=list(pitch), origin=list(origin),
+                                   elems=list(universes), outer=outer))
+        return uid
+
+    def material(self, id, total, absorption, scatter, fission=None, nu=None,
+                 nu_delayed=None, chi=None, delayed_probabilities=None,
+                 delayed_constants=None):
+        G = len(total)
+        chi_arr = None
+        if chi is not None:
+            chi_arr = np.asarray(chi, dtype=np.float64)
+            if chi_arr.ndim == 1:
+                chi_arr = chi_arr[None, :]
+            if chi_arr.shape[0] == 1:  # single row replicated (src/mg_nuclide.cpp:842-853)
+                chi_arr = np.repeat(chi_arr, G, axis=0)
+        self.materials.append(dict(
+            id=id, total=np.asarray(total, np.float64),
+            absorption=np.asarray(absorption, np.float64),
+            scatter=np.ascontiguousarray(np.asarray(scatter, np.float64).reshape(G, G)),
+            fission=None if fission is None else np.asarray(fission, np.float64),
+            nu=None if nu is None else np.asarray(nu, np.float64),
+            nu_delayed=None if nu_delayed is None else np.asarray(nu_delayed, np.float64),
+            chi=None if chi_arr is None else np.ascontiguousarray(chi_arr),
+            dp=None if delayed_probabilities is None else np.asarray(delayed_probabilities, np.float64),
+            dc=None if delayed_constants is None else np.asarray(delayed_constants, np.float64)))
+        return id
+
+    def point_source(self, position, energy, weight=1.0):
+        self.sources.append(dict(spatial=abi.SRC_POINT, low=list(position),
+                                 high=[0, 0, 0], energy=energy, weight=weight,
+                                 fissile_only=False, direction=abi.DIR_ISOTROPIC,
+                                 dir=[1, 0, 0]))
+
+    def box_source(self, low, high, energy, weight=1.0, fissile_only=False):
+        self.sources.append(dict(spatial=abi.SRC_BOX, low=list(low), high=list(high),
+                                 energy=energy, weight=weight,
+                                 fissile_only=fissile_only,
+                                 direction=abi.DIR_ISOTROPIC, dir=[1, 0, 0]))
+
+    def mesh_tally(self, low, high, shape, energy_bounds, estimator="collision",
+                   quantity="flux"):
+        est = {"collision": abi.EST_COLLISION, "track-length": abi.EST_TRACK_LENGTH}[estimator]
+        q = {"flux": abi.Q_FLUX, "elastic": abi.Q_ELASTIC, "absorption": abi.Q_ABSORPTION,
+             "fission": abi.Q_FISSION, "total": abi.Q_TOTAL}[quantity]
+        self.tallies.append(dict(estimator=est, quantity=q, low=list(low),
+                                 high=list(high), shape=list(shape),
+                                 ebounds=np.asarray(energy_bounds, np.float64)))
+        return len(self.tallies) - 1
+
+    def set(self, **kw):
+        for k, v in kw.items():
+            if k not in self.settings:
+                raise KeyError(k)
+            self.settings[k] = v
+
+    @property
+    def ngroups(self):
+        return len(self.settings["energy_bounds"]) - 1
+
+    # ---- flattening ----------------------------------------------------
+    def desc(self):
+        keep = []
+        sidx = {sid: i for i, (_, _, _, sid) in enumerate(self.surfaces)}
+        midx = {m["id"]: i for i, m in enumerate(self.materials)}
+        uidx = {u["id"]: i for i, u in enumerate(self.universes)}
+        cidx = {c["id"]: i for i, c in enumerate(self.cells)}
+
+        surfs = (abi.Surface * max(1, len(self.surfaces)))()
+        for i, (t, p, bc, sid) in enumerate(self.surfaces):
+            surfs[i].type = t
+            surfs[i].boundary = bc
+            surfs[i].id = sid
+            for k, v in enumerate(p):
+                surfs[i].p[k] = v
+
+        rpn_all = []
+        cells = (abi.Cell * max(1, len(self.cells)))()
+        for i, c in enumerate(self.cells):
+            rpn = abi.infix_to_rpn(abi.tokenize_region(c["region"], sidx))
+            cells[i].id = c["id"]
+            if c["material"] is not None:
+                cells[i].fill_type = abi.FILL_MATERIAL
+                cells[i].fill = midx[c["material"]]
+            else:
+                cells[i].fill_type = abi.FILL_UNIVERSE
+                cells[i].fill = uidx[c["universe"]]
+            cells[i].rpn_offset = len(rpn_all)
+            cells[i].rpn_len = len(rpn)
+            rpn_all += rpn
+        rpn_arr = np.asarray(rpn_all if rpn_all else [0], np.int32)
+
+        ucells, elems = [], []
+        unis = (abi.Universe * max(1, len(self.universes)))()
+        for i, u in enumerate(self.universes):
+            unis[i].id = u["id"]
+            unis[i].type = u["type"]
+            unis[i].outer = -1
+            unis[i].elems_offset = -1
+            if u["type"] == abi.CELL_UNIVERSE:
+                unis[i].cells_offset = len(ucells)
+                unis[i].n_cells = len(u["cells"])
+                ucells += [cidx[c] for c in u["cells"]]
+            else:
+                for k in range(3):
+                    unis[i].shape[k] = u["shape"][k]
+                    unis[i].pitch[k] = u["pitch"][k]
+                    unis[i].origin[k] = u["origin"][k]
+                unis[i].elems_offset = len(elems)
+                elems += [(-1 if e == -1 else uidx[e]) for e in u["elems"]]
+                unis[i].outer = -1 if u["outer"] is None else uidx[u["outer"]]
+        ucell_arr = np.asarray(ucells if ucells else [0], np.int32)
+        elem_arr = np.asarray(elems if elems else [0], np.int32)
+
+        mats = (abi.Material * max(1, len(self.materials)))()
+        for i, m in enumerate(self.materials):
+            mats[i].id = m["id"]
+            mats[i].total = abi.ptr(m["total"])
+            mats[i].absorption = abi.ptr(m["absorption"])
+            mats[i].scatter = abi.ptr(m["scatter"])
+            mats[i].fission = abi.ptr(m["fission"])
+            mats[i].nu_prompt = abi.ptr(m["nu"])
+            mats[i].nu_delayed = abi.ptr(m["nu_delayed"])
+            mats[i].chi = abi.ptr(m["chi"])
+            n_d = 0 if m["dp"] is None else len(m["dp"])
+            mats[i].n_delayed = n_d
+            mats[i].delayed_probabilities = abi.ptr(m["dp"])
+            mats[i].delayed_constants = abi.ptr(m["dc"])
+            mats[i].angle_offsets = abi.ptr(None, C.c_int32)
+            mats[i].angle_mu = abi.ptr(None)
+            mats[i].angle_pdf = abi.ptr(None)
+            mats[i].angle_cdf = abi.ptr(None)
+
+        srcs = (abi.Source * max(1, len(self.sources)))()
+        for i, s in enumerate(self.sources):
+            srcs[i].spatial = s["spatial"]
+            srcs[i].direction = s["direction"]
+            srcs[i].fissile_only = 1 if s["fissile_only"] else 0
+            for k in range(3):
+                srcs[i].low[k] = s["low"][k]
+                srcs[i].high[k] = s["high"][k]
+                srcs[i].dir[k] = s["dir"][k]
+            srcs[i].energy = s["energy"]
+            srcs[i].weight = s["weight"]
+
+        tals = (abi.MeshTally * max(1, len(self.tallies)))()
+        for i, t in enumerate(self.tallies):
+            tals[i].estimator = t["estimator"]
+            tals[i].quantity = t["quantity"]
+            for k in range(3):
+                tals[i].shape[k] = t["shape"][k]
+                tals[i].low[k] = t["low"][k]
+                tals[i].high[k] = t["high"][k]
+            tals[i].n_ebounds = len(t["ebounds"])
+            tals[i].ebounds = abi.ptr(t["ebounds"])
+
+        st = self.settings
+        eb = np.asarray(st["energy_bounds"], np.float64)
+        ratio = None if st["sample_xs_ratio"] is None else np.asarray(st["sample_xs_ratio"], np.float64)
+        d = abi.ProblemDesc()
+        d.n_surfaces = len(self.surfaces)
+        d.surfaces = surfs
+        d.n_cells = len(self.cells)
+        d.cells = cells
+        d.n_rpn = len(rpn_all)
+        d.rpn = abi.ptr(rpn_arr, C.c_int32)
+        d.n_universes = len(self.universes)
+        d.universes = unis
+        d.n_universe_cells = len(ucells)
+        d.universe_cells = abi.ptr(ucell_arr, C.c_int32)
+        d.n_lattice_elems = len(elems)
+        d.lattice_elems = abi.ptr(elem_arr, C.c_int32)
+        d.root_universe = uidx[self.root]
+        d.n_materials = len(self.materials)
+        d.materials = mats
+        d.n_sources = len(self.sources)
+        d.sources = srcs
+        d.n_tallies = len(self.tallies)
+        d.tallies = tals
+        s = d.settings
+        s.mode = st["mode"]
+        s.tracking = st["tracking"]
+        s.seed = st["seed"]
+        s.stride = st["stride"]
+        s.wgt_cutoff = st["wgt_cutoff"]
+        s.wgt_survival = st["wgt_survival"]
+        s.wgt_split = st["wgt_split"]
+        s.nparticles = st["nparticles"]
+        s.ngroups = len(eb) - 1
+        s.precision = st["precision"]
+        s.energy_bounds = abi.ptr(eb)
+        s.sample_xs_ratio = abi.ptr(ratio)
+        keep += [surfs, cells, rpn_arr, unis, ucell_arr, elem_arr, mats, srcs, tals, eb,
+                 ratio, d]
+        self._keep = keep
+        return d

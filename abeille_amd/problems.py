@@ -1,0 +1,283 @@
+"""Seeded generators of the benchmark problems of the transport path.
+
+  fixed_source_cube   20x20x20 cm vacuum cube, 2 groups, point source, delta
+                      tracking, 20^3 collision flux tally (the CPU-runnable
+                      fixed-source case).
+  pin_lattice_2d      17x17 pin lattice, pitch 1.26 cm, pin radius 0.54 cm,
+                      reflective, 7 groups, k-eigenvalue, delta tracking,
+                      51x51x7 track-length flux tally.
+  core_3x3            3x3 assemblies of the 17x17 lattice as nested lattices,
+                      0-150 cm axially (vacuum top/bottom), surface tracking,
+                      153x153x30x7 track-length tally.
+  slab_problem        one random fixed-source slab problem (batched form).
+  la13511             analytic one-group cases (LA-13511) with their published
+                      k values, used as known answers.
+
+Cross sections of the lattice problems are drawn from a numpy Generator with
+a fixed seed: per group sigma_t ~ U[0.2, 2.0], scattering ratio ~ U[0.5, 0.99],
+scatter rows downscatter-dominant with <= 10 % upscatter from the two lowest
+groups only; the fuel has nu*sigma_f ~ U[0.005, 0.3] written as nu = 2.5 and
+chi concentrated in the first three groups.  Sigma_f <= Sigma_a is enforced
+by raising Sigma_a (src/mg_nuclide.cpp:267-272 requires it).
+"""
+import numpy as np
+
+from . import abi
+from .model import Model
+
+# The 25 guide-tube / instrument positions of a 17x17 PWR assembly
+# (row = x index, column = y index; the layout of input_files/c5g7.yaml:257-273).
+GT_POSITIONS = [(2, 5), (2, 8), (2, 11), (3, 3), (3, 13), (5, 2), (5, 5), (5, 8),
+                (5, 11), (5, 14), (8, 2), (8, 5), (8, 8), (8, 11), (8, 14), (11, 2),
+                (11, 5), (11, 8), (11, 11), (11, 14), (13, 3), (13, 13), (14, 5),
+                (14, 8), (14, 11)]
+
+
+def seven_group_xs(seed=20240607, G=7):
+    """Fuel, moderator and guide-tube cross sections (dicts of arrays)."""
+    rng = np.random.default_rng(seed)
+    mats = {}
+    for name in ("fuel", "moderator", "guide_tube"):
+        st = rng.uniform(0.2, 2.0, G)
+        c = rng.uniform(0.5, 0.99, G)
+        sa = st * (1.0 - c)
+        nusf = rng.uniform(0.005, 0.3, G) if name == "fuel" else None
+        sf = None
+        if nusf is not None:
+            sf = nusf / 2.5
+            sa = np.maximum(sa, sf)
+        ss = st - sa
+        scat = np.zeros((G, G))
+        for g in range(G):
+            w = np.zeros(G)
+            w[g] = rng.uniform(0.5, 1.0)               # in-group
+            for o in range(g + 1, G):                   # downscatter, decaying
+                w[o] = rng.uniform(0.05, 0.4) * 0.5 ** (o - g - 1)
+            w = w / w.sum()
+            if g >= G - 2:                              # groups 6-7: upscatter
+                up = rng.uniform(0.0, 0.10)
+                w *= (1.0 - up)
+                w[g - 1] += up
+            scat[g] = ss[g] * w
+        # make Sigma_t exactly Sigma_a + row sums so check_xs passes tightly
+        st = sa + scat.sum(axis=1)
+        m = dict(total=st, absorption=sa, scatter=scat)
+        if sf is not None:
+            chi = np.zeros(G)
+            chi[:3] = rng.uniform(0.2, 1.0, 3)
+            chi[3:] = rng.uniform(0.0, 0.02, G - 3)
+            chi /= chi.sum()
+            m.update(fission=sf, nu=np.full(G, 2.5), chi=chi)
+        mats[name] = m
+    return mats
+
+
+def fixed_source_cube(nparticles=25000, tracking="delta"):
+    """20x20x20 cm vacuum cube of one 2-group material, point source at the
+    centre in group 1 (E = 0.5), collision-estimator flux on 20^3 x 2."""
+    m = Model()
+    s = [m.surface("xplane", [-10.], "vacuum"), m.surface("xplane", [10.], "vacuum"),
+         m.surface("yplane", [-10.], "vacuum"), m.surface("yplane", [10.], "vacuum"),
+         m.surface("zplane", [-10.], "vacuum"), m.surface("zplane", [10.], "vacuum")]
+    m.material(1, total=[0.5, 1.2], absorption=[0.05, 0.20],
+               scatter=[[0.30, 0.15], [0.00, 1.00]])
+    m.cell(f"+{s[0]} & -{s[1]} & +{s[2]} & -{s[3]} & +{s[4]} & -{s[5]}", material=1, id=1)
+    m.root = m.cell_universe([1], id=1)
+    m.point_source([0., 0., 0.], energy=0.5)
+    m.mesh_tally([-10., -10., -10.], [10., 10., 10.], [20, 20, 20], [0., 1., 2.],
+                 estimator="collision")
+    m.set(mode=abi.FIXED_SOURCE, nparticles=nparticles, energy_bounds=[0., 1., 2.],
+          tracking=_tracking(tracking))
+    return m
+
+
+def _tracking(name):
+    return {"delta": abi.DELTA_TRACKING, "surface": abi.SURFACE_TRACKING,
+            "carter": abi.CARTER_TRACKING}[name]
+
+
+def _pin_universes(m, fuel_id, mod_id, gt_id, cyl):
+    """Fuel-pin and guide-tube-pin cell universes inside one cylinder."""
+    cf = m.cell(f"-{cyl}", material=fuel_id, id=101)
+    cm = m.cell(f"+{cyl}", material=mod_id, id=102)
+    cg = m.cell(f"-{cyl}", material=gt_id, id=103)
+    cm2 = m.cell(f"+{cyl}", material=mod_id, id=104)
+    u_fuel = m.cell_universe([cf, cm], id=1)
+    u_gt = m.cell_universe([cg, cm2], id=2)
+    return u_fuel, u_gt
+
+
+def _assembly_elems(u_fuel, u_gt):
+    elems = []
+    gt = set(GT_POSITIONS)
+    for nx in range(17):          # x runs down the list, y fastest
+        for ny in range(17):
+            elems.append(u_gt if (nx, ny) in gt else u_fuel)
+    return elems
+
+
+def _add_lattice_materials(m, xs):
+    m.material(1, **xs["fuel"])
+    m.material(2, **xs["moderator"])
+    m.material(3, **xs["guide_tube"])
+    return 1, 2, 3
+
+
+def pin_lattice_2d(nparticles=1_000_000, tracking="delta", xs_seed=20240607,
+                   tally=True):
+    """17x17 reflective pin lattice, 7 groups, k-eigenvalue.  The lattice is
+    the root universe; its outer universe holds the reflective box, so
+    boundary conditions are found through Lattice::get_boundary_condition
+    (src/lattice.cpp:77-91) as in input_files/c5g7.yaml."""
+    G = 7
+    m = Model()
+    xs = seven_group_xs(xs_seed, G)
+    fuel, mod, gt = _add_lattice_materials(m, xs)
+    cyl = m.surface("zcylinder", [0., 0., 0.54], id=1)
+    h = 17 * 1.26 / 2.0  # 10.71
+    b = [m.surface("xplane", [-h], "reflective", id=11),
+         m.surface("xplane", [h], "reflective", id=12),
+         m.surface("yplane", [-h], "reflective", id=13),
+         m.surface("yplane", [h], "reflective", id=14),
+         m.surface("zplane", [-h], "reflective", id=15),
+         m.surface("zplane", [h], "reflective", id=16)]
+    u_fuel, u_gt = _pin_universes(m, fuel, mod, gt, cyl)
+    cbox = m.cell(f"+{b[0]} & -{b[1]} & +{b[2]} & -{b[3]} & +{b[4]} & -{b[5]}",
+                  material=mod, id=105)
+    u_out = m.cell_universe([cbox], id=3)
+    m.root = m.rect_lattice([17, 17, 1], [1.26, 1.26, 2 * h], [0., 0., 0.],
+                            _assembly_elems(u_fuel, u_gt), outer=u_out, id=10)
+    m.box_source([-h, -h, -h], [h, h, h], energy=0.5, fissile_only=True)
+    if tally:
+        m.mesh_tally([-h, -h, -h], [h, h, h], [51, 51, 1], np.arange(G + 1.0),
+                     estimator="track-length")
+    m.set(mode=abi.K_EIGENVALUE, nparticles=nparticles,
+          energy_bounds=np.arange(G + 1.0), tracking=_tracking(tracking))
+    return m
+
+
+def core_3x3(nparticles=2_000_000, tracking="surface", xs_seed=20240607,
+             precision=abi.FP64, tally=True):
+    """3x3 core of the 17x17 assembly as nested lattices, z in [0, 150] cm
+    with vacuum top and bottom and reflective radial faces."""
+    G = 7
+    m = Model()
+    xs = seven_group_xs(xs_seed, G)
+    fuel, mod, gt = _add_lattice_materials(m, xs)
+    cyl = m.surface("zcylinder", [0., 0., 0.54], id=1)
+    H = 3 * 17 * 1.26 / 2.0  # 32.13
+    b = [m.surface("xplane", [-H], "reflective", id=11),
+         m.surface("xplane", [H], "reflective", id=12),
+         m.surface("yplane", [-H], "reflective", id=13),
+         m.surface("yplane", [H], "reflective", id=14),
+         m.surface("zplane", [0.], "vacuum", id=15),
+         m.surface("zplane", [150.], "vacuum", id=16)]
+    u_fuel, u_gt = _pin_universes(m, fuel, mod, gt, cyl)
+    cbox = m.cell(f"+{b[0]} & -{b[1]} & +{b[2]} & -{b[3]} & +{b[4]} & -{b[5]}",
+                  material=mod, id=105)
+    u_out = m.cell_universe([cbox], id=3)
+    assy = m.rect_lattice([17, 17, 1], [1.26, 1.26, 150.], [0., 0., 0.],
+                          _assembly_elems(u_fuel, u_gt), outer=None, id=10)
+    m.root = m.rect_lattice([3, 3, 1], [21.42, 21.42, 150.], [0., 0., 75.],
+                            [assy] * 9, outer=u_out, id=20)
+    m.box_source([-H, -H, 0.], [H, H, 150.], energy=0.5, fissile_only=True)
+    if tally:
+        m.mesh_tally([-H, -H, 0.], [H, H, 150.], [153, 153, 30], np.arange(G + 1.0),
+                     estimator="track-length")
+    m.set(mode=abi.K_EIGENVALUE, nparticles=nparticles,
+          energy_bounds=np.arange(G + 1.0), tracking=_tracking(tracking),
+          precision=precision)
+    return m
+
+
+def la13511(case, nparticles=100_000, tracking="surface"):
+    """One-group LA-13511 cases with analytic k: 'PUa-1-0-IN' (2.612903),
+    'PUb-1-0-IN' (2.290323), 'PUa-1-0-SL' (1.0), 'PUb-1-0-SL' (1.0),
+    'Ua-1-1-IN' (2.25).  Infinite media are reflective 10 cm boxes; slabs are
+    vacuum in x, reflective in y and z."""
+    data = {
+        "PUa-1-0-IN": (0.32640, 0.101184, 0.081600, 3.24, 0.225216, None, 2.612903),
+        "PUb-1-0-IN": (0.32640, 0.101184, 0.081600, 2.84, 0.225216, None, 2.290323),
+        "PUa-1-0-SL": (0.32640, 0.101184, 0.081600, 3.24, 0.225216, 1.853722, 1.0),
+        "PUb-1-0-SL": (0.32640, 0.101184, 0.081600, 2.84, 0.225216, 2.256751, 1.0),
+        "Ua-1-1-IN": (0.32640, 0.078336, 0.065280, 2.7, 0.248064, None, 2.25),
+    }
+    st, sa, sf, nu, ss, half, k = data[case]
+    m = Model()
+    m.material(1, total=[st], absorption=[sa], scatter=[[ss]], fission=[sf], nu=[nu],
+               chi=[[1.0]])
+    xb = "vacuum" if half else "reflective"
+    hx = half if half else 5.0
+    s = [m.surface("xplane", [-hx], xb), m.surface("xplane", [hx], xb),
+         m.surface("yplane", [-5.], "reflective"), m.surface("yplane", [5.], "reflective"),
+         m.surface("zplane", [-5.], "reflective"), m.surface("zplane", [5.], "reflective")]
+    m.cell(f"+{s[0]} & -{s[1]} & +{s[2]} & -{s[3]} & +{s[4]} & -{s[5]}", material=1, id=1)
+    m.root = m.cell_universe([1], id=1)
+    m.point_source([0., 0., 0.], energy=0.5)
+    m.set(mode=abi.K_EIGENVALUE, nparticles=nparticles, energy_bounds=[0., 1.],
+          tracking=_tracking(tracking))
+    return m, k
+
+
+def infinite_medium_2g(nparticles=20000, tracking="delta", mode=abi.FIXED_SOURCE):
+    """The cube's 2-group material in a reflective 20 cm box: the volume-
+    integrated collision-estimator flux per source particle is
+    phi_1 = 1/(St1 - Ss11) = 5.0 cm and phi_2 = Ss12 phi_1/(St2 - Ss22) = 3.75 cm."""
+    m = fixed_source_cube(nparticles, tracking)
+    m.surfaces = [(t, p, abi.REFLECTIVE, sid) for (t, p, _, sid) in m.surfaces]
+    m.tallies = []
+    m.mesh_tally([-10., -10., -10.], [10., 10., 10.], [1, 1, 1], [0., 1., 2.],
+                 estimator="collision")
+    m.mesh_tally([-10., -10., -10.], [10., 10., 10.], [1, 1, 1], [0., 1., 2.],
+                 estimator="track-length")
+    m.set(mode=mode)
+    return m
+
+
+def slab_problem(rng, G=None, tracking=None, nhist=None, fission_free=None):
+    """One batched-form problem: a box of 1-8 x-stacked slab regions with a
+    vacuum outer boundary, G in {2,4,7,16,33}, per-group XS from the lattice
+    distributions (fission-free subset), a point source in the first slab,
+    and a 16^3 collision flux tally.  Returns (model, histories)."""
+    if G is None:
+        G = int(rng.choice([2, 4, 7, 16, 33]))
+    nreg = int(rng.integers(1, 9))
+    if tracking is None:
+        tracking = "delta" if rng.random() < 0.5 else "surface"
+    if nhist is None:
+        nhist = int(np.exp(rng.uniform(np.log(1e3), np.log(1e6))))
+    if fission_free is None:
+        fission_free = True
+    m = Model()
+    width = rng.uniform(2.0, 10.0, nreg)
+    x = np.concatenate([[0.0], np.cumsum(width)])
+    x -= x[-1] / 2.0
+    hy = float(rng.uniform(5.0, 20.0))
+    sx = [m.surface("xplane", [float(v)], "vacuum" if i in (0, nreg) else "normal")
+          for i, v in enumerate(x)]
+    sy = [m.surface("yplane", [-hy], "vacuum"), m.surface("yplane", [hy], "vacuum")]
+    sz = [m.surface("zplane", [-hy], "vacuum"), m.surface("zplane", [hy], "vacuum")]
+    cells = []
+    for r in range(nreg):
+        st = rng.uniform(0.2, 2.0, G)
+        c = rng.uniform(0.5, 0.99, G)
+        sa = st * (1 - c)
+        scat = np.zeros((G, G))
+        for g in range(G):
+            w = np.zeros(G)
+            w[g] = rng.uniform(0.5, 1.0)
+            for o in range(g + 1, min(G, g + 4)):
+                w[o] = rng.uniform(0.05, 0.4)
+            w /= w.sum()
+            scat[g] = (st[g] - sa[g]) * w
+        st = sa + scat.sum(axis=1)
+        m.material(r + 1, total=st, absorption=sa, scatter=scat)
+        cells.append(m.cell(f"+{sx[r]} & -{sx[r + 1]} & +{sy[0]} & -{sy[1]} & "
+                            f"+{sz[0]} & -{sz[1]}", material=r + 1, id=r + 1))
+    m.root = m.cell_universe(cells, id=1)
+    m.point_source([float(0.5 * (x[0] + x[1])), 0., 0.], energy=0.5)
+    m.mesh_tally([float(x[0]), -hy, -hy], [float(x[-1]), hy, hy], [16, 16, 16],
+                 np.arange(G + 1.0), estimator="collision")
+    m.set(mode=abi.FIXED_SOURCE, nparticles=nhist, energy_bounds=np.arange(G + 1.0),
+          tracking=_tracking(tracking))
+    return m, nhist

@@ -1,0 +1,384 @@
+/*
+ * abx.h - C ABI of the MI355X multigroup transport path.
+ *
+ * The boundary is the reference's Transporter::transport() plus the mesh /
+ * scalar tally readback (include/simulation/transporter.hpp:39-87,
+ * include/simulation/tallies.hpp:38-186).  A caller describes the problem once
+ * in the flat form below (what parse_input_file() builds in the reference,
+ * src/parser.cpp:77-137), then hands banks of particles to abx_transport()
+ * and reads tallies back between batches / generations.
+ *
+ * Conventions
+ *  - plain C structs, caller-owned host buffers, no torch types;
+ *  - every entry point returns ABX_OK (0) or a negative ABX_E* code; the
+ *    reference's fatal_error() sites (src/error.cpp:36-45) become codes and
+ *    abx_last_error() gives the message;
+ *  - one host thread per abx_problem; calls are synchronous on the problem's
+ *    own HIP stream;
+ *  - indices, not ids: tokens, fills and universe references are 0-based
+ *    indices into the arrays of the description (ids are kept for output).
+ */
+#ifndef ABX_H
+#define ABX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ABX_OK 0
+#define ABX_E_INVALID (-1)     /* bad description or argument */
+#define ABX_E_HIP (-2)         /* HIP runtime failure */
+#define ABX_E_LOST (-3)        /* particle lost (fatal_error in reference) */
+#define ABX_E_MAJORANT (-4)    /* Sigma_t exceeded the majorant */
+#define ABX_E_CAPACITY (-5)    /* output bank / secondary stack too small */
+#define ABX_E_NO_DEVICE (-6)   /* no gfx950 device */
+#define ABX_E_BUDGET (-7)      /* serving: request does not fit HBM budget */
+
+/* ---- geometry (include/geometry/) ------------------------------------ */
+
+/* surface types: src/{x,y,z}plane.cpp, src/plane.cpp, src/{x,y,z}cylinder.cpp,
+ * src/sphere.cpp */
+enum abx_surface_type {
+  ABX_XPLANE = 0, /* p = {x0}            */
+  ABX_YPLANE = 1, /* p = {y0}            */
+  ABX_ZPLANE = 2, /* p = {z0}            */
+  ABX_PLANE = 3,  /* p = {A, B, C, D}    */
+  ABX_XCYL = 4,   /* p = {y0, z0, R}     */
+  ABX_YCYL = 5,   /* p = {x0, z0, R}     */
+  ABX_ZCYL = 6,   /* p = {x0, y0, R}     */
+  ABX_SPHERE = 7  /* p = {x0, y0, z0, R} */
+};
+
+/* boundary types: include/geometry/surfaces/surface.hpp:34 (same order) */
+enum abx_boundary { ABX_VACUUM = 0, ABX_REFLECTIVE = 1, ABX_NORMAL = 2 };
+
+typedef struct abx_surface {
+  int32_t type;     /* abx_surface_type */
+  int32_t boundary; /* abx_boundary */
+  uint32_t id;
+  int32_t pad_;
+  double p[4];
+} abx_surface;
+
+/* Cell region tokens in reverse Polish notation (src/cell.cpp:248-294):
+ * +-(surface index + 1) for half-spaces, or one of the operators below
+ * (include/geometry/cell.hpp:42-48). */
+#define ABX_OP_UNIN (2147483647 - 4)
+#define ABX_OP_INTR (2147483647 - 3)
+#define ABX_OP_COMP (2147483647 - 2)
+
+enum abx_fill { ABX_FILL_MATERIAL = 0, ABX_FILL_UNIVERSE = 1 };
+
+typedef struct abx_cell {
+  uint32_t id;
+  int32_t fill_type; /* abx_fill */
+  int32_t fill;      /* material index or universe index */
+  int32_t rpn_offset; /* into abx_problem_desc.rpn */
+  int32_t rpn_len;
+} abx_cell;
+
+enum abx_universe_type { ABX_CELL_UNIVERSE = 0, ABX_RECT_LATTICE = 1 };
+
+typedef struct abx_universe {
+  uint32_t id;
+  int32_t type;         /* abx_universe_type */
+  /* cell universe (src/cell_universe.cpp) */
+  int32_t cells_offset; /* into abx_problem_desc.universe_cells */
+  int32_t n_cells;
+  /* rectilinear lattice (src/rect_lattice.cpp:33-52); `origin` is the
+   * lattice centre as in the deck, elements are x-major, y fastest:
+   * index = nz*Nx*Ny + nx*Ny + ny (src/rect_lattice.cpp:300) */
+  int32_t shape[3];
+  int32_t elems_offset; /* into abx_problem_desc.lattice_elems; -1 = dummy */
+  int32_t outer;        /* outer universe index or -1 */
+  int32_t pad_;
+  double pitch[3];
+  double origin[3];
+} abx_universe;
+
+/* ---- multigroup materials (src/mg_nuclide.cpp:579-922) ----------------- */
+/* One nuclide per material at 1 atom/b-cm (src/material.cpp:51-63).  All
+ * arrays point at the raw deck values; derived tables are built by the
+ * library exactly as MGNuclide's constructor does. */
+typedef struct abx_material {
+  uint32_t id;
+  int32_t n_delayed;          /* delayed groups (0 = none) */
+  const double* total;        /* [G] */
+  const double* absorption;   /* [G] */
+  const double* scatter;      /* [G][G] row = incoming group */
+  const double* fission;      /* [G] or NULL */
+  const double* nu_prompt;    /* [G] or NULL ("nu" in the deck) */
+  const double* nu_delayed;   /* [G] or NULL */
+  const double* chi;          /* [G][G] (a single row may be replicated) or NULL */
+  const double* delayed_probabilities; /* [n_delayed] */
+  const double* delayed_constants;     /* [n_delayed] */
+  /* optional piecewise-linear scattering cosine tables per (g_in, g_out)
+   * (MGAngleDistribution, include/materials/mg_angle_distribution.hpp);
+   * NULL = isotropic everywhere.  angle_offsets has G*G+1 entries into the
+   * mu/pdf/cdf arrays. */
+  const int32_t* angle_offsets;
+  const double* angle_mu;
+  const double* angle_pdf;
+  const double* angle_cdf;
+} abx_material;
+
+/* ---- sources (src/source.cpp:44-90) ------------------------------------ */
+enum abx_spatial { ABX_SRC_POINT = 0, ABX_SRC_BOX = 1 };
+enum abx_direction { ABX_DIR_ISOTROPIC = 0, ABX_DIR_MONO = 1 };
+
+typedef struct abx_source {
+  int32_t spatial;      /* abx_spatial */
+  int32_t direction;    /* abx_direction */
+  int32_t fissile_only;
+  int32_t pad_;
+  double low[3];        /* point position, or box low corner */
+  double high[3];       /* box high corner */
+  double dir[3];        /* mono-directional direction */
+  double energy;        /* mono-energetic energy (MeV) */
+  double weight;        /* source weight (picks among sources) */
+} abx_source;
+
+/* ---- mesh tallies (src/mesh_tally.cpp, src/collision_mesh_tally.cpp,
+ *      src/track_length_mesh_tally.cpp) ---------------------------------- */
+enum abx_estimator { ABX_EST_COLLISION = 0, ABX_EST_TRACK_LENGTH = 1 };
+enum abx_quantity {
+  ABX_Q_FLUX = 0,
+  ABX_Q_ELASTIC = 1,
+  ABX_Q_ABSORPTION = 2,
+  ABX_Q_FISSION = 3,
+  ABX_Q_TOTAL = 4,
+  ABX_Q_REAL_FLUX = 6,
+  ABX_Q_IMG_FLUX = 7
+};
+
+typedef struct abx_mesh_tally {
+  int32_t estimator; /* abx_estimator */
+  int32_t quantity;  /* abx_quantity */
+  int32_t shape[3];  /* Nx, Ny, Nz */
+  int32_t n_ebounds; /* energy bin edges (>= 2) */
+  double low[3];
+  double high[3];
+  const double* ebounds;
+} abx_mesh_tally;
+
+/* ---- settings (src/settings.cpp:34-114) -------------------------------- */
+enum abx_mode {
+  ABX_K_EIGENVALUE = 0,
+  ABX_FIXED_SOURCE = 1,
+  ABX_MODIFIED_FIXED_SOURCE = 2
+};
+enum abx_tracking {
+  ABX_SURFACE_TRACKING = 0,
+  ABX_DELTA_TRACKING = 1,
+  ABX_CARTER_TRACKING = 2
+};
+
+typedef struct abx_settings {
+  int32_t mode;        /* abx_mode */
+  int32_t tracking;    /* abx_tracking */
+  uint64_t seed;       /* default 19073486328125 */
+  uint64_t stride;     /* default 152917 */
+  double wgt_cutoff;   /* default 0.25 */
+  double wgt_survival; /* default 1.0 */
+  double wgt_split;    /* default 2.0 */
+  int64_t nparticles;  /* per batch / generation; tallies divide by it */
+  int32_t ngroups;
+  int32_t precision;   /* abx_precision */
+  const double* energy_bounds;   /* [ngroups + 1] */
+  const double* sample_xs_ratio; /* [ngroups], carter tracking only */
+} abx_settings;
+
+/* plain: FP64 throughout.  split: cross sections, directions and flight
+ * distances in FP32; positions, surface tests and tallies FP64. */
+enum abx_precision { ABX_FP64 = 0, ABX_SPLIT = 1 };
+
+typedef struct abx_problem_desc {
+  int32_t n_surfaces;
+  const abx_surface* surfaces;
+  int32_t n_cells;
+  const abx_cell* cells;
+  int32_t n_rpn;
+  const int32_t* rpn;
+  int32_t n_universes;
+  const abx_universe* universes;
+  int32_t n_universe_cells;
+  const int32_t* universe_cells;
+  int32_t n_lattice_elems;
+  const int32_t* lattice_elems; /* universe index or -1 */
+  int32_t root_universe;
+  int32_t n_materials;
+  const abx_material* materials;
+  int32_t n_sources;
+  const abx_source* sources;
+  int32_t n_tallies;
+  const abx_mesh_tally* tallies;
+  abx_settings settings;
+} abx_problem_desc;
+
+/* ---- particles at the boundary ----------------------------------------- */
+
+/* A history handed to transport(): the reference's Particle right after
+ * source sampling or re-banking (include/simulation/particle.hpp:68-242).
+ * rng_state is the CURRENT pcg32 state (source sampling has advanced it). */
+typedef struct abx_particle {
+  double r[3];
+  double u[3];
+  double E;
+  double wgt;
+  double wgt2;
+  uint64_t history_id;
+  uint64_t family_id;
+  uint64_t rng_state;
+} abx_particle;
+
+/* A fission daughter: the reference's BankedParticle
+ * (include/simulation/particle.hpp:38-66) without the exact-cancellation
+ * bookkeeping fields, in parent-bank order then push order. */
+typedef struct abx_banked {
+  double r[3];
+  double u[3];
+  double E;
+  double wgt;
+  double wgt2;
+  uint64_t parent_history_id;
+  uint64_t parent_daughter_id;
+  uint64_t family_id;
+} abx_banked;
+
+/* Per-call scalar scores, as sums (divide by nparticles for the per-
+ * generation values, src/tallies.cpp:169-174) - ThreadLocalScores in
+ * include/simulation/transporter.hpp:52-59. */
+typedef struct abx_scores {
+  double k_col, k_abs, k_trk, k_tot, leakage, mig;
+} abx_scores;
+
+typedef struct abx_diag {
+  uint64_t histories;          /* source histories completed */
+  uint64_t collisions;         /* real collisions */
+  uint64_t virtual_collisions; /* delta / carter rejections */
+  uint64_t flights;            /* flight events (all trackers) */
+  uint64_t crossings;          /* surface crossings + reflections */
+  uint64_t tally_updates;      /* mesh-tally atomic updates */
+  uint64_t lost;               /* histories lost at birth (killed) */
+  uint64_t secondaries;        /* secondaries resurrected */
+  int32_t error;               /* first fatal code seen, 0 if none */
+  int32_t pad_;
+  uint64_t error_history;      /* history id of the first offender */
+} abx_diag;
+
+/* ---- entry points -------------------------------------------------------- */
+typedef struct abx_problem abx_problem;
+
+/* Which kernel family transport uses. */
+enum abx_engine {
+  ABX_ENGINE_EVENT = 0,   /* event-based: SoA banks, per-event kernels */
+  ABX_ENGINE_HISTORY = 1  /* one lane per history (parity vehicle) */
+};
+
+const char* abx_last_error(void);
+const char* abx_version(void);
+
+/* Replaces parse_input_file()'s object graph (src/parser.cpp:77-137) and the
+ * DeltaTracker/CarterTracker majorant set-up (src/delta_tracker.cpp:50-70,
+ * src/majorant.cpp:133-176): validates, builds derived tables, stages them in
+ * HBM.  The description's arrays may be freed after the call. */
+int abx_create(const abx_problem_desc* desc, abx_problem** out);
+void abx_destroy(abx_problem* p);
+int abx_set_engine(abx_problem* p, int engine);
+
+/* Simulation::sample_sources (src/simulation.cpp:55-77): n source particles
+ * for histories first_history .. first_history+n-1, sampled on the GPU. */
+int abx_sample_sources(abx_problem* p, uint64_t first_history, size_t n,
+                       abx_particle* out);
+
+/* Transporter::transport (include/simulation/transporter.hpp:44-47,
+ * src/delta_tracker.cpp:72-265, src/surface_tracker.cpp:40-221,
+ * src/carter_tracker.cpp:92-294).  `bank` is consumed (read only here);
+ * fission daughters (k-eigenvalue / modified fixed source) are written in
+ * the reference's order to `fission_out` (capacity `cap`, count in *n_out).
+ * `kcol` is Tallies::kcol() of the previous generation; `converged` gates
+ * the mesh tallies (Tallies::score_collision / score_flight).  Scores add to
+ * the problem's current-generation tallies. */
+int abx_transport(abx_problem* p, const abx_particle* bank, size_t n,
+                  double kcol, int converged, abx_banked* fission_out,
+                  size_t cap, size_t* n_out, abx_scores* scores,
+                  abx_diag* diag);
+
+/* MeshTally generation array [Ne][Nx][Ny][Nz] (src/mesh_tally.cpp:106). */
+int abx_tally_size(abx_problem* p, int tally, size_t* n_bins);
+int abx_tally_fetch(abx_problem* p, int tally, double* gen_out);
+/* MeshTally::clear_generation / Tallies::clear_generation */
+int abx_tally_clear(abx_problem* p);
+
+/* ---- device-resident drivers (the reference's Simulation loops) ---------- */
+
+/* Result of a whole run: Tallies' running statistics. */
+typedef struct abx_run_stats {
+  double kcol_avg, kcol_err;
+  double ktrk_avg, ktrk_err;
+  double leak_avg, leak_err;
+  double mig_avg, mig_err;
+  int32_t active_generations;
+  int32_t pad_;
+  double transport_seconds; /* GPU time of the transport phase */
+  abx_diag diag;            /* summed over the run */
+} abx_run_stats;
+
+/* FixedSource::run (src/fixed_source.cpp:81-173) / PowerIterator::run
+ * (src/power_iterator.cpp:305-429) with banks resident in HBM.  kcol_hist
+ * (optional, [ngenerations]) receives k_col per generation.  Mesh-tally mean
+ * and std-error arrays follow MeshTally::record_generation
+ * (src/mesh_tally.cpp:121-150) and write_tally (:191-197). */
+int abx_run(abx_problem* p, int ngenerations, int nignored,
+            double* kcol_hist, abx_run_stats* stats);
+int abx_tally_stats(abx_problem* p, int tally, double* avg, double* std_err);
+
+/* Device bank of the last k-eigenvalue generation in the reference's
+ * source-file layout [N][9] = x,y,z,ux,uy,uz,E,w,w2
+ * (src/simulation.cpp:137-175). */
+int abx_bank_size(abx_problem* p, size_t* n);
+int abx_bank_fetch(abx_problem* p, double* out9);
+
+/* ---- per-history traces (parity) ----------------------------------------- */
+/* One record per loop iteration of a history (after the flight / collision
+ * decision), for bit-for-bit comparison with the CPU restatement. */
+typedef struct abx_trace_rec {
+  double r[3];
+  double u[3];
+  double wgt;
+  uint64_t history_id;
+  int32_t secondary_id;
+  int32_t event;     /* abx_event */
+  int32_t cell;      /* cell index, -1 lost */
+  int32_t material;  /* material index, -1 */
+  int32_t group;
+  int32_t rng_calls; /* 32-bit draws consumed by the history so far */
+} abx_trace_rec;
+
+enum abx_event {
+  ABX_EV_VIRTUAL = 0,   /* delta/carter virtual collision */
+  ABX_EV_COLLISION = 1, /* real collision (after physics) */
+  ABX_EV_CROSS = 2,     /* surface / lattice crossing */
+  ABX_EV_REFLECT = 3,   /* reflective boundary */
+  ABX_EV_LEAK = 4,      /* vacuum boundary */
+  ABX_EV_DEATH = 5,     /* killed at collision (roulette) */
+  ABX_EV_RESURRECT = 6  /* secondary popped */
+};
+
+/* Transport with tracing on: at most `cap_per_history` records per history
+ * in `trace` ([n][cap_per_history]); n_rec[i] gets the count (may exceed cap,
+ * extra records are dropped). */
+int abx_transport_trace(abx_problem* p, const abx_particle* bank, size_t n,
+                        double kcol, int converged, abx_banked* fission_out,
+                        size_t cap, size_t* n_out, abx_scores* scores,
+                        abx_trace_rec* trace, int cap_per_history,
+                        int32_t* n_rec);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ABX_H */

@@ -1,0 +1,2265 @@
+// oracle.cpp - CPU restatement of Abeille's multigroup transport path.
+//
+// TEST INFRASTRUCTURE ONLY (see oracle.h): the checker the HIP path is
+// compared with, and the CPU baseline.  Written from the reference sources;
+// every function names the reference lines it restates.  Structure follows
+// the reference on purpose (history-based OpenMP loop, a Tracker with a
+// vector of lily pads, a discrete_distribution built per sample) so that it
+// behaves - and costs - like the reference's CPU path.
+//
+// Parity status of each part (how it is pinned, see tests/):
+//  * pcg32 / rand / exponential / discrete: golden vectors from pcg-cpp and
+//    libstdc++ (tests/golden/rng_golden.json) + O'Neill's demo output.
+//  * log/sin/cos: glibc, or the portable detmath.h checked <= 1 ulp to glibc.
+//  * surfaces, cells, lattices, tracker: closed-form distances, brute-force
+//    point location, and face tie-break cases (tests/test_oracle_geometry.py).
+//  * collision physics and tallies: closed forms (infinite-medium fluxes,
+//    k-infinity, leakage + absorption balance, exact ray/mesh segment
+//    lengths) and the LA-13511 analytic k values.
+#include "oracle.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <string>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../abeille_amd/csrc/detmath.h"
+
+namespace orc {
+
+// include/utils/constants.hpp:42-57
+constexpr double INF = DBL_MAX;
+constexpr double PI = 3.14159265358979323846264338327950288;
+constexpr double SURFACE_COINCIDENT = 1E-12;
+constexpr double BOUNDRY_TOL = 500. * SURFACE_COINCIDENT;
+
+static thread_local std::string g_err;
+static bool g_portable = true;
+
+static double LOG(double x) { return g_portable ? dm_log(x) : std::log(x); }
+static double SIN(double x) { return g_portable ? dm_sin(x) : std::sin(x); }
+static double COS(double x) { return g_portable ? dm_cos(x) : std::cos(x); }
+
+struct Fatal {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] static void fatal(int code, const std::string& m) {
+  throw Fatal{code, m};
+}
+
+//==========================================================================
+// Vector math: include/utils/{vector,position,direction}.hpp
+struct V3 {
+  double x, y, z;
+};
+static inline V3 add(const V3& a, const V3& b) {
+  return {a.x + b.x, a.y + b.y, a.z + b.z};
+}
+static inline V3 sub(const V3& a, const V3& b) {
+  return {a.x - b.x, a.y - b.y, a.z - b.z};
+}
+static inline V3 scale(const V3& v, double d) {
+  return {v.x * d, v.y * d, v.z * d};
+}
+static inline double dot(const V3& a, const V3& b) {
+  return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+static inline double norm(const V3& a) {
+  return std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
+}
+// Direction(x, y, z): normalising constructor (direction.hpp:37-42)
+static inline V3 direction(double x, double y, double z) {
+  V3 v{x, y, z};
+  double m = norm(v);
+  return {x / m, y / m, z / m};
+}
+// Direction(mu, phi) (direction.hpp:43-62)
+static inline V3 direction_mu_phi(double mu, double phi) {
+  if (mu < -1.)
+    mu = -1.;
+  else if (mu > 1.)
+    mu = 1.;
+  if (phi < 0.)
+    phi = 0.;
+  else if (phi > 2 * PI)
+    phi = 2 * PI;
+  double x = std::sqrt(1. - mu * mu) * COS(phi);
+  double y = std::sqrt(1. - mu * mu) * SIN(phi);
+  double z = mu;
+  return direction(x, y, z);
+}
+// rotate_direction (direction.hpp:130-151)
+static inline V3 rotate_direction(V3 u, double mu, double phi) {
+  double c = COS(phi);
+  double s = SIN(phi);
+  double sqrt_mu = std::sqrt(1. - mu * mu);
+  double sqrt_w = std::sqrt(1. - u.z * u.z);
+  double ux, uy, uz;
+  if (sqrt_w > 1.E-10) {
+    ux = mu * u.x + sqrt_mu * (u.x * u.z * c - u.y * s) / sqrt_w;
+    uy = mu * u.y + sqrt_mu * (u.y * u.z * c + u.x * s) / sqrt_w;
+    uz = mu * u.z - sqrt_mu * sqrt_w * c;
+  } else {
+    double sqrt_v = std::sqrt(1. - u.y * u.y);
+    ux = mu * u.x + sqrt_mu * (u.x * u.y * c + u.z * s) / sqrt_v;
+    uy = mu * u.y - sqrt_mu * sqrt_v * c;
+    uz = mu * u.z + sqrt_mu * (u.y * u.z * c - u.x * s) / sqrt_v;
+  }
+  return direction(ux, uy, uz);
+}
+
+//==========================================================================
+// pcg32 restated from pcg-cpp (pcg_random.hpp: engine ctor, bump, XSH-RR
+// output of the previous state, Brown advance), usable as a
+// UniformRandomBitGenerator for libstdc++'s distributions.
+struct Pcg32 {
+  using result_type = uint32_t;
+  static constexpr uint64_t MULT = 6364136223846793005ULL;
+  static constexpr uint64_t INC = 1442695040888963407ULL;
+  uint64_t state = 0;
+  uint64_t calls = 0;  // 32-bit outputs drawn
+  static constexpr result_type min() { return 0u; }
+  static constexpr result_type max() { return 0xffffffffu; }
+  void seed(uint64_t s) { state = (s + INC) * MULT + INC; }
+  void advance(uint64_t delta) {
+    uint64_t cm = MULT, cp = INC, am = 1, ap = 0;
+    while (delta > 0) {
+      if (delta & 1) {
+        am *= cm;
+        ap = ap * cm + cp;
+      }
+      cp = (cm + 1) * cp;
+      cm *= cm;
+      delta >>= 1;
+    }
+    state = am * state + ap;
+  }
+  result_type operator()() {
+    uint64_t old = state;
+    state = old * MULT + INC;
+    calls++;
+    uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((-rot) & 31));
+  }
+};
+
+// RNG (include/utils/rng.hpp:41-96), on libstdc++'s own distributions.
+struct RNG {
+  static double rand(Pcg32& g) {
+    std::uniform_real_distribution<double> unit(0., 1.);
+    return unit(g);
+  }
+  static double exponential(Pcg32& g, double lambda) {
+    if (lambda == 0.) return INF;
+    if (!g_portable) {
+      std::exponential_distribution<double> d(lambda);
+      return d(g);
+    }
+    // same draw as exponential_distribution, with the portable log
+    double u = std::generate_canonical<double, 53>(g);
+    return -dm_log(1.0 - u) / lambda;
+  }
+  static int discrete(Pcg32& g, const double* b, const double* e) {
+    std::discrete_distribution<int> d(b, e);
+    return d(g);
+  }
+};
+
+//==========================================================================
+// Surfaces (src/{x,y,z}plane.cpp, src/plane.cpp, src/{x,y,z}cylinder.cpp,
+// src/sphere.cpp)
+struct Surface {
+  int type;
+  int bc;
+  uint32_t id;
+  double p[4];
+
+  V3 norm(const V3& r) const {
+    switch (type) {
+      case ABX_XPLANE:
+        return direction(1., 0., 0.);
+      case ABX_YPLANE:
+        return direction(0., 1., 0.);
+      case ABX_ZPLANE:
+        return direction(0., 0., 1.);
+      case ABX_PLANE:
+        return direction(p[0], p[1], p[2]);
+      case ABX_XCYL:
+        return direction(0., r.y - p[0], r.z - p[1]);
+      case ABX_YCYL:
+        return direction(r.x - p[0], 0., r.z - p[1]);
+      case ABX_ZCYL:
+        return direction(r.x - p[0], r.y - p[1], 0.);
+      default:
+        return direction(r.x - p[0], r.y - p[1], r.z - p[2]);
+    }
+  }
+
+  int sign(const V3& r, const V3& u) const {
+    double eval;
+    switch (type) {
+      case ABX_XPLANE:
+        eval = r.x - p[0];
+        break;
+      case ABX_YPLANE:
+        eval = r.y - p[0];
+        break;
+      case ABX_ZPLANE:
+        eval = r.z - p[0];
+        break;
+      case ABX_PLANE:
+        eval = p[0] * r.x + p[1] * r.y + p[2] * r.z - p[3];
+        break;
+      case ABX_XCYL: {
+        const double y = r.y - p[0], z = r.z - p[1];
+        eval = y * y + z * z - p[2] * p[2];
+        break;
+      }
+      case ABX_YCYL: {
+        const double x = r.x - p[0], z = r.z - p[1];
+        eval = x * x + z * z - p[2] * p[2];
+        break;
+      }
+      case ABX_ZCYL: {
+        const double x = r.x - p[0], y = r.y - p[1];
+        eval = y * y + x * x - p[2] * p[2];
+        break;
+      }
+      default: {
+        const double x = r.x - p[0], y = r.y - p[1], z = r.z - p[2];
+        eval = (x * x) + (y * y) + (z * z) - p[3] * p[3];
+        break;
+      }
+    }
+    if (eval > SURFACE_COINCIDENT)
+      return 1;
+    else if (eval < -SURFACE_COINCIDENT)
+      return -1;
+    else
+      return dot(u, norm(r)) > 0. ? 1 : -1;
+  }
+
+  double plane_axis(double r0, double u0, bool on_surf) const {
+    const double diff = p[0] - r0;
+    if (on_surf || std::abs(diff) < SURFACE_COINCIDENT || u0 == 0.)
+      return INF;
+    else if (diff / u0 < 0.)
+      return INF;
+    else
+      return diff / u0;
+  }
+
+  // a = u1^2 + u2^2 etc. in the reference's operand order per type
+  static double cyl(double a, double k, double c, bool on_surf) {
+    const double quad = k * k - a * c;
+    if (quad < 0.)
+      return INF;
+    else if (on_surf || std::abs(c) < SURFACE_COINCIDENT) {
+      if (k >= 0.)
+        return INF;
+      else
+        return (-k + std::sqrt(quad)) / a;
+    } else if (c < 0.) {
+      return (-k + std::sqrt(quad)) / a;
+    } else {
+      const double d = (-k - std::sqrt(quad)) / a;
+      if (d < 0.)
+        return INF;
+      else
+        return d;
+    }
+  }
+
+  double distance(const V3& r, const V3& u, bool on_surf) const {
+    switch (type) {
+      case ABX_XPLANE:
+        return plane_axis(r.x, u.x, on_surf);
+      case ABX_YPLANE:
+        return plane_axis(r.y, u.y, on_surf);
+      case ABX_ZPLANE:
+        return plane_axis(r.z, u.z, on_surf);
+      case ABX_PLANE: {
+        const double num = p[3] - p[0] * r.x - p[1] * r.y - p[2] * r.z;
+        const double denom = p[0] * u.x + p[1] * u.y + p[2] * u.z;
+        const double d = num / denom;
+        if (on_surf || std::abs(d) < SURFACE_COINCIDENT || denom == 0.)
+          return INF;
+        else if (d < 0.)
+          return INF;
+        else
+          return d;
+      }
+      case ABX_XCYL: {
+        const double a = u.y * u.y + u.z * u.z;
+        if (a == 0.) return INF;
+        const double y = r.y - p[0], z = r.z - p[1];
+        const double k = y * u.y + z * u.z;
+        const double c = y * y + z * z - p[2] * p[2];
+        return cyl(a, k, c, on_surf);
+      }
+      case ABX_YCYL: {
+        const double a = u.x * u.x + u.z * u.z;
+        if (a == 0.) return INF;
+        const double x = r.x - p[0], z = r.z - p[1];
+        const double k = x * u.x + z * u.z;
+        const double c = x * x + z * z - p[2] * p[2];
+        return cyl(a, k, c, on_surf);
+      }
+      case ABX_ZCYL: {
+        const double a = u.y * u.y + u.x * u.x;
+        if (a == 0.) return INF;
+        const double x = r.x - p[0], y = r.y - p[1];
+        const double k = y * u.y + x * u.x;
+        const double c = y * y + x * x - p[2] * p[2];
+        return cyl(a, k, c, on_surf);
+      }
+      default: {  // sphere (src/sphere.cpp:51-76)
+        const double x = r.x - p[0], y = r.y - p[1], z = r.z - p[2];
+        const double k = x * u.x + y * u.y + z * u.z;
+        const double c = x * x + y * y + z * z - p[3] * p[3];
+        const double quad = k * k - c;
+        if (quad < 0.)
+          return INF;
+        else if (on_surf || std::abs(c) < SURFACE_COINCIDENT) {
+          if (k >= 0.)
+            return INF;
+          else
+            return -k + std::sqrt(quad);
+        } else if (c < 0.) {
+          return -k + std::sqrt(quad);
+        } else {
+          const double d = -k - std::sqrt(quad);
+          if (d < 0.)
+            return INF;
+          else
+            return d;
+        }
+      }
+    }
+  }
+};
+
+//==========================================================================
+// Cells (src/cell.cpp)
+struct Problem;
+struct Cell {
+  uint32_t id;
+  int fill_type;
+  int fill;
+  std::vector<int32_t> rpn;
+  bool simple = true;
+  bool vacuum_or_reflective = false;
+};
+
+struct Pad {  // include/geometry/geo_lily_pad.hpp
+  enum Type { Universe = 0, Lattice = 1, CellT = 2 };
+  int type;
+  int idx;  // universe index (Universe/Lattice) or cell index (Cell)
+  V3 r_local;
+  int tile[3];
+  bool in_outer;
+};
+
+struct BoundaryInfo {  // include/geometry/boundary.hpp
+  double distance;
+  int surface_index;
+  int btype;
+  int32_t token;
+};
+
+struct Universe {
+  uint32_t id;
+  int type;
+  std::vector<int> cells;
+  bool has_bc = false;
+  // rect lattice (src/rect_lattice.cpp:33-52)
+  uint32_t Nx = 0, Ny = 0, Nz = 0;
+  double Px = 0, Py = 0, Pz = 0, Px_inv = 0, Py_inv = 0, Pz_inv = 0;
+  double Xl = 0, Yl = 0, Zl = 0;
+  std::vector<int32_t> elems;
+  int outer = -1;
+};
+
+//==========================================================================
+// Multigroup nuclide (src/mg_nuclide.cpp) + angle tables
+struct AngleDist {  // include/materials/mg_angle_distribution.hpp
+  std::vector<double> mu{-1., 1.}, pdf{0.5, 0.5}, cdf{0., 1.};
+  double sample_mu(Pcg32& g) const {
+    const double xi = RNG::rand(g);
+    auto it = std::lower_bound(cdf.begin(), cdf.end(), xi);
+    std::size_t l = (std::size_t)std::distance(cdf.begin(), it);
+    if (xi == *it) return mu[l];
+    l--;
+    if (pdf[l] == pdf[l + 1]) return mu[l] + ((xi - cdf[l]) / pdf[l]);
+    double m = (pdf[l + 1] - pdf[l]) / (mu[l + 1] - mu[l]);
+    return mu[l] +
+           (1. / m) * (std::sqrt(pdf[l] * pdf[l] + 2. * m * (xi - cdf[l])) -
+                       pdf[l]);
+  }
+};
+
+struct Material {
+  uint32_t id;
+  int G;
+  std::vector<double> Et, Ea, Ef, nu_p, nu_d, Es;
+  std::vector<std::vector<double>> chi, Ps;
+  std::vector<std::vector<AngleDist>> angle;
+  std::vector<double> P_delayed_group, delayed_constants;
+  bool fissile = false;
+  bool has_nu_d = false;
+};
+
+struct MicroXS {  // include/materials/nuclide.hpp:40-53
+  double total, fission, absorption, elastic, nu_total, nu_delayed;
+  int energy_index;
+};
+
+struct MeshTally {
+  int estimator, quantity;
+  int Nx, Ny, Nz;
+  V3 r_low, r_hi;
+  double dx, dy, dz, dx_inv, dy_inv, dz_inv;
+  double net_weight = 1.;
+  std::vector<double> ebounds;
+  std::vector<double> gen, avg, var;
+  int g = 0;
+};
+
+struct ParticleState {
+  V3 r, u;
+  double E, w, w2;
+};
+
+struct Particle {  // include/simulation/particle.hpp:68-242
+  ParticleState s;
+  Pcg32 rng;
+  uint64_t history_id = 0, family_id = 0, secondary_id = 0,
+           daughter_counter = 0;
+  V3 prev_r{0, 0, 0}, prev_u{1, 0, 0};
+  double prev_E = 0, Esmp = 0;
+  bool alive = true, reflected = false, prev_virtual = false;
+  V3 r_birth;
+  std::vector<ParticleState> secondaries;
+  std::vector<abx_banked> fission;
+
+  void move(double d) {
+    if (!reflected) {
+      prev_r = s.r;
+      s.r = add(s.r, scale(s.u, d));
+    } else {
+      s.r = add(s.r, scale(s.u, d));
+      reflected = false;
+    }
+  }
+  void set_position(V3 r) {
+    prev_r = s.r;
+    s.r = r;
+  }
+  void set_direction(V3 u) {
+    prev_u = s.u;
+    s.u = u;
+  }
+  void set_energy(double E) {
+    prev_E = s.E;
+    s.E = E;
+  }
+  void make_secondary(V3 u, double E, double w, double w2) {
+    secondaries.push_back({s.r, u, E, w, w2});
+  }
+  void resurect() {
+    if (!alive && !secondaries.empty()) {
+      alive = true;
+      s = secondaries.back();
+      secondaries.pop_back();
+      secondary_id++;
+    }
+  }
+  void split(int n_new) {
+    if (n_new > 1) {
+      s.w = s.w / (double)n_new;
+      s.w2 = s.w2 / (double)n_new;
+      for (int np = 0; np < n_new - 1; np++) make_secondary(s.u, s.E, s.w, s.w2);
+    }
+  }
+};
+
+struct Scores {  // ThreadLocalScores (transporter.hpp:52-59)
+  double k_col = 0, k_abs = 0, k_trk = 0, k_tot = 0, leak = 0, mig = 0;
+};
+
+struct Source {
+  abx_source d;
+};
+
+//==========================================================================
+struct Problem {
+  std::vector<Surface> surfaces;
+  std::vector<Cell> cells;
+  std::vector<Universe> universes;
+  int root = 0;
+  std::vector<Material> materials;
+  std::vector<Source> sources;
+  std::vector<MeshTally> tallies;
+  abx_settings st;
+  std::vector<double> ebounds;  // settings::energy_bounds
+  std::vector<double> ratio;    // sample_xs_ratio
+  std::vector<double> majorant, sampling;
+  double min_energy = 0., max_energy = 100000.;
+  bool portable = true;
+
+  // Tallies scalars (src/tallies.cpp)
+  Scores gen_scores;
+  double kcol_prev = 1.0;
+  int gen = 0;
+  double k_col_avg = 0, k_col_var = 0, k_trk_avg = 0, k_trk_var = 0,
+         leak_avg = 0, leak_var = 0, mig_avg = 0, mig_var = 0;
+  std::vector<abx_particle> last_bank;
+  abx_diag diag_total{};
+  int universe_by_id(uint32_t id) const {
+    for (size_t i = 0; i < universes.size(); i++)
+      if (universes[i].id == id) return (int)i;
+    return 0;  // std::map::operator[] default-inserts index 0
+  }
+};
+
+//------------------------------------------------------------------------
+// Cell::is_inside_simple / is_inside_complex (src/cell.cpp:144-193)
+static bool cell_is_inside(const Problem& P, const Cell& c, const V3& r,
+                           const V3& u, int32_t on_surf) {
+  if (c.simple) {
+    for (int32_t token : c.rpn) {
+      if (token == on_surf) {
+      } else if (-token == on_surf)
+        return false;
+      else {
+        int sign = P.surfaces[(size_t)(std::abs(token) - 1)].sign(r, u);
+        if ((sign > 0 && token < 0) || (sign < 0 && token > 0)) return false;
+      }
+    }
+    return true;
+  }
+  std::vector<bool> stck(c.rpn.size());
+  int i = -1;
+  for (int32_t token : c.rpn) {
+    if (token == ABX_OP_UNIN) {
+      stck[(size_t)(i - 1)] = stck[(size_t)(i - 1)] || stck[(size_t)i];
+      i--;
+    } else if (token == ABX_OP_INTR) {
+      stck[(size_t)(i - 1)] = stck[(size_t)(i - 1)] && stck[(size_t)i];
+      i--;
+    } else if (token == ABX_OP_COMP) {
+      stck[(size_t)i] = !stck[(size_t)i];
+    } else {
+      i++;
+      if (token == on_surf)
+        stck[(size_t)i] = true;
+      else if (-token == on_surf)
+        stck[(size_t)i] = false;
+      else {
+        int sign = P.surfaces[(size_t)(std::abs(token) - 1)].sign(r, u);
+        stck[(size_t)i] = (sign > 0 && token > 0) || (sign < 0 && token < 0);
+      }
+    }
+  }
+  if (i == 0) return stck[0];
+  return true;
+}
+
+// Cell::distance_to_boundary (src/cell.cpp:79-106)
+static std::pair<double, int32_t> cell_distance(const Problem& P,
+                                                const Cell& c, const V3& r,
+                                                const V3& u, int32_t on_surf,
+                                                bool bc_only) {
+  if (bc_only && !c.vacuum_or_reflective) return {INF, 0};
+  double min_dist = INF;
+  int32_t i_surf = 0;
+  for (int32_t token : c.rpn) {
+    if (token >= ABX_OP_UNIN) continue;
+    bool coincident = std::abs(token) == std::abs(on_surf);
+    const Surface& s = P.surfaces[(size_t)(std::abs(token) - 1)];
+    if (bc_only && s.bc == ABX_NORMAL) continue;
+    double d = s.distance(r, u, coincident);
+    if (d < min_dist) {
+      if (std::abs(d - min_dist) / min_dist >= 1e-14) {
+        min_dist = d;
+        i_surf = -token;
+      }
+    }
+  }
+  return {min_dist, i_surf};
+}
+
+// RectLattice::get_tile (src/rect_lattice.cpp:209-237)
+static void lattice_tile_center(const Universe& L, int nx, int ny, int nz,
+                                V3& c) {
+  c.x = ((double)nx + 0.5) * L.Px + L.Xl;
+  c.y = ((double)ny + 0.5) * L.Py + L.Yl;
+  c.z = ((double)nz + 0.5) * L.Pz + L.Zl;
+}
+
+static void lattice_get_tile(const Universe& L, const V3& r, const V3& u,
+                             int t[3]) {
+  int nx = (int)std::floor((r.x - L.Xl) * L.Px_inv);
+  int ny = (int)std::floor((r.y - L.Yl) * L.Py_inv);
+  int nz = (int)std::floor((r.z - L.Zl) * L.Pz_inv);
+  V3 c;
+  lattice_tile_center(L, nx, ny, nz, c);
+  double xl = c.x - L.Px * 0.5;
+  if (std::abs(xl - r.x) < SURFACE_COINCIDENT && u.x < 0.) nx--;
+  double xh = c.x + L.Px * 0.5;
+  if (std::abs(xh - r.x) < SURFACE_COINCIDENT && u.x >= 0.) nx++;
+  double yl = c.y - L.Py * 0.5;
+  if (std::abs(yl - r.y) < SURFACE_COINCIDENT && u.y < 0.) ny--;
+  double yh = c.y + L.Py * 0.5;
+  if (std::abs(yh - r.y) < SURFACE_COINCIDENT && u.y >= 0.) ny++;
+  double zl = c.z - L.Pz * 0.5;
+  if (std::abs(zl - r.z) < SURFACE_COINCIDENT && u.z < 0.) nz--;
+  double zh = c.z + L.Pz * 0.5;
+  if (std::abs(zh - r.z) < SURFACE_COINCIDENT && u.z >= 0.) nz++;
+  t[0] = nx;
+  t[1] = ny;
+  t[2] = nz;
+}
+
+// RectLattice::distance_to_tile_boundary (src/rect_lattice.cpp:239-282)
+static double lattice_tile_distance(const Universe& L, const V3& r_local,
+                                    const V3& u, const int tile[3]) {
+  V3 c;
+  lattice_tile_center(L, tile[0], tile[1], tile[2], c);
+  V3 rt = sub(r_local, c);
+  double dist = INF;
+  const double diff_xl = -L.Px * 0.5 - rt.x;
+  const double diff_xh = L.Px * 0.5 - rt.x;
+  const double diff_yl = -L.Py * 0.5 - rt.y;
+  const double diff_yh = L.Py * 0.5 - rt.y;
+  const double diff_zl = -L.Pz * 0.5 - rt.z;
+  const double diff_zh = L.Pz * 0.5 - rt.z;
+  const double ux_inv = 1. / u.x;
+  const double uy_inv = 1. / u.y;
+  const double uz_inv = 1. / u.z;
+  const double d_xl = diff_xl * ux_inv;
+  const double d_xh = diff_xh * ux_inv;
+  const double d_yl = diff_yl * uy_inv;
+  const double d_yh = diff_yh * uy_inv;
+  const double d_zl = diff_zl * uz_inv;
+  const double d_zh = diff_zh * uz_inv;
+  const double tol = 100 * SURFACE_COINCIDENT;
+  if (d_xl > 0. && d_xl < dist && std::abs(diff_xl) > tol) dist = d_xl;
+  if (d_xh > 0. && d_xh < dist && std::abs(diff_xh) > tol) dist = d_xh;
+  if (d_yl > 0. && d_yl < dist && std::abs(diff_yl) > tol) dist = d_yl;
+  if (d_yh > 0. && d_yh < dist && std::abs(diff_yh) > tol) dist = d_yh;
+  if (d_zl > 0. && d_zl < dist && std::abs(diff_zl) > tol) dist = d_zl;
+  if (d_zh > 0. && d_zh < dist && std::abs(diff_zh) > tol) dist = d_zh;
+  return dist;
+}
+
+static bool tile_in_range(const Universe& L, const int t[3]) {
+  return !(t[0] < 0 || t[0] >= (int)L.Nx || t[1] < 0 || t[1] >= (int)L.Ny ||
+           t[2] < 0 || t[2] >= (int)L.Nz);
+}
+
+static size_t lattice_linear_index(const Universe& L, int nx, int ny, int nz) {
+  return (size_t)((uint32_t)nz * (L.Nx * L.Ny) + (uint32_t)nx * L.Ny +
+                  (uint32_t)ny);
+}
+
+// Universe::get_cell with the lily-pad stack: CellUniverse
+// (src/cell_universe.cpp:72-109) and RectLattice (src/rect_lattice.cpp:128-207).
+// Returns the material cell index or -1.
+static int universe_get_cell(const Problem& P, int ui, std::vector<Pad>& stack,
+                             V3 r, V3 u, int32_t on_surf) {
+  const Universe& U = P.universes[(size_t)ui];
+  if (U.type == ABX_CELL_UNIVERSE) {
+    stack.push_back({Pad::Universe, ui, r, {0, 0, 0}, false});
+    for (int ci : U.cells) {
+      const Cell& c = P.cells[(size_t)ci];
+      if (cell_is_inside(P, c, r, u, on_surf)) {
+        stack.push_back({Pad::CellT, ci, r, {0, 0, 0}, false});
+        if (c.fill_type == ABX_FILL_MATERIAL) return ci;
+        return universe_get_cell(P, c.fill, stack, r, u, on_surf);
+      }
+    }
+    return -1;
+  }
+  int t[3];
+  lattice_get_tile(U, r, u, t);
+  if (!tile_in_range(U, t)) {
+    if (U.outer >= 0) {
+      stack.push_back({Pad::Lattice, ui, r, {t[0], t[1], t[2]}, true});
+      return universe_get_cell(P, U.outer, stack, r, u, on_surf);
+    }
+    stack.push_back({Pad::Lattice, ui, r, {t[0], t[1], t[2]}, false});
+    return -1;
+  }
+  size_t li = lattice_linear_index(U, t[0], t[1], t[2]);
+  if (U.elems[li] >= 0) {
+    V3 c;
+    lattice_tile_center(U, t[0], t[1], t[2], c);
+    V3 r_local = sub(r, c);
+    stack.push_back({Pad::Lattice, ui, r, {t[0], t[1], t[2]}, false});
+    return universe_get_cell(P, U.elems[li], stack, r_local, u, on_surf);
+  }
+  if (U.outer >= 0) {
+    stack.push_back({Pad::Lattice, ui, r, {t[0], t[1], t[2]}, true});
+    return universe_get_cell(P, U.outer, stack, r, u, on_surf);
+  }
+  stack.push_back({Pad::Lattice, ui, r, {t[0], t[1], t[2]}, false});
+  return -1;
+}
+
+// Universe::get_boundary_condition: CellUniverse (src/cell_universe.cpp:111-150)
+// and Lattice (src/lattice.cpp:77-91).
+static BoundaryInfo universe_boundary_condition(const Problem& P, int ui,
+                                                const V3& r, const V3& u,
+                                                int32_t on_surf) {
+  const Universe& U = P.universes[(size_t)ui];
+  if (U.type == ABX_RECT_LATTICE) {
+    if (U.has_bc) return universe_boundary_condition(P, U.outer, r, u, on_surf);
+    return {INF, -1, ABX_VACUUM, 0};
+  }
+  double dist = INF;
+  int btype = ABX_VACUUM;
+  int surface_index = -1;
+  int32_t token = 0;
+  if (U.has_bc) {
+    for (int ci : U.cells) {
+      const Cell& c = P.cells[(size_t)ci];
+      if (!c.vacuum_or_reflective) continue;
+      auto dt = cell_distance(P, c, r, u, on_surf, true);
+      if (dt.first < dist && std::abs(dt.first - dist) > BOUNDRY_TOL) {
+        double tmp_dist = dt.first;
+        int32_t tmp_token = std::abs(dt.second);
+        if (tmp_token) {
+          token = tmp_token;
+          dist = tmp_dist;
+          surface_index = token - 1;
+        } else {
+          continue;
+        }
+        btype = P.surfaces[(size_t)surface_index].bc;
+        if (P.surfaces[(size_t)surface_index].sign(r, u) < 0) token *= -1;
+      }
+    }
+  }
+  return {dist, surface_index, btype, token};
+}
+
+//==========================================================================
+// Tracker (include/simulation/tracker.hpp:41-372)
+struct Tracker {
+  const Problem* P;
+  V3 r_, u_;
+  std::vector<Pad> tree;
+  int cell = -1;
+  int32_t token = 0;
+
+  Tracker(const Problem* p, V3 r, V3 u, int32_t tok = 0)
+      : P(p), r_(r), u_(u), token(tok) {
+    tree.reserve(10);
+    cell = universe_get_cell(*P, P->root, tree, r_, u_, token);
+  }
+  bool is_lost() const { return cell < 0; }
+  int material() const {
+    return cell < 0 ? -1 : P->cells[(size_t)cell].fill;
+  }
+  void set_r(V3 r) {
+    r_ = r;
+    token = 0;
+  }
+  void set_u(V3 u) { u_ = u; }
+  void restart_get_current() {
+    tree.clear();
+    cell = universe_get_cell(*P, P->root, tree, r_, u_, token);
+  }
+  void move(double d) {
+    r_ = add(r_, scale(u_, d));
+    for (auto& leaf : tree) leaf.r_local = add(leaf.r_local, scale(u_, d));
+    token = 0;
+  }
+  bool check_tree() const {
+    return r_.x == tree.front().r_local.x && r_.y == tree.front().r_local.y &&
+           r_.z == tree.front().r_local.z;
+  }
+  // Tracker::get_current (tracker.hpp:235-306)
+  void get_current() {
+    size_t first_bad = tree.size();
+    if (!check_tree()) {
+      restart_get_current();
+      if (!check_tree()) fatal(ABX_E_LOST, "BAD POSITIONS");
+    }
+    for (size_t i = 0; i < tree.size(); i++) {
+      const Pad& pad = tree[i];
+      if (pad.type == Pad::CellT) {
+        if (!cell_is_inside(*P, P->cells[(size_t)pad.idx], pad.r_local, u_,
+                            token)) {
+          first_bad = i;
+          break;
+        }
+      } else if (pad.type == Pad::Lattice) {
+        int t[3];
+        lattice_get_tile(P->universes[(size_t)pad.idx], pad.r_local, u_, t);
+        if (pad.tile[0] != t[0] || pad.tile[1] != t[1] || pad.tile[2] != t[2]) {
+          first_bad = i;
+          break;
+        }
+      }
+    }
+    if (first_bad != tree.size()) {
+      if (first_bad == 0) return restart_get_current();
+      // The reference keeps an end() iterator of the tree from before a
+      // restart; when the restarted tree is shallower it resizes into
+      // default-constructed pads.  That only happens after losing a particle
+      // deeper than where it is found again (a geometry hole); report it.
+      if (first_bad > tree.size())
+        fatal(ABX_E_LOST, "relocation from a deeper lost tree");
+      tree.resize(first_bad);
+      // universe_id_to_indx[tree.back().id]: a Cell pad's id is looked up
+      // among universe ids (tracker.hpp:285-286).
+      const Pad& back = tree.back();
+      int ui = back.type == Pad::CellT
+                   ? P->universe_by_id(P->cells[(size_t)back.idx].id)
+                   : back.idx;
+      V3 r_local = back.r_local;
+      tree.pop_back();
+      cell = universe_get_cell(*P, ui, tree, r_local, u_, token);
+      if (cell < 0) restart_get_current();
+      if (cell >= 0 && P->cells[(size_t)cell].fill_type != ABX_FILL_MATERIAL)
+        fatal(ABX_E_LOST, "Did not find a cell with a material.");
+    }
+  }
+  // Tracker::get_boundary_condition (tracker.hpp:94-161)
+  BoundaryInfo get_boundary_condition() const {
+    if (is_lost())
+      return universe_boundary_condition(*P, P->root, r_, u_, token);
+    double dist = INF;
+    int btype = ABX_VACUUM;
+    int surface_index = -1;
+    int32_t tok = 0;
+    for (const Pad& pad : tree) {
+      if (pad.type == Pad::CellT) {
+        const Cell& c = P->cells[(size_t)pad.idx];
+        if (!c.vacuum_or_reflective) continue;
+        auto dt = cell_distance(*P, c, pad.r_local, u_, token, true);
+        if (dt.first < dist && std::abs(dt.first - dist) > BOUNDRY_TOL) {
+          double tmp_dist = dt.first;
+          int32_t tmp_token = std::abs(dt.second);
+          if (tmp_token) {
+            dist = tmp_dist;
+            tok = tmp_token;
+            surface_index = tok - 1;
+          } else {
+            continue;
+          }
+          btype = P->surfaces[(size_t)surface_index].bc;
+          if (P->surfaces[(size_t)surface_index].sign(pad.r_local, u_) < 0)
+            tok *= -1;
+        }
+      } else {
+        const Universe& U = P->universes[(size_t)pad.idx];
+        if (U.has_bc) {
+          BoundaryInfo ub =
+              universe_boundary_condition(*P, pad.idx, pad.r_local, u_, token);
+          if (ub.distance < dist && std::abs(ub.distance - dist) > BOUNDRY_TOL) {
+            dist = ub.distance;
+            tok = ub.token;
+            surface_index = ub.surface_index;
+            btype = ub.btype;
+          }
+        }
+      }
+    }
+    return {dist, surface_index, btype, tok};
+  }
+  // Tracker::get_nearest_boundary (tracker.hpp:163-225)
+  BoundaryInfo get_nearest_boundary() const {
+    if (is_lost()) fatal(ABX_E_LOST, "nearest boundary of a lost particle");
+    BoundaryInfo b = get_boundary_condition();
+    double dist = b.distance;
+    int btype = b.btype;
+    int surface_index = b.surface_index;
+    int32_t tok = b.token;
+    for (const Pad& pad : tree) {
+      if (pad.type == Pad::Lattice) {
+        const Universe& L = P->universes[(size_t)pad.idx];
+        double d = lattice_tile_distance(L, pad.r_local, u_, pad.tile);
+        if (d < dist && std::abs(d - dist) > BOUNDRY_TOL) {
+          dist = d;
+          btype = ABX_NORMAL;
+          surface_index = -1;
+          tok = 0;
+        }
+      } else if (pad.type == Pad::CellT) {
+        auto dt = cell_distance(*P, P->cells[(size_t)pad.idx], pad.r_local, u_,
+                                token, false);
+        if (dt.first < dist && std::abs(dt.first - dist) > BOUNDRY_TOL) {
+          dist = dt.first;
+          tok = std::abs(dt.second);
+          surface_index = tok ? tok - 1 : -1;
+          if (surface_index >= 0)
+            btype = P->surfaces[(size_t)surface_index].bc;
+          else
+            btype = ABX_NORMAL;
+          if (surface_index >= 0 &&
+              P->surfaces[(size_t)surface_index].sign(pad.r_local, u_) < 0)
+            tok *= -1;
+        }
+      }
+    }
+    return {dist, surface_index, btype, tok};
+  }
+  void cross_surface(const BoundaryInfo& b) {
+    move(b.distance);
+    token = -b.token;
+  }
+  // Tracker::do_reflection (tracker.hpp:314-359)
+  void do_reflection(Particle& p, const BoundaryInfo& b) {
+    if (b.surface_index < 0) fatal(ABX_E_LOST, "Bad surface index in reflection");
+    const Surface& s = P->surfaces[(size_t)b.surface_index];
+    V3 r_pre_refs = p.s.r;
+    if (p.reflected) r_pre_refs = p.prev_r;
+    V3 u = p.s.u;
+    V3 r_on_surf = add(p.s.r, scale(u, b.distance));
+    V3 n = s.norm(r_on_surf);
+    V3 nd = sub(u, scale(n, 2. * dot(u, n)));
+    V3 u_new = direction(nd.x, nd.y, nd.z);
+    double d = b.distance + norm(sub(p.s.r, r_pre_refs));
+    V3 r_prev = sub(r_on_surf, scale(u_new, d));
+    p.set_position(r_on_surf);
+    p.prev_r = r_prev;
+    p.set_direction(u_new);
+    p.reflected = true;
+    set_r(p.s.r);
+    set_u(p.s.u);
+    restart_get_current();
+  }
+};
+
+//==========================================================================
+// Material helpers (include/materials/material_helper.hpp, MG nuclide xs
+// getters src/mg_nuclide.cpp:348-411)
+static int energy_grid_index(const Problem& P, double E) {
+  size_t i = 0;
+  for (i = 0; i < P.ebounds.size() - 1; i++)
+    if (P.ebounds[i] <= E && E < P.ebounds[i + 1]) break;
+  return (int)i;
+}
+
+static MicroXS micro_xs(const Problem& P, const Material& m, double E) {
+  MicroXS x;
+  x.energy_index = energy_grid_index(P, E);
+  if (x.energy_index >= m.G) fatal(ABX_E_INVALID, "energy outside group bounds");
+  const size_t i = (size_t)x.energy_index;
+  x.total = m.Et[i];
+  x.fission = m.Ef[i];
+  x.absorption = x.fission + (m.Ea[i] - m.Ef[i]);
+  x.elastic = m.Es[i];
+  x.nu_total = m.nu_p[i];
+  if (m.has_nu_d) x.nu_total += m.nu_d[i];
+  x.nu_delayed = m.has_nu_d ? m.nu_d[i] : 0.;
+  return x;
+}
+
+// macroscopic sums over the single component at 1 atom/b-cm
+static double mat_Et(const Problem& P, int mi, double E) {
+  return 0. + 1.0 * micro_xs(P, P.materials[(size_t)mi], E).total;
+}
+static double mat_Ea(const Problem& P, int mi, double E) {
+  return 0. + 1.0 * micro_xs(P, P.materials[(size_t)mi], E).absorption;
+}
+static double mat_Ef(const Problem& P, int mi, double E) {
+  return 0. + 1.0 * micro_xs(P, P.materials[(size_t)mi], E).fission;
+}
+static double mat_vEf(const Problem& P, int mi, double E) {
+  MicroXS x = micro_xs(P, P.materials[(size_t)mi], E);
+  return 0. + 1.0 * x.nu_total * x.fission;
+}
+static double mat_Eel(const Problem& P, int mi, double E) {
+  return 0. + 1.0 * micro_xs(P, P.materials[(size_t)mi], E).elastic;
+}
+
+//==========================================================================
+// Tallies (src/collision_mesh_tally.cpp:33-110, src/track_length_mesh_tally.cpp)
+static double quantity_factor(const Problem& P, const MeshTally& t,
+                              const Particle& p, int mi) {
+  switch (t.quantity) {
+    case ABX_Q_ELASTIC:
+      return p.s.w * mat_Eel(P, mi, p.s.E);
+    case ABX_Q_ABSORPTION:
+      return p.s.w * mat_Ea(P, mi, p.s.E);
+    case ABX_Q_FISSION:
+      return p.s.w * mat_Ef(P, mi, p.s.E);
+    case ABX_Q_TOTAL:
+      return p.s.w * mat_Et(P, mi, p.s.E);
+    case ABX_Q_IMG_FLUX:
+      return p.s.w2;
+    default:
+      return p.s.w;
+  }
+}
+
+static int tally_ebin(const MeshTally& t, double E) {
+  for (size_t e = 0; e < t.ebounds.size() - 1; e++)
+    if (t.ebounds[e] <= E && E <= t.ebounds[e + 1]) return (int)e;
+  return -1;
+}
+
+static inline void atomic_add(double& a, double v) {
+#pragma omp atomic
+  a += v;
+}
+
+static uint64_t score_collision(Problem& P, MeshTally& t, const Particle& p,
+                                int mi) {
+  double Et = mat_Et(P, mi, p.s.E);
+  double scr = 1. / (Et * t.net_weight);
+  int i = (int)std::floor((p.s.r.x - t.r_low.x) * t.dx_inv);
+  int j = (int)std::floor((p.s.r.y - t.r_low.y) * t.dy_inv);
+  int k = (int)std::floor((p.s.r.z - t.r_low.z) * t.dz_inv);
+  int l = tally_ebin(t, p.s.E);
+  if (l == -1) return 0;
+  if (i >= 0 && i < t.Nx && j >= 0 && j < t.Ny && k >= 0 && k < t.Nz) {
+    if (t.quantity == ABX_Q_TOTAL)
+      scr *= p.s.w * Et;
+    else
+      scr *= quantity_factor(P, t, p, mi);
+    size_t idx = (((size_t)l * t.Nx + (size_t)i) * t.Ny + (size_t)j) * t.Nz +
+                 (size_t)k;
+    atomic_add(t.gen[idx], scr);
+    return 1;
+  }
+  return 0;
+}
+
+// TrackLengthMeshTally helpers (src/track_length_mesh_tally.cpp:182-393)
+static bool tle_find_entry_point(const MeshTally& t, V3& r, const V3& u,
+                                 double& d_flight) {
+  const double ux_inv = 1. / u.x;
+  const double uy_inv = 1. / u.y;
+  const double uz_inv = 1. / u.z;
+  double d_min = (t.r_low.x - r.x) * ux_inv;
+  double d_max = (t.r_hi.x - r.x) * ux_inv;
+  if (d_min > d_max) std::swap(d_min, d_max);
+  double d_y_min = (t.r_low.y - r.y) * uy_inv;
+  double d_y_max = (t.r_hi.y - r.y) * uy_inv;
+  if (d_y_min > d_y_max) std::swap(d_y_min, d_y_max);
+  if ((d_min > d_y_max) || (d_y_min > d_max)) return false;
+  if (d_y_min > d_min) d_min = d_y_min;
+  if (d_y_max < d_max) d_max = d_y_max;
+  double d_z_min = (t.r_low.z - r.z) * uz_inv;
+  double d_z_max = (t.r_hi.z - r.z) * uz_inv;
+  if (d_z_min > d_z_max) std::swap(d_z_min, d_z_max);
+  if ((d_min > d_z_max) || (d_z_min > d_max)) return false;
+  if (d_z_min > d_min) d_min = d_z_min;
+  if (d_z_max < d_max) d_max = d_z_max;
+  if (d_max < d_min) std::swap(d_max, d_min);
+  if ((d_max < 0.) && (d_min < 0.)) return false;
+  if (d_min < 0.) return false;
+  r = add(r, scale(u, d_min));
+  d_flight -= d_min;
+  return true;
+}
+
+static void tle_init_indices(const MeshTally& t, const V3& r, const V3& u,
+                             int& i, int& j, int& k, int on[3]) {
+  i = (int)std::floor((r.x - t.r_low.x) * t.dx_inv);
+  j = (int)std::floor((r.y - t.r_low.y) * t.dy_inv);
+  k = (int)std::floor((r.z - t.r_low.z) * t.dz_inv);
+  on[0] = on[1] = on[2] = 0;
+  double xc = t.r_low.x + i * t.dx + 0.5 * t.dx;
+  double yc = t.r_low.y + j * t.dy + 0.5 * t.dy;
+  double zc = t.r_low.z + k * t.dz + 0.5 * t.dz;
+  double xl = xc - 0.5 * t.dx, xh = xc + 0.5 * t.dx;
+  double yl = yc - 0.5 * t.dy, yh = yc + 0.5 * t.dy;
+  double zl = zc - 0.5 * t.dz, zh = zc + 0.5 * t.dz;
+  if (std::abs(xl - r.x) < SURFACE_COINCIDENT) {
+    if (u.x < 0.) {
+      i--;
+      on[0] = 1;
+    } else
+      on[0] = -1;
+  } else if (std::abs(xh - r.x) < SURFACE_COINCIDENT) {
+    if (u.x < 0.)
+      on[0] = 1;
+    else {
+      i++;
+      on[0] = -1;
+    }
+  }
+  if (std::abs(yl - r.y) < SURFACE_COINCIDENT) {
+    if (u.y < 0.) {
+      j--;
+      on[1] = 1;
+    } else
+      on[1] = -1;
+  } else if (std::abs(yh - r.y) < SURFACE_COINCIDENT) {
+    if (u.y < 0.)
+      on[1] = 1;
+    else {
+      j++;
+      on[1] = -1;
+    }
+  }
+  if (std::abs(zl - r.z) < SURFACE_COINCIDENT) {
+    if (u.z < 0.) {
+      k--;
+      on[2] = 1;
+    } else
+      on[2] = -1;
+  } else if (std::abs(zh - r.z) < SURFACE_COINCIDENT) {
+    if (u.z < 0.)
+      on[2] = 1;
+    else {
+      k++;
+      on[2] = -1;
+    }
+  }
+}
+
+static std::pair<double, int> tle_next(const MeshTally& t, const V3& r,
+                                       const V3& u, int i, int j, int k,
+                                       const int on[3]) {
+  double xc = t.r_low.x + i * t.dx + 0.5 * t.dx;
+  double yc = t.r_low.y + j * t.dy + 0.5 * t.dy;
+  double zc = t.r_low.z + k * t.dz + 0.5 * t.dz;
+  V3 rt{r.x - xc, r.y - yc, r.z - zc};
+  double dist = INF;
+  int key = 0;
+  const double diff_xl = -t.dx * 0.5 - rt.x;
+  const double diff_xh = t.dx * 0.5 - rt.x;
+  const double diff_yl = -t.dy * 0.5 - rt.y;
+  const double diff_yh = t.dy * 0.5 - rt.y;
+  const double diff_zl = -t.dz * 0.5 - rt.z;
+  const double diff_zh = t.dz * 0.5 - rt.z;
+  const double ux_inv = 1. / u.x;
+  const double uy_inv = 1. / u.y;
+  const double uz_inv = 1. / u.z;
+  const double d_xl = diff_xl * ux_inv;
+  const double d_xh = diff_xh * ux_inv;
+  const double d_yl = diff_yl * uy_inv;
+  const double d_yh = diff_yh * uy_inv;
+  const double d_zl = diff_zl * uz_inv;
+  const double d_zh = diff_zh * uz_inv;
+  if (d_xl > 0. && d_xl < dist && on[0] != -1) {
+    dist = d_xl;
+    key = -1;
+  }
+  if (d_xh > 0. && d_xh < dist && on[0] != 1) {
+    dist = d_xh;
+    key = 1;
+  }
+  if (d_yl > 0. && d_yl < dist && on[1] != -1) {
+    dist = d_yl;
+    key = -2;
+  }
+  if (d_yh > 0. && d_yh < dist && on[1] != 1) {
+    dist = d_yh;
+    key = 2;
+  }
+  if (d_zl > 0. && d_zl < dist && on[2] != -1) {
+    dist = d_zl;
+    key = -3;
+  }
+  if (d_zh > 0. && d_zh < dist && on[2] != 1) {
+    dist = d_zh;
+    key = 3;
+  }
+  return {dist, key};
+}
+
+static uint64_t score_flight(Problem& P, MeshTally& t, const Particle& p,
+                             double d, int mi) {
+  V3 r = p.s.r;
+  V3 u = p.s.u;
+  int i = 0, j = 0, k = 0;
+  int on[3] = {0, 0, 0};
+  uint64_t n = 0;
+  tle_init_indices(t, r, u, i, j, k, on);
+  bool inside = i >= 0 && i < t.Nx && j >= 0 && j < t.Ny && k >= 0 && k < t.Nz;
+  if (!inside) {
+    if (!tle_find_entry_point(t, r, u, d)) return 0;
+    tle_init_indices(t, r, u, i, j, k, on);
+  }
+  double base = 1. / t.net_weight;
+  base *= quantity_factor(P, t, p, mi);
+  int l = tally_ebin(t, p.s.E);
+  if (l == -1) return 0;
+  double remaining = d;
+  while (remaining > 0.) {
+    auto nt = tle_next(t, r, u, i, j, k, on);
+    if (nt.first == INF) break;
+    double d_tile = std::min(nt.first, remaining);
+    if (i >= 0 && i < t.Nx && j >= 0 && j < t.Ny && k >= 0 && k < t.Nz) {
+      size_t idx = (((size_t)l * t.Nx + (size_t)i) * t.Ny + (size_t)j) * t.Nz +
+                   (size_t)k;
+      atomic_add(t.gen[idx], d_tile * base);
+      n++;
+    } else {
+      return n;
+    }
+    remaining -= d_tile;
+    if (remaining <= 0.) break;
+    r = add(r, scale(u, d_tile));
+    on[0] = on[1] = on[2] = 0;
+    switch (nt.second) {
+      case -1:
+        i--;
+        on[0] = 1;
+        break;
+      case 1:
+        i++;
+        on[0] = -1;
+        break;
+      case -2:
+        j--;
+        on[1] = 1;
+        break;
+      case 2:
+        j++;
+        on[1] = -1;
+        break;
+      case -3:
+        k--;
+        on[2] = 1;
+        break;
+      case 3:
+        k++;
+        on[2] = -1;
+        break;
+      default:
+        break;
+    }
+  }
+  return n;
+}
+
+//==========================================================================
+// Per-history context
+struct Ctx {
+  Problem* P;
+  double kcol;
+  bool converged;
+  Scores sc;
+  abx_diag diag{};
+  std::vector<abx_trace_rec>* trace = nullptr;
+  int trace_cap = 0;
+  int32_t n_rec = 0;
+  uint64_t rng0_calls = 0;
+
+  void rec(const Particle& p, const Tracker& t, int ev) {
+    if (!trace) return;
+    if (n_rec < trace_cap) {
+      abx_trace_rec r{};
+      r.r[0] = p.s.r.x;
+      r.r[1] = p.s.r.y;
+      r.r[2] = p.s.r.z;
+      r.u[0] = p.s.u.x;
+      r.u[1] = p.s.u.y;
+      r.u[2] = p.s.u.z;
+      r.wgt = p.s.w;
+      r.history_id = p.history_id;
+      r.secondary_id = (int32_t)p.secondary_id;
+      r.event = ev;
+      r.cell = t.cell;
+      r.material = t.material();
+      r.group = energy_grid_index(*P, p.s.E);
+      r.rng_calls = (int32_t)(p.rng.calls - rng0_calls);
+      trace->push_back(r);
+    }
+    n_rec++;
+  }
+};
+
+static void tally_collision_all(Ctx& c, const Particle& p, int mi) {
+  if (!c.converged) return;
+  for (auto& t : c.P->tallies)
+    if (t.estimator == ABX_EST_COLLISION)
+      c.diag.tally_updates += score_collision(*c.P, t, p, mi);
+}
+
+static void tally_flight_all(Ctx& c, const Particle& p, double d, int mi) {
+  if (!c.converged) return;
+  for (auto& t : c.P->tallies)
+    if (t.estimator == ABX_EST_TRACK_LENGTH)
+      c.diag.tally_updates += score_flight(*c.P, t, p, d, mi);
+}
+
+//==========================================================================
+// Transporter physics (src/transporter.cpp)
+static void russian_roulette(Ctx& c, Particle& p) {  // :35-57
+  const abx_settings& st = c.P->st;
+  if (std::abs(p.s.w) < st.wgt_cutoff) {
+    double P_kill = 1.0 - (std::abs(p.s.w) / st.wgt_survival);
+    if (RNG::rand(p.rng) < P_kill)
+      p.s.w = 0.;
+    else
+      p.s.w = std::copysign(st.wgt_survival, p.s.w);
+  }
+  if (std::abs(p.s.w2) < st.wgt_cutoff) {
+    double P_kill = 1.0 - (std::abs(p.s.w2) / st.wgt_survival);
+    if (RNG::rand(p.rng) < P_kill)
+      p.s.w2 = 0.;
+    else
+      p.s.w2 = std::copysign(st.wgt_survival, p.s.w2);
+  }
+  if (p.s.w == 0. && p.s.w2 == 0.) p.alive = false;
+}
+
+static void make_fission_neutrons(Ctx& c, Particle& p, const MicroXS& x,
+                                  const Material& m) {  // :358-487
+  const abx_settings& st = c.P->st;
+  double k_abs_scr = p.s.w * x.nu_total * x.fission / x.total;
+  int n_new = 0;
+  if (st.mode == ABX_K_EIGENVALUE)
+    n_new = (int)std::floor(std::abs(k_abs_scr) / c.kcol + RNG::rand(p.rng));
+  else
+    n_new = (int)std::floor(std::abs(k_abs_scr) + RNG::rand(p.rng));
+  double P_delayed = x.nu_delayed / x.nu_total;
+  for (int i = 0; i < n_new; i++) {
+    // MGNuclide::sample_fission (src/mg_nuclide.cpp:504-544)
+    const auto& chi = m.chi[(size_t)x.energy_index];
+    size_t ei = (size_t)RNG::discrete(p.rng, chi.data(), chi.data() + chi.size());
+    double E_out = 0.5 * (c.P->ebounds[ei] + c.P->ebounds[ei + 1]);
+    double mu = 2. * RNG::rand(p.rng) - 1.;
+    double phi = 2. * PI * RNG::rand(p.rng);
+    V3 uout = rotate_direction(p.s.u, mu, phi);
+    if (RNG::rand(p.rng) < P_delayed) {
+      (void)RNG::discrete(p.rng, m.P_delayed_group.data(),
+                          m.P_delayed_group.data() + m.P_delayed_group.size());
+    }
+    double wgt = p.s.w > 0. ? 1. : -1.;
+    double wgt2 = 0.;
+    if (st.mode == ABX_FIXED_SOURCE) {
+      p.make_secondary(uout, E_out, wgt, wgt2);
+    } else {
+      abx_banked b{};
+      b.r[0] = p.s.r.x;
+      b.r[1] = p.s.r.y;
+      b.r[2] = p.s.r.z;
+      b.u[0] = uout.x;
+      b.u[1] = uout.y;
+      b.u[2] = uout.z;
+      b.E = E_out;
+      b.wgt = wgt;
+      b.wgt2 = wgt2;
+      b.parent_history_id = p.history_id;
+      b.parent_daughter_id = p.daughter_counter++;
+      b.family_id = p.family_id;
+      p.fission.push_back(b);
+    }
+  }
+}
+
+static void do_scatter(Ctx& c, Particle& p, const Material& m,
+                       const MicroXS& x) {  // :317-347, mg_nuclide.cpp:442-461
+  const auto& ps = m.Ps[(size_t)x.energy_index];
+  size_t ei = (size_t)RNG::discrete(p.rng, ps.data(), ps.data() + ps.size());
+  double E_out = 0.5 * (c.P->ebounds[ei] + c.P->ebounds[ei + 1]);
+  double mu = m.angle[(size_t)x.energy_index][ei].sample_mu(p.rng);
+  double phi = 2. * PI * RNG::rand(p.rng);
+  V3 u_out = rotate_direction(p.s.u, mu, phi);
+  p.set_direction(u_out);
+  p.set_energy(E_out);
+  p.s.w = p.s.w * 1.;
+  p.s.w2 = p.s.w2 * 1.;
+}
+
+// Transporter::collision + branching_collision (:60-93, :269-315)
+static void collision(Ctx& c, Particle& p, int mi) {
+  const Problem& P = *c.P;
+  tally_collision_all(c, p, mi);
+  {
+    double k_col_scr = p.s.w * mat_vEf(P, mi, p.s.E) / mat_Et(P, mi, p.s.E);
+    double mig_dist = norm(sub(p.s.r, p.r_birth));
+    double mig_area_scr =
+        p.s.w * mat_Ea(P, mi, p.s.E) / mat_Et(P, mi, p.s.E) * mig_dist * mig_dist;
+    c.sc.k_col += k_col_scr;
+    c.sc.mig += mig_area_scr;
+  }
+  const Material& m = P.materials[(size_t)mi];
+  // MaterialHelper::sample_nuclide: one draw, single component
+  (void)RNG::rand(p.rng);
+  MicroXS x = micro_xs(P, m, p.s.E);
+  c.sc.k_abs += p.s.w * x.nu_total * x.fission / x.total;
+  make_fission_neutrons(c, p, x, m);
+  p.s.w = p.s.w * (1. - (x.absorption + 0.) / x.total);
+  p.s.w2 = p.s.w2 * (1. - (x.absorption + 0.) / x.total);
+  russian_roulette(c, p);
+  if (p.alive) {
+    do_scatter(c, p, m, x);
+    if (p.s.E < P.min_energy) p.alive = false;
+  }
+}
+
+static void leak_score(Ctx& c, Particle& p, const BoundaryInfo& b) {
+  p.alive = false;
+  c.sc.leak += p.s.w;
+  V3 r_leak = add(p.s.r, scale(p.s.u, b.distance));
+  V3 dr = sub(r_leak, p.r_birth);
+  // w * (r_leak - r_birth) * (r_leak - r_birth): scale first, then dot
+  c.sc.mig += dot(scale(dr, p.s.w), dr);
+}
+
+// Shared tail of each loop iteration: resurrection of secondaries.
+static void try_resurrect(Ctx& c, Particle& p, Tracker& t, int& mi) {
+  if (p.alive) return;
+  p.resurect();
+  if (p.alive) {
+    c.diag.secondaries++;
+    t.set_r(p.s.r);
+    t.set_u(p.s.u);
+    t.restart_get_current();
+    if (t.is_lost()) fatal(ABX_E_LOST, "lost at resurrection");
+    mi = t.material();
+    c.rec(p, t, ABX_EV_RESURRECT);
+  }
+}
+
+// DeltaTracker::transport loop body (src/delta_tracker.cpp:86-229) and
+// CarterTracker (src/carter_tracker.cpp:104-260)
+static void run_history_dt(Ctx& c, Particle& p, bool carter) {
+  const Problem& P = *c.P;
+  Tracker t(&P, p.s.r, p.s.u);
+  if (t.is_lost()) {
+    p.alive = false;
+    c.diag.lost++;
+  }
+  int mi = t.material();
+  while (p.alive) {
+    bool had_collision = false;
+    bool crossed = false;
+    int g = energy_grid_index(P, p.s.E);
+    double Emaj = (carter ? P.sampling[(size_t)g] : P.majorant[(size_t)g]) + 0.;
+    p.Esmp = Emaj;
+    double d_coll = RNG::exponential(p.rng, Emaj);
+    c.diag.flights++;
+    BoundaryInfo bound{INF, -1, ABX_NORMAL, 0};
+    t.move(d_coll);
+    t.get_current();
+    if (t.is_lost()) {
+      t.set_r(p.s.r);
+      t.get_current();
+      bound = t.get_boundary_condition();
+      crossed = true;
+    }
+    tally_flight_all(c, p, std::min(d_coll, bound.distance), mi);
+    int ev = ABX_EV_VIRTUAL;
+    if (crossed) {
+      c.diag.crossings++;
+      if (bound.btype == ABX_VACUUM) {
+        leak_score(c, p, bound);
+        ev = ABX_EV_LEAK;
+      } else if (bound.btype == ABX_REFLECTIVE) {
+        t.do_reflection(p, bound);
+        if (t.is_lost()) fatal(ABX_E_LOST, "lost after reflection");
+        ev = ABX_EV_REFLECT;
+      } else {
+        fatal(ABX_E_LOST, "Help me, how did I get here ?");
+      }
+    } else {
+      p.move(d_coll);
+      mi = t.material();
+      double Et = mat_Et(P, mi, p.s.E);
+      if (!carter) {
+        if (Et - Emaj > 1.E-10) fatal(ABX_E_MAJORANT, "Sigma_t exceeded majorant");
+        if (RNG::rand(p.rng) < (Et / Emaj)) had_collision = true;
+      } else {
+        if (Emaj >= Et) {
+          if (RNG::rand(p.rng) < (Et / Emaj)) had_collision = true;
+        } else {
+          double D = Et / (2. * Et - Emaj);
+          double F = Et / (D * Emaj);
+          if ((D - RNG::rand(p.rng)) > 0.) {
+            p.s.w = p.s.w * F;
+            had_collision = true;
+          } else {
+            p.s.w = -p.s.w * F;
+          }
+        }
+      }
+    }
+    if (p.alive && had_collision) {
+      c.diag.collisions++;
+      collision(c, p, mi);
+      t.set_u(p.s.u);
+      p.prev_virtual = false;
+      ev = p.alive ? ABX_EV_COLLISION : ABX_EV_DEATH;
+    } else if (p.alive) {
+      if (!crossed) c.diag.virtual_collisions++;
+      p.prev_virtual = true;
+    }
+    if (carter && p.alive && std::abs(p.s.w) >= P.st.wgt_split) {
+      int n_new = (int)std::ceil(std::abs(p.s.w));
+      p.split(n_new);
+    }
+    c.rec(p, t, ev);
+    try_resurrect(c, p, t, mi);
+  }
+}
+
+// SurfaceTracker::transport loop body (src/surface_tracker.cpp:70-200)
+static void run_history_st(Ctx& c, Particle& p) {
+  const Problem& P = *c.P;
+  Tracker t(&P, p.s.r, p.s.u);
+  if (t.is_lost()) {
+    p.alive = false;
+    c.diag.lost++;
+  }
+  int mi = t.material();
+  while (p.alive) {
+    bool had_collision = false;
+    double d_coll = RNG::exponential(p.rng, mat_Et(P, mi, p.s.E));
+    c.diag.flights++;
+    BoundaryInfo bound = t.get_nearest_boundary();
+    double dmin = std::min(d_coll, bound.distance);
+    tally_flight_all(c, p, dmin, mi);
+    c.sc.k_trk += p.s.w * dmin * mat_vEf(P, mi, p.s.E);
+    int ev = ABX_EV_CROSS;
+    if (bound.distance < d_coll || std::abs(bound.distance - d_coll) < BOUNDRY_TOL) {
+      c.diag.crossings++;
+      if (bound.btype == ABX_VACUUM) {
+        leak_score(c, p, bound);
+        ev = ABX_EV_LEAK;
+      } else if (bound.btype == ABX_REFLECTIVE) {
+        t.do_reflection(p, bound);
+        if (t.is_lost()) fatal(ABX_E_LOST, "lost after reflection");
+        ev = ABX_EV_REFLECT;
+      } else {
+        t.cross_surface(bound);
+        t.get_current();
+        p.move(bound.distance);
+        if (t.is_lost()) fatal(ABX_E_LOST, "lost after crossing");
+        mi = t.material();
+      }
+    } else {
+      p.move(d_coll);
+      t.move(d_coll);
+      had_collision = true;
+    }
+    if (p.alive && had_collision) {
+      c.diag.collisions++;
+      collision(c, p, mi);
+      t.set_u(p.s.u);
+      ev = p.alive ? ABX_EV_COLLISION : ABX_EV_DEATH;
+    }
+    c.rec(p, t, ev);
+    try_resurrect(c, p, t, mi);
+  }
+}
+
+//==========================================================================
+// Source sampling (src/simulation.cpp:55-77, src/source.cpp:44-90)
+static abx_particle sample_one(Problem& P, uint64_t h) {
+  Pcg32 rng;
+  rng.seed(P.st.seed);
+  rng.advance(P.st.stride * h);
+  std::vector<double> wgts;
+  for (auto& s : P.sources) wgts.push_back(s.d.weight);
+  size_t indx = (size_t)RNG::discrete(rng, wgts.data(), wgts.data() + wgts.size());
+  const abx_source& s = P.sources[indx].d;
+  V3 u;
+  if (s.direction == ABX_DIR_ISOTROPIC) {
+    double mu = 2. * RNG::rand(rng) - 1.;
+    double phi = 2. * PI * RNG::rand(rng);
+    u = direction_mu_phi(mu, phi);
+  } else {
+    u = direction(s.dir[0], s.dir[1], s.dir[2]);
+  }
+  double E = s.energy;
+  int E_count = 0;
+  do {
+    if (E_count > 200) fatal(ABX_E_INVALID, "Exceded 200 samplings of energy.");
+    E = s.energy;
+    E_count++;
+  } while (E <= P.min_energy || P.max_energy <= E);
+  auto sample_r = [&](Pcg32& g) -> V3 {
+    if (s.spatial == ABX_SRC_POINT) return {s.low[0], s.low[1], s.low[2]};
+    double x = (s.high[0] - s.low[0]) * RNG::rand(g) + s.low[0];
+    double y = (s.high[1] - s.low[1]) * RNG::rand(g) + s.low[1];
+    double z = (s.high[2] - s.low[2]) * RNG::rand(g) + s.low[2];
+    return {x, y, z};
+  };
+  V3 r = sample_r(rng);
+  Tracker t(&P, r, u);
+  int guard = 0;
+  while (t.is_lost()) {
+    if (++guard > 100000) fatal(ABX_E_LOST, "source never inside geometry");
+    r = sample_r(rng);
+    t.set_r(r);
+    t.restart_get_current();
+  }
+  if (s.fissile_only) {
+    int count = 0;
+    while (t.is_lost() || !P.materials[(size_t)t.material()].fissile) {
+      if (count == 201)
+        fatal(ABX_E_INVALID, "Exceded 200 samplings of position in fissile-only source.");
+      r = sample_r(rng);
+      t.set_r(r);
+      t.restart_get_current();
+      count++;
+    }
+  }
+  abx_particle out{};
+  out.r[0] = r.x;
+  out.r[1] = r.y;
+  out.r[2] = r.z;
+  out.u[0] = u.x;
+  out.u[1] = u.y;
+  out.u[2] = u.z;
+  out.E = E;
+  out.wgt = 1.0;
+  out.wgt2 = 0.;
+  out.history_id = h;
+  out.family_id = h;
+  out.rng_state = rng.state;
+  return out;
+}
+
+//==========================================================================
+// Problem construction (validation as MGNuclide checks, src/mg_nuclide.cpp:235-282)
+static void build(Problem& P, const abx_problem_desc* d) {
+  P.st = d->settings;
+  const int G = d->settings.ngroups;
+  if (G <= 0 || !d->settings.energy_bounds) fatal(ABX_E_INVALID, "ngroups");
+  P.ebounds.assign(d->settings.energy_bounds, d->settings.energy_bounds + G + 1);
+  for (int i = 0; i < d->n_surfaces; i++) {
+    const abx_surface& s = d->surfaces[i];
+    Surface S;
+    S.type = s.type;
+    S.bc = s.boundary;
+    S.id = s.id;
+    for (int k = 0; k < 4; k++) S.p[k] = s.p[k];
+    P.surfaces.push_back(S);
+  }
+  for (int i = 0; i < d->n_cells; i++) {
+    const abx_cell& c = d->cells[i];
+    Cell C;
+    C.id = c.id;
+    C.fill_type = c.fill_type;
+    C.fill = c.fill;
+    C.rpn.assign(d->rpn + c.rpn_offset, d->rpn + c.rpn_offset + c.rpn_len);
+    // Cell::simplify (src/cell.cpp:210-232)
+    C.simple = true;
+    for (int32_t el : C.rpn)
+      if (el == ABX_OP_COMP || el == ABX_OP_UNIN) {
+        C.simple = false;
+        break;
+      }
+    if (C.simple) {
+      std::vector<int32_t> t;
+      for (int32_t el : C.rpn)
+        if (el < ABX_OP_UNIN) t.push_back(el);
+      C.rpn = t;
+    }
+    // Cell::check_for_bc (src/cell.cpp:196-208)
+    for (int32_t token : C.rpn) {
+      if (token >= ABX_OP_UNIN) continue;
+      int b = P.surfaces[(size_t)(std::abs(token) - 1)].bc;
+      if (b == ABX_VACUUM || b == ABX_REFLECTIVE) C.vacuum_or_reflective = true;
+    }
+    P.cells.push_back(C);
+  }
+  for (int i = 0; i < d->n_universes; i++) {
+    const abx_universe& u = d->universes[i];
+    Universe U;
+    U.id = u.id;
+    U.type = u.type;
+    if (u.type == ABX_CELL_UNIVERSE) {
+      U.cells.assign(d->universe_cells + u.cells_offset,
+                     d->universe_cells + u.cells_offset + u.n_cells);
+    } else {
+      U.Nx = (uint32_t)u.shape[0];
+      U.Ny = (uint32_t)u.shape[1];
+      U.Nz = (uint32_t)u.shape[2];
+      U.Px = u.pitch[0];
+      U.Py = u.pitch[1];
+      U.Pz = u.pitch[2];
+      U.Px_inv = 1. / U.Px;
+      U.Py_inv = 1. / U.Py;
+      U.Pz_inv = 1. / U.Pz;
+      U.Xl = u.origin[0] - (double)U.Nx * 0.5 * U.Px;
+      U.Yl = u.origin[1] - (double)U.Ny * 0.5 * U.Py;
+      U.Zl = u.origin[2] - (double)U.Nz * 0.5 * U.Pz;
+      size_t ne = (size_t)U.Nx * U.Ny * U.Nz;
+      U.elems.assign(d->lattice_elems + u.elems_offset,
+                     d->lattice_elems + u.elems_offset + ne);
+      U.outer = u.outer;
+    }
+    P.universes.push_back(U);
+  }
+  // has_boundary_conditions: cell universes from their cells
+  // (src/cell_universe.cpp:30-41), lattices from the outer universe
+  // (src/lattice.cpp:35-52) - lattices are built after the universes they
+  // reference, so resolve cell universes first.
+  for (auto& U : P.universes)
+    if (U.type == ABX_CELL_UNIVERSE)
+      for (int ci : U.cells)
+        if (P.cells[(size_t)ci].vacuum_or_reflective) U.has_bc = true;
+  for (int pass = 0; pass < 8; pass++)
+    for (auto& U : P.universes)
+      if (U.type == ABX_RECT_LATTICE && U.outer >= 0)
+        U.has_bc = P.universes[(size_t)U.outer].has_bc;
+  P.root = d->root_universe;
+
+  for (int mi = 0; mi < d->n_materials; mi++) {
+    const abx_material& a = d->materials[mi];
+    Material M;
+    M.id = a.id;
+    M.G = G;
+    M.Et.assign(a.total, a.total + G);
+    M.Ea.assign(a.absorption, a.absorption + G);
+    M.Ef.assign((size_t)G, 0.);
+    M.nu_p.assign((size_t)G, 0.);
+    M.nu_d.assign((size_t)G, 0.);
+    bool fis = false;
+    if (a.fission) {
+      M.Ef.assign(a.fission, a.fission + G);
+      for (double f : M.Ef)
+        if (f > 0.) fis = true;
+    }
+    if (fis) {
+      if (!a.nu_prompt) fatal(ABX_E_INVALID, "No nu data");
+      M.nu_p.assign(a.nu_prompt, a.nu_prompt + G);
+      if (a.nu_delayed) {
+        M.nu_d.assign(a.nu_delayed, a.nu_delayed + G);
+      }
+    }
+    // nu_delyd_ is always sized G (zeros without delayed data), so nu_total
+    // adds it (src/mg_nuclide.cpp:353-359)
+    M.has_nu_d = true;
+    M.chi.assign((size_t)G, std::vector<double>((size_t)G, 0.));
+    if (fis) {
+      if (!a.chi) fatal(ABX_E_INVALID, "Invalid chi matrix");
+      for (int i = 0; i < G; i++)
+        for (int o = 0; o < G; o++) M.chi[(size_t)i][(size_t)o] = a.chi[i * G + o];
+    }
+    // MGNuclide::normalize_chi (:73-93)
+    for (int i = 0; i < G; i++) {
+      double chi_i = 0.;
+      for (int o = 0; o < G; o++) chi_i += M.chi[(size_t)i][(size_t)o];
+      for (int o = 0; o < G; o++) M.chi[(size_t)i][(size_t)o] /= chi_i;
+    }
+    // make_scatter_xs (:95-118)
+    M.Ps.assign((size_t)G, std::vector<double>((size_t)G, 0.));
+    M.Es.assign((size_t)G, 0.);
+    for (int i = 0; i < G; i++)
+      for (int o = 0; o < G; o++) M.Ps[(size_t)i][(size_t)o] = a.scatter[i * G + o];
+    for (int i = 0; i < G; i++) {
+      M.Es[(size_t)i] = 0.;
+      for (int o = 0; o < G; o++) M.Es[(size_t)i] += M.Ps[(size_t)i][(size_t)o];
+      for (int o = 0; o < G; o++) M.Ps[(size_t)i][(size_t)o] /= M.Es[(size_t)i];
+    }
+    M.angle.assign((size_t)G, std::vector<AngleDist>((size_t)G));
+    if (a.angle_offsets) {
+      for (int i = 0; i < G; i++)
+        for (int o = 0; o < G; o++) {
+          int b0 = a.angle_offsets[i * G + o], b1 = a.angle_offsets[i * G + o + 1];
+          AngleDist& ad = M.angle[(size_t)i][(size_t)o];
+          ad.mu.assign(a.angle_mu + b0, a.angle_mu + b1);
+          ad.pdf.assign(a.angle_pdf + b0, a.angle_pdf + b1);
+          ad.cdf.assign(a.angle_cdf + b0, a.angle_cdf + b1);
+        }
+    }
+    if (a.n_delayed > 0) {
+      M.P_delayed_group.assign(a.delayed_probabilities,
+                               a.delayed_probabilities + a.n_delayed);
+      M.delayed_constants.assign(a.delayed_constants,
+                                 a.delayed_constants + a.n_delayed);
+    }
+    // check_xs (:235-282)
+    for (int i = 0; i < G; i++) {
+      if (M.Ea[(size_t)i] < 0. || M.Ef[(size_t)i] < 0.)
+        fatal(ABX_E_INVALID, "negative cross section");
+      if (M.Ef[(size_t)i] > M.Ea[(size_t)i]) fatal(ABX_E_INVALID, "Ef > Ea");
+      double diff = M.Et[(size_t)i] - (M.Es[(size_t)i] + M.Ea[(size_t)i]);
+      if (std::abs(diff) / M.Et[(size_t)i] > 0.001)
+        fatal(ABX_E_INVALID, "Es + Ea != Et");
+    }
+    // check_fissile (:323-330)
+    for (int i = 0; i < G; i++)
+      if (M.Ef[(size_t)i] != 0. && (M.nu_p[(size_t)i] + M.nu_d[(size_t)i]) > 0.) {
+        M.fissile = true;
+        break;
+      }
+    P.materials.push_back(M);
+  }
+  // make_materials: energy limits (src/parser.cpp:152-167)
+  P.min_energy = std::max(0., P.ebounds.front());
+  P.max_energy = std::min(100000., P.ebounds.back());
+  // make_majorant_xs, MG branch (src/majorant.cpp:133-176)
+  P.majorant.assign((size_t)G, 0.);
+  for (const auto& M : P.materials)
+    for (int g = 0; g < G; g++)
+      if (M.Et[(size_t)g] > P.majorant[(size_t)g]) P.majorant[(size_t)g] = M.Et[(size_t)g];
+  // CarterTracker sampling xs (src/carter_tracker.cpp:60-75)
+  P.sampling = P.majorant;
+  if (d->settings.sample_xs_ratio) {
+    P.ratio.assign(d->settings.sample_xs_ratio, d->settings.sample_xs_ratio + G);
+    for (int g = 0; g < G; g++) P.sampling[(size_t)g] = P.majorant[(size_t)g] * P.ratio[(size_t)g];
+  }
+  for (int i = 0; i < d->n_sources; i++) P.sources.push_back({d->sources[i]});
+  for (int i = 0; i < d->n_tallies; i++) {
+    const abx_mesh_tally& a = d->tallies[i];
+    MeshTally T;
+    T.estimator = a.estimator;
+    T.quantity = a.quantity;
+    T.Nx = a.shape[0];
+    T.Ny = a.shape[1];
+    T.Nz = a.shape[2];
+    T.r_low = {a.low[0], a.low[1], a.low[2]};
+    T.r_hi = {a.high[0], a.high[1], a.high[2]};
+    // MeshTally ctor (src/mesh_tally.cpp:62-116)
+    T.dx = (T.r_hi.x - T.r_low.x) / (double)T.Nx;
+    T.dy = (T.r_hi.y - T.r_low.y) / (double)T.Ny;
+    T.dz = (T.r_hi.z - T.r_low.z) / (double)T.Nz;
+    T.dx_inv = 1. / T.dx;
+    T.dy_inv = 1. / T.dy;
+    T.dz_inv = 1. / T.dz;
+    T.ebounds.assign(a.ebounds, a.ebounds + a.n_ebounds);
+    size_t nb = (size_t)(a.n_ebounds - 1) * T.Nx * T.Ny * T.Nz;
+    T.gen.assign(nb, 0.);
+    T.avg.assign(nb, 0.);
+    T.var.assign(nb, 0.);
+    // Tallies::add_*_mesh_tally: net weight = total weight = nparticles
+    T.net_weight = (double)d->settings.nparticles;
+    P.tallies.push_back(T);
+  }
+}
+
+static Particle make_particle(const abx_particle& a) {
+  Particle p;
+  p.s.r = {a.r[0], a.r[1], a.r[2]};
+  p.s.u = {a.u[0], a.u[1], a.u[2]};
+  p.s.E = a.E;
+  p.s.w = a.wgt;
+  p.s.w2 = a.wgt2;
+  p.history_id = a.history_id;
+  p.family_id = a.family_id;
+  p.rng.state = a.rng_state;
+  p.r_birth = p.s.r;
+  return p;
+}
+
+static void transport(Problem& P, const abx_particle* bank, size_t n,
+                      double kcol, bool converged, int nthreads,
+                      std::vector<abx_banked>& fission_out, Scores& total,
+                      abx_diag& diag, abx_trace_rec* trace, int cap,
+                      int32_t* n_rec) {
+  std::vector<std::vector<abx_banked>> fis(n);
+  std::string err;
+  int errcode = 0;
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel num_threads(nthreads)
+#endif
+  {
+    Ctx c;
+    c.P = &P;
+    c.kcol = kcol;
+    c.converged = converged;
+    std::vector<abx_trace_rec> tr;
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic)
+#endif
+    for (size_t i = 0; i < n; i++) {
+      if (errcode) continue;
+      try {
+        Particle p = make_particle(bank[i]);
+        c.rng0_calls = p.rng.calls;
+        if (trace) {
+          tr.clear();
+          c.trace = &tr;
+          c.trace_cap = cap;
+          c.n_rec = 0;
+        }
+        if (P.st.tracking == ABX_SURFACE_TRACKING)
+          run_history_st(c, p);
+        else
+          run_history_dt(c, p, P.st.tracking == ABX_CARTER_TRACKING);
+        c.diag.histories++;
+        fis[i] = std::move(p.fission);
+        if (trace) {
+          std::copy(tr.begin(), tr.end(), trace + i * (size_t)cap);
+          n_rec[i] = c.n_rec;
+        }
+      } catch (const Fatal& f) {
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+        {
+          if (!errcode) {
+            errcode = f.code;
+            err = f.msg;
+            diag.error_history = bank[i].history_id;
+          }
+        }
+      }
+    }
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+    {
+      total.k_col += c.sc.k_col;
+      total.k_abs += c.sc.k_abs;
+      total.k_trk += c.sc.k_trk;
+      total.k_tot += c.sc.k_tot;
+      total.leak += c.sc.leak;
+      total.mig += c.sc.mig;
+      diag.histories += c.diag.histories;
+      diag.collisions += c.diag.collisions;
+      diag.virtual_collisions += c.diag.virtual_collisions;
+      diag.flights += c.diag.flights;
+      diag.crossings += c.diag.crossings;
+      diag.tally_updates += c.diag.tally_updates;
+      diag.lost += c.diag.lost;
+      diag.secondaries += c.diag.secondaries;
+    }
+  }
+  if (errcode) {
+    diag.error = errcode;
+    fatal(errcode, err);
+  }
+  for (auto& v : fis) fission_out.insert(fission_out.end(), v.begin(), v.end());
+}
+
+static void tally_clear(Problem& P) {
+  P.gen_scores = Scores();
+  for (auto& t : P.tallies) std::fill(t.gen.begin(), t.gen.end(), 0.);
+}
+
+// Tallies::update_avg_and_var (src/tallies.cpp:268-280)
+static void upd(int gen, double x, double& avg, double& var) {
+  double dgen = (double)gen;
+  double avg_old = avg, var_old = var;
+  avg = avg_old + (x - avg_old) / dgen;
+  if (gen > 1) var = var_old + ((x - avg_old) * (x - avg_old) / dgen) - (var_old / (dgen - 1.));
+}
+
+static void record_generation(Problem& P, double kc, double kt, double lk,
+                              double mg) {
+  P.gen++;
+  upd(P.gen, kc, P.k_col_avg, P.k_col_var);
+  upd(P.gen, kt, P.k_trk_avg, P.k_trk_var);
+  upd(P.gen, lk, P.leak_avg, P.leak_var);
+  upd(P.gen, mg, P.mig_avg, P.mig_var);
+  for (auto& t : P.tallies) {  // MeshTally::record_generation (:121-150)
+    t.g++;
+    const double dg = (double)t.g;
+    for (size_t i = 0; i < t.gen.size(); i++) {
+      double old_avg = t.avg[i];
+      double val = t.gen[i] * 1.0;
+      double avg = old_avg + (val - old_avg) / dg;
+      t.avg[i] = avg;
+      double var = t.var[i];
+      var = var + (((val - old_avg) * (val - avg) - (var)) / dg);
+      t.var[i] = var;
+    }
+  }
+}
+
+static void add_diag(abx_diag& a, const abx_diag& b) {
+  a.histories += b.histories;
+  a.collisions += b.collisions;
+  a.virtual_collisions += b.virtual_collisions;
+  a.flights += b.flights;
+  a.crossings += b.crossings;
+  a.tally_updates += b.tally_updates;
+  a.lost += b.lost;
+  a.secondaries += b.secondaries;
+}
+
+// FixedSource::run, ModifiedFixedSource::run and PowerIterator::run
+static void run(Problem& P, int ngen, int nignored, int nthreads,
+                double* kcol_hist, abx_run_stats& S) {
+  const uint64_t N = (uint64_t)P.st.nparticles;
+  const double W = (double)N;
+  uint64_t histories_counter = 0;
+  std::vector<abx_particle> bank;
+  double kcol = 1.0;
+  bool converged = P.st.mode != ABX_K_EIGENVALUE || nignored == 0;
+  S = abx_run_stats{};
+  double seconds = 0;
+  if (P.st.mode == ABX_K_EIGENVALUE) {
+    bank.resize(N);
+    for (uint64_t i = 0; i < N; i++) bank[i] = sample_one(P, histories_counter++);
+  }
+  uint64_t global_counter = histories_counter;
+  for (int g = 1; g <= ngen; g++) {
+    tally_clear(P);
+    Scores sc;
+    abx_diag dg{};
+    std::vector<abx_banked> next;
+    if (P.st.mode == ABX_FIXED_SOURCE || P.st.mode == ABX_MODIFIED_FIXED_SOURCE) {
+      bank.resize(N);
+      for (uint64_t i = 0; i < N; i++) bank[i] = sample_one(P, histories_counter++);
+      double t0 = omp_get_wtime();
+      transport(P, bank.data(), bank.size(), kcol, true, nthreads, next, sc, dg,
+                nullptr, 0, nullptr);
+      while (P.st.mode == ABX_MODIFIED_FIXED_SOURCE && !next.empty()) {
+        bank.clear();
+        for (auto& b : next) {  // src/modified_fixed_source.cpp:88-93
+          abx_particle a{};
+          for (int k = 0; k < 3; k++) {
+            a.r[k] = b.r[k];
+            a.u[k] = b.u[k];
+          }
+          a.E = b.E;
+          a.wgt = b.wgt;
+          a.wgt2 = b.wgt2;
+          a.history_id = histories_counter++;
+          a.family_id = 0;
+          Pcg32 r;
+          r.seed(P.st.seed);
+          r.advance(P.st.stride * a.history_id);
+          a.rng_state = r.state;
+          bank.push_back(a);
+        }
+        next.clear();
+        transport(P, bank.data(), bank.size(), kcol, true, nthreads, next, sc,
+                  dg, nullptr, 0, nullptr);
+      }
+      seconds += omp_get_wtime() - t0;
+      double kc = sc.k_col / W;
+      if (kcol_hist) kcol_hist[g - 1] = kc;
+      record_generation(P, kc, sc.k_trk / W, sc.leak / W, sc.mig / W);
+    } else {
+      double t0 = omp_get_wtime();
+      transport(P, bank.data(), bank.size(), kcol, converged, nthreads, next, sc,
+                dg, nullptr, 0, nullptr);
+      seconds += omp_get_wtime() - t0;
+      if (next.empty()) fatal(ABX_E_INVALID, "No fission neutrons were produced.");
+      double kc = sc.k_col / W;
+      if (kcol_hist) kcol_hist[g - 1] = kc;
+      // normalize_weights (src/power_iterator.cpp:538-586)
+      double W_pos = 0., W_neg = 0.;
+      for (auto& b : next) {
+        if (b.wgt > 0.)
+          W_pos += b.wgt;
+        else
+          W_neg -= b.wgt;
+      }
+      double Wn = W_pos - W_neg;
+      double w_per_part = (double)P.st.nparticles / Wn;
+      for (auto& b : next) b.wgt *= w_per_part;
+      if (converged) record_generation(P, kc, sc.k_trk / W, sc.leak / W, sc.mig / W);
+      // rebank (:394-404)
+      histories_counter = global_counter;
+      bank.clear();
+      for (auto& b : next) {
+        abx_particle a{};
+        for (int k = 0; k < 3; k++) {
+          a.r[k] = b.r[k];
+          a.u[k] = b.u[k];
+        }
+        a.E = b.E;
+        a.wgt = b.wgt;
+        a.wgt2 = 0.;
+        a.history_id = histories_counter++;
+        a.family_id = b.family_id;
+        Pcg32 r;
+        r.seed(P.st.seed);
+        r.advance(P.st.stride * a.history_id);
+        a.rng_state = r.state;
+        bank.push_back(a);
+      }
+      global_counter += next.size();
+      kcol = kc;
+      if (g == nignored) converged = true;
+    }
+    add_diag(S.diag, dg);
+  }
+  P.last_bank = bank;
+  S.kcol_avg = P.k_col_avg;
+  S.kcol_err = std::sqrt(P.k_col_var / (double)P.gen);
+  S.ktrk_avg = P.k_trk_avg;
+  S.ktrk_err = std::sqrt(P.k_trk_var / (double)P.gen);
+  S.leak_avg = P.leak_avg;
+  S.leak_err = std::sqrt(P.leak_var / (double)P.gen);
+  S.mig_avg = P.mig_avg;
+  S.mig_err = std::sqrt(P.mig_var / (double)P.gen);
+  S.active_generations = P.gen;
+  S.transport_seconds = seconds;
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+struct orc_problem {
+  Problem P;
+};
+
+static thread_local std::string g_last;
+
+extern "C" {
+
+const char* orc_last_error(void) { return g_last.c_str(); }
+
+double orc_log(int portable, double x) { return portable ? dm_log(x) : std::log(x); }
+double orc_sin(int portable, double x) { return portable ? dm_sin(x) : std::sin(x); }
+double orc_cos(int portable, double x) { return portable ? dm_cos(x) : std::cos(x); }
+
+#define ORC_TRY(body)                     \
+  try {                                   \
+    g_portable = p->P.portable;           \
+    body;                                 \
+    return ABX_OK;                        \
+  } catch (const Fatal& f) {              \
+    g_last = f.msg;                       \
+    return f.code;                        \
+  } catch (const std::exception& e) {     \
+    g_last = e.what();                    \
+    return ABX_E_INVALID;                 \
+  }
+
+int orc_create(const abx_problem_desc* desc, int portable_math,
+               orc_problem** out) {
+  try {
+    auto* p = new orc_problem;
+    p->P.portable = portable_math != 0;
+    g_portable = p->P.portable;
+    build(p->P, desc);
+    *out = p;
+    return ABX_OK;
+  } catch (const Fatal& f) {
+    g_last = f.msg;
+    return f.code;
+  }
+}
+
+void orc_destroy(orc_problem* p) { delete p; }
+
+int orc_sample_sources(orc_problem* p, uint64_t first, size_t n,
+                       abx_particle* out) {
+  ORC_TRY(for (size_t i = 0; i < n; i++) out[i] = sample_one(p->P, first + i));
+}
+
+int orc_transport(orc_problem* p, const abx_particle* bank, size_t n,
+                  double kcol, int converged, int nthreads,
+                  abx_banked* fission_out, size_t cap, size_t* n_out,
+                  abx_scores* scores, abx_diag* diag, abx_trace_rec* trace,
+                  int cap_per_history, int32_t* n_rec) {
+  ORC_TRY({
+    std::vector<abx_banked> fo;
+    Scores sc;
+    abx_diag dg{};
+    transport(p->P, bank, n, kcol, converged != 0, nthreads, fo, sc, dg, trace,
+              cap_per_history, n_rec);
+    if (n_out) *n_out = fo.size();
+    if (fo.size() > cap && fission_out) fatal(ABX_E_CAPACITY, "fission bank capacity");
+    if (fission_out) std::copy(fo.begin(), fo.end(), fission_out);
+    p->P.gen_scores.k_col += sc.k_col;
+    if (scores) {
+      scores->k_col = sc.k_col;
+      scores->k_abs = sc.k_abs;
+      scores->k_trk = sc.k_trk;
+      scores->k_tot = sc.k_tot;
+      scores->leakage = sc.leak;
+      scores->mig = sc.mig;
+    }
+    if (diag) *diag = dg;
+  });
+}
+
+int orc_tally_size(orc_problem* p, int tally, size_t* n_bins) {
+  ORC_TRY({
+    if (tally < 0 || tally >= (int)p->P.tallies.size()) fatal(ABX_E_INVALID, "tally index");
+    *n_bins = p->P.tallies[(size_t)tally].gen.size();
+  });
+}
+
+int orc_tally_fetch(orc_problem* p, int tally, double* gen_out) {
+  ORC_TRY({
+    if (tally < 0 || tally >= (int)p->P.tallies.size()) fatal(ABX_E_INVALID, "tally index");
+    auto& t = p->P.tallies[(size_t)tally];
+    std::copy(t.gen.begin(), t.gen.end(), gen_out);
+  });
+}
+
+int orc_tally_clear(orc_problem* p) { ORC_TRY(tally_clear(p->P)); }
+
+int orc_run(orc_problem* p, int ngenerations, int nignored, int nthreads,
+            double* kcol_hist, abx_run_stats* stats) {
+  ORC_TRY(run(p->P, ngenerations, nignored, nthreads, kcol_hist, *stats));
+}
+
+int orc_tally_stats(orc_problem* p, int tally, double* avg, double* std_err) {
+  ORC_TRY({
+    if (tally < 0 || tally >= (int)p->P.tallies.size()) fatal(ABX_E_INVALID, "tally index");
+    auto& t = p->P.tallies[(size_t)tally];
+    for (size_t i = 0; i < t.avg.size(); i++) {
+      avg[i] = t.avg[i];
+      std_err[i] = std::sqrt(t.var[i] / (double)t.g);
+    }
+  });
+}
+
+int orc_bank_size(orc_problem* p, size_t* n) { ORC_TRY(*n = p->P.last_bank.size()); }
+
+int orc_bank_fetch(orc_problem* p, double* out9) {
+  ORC_TRY({
+    const auto& b = p->P.last_bank;
+    for (size_t i = 0; i < b.size(); i++) {
+      double* o = out9 + 9 * i;
+      o[0] = b[i].r[0];
+      o[1] = b[i].r[1];
+      o[2] = b[i].r[2];
+      o[3] = b[i].u[0];
+      o[4] = b[i].u[1];
+      o[5] = b[i].u[2];
+      o[6] = b[i].E;
+      o[7] = b[i].wgt;
+      o[8] = b[i].wgt2;
+    }
+  });
+}
+
+int orc_locate(orc_problem* p, const double* r, const double* u, size_t n,
+               int32_t* cell, int32_t* material) {
+  ORC_TRY({
+    for (size_t i = 0; i < n; i++) {
+      Tracker t(&p->P, {r[3 * i], r[3 * i + 1], r[3 * i + 2]},
+                {u[3 * i], u[3 * i + 1], u[3 * i + 2]});
+      cell[i] = t.cell;
+      material[i] = t.material();
+    }
+  });
+}
+
+int orc_nearest_boundary(orc_problem* p, const double* r, const double* u,
+                         size_t n, double* dist, int32_t* surface,
+                         int32_t* btype) {
+  ORC_TRY({
+    for (size_t i = 0; i < n; i++) {
+      Tracker t(&p->P, {r[3 * i], r[3 * i + 1], r[3 * i + 2]},
+                {u[3 * i], u[3 * i + 1], u[3 * i + 2]});
+      if (t.is_lost()) {
+        dist[i] = -1;
+        surface[i] = -1;
+        btype[i] = -1;
+        continue;
+      }
+      BoundaryInfo b = t.get_nearest_boundary();
+      dist[i] = b.distance;
+      surface[i] = b.surface_index;
+      btype[i] = b.btype;
+    }
+  });
+}
+
+int orc_majorant(orc_problem* p, double* maj) {
+  ORC_TRY(std::copy(p->P.majorant.begin(), p->P.majorant.end(), maj));
+}
+
+}  // extern "C"

@@ -1,0 +1,71 @@
+/*
+ * oracle.h - C API of the CPU restatement (test infrastructure).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library restates, on the CPU and from the
+ * reference sources, Abeille's multigroup transport path so that the HIP
+ * implementation can be checked against it.  Only tests/, the smoke check
+ * and bench.py's CPU-baseline leg may load it; the product never does.
+ *
+ * It consumes the same flat description as the product (include/abx.h).
+ * `portable_math` selects the deterministic log/sin/cos shared with the GPU
+ * (1) or glibc's (0, the reference's own libm).
+ */
+#ifndef ORACLE_H
+#define ORACLE_H
+
+#include "../include/abx.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_problem orc_problem;
+
+const char* orc_last_error(void);
+int orc_create(const abx_problem_desc* desc, int portable_math,
+               orc_problem** out);
+void orc_destroy(orc_problem* p);
+
+int orc_sample_sources(orc_problem* p, uint64_t first_history, size_t n,
+                       abx_particle* out);
+
+/* One Transporter::transport call.  trace may be NULL. */
+int orc_transport(orc_problem* p, const abx_particle* bank, size_t n,
+                  double kcol, int converged, int nthreads,
+                  abx_banked* fission_out, size_t cap, size_t* n_out,
+                  abx_scores* scores, abx_diag* diag, abx_trace_rec* trace,
+                  int cap_per_history, int32_t* n_rec);
+
+int orc_tally_size(orc_problem* p, int tally, size_t* n_bins);
+int orc_tally_fetch(orc_problem* p, int tally, double* gen_out);
+int orc_tally_clear(orc_problem* p);
+
+/* Whole FixedSource / PowerIterator run. */
+int orc_run(orc_problem* p, int ngenerations, int nignored, int nthreads,
+            double* kcol_hist, abx_run_stats* stats);
+int orc_tally_stats(orc_problem* p, int tally, double* avg, double* std_err);
+int orc_bank_size(orc_problem* p, size_t* n);
+int orc_bank_fetch(orc_problem* p, double* out9);
+
+/* Geometry probe: locate points (Tracker construction) -> cell / material
+ * index (-1 if lost). */
+int orc_locate(orc_problem* p, const double* r, const double* u, size_t n,
+               int32_t* cell, int32_t* material);
+/* Nearest boundary (Tracker::get_nearest_boundary) from located points. */
+int orc_nearest_boundary(orc_problem* p, const double* r, const double* u,
+                         size_t n, double* dist, int32_t* surface,
+                         int32_t* btype);
+
+/* Majorant per group (src/majorant.cpp:133-176). */
+int orc_majorant(orc_problem* p, double* maj);
+
+/* Math used by the restatement. */
+double orc_log(int portable, double x);
+double orc_sin(int portable, double x);
+double orc_cos(int portable, double x);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,161 @@
+"""ctypes wrapper of the CPU restatement (test infrastructure only).
+
+Loaded by tests/, the smoke check and bench.py's CPU-baseline leg; never by
+the product.  Builds oracle/_build/liboracle.so with `make` if missing.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from abeille_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+_lib = None
+
+
+def build(force=False):
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_create.argtypes = [C.POINTER(abi.ProblemDesc), C.c_int, C.POINTER(P)]
+        L.orc_destroy.argtypes = [P]
+        L.orc_destroy.restype = None
+        L.orc_sample_sources.argtypes = [P, C.c_uint64, C.c_size_t, C.c_void_p]
+        L.orc_transport.argtypes = [P, C.c_void_p, C.c_size_t, C.c_double, C.c_int,
+                                    C.c_int, C.c_void_p, C.c_size_t,
+                                    C.POINTER(C.c_size_t), C.POINTER(abi.Scores),
+                                    C.POINTER(abi.Diag), C.c_void_p, C.c_int, C.c_void_p]
+        L.orc_tally_size.argtypes = [P, C.c_int, C.POINTER(C.c_size_t)]
+        L.orc_tally_fetch.argtypes = [P, C.c_int, C.c_void_p]
+        L.orc_tally_clear.argtypes = [P]
+        L.orc_run.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                              C.POINTER(abi.RunStats)]
+        L.orc_tally_stats.argtypes = [P, C.c_int, C.c_void_p, C.c_void_p]
+        L.orc_bank_size.argtypes = [P, C.POINTER(C.c_size_t)]
+        L.orc_bank_fetch.argtypes = [P, C.c_void_p]
+        L.orc_locate.argtypes = [P, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
+                                 C.c_void_p]
+        L.orc_nearest_boundary.argtypes = [P, C.c_void_p, C.c_void_p, C.c_size_t,
+                                           C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_majorant.argtypes = [P, C.c_void_p]
+        for f in ("orc_log", "orc_sin", "orc_cos"):
+            getattr(L, f).argtypes = [C.c_int, C.c_double]
+            getattr(L, f).restype = C.c_double
+        _lib = L
+    return _lib
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc != 0:
+        raise OracleError(f"{abi.ERRORS.get(rc, rc)}: {lib().orc_last_error().decode()}")
+
+
+class Oracle:
+    """One problem in the CPU restatement."""
+
+    def __init__(self, model, portable_math=True):
+        self.model = model
+        self.desc = model.desc()
+        h = C.c_void_p()
+        _check(lib().orc_create(C.byref(self.desc), 1 if portable_math else 0, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def sample_sources(self, first, n):
+        out = np.zeros(n, abi.PARTICLE_DTYPE)
+        _check(lib().orc_sample_sources(self.h, first, n, out.ctypes.data))
+        return out
+
+    def transport(self, bank, kcol=1.0, converged=True, nthreads=0, cap=None,
+                  trace_cap=0):
+        n = len(bank)
+        bank = np.ascontiguousarray(bank)
+        cap = cap if cap is not None else max(16, 4 * n)
+        fis = np.zeros(cap, abi.BANKED_DTYPE)
+        n_out = C.c_size_t()
+        sc = abi.Scores()
+        dg = abi.Diag()
+        tr = nrec = None
+        if trace_cap:
+            tr = np.zeros(n * trace_cap, abi.TRACE_DTYPE)
+            nrec = np.zeros(n, np.int32)
+        _check(lib().orc_transport(self.h, bank.ctypes.data, n, kcol, 1 if converged else 0,
+                                   nthreads, fis.ctypes.data, cap, C.byref(n_out),
+                                   C.byref(sc), C.byref(dg),
+                                   None if tr is None else tr.ctypes.data, trace_cap,
+                                   None if nrec is None else nrec.ctypes.data))
+        res = dict(fission=fis[:n_out.value].copy(), scores=sc, diag=dg)
+        if trace_cap:
+            res["trace"] = tr.reshape(n, trace_cap)
+            res["n_rec"] = nrec
+        return res
+
+    def tally(self, i):
+        nb = C.c_size_t()
+        _check(lib().orc_tally_size(self.h, i, C.byref(nb)))
+        out = np.zeros(nb.value)
+        _check(lib().orc_tally_fetch(self.h, i, out.ctypes.data))
+        return out
+
+    def tally_clear(self):
+        _check(lib().orc_tally_clear(self.h))
+
+    def run(self, ngenerations, nignored=0, nthreads=0):
+        st = abi.RunStats()
+        kh = np.zeros(ngenerations)
+        _check(lib().orc_run(self.h, ngenerations, nignored, nthreads, kh.ctypes.data,
+                             C.byref(st)))
+        return st, kh
+
+    def tally_stats(self, i):
+        nb = C.c_size_t()
+        _check(lib().orc_tally_size(self.h, i, C.byref(nb)))
+        avg = np.zeros(nb.value)
+        err = np.zeros(nb.value)
+        _check(lib().orc_tally_stats(self.h, i, avg.ctypes.data, err.ctypes.data))
+        return avg, err
+
+    def locate(self, r, u):
+        r = np.ascontiguousarray(r, np.float64)
+        u = np.ascontiguousarray(u, np.float64)
+        n = len(r)
+        cell = np.zeros(n, np.int32)
+        mat = np.zeros(n, np.int32)
+        _check(lib().orc_locate(self.h, r.ctypes.data, u.ctypes.data, n, cell.ctypes.data,
+                                mat.ctypes.data))
+        return cell, mat
+
+    def nearest_boundary(self, r, u):
+        r = np.ascontiguousarray(r, np.float64)
+        u = np.ascontiguousarray(u, np.float64)
+        n = len(r)
+        d = np.zeros(n)
+        s = np.zeros(n, np.int32)
+        b = np.zeros(n, np.int32)
+        _check(lib().orc_nearest_boundary(self.h, r.ctypes.data, u.ctypes.data, n,
+                                          d.ctypes.data, s.ctypes.data, b.ctypes.data))
+        return d, s, b
+
+    def majorant(self):
+        out = np.zeros(self.model.ngroups)
+        _check(lib().orc_majorant(self.h, out.ctypes.data))
+        return out
