@@ -1,0 +1,147 @@
+"""abeille_amd - MI355X-native multigroup transport hot path.
+
+Python handle on the C ABI (include/abx.h) exported by the in-tree gfx950
+library `abeille_amd/libabx.so`.  There is no CPU fallback: if the library
+is missing or no GPU is visible, constructing a `Problem` raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+from .model import Model  # noqa: F401
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libabx.so")
+_lib = None
+
+
+class AbxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{abi.ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libabx.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() "
+                               "(python -m abeille_amd.build) first")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.abx_last_error.restype = C.c_char_p
+        L.abx_version.restype = C.c_char_p
+        L.abx_create.argtypes = [C.POINTER(abi.ProblemDesc), C.POINTER(P)]
+        L.abx_destroy.argtypes = [P]
+        L.abx_destroy.restype = None
+        L.abx_set_engine.argtypes = [P, C.c_int]
+        L.abx_sample_sources.argtypes = [P, C.c_uint64, C.c_size_t, C.c_void_p]
+        L.abx_transport.argtypes = [P, C.c_void_p, C.c_size_t, C.c_double, C.c_int, C.c_void_p,
+                                    C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(abi.Scores),
+                                    C.POINTER(abi.Diag)]
+        L.abx_transport_trace.argtypes = [P, C.c_void_p, C.c_size_t, C.c_double, C.c_int,
+                                          C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t),
+                                          C.POINTER(abi.Scores), C.c_void_p, C.c_int,
+                                          C.c_void_p]
+        L.abx_tally_size.argtypes = [P, C.c_int, C.POINTER(C.c_size_t)]
+        L.abx_tally_fetch.argtypes = [P, C.c_int, C.c_void_p]
+        L.abx_tally_clear.argtypes = [P]
+        L.abx_tally_stats.argtypes = [P, C.c_int, C.c_void_p, C.c_void_p]
+        L.abx_run.argtypes = [P, C.c_int, C.c_int, C.c_void_p, C.POINTER(abi.RunStats)]
+        L.abx_bank_size.argtypes = [P, C.POINTER(C.c_size_t)]
+        L.abx_bank_fetch.argtypes = [P, C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != abi.ABX_OK:
+        raise AbxError(rc, lib().abx_last_error().decode())
+
+
+class Problem:
+    """One problem staged on the GPU (abx_create)."""
+
+    def __init__(self, model, engine=abi.ENGINE_EVENT):
+        self.model = model
+        self.desc = model.desc()
+        h = C.c_void_p()
+        _check(lib().abx_create(C.byref(self.desc), C.byref(h)))
+        self.h = h
+        self.set_engine(engine)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().abx_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def set_engine(self, engine):
+        _check(lib().abx_set_engine(self.h, engine))
+
+    def sample_sources(self, first, n):
+        out = np.zeros(n, abi.PARTICLE_DTYPE)
+        _check(lib().abx_sample_sources(self.h, first, n, out.ctypes.data))
+        return out
+
+    def transport(self, bank, kcol=1.0, converged=True, cap=None, trace_cap=0):
+        n = len(bank)
+        bank = np.ascontiguousarray(bank, dtype=abi.PARTICLE_DTYPE)
+        cap = cap if cap is not None else max(16, 4 * n)
+        fis = np.zeros(cap, abi.BANKED_DTYPE)
+        n_out = C.c_size_t()
+        sc = abi.Scores()
+        res = {}
+        if trace_cap:
+            tr = np.zeros(n * trace_cap, abi.TRACE_DTYPE)
+            nrec = np.zeros(n, np.int32)
+            _check(lib().abx_transport_trace(self.h, bank.ctypes.data, n, kcol,
+                                             1 if converged else 0, fis.ctypes.data, cap,
+                                             C.byref(n_out), C.byref(sc), tr.ctypes.data,
+                                             trace_cap, nrec.ctypes.data))
+            res["trace"] = tr.reshape(n, trace_cap)
+            res["n_rec"] = nrec
+        else:
+            dg = abi.Diag()
+            _check(lib().abx_transport(self.h, bank.ctypes.data, n, kcol,
+                                       1 if converged else 0, fis.ctypes.data, cap,
+                                       C.byref(n_out), C.byref(sc), C.byref(dg)))
+            res["diag"] = dg
+        res["fission"] = fis[:n_out.value].copy()
+        res["scores"] = sc
+        return res
+
+    def tally(self, i):
+        nb = C.c_size_t()
+        _check(lib().abx_tally_size(self.h, i, C.byref(nb)))
+        out = np.zeros(nb.value)
+        _check(lib().abx_tally_fetch(self.h, i, out.ctypes.data))
+        return out
+
+    def tally_clear(self):
+        _check(lib().abx_tally_clear(self.h))
+
+    def tally_stats(self, i):
+        nb = C.c_size_t()
+        _check(lib().abx_tally_size(self.h, i, C.byref(nb)))
+        avg = np.zeros(nb.value)
+        err = np.zeros(nb.value)
+        _check(lib().abx_tally_stats(self.h, i, avg.ctypes.data, err.ctypes.data))
+        return avg, err
+
+    def run(self, ngenerations, nignored=0):
+        st = abi.RunStats()
+        kh = np.zeros(ngenerations)
+        _check(lib().abx_run(self.h, ngenerations, nignored, kh.ctypes.data, C.byref(st)))
+        return st, kh
+
+    def bank(self):
+        n = C.c_size_t()
+        _check(lib().abx_bank_size(self.h, C.byref(n)))
+        out = np.zeros((n.value, 9))
+        _check(lib().abx_bank_fetch(self.h, out.ctypes.data))
+        return out

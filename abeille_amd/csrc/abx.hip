@@ -1,0 +1,644 @@
+// abx.hip - the C ABI (include/abx.h) on gfx950.
+//
+// Host side of the drop-in boundary: builds and stages the problem tables,
+// launches the transport kernels on the problem's own stream, orders the
+// fission bank like the reference, and runs the batch / generation drivers
+// with banks and tallies resident in HBM.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "event.h"
+#include "kernels.h"
+#include "tables.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+struct AbxError {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string& m) { throw AbxError{code, m}; }
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) fail(ABX_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPC(x) hip_check((x), #x)
+
+// Grow-only device buffer.
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void reserve(size_t b) {
+    if (b <= bytes) return;
+    if (p) HIPC(hipFree(p));
+    p = nullptr;
+    HIPC(hipMalloc(&p, b));
+    bytes = b;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+int64_t blocks_for(int64_t n, int bs) { return (n + bs - 1) / bs; }
+
+}  // namespace
+
+struct abx_problem {
+  HostTables T;
+  std::vector<DBuf> arrays;  // uploaded tables
+  DevProblem P{};
+  DevSources S{};
+  hipStream_t stream = nullptr;
+  int engine = ABX_ENGINE_EVENT;
+  int device = 0;
+  // tallies
+  DBuf tally_gen, tally_avg, tally_var;
+  int tally_g = 0;
+  // transport scratch
+  DBuf in, sec, stage, stage_parent, stage_daughter, n_stage, daughters, offsets,
+      block_sums, block_sums2, ordered, scores, counters, err, err_hist, trace, n_rec, wsum;
+  EventBuffers ev;
+  // driver state
+  DBuf bank_a, bank_b;  // abx_particle banks
+  size_t bank_n = 0;
+  // scalar statistics (src/tallies.cpp:268-280)
+  int gen = 0;
+  double kc_avg = 0, kc_var = 0, kt_avg = 0, kt_var = 0, lk_avg = 0, lk_var = 0, mg_avg = 0,
+         mg_var = 0;
+  abx_scores gen_scores{};
+
+  template <class V>
+  const typename V::value_type* upload(const V& v) {
+    DBuf b;
+    const size_t bytes = std::max<size_t>(1, v.size()) * sizeof(typename V::value_type);
+    b.reserve(bytes);
+    if (!v.empty()) HIPC(hipMemcpy(b.p, v.data(), v.size() * sizeof(typename V::value_type), hipMemcpyHostToDevice));
+    arrays.push_back(b);
+    return b.as<typename V::value_type>();
+  }
+
+  ~abx_problem() {
+    for (auto& b : arrays) b.release();
+    DBuf* all[] = {&tally_gen, &tally_avg, &tally_var, &in, &sec, &stage, &stage_parent,
+                   &stage_daughter, &n_stage, &daughters, &offsets, &block_sums, &block_sums2,
+                   &ordered, &scores, &counters, &err, &err_hist, &trace, &n_rec, &wsum,
+                   &bank_a, &bank_b};
+    for (DBuf* b : all) b->release();
+    ev.release();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+// Exclusive scan of n uint32 counts (device) into uint64 offsets; returns total.
+uint64_t device_scan(abx_problem* p, const uint32_t* in, uint64_t* out, int64_t n) {
+  const int bs = 1024;
+  const int64_t nb = blocks_for(n, bs);
+  p->block_sums.reserve((size_t)(nb + 1) * sizeof(uint64_t));
+  hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(bs), 0, p->stream, in, out,
+                     p->block_sums.as<uint64_t>(), n);
+  HIPC(hipGetLastError());
+  // block sums are few (n / 1024): scan them on the host
+  std::vector<uint64_t> bsum((size_t)nb);
+  HIPC(hipMemcpyAsync(bsum.data(), p->block_sums.p, nb * sizeof(uint64_t), hipMemcpyDeviceToHost, p->stream));
+  HIPC(hipStreamSynchronize(p->stream));
+  uint64_t acc = 0;
+  for (auto& v : bsum) {
+    const uint64_t t = v;
+    v = acc;
+    acc += t;
+  }
+  p->block_sums2.reserve((size_t)(nb + 1) * sizeof(uint64_t));
+  HIPC(hipMemcpyAsync(p->block_sums2.p, bsum.data(), nb * sizeof(uint64_t), hipMemcpyHostToDevice, p->stream));
+  hipLaunchKernelGGL(k_add_block_offsets, dim3((unsigned)nb), dim3(bs), 0, p->stream, out,
+                     p->block_sums2.as<uint64_t>(), n);
+  HIPC(hipGetLastError());
+  return acc;
+}
+
+bool needs_secondaries(const HostTables& T) {
+  if (T.st.tracking == ABX_CARTER_TRACKING) return true;
+  if (T.st.mode != ABX_FIXED_SOURCE) return false;
+  for (int f : T.fissile)
+    if (f) return true;
+  return false;
+}
+
+struct TransportOut {
+  int64_t n_sites = 0;
+  abx_scores scores{};
+  abx_diag diag{};
+};
+
+// One transport() over a device-resident bank; ordered fission sites are
+// left in p->ordered.  Tallies accumulate into tally_gen.
+TransportOut transport_device(abx_problem* p, const abx_particle* d_in, int64_t n, double kcol,
+                              int converged, abx_trace_rec* d_trace, int trace_cap,
+                              int32_t* d_nrec) {
+  TransportOut res;
+  const HostTables& T = p->T;
+  p->scores.reserve(8 * sizeof(double));
+  p->counters.reserve(8 * sizeof(unsigned long long));
+  p->err.reserve(sizeof(int32_t));
+  p->err_hist.reserve(sizeof(unsigned long long));
+  p->n_stage.reserve(sizeof(unsigned long long));
+  HIPC(hipMemsetAsync(p->scores.p, 0, 8 * sizeof(double), p->stream));
+  HIPC(hipMemsetAsync(p->counters.p, 0, 8 * sizeof(unsigned long long), p->stream));
+  HIPC(hipMemsetAsync(p->err.p, 0, sizeof(int32_t), p->stream));
+  HIPC(hipMemsetAsync(p->n_stage.p, 0, sizeof(unsigned long long), p->stream));
+  const bool banks = T.st.mode != ABX_FIXED_SOURCE;
+  const int64_t stage_cap = banks ? 4 * n + 4096 : 0;
+  if (banks) {
+    p->stage.reserve((size_t)stage_cap * sizeof(abx_banked));
+    p->stage_parent.reserve((size_t)stage_cap * sizeof(uint32_t));
+    p->stage_daughter.reserve((size_t)stage_cap * sizeof(uint32_t));
+    p->daughters.reserve((size_t)std::max<int64_t>(1, n) * sizeof(uint32_t));
+  }
+  const int sec_cap = needs_secondaries(T) ? 16 : 0;
+  if (sec_cap) p->sec.reserve((size_t)std::max<int64_t>(1, n) * sec_cap * sizeof(SecRec));
+
+  KCtx K{};
+  K.in = d_in;
+  K.n = n;
+  K.kcol = kcol;
+  K.converged = converged;
+  K.sec_cap = sec_cap;
+  K.sec = sec_cap ? p->sec.as<SecRec>() : nullptr;
+  K.stage = banks ? p->stage.as<abx_banked>() : nullptr;
+  K.stage_parent = banks ? p->stage_parent.as<uint32_t>() : nullptr;
+  K.stage_daughter = banks ? p->stage_daughter.as<uint32_t>() : nullptr;
+  K.n_stage = p->n_stage.as<unsigned long long>();
+  K.stage_cap = stage_cap;
+  K.daughters = banks ? p->daughters.as<uint32_t>() : nullptr;
+  K.scores = p->scores.as<double>();
+  K.counters = p->counters.as<unsigned long long>();
+  K.err = p->err.as<int32_t>();
+  K.err_hist = p->err_hist.as<unsigned long long>();
+  K.trace = d_trace;
+  K.trace_cap = trace_cap;
+  K.n_rec = d_nrec;
+  if (n > 0) {
+    if (p->engine == ABX_ENGINE_HISTORY || d_trace != nullptr) {
+      const int bs = 256;
+      hipLaunchKernelGGL(k_history, dim3((unsigned)blocks_for(n, bs)), dim3(bs), 0, p->stream,
+                         p->P, K);
+      HIPC(hipGetLastError());
+    } else {
+      HIPC(event_transport(p->P, K, p->ev, p->stream));
+    }
+  }
+  // read back scalars
+  double sc[8];
+  unsigned long long cn[8];
+  int32_t err = 0;
+  unsigned long long err_hist = 0, nst = 0;
+  HIPC(hipMemcpyAsync(sc, p->scores.p, 8 * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  HIPC(hipMemcpyAsync(cn, p->counters.p, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, p->stream));
+  HIPC(hipMemcpyAsync(&err, p->err.p, sizeof(int32_t), hipMemcpyDeviceToHost, p->stream));
+  HIPC(hipMemcpyAsync(&err_hist, p->err_hist.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, p->stream));
+  HIPC(hipMemcpyAsync(&nst, p->n_stage.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, p->stream));
+  HIPC(hipStreamSynchronize(p->stream));
+  res.scores = abx_scores{sc[0], sc[1], sc[2], sc[3], sc[4], sc[5]};
+  res.diag.histories = cn[0];
+  res.diag.collisions = cn[1];
+  res.diag.virtual_collisions = cn[2];
+  res.diag.flights = cn[3];
+  res.diag.crossings = cn[4];
+  res.diag.tally_updates = cn[5];
+  res.diag.lost = cn[6];
+  res.diag.secondaries = cn[7];
+  if (err) {
+    res.diag.error = err;
+    res.diag.error_history = err_hist;
+    fail(err, "transport failed at history " + std::to_string(err_hist));
+  }
+  if (banks) {
+    if ((int64_t)nst > stage_cap) fail(ABX_E_CAPACITY, "fission staging overflow");
+    p->offsets.reserve((size_t)std::max<int64_t>(1, n) * sizeof(uint64_t));
+    const uint64_t total = device_scan(p, p->daughters.as<uint32_t>(), p->offsets.as<uint64_t>(), n);
+    if (total != nst) fail(ABX_E_INVALID, "fission site count mismatch");
+    p->ordered.reserve((size_t)std::max<uint64_t>(1, total) * sizeof(abx_banked));
+    if (total) {
+      hipLaunchKernelGGL(k_order_bank, dim3((unsigned)blocks_for((int64_t)total, 256)), dim3(256), 0,
+                         p->stream, p->stage.as<abx_banked>(), p->stage_parent.as<uint32_t>(),
+                         p->stage_daughter.as<uint32_t>(), p->offsets.as<uint64_t>(), (int64_t)total,
+                         p->ordered.as<abx_banked>());
+      HIPC(hipGetLastError());
+    }
+    res.n_sites = (int64_t)total;
+  }
+  p->gen_scores.k_col += res.scores.k_col;
+  p->gen_scores.k_abs += res.scores.k_abs;
+  p->gen_scores.k_trk += res.scores.k_trk;
+  p->gen_scores.k_tot += res.scores.k_tot;
+  p->gen_scores.leakage += res.scores.leakage;
+  p->gen_scores.mig += res.scores.mig;
+  return res;
+}
+
+void upd(int gen, double x, double& avg, double& var) {
+  const double dgen = (double)gen;
+  const double avg_old = avg, var_old = var;
+  avg = avg_old + (x - avg_old) / dgen;
+  if (gen > 1) var = var_old + ((x - avg_old) * (x - avg_old) / dgen) - (var_old / (dgen - 1.));
+}
+
+void record_generation(abx_problem* p, const abx_scores& s, double W) {
+  p->gen++;
+  upd(p->gen, s.k_col / W, p->kc_avg, p->kc_var);
+  upd(p->gen, s.k_trk / W, p->kt_avg, p->kt_var);
+  upd(p->gen, s.leakage / W, p->lk_avg, p->lk_var);
+  upd(p->gen, s.mig / W, p->mg_avg, p->mg_var);
+  p->tally_g++;
+  const int64_t nb = p->T.tally_bins;
+  if (nb > 0) {
+    hipLaunchKernelGGL(k_record_generation, dim3((unsigned)blocks_for(nb, 256)), dim3(256), 0,
+                       p->stream, p->tally_gen.as<double>(), p->tally_avg.as<double>(),
+                       p->tally_var.as<double>(), nb, (double)p->tally_g, 0);
+    HIPC(hipGetLastError());
+  }
+}
+
+void clear_generation(abx_problem* p) {
+  p->gen_scores = abx_scores{};
+  const int64_t nb = p->T.tally_bins;
+  if (nb > 0) {
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks_for(nb, 256)), dim3(256), 0, p->stream,
+                       p->tally_gen.as<double>(), nb, 0.0);
+    HIPC(hipGetLastError());
+  }
+}
+
+void sample_sources_device(abx_problem* p, uint64_t first, int64_t n, abx_particle* d_out) {
+  HIPC(hipMemsetAsync(p->err.p, 0, sizeof(int32_t), p->stream));
+  if (n > 0) {
+    hipLaunchKernelGGL(k_sample_sources, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0, p->stream,
+                       p->P, p->S, first, n, d_out, p->err.as<int32_t>(),
+                       p->err_hist.as<unsigned long long>());
+    HIPC(hipGetLastError());
+  }
+  int32_t err = 0;
+  unsigned long long h = 0;
+  HIPC(hipMemcpyAsync(&err, p->err.p, sizeof(int32_t), hipMemcpyDeviceToHost, p->stream));
+  HIPC(hipMemcpyAsync(&h, p->err_hist.p, sizeof(h), hipMemcpyDeviceToHost, p->stream));
+  HIPC(hipStreamSynchronize(p->stream));
+  if (err) fail(err, "source sampling failed at history " + std::to_string(h));
+}
+
+void add_diag(abx_diag& a, const abx_diag& b) {
+  a.histories += b.histories;
+  a.collisions += b.collisions;
+  a.virtual_collisions += b.virtual_collisions;
+  a.flights += b.flights;
+  a.crossings += b.crossings;
+  a.tally_updates += b.tally_updates;
+  a.lost += b.lost;
+  a.secondaries += b.secondaries;
+}
+
+}  // namespace
+
+#define ABX_TRY(...)                           \
+  try {                                        \
+    __VA_ARGS__;                               \
+    return ABX_OK;                             \
+  } catch (const AbxError& e) {                \
+    g_err = e.msg;                             \
+    return e.code;                             \
+  } catch (const TableError& e) {              \
+    g_err = e.what();                          \
+    return e.code;                             \
+  } catch (const std::exception& e) {          \
+    g_err = e.what();                          \
+    return ABX_E_INVALID;                      \
+  }
+
+extern "C" {
+
+const char* abx_last_error(void) { return g_err.c_str(); }
+const char* abx_version(void) { return "abeille-amd 0.1 (gfx950)"; }
+
+int abx_create(const abx_problem_desc* desc, abx_problem** out) {
+  abx_problem* p = nullptr;
+  try {
+    if (!out) fail(ABX_E_INVALID, "null output handle");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) fail(ABX_E_NO_DEVICE, "no HIP device");
+    p = new abx_problem;
+    HIPC(hipGetDevice(&p->device));
+    build_tables(desc, p->T);
+    HIPC(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+    const HostTables& T = p->T;
+    p->tally_gen.reserve((size_t)std::max<int64_t>(1, T.tally_bins) * sizeof(double));
+    p->tally_avg.reserve((size_t)std::max<int64_t>(1, T.tally_bins) * sizeof(double));
+    p->tally_var.reserve((size_t)std::max<int64_t>(1, T.tally_bins) * sizeof(double));
+    const size_t tb = (size_t)std::max<int64_t>(1, T.tally_bins) * sizeof(double);
+    HIPC(hipMemset(p->tally_gen.p, 0, tb));
+    HIPC(hipMemset(p->tally_avg.p, 0, tb));
+    HIPC(hipMemset(p->tally_var.p, 0, tb));
+    auto map = [p](const auto& v) { return p->upload(v); };
+    p->P = dev_view(T, map, p->tally_gen.as<double>());
+    std::vector<double> w, cdf;
+    for (auto& s : T.sources) w.push_back(s.weight);
+    cdf.resize(w.size());
+    build_discrete_cdf(w.data(), (int)w.size(), cdf.data());
+    p->S.src = p->upload(T.sources);
+    p->S.cdf = p->upload(cdf);
+    p->S.n = (int32_t)w.size();
+    p->err.reserve(sizeof(int32_t));
+    p->err_hist.reserve(sizeof(unsigned long long));
+    *out = p;
+    return ABX_OK;
+  } catch (const AbxError& e) {
+    g_err = e.msg;
+    delete p;
+    return e.code;
+  } catch (const TableError& e) {
+    g_err = e.what();
+    delete p;
+    return e.code;
+  }
+}
+
+void abx_destroy(abx_problem* p) { delete p; }
+
+int abx_set_engine(abx_problem* p, int engine) {
+  ABX_TRY({
+    if (!p || (engine != ABX_ENGINE_EVENT && engine != ABX_ENGINE_HISTORY)) fail(ABX_E_INVALID, "engine");
+    p->engine = engine;
+  });
+}
+
+int abx_sample_sources(abx_problem* p, uint64_t first_history, size_t n, abx_particle* out) {
+  ABX_TRY({
+    if (!p || (!out && n)) fail(ABX_E_INVALID, "null argument");
+    p->in.reserve(std::max<size_t>(1, n) * sizeof(abx_particle));
+    sample_sources_device(p, first_history, (int64_t)n, p->in.as<abx_particle>());
+    HIPC(hipMemcpyAsync(out, p->in.p, n * sizeof(abx_particle), hipMemcpyDeviceToHost, p->stream));
+    HIPC(hipStreamSynchronize(p->stream));
+  });
+}
+
+static int transport_host(abx_problem* p, const abx_particle* bank, size_t n, double kcol,
+                          int converged, abx_banked* fission_out, size_t cap, size_t* n_out,
+                          abx_scores* scores, abx_diag* diag, abx_trace_rec* trace,
+                          int cap_per_history, int32_t* n_rec) {
+  ABX_TRY({
+    if (!p || (!bank && n)) fail(ABX_E_INVALID, "null argument");
+    if (n > 0xffffffffull) fail(ABX_E_INVALID, "bank larger than 2^32 histories");
+    p->in.reserve(std::max<size_t>(1, n) * sizeof(abx_particle));
+    if (n) HIPC(hipMemcpyAsync(p->in.p, bank, n * sizeof(abx_particle), hipMemcpyHostToDevice, p->stream));
+    abx_trace_rec* d_trace = nullptr;
+    int32_t* d_nrec = nullptr;
+    if (trace) {
+      if (cap_per_history <= 0 || !n_rec) fail(ABX_E_INVALID, "trace arguments");
+      p->trace.reserve(std::max<size_t>(1, n * (size_t)cap_per_history) * sizeof(abx_trace_rec));
+      p->n_rec.reserve(std::max<size_t>(1, n) * sizeof(int32_t));
+      d_trace = p->trace.as<abx_trace_rec>();
+      d_nrec = p->n_rec.as<int32_t>();
+    }
+    TransportOut r = transport_device(p, p->in.as<abx_particle>(), (int64_t)n, kcol, converged,
+                                      d_trace, cap_per_history, d_nrec);
+    if (n_out) *n_out = (size_t)r.n_sites;
+    if (r.n_sites > 0) {
+      if (!fission_out || (size_t)r.n_sites > cap) fail(ABX_E_CAPACITY, "fission_out capacity");
+      HIPC(hipMemcpyAsync(fission_out, p->ordered.p, (size_t)r.n_sites * sizeof(abx_banked),
+                          hipMemcpyDeviceToHost, p->stream));
+    }
+    if (trace) {
+      HIPC(hipMemcpyAsync(trace, d_trace, n * (size_t)cap_per_history * sizeof(abx_trace_rec),
+                          hipMemcpyDeviceToHost, p->stream));
+      HIPC(hipMemcpyAsync(n_rec, d_nrec, n * sizeof(int32_t), hipMemcpyDeviceToHost, p->stream));
+    }
+    HIPC(hipStreamSynchronize(p->stream));
+    if (scores) *scores = r.scores;
+    if (diag) *diag = r.diag;
+  });
+}
+
+int abx_transport(abx_problem* p, const abx_particle* bank, size_t n, double kcol, int converged,
+                  abx_banked* fission_out, size_t cap, size_t* n_out, abx_scores* scores,
+                  abx_diag* diag) {
+  return transport_host(p, bank, n, kcol, converged, fission_out, cap, n_out, scores, diag,
+                        nullptr, 0, nullptr);
+}
+
+int abx_transport_trace(abx_problem* p, const abx_particle* bank, size_t n, double kcol,
+                        int converged, abx_banked* fission_out, size_t cap, size_t* n_out,
+                        abx_scores* scores, abx_trace_rec* trace, int cap_per_history,
+                        int32_t* n_rec) {
+  if (!trace) {
+    g_err = "trace buffer is required";
+    return ABX_E_INVALID;
+  }
+  return transport_host(p, bank, n, kcol, converged, fission_out, cap, n_out, scores, nullptr,
+                        trace, cap_per_history, n_rec);
+}
+
+int abx_tally_size(abx_problem* p, int tally, size_t* n_bins) {
+  ABX_TRY({
+    if (!p || !n_bins || tally < 0 || tally >= (int)p->T.tally.size()) fail(ABX_E_INVALID, "tally index");
+    const DTally& t = p->T.tally[tally];
+    *n_bins = (size_t)t.Ne * t.Nx * t.Ny * t.Nz;
+  });
+}
+
+int abx_tally_fetch(abx_problem* p, int tally, double* gen_out) {
+  ABX_TRY({
+    size_t nb = 0;
+    if (abx_tally_size(p, tally, &nb) != ABX_OK) fail(ABX_E_INVALID, "tally index");
+    const DTally& t = p->T.tally[tally];
+    HIPC(hipMemcpyAsync(gen_out, p->tally_gen.as<double>() + t.data_off, nb * sizeof(double),
+                        hipMemcpyDeviceToHost, p->stream));
+    HIPC(hipStreamSynchronize(p->stream));
+  });
+}
+
+int abx_tally_clear(abx_problem* p) {
+  ABX_TRY({
+    if (!p) fail(ABX_E_INVALID, "null problem");
+    clear_generation(p);
+    HIPC(hipStreamSynchronize(p->stream));
+  });
+}
+
+int abx_tally_stats(abx_problem* p, int tally, double* avg, double* std_err) {
+  ABX_TRY({
+    size_t nb = 0;
+    if (abx_tally_size(p, tally, &nb) != ABX_OK) fail(ABX_E_INVALID, "tally index");
+    const DTally& t = p->T.tally[tally];
+    std::vector<double> var(nb);
+    HIPC(hipMemcpyAsync(avg, p->tally_avg.as<double>() + t.data_off, nb * sizeof(double),
+                        hipMemcpyDeviceToHost, p->stream));
+    HIPC(hipMemcpyAsync(var.data(), p->tally_var.as<double>() + t.data_off, nb * sizeof(double),
+                        hipMemcpyDeviceToHost, p->stream));
+    HIPC(hipStreamSynchronize(p->stream));
+    for (size_t i = 0; i < nb; i++) std_err[i] = std::sqrt(var[i] / (double)p->tally_g);
+  });
+}
+
+int abx_run(abx_problem* p, int ngenerations, int nignored, double* kcol_hist,
+            abx_run_stats* stats) {
+  ABX_TRY({
+    if (!p || !stats || ngenerations < 1 || nignored < 0) fail(ABX_E_INVALID, "run arguments");
+    const HostTables& T = p->T;
+    const int64_t N = T.st.nparticles;
+    const double W = (double)N;
+    *stats = abx_run_stats{};
+    // fresh statistics
+    p->gen = 0;
+    p->kc_avg = p->kc_var = p->kt_avg = p->kt_var = p->lk_avg = p->lk_var = p->mg_avg = p->mg_var = 0;
+    p->tally_g = 0;
+    const size_t tb = (size_t)std::max<int64_t>(1, T.tally_bins) * sizeof(double);
+    HIPC(hipMemsetAsync(p->tally_avg.p, 0, tb, p->stream));
+    HIPC(hipMemsetAsync(p->tally_var.p, 0, tb, p->stream));
+    clear_generation(p);
+    uint64_t hist = 0;
+    double kcol = 1.0;
+    bool converged = T.st.mode != ABX_K_EIGENVALUE || nignored == 0;
+    hipEvent_t e0, e1;
+    HIPC(hipEventCreate(&e0));
+    HIPC(hipEventCreate(&e1));
+    float ms_total = 0.f;
+    p->bank_a.reserve((size_t)N * sizeof(abx_particle));
+    if (T.st.mode == ABX_K_EIGENVALUE) {
+      sample_sources_device(p, 0, N, p->bank_a.as<abx_particle>());
+      hist = (uint64_t)N;
+      p->bank_n = (size_t)N;
+    }
+    uint64_t global_counter = hist;
+    for (int g = 1; g <= ngenerations; g++) {
+      abx_diag dg{};
+      abx_scores sc{};
+      if (T.st.mode != ABX_K_EIGENVALUE) {
+        sample_sources_device(p, hist, N, p->bank_a.as<abx_particle>());
+        hist += (uint64_t)N;
+        int64_t n = N;
+        HIPC(hipEventRecord(e0, p->stream));
+        TransportOut r = transport_device(p, p->bank_a.as<abx_particle>(), n, kcol, 1, nullptr, 0, nullptr);
+        add_diag(dg, r.diag);
+        sc = r.scores;
+        while (T.st.mode == ABX_MODIFIED_FIXED_SOURCE && r.n_sites > 0) {
+          n = r.n_sites;
+          p->bank_b.reserve((size_t)n * sizeof(abx_particle));
+          hipLaunchKernelGGL(k_rebank, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0, p->stream,
+                             p->ordered.as<abx_banked>(), n, 1.0, hist, T.st.seed, T.st.stride, 1, 0,
+                             p->bank_b.as<abx_particle>());
+          HIPC(hipGetLastError());
+          hist += (uint64_t)n;
+          std::swap(p->bank_a, p->bank_b);
+          r = transport_device(p, p->bank_a.as<abx_particle>(), n, kcol, 1, nullptr, 0, nullptr);
+          add_diag(dg, r.diag);
+          sc.k_col += r.scores.k_col;
+          sc.k_abs += r.scores.k_abs;
+          sc.k_trk += r.scores.k_trk;
+          sc.leakage += r.scores.leakage;
+          sc.mig += r.scores.mig;
+        }
+        HIPC(hipEventRecord(e1, p->stream));
+        HIPC(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPC(hipEventElapsedTime(&ms, e0, e1));
+        ms_total += ms;
+        if (kcol_hist) kcol_hist[g - 1] = sc.k_col / W;
+        record_generation(p, sc, W);
+      } else {
+        HIPC(hipEventRecord(e0, p->stream));
+        TransportOut r = transport_device(p, p->bank_a.as<abx_particle>(), (int64_t)p->bank_n, kcol,
+                                          converged ? 1 : 0, nullptr, 0, nullptr);
+        HIPC(hipEventRecord(e1, p->stream));
+        HIPC(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPC(hipEventElapsedTime(&ms, e0, e1));
+        ms_total += ms;
+        add_diag(dg, r.diag);
+        sc = r.scores;
+        if (r.n_sites == 0) fail(ABX_E_INVALID, "No fission neutrons were produced.");
+        const double kc = sc.k_col / W;
+        if (kcol_hist) kcol_hist[g - 1] = kc;
+        // normalize_weights: W = W_pos - W_neg, w *= nparticles / W
+        p->wsum.reserve(2 * sizeof(double));
+        HIPC(hipMemsetAsync(p->wsum.p, 0, 2 * sizeof(double), p->stream));
+        hipLaunchKernelGGL(k_weight_sums, dim3((unsigned)blocks_for(r.n_sites, 256)), dim3(256), 0,
+                           p->stream, p->ordered.as<abx_banked>(), r.n_sites, p->wsum.as<double>());
+        HIPC(hipGetLastError());
+        double ws[2];
+        HIPC(hipMemcpyAsync(ws, p->wsum.p, 2 * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+        HIPC(hipStreamSynchronize(p->stream));
+        const double w_per_part = (double)T.st.nparticles / (ws[0] - ws[1]);
+        if (converged) record_generation(p, sc, W);
+        p->bank_b.reserve((size_t)r.n_sites * sizeof(abx_particle));
+        hipLaunchKernelGGL(k_rebank, dim3((unsigned)blocks_for(r.n_sites, 256)), dim3(256), 0,
+                           p->stream, p->ordered.as<abx_banked>(), r.n_sites, w_per_part,
+                           global_counter, T.st.seed, T.st.stride, 0, 1, p->bank_b.as<abx_particle>());
+        HIPC(hipGetLastError());
+        std::swap(p->bank_a, p->bank_b);
+        p->bank_n = (size_t)r.n_sites;
+        global_counter += (uint64_t)r.n_sites;
+        kcol = kc;
+        if (g == nignored) converged = true;
+      }
+      clear_generation(p);
+      add_diag(stats->diag, dg);
+    }
+    HIPC(hipStreamSynchronize(p->stream));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    stats->kcol_avg = p->kc_avg;
+    stats->kcol_err = std::sqrt(p->kc_var / (double)p->gen);
+    stats->ktrk_avg = p->kt_avg;
+    stats->ktrk_err = std::sqrt(p->kt_var / (double)p->gen);
+    stats->leak_avg = p->lk_avg;
+    stats->leak_err = std::sqrt(p->lk_var / (double)p->gen);
+    stats->mig_avg = p->mg_avg;
+    stats->mig_err = std::sqrt(p->mg_var / (double)p->gen);
+    stats->active_generations = p->gen;
+    stats->transport_seconds = ms_total * 1e-3;
+  });
+}
+
+int abx_bank_size(abx_problem* p, size_t* n) {
+  ABX_TRY({
+    if (!p || !n) fail(ABX_E_INVALID, "null argument");
+    *n = p->bank_n;
+  });
+}
+
+int abx_bank_fetch(abx_problem* p, double* out9) {
+  ABX_TRY({
+    if (!p || !out9) fail(ABX_E_INVALID, "null argument");
+    std::vector<abx_particle> b(p->bank_n);
+    if (p->bank_n)
+      HIPC(hipMemcpyAsync(b.data(), p->bank_a.p, p->bank_n * sizeof(abx_particle), hipMemcpyDeviceToHost,
+                          p->stream));
+    HIPC(hipStreamSynchronize(p->stream));
+    for (size_t i = 0; i < b.size(); i++) {
+      double* o = out9 + 9 * i;
+      for (int k = 0; k < 3; k++) {
+        o[k] = b[i].r[k];
+        o[3 + k] = b[i].u[k];
+      }
+      o[6] = b[i].E;
+      o[7] = b[i].wgt;
+      o[8] = b[i].wgt2;
+    }
+  });
+}
+
+}  // extern "C"

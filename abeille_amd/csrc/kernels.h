@@ -1,0 +1,301 @@
+// kernels.h - GPU sinks, reductions and the history-based transport kernel.
+//
+// The history kernel runs one lane per history (the reference's
+// `omp for schedule(dynamic)` body, src/delta_tracker.cpp:83-86) and is the
+// parity vehicle; the event-based kernels (event.h) share physics.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/abx.h"
+#include "physics.h"
+#include "source.h"
+
+// Per-call device context of a transport launch.
+struct KCtx {
+  const abx_particle* in;
+  int64_t n;
+  double kcol;
+  int32_t converged;
+  int32_t sec_cap;        // secondary-stack entries per slot (0 = none possible)
+  SecRec* sec;            // [n][sec_cap]
+  // fission staging (k-eigenvalue / modified fixed source)
+  abx_banked* stage;
+  uint32_t* stage_parent;
+  uint32_t* stage_daughter;
+  unsigned long long* n_stage;
+  int64_t stage_cap;
+  uint32_t* daughters;    // [n] sites per parent
+  double* scores;         // [6] k_col k_abs k_trk k_tot leak mig
+  unsigned long long* counters;  // [8] histories collisions virtual flights crossings tally lost secondaries
+  int32_t* err;
+  unsigned long long* err_hist;
+  abx_trace_rec* trace;
+  int32_t trace_cap;
+  int32_t* n_rec;
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Sums six scores and eight counters over the wave, lane 0 adds them in.
+__device__ __forceinline__ void flush_scores(const KCtx& K, const Scores6& sc,
+                                             const unsigned long long c[8]) {
+  const double s[6] = {sc.k_col, sc.k_abs, sc.k_trk, sc.k_tot, sc.leak, sc.mig};
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    const double t = wave_sum(s[i]);
+    if (lane == 0 && t != 0.0) unsafeAtomicAdd(K.scores + i, t);
+  }
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const unsigned long long t = wave_sum_u64(c[i]);
+    if (lane == 0 && t) atomicAdd(K.counters + i, t);
+  }
+}
+
+__device__ __forceinline__ void raise_error(const KCtx& K, int code, uint64_t h) {
+  if (atomicCAS(K.err, 0, code) == 0) *K.err_hist = (unsigned long long)h;
+}
+
+struct GpuSink {
+  const KCtx* K;
+  const DevProblem* P;
+  int64_t slot;
+  int32_t nsec;
+  int32_t n_rec;
+  unsigned long long c[8];
+
+  __device__ void tally_add(double* a, double v) {
+    unsafeAtomicAdd(a, v);
+    c[5]++;
+  }
+  __device__ bool sec_push(const SecRec& q) {
+    if (nsec >= K->sec_cap) return false;
+    K->sec[slot * K->sec_cap + nsec] = q;
+    nsec++;
+    return true;
+  }
+  __device__ bool sec_pop(SecRec& q) {
+    if (nsec == 0) return false;
+    nsec--;
+    q = K->sec[slot * K->sec_cap + nsec];
+    return true;
+  }
+  __device__ void bank_fission(const Part& p, const V3& u, double E, double w) {
+    const unsigned long long i = atomicAdd(K->n_stage, 1ull);
+    if ((int64_t)i >= K->stage_cap) return;  // counted; host reports capacity
+    abx_banked b;
+    b.r[0] = p.r.x;
+    b.r[1] = p.r.y;
+    b.r[2] = p.r.z;
+    b.u[0] = u.x;
+    b.u[1] = u.y;
+    b.u[2] = u.z;
+    b.E = E;
+    b.wgt = w;
+    b.wgt2 = 0.;
+    b.parent_history_id = p.hid;
+    b.parent_daughter_id = p.daughters;
+    b.family_id = p.fam;
+    K->stage[i] = b;
+    K->stage_parent[i] = (uint32_t)slot;
+    K->stage_daughter[i] = p.daughters;
+  }
+  __device__ void trace(const Part& p, const Trk& t, int ev) {
+    if (!K->trace) return;
+    if (n_rec < K->trace_cap) {
+      abx_trace_rec r;
+      r.r[0] = p.r.x;
+      r.r[1] = p.r.y;
+      r.r[2] = p.r.z;
+      r.u[0] = p.u.x;
+      r.u[1] = p.u.y;
+      r.u[2] = p.u.z;
+      r.wgt = p.w;
+      r.history_id = p.hid;
+      r.secondary_id = p.sec_id;
+      r.event = ev;
+      r.cell = t.cell;
+      r.material = t.cell >= 0 ? P->cell[t.cell].fill : -1;
+      r.group = p.g;
+      r.rng_calls = (int32_t)p.calls;
+      K->trace[slot * K->trace_cap + n_rec] = r;
+    }
+    n_rec++;
+  }
+  __device__ void error(int code, uint64_t h) { raise_error(*K, code, h); }
+  __device__ void count_flight() { c[3]++; }
+  __device__ void count_crossing() { c[4]++; }
+  __device__ void count_collision() { c[1]++; }
+  __device__ void count_virtual() { c[2]++; }
+  __device__ void count_lost() { c[6]++; }
+  __device__ void count_secondary() { c[7]++; }
+};
+
+__device__ __forceinline__ void load_part(const DevProblem& P, const abx_particle& a, Part& p) {
+  p.r = V3{a.r[0], a.r[1], a.r[2]};
+  p.u = V3{a.u[0], a.u[1], a.u[2]};
+  p.rb = p.r;
+  p.E = a.E;
+  p.g = energy_index(P, a.E);
+  p.w = a.wgt;
+  p.w2 = a.wgt2;
+  p.rng = a.rng_state;
+  p.hid = a.history_id;
+  p.fam = a.family_id;
+  p.calls = 0;
+  p.daughters = 0;
+  p.sec_id = 0;
+  p.alive = 1;
+}
+
+__global__ void __launch_bounds__(256) k_history(DevProblem P, KCtx K) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  Scores6 sc{0, 0, 0, 0, 0, 0};
+  GpuSink s;
+  s.K = &K;
+  s.P = &P;
+  s.slot = i;
+  s.nsec = 0;
+  s.n_rec = 0;
+  for (int k = 0; k < 8; k++) s.c[k] = 0;
+  if (i < K.n) {
+    Part p;
+    load_part(P, K.in[i], p);
+    run_history(P, s, p, K.kcol, K.converged != 0, sc);
+    s.c[0] = 1;
+    if (K.daughters) K.daughters[i] = p.daughters;
+    if (K.n_rec) K.n_rec[i] = s.n_rec;
+  }
+  flush_scores(K, sc, s.c);
+}
+
+__global__ void k_sample_sources(DevProblem P, DevSources S, uint64_t first, int64_t n,
+                                 abx_particle* out, int32_t* err,
+                                 unsigned long long* err_hist) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  abx_particle a;
+  const int rc = sample_source(P, S, first + (uint64_t)i, a);
+  if (rc) {
+    if (atomicCAS(err, 0, rc) == 0) *err_hist = first + (uint64_t)i;
+    return;
+  }
+  out[i] = a;
+}
+
+// ---------------------------------------------------------------- scans
+// Exclusive scan of uint32 counts into uint64 offsets, three phases.
+__global__ void k_scan_blocks(const uint32_t* in, uint64_t* out, uint64_t* block_sums,
+                              int64_t n) {
+  __shared__ uint64_t wsum[16];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  uint64_t v = i < n ? in[i] : 0;
+  // inclusive wave scan
+  uint64_t x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint64_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[wid] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t acc = 0;
+    const int nw = blockDim.x >> 6;
+    for (int w = 0; w < nw; w++) {
+      const uint64_t t = wsum[w];
+      wsum[w] = acc;
+      acc += t;
+    }
+    block_sums[blockIdx.x] = acc;
+  }
+  __syncthreads();
+  if (i < n) out[i] = x - v + wsum[wid];
+}
+
+__global__ void k_add_block_offsets(uint64_t* out, const uint64_t* block_off, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] += block_off[blockIdx.x];
+}
+
+__global__ void k_u64_to_u32_dummy() {}
+
+// scatter staged fission sites to parent order
+__global__ void k_order_bank(const abx_banked* stage, const uint32_t* parent,
+                             const uint32_t* daughter, const uint64_t* offsets, int64_t m,
+                             abx_banked* out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  out[offsets[parent[k]] + daughter[k]] = stage[k];
+}
+
+// MeshTally::record_generation (src/mesh_tally.cpp:121-150), then zero the
+// generation array (clear_generation).
+__global__ void k_record_generation(double* gen, double* avg, double* var, int64_t n,
+                                    double dg, int clear) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double old_avg = avg[i];
+  const double val = gen[i] * 1.0;
+  const double a = old_avg + (val - old_avg) / dg;
+  avg[i] = a;
+  double v = var[i];
+  v = v + (((val - old_avg) * (val - a) - (v)) / dg);
+  var[i] = v;
+  if (clear) gen[i] = 0.0;
+}
+
+__global__ void k_fill(double* a, int64_t n, double v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+// PowerIterator::normalize_weights partial sums (positive / negative weight)
+__global__ void k_weight_sums(const abx_banked* b, int64_t n, double* sums) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double wp = 0., wn = 0.;
+  if (i < n) {
+    const double w = b[i].wgt;
+    if (w > 0.)
+      wp = w;
+    else
+      wn = -w;
+  }
+  wp = wave_sum(wp);
+  wn = wave_sum(wn);
+  if ((threadIdx.x & 63) == 0) {
+    if (wp != 0.) unsafeAtomicAdd(sums, wp);
+    if (wn != 0.) unsafeAtomicAdd(sums + 1, wn);
+  }
+}
+
+// Re-bank (src/power_iterator.cpp:394-404, src/modified_fixed_source.cpp:88-93):
+// Particle(r, u, E, w * w_per_part, history_counter++) with a fresh stream.
+__global__ void k_rebank(const abx_banked* b, int64_t n, double w_per_part, uint64_t first_hist,
+                         uint64_t seed, uint64_t stride, int keep_wgt2, int keep_family,
+                         abx_particle* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const abx_banked& s = b[i];
+  abx_particle p;
+  for (int k = 0; k < 3; k++) {
+    p.r[k] = s.r[k];
+    p.u[k] = s.u[k];
+  }
+  p.E = s.E;
+  p.wgt = s.wgt * w_per_part;
+  p.wgt2 = keep_wgt2 ? s.wgt2 : 0.;
+  p.history_id = first_hist + (uint64_t)i;
+  p.family_id = keep_family ? s.family_id : 0;
+  p.rng_state = pcg_history_state(seed, stride, p.history_id);
+  out[i] = p;
+}

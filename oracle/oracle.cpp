@@ -2110,10 +2110,10 @@ double orc_log(int portable, double x) { return portable ? dm_log(x) : std::log(
 double orc_sin(int portable, double x) { return portable ? dm_sin(x) : std::sin(x); }
 double orc_cos(int portable, double x) { return portable ? dm_cos(x) : std::cos(x); }
 
-#define ORC_TRY(body)                     \
+#define ORC_TRY(...)                      \
   try {                                   \
     g_portable = p->P.portable;           \
-    body;                                 \
+    __VA_ARGS__;                          \
     return ABX_OK;                        \
   } catch (const Fatal& f) {              \
     g_last = f.msg;                       \
@@ -2263,3 +2263,35 @@ int orc_majorant(orc_problem* p, double* maj) {
 }
 
 }  // extern "C"
+
+// ---- RNG probes (golden-vector tests) ---------------------------------------
+extern "C" {
+void orc_rng_stream(uint64_t seed, uint64_t stride, uint64_t h, int n, uint32_t* out) {
+  Pcg32 g;
+  g.seed(seed);
+  g.advance(stride * h);
+  for (int i = 0; i < n; i++) out[i] = g();
+}
+void orc_rng_rand(uint64_t seed, uint64_t stride, uint64_t h, int n, double* out) {
+  Pcg32 g;
+  g.seed(seed);
+  g.advance(stride * h);
+  for (int i = 0; i < n; i++) out[i] = RNG::rand(g);
+}
+void orc_rng_exponential(int portable, uint64_t seed, uint64_t stride, uint64_t h, int n,
+                         double lambda, double* out) {
+  g_portable = portable != 0;
+  Pcg32 g;
+  g.seed(seed);
+  g.advance(stride * h);
+  for (int i = 0; i < n; i++) out[i] = RNG::exponential(g, lambda);
+}
+uint32_t orc_rng_discrete(uint64_t seed, uint64_t stride, uint64_t h, const double* w, int nw,
+                          int n, int32_t* out) {
+  Pcg32 g;
+  g.seed(seed);
+  g.advance(stride * h);
+  for (int i = 0; i < n; i++) out[i] = RNG::discrete(g, w, w + nw);
+  return g();
+}
+}

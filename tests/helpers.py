@@ -1,0 +1,32 @@
+"""Shared comparison helpers for the parity tests."""
+import numpy as np
+
+
+def first_trace_diff(a, b, na, nb, cap):
+    """Index of the first history whose trace differs, with a description."""
+    if not np.array_equal(na, nb):
+        i = int(np.nonzero(na != nb)[0][0])
+        return i, f"history {i}: {na[i]} vs {nb[i]} records"
+    for i in range(len(na)):
+        k = min(int(na[i]), cap)
+        if a[i][:k].tobytes() != b[i][:k].tobytes():
+            j = int(np.nonzero([x.tobytes() != y.tobytes() for x, y in zip(a[i][:k], b[i][:k])])[0][0])
+            return i, f"history {i} record {j}:\n {a[i][j]}\n {b[i][j]}"
+    return None, ""
+
+
+def assert_traces_equal(ra, rb, cap):
+    i, msg = first_trace_diff(ra["trace"], rb["trace"], ra["n_rec"], rb["n_rec"], cap)
+    assert i is None, msg
+
+
+def assert_within_sigma(avg_a, err_a, avg_b, err_b, nsig=3.0, min_frac=0.99):
+    """Fraction of bins whose difference is within nsig combined standard
+    errors must be at least min_frac (bins with zero error on both sides
+    must agree exactly)."""
+    comb = np.sqrt(err_a ** 2 + err_b ** 2)
+    d = np.abs(avg_a - avg_b)
+    ok = (d <= nsig * comb) | ((comb == 0) & (d == 0))
+    frac = ok.mean()
+    assert frac >= min_frac, f"only {frac:.4f} of bins within {nsig} sigma"
+    return frac

@@ -175,3 +175,44 @@ int hd_tally_fetch(hd_problem* h, double* out) {
   return 0;
 }
 }
+
+// ---- RNG / math probes of the device headers (rng.h, detmath.h) ----------
+extern "C" {
+void hd_rng_stream(uint64_t seed, uint64_t stride, uint64_t h, int n, uint32_t* out) {
+  uint64_t s = pcg_history_state(seed, stride, h);
+  for (int i = 0; i < n; i++) out[i] = pcg_next(s);
+}
+void hd_rng_rand(uint64_t seed, uint64_t stride, uint64_t h, int n, double* out) {
+  uint64_t s = pcg_history_state(seed, stride, h);
+  for (int i = 0; i < n; i++) out[i] = rng_rand(s);
+}
+void hd_rng_exponential(uint64_t seed, uint64_t stride, uint64_t h, int n, double lambda,
+                        double* out) {
+  uint64_t s = pcg_history_state(seed, stride, h);
+  for (int i = 0; i < n; i++) out[i] = rng_exponential(s, lambda);
+}
+// n draws of discrete(w) with the cdf built once; returns the next raw output
+uint32_t hd_rng_discrete(uint64_t seed, uint64_t stride, uint64_t h, const double* w, int nw,
+                         int n, int32_t* out) {
+  std::vector<double> cdf(nw);
+  build_discrete_cdf(w, nw, cdf.data());
+  uint64_t s = pcg_history_state(seed, stride, h);
+  for (int i = 0; i < n; i++) out[i] = rng_discrete(s, cdf.data(), nw);
+  return pcg_next(s);
+}
+uint32_t hd_pcg_seeded_stream(uint64_t state_seed, uint64_t stream, int n, uint32_t* out) {
+  // pcg32(state, stream): inc = (stream << 1) | 1, state = bump(seed + inc)
+  const uint64_t inc = (stream << 1) | 1u;
+  uint64_t s = (state_seed + inc) * ABX_PCG_MULT + inc;
+  for (int i = 0; i < n; i++) {
+    const uint64_t old = s;
+    s = old * ABX_PCG_MULT + inc;
+    const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    const uint32_t rot = (uint32_t)(old >> 59u);
+    out[i] = (xs >> rot) | (xs << ((0u - rot) & 31u));
+  }
+  return 0;
+}
+double hd_log(double x) { return dm_log(x); }
+void hd_sincos(double x, double* s, double* c) { dm_sincos(x, s, c); }
+}
