@@ -51,6 +51,8 @@ def lib():
         L.abx_tally_clear.argtypes = [P]
         L.abx_tally_stats.argtypes = [P, C.c_int, C.c_void_p, C.c_void_p]
         L.abx_run.argtypes = [P, C.c_int, C.c_int, C.c_void_p, C.POINTER(abi.RunStats)]
+        L.abx_run_continue.argtypes = [P, C.c_int, C.c_int, C.c_void_p,
+                                       C.POINTER(abi.RunStats)]
         L.abx_bank_size.argtypes = [P, C.POINTER(C.c_size_t)]
         L.abx_bank_fetch.argtypes = [P, C.c_void_p]
         _lib = L
@@ -137,6 +139,13 @@ class Problem:
         st = abi.RunStats()
         kh = np.zeros(ngenerations)
         _check(lib().abx_run(self.h, ngenerations, nignored, kh.ctypes.data, C.byref(st)))
+        return st, kh
+
+    def run_continue(self, ngenerations, nignored=0):
+        st = abi.RunStats()
+        kh = np.zeros(ngenerations)
+        _check(lib().abx_run_continue(self.h, ngenerations, nignored, kh.ctypes.data,
+                                      C.byref(st)))
         return st, kh
 
     def bank(self):

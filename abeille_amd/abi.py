@@ -144,7 +144,9 @@ class RunStats(C.Structure):
                 ("leak_avg", C.c_double), ("leak_err", C.c_double),
                 ("mig_avg", C.c_double), ("mig_err", C.c_double),
                 ("active_generations", C.c_int32), ("pad_", C.c_int32),
-                ("transport_seconds", C.c_double), ("diag", Diag)]
+                ("transport_seconds", C.c_double), ("kernel_seconds", C.c_double),
+                ("kernel_launches", C.c_int64), ("kernel_items", C.c_int64),
+                ("histories", C.c_int64), ("diag", Diag)]
 
 
 class TraceRec(C.Structure):

@@ -76,6 +76,15 @@ struct abx_problem {
   // driver state
   DBuf bank_a, bank_b;  // abx_particle banks
   size_t bank_n = 0;
+  bool run_started = false;
+  uint64_t run_hist = 0;  // histories_counter
+  double run_kcol = 1.0;  // Tallies::kcol() of the previous generation
+  // per-kernel timing of the transport kernels (HIP events on p->stream)
+  bool timing = false;
+  double kernel_ms = 0.0;
+  int64_t kernel_launches = 0;
+  int64_t kernel_items = 0;
+  hipEvent_t ke0 = nullptr, ke1 = nullptr;
   // scalar statistics (src/tallies.cpp:268-280)
   int gen = 0;
   double kc_avg = 0, kc_var = 0, kt_avg = 0, kt_var = 0, lk_avg = 0, lk_var = 0, mg_avg = 0,
@@ -100,6 +109,8 @@ struct abx_problem {
                    &bank_a, &bank_b};
     for (DBuf* b : all) b->release();
     ev.release();
+    if (ke0) (void)hipEventDestroy(ke0);
+    if (ke1) (void)hipEventDestroy(ke1);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -194,13 +205,33 @@ TransportOut transport_device(abx_problem* p, const abx_particle* d_in, int64_t 
   K.trace_cap = trace_cap;
   K.n_rec = d_nrec;
   if (n > 0) {
-    if (p->engine == ABX_ENGINE_HISTORY || d_trace != nullptr) {
+    if (p->timing && !p->ke0) {
+      HIPC(hipEventCreate(&p->ke0));
+      HIPC(hipEventCreate(&p->ke1));
+    }
+    if (p->engine == ABX_ENGINE_HISTORY) {
       const int bs = 256;
+      if (p->timing) HIPC(hipEventRecord(p->ke0, p->stream));
       hipLaunchKernelGGL(k_history, dim3((unsigned)blocks_for(n, bs)), dim3(bs), 0, p->stream,
                          p->P, K);
       HIPC(hipGetLastError());
+      if (p->timing) {
+        HIPC(hipEventRecord(p->ke1, p->stream));
+        HIPC(hipEventSynchronize(p->ke1));
+        float ms = 0.f;
+        HIPC(hipEventElapsedTime(&ms, p->ke0, p->ke1));
+        p->kernel_ms += ms;
+        p->kernel_launches++;
+        p->kernel_items += n;  // every event of the bank: counted after the launch
+      }
     } else {
-      HIPC(event_transport(p->P, K, p->ev, p->stream));
+      double ms = 0.0;
+      int64_t launches = 0;
+      HIPC(event_transport(p->P, K, p->ev, p->stream, CursorDepth{T.max_pads, T.max_levels},
+                           p->timing, &ms, &launches));
+      p->kernel_ms += ms;
+      p->kernel_launches += launches;
+      p->kernel_items += p->ev.flight_items;
     }
   }
   // read back scalars
@@ -493,123 +524,161 @@ int abx_tally_stats(abx_problem* p, int tally, double* avg, double* std_err) {
   });
 }
 
+static void run_reset(abx_problem* p) {
+  const HostTables& T = p->T;
+  p->gen = 0;
+  p->kc_avg = p->kc_var = p->kt_avg = p->kt_var = p->lk_avg = p->lk_var = p->mg_avg = p->mg_var = 0;
+  p->tally_g = 0;
+  const size_t tb = (size_t)std::max<int64_t>(1, T.tally_bins) * sizeof(double);
+  HIPC(hipMemsetAsync(p->tally_avg.p, 0, tb, p->stream));
+  HIPC(hipMemsetAsync(p->tally_var.p, 0, tb, p->stream));
+  clear_generation(p);
+  p->run_started = false;
+  p->run_hist = 0;
+  p->run_kcol = 1.0;
+  p->bank_n = 0;
+}
+
+// FixedSource::run / ModifiedFixedSource::run / PowerIterator::run
+// (src/fixed_source.cpp:81-173, src/modified_fixed_source.cpp:59-143,
+// src/power_iterator.cpp:305-429) with every bank resident in HBM.
+static void run_generations(abx_problem* p, int ngenerations, int nignored, double* kcol_hist,
+                            abx_run_stats* stats) {
+  const HostTables& T = p->T;
+  const int64_t N = T.st.nparticles;
+  const double W = (double)N;
+  *stats = abx_run_stats{};
+  bool converged = T.st.mode != ABX_K_EIGENVALUE || nignored == 0;
+  p->timing = true;
+  p->kernel_ms = 0.0;
+  p->kernel_launches = 0;
+  p->kernel_items = 0;
+  hipEvent_t e0, e1;
+  HIPC(hipEventCreate(&e0));
+  HIPC(hipEventCreate(&e1));
+  float ms_total = 0.f;
+  int64_t histories = 0;
+  p->bank_a.reserve((size_t)N * sizeof(abx_particle));
+  if (T.st.mode == ABX_K_EIGENVALUE && !p->run_started) {
+    sample_sources_device(p, 0, N, p->bank_a.as<abx_particle>());
+    p->run_hist = (uint64_t)N;
+    p->bank_n = (size_t)N;
+  }
+  p->run_started = true;
+  for (int g = 1; g <= ngenerations; g++) {
+    abx_diag dg{};
+    abx_scores sc{};
+    if (T.st.mode != ABX_K_EIGENVALUE) {
+      sample_sources_device(p, p->run_hist, N, p->bank_a.as<abx_particle>());
+      p->run_hist += (uint64_t)N;
+      int64_t n = N;
+      HIPC(hipEventRecord(e0, p->stream));
+      TransportOut r = transport_device(p, p->bank_a.as<abx_particle>(), n, p->run_kcol, 1, nullptr, 0, nullptr);
+      histories += n;
+      add_diag(dg, r.diag);
+      sc = r.scores;
+      while (T.st.mode == ABX_MODIFIED_FIXED_SOURCE && r.n_sites > 0) {
+        n = r.n_sites;
+        p->bank_b.reserve((size_t)n * sizeof(abx_particle));
+        hipLaunchKernelGGL(k_rebank, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0, p->stream,
+                           p->ordered.as<abx_banked>(), n, 1.0, p->run_hist, T.st.seed, T.st.stride, 1, 0,
+                           p->bank_b.as<abx_particle>());
+        HIPC(hipGetLastError());
+        p->run_hist += (uint64_t)n;
+        std::swap(p->bank_a, p->bank_b);
+        r = transport_device(p, p->bank_a.as<abx_particle>(), n, p->run_kcol, 1, nullptr, 0, nullptr);
+        histories += n;
+        add_diag(dg, r.diag);
+        sc.k_col += r.scores.k_col;
+        sc.k_abs += r.scores.k_abs;
+        sc.k_trk += r.scores.k_trk;
+        sc.leakage += r.scores.leakage;
+        sc.mig += r.scores.mig;
+      }
+      HIPC(hipEventRecord(e1, p->stream));
+      HIPC(hipEventSynchronize(e1));
+      float ms = 0;
+      HIPC(hipEventElapsedTime(&ms, e0, e1));
+      ms_total += ms;
+      if (kcol_hist) kcol_hist[g - 1] = sc.k_col / W;
+      record_generation(p, sc, W);
+    } else {
+      HIPC(hipEventRecord(e0, p->stream));
+      TransportOut r = transport_device(p, p->bank_a.as<abx_particle>(), (int64_t)p->bank_n, p->run_kcol,
+                                        converged ? 1 : 0, nullptr, 0, nullptr);
+      HIPC(hipEventRecord(e1, p->stream));
+      HIPC(hipEventSynchronize(e1));
+      float ms = 0;
+      HIPC(hipEventElapsedTime(&ms, e0, e1));
+      ms_total += ms;
+      histories += (int64_t)p->bank_n;
+      add_diag(dg, r.diag);
+      sc = r.scores;
+      if (r.n_sites == 0) fail(ABX_E_INVALID, "No fission neutrons were produced.");
+      const double kc = sc.k_col / W;
+      if (kcol_hist) kcol_hist[g - 1] = kc;
+      // normalize_weights: W = W_pos - W_neg, w *= nparticles / W
+      p->wsum.reserve(2 * sizeof(double));
+      HIPC(hipMemsetAsync(p->wsum.p, 0, 2 * sizeof(double), p->stream));
+      hipLaunchKernelGGL(k_weight_sums, dim3((unsigned)blocks_for(r.n_sites, 256)), dim3(256), 0,
+                         p->stream, p->ordered.as<abx_banked>(), r.n_sites, p->wsum.as<double>());
+      HIPC(hipGetLastError());
+      double ws[2];
+      HIPC(hipMemcpyAsync(ws, p->wsum.p, 2 * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+      HIPC(hipStreamSynchronize(p->stream));
+      const double w_per_part = (double)T.st.nparticles / (ws[0] - ws[1]);
+      if (converged) record_generation(p, sc, W);
+      p->bank_b.reserve((size_t)r.n_sites * sizeof(abx_particle));
+      hipLaunchKernelGGL(k_rebank, dim3((unsigned)blocks_for(r.n_sites, 256)), dim3(256), 0,
+                         p->stream, p->ordered.as<abx_banked>(), r.n_sites, w_per_part,
+                         p->run_hist, T.st.seed, T.st.stride, 0, 1, p->bank_b.as<abx_particle>());
+      HIPC(hipGetLastError());
+      std::swap(p->bank_a, p->bank_b);
+      p->bank_n = (size_t)r.n_sites;
+      p->run_hist += (uint64_t)r.n_sites;
+      p->run_kcol = kc;
+      if (g == nignored) converged = true;
+    }
+    clear_generation(p);
+    add_diag(stats->diag, dg);
+  }
+  HIPC(hipStreamSynchronize(p->stream));
+  p->timing = false;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  const double dg = (double)std::max(1, p->gen);
+  stats->kcol_avg = p->kc_avg;
+  stats->kcol_err = std::sqrt(p->kc_var / dg);
+  stats->ktrk_avg = p->kt_avg;
+  stats->ktrk_err = std::sqrt(p->kt_var / dg);
+  stats->leak_avg = p->lk_avg;
+  stats->leak_err = std::sqrt(p->lk_var / dg);
+  stats->mig_avg = p->mg_avg;
+  stats->mig_err = std::sqrt(p->mg_var / dg);
+  stats->active_generations = p->gen;
+  stats->transport_seconds = ms_total * 1e-3;
+  stats->kernel_seconds = p->kernel_ms * 1e-3;
+  stats->kernel_launches = p->kernel_launches;
+  stats->kernel_items = p->engine == ABX_ENGINE_HISTORY
+                            ? (int64_t)(stats->diag.flights + stats->diag.collisions)
+                            : p->kernel_items;
+  stats->histories = histories;
+}
+
 int abx_run(abx_problem* p, int ngenerations, int nignored, double* kcol_hist,
             abx_run_stats* stats) {
   ABX_TRY({
     if (!p || !stats || ngenerations < 1 || nignored < 0) fail(ABX_E_INVALID, "run arguments");
-    const HostTables& T = p->T;
-    const int64_t N = T.st.nparticles;
-    const double W = (double)N;
-    *stats = abx_run_stats{};
-    // fresh statistics
-    p->gen = 0;
-    p->kc_avg = p->kc_var = p->kt_avg = p->kt_var = p->lk_avg = p->lk_var = p->mg_avg = p->mg_var = 0;
-    p->tally_g = 0;
-    const size_t tb = (size_t)std::max<int64_t>(1, T.tally_bins) * sizeof(double);
-    HIPC(hipMemsetAsync(p->tally_avg.p, 0, tb, p->stream));
-    HIPC(hipMemsetAsync(p->tally_var.p, 0, tb, p->stream));
-    clear_generation(p);
-    uint64_t hist = 0;
-    double kcol = 1.0;
-    bool converged = T.st.mode != ABX_K_EIGENVALUE || nignored == 0;
-    hipEvent_t e0, e1;
-    HIPC(hipEventCreate(&e0));
-    HIPC(hipEventCreate(&e1));
-    float ms_total = 0.f;
-    p->bank_a.reserve((size_t)N * sizeof(abx_particle));
-    if (T.st.mode == ABX_K_EIGENVALUE) {
-      sample_sources_device(p, 0, N, p->bank_a.as<abx_particle>());
-      hist = (uint64_t)N;
-      p->bank_n = (size_t)N;
-    }
-    uint64_t global_counter = hist;
-    for (int g = 1; g <= ngenerations; g++) {
-      abx_diag dg{};
-      abx_scores sc{};
-      if (T.st.mode != ABX_K_EIGENVALUE) {
-        sample_sources_device(p, hist, N, p->bank_a.as<abx_particle>());
-        hist += (uint64_t)N;
-        int64_t n = N;
-        HIPC(hipEventRecord(e0, p->stream));
-        TransportOut r = transport_device(p, p->bank_a.as<abx_particle>(), n, kcol, 1, nullptr, 0, nullptr);
-        add_diag(dg, r.diag);
-        sc = r.scores;
-        while (T.st.mode == ABX_MODIFIED_FIXED_SOURCE && r.n_sites > 0) {
-          n = r.n_sites;
-          p->bank_b.reserve((size_t)n * sizeof(abx_particle));
-          hipLaunchKernelGGL(k_rebank, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0, p->stream,
-                             p->ordered.as<abx_banked>(), n, 1.0, hist, T.st.seed, T.st.stride, 1, 0,
-                             p->bank_b.as<abx_particle>());
-          HIPC(hipGetLastError());
-          hist += (uint64_t)n;
-          std::swap(p->bank_a, p->bank_b);
-          r = transport_device(p, p->bank_a.as<abx_particle>(), n, kcol, 1, nullptr, 0, nullptr);
-          add_diag(dg, r.diag);
-          sc.k_col += r.scores.k_col;
-          sc.k_abs += r.scores.k_abs;
-          sc.k_trk += r.scores.k_trk;
-          sc.leakage += r.scores.leakage;
-          sc.mig += r.scores.mig;
-        }
-        HIPC(hipEventRecord(e1, p->stream));
-        HIPC(hipEventSynchronize(e1));
-        float ms = 0;
-        HIPC(hipEventElapsedTime(&ms, e0, e1));
-        ms_total += ms;
-        if (kcol_hist) kcol_hist[g - 1] = sc.k_col / W;
-        record_generation(p, sc, W);
-      } else {
-        HIPC(hipEventRecord(e0, p->stream));
-        TransportOut r = transport_device(p, p->bank_a.as<abx_particle>(), (int64_t)p->bank_n, kcol,
-                                          converged ? 1 : 0, nullptr, 0, nullptr);
-        HIPC(hipEventRecord(e1, p->stream));
-        HIPC(hipEventSynchronize(e1));
-        float ms = 0;
-        HIPC(hipEventElapsedTime(&ms, e0, e1));
-        ms_total += ms;
-        add_diag(dg, r.diag);
-        sc = r.scores;
-        if (r.n_sites == 0) fail(ABX_E_INVALID, "No fission neutrons were produced.");
-        const double kc = sc.k_col / W;
-        if (kcol_hist) kcol_hist[g - 1] = kc;
-        // normalize_weights: W = W_pos - W_neg, w *= nparticles / W
-        p->wsum.reserve(2 * sizeof(double));
-        HIPC(hipMemsetAsync(p->wsum.p, 0, 2 * sizeof(double), p->stream));
-        hipLaunchKernelGGL(k_weight_sums, dim3((unsigned)blocks_for(r.n_sites, 256)), dim3(256), 0,
-                           p->stream, p->ordered.as<abx_banked>(), r.n_sites, p->wsum.as<double>());
-        HIPC(hipGetLastError());
-        double ws[2];
-        HIPC(hipMemcpyAsync(ws, p->wsum.p, 2 * sizeof(double), hipMemcpyDeviceToHost, p->stream));
-        HIPC(hipStreamSynchronize(p->stream));
-        const double w_per_part = (double)T.st.nparticles / (ws[0] - ws[1]);
-        if (converged) record_generation(p, sc, W);
-        p->bank_b.reserve((size_t)r.n_sites * sizeof(abx_particle));
-        hipLaunchKernelGGL(k_rebank, dim3((unsigned)blocks_for(r.n_sites, 256)), dim3(256), 0,
-                           p->stream, p->ordered.as<abx_banked>(), r.n_sites, w_per_part,
-                           global_counter, T.st.seed, T.st.stride, 0, 1, p->bank_b.as<abx_particle>());
-        HIPC(hipGetLastError());
-        std::swap(p->bank_a, p->bank_b);
-        p->bank_n = (size_t)r.n_sites;
-        global_counter += (uint64_t)r.n_sites;
-        kcol = kc;
-        if (g == nignored) converged = true;
-      }
-      clear_generation(p);
-      add_diag(stats->diag, dg);
-    }
-    HIPC(hipStreamSynchronize(p->stream));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    stats->kcol_avg = p->kc_avg;
-    stats->kcol_err = std::sqrt(p->kc_var / (double)p->gen);
-    stats->ktrk_avg = p->kt_avg;
-    stats->ktrk_err = std::sqrt(p->kt_var / (double)p->gen);
-    stats->leak_avg = p->lk_avg;
-    stats->leak_err = std::sqrt(p->lk_var / (double)p->gen);
-    stats->mig_avg = p->mg_avg;
-    stats->mig_err = std::sqrt(p->mg_var / (double)p->gen);
-    stats->active_generations = p->gen;
-    stats->transport_seconds = ms_total * 1e-3;
+    run_reset(p);
+    run_generations(p, ngenerations, nignored, kcol_hist, stats);
+  });
+}
+
+int abx_run_continue(abx_problem* p, int ngenerations, int nignored, double* kcol_hist,
+                     abx_run_stats* stats) {
+  ABX_TRY({
+    if (!p || !stats || ngenerations < 1 || nignored < 0) fail(ABX_E_INVALID, "run arguments");
+    run_generations(p, ngenerations, nignored, kcol_hist, stats);
   });
 }
 

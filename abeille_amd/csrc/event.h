@@ -1,17 +1,458 @@
 // event.h - event-based transport engine.
+//
+// Instead of one lane carrying a history from birth to death (the
+// reference's per-history OpenMP loop, src/delta_tracker.cpp:83-229), the
+// bank lives in HBM as structure-of-arrays and each kernel advances every
+// queued particle by ONE event:
+//   k_ev_init     locate the source particle (Tracker construction)
+//   k_ev_flight   sample a flight, move / relocate, boundary conditions,
+//                 track-length tally, real/virtual decision
+//   k_ev_collide  collision physics (tallies, k scores, fission, implicit
+//                 capture, roulette, scatter)
+// and appends the particle's slot to the queue of its next event with one
+// wavefront-aggregated atomic (ballot + popcount).  Queues shrink as
+// histories end; when few particles remain (the long tail) the rest are
+// finished by k_ev_tail, which runs the whole remaining history in a lane.
+// Per-particle arithmetic is the same physics.h code as the history kernel,
+// so results are bit-identical per history.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "kernels.h"
 
-struct EventBuffers {
-  void release() {}
+// Structure-of-arrays particle + cursor state, one slot per history.
+struct EvState {
+  double *rx, *ry, *rz, *ux, *uy, *uz, *E, *w, *w2, *bx, *by, *bz;
+  uint64_t *rng, *hid, *fam;
+  int32_t *calls, *daughters, *secid, *g, *m, *nsec, *nrec;
+  int32_t *np, *cell, *token;
+  int32_t* pad;   // [D][n]
+  int32_t* tile;  // [3 D][n]
+  double* lv;     // [3 L][n]
+  int64_t n;
+  int32_t D, L;
 };
 
-static inline hipError_t event_transport(const DevProblem& P, const KCtx& K, EventBuffers& ev,
-                                         hipStream_t s) {
-  (void)ev;
-  hipLaunchKernelGGL(k_history, dim3((unsigned)((K.n + 255) / 256)), dim3(256), 0, s, P, K);
-  return hipGetLastError();
+__device__ __forceinline__ void ev_load(const EvState& S, int64_t i, Part& p, Trk& t, int& m) {
+  p.r = V3{S.rx[i], S.ry[i], S.rz[i]};
+  p.u = V3{S.ux[i], S.uy[i], S.uz[i]};
+  p.rb = V3{S.bx[i], S.by[i], S.bz[i]};
+  p.E = S.E[i];
+  p.w = S.w[i];
+  p.w2 = S.w2[i];
+  p.rng = S.rng[i];
+  p.hid = S.hid[i];
+  p.fam = S.fam[i];
+  p.calls = (uint32_t)S.calls[i];
+  p.daughters = (uint32_t)S.daughters[i];
+  p.sec_id = S.secid[i];
+  p.g = S.g[i];
+  p.alive = 1;
+  m = S.m[i];
+  t.np = S.np[i];
+  t.cell = S.cell[i];
+  t.token = S.token[i];
+  t.err = 0;
+  const int64_t n = S.n;
+  for (int k = 0; k < S.D; k++) {
+    if (k < t.np) {
+      t.pad[k] = S.pad[k * n + i];
+      t.tile[k][0] = S.tile[(3 * k) * n + i];
+      t.tile[k][1] = S.tile[(3 * k + 1) * n + i];
+      t.tile[k][2] = S.tile[(3 * k + 2) * n + i];
+    }
+  }
+  int maxl = 0;
+  for (int k = 0; k < t.np && k < S.D; k++) {
+    const int l = pad_lvl(t.pad[k]);
+    maxl = l > maxl ? l : maxl;
+  }
+  for (int l = 0; l < S.L; l++) {
+    if (l <= maxl) {
+      t.lv[l] = V3{S.lv[(3 * l) * n + i], S.lv[(3 * l + 1) * n + i], S.lv[(3 * l + 2) * n + i]};
+    }
+  }
+}
+
+__device__ __forceinline__ void ev_store(const EvState& S, int64_t i, const Part& p, const Trk& t,
+                                         int m) {
+  S.rx[i] = p.r.x;
+  S.ry[i] = p.r.y;
+  S.rz[i] = p.r.z;
+  S.ux[i] = p.u.x;
+  S.uy[i] = p.u.y;
+  S.uz[i] = p.u.z;
+  S.E[i] = p.E;
+  S.w[i] = p.w;
+  S.w2[i] = p.w2;
+  S.rng[i] = p.rng;
+  S.calls[i] = (int32_t)p.calls;
+  S.daughters[i] = (int32_t)p.daughters;
+  S.secid[i] = p.sec_id;
+  S.g[i] = p.g;
+  S.m[i] = m;
+  S.np[i] = t.np;
+  S.cell[i] = t.cell;
+  S.token[i] = t.token;
+  const int64_t n = S.n;
+  int maxl = 0;
+  for (int k = 0; k < S.D; k++) {
+    if (k < t.np) {
+      S.pad[k * n + i] = t.pad[k];
+      S.tile[(3 * k) * n + i] = t.tile[k][0];
+      S.tile[(3 * k + 1) * n + i] = t.tile[k][1];
+      S.tile[(3 * k + 2) * n + i] = t.tile[k][2];
+      const int l = pad_lvl(t.pad[k]);
+      maxl = l > maxl ? l : maxl;
+    }
+  }
+  for (int l = 0; l < S.L; l++) {
+    if (l <= maxl) {
+      S.lv[(3 * l) * n + i] = t.lv[l].x;
+      S.lv[(3 * l + 1) * n + i] = t.lv[l].y;
+      S.lv[(3 * l + 2) * n + i] = t.lv[l].z;
+    }
+  }
+}
+
+// Sink for the event kernels: the secondary stack depth and the trace
+// record count live in the slot's state between kernels.
+struct EvSink : GpuSink {};
+
+__device__ __forceinline__ void ev_sink_init(EvSink& s, const KCtx* K, const DevProblem* P,
+                                             const EvState& S, int64_t i) {
+  s.K = K;
+  s.P = P;
+  s.slot = i;
+  s.nsec = S.nsec[i];
+  s.n_rec = S.nrec[i];
+  for (int k = 0; k < 8; k++) s.c[k] = 0;
+}
+
+// Append `slot` to a queue with one atomic per wavefront.
+__device__ __forceinline__ void wave_push(bool pred, int32_t slot, int32_t* q, unsigned* count) {
+  const unsigned long long mask = __ballot(pred);
+  if (mask == 0ull) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)mask) - 1;
+  unsigned base = 0;
+  if (lane == leader) base = atomicAdd(count, (unsigned)__popcll(mask));
+  base = __shfl(base, leader, 64);
+  if (pred) {
+    const unsigned long long below = mask & ((1ull << lane) - 1ull);
+    q[base + (unsigned)__popcll(below)] = slot;
+  }
+}
+
+// The slot's history is over: publish its per-history outputs.
+__device__ __forceinline__ void ev_finalize(const KCtx& K, const EvSink& s, int64_t i,
+                                            const Part& p) {
+  if (K.daughters) K.daughters[i] = p.daughters;
+  if (K.n_rec) K.n_rec[i] = s.n_rec;
+}
+
+__global__ void __launch_bounds__(256) k_ev_init(DevProblem P, KCtx K, EvState S, int32_t* qf,
+                                                 unsigned* nqf) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  Scores6 sc{0, 0, 0, 0, 0, 0};
+  EvSink s;
+  s.K = &K;
+  s.P = &P;
+  s.slot = i;
+  s.nsec = 0;
+  s.n_rec = 0;
+  for (int k = 0; k < 8; k++) s.c[k] = 0;
+  bool live = false;
+  if (i < K.n) {
+    Part p;
+    load_part(P, K.in[i], p);
+    Trk t;
+    int m = -1;
+    live = start_history(P, s, p, t, m);
+    s.c[0] = 1;
+    S.bx[i] = p.rb.x;
+    S.by[i] = p.rb.y;
+    S.bz[i] = p.rb.z;
+    S.hid[i] = p.hid;
+    S.fam[i] = p.fam;
+    S.nsec[i] = 0;
+    S.nrec[i] = 0;
+    if (live)
+      ev_store(S, i, p, t, m);
+    else
+      ev_finalize(K, s, i, p);
+  }
+  wave_push(live, (int32_t)i, qf, nqf);
+  flush_scores(K, sc, s.c);
+}
+
+__global__ void __launch_bounds__(256) k_ev_flight(DevProblem P, KCtx K, EvState S,
+                                                   const int32_t* qin, const unsigned* nqin,
+                                                   int32_t* qf, unsigned* nqf, int32_t* qc,
+                                                   unsigned* nqc) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  Scores6 sc{0, 0, 0, 0, 0, 0};
+  EvSink s;
+  for (int c = 0; c < 8; c++) s.c[c] = 0;
+  s.K = &K;
+  s.P = &P;
+  bool to_f = false, to_c = false;
+  int32_t slot = 0;
+  if (k < (int64_t)*nqin) {
+    slot = qin[k];
+    ev_sink_init(s, &K, &P, S, slot);
+    Part p;
+    Trk t;
+    int m;
+    ev_load(S, slot, p, t, m);
+    int ev = 0;
+    const bool hc = step_flight(P, s, p, t, m, K.converged != 0, sc, ev);
+    if (p.alive && hc) {
+      to_c = true;
+    } else if (step_finish(P, s, p, t, m, ev)) {
+      to_f = true;
+    }
+    if (to_f || to_c) {
+      ev_store(S, slot, p, t, m);
+      S.nsec[slot] = s.nsec;
+      S.nrec[slot] = s.n_rec;
+    } else {
+      ev_finalize(K, s, slot, p);
+    }
+  }
+  wave_push(to_f, slot, qf, nqf);
+  wave_push(to_c, slot, qc, nqc);
+  flush_scores(K, sc, s.c);
+}
+
+__global__ void __launch_bounds__(256) k_ev_collide(DevProblem P, KCtx K, EvState S,
+                                                    const int32_t* qin, const unsigned* nqin,
+                                                    int32_t* qf, unsigned* nqf) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  Scores6 sc{0, 0, 0, 0, 0, 0};
+  EvSink s;
+  for (int c = 0; c < 8; c++) s.c[c] = 0;
+  s.K = &K;
+  s.P = &P;
+  bool to_f = false;
+  int32_t slot = 0;
+  if (k < (int64_t)*nqin) {
+    slot = qin[k];
+    ev_sink_init(s, &K, &P, S, slot);
+    Part p;
+    Trk t;
+    int m;
+    ev_load(S, slot, p, t, m);
+    int ev = 0;
+    step_collision(P, s, p, m, K.kcol, K.converged != 0, sc, ev);
+    to_f = step_finish(P, s, p, t, m, ev);
+    if (to_f) {
+      ev_store(S, slot, p, t, m);
+      S.nsec[slot] = s.nsec;
+      S.nrec[slot] = s.n_rec;
+    } else {
+      ev_finalize(K, s, slot, p);
+    }
+  }
+  wave_push(to_f, slot, qf, nqf);
+  flush_scores(K, sc, s.c);
+}
+
+// Finish the remaining histories in place (flight queue first, then the
+// collision queue whose particles have a collision pending).
+__global__ void __launch_bounds__(256) k_ev_tail(DevProblem P, KCtx K, EvState S,
+                                                 const int32_t* qf, const unsigned* nqf,
+                                                 const int32_t* qc, const unsigned* nqc) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  Scores6 sc{0, 0, 0, 0, 0, 0};
+  EvSink s;
+  for (int c = 0; c < 8; c++) s.c[c] = 0;
+  s.K = &K;
+  s.P = &P;
+  const int64_t nf = *nqf, nc = *nqc;
+  if (k < nf + nc) {
+    const bool pending = k >= nf;
+    const int32_t slot = pending ? qc[k - nf] : qf[k];
+    ev_sink_init(s, &K, &P, S, slot);
+    Part p;
+    Trk t;
+    int m;
+    ev_load(S, slot, p, t, m);
+    run_loop(P, s, p, t, m, K.kcol, K.converged != 0, sc, pending);
+    ev_finalize(K, s, slot, p);
+  }
+  flush_scores(K, sc, s.c);
+}
+
+// Device buffers of the engine (grow-only).
+struct EventBuffers {
+  void* state = nullptr;
+  size_t state_bytes = 0;
+  int32_t* q[3] = {nullptr, nullptr, nullptr};
+  size_t q_cap = 0;
+  unsigned* counts = nullptr;     // device [8]
+  unsigned* h_counts = nullptr;   // pinned host mirror
+  std::vector<hipEvent_t> evs;
+  int64_t tail_threshold = 0;     // 0 = automatic
+  int64_t iterations = 0;
+  int64_t flight_items = 0;       // particles processed by k_ev_flight (last call)
+  void release() {
+    if (state) (void)hipFree(state);
+    for (auto& x : q)
+      if (x) (void)hipFree(x);
+    if (counts) (void)hipFree(counts);
+    if (h_counts) (void)hipHostFree(h_counts);
+    for (auto e : evs) (void)hipEventDestroy(e);
+    evs.clear();
+    state = nullptr;
+    counts = nullptr;
+    h_counts = nullptr;
+    for (auto& x : q) x = nullptr;
+  }
+};
+
+// Problem-wide bound on cursor depth (pads) and coordinate frames.
+struct CursorDepth {
+  int D, L;
+};
+
+static inline size_t ev_state_bytes(int64_t n, int D, int L) {
+  return (size_t)n * (12 * sizeof(double) + 3 * sizeof(uint64_t) + 10 * sizeof(int32_t) +
+                      (size_t)D * 4 * sizeof(int32_t) + (size_t)L * 3 * sizeof(double));
+}
+
+static inline EvState ev_carve(void* base, int64_t n, int D, int L) {
+  EvState S;
+  char* b = static_cast<char*>(base);
+  auto take = [&](size_t bytes) {
+    char* r = b;
+    b += (bytes + 255) & ~(size_t)255;
+    return r;
+  };
+  double** d[] = {&S.rx, &S.ry, &S.rz, &S.ux, &S.uy, &S.uz, &S.E, &S.w, &S.w2, &S.bx, &S.by, &S.bz};
+  for (auto pp : d) *pp = reinterpret_cast<double*>(take(n * sizeof(double)));
+  uint64_t** u[] = {&S.rng, &S.hid, &S.fam};
+  for (auto pp : u) *pp = reinterpret_cast<uint64_t*>(take(n * sizeof(uint64_t)));
+  int32_t** ii[] = {&S.calls, &S.daughters, &S.secid, &S.g, &S.m, &S.nsec, &S.nrec, &S.np, &S.cell, &S.token};
+  for (auto pp : ii) *pp = reinterpret_cast<int32_t*>(take(n * sizeof(int32_t)));
+  S.pad = reinterpret_cast<int32_t*>(take((size_t)D * n * sizeof(int32_t)));
+  S.tile = reinterpret_cast<int32_t*>(take((size_t)3 * D * n * sizeof(int32_t)));
+  S.lv = reinterpret_cast<double*>(take((size_t)3 * L * n * sizeof(double)));
+  S.n = n;
+  S.D = D;
+  S.L = L;
+  return S;
+}
+
+// Runs one transport() with the event engine.  `kernel_ms` receives the
+// summed duration of the flight kernels (the dominant kernel), measured with
+// HIP events on `s`, and `launches` their count, when `timing` is set.
+static inline hipError_t event_transport(const DevProblem& P, const KCtx& K, EventBuffers& B,
+                                         hipStream_t s, const CursorDepth& cd, bool timing,
+                                         double* kernel_ms, int64_t* launches) {
+  hipError_t e;
+  const int64_t n = K.n;
+  const int D = cd.D, L = cd.L;
+  const size_t need = ev_state_bytes(n, D, L) + 64 * 256;
+  if (need > B.state_bytes) {
+    if (B.state) (void)hipFree(B.state);
+    B.state = nullptr;
+    if ((e = hipMalloc(&B.state, need)) != hipSuccess) return e;
+    B.state_bytes = need;
+  }
+  if ((size_t)n > B.q_cap) {
+    for (auto& x : B.q) {
+      if (x) (void)hipFree(x);
+      x = nullptr;
+      if ((e = hipMalloc(&x, (size_t)n * sizeof(int32_t))) != hipSuccess) return e;
+    }
+    B.q_cap = (size_t)n;
+  }
+  if (!B.counts) {
+    if ((e = hipMalloc(&B.counts, 8 * sizeof(unsigned))) != hipSuccess) return e;
+    if ((e = hipHostMalloc(&B.h_counts, 8 * sizeof(unsigned), hipHostMallocDefault)) != hipSuccess)
+      return e;
+  }
+  const EvState S = ev_carve(B.state, n, D, L);
+  // counts: [0] flight queue A, [1] flight queue B, [2] collision queue
+  int32_t* qa = B.q[0];
+  int32_t* qb = B.q[1];
+  int32_t* qc = B.q[2];
+  unsigned* ca = B.counts + 0;
+  unsigned* cb = B.counts + 1;
+  unsigned* cc = B.counts + 2;
+  if ((e = hipMemsetAsync(B.counts, 0, 8 * sizeof(unsigned), s)) != hipSuccess) return e;
+  const int bs = 256;
+  hipLaunchKernelGGL(k_ev_init, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, s, P, K, S, qa, ca);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  const int64_t tail = B.tail_threshold > 0 ? B.tail_threshold : std::max<int64_t>(8192, n / 64);
+  size_t ev_used = 0;
+  auto get_event = [&]() -> hipEvent_t {
+    if (ev_used == B.evs.size()) {
+      hipEvent_t x;
+      if (hipEventCreate(&x) != hipSuccess) return nullptr;
+      B.evs.push_back(x);
+    }
+    return B.evs[ev_used++];
+  };
+  if ((e = hipMemcpyAsync(B.h_counts, B.counts, 8 * sizeof(unsigned), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+  int64_t nf = B.h_counts[0], nc = 0;
+  int64_t iters = 0, items = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> timed;
+  while (nf + nc > 0) {
+    if (nf + nc <= tail) {
+      hipLaunchKernelGGL(k_ev_tail, dim3((unsigned)((nf + nc + bs - 1) / bs)), dim3(bs), 0, s, P, K,
+                         S, qa, ca, qc, cc);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+      break;
+    }
+    // collisions first (they refill the flight queue), then flights
+    if ((e = hipMemsetAsync(cb, 0, sizeof(unsigned), s)) != hipSuccess) return e;
+    if (nc > 0) {
+      hipLaunchKernelGGL(k_ev_collide, dim3((unsigned)((nc + bs - 1) / bs)), dim3(bs), 0, s, P, K,
+                         S, qc, cc, qa, ca);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    const int64_t nf_all = nf + nc;  // upper bound on the flight queue
+    if ((e = hipMemsetAsync(cc, 0, sizeof(unsigned), s)) != hipSuccess) return e;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (timing) {
+      t0 = get_event();
+      t1 = get_event();
+      (void)hipEventRecord(t0, s);
+    }
+    hipLaunchKernelGGL(k_ev_flight, dim3((unsigned)((nf_all + bs - 1) / bs)), dim3(bs), 0, s, P, K,
+                       S, qa, ca, qb, cb, qc, cc);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (timing) {
+      (void)hipEventRecord(t1, s);
+      timed.push_back({t0, t1});
+    }
+    if ((e = hipMemcpyAsync(B.h_counts, B.counts, 8 * sizeof(unsigned), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+    items += B.h_counts[ca - B.counts];  // input of this flight launch
+    std::swap(qa, qb);
+    std::swap(ca, cb);
+    nf = B.h_counts[ca - B.counts];
+    nc = B.h_counts[2];
+    iters++;
+  }
+  B.iterations = iters;
+  B.flight_items = items;
+  if (timing) {
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+    double ms = 0.0;
+    for (auto& pr : timed) {
+      float x = 0.f;
+      (void)hipEventElapsedTime(&x, pr.first, pr.second);
+      ms += x;
+    }
+    *kernel_ms = ms;
+    *launches = (int64_t)timed.size();
+  }
+  return hipSuccess;
 }

@@ -438,139 +438,183 @@ ABX_HD void split(Sink& s, Part& p, const DevProblem& P) {
   }
 }
 
-// One whole history (plus its secondaries).
+// Start of a history: locate; returns false if the particle is dead
+// (lost at birth, DeltaTracker::transport src/delta_tracker.cpp:88-98).
 template <class Sink>
-ABX_HD void run_history(const DevProblem& P, Sink& s, Part& p, double kcol, bool converged,
-                        Scores6& sc) {
-  Trk t;
+ABX_HD bool start_history(const DevProblem& P, Sink& s, Part& p, Trk& t, int& m) {
   trk_init(P, t, p.r, p.u, 0);
+  if (t.err) {
+    s.error(-3, p.hid);
+    p.alive = 0;
+    return false;
+  }
   if (t.cell < 0) {
     p.alive = 0;
     s.count_lost();
+    return false;
   }
+  m = P.cell[t.cell].fill;
+  return true;
+}
+
+// One flight of the transport loop up to the collision decision.  Returns
+// true when a real collision is pending; otherwise `ev` holds the event.
+// Sets p.alive = 0 and returns false after raising an error.
+template <class Sink>
+ABX_HD bool step_flight(const DevProblem& P, Sink& s, Part& p, Trk& t, int& m, bool converged,
+                        Scores6& sc, int& ev) {
+  const int trk = P.tracking;
+  if (trk == 0) {
+    // ---------------- surface tracking (src/surface_tracker.cpp:70-151)
+    const double d_coll = p_exponential(p, P.Et[m * P.G + p.g]);
+    s.count_flight();
+    const Bound b = trk_nearest(P, t, p.u);
+    const double dmin = d_coll < b.d ? d_coll : b.d;
+    if (converged) score_flight_tallies(P, s, p, dmin, m);
+    sc.k_trk += p.w * dmin * P.vEf[m * P.G + p.g];
+    ev = 2;
+    if (b.d < d_coll || dabs(b.d - d_coll) < ABX_BOUNDRY_TOL) {
+      s.count_crossing();
+      if (b.btype == 0) {
+        leak<Sink>(p, b, sc);
+        ev = 4;
+      } else if (b.btype == 1) {
+        reflect(P, t, p, b);
+        if (t.cell < 0 || t.err) {
+          s.error(-3, p.hid);
+          p.alive = 0;
+          return false;
+        }
+        ev = 3;
+      } else {
+        trk_move(t, p.u, b.d);
+        t.token = -b.tok;
+        trk_get_current(P, t, p.u);
+        p.r = vadd(p.r, vscale(p.u, b.d));
+        if (t.cell < 0 || t.err) {
+          s.error(-3, p.hid);
+          p.alive = 0;
+          return false;
+        }
+        m = P.cell[t.cell].fill;
+      }
+      return false;
+    }
+    p.r = vadd(p.r, vscale(p.u, d_coll));
+    trk_move(t, p.u, d_coll);
+    return true;
+  }
+  // ---------------- delta / carter tracking (src/delta_tracker.cpp:105-195,
+  // src/carter_tracker.cpp:124-216)
+  const bool carter = trk == 2;
+  const double Emaj = (carter ? P.sampling[p.g] : P.majorant[p.g]) + 0.;
+  const double d_coll = p_exponential(p, Emaj);
+  s.count_flight();
+  trk_move(t, p.u, d_coll);
+  trk_get_current(P, t, p.u);
   if (t.err) {
     s.error(-3, p.hid);
-    return;
+    p.alive = 0;
+    return false;
   }
-  int m = t.cell >= 0 ? P.cell[t.cell].fill : -1;
-  const int trk = P.tracking;
+  if (t.cell < 0) {
+    trk_reset_position(P, t, p.r, p.u);
+    if (t.err) {
+      s.error(-3, p.hid);
+      p.alive = 0;
+      return false;
+    }
+    const Bound b = trk_bc(P, t, p.u);
+    if (converged) score_flight_tallies(P, s, p, d_coll < b.d ? d_coll : b.d, m);
+    s.count_crossing();
+    if (b.btype == 0) {
+      leak<Sink>(p, b, sc);
+      ev = 4;
+    } else if (b.btype == 1) {
+      reflect(P, t, p, b);
+      if (t.cell < 0 || t.err) {
+        s.error(-3, p.hid);
+        p.alive = 0;
+        return false;
+      }
+      ev = 3;
+    } else {
+      s.error(-3, p.hid);
+      p.alive = 0;
+    }
+    return false;
+  }
+  if (converged) score_flight_tallies(P, s, p, d_coll, m);
+  ev = 0;
+  p.r = vadd(p.r, vscale(p.u, d_coll));
+  m = P.cell[t.cell].fill;
+  const double Et = P.Et[m * P.G + p.g];
+  bool had_collision = false;
+  if (!carter) {
+    if (Et - Emaj > 1.E-10) {
+      s.error(-4, p.hid);
+      p.alive = 0;
+      return false;
+    }
+    if (p_rand(p) < (Et / Emaj)) had_collision = true;
+  } else if (Emaj >= Et) {
+    if (p_rand(p) < (Et / Emaj)) had_collision = true;
+  } else {
+    const double D = Et / (2. * Et - Emaj);
+    const double F = Et / (D * Emaj);
+    if ((D - p_rand(p)) > 0.) {
+      p.w = p.w * F;
+      had_collision = true;
+    } else {
+      p.w = -p.w * F;
+    }
+  }
+  if (!had_collision) s.count_virtual();
+  return had_collision;
+}
+
+// Real collision; sets the event code.
+template <class Sink>
+ABX_HD void step_collision(const DevProblem& P, Sink& s, Part& p, int m, double kcol,
+                           bool converged, Scores6& sc, int& ev) {
+  s.count_collision();
+  collide(P, s, p, m, kcol, converged, sc);
+  ev = p.alive ? 1 : 5;
+}
+
+// End of a loop iteration: carter split, trace, resurrection.  Returns
+// whether the slot still carries a live particle.
+template <class Sink>
+ABX_HD bool step_finish(const DevProblem& P, Sink& s, Part& p, Trk& t, int& m, int ev) {
+  if (P.tracking == 2) split(s, p, P);
+  s.trace(p, t, ev);
+  if (!p.alive) try_resurrect(P, s, p, t, m);
+  return p.alive != 0;
+}
+
+// One whole history (plus its secondaries).
+template <class Sink>
+ABX_HD void run_loop(const DevProblem& P, Sink& s, Part& p, Trk& t, int& m, double kcol,
+                     bool converged, Scores6& sc, bool collision_pending) {
   for (int guard = 0; p.alive; guard++) {
     if (guard > 100000000) {
       s.error(-3, p.hid);
       return;
     }
-    int ev;
-    bool had_collision = false;
-    if (trk == 0) {
-      // ---------------- surface tracking
-      const double d_coll = p_exponential(p, P.Et[m * P.G + p.g]);
-      s.count_flight();
-      const Bound b = trk_nearest(P, t, p.u);
-      const double dmin = d_coll < b.d ? d_coll : b.d;
-      if (converged) score_flight_tallies(P, s, p, dmin, m);
-      sc.k_trk += p.w * dmin * P.vEf[m * P.G + p.g];
-      ev = 2;
-      if (b.d < d_coll || dabs(b.d - d_coll) < ABX_BOUNDRY_TOL) {
-        s.count_crossing();
-        if (b.btype == 0) {
-          leak<Sink>(p, b, sc);
-          ev = 4;
-        } else if (b.btype == 1) {
-          reflect(P, t, p, b);
-          if (t.cell < 0 || t.err) {
-            s.error(-3, p.hid);
-            return;
-          }
-          ev = 3;
-        } else {
-          trk_move(t, p.u, b.d);
-          t.token = -b.tok;
-          trk_get_current(P, t, p.u);
-          p.r = vadd(p.r, vscale(p.u, b.d));
-          if (t.cell < 0 || t.err) {
-            s.error(-3, p.hid);
-            return;
-          }
-          m = P.cell[t.cell].fill;
-        }
-      } else {
-        p.r = vadd(p.r, vscale(p.u, d_coll));
-        trk_move(t, p.u, d_coll);
-        had_collision = true;
-      }
-    } else {
-      // ---------------- delta / carter tracking
-      const bool carter = trk == 2;
-      const double Emaj = (carter ? P.sampling[p.g] : P.majorant[p.g]) + 0.;
-      const double d_coll = p_exponential(p, Emaj);
-      s.count_flight();
-      Bound b{ABX_INF, -1, 2, 0};
-      bool crossed = false;
-      trk_move(t, p.u, d_coll);
-      trk_get_current(P, t, p.u);
-      if (t.err) {
-        s.error(-3, p.hid);
-        return;
-      }
-      if (t.cell < 0) {
-        trk_reset_position(P, t, p.r, p.u);
-        if (t.err) {
-          s.error(-3, p.hid);
-          return;
-        }
-        b = trk_bc(P, t, p.u);
-        crossed = true;
-      }
-      if (converged) score_flight_tallies(P, s, p, d_coll < b.d ? d_coll : b.d, m);
-      ev = 0;
-      if (crossed) {
-        s.count_crossing();
-        if (b.btype == 0) {
-          leak<Sink>(p, b, sc);
-          ev = 4;
-        } else if (b.btype == 1) {
-          reflect(P, t, p, b);
-          if (t.cell < 0 || t.err) {
-            s.error(-3, p.hid);
-            return;
-          }
-          ev = 3;
-        } else {
-          s.error(-3, p.hid);
-          return;
-        }
-      } else {
-        p.r = vadd(p.r, vscale(p.u, d_coll));
-        m = P.cell[t.cell].fill;
-        const double Et = P.Et[m * P.G + p.g];
-        if (!carter) {
-          if (Et - Emaj > 1.E-10) {
-            s.error(-4, p.hid);
-            return;
-          }
-          if (p_rand(p) < (Et / Emaj)) had_collision = true;
-        } else if (Emaj >= Et) {
-          if (p_rand(p) < (Et / Emaj)) had_collision = true;
-        } else {
-          const double D = Et / (2. * Et - Emaj);
-          const double F = Et / (D * Emaj);
-          if ((D - p_rand(p)) > 0.) {
-            p.w = p.w * F;
-            had_collision = true;
-          } else {
-            p.w = -p.w * F;
-          }
-        }
-        if (!had_collision) s.count_virtual();
-      }
-    }
-    if (p.alive && had_collision) {
-      s.count_collision();
-      collide(P, s, p, m, kcol, converged, sc);
-      ev = p.alive ? 1 : 5;
-    }
-    if (trk == 2) split(s, p, P);
-    s.trace(p, t, ev);
-    if (!p.alive) try_resurrect(P, s, p, t, m);
+    int ev = 0;
+    bool hc = collision_pending;
+    collision_pending = false;
+    if (!hc) hc = step_flight(P, s, p, t, m, converged, sc, ev);
+    if (p.alive && hc) step_collision(P, s, p, m, kcol, converged, sc, ev);
+    if (!step_finish(P, s, p, t, m, ev)) return;
   }
+}
+
+template <class Sink>
+ABX_HD void run_history(const DevProblem& P, Sink& s, Part& p, double kcol, bool converged,
+                        Scores6& sc) {
+  Trk t;
+  int m = -1;
+  if (!start_history(P, s, p, t, m)) return;
+  run_loop(P, s, p, t, m, kcol, converged, sc, false);
 }

@@ -20,6 +20,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -49,6 +50,7 @@ struct HostTables {
   std::vector<int32_t> tally_gbin;
   int64_t tally_bins = 0;
   std::vector<abx_source> sources;
+  int32_t max_pads = 1, max_levels = 1;  // cursor depth bound over all descents
   abx_settings st{};
   double min_energy = 0., max_energy = 100000.;
 };
@@ -193,6 +195,55 @@ static inline void build_tables(const abx_problem_desc* d, HostTables& T) {
       if (U.type == ABX_RECT_LATTICE && U.outer >= 0) U.has_bc = T.univ[U.outer].has_bc;
   tables_require(d->root_universe >= 0 && d->root_universe < d->n_universes, "root universe");
   T.root = d->root_universe;
+  // cursor depth: pads along the deepest descent and coordinate frames
+  // (a lattice element opens a new frame).  Cycles are rejected like the
+  // reference's contains_universe check (src/parser.cpp:206-214).
+  {
+    const int NU = (int)T.univ.size();
+    std::vector<int> pads(NU, -1), lvls(NU, -1), state(NU, 0);
+    std::function<void(int)> visit = [&](int ui) {
+      if (state[ui] == 2) return;
+      tables_require(state[ui] == 0, "Cyclical universe reference found");
+      state[ui] = 1;
+      const DUniv& U = T.univ[ui];
+      int pd = 0, lv = 1;
+      if (U.type == ABX_CELL_UNIVERSE) {
+        pd = 2;
+        for (int k = 0; k < U.ncells; k++) {
+          const DCell& c = T.cell[T.ucells[U.cells_off + k]];
+          if (c.fill_type == ABX_FILL_UNIVERSE) {
+            visit(c.fill);
+            pd = std::max(pd, 2 + pads[c.fill]);
+            lv = std::max(lv, lvls[c.fill]);
+          }
+        }
+      } else {
+        pd = 1;
+        const int64_t ne = (int64_t)U.Nx * U.Ny * U.Nz;
+        for (int64_t k = 0; k < ne; k++) {
+          const int e = T.elems[U.elems_off + k];
+          if (e >= 0) {
+            visit(e);
+            pd = std::max(pd, 1 + pads[e]);
+            lv = std::max(lv, 1 + lvls[e]);
+          }
+        }
+        if (U.outer >= 0) {
+          visit(U.outer);
+          pd = std::max(pd, 1 + pads[U.outer]);
+          lv = std::max(lv, lvls[U.outer]);
+        }
+      }
+      pads[ui] = pd;
+      lvls[ui] = lv;
+      state[ui] = 2;
+    };
+    visit(T.root);
+    T.max_pads = pads[T.root];
+    T.max_levels = lvls[T.root];
+    tables_require(T.max_pads <= ABX_MAXPADS, "geometry nests deeper than the cursor supports");
+    tables_require(T.max_levels <= ABX_MAXLVL, "too many nested lattices");
+  }
 
   // materials
   const int M = d->n_materials;

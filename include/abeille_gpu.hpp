@@ -1,0 +1,223 @@
+// abeille_gpu.hpp - C++ host mirror of the reference's transport interface
+// on top of the C ABI (abx.h).
+//
+// Same names, argument meaning and error behaviour as the reference's
+//   Transporter          include/simulation/transporter.hpp:39-87
+//   Tallies              include/simulation/tallies.hpp:38-186
+//   Particle / BankedParticle  include/simulation/particle.hpp:38-242
+// so a driver written against them (FixedSource::run, PowerIterator::run)
+// runs unchanged with GpuTransporter in place of DeltaTracker /
+// SurfaceTracker / CarterTracker.  Fatal conditions throw abx::Error (the
+// reference calls fatal_error(), which exits).
+#ifndef ABEILLE_GPU_HPP
+#define ABEILLE_GPU_HPP
+
+#include <cmath>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "abx.h"
+
+namespace abx {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+inline void check(int rc) {
+  if (rc != ABX_OK) throw Error(rc, abx_last_error());
+}
+
+struct Vec3 {
+  double x = 0, y = 0, z = 0;
+};
+
+// BankedParticle (particle.hpp:38-66) without the exact-cancellation fields.
+struct BankedParticle {
+  Vec3 r, u;
+  double E = 0, wgt = 0, wgt2 = 0;
+  uint64_t parent_history_id = 0, parent_daughter_id = 0, family_id = 0;
+};
+
+// Particle as handed to transport(): state plus its pcg32 stream.
+class Particle {
+ public:
+  Particle(Vec3 r, Vec3 u, double E, double wgt, uint64_t id = 0)
+      : r_(r), u_(u), E_(E), w_(wgt), w2_(0.), id_(id) {}
+  Particle(Vec3 r, Vec3 u, double E, double wgt, double wgt2, uint64_t id = 0)
+      : r_(r), u_(u), E_(E), w_(wgt), w2_(wgt2), id_(id) {}
+  const Vec3& r() const { return r_; }
+  const Vec3& u() const { return u_; }
+  double E() const { return E_; }
+  double wgt() const { return w_; }
+  double wgt2() const { return w2_; }
+  uint64_t history_id() const { return id_; }
+  uint64_t family_id() const { return fam_; }
+  void set_family_id(uint64_t f) { fam_ = f; }
+  // Particle::initialize_rng (particle.hpp:188-193): pcg32(seed) advanced
+  // by stride * history_id (Brown's jump).
+  void initialize_rng(uint64_t seed, uint64_t stride) {
+    const uint64_t M = 6364136223846793005ULL, I = 1442695040888963407ULL;
+    uint64_t s = (seed + I) * M + I;
+    uint64_t delta = stride * id_, cm = M, cp = I, am = 1, ap = 0;
+    while (delta) {
+      if (delta & 1u) {
+        am *= cm;
+        ap = ap * cm + cp;
+      }
+      cp = (cm + 1) * cp;
+      cm *= cm;
+      delta >>= 1;
+    }
+    rng_state = am * s + ap;
+  }
+  uint64_t rng_state = 0;
+
+ private:
+  Vec3 r_, u_;
+  double E_, w_, w2_;
+  uint64_t id_, fam_ = 0;
+};
+
+// Owns the staged problem (abx_create) - the object graph that
+// parse_input_file() builds in the reference.
+class Problem {
+ public:
+  explicit Problem(const abx_problem_desc& d) { check(abx_create(&d, &p_)); }
+  ~Problem() { abx_destroy(p_); }
+  Problem(const Problem&) = delete;
+  Problem& operator=(const Problem&) = delete;
+  abx_problem* get() const { return p_; }
+
+ private:
+  abx_problem* p_ = nullptr;
+};
+
+// Tallies (tallies.hpp:38-186): scalar scores on the host, mesh tallies
+// resident on the device.
+class Tallies {
+ public:
+  Tallies(std::shared_ptr<Problem> prob, double tot_wgt) : prob_(std::move(prob)), W_(tot_wgt) {}
+  void score_k_col(double s) { k_col_s_ += s; }
+  void score_k_abs(double s) { k_abs_s_ += s; }
+  void score_k_trk(double s) { k_trk_s_ += s; }
+  void score_k_tot(double s) { k_tot_s_ += s; }
+  void score_leak(double s) { leak_s_ += s; }
+  void score_mig_area(double s) { mig_s_ += s; }
+  void clear_generation() {
+    k_col_s_ = k_abs_s_ = k_trk_s_ = k_tot_s_ = leak_s_ = mig_s_ = 0.;
+    check(abx_tally_clear(prob_->get()));
+  }
+  // Tallies::calc_gen_values (tallies.cpp:159-181)
+  void calc_gen_values() {
+    k_col_ = k_col_s_ / W_;
+    k_trk_ = k_trk_s_ / W_;
+    leak_ = leak_s_ / W_;
+    mig_ = mig_s_ / W_;
+    k_col_vec.push_back(k_col_);
+  }
+  // Tallies::record_generation (tallies.cpp:183-203) for the scalars; mesh
+  // statistics are kept by abx_run on the device.
+  void record_generation() {
+    gen_++;
+    upd(k_col_, k_col_avg_, k_col_var_);
+    upd(k_trk_, k_trk_avg_, k_trk_var_);
+    upd(leak_, leak_avg_, leak_var_);
+    upd(mig_, mig_avg_, mig_var_);
+  }
+  double kcol() const { return k_col_; }
+  double kcol_avg() const { return k_col_avg_; }
+  double kcol_err() const { return std::sqrt(k_col_var_ / gen_); }
+  double ktrk_avg() const { return k_trk_avg_; }
+  double leakage_avg() const { return leak_avg_; }
+  double leakage_err() const { return std::sqrt(leak_var_ / gen_); }
+  int generations() const { return gen_; }
+  std::vector<double> k_col_vec;
+  Problem& problem() { return *prob_; }
+
+ private:
+  void upd(double x, double& avg, double& var) {
+    const double dg = (double)gen_, old = avg, vold = var;
+    avg = old + (x - old) / dg;
+    if (gen_ > 1) var = vold + ((x - old) * (x - old) / dg) - (vold / (dg - 1.));
+  }
+  std::shared_ptr<Problem> prob_;
+  double W_;
+  double k_col_s_ = 0, k_abs_s_ = 0, k_trk_s_ = 0, k_tot_s_ = 0, leak_s_ = 0, mig_s_ = 0;
+  double k_col_ = 1.0, k_trk_ = 1.0, leak_ = 0., mig_ = 0.;
+  double k_col_avg_ = 0, k_col_var_ = 0, k_trk_avg_ = 0, k_trk_var_ = 0, leak_avg_ = 0,
+         leak_var_ = 0, mig_avg_ = 0, mig_var_ = 0;
+  int gen_ = 0;
+};
+
+// Transporter (transporter.hpp:39-87).
+class Transporter {
+ public:
+  explicit Transporter(std::shared_ptr<Tallies> t) : tallies(std::move(t)) {}
+  virtual ~Transporter() = default;
+  virtual std::vector<BankedParticle> transport(std::vector<Particle>& bank,
+                                                bool converged) = 0;
+
+ protected:
+  std::shared_ptr<Tallies> tallies;
+};
+
+// The GPU transporter: whichever tracker the problem's settings name
+// (delta / surface / carter) runs on the device.  `bank` is consumed, the
+// fission bank comes back in parent order then push order.
+class GpuTransporter : public Transporter {
+ public:
+  using Transporter::Transporter;
+  std::vector<BankedParticle> transport(std::vector<Particle>& bank, bool converged) override {
+    std::vector<abx_particle> in(bank.size());
+    for (size_t i = 0; i < bank.size(); i++) {
+      const Particle& p = bank[i];
+      abx_particle& a = in[i];
+      a.r[0] = p.r().x;
+      a.r[1] = p.r().y;
+      a.r[2] = p.r().z;
+      a.u[0] = p.u().x;
+      a.u[1] = p.u().y;
+      a.u[2] = p.u().z;
+      a.E = p.E();
+      a.wgt = p.wgt();
+      a.wgt2 = p.wgt2();
+      a.history_id = p.history_id();
+      a.family_id = p.family_id();
+      a.rng_state = p.rng_state;
+    }
+    std::vector<abx_banked> out(4 * bank.size() + 1024);
+    size_t n_out = 0;
+    abx_scores sc{};
+    abx_diag dg{};
+    check(abx_transport(tallies->problem().get(), in.data(), in.size(), tallies->kcol(),
+                        converged ? 1 : 0, out.data(), out.size(), &n_out, &sc, &dg));
+    tallies->score_k_col(sc.k_col);
+    tallies->score_k_abs(sc.k_abs);
+    tallies->score_k_trk(sc.k_trk);
+    tallies->score_k_tot(sc.k_tot);
+    tallies->score_leak(sc.leakage);
+    tallies->score_mig_area(sc.mig);
+    std::vector<BankedParticle> res(n_out);
+    for (size_t i = 0; i < n_out; i++) {
+      const abx_banked& b = out[i];
+      res[i].r = {b.r[0], b.r[1], b.r[2]};
+      res[i].u = {b.u[0], b.u[1], b.u[2]};
+      res[i].E = b.E;
+      res[i].wgt = b.wgt;
+      res[i].wgt2 = b.wgt2;
+      res[i].parent_history_id = b.parent_history_id;
+      res[i].parent_daughter_id = b.parent_daughter_id;
+      res[i].family_id = b.family_id;
+    }
+    bank.clear();
+    return res;
+  }
+};
+
+}  // namespace abx
+
+#endif

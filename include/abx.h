@@ -324,6 +324,10 @@ typedef struct abx_run_stats {
   int32_t active_generations;
   int32_t pad_;
   double transport_seconds; /* GPU time of the transport phase */
+  double kernel_seconds;    /* GPU time of the dominant transport kernel */
+  int64_t kernel_launches;  /* its launches */
+  int64_t kernel_items;     /* events it processed */
+  int64_t histories;        /* histories transported (incl. re-banked) */
   abx_diag diag;            /* summed over the run */
 } abx_run_stats;
 
@@ -334,6 +338,12 @@ typedef struct abx_run_stats {
  * (src/mesh_tally.cpp:121-150) and write_tally (:191-197). */
 int abx_run(abx_problem* p, int ngenerations, int nignored,
             double* kcol_hist, abx_run_stats* stats);
+/* Continue the current run for `ngenerations` more batches / generations
+ * (the bank, k and statistics carry over; the first call samples the
+ * source).  `nignored` counts inactive generations from here on.  Stats
+ * cover only these generations' timing and diagnostics. */
+int abx_run_continue(abx_problem* p, int ngenerations, int nignored,
+                     double* kcol_hist, abx_run_stats* stats);
 int abx_tally_stats(abx_problem* p, int tally, double* avg, double* std_err);
 
 /* Device bank of the last k-eigenvalue generation in the reference's

@@ -27,7 +27,7 @@ CASES = {
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
-@pytest.mark.parametrize("engine", [abi.ENGINE_HISTORY])
+@pytest.mark.parametrize("engine", [abi.ENGINE_EVENT, abi.ENGINE_HISTORY], ids=["event", "history"])
 def test_trace_bit_exact(name, engine):
     print(f"[parity] case={name} engine={engine}", flush=True)
     m = CASES[name]()
@@ -48,12 +48,13 @@ def test_trace_bit_exact(name, engine):
 
 
 @pytest.mark.parametrize("name", ["cube_dt", "lattice_dt", "core_st"])
-def test_tally_generation_matches(name):
+@pytest.mark.parametrize("engine", [abi.ENGINE_EVENT, abi.ENGINE_HISTORY], ids=["event", "history"])
+def test_tally_generation_matches(name, engine):
     """One batch: the GPU tally array equals the oracle's up to summation
     order (atomics), bin by bin to 1e-9 relative."""
     m = CASES[name]()
     o = Oracle(m)
-    g = Problem(m, engine=abi.ENGINE_HISTORY)
+    g = Problem(m, engine=engine)
     so = o.sample_sources(0, 2000)
     o.tally_clear()
     g.tally_clear()
