@@ -53,6 +53,7 @@ def lib():
         L.abx_run.argtypes = [P, C.c_int, C.c_int, C.c_void_p, C.POINTER(abi.RunStats)]
         L.abx_run_continue.argtypes = [P, C.c_int, C.c_int, C.c_void_p,
                                        C.POINTER(abi.RunStats)]
+        L.abx_cancel_bank.argtypes = [P, C.c_void_p, C.c_size_t]
         L.abx_bank_size.argtypes = [P, C.POINTER(C.c_size_t)]
         L.abx_bank_fetch.argtypes = [P, C.c_void_p]
         _lib = L
@@ -147,6 +148,11 @@ class Problem:
         _check(lib().abx_run_continue(self.h, ngenerations, nignored, kh.ctypes.data,
                                       C.byref(st)))
         return st, kh
+
+    def cancel_bank(self, bank):
+        b = np.ascontiguousarray(bank, dtype=abi.BANKED_DTYPE).copy()
+        _check(lib().abx_cancel_bank(self.h, b.ctypes.data, len(b)))
+        return b
 
     def bank(self):
         n = C.c_size_t()

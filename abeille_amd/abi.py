@@ -98,6 +98,11 @@ class Settings(C.Structure):
                 ("sample_xs_ratio", PD)]
 
 
+class CancelMesh(C.Structure):
+    _fields_ = [("low", C.c_double * 3), ("high", C.c_double * 3), ("shape", C.c_int32 * 3),
+                ("n_ebounds", C.c_int32), ("ebounds", PD)]
+
+
 class ProblemDesc(C.Structure):
     _fields_ = [("n_surfaces", C.c_int32), ("surfaces", C.POINTER(Surface)),
                 ("n_cells", C.c_int32), ("cells", C.POINTER(Cell)),
@@ -109,7 +114,7 @@ class ProblemDesc(C.Structure):
                 ("n_materials", C.c_int32), ("materials", C.POINTER(Material)),
                 ("n_sources", C.c_int32), ("sources", C.POINTER(Source)),
                 ("n_tallies", C.c_int32), ("tallies", C.POINTER(MeshTally)),
-                ("settings", Settings)]
+                ("settings", Settings), ("cancel", C.POINTER(CancelMesh))]
 
 
 class Particle(C.Structure):

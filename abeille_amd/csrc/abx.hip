@@ -12,6 +12,8 @@
 #include <string>
 #include <vector>
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "event.h"
 #include "kernels.h"
 #include "tables.h"
@@ -73,6 +75,10 @@ struct abx_problem {
   DBuf in, sec, stage, stage_parent, stage_daughter, n_stage, daughters, offsets,
       block_sums, block_sums2, ordered, scores, counters, err, err_hist, trace, n_rec, wsum;
   EventBuffers ev;
+  // cancellation
+  bool cancel = false;
+  DevCancel cm{};
+  DBuf ckeys, ckeys2, ctemp;
   // driver state
   DBuf bank_a, bank_b;  // abx_particle banks
   size_t bank_n = 0;
@@ -106,7 +112,7 @@ struct abx_problem {
     DBuf* all[] = {&tally_gen, &tally_avg, &tally_var, &in, &sec, &stage, &stage_parent,
                    &stage_daughter, &n_stage, &daughters, &offsets, &block_sums, &block_sums2,
                    &ordered, &scores, &counters, &err, &err_hist, &trace, &n_rec, &wsum,
-                   &bank_a, &bank_b};
+                   &bank_a, &bank_b, &ckeys, &ckeys2, &ctemp};
     for (DBuf* b : all) b->release();
     ev.release();
     if (ke0) (void)hipEventDestroy(ke0);
@@ -141,6 +147,25 @@ uint64_t device_scan(abx_problem* p, const uint32_t* in, uint64_t* out, int64_t 
                      p->block_sums2.as<uint64_t>(), n);
   HIPC(hipGetLastError());
   return acc;
+}
+
+// Approximate mesh cancellation of a device bank (in place).
+void cancel_device(abx_problem* p, abx_banked* d_bank, int64_t n) {
+  if (n <= 1) return;
+  p->ckeys.reserve((size_t)n * sizeof(uint64_t));
+  p->ckeys2.reserve((size_t)n * sizeof(uint64_t));
+  hipLaunchKernelGGL(k_cancel_keys, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0, p->stream,
+                     d_bank, n, p->cm, p->ckeys.as<uint64_t>());
+  HIPC(hipGetLastError());
+  size_t temp = 0;
+  HIPC(rocprim::radix_sort_keys(nullptr, temp, p->ckeys.as<uint64_t>(), p->ckeys2.as<uint64_t>(),
+                                (size_t)n, 0, 64, p->stream));
+  p->ctemp.reserve(std::max<size_t>(temp, 16));
+  HIPC(rocprim::radix_sort_keys(p->ctemp.p, temp, p->ckeys.as<uint64_t>(), p->ckeys2.as<uint64_t>(),
+                                (size_t)n, 0, 64, p->stream));
+  hipLaunchKernelGGL(k_cancel_apply, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0, p->stream,
+                     p->ckeys2.as<uint64_t>(), n, d_bank);
+  HIPC(hipGetLastError());
 }
 
 bool needs_secondaries(const HostTables& T) {
@@ -210,11 +235,8 @@ TransportOut transport_device(abx_problem* p, const abx_particle* d_in, int64_t 
       HIPC(hipEventCreate(&p->ke1));
     }
     if (p->engine == ABX_ENGINE_HISTORY) {
-      const int bs = 256;
       if (p->timing) HIPC(hipEventRecord(p->ke0, p->stream));
-      hipLaunchKernelGGL(k_history, dim3((unsigned)blocks_for(n, bs)), dim3(bs), 0, p->stream,
-                         p->P, K);
-      HIPC(hipGetLastError());
+      HIPC(history_transport(p->P, K, p->stream, sec_cap > 0));
       if (p->timing) {
         HIPC(hipEventRecord(p->ke1, p->stream));
         HIPC(hipEventSynchronize(p->ke1));
@@ -222,13 +244,12 @@ TransportOut transport_device(abx_problem* p, const abx_particle* d_in, int64_t 
         HIPC(hipEventElapsedTime(&ms, p->ke0, p->ke1));
         p->kernel_ms += ms;
         p->kernel_launches++;
-        p->kernel_items += n;  // every event of the bank: counted after the launch
+        p->kernel_items += n;
       }
     } else {
       double ms = 0.0;
       int64_t launches = 0;
-      HIPC(event_transport(p->P, K, p->ev, p->stream, CursorDepth{T.max_pads, T.max_levels},
-                           p->timing, &ms, &launches));
+      HIPC(event_transport(p->P, K, p->ev, p->stream, sec_cap > 0, p->timing, &ms, &launches));
       p->kernel_ms += ms;
       p->kernel_launches += launches;
       p->kernel_items += p->ev.flight_items;
@@ -319,7 +340,8 @@ void clear_generation(abx_problem* p) {
 void sample_sources_device(abx_problem* p, uint64_t first, int64_t n, abx_particle* d_out) {
   HIPC(hipMemsetAsync(p->err.p, 0, sizeof(int32_t), p->stream));
   if (n > 0) {
-    hipLaunchKernelGGL(k_sample_sources, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0, p->stream,
+    hipLaunchKernelGGL(k_sample_sources, dim3((unsigned)blocks_for(n, 256)), dim3(256),
+                       (size_t)256 * trk_bytes(p->P.trk_D, p->P.trk_L), p->stream,
                        p->P, p->S, first, n, d_out, p->err.as<int32_t>(),
                        p->err_hist.as<unsigned long long>());
     HIPC(hipGetLastError());
@@ -394,6 +416,29 @@ int abx_create(const abx_problem_desc* desc, abx_problem** out) {
     p->S.n = (int32_t)w.size();
     p->err.reserve(sizeof(int32_t));
     p->err_hist.reserve(sizeof(unsigned long long));
+    if (desc->cancel) {
+      const abx_cancel_mesh& c = *desc->cancel;
+      if (!(c.low[0] < c.high[0] && c.low[1] < c.high[1] && c.low[2] < c.high[2]))
+        fail(ABX_E_INVALID, "Low position is not lower than hi position in ApproximateMeshCancelator.");
+      if (c.shape[0] < 1 || c.shape[1] < 1 || c.shape[2] < 1) fail(ABX_E_INVALID, "cancellation shape");
+      if (c.n_ebounds == 1 || c.n_ebounds < 0) fail(ABX_E_INVALID, "energy_edges must have at least two entries");
+      p->cancel = true;
+      p->cm.lx = c.low[0];
+      p->cm.ly = c.low[1];
+      p->cm.lz = c.low[2];
+      p->cm.nx = c.shape[0];
+      p->cm.ny = c.shape[1];
+      p->cm.nz = c.shape[2];
+      p->cm.dx = (c.high[0] - c.low[0]) / (double)(uint32_t)c.shape[0];
+      p->cm.dy = (c.high[1] - c.low[1]) / (double)(uint32_t)c.shape[1];
+      p->cm.dz = (c.high[2] - c.low[2]) / (double)(uint32_t)c.shape[2];
+      p->cm.n_edges = c.n_ebounds;
+      p->cm.ne = c.n_ebounds > 0 ? c.n_ebounds - 1 : 1;
+      std::vector<double> edges(c.ebounds, c.ebounds + std::max(0, c.n_ebounds));
+      p->cm.edges = p->upload(edges);
+      if ((int64_t)c.shape[0] * c.shape[1] * c.shape[2] * p->cm.ne >= 0xffffffffll)
+        fail(ABX_E_INVALID, "cancellation mesh too large");
+    }
     *out = p;
     return ABX_OK;
   } catch (const AbxError& e) {
@@ -579,6 +624,7 @@ static void run_generations(abx_problem* p, int ngenerations, int nignored, doub
       sc = r.scores;
       while (T.st.mode == ABX_MODIFIED_FIXED_SOURCE && r.n_sites > 0) {
         n = r.n_sites;
+        if (p->cancel) cancel_device(p, p->ordered.as<abx_banked>(), n);
         p->bank_b.reserve((size_t)n * sizeof(abx_particle));
         hipLaunchKernelGGL(k_rebank, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0, p->stream,
                            p->ordered.as<abx_banked>(), n, 1.0, p->run_hist, T.st.seed, T.st.stride, 1, 0,
@@ -679,6 +725,19 @@ int abx_run_continue(abx_problem* p, int ngenerations, int nignored, double* kco
   ABX_TRY({
     if (!p || !stats || ngenerations < 1 || nignored < 0) fail(ABX_E_INVALID, "run arguments");
     run_generations(p, ngenerations, nignored, kcol_hist, stats);
+  });
+}
+
+int abx_cancel_bank(abx_problem* p, abx_banked* bank, size_t n) {
+  ABX_TRY({
+    if (!p || (!bank && n)) fail(ABX_E_INVALID, "null argument");
+    if (!p->cancel) fail(ABX_E_INVALID, "no cancellation mesh in the description");
+    if (n > 0xffffffffull) fail(ABX_E_INVALID, "bank larger than 2^32");
+    p->ordered.reserve(std::max<size_t>(1, n) * sizeof(abx_banked));
+    if (n) HIPC(hipMemcpyAsync(p->ordered.p, bank, n * sizeof(abx_banked), hipMemcpyHostToDevice, p->stream));
+    cancel_device(p, p->ordered.as<abx_banked>(), (int64_t)n);
+    if (n) HIPC(hipMemcpyAsync(bank, p->ordered.p, n * sizeof(abx_banked), hipMemcpyDeviceToHost, p->stream));
+    HIPC(hipStreamSynchronize(p->stream));
   });
 }
 

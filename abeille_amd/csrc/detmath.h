@@ -24,8 +24,10 @@
 
 #if defined(__HIPCC__)
 #define DM_HD __host__ __device__ __forceinline__
+#define DM_NOINL __host__ __device__ __attribute__((noinline))
 #else
 #define DM_HD static inline
+#define DM_NOINL static
 #endif
 
 DM_HD uint64_t dm_bits(double x) {
@@ -133,7 +135,16 @@ DM_HD double dm_kcos(double r) {
 // sin and cos together.  Accurate for |x| up to about 1e5 (the reduction
 // uses a three-part Cody-Waite split of pi/2); the transport path only
 // calls it on [0, 2 pi].
-DM_HD void dm_sincos(double x, double* sn, double* cs) {
+struct DmSinCos {
+  double s, c;
+};
+
+// Returned by value (registers) so that the out-of-line device version
+// needs no stack.
+DM_NOINL DmSinCos dm_sincos2(double x) {
+  double sv, cv;
+  double* sn = &sv;
+  double* cs = &cv;
   const double invpio2 = 6.36619772367581382433e-01; /* 2/pi */
   const double pio2_1 = 1.57079632673412561417e+00;  /* first 33 bits */
   const double pio2_2 = 6.07710050630396597660e-11;  /* next 33 bits */
@@ -141,14 +152,10 @@ DM_HD void dm_sincos(double x, double* sn, double* cs) {
   const double pio4 = 7.85398163397448278999e-01;
   double ax = x < 0.0 ? -x : x;
   if (!(ax == ax) || ax == 1.0 / 0.0) {  // NaN or inf
-    *sn = x - x;
-    *cs = x - x;
-    return;
+    return DmSinCos{x - x, x - x};
   }
   if (ax <= pio4) {
-    *sn = dm_ksin(x);
-    *cs = dm_kcos(x);
-    return;
+    return DmSinCos{dm_ksin(x), dm_kcos(x)};
   }
   // n = nearest integer to x * 2/pi (ties away from zero)
   double t = x * invpio2;
@@ -175,6 +182,13 @@ DM_HD void dm_sincos(double x, double* sn, double* cs) {
       *cs = s;
       break;
   }
+  return DmSinCos{sv, cv};
+}
+
+DM_HD void dm_sincos(double x, double* sn, double* cs) {
+  const DmSinCos r = dm_sincos2(x);
+  *sn = r.s;
+  *cs = r.c;
 }
 
 DM_HD double dm_sin(double x) {

@@ -15,8 +15,19 @@
 
 #if defined(__HIPCC__)
 #define ABX_HD __host__ __device__ __forceinline__
+// Out of line on the device: rare or bulky paths kept out of the hot loop so
+// the kernels' code stays within the instruction cache.
+#define ABX_NOINL __host__ __device__ __attribute__((noinline))
 #else
 #define ABX_HD static inline
+#define ABX_NOINL static
+#endif
+
+// Member functions (no `static` on the host).
+#if defined(__HIPCC__)
+#define ABX_HDM __host__ __device__ __forceinline__
+#else
+#define ABX_HDM inline
 #endif
 
 #define ABX_MAXPADS 10   // lily pads per tracker (C3 needs 6)
@@ -80,6 +91,7 @@ struct DevProblem {
   int32_t ntally;
   int32_t n_dgrp_max;
   int32_t precision;
+  int32_t trk_D, trk_L;  // cursor capacity (pads, frames) for this geometry
   double wgt_cutoff, wgt_survival, wgt_split;
   double min_energy, max_energy;
   uint64_t seed, stride;

@@ -57,24 +57,21 @@ __device__ __forceinline__ void ev_load(const EvState& S, int64_t i, Part& p, Tr
   t.token = S.token[i];
   t.err = 0;
   const int64_t n = S.n;
-  for (int k = 0; k < S.D; k++) {
-    if (k < t.np) {
-      t.pad[k] = S.pad[k * n + i];
-      t.tile[k][0] = S.tile[(3 * k) * n + i];
-      t.tile[k][1] = S.tile[(3 * k + 1) * n + i];
-      t.tile[k][2] = S.tile[(3 * k + 2) * n + i];
-    }
-  }
   int maxl = 0;
-  for (int k = 0; k < t.np && k < S.D; k++) {
-    const int l = pad_lvl(t.pad[k]);
+  for (int k = 0; k < t.np; k++) {
+    const int32_t w = S.pad[k * n + i];
+    t.pad_[k * t.st] = w;
+    if (pad_type(w) == PAD_L) {
+      t.tile_[(3 * k) * t.st] = S.tile[(3 * k) * n + i];
+      t.tile_[(3 * k + 1) * t.st] = S.tile[(3 * k + 1) * n + i];
+      t.tile_[(3 * k + 2) * t.st] = S.tile[(3 * k + 2) * n + i];
+    }
+    const int l = pad_lvl(w);
     maxl = l > maxl ? l : maxl;
   }
-  for (int l = 0; l < S.L; l++) {
-    if (l <= maxl) {
-      t.lv[l] = V3{S.lv[(3 * l) * n + i], S.lv[(3 * l + 1) * n + i], S.lv[(3 * l + 2) * n + i]};
-    }
-  }
+  t.set_lv(0, p.r);  // frame 0 is the global position
+  for (int l = 1; l <= maxl; l++)
+    t.set_lv(l, V3{S.lv[(3 * l) * n + i], S.lv[(3 * l + 1) * n + i], S.lv[(3 * l + 2) * n + i]});
 }
 
 __device__ __forceinline__ void ev_store(const EvState& S, int64_t i, const Part& p, const Trk& t,
@@ -99,37 +96,23 @@ __device__ __forceinline__ void ev_store(const EvState& S, int64_t i, const Part
   S.token[i] = t.token;
   const int64_t n = S.n;
   int maxl = 0;
-  for (int k = 0; k < S.D; k++) {
-    if (k < t.np) {
-      S.pad[k * n + i] = t.pad[k];
-      S.tile[(3 * k) * n + i] = t.tile[k][0];
-      S.tile[(3 * k + 1) * n + i] = t.tile[k][1];
-      S.tile[(3 * k + 2) * n + i] = t.tile[k][2];
-      const int l = pad_lvl(t.pad[k]);
-      maxl = l > maxl ? l : maxl;
+  for (int k = 0; k < t.np; k++) {
+    const int32_t w = t.pad(k);
+    S.pad[k * n + i] = w;
+    if (pad_type(w) == PAD_L) {
+      S.tile[(3 * k) * n + i] = t.tile(k, 0);
+      S.tile[(3 * k + 1) * n + i] = t.tile(k, 1);
+      S.tile[(3 * k + 2) * n + i] = t.tile(k, 2);
     }
+    const int l = pad_lvl(w);
+    maxl = l > maxl ? l : maxl;
   }
-  for (int l = 0; l < S.L; l++) {
-    if (l <= maxl) {
-      S.lv[(3 * l) * n + i] = t.lv[l].x;
-      S.lv[(3 * l + 1) * n + i] = t.lv[l].y;
-      S.lv[(3 * l + 2) * n + i] = t.lv[l].z;
-    }
+  for (int l = 1; l <= maxl; l++) {
+    const V3 v = t.lv(l);
+    S.lv[(3 * l) * n + i] = v.x;
+    S.lv[(3 * l + 1) * n + i] = v.y;
+    S.lv[(3 * l + 2) * n + i] = v.z;
   }
-}
-
-// Sink for the event kernels: the secondary stack depth and the trace
-// record count live in the slot's state between kernels.
-struct EvSink : GpuSink {};
-
-__device__ __forceinline__ void ev_sink_init(EvSink& s, const KCtx* K, const DevProblem* P,
-                                             const EvState& S, int64_t i) {
-  s.K = K;
-  s.P = P;
-  s.slot = i;
-  s.nsec = S.nsec[i];
-  s.n_rec = S.nrec[i];
-  for (int k = 0; k < 8; k++) s.c[k] = 0;
 }
 
 // Append `slot` to a queue with one atomic per wavefront.
@@ -148,28 +131,39 @@ __device__ __forceinline__ void wave_push(bool pred, int32_t slot, int32_t* q, u
 }
 
 // The slot's history is over: publish its per-history outputs.
-__device__ __forceinline__ void ev_finalize(const KCtx& K, const EvSink& s, int64_t i,
+template <class Sink>
+__device__ __forceinline__ void ev_finalize(const KCtx& K, const Sink& s, int64_t i,
                                             const Part& p) {
   if (K.daughters) K.daughters[i] = p.daughters;
   if (K.n_rec) K.n_rec[i] = s.n_rec;
 }
 
+template <class Sink>
+__device__ __forceinline__ void ev_resume(Sink& s, const EvState& S, int64_t i) {
+  s.slot = i;
+  s.nsec = S.nsec[i];
+  s.n_rec = S.nrec[i];
+}
+
+template <class Sink>
+__device__ __forceinline__ void ev_save(const Sink& s, const EvState& S, int64_t i) {
+  S.nsec[i] = s.nsec;
+  S.nrec[i] = s.n_rec;
+}
+
+template <bool TRACE, bool SEC>
 __global__ void __launch_bounds__(256) k_ev_init(DevProblem P, KCtx K, EvState S, int32_t* qf,
                                                  unsigned* nqf) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   Scores6 sc{0, 0, 0, 0, 0, 0};
-  EvSink s;
-  s.K = &K;
-  s.P = &P;
-  s.slot = i;
-  s.nsec = 0;
-  s.n_rec = 0;
-  for (int k = 0; k < 8; k++) s.c[k] = 0;
+  GpuSink<TRACE, SEC> s;
+  s.init(&K, &P, i);
   bool live = false;
   if (i < K.n) {
     Part p;
     load_part(P, K.in[i], p);
     Trk t;
+    bind_lds_cursor(t, P);
     int m = -1;
     live = start_history(P, s, p, t, m);
     s.c[0] = 1;
@@ -189,36 +183,35 @@ __global__ void __launch_bounds__(256) k_ev_init(DevProblem P, KCtx K, EvState S
   flush_scores(K, sc, s.c);
 }
 
+template <int TRK, bool TRACE, bool SEC>
 __global__ void __launch_bounds__(256) k_ev_flight(DevProblem P, KCtx K, EvState S,
                                                    const int32_t* qin, const unsigned* nqin,
                                                    int32_t* qf, unsigned* nqf, int32_t* qc,
                                                    unsigned* nqc) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   Scores6 sc{0, 0, 0, 0, 0, 0};
-  EvSink s;
-  for (int c = 0; c < 8; c++) s.c[c] = 0;
-  s.K = &K;
-  s.P = &P;
+  GpuSink<TRACE, SEC> s;
+  s.init(&K, &P, 0);
   bool to_f = false, to_c = false;
   int32_t slot = 0;
   if (k < (int64_t)*nqin) {
     slot = qin[k];
-    ev_sink_init(s, &K, &P, S, slot);
+    ev_resume(s, S, slot);
     Part p;
     Trk t;
+    bind_lds_cursor(t, P);
     int m;
     ev_load(S, slot, p, t, m);
     int ev = 0;
-    const bool hc = step_flight(P, s, p, t, m, K.converged != 0, sc, ev);
+    const bool hc = step_flight<TRK>(P, s, p, t, m, K.converged != 0, sc, ev);
     if (p.alive && hc) {
       to_c = true;
-    } else if (step_finish(P, s, p, t, m, ev)) {
+    } else if (step_finish<TRK>(P, s, p, t, m, ev)) {
       to_f = true;
     }
     if (to_f || to_c) {
       ev_store(S, slot, p, t, m);
-      S.nsec[slot] = s.nsec;
-      S.nrec[slot] = s.n_rec;
+      ev_save(s, S, slot);
     } else {
       ev_finalize(K, s, slot, p);
     }
@@ -228,31 +221,30 @@ __global__ void __launch_bounds__(256) k_ev_flight(DevProblem P, KCtx K, EvState
   flush_scores(K, sc, s.c);
 }
 
+template <int TRK, bool TRACE, bool SEC>
 __global__ void __launch_bounds__(256) k_ev_collide(DevProblem P, KCtx K, EvState S,
                                                     const int32_t* qin, const unsigned* nqin,
                                                     int32_t* qf, unsigned* nqf) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   Scores6 sc{0, 0, 0, 0, 0, 0};
-  EvSink s;
-  for (int c = 0; c < 8; c++) s.c[c] = 0;
-  s.K = &K;
-  s.P = &P;
+  GpuSink<TRACE, SEC> s;
+  s.init(&K, &P, 0);
   bool to_f = false;
   int32_t slot = 0;
   if (k < (int64_t)*nqin) {
     slot = qin[k];
-    ev_sink_init(s, &K, &P, S, slot);
+    ev_resume(s, S, slot);
     Part p;
     Trk t;
+    bind_lds_cursor(t, P);
     int m;
     ev_load(S, slot, p, t, m);
     int ev = 0;
     step_collision(P, s, p, m, K.kcol, K.converged != 0, sc, ev);
-    to_f = step_finish(P, s, p, t, m, ev);
+    to_f = step_finish<TRK>(P, s, p, t, m, ev);
     if (to_f) {
       ev_store(S, slot, p, t, m);
-      S.nsec[slot] = s.nsec;
-      S.nrec[slot] = s.n_rec;
+      ev_save(s, S, slot);
     } else {
       ev_finalize(K, s, slot, p);
     }
@@ -263,25 +255,25 @@ __global__ void __launch_bounds__(256) k_ev_collide(DevProblem P, KCtx K, EvStat
 
 // Finish the remaining histories in place (flight queue first, then the
 // collision queue whose particles have a collision pending).
+template <int TRK, bool TRACE, bool SEC>
 __global__ void __launch_bounds__(256) k_ev_tail(DevProblem P, KCtx K, EvState S,
                                                  const int32_t* qf, const unsigned* nqf,
                                                  const int32_t* qc, const unsigned* nqc) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   Scores6 sc{0, 0, 0, 0, 0, 0};
-  EvSink s;
-  for (int c = 0; c < 8; c++) s.c[c] = 0;
-  s.K = &K;
-  s.P = &P;
+  GpuSink<TRACE, SEC> s;
+  s.init(&K, &P, 0);
   const int64_t nf = *nqf, nc = *nqc;
   if (k < nf + nc) {
     const bool pending = k >= nf;
     const int32_t slot = pending ? qc[k - nf] : qf[k];
-    ev_sink_init(s, &K, &P, S, slot);
+    ev_resume(s, S, slot);
     Part p;
     Trk t;
+    bind_lds_cursor(t, P);
     int m;
     ev_load(S, slot, p, t, m);
-    run_loop(P, s, p, t, m, K.kcol, K.converged != 0, sc, pending);
+    run_loop<TRK>(P, s, p, t, m, K.kcol, K.converged != 0, sc, pending);
     ev_finalize(K, s, slot, p);
   }
   flush_scores(K, sc, s.c);
@@ -314,14 +306,10 @@ struct EventBuffers {
   }
 };
 
-// Problem-wide bound on cursor depth (pads) and coordinate frames.
-struct CursorDepth {
-  int D, L;
-};
-
 static inline size_t ev_state_bytes(int64_t n, int D, int L) {
   return (size_t)n * (12 * sizeof(double) + 3 * sizeof(uint64_t) + 10 * sizeof(int32_t) +
-                      (size_t)D * 4 * sizeof(int32_t) + (size_t)L * 3 * sizeof(double));
+                      (size_t)D * 4 * sizeof(int32_t) + (size_t)L * 3 * sizeof(double)) +
+         40 * 256;
 }
 
 static inline EvState ev_carve(void* base, int64_t n, int D, int L) {
@@ -350,12 +338,15 @@ static inline EvState ev_carve(void* base, int64_t n, int D, int L) {
 // Runs one transport() with the event engine.  `kernel_ms` receives the
 // summed duration of the flight kernels (the dominant kernel), measured with
 // HIP events on `s`, and `launches` their count, when `timing` is set.
-static inline hipError_t event_transport(const DevProblem& P, const KCtx& K, EventBuffers& B,
-                                         hipStream_t s, const CursorDepth& cd, bool timing,
-                                         double* kernel_ms, int64_t* launches) {
+template <int TRK, bool TRACE, bool SEC>
+static hipError_t event_transport_t(const DevProblem& P, const KCtx& K, EventBuffers& B,
+                                    hipStream_t s, bool timing, double* kernel_ms,
+                                    int64_t* launches) {
   hipError_t e;
   const int64_t n = K.n;
-  const int D = cd.D, L = cd.L;
+  const int D = P.trk_D, L = P.trk_L;
+  const int bs = 256;
+  const size_t lds = (size_t)bs * trk_bytes(D, L);
   const size_t need = ev_state_bytes(n, D, L) + 64 * 256;
   if (need > B.state_bytes) {
     if (B.state) (void)hipFree(B.state);
@@ -385,8 +376,8 @@ static inline hipError_t event_transport(const DevProblem& P, const KCtx& K, Eve
   unsigned* cb = B.counts + 1;
   unsigned* cc = B.counts + 2;
   if ((e = hipMemsetAsync(B.counts, 0, 8 * sizeof(unsigned), s)) != hipSuccess) return e;
-  const int bs = 256;
-  hipLaunchKernelGGL(k_ev_init, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, s, P, K, S, qa, ca);
+  hipLaunchKernelGGL((k_ev_init<TRACE, SEC>), dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), lds, s,
+                     P, K, S, qa, ca);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   const int64_t tail = B.tail_threshold > 0 ? B.tail_threshold : std::max<int64_t>(8192, n / 64);
   size_t ev_used = 0;
@@ -405,16 +396,16 @@ static inline hipError_t event_transport(const DevProblem& P, const KCtx& K, Eve
   std::vector<std::pair<hipEvent_t, hipEvent_t>> timed;
   while (nf + nc > 0) {
     if (nf + nc <= tail) {
-      hipLaunchKernelGGL(k_ev_tail, dim3((unsigned)((nf + nc + bs - 1) / bs)), dim3(bs), 0, s, P, K,
-                         S, qa, ca, qc, cc);
+      hipLaunchKernelGGL((k_ev_tail<TRK, TRACE, SEC>), dim3((unsigned)((nf + nc + bs - 1) / bs)),
+                         dim3(bs), lds, s, P, K, S, qa, ca, qc, cc);
       if ((e = hipGetLastError()) != hipSuccess) return e;
       break;
     }
     // collisions first (they refill the flight queue), then flights
     if ((e = hipMemsetAsync(cb, 0, sizeof(unsigned), s)) != hipSuccess) return e;
     if (nc > 0) {
-      hipLaunchKernelGGL(k_ev_collide, dim3((unsigned)((nc + bs - 1) / bs)), dim3(bs), 0, s, P, K,
-                         S, qc, cc, qa, ca);
+      hipLaunchKernelGGL((k_ev_collide<TRK, TRACE, SEC>), dim3((unsigned)((nc + bs - 1) / bs)),
+                         dim3(bs), lds, s, P, K, S, qc, cc, qa, ca);
       if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     const int64_t nf_all = nf + nc;  // upper bound on the flight queue
@@ -425,8 +416,8 @@ static inline hipError_t event_transport(const DevProblem& P, const KCtx& K, Eve
       t1 = get_event();
       (void)hipEventRecord(t0, s);
     }
-    hipLaunchKernelGGL(k_ev_flight, dim3((unsigned)((nf_all + bs - 1) / bs)), dim3(bs), 0, s, P, K,
-                       S, qa, ca, qb, cb, qc, cc);
+    hipLaunchKernelGGL((k_ev_flight<TRK, TRACE, SEC>), dim3((unsigned)((nf_all + bs - 1) / bs)),
+                       dim3(bs), lds, s, P, K, S, qa, ca, qb, cb, qc, cc);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (timing) {
       (void)hipEventRecord(t1, s);
@@ -455,4 +446,36 @@ static inline hipError_t event_transport(const DevProblem& P, const KCtx& K, Eve
     *launches = (int64_t)timed.size();
   }
   return hipSuccess;
+}
+
+template <int TRK, bool TRACE, bool SEC>
+static hipError_t history_transport_t(const DevProblem& P, const KCtx& K, hipStream_t s) {
+  const int bs = 256;
+  const size_t lds = (size_t)bs * trk_bytes(P.trk_D, P.trk_L);
+  hipLaunchKernelGGL((k_history<TRK, TRACE, SEC>), dim3((unsigned)((K.n + bs - 1) / bs)), dim3(bs),
+                     lds, s, P, K);
+  return hipGetLastError();
+}
+
+// Runtime dispatch over the compiled variants (tracking mode x trace x
+// secondary stack).  Carter tracking always carries the stack (splitting).
+#define ABX_DISPATCH(FN, TRK, TRACE, SEC, ...)                                   \
+  (TRK == 0 ? (TRACE ? (SEC ? FN<0, true, true>(__VA_ARGS__) : FN<0, true, false>(__VA_ARGS__))   \
+                     : (SEC ? FN<0, false, true>(__VA_ARGS__) : FN<0, false, false>(__VA_ARGS__))) \
+   : TRK == 1 ? (TRACE ? (SEC ? FN<1, true, true>(__VA_ARGS__) : FN<1, true, false>(__VA_ARGS__)) \
+                       : (SEC ? FN<1, false, true>(__VA_ARGS__) : FN<1, false, false>(__VA_ARGS__))) \
+              : (TRACE ? FN<2, true, true>(__VA_ARGS__) : FN<2, false, true>(__VA_ARGS__)))
+
+static inline hipError_t event_transport(const DevProblem& P, const KCtx& K, EventBuffers& B,
+                                         hipStream_t s, bool sec, bool timing, double* kernel_ms,
+                                         int64_t* launches) {
+  const bool trace = K.trace != nullptr;
+  return ABX_DISPATCH(event_transport_t, P.tracking, trace, sec, P, K, B, s, timing, kernel_ms,
+                      launches);
+}
+
+static inline hipError_t history_transport(const DevProblem& P, const KCtx& K, hipStream_t s,
+                                           bool sec) {
+  const bool trace = K.trace != nullptr;
+  return ABX_DISPATCH(history_transport_t, P.tracking, trace, sec, P, K, s);
 }

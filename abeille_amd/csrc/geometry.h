@@ -105,7 +105,7 @@ ABX_HD double quad_distance(double a, double k, double c, bool on) {
   return d < 0. ? ABX_INF : d;
 }
 
-ABX_HD double surf_distance(const DSurf& s, const V3& r, const V3& u, bool on) {
+ABX_NOINL double surf_distance(const DSurf& s, const V3& r, const V3& u, bool on) {
   switch (s.type) {
     case 0: return plane_axis_distance(s.p[0], r.x, u.x, on);
     case 1: return plane_axis_distance(s.p[0], r.y, u.y, on);
@@ -208,7 +208,7 @@ struct DistTok {
 };
 
 // Cell::distance_to_boundary / distance_to_boundary_condition
-ABX_HD DistTok cell_distance(const DevProblem& P, int ci, const V3& r, const V3& u,
+ABX_NOINL DistTok cell_distance(const DevProblem& P, int ci, const V3& r, const V3& u,
                              int32_t on_surf, bool bc_only) {
   const DCell& c = P.cell[ci];
   DistTok res{ABX_INF, 0};
@@ -302,15 +302,48 @@ ABX_HD bool tile_eq(const int32_t a[3], const int b[3]) {
   return a[0] == b[0] && a[1] == b[1] && a[2] == b[2];
 }
 
+// The cursor proper.  Pads and frames live outside the struct (per-thread
+// slices of LDS on the GPU, a small local buffer on the host) at a stride,
+// so that runtime nesting depth costs neither registers nor scratch.
 struct Trk {
-  int32_t np;                  // pads in use
-  int32_t cell;                // material cell index, -1 = lost
-  int32_t token;               // surface token the particle sits on
-  int32_t err;                 // ABX_E_* raised inside navigation
-  int32_t pad[ABX_MAXPADS];
-  int32_t tile[ABX_MAXPADS][3];  // lattice pads only
-  V3 lv[ABX_MAXLVL];           // frame positions; lv[0] is the global position
+  int32_t np;     // pads in use
+  int32_t cell;   // material cell index, -1 = lost
+  int32_t token;  // surface token the particle sits on
+  int32_t err;    // ABX_E_* raised inside navigation
+  int32_t D, L;   // pad and frame capacity
+  int32_t st;     // element stride
+  int32_t* pad_;  // pad k:        pad_[k * st]
+  int32_t* tile_; // tile (k, c):  tile_[(3 k + c) * st]
+  double* lv_;    // frame (l, c): lv_[(3 l + c) * st]; frame 0 is global
+
+  ABX_HDM int32_t pad(int k) const { return pad_[k * st]; }
+  ABX_HDM int32_t tile(int k, int c) const { return tile_[(3 * k + c) * st]; }
+  ABX_HDM V3 lv(int l) const {
+    return V3{lv_[(3 * l) * st], lv_[(3 * l + 1) * st], lv_[(3 * l + 2) * st]};
+  }
+  ABX_HDM void set_lv(int l, const V3& v) {
+    lv_[(3 * l) * st] = v.x;
+    lv_[(3 * l + 1) * st] = v.y;
+    lv_[(3 * l + 2) * st] = v.z;
+  }
+  ABX_HDM bool tile_is(int k, const int t[3]) const {
+    return tile(k, 0) == t[0] && tile(k, 1) == t[1] && tile(k, 2) == t[2];
+  }
 };
+
+// Bytes of cursor storage per lane for capacities (D, L).
+ABX_HD int trk_bytes(int D, int L) { return 16 * D + 24 * L; }
+
+// Point a cursor at a block of `stride` lanes' storage (lane `lane`).
+ABX_HD void trk_bind(Trk& t, void* base, int stride, int lane, int D, int L) {
+  int32_t* b = (int32_t*)base;
+  t.D = D;
+  t.L = L;
+  t.st = stride;
+  t.pad_ = b + lane;
+  t.tile_ = b + (size_t)D * stride + lane;
+  t.lv_ = (double*)(b + (size_t)4 * D * stride) + lane;
+}
 
 ABX_HD bool tile_in_range(const DUniv& L, const int t[3]) {
   return !(t[0] < 0 || t[0] >= L.Nx || t[1] < 0 || t[1] >= L.Ny || t[2] < 0 ||
@@ -318,22 +351,22 @@ ABX_HD bool tile_in_range(const DUniv& L, const int t[3]) {
 }
 
 ABX_HD bool trk_push(Trk& t, int32_t w, const int* tile) {
-  if (t.np >= ABX_MAXPADS) {
+  if (t.np >= t.D) {
     t.err = -3;
     return false;
   }
-  t.pad[t.np] = w;
+  t.pad_[t.np * t.st] = w;
   if (tile) {
-    t.tile[t.np][0] = tile[0];
-    t.tile[t.np][1] = tile[1];
-    t.tile[t.np][2] = tile[2];
+    t.tile_[(3 * t.np) * t.st] = tile[0];
+    t.tile_[(3 * t.np + 1) * t.st] = tile[1];
+    t.tile_[(3 * t.np + 2) * t.st] = tile[2];
   }
   t.np++;
   return true;
 }
 
 // Universe::get_cell with the stack, iteratively; r is frame `lvl`.
-ABX_HD int trk_descend(const DevProblem& P, Trk& t, int ui, int lvl, V3 r, const V3& u) {
+ABX_NOINL int trk_descend(const DevProblem& P, Trk& t, int ui, int lvl, V3 r, const V3& u) {
   for (int guard = 0; guard < 4 * ABX_MAXPADS; guard++) {
     const DUniv& U = P.univ[ui];
     if (U.type == 0) {
@@ -360,13 +393,13 @@ ABX_HD int trk_descend(const DevProblem& P, Trk& t, int ui, int lvl, V3 r, const
       elem = P.elems[U.elems_off + tl[2] * (U.Nx * U.Ny) + tl[0] * U.Ny + tl[1]];
     if (!trk_push(t, pad_word(PAD_L, lvl, ui), tl)) return -1;
     if (elem >= 0) {
-      if (lvl + 1 >= ABX_MAXLVL) {
+      if (lvl + 1 >= t.L) {
         t.err = -3;
         return -1;
       }
       r = vsub(r, tile_center(U, tl[0], tl[1], tl[2]));
       lvl++;
-      t.lv[lvl] = r;
+      t.set_lv(lvl, r);
       ui = elem;
       continue;
     }
@@ -381,26 +414,26 @@ ABX_HD int trk_descend(const DevProblem& P, Trk& t, int ui, int lvl, V3 r, const
 }
 
 // Tracker::restart_get_current from the global position lv[0].
-ABX_HD void trk_restart(const DevProblem& P, Trk& t, const V3& u) {
+ABX_NOINL void trk_restart(const DevProblem& P, Trk& t, const V3& u) {
   t.np = 0;
-  t.cell = trk_descend(P, t, P.root, 0, t.lv[0], u);
+  t.cell = trk_descend(P, t, P.root, 0, t.lv(0), u);
 }
 
 ABX_HD void trk_init(const DevProblem& P, Trk& t, const V3& r, const V3& u, int32_t token) {
   t.err = 0;
   t.token = token;
-  t.lv[0] = r;
+  t.set_lv(0, r);
   trk_restart(P, t, u);
 }
 
 // Re-descend after pad `first_bad` (>= 1) went stale (tracker.hpp:276-305).
-ABX_HD void trk_redescend(const DevProblem& P, Trk& t, const V3& u, int first_bad) {
+ABX_NOINL void trk_redescend(const DevProblem& P, Trk& t, const V3& u, int first_bad) {
   t.np = first_bad;
-  const int32_t back = t.pad[t.np - 1];
+  const int32_t back = t.pad(t.np - 1);
   const int lvl = pad_lvl(back);
   const int ui = pad_type(back) == PAD_C ? P.cell[pad_idx(back)].redescend : pad_idx(back);
   t.np--;
-  t.cell = trk_descend(P, t, ui, lvl, t.lv[lvl], u);
+  t.cell = trk_descend(P, t, ui, lvl, t.lv(lvl), u);
   if (t.cell < 0) trk_restart(P, t, u);
 }
 
@@ -408,17 +441,17 @@ ABX_HD void trk_redescend(const DevProblem& P, Trk& t, const V3& u, int first_ba
 ABX_HD void trk_get_current(const DevProblem& P, Trk& t, const V3& u) {
   int first_bad = t.np;
   for (int i = 0; i < t.np; i++) {
-    const int32_t w = t.pad[i];
+    const int32_t w = t.pad(i);
     const int ty = pad_type(w);
     if (ty == PAD_C) {
-      if (!cell_inside(P, pad_idx(w), t.lv[pad_lvl(w)], u, t.token)) {
+      if (!cell_inside(P, pad_idx(w), t.lv(pad_lvl(w)), u, t.token)) {
         first_bad = i;
         break;
       }
     } else if (ty == PAD_L) {
       int tl[3];
-      get_tile(P.univ[pad_idx(w)], t.lv[pad_lvl(w)], u, tl);
-      if (!tile_eq(t.tile[i], tl)) {
+      get_tile(P.univ[pad_idx(w)], t.lv(pad_lvl(w)), u, tl);
+      if (!t.tile_is(i, tl)) {
         first_bad = i;
         break;
       }
@@ -435,10 +468,10 @@ ABX_HD void trk_get_current(const DevProblem& P, Trk& t, const V3& u) {
 // Tracker::set_r(r) followed by get_current(): the stale tree forces a
 // restart, after which the reference still compares against the old tree
 // size (tracker.hpp:237-278).
-ABX_HD void trk_reset_position(const DevProblem& P, Trk& t, const V3& r, const V3& u) {
+ABX_NOINL void trk_reset_position(const DevProblem& P, Trk& t, const V3& r, const V3& u) {
   const int old_np = t.np;
   t.token = 0;
-  t.lv[0] = r;
+  t.set_lv(0, r);
   trk_restart(P, t, u);
   if (old_np == t.np) return;
   if (old_np > t.np) {
@@ -452,10 +485,10 @@ ABX_HD void trk_move(Trk& t, const V3& u, double d) {
   const V3 du = vscale(u, d);
   int maxl = 0;
   for (int i = 0; i < t.np; i++) {
-    const int l = pad_lvl(t.pad[i]);
+    const int l = pad_lvl(t.pad(i));
     maxl = l > maxl ? l : maxl;
   }
-  for (int l = 0; l <= maxl && l < ABX_MAXLVL; l++) t.lv[l] = vadd(t.lv[l], du);
+  for (int l = 0; l <= maxl && l < t.L; l++) t.set_lv(l, vadd(t.lv(l), du));
   t.token = 0;
 }
 
@@ -467,7 +500,7 @@ struct Bound {
 };
 
 // Universe::get_boundary_condition (cell universe / lattice via outer)
-ABX_HD Bound univ_bc(const DevProblem& P, int ui, const V3& r, const V3& u, int32_t on) {
+ABX_NOINL Bound univ_bc(const DevProblem& P, int ui, const V3& r, const V3& u, int32_t on) {
   Bound b{ABX_INF, -1, 0, 0};
   for (int guard = 0; guard < 16; guard++) {
     const DUniv& U = P.univ[ui];
@@ -498,12 +531,12 @@ ABX_HD Bound univ_bc(const DevProblem& P, int ui, const V3& r, const V3& u, int3
 }
 
 // Tracker::get_boundary_condition
-ABX_HD Bound trk_bc(const DevProblem& P, const Trk& t, const V3& u) {
-  if (t.cell < 0) return univ_bc(P, P.root, t.lv[0], u, t.token);
+ABX_NOINL Bound trk_bc(const DevProblem& P, const Trk& t, const V3& u) {
+  if (t.cell < 0) return univ_bc(P, P.root, t.lv(0), u, t.token);
   Bound b{ABX_INF, -1, 0, 0};
   for (int i = 0; i < t.np; i++) {
-    const int32_t w = t.pad[i];
-    const V3& rl = t.lv[pad_lvl(w)];
+    const int32_t w = t.pad(i);
+    const V3 rl = t.lv(pad_lvl(w));
     if (pad_type(w) == PAD_C) {
       const int ci = pad_idx(w);
       if (!P.cell[ci].has_bc) continue;
@@ -529,14 +562,15 @@ ABX_HD Bound trk_bc(const DevProblem& P, const Trk& t, const V3& u) {
 }
 
 // Tracker::get_nearest_boundary
-ABX_HD Bound trk_nearest(const DevProblem& P, const Trk& t, const V3& u) {
+ABX_NOINL Bound trk_nearest(const DevProblem& P, const Trk& t, const V3& u) {
   Bound b = trk_bc(P, t, u);
   for (int i = 0; i < t.np; i++) {
-    const int32_t w = t.pad[i];
-    const V3& rl = t.lv[pad_lvl(w)];
+    const int32_t w = t.pad(i);
+    const V3 rl = t.lv(pad_lvl(w));
     const int ty = pad_type(w);
     if (ty == PAD_L) {
-      const double d = tile_distance(P.univ[pad_idx(w)], rl, u, t.tile[i]);
+      const int32_t tl[3] = {t.tile(i, 0), t.tile(i, 1), t.tile(i, 2)};
+      const double d = tile_distance(P.univ[pad_idx(w)], rl, u, tl);
       if (d < b.d && dabs(d - b.d) > ABX_BOUNDRY_TOL) {
         b.d = d;
         b.btype = 2;

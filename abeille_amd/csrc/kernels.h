@@ -65,6 +65,9 @@ __device__ __forceinline__ void raise_error(const KCtx& K, int code, uint64_t h)
   if (atomicCAS(K.err, 0, code) == 0) *K.err_hist = (unsigned long long)h;
 }
 
+// Global-memory side of the physics for one lane.  TRACE compiles the
+// per-event trace records in; SEC the per-history secondary stack.
+template <bool TRACE, bool SEC>
 struct GpuSink {
   const KCtx* K;
   const DevProblem* P;
@@ -78,16 +81,24 @@ struct GpuSink {
     c[5]++;
   }
   __device__ bool sec_push(const SecRec& q) {
-    if (nsec >= K->sec_cap) return false;
-    K->sec[slot * K->sec_cap + nsec] = q;
-    nsec++;
-    return true;
+    if constexpr (!SEC) {
+      return false;
+    } else {
+      if (nsec >= K->sec_cap) return false;
+      K->sec[slot * K->sec_cap + nsec] = q;
+      nsec++;
+      return true;
+    }
   }
   __device__ bool sec_pop(SecRec& q) {
-    if (nsec == 0) return false;
-    nsec--;
-    q = K->sec[slot * K->sec_cap + nsec];
-    return true;
+    if constexpr (!SEC) {
+      return false;
+    } else {
+      if (nsec == 0) return false;
+      nsec--;
+      q = K->sec[slot * K->sec_cap + nsec];
+      return true;
+    }
   }
   __device__ void bank_fission(const Part& p, const V3& u, double E, double w) {
     const unsigned long long i = atomicAdd(K->n_stage, 1ull);
@@ -110,26 +121,27 @@ struct GpuSink {
     K->stage_daughter[i] = p.daughters;
   }
   __device__ void trace(const Part& p, const Trk& t, int ev) {
-    if (!K->trace) return;
-    if (n_rec < K->trace_cap) {
-      abx_trace_rec r;
-      r.r[0] = p.r.x;
-      r.r[1] = p.r.y;
-      r.r[2] = p.r.z;
-      r.u[0] = p.u.x;
-      r.u[1] = p.u.y;
-      r.u[2] = p.u.z;
-      r.wgt = p.w;
-      r.history_id = p.hid;
-      r.secondary_id = p.sec_id;
-      r.event = ev;
-      r.cell = t.cell;
-      r.material = t.cell >= 0 ? P->cell[t.cell].fill : -1;
-      r.group = p.g;
-      r.rng_calls = (int32_t)p.calls;
-      K->trace[slot * K->trace_cap + n_rec] = r;
+    if constexpr (TRACE) {
+      if (n_rec < K->trace_cap) {
+        abx_trace_rec r;
+        r.r[0] = p.r.x;
+        r.r[1] = p.r.y;
+        r.r[2] = p.r.z;
+        r.u[0] = p.u.x;
+        r.u[1] = p.u.y;
+        r.u[2] = p.u.z;
+        r.wgt = p.w;
+        r.history_id = p.hid;
+        r.secondary_id = p.sec_id;
+        r.event = ev;
+        r.cell = t.cell;
+        r.material = t.cell >= 0 ? P->cell[t.cell].fill : -1;
+        r.group = p.g;
+        r.rng_calls = (int32_t)p.calls;
+        K->trace[slot * K->trace_cap + n_rec] = r;
+      }
+      n_rec++;
     }
-    n_rec++;
   }
   __device__ void error(int code, uint64_t h) { raise_error(*K, code, h); }
   __device__ void count_flight() { c[3]++; }
@@ -138,6 +150,14 @@ struct GpuSink {
   __device__ void count_virtual() { c[2]++; }
   __device__ void count_lost() { c[6]++; }
   __device__ void count_secondary() { c[7]++; }
+  __device__ void init(const KCtx* k, const DevProblem* p, int64_t s) {
+    K = k;
+    P = p;
+    slot = s;
+    nsec = 0;
+    n_rec = 0;
+    for (int i = 0; i < 8; i++) c[i] = 0;
+  }
 };
 
 __device__ __forceinline__ void load_part(const DevProblem& P, const abx_particle& a, Part& p) {
@@ -157,23 +177,28 @@ __device__ __forceinline__ void load_part(const DevProblem& P, const abx_particl
   p.alive = 1;
 }
 
+// Dynamic LDS: every lane's cursor pads / tiles / frames, lane-interleaved.
+extern __shared__ __align__(16) unsigned char abx_lds[];
+
+__device__ __forceinline__ void bind_lds_cursor(Trk& t, const DevProblem& P) {
+  trk_bind(t, abx_lds, blockDim.x, threadIdx.x, P.trk_D, P.trk_L);
+}
+
+template <int TRK, bool TRACE, bool SEC>
 __global__ void __launch_bounds__(256) k_history(DevProblem P, KCtx K) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   Scores6 sc{0, 0, 0, 0, 0, 0};
-  GpuSink s;
-  s.K = &K;
-  s.P = &P;
-  s.slot = i;
-  s.nsec = 0;
-  s.n_rec = 0;
-  for (int k = 0; k < 8; k++) s.c[k] = 0;
+  GpuSink<TRACE, SEC> s;
+  s.init(&K, &P, i);
   if (i < K.n) {
     Part p;
     load_part(P, K.in[i], p);
-    run_history(P, s, p, K.kcol, K.converged != 0, sc);
+    Trk t;
+    bind_lds_cursor(t, P);
+    run_history<TRK>(P, s, p, t, K.kcol, K.converged != 0, sc);
     s.c[0] = 1;
     if (K.daughters) K.daughters[i] = p.daughters;
-    if (K.n_rec) K.n_rec[i] = s.n_rec;
+    if (TRACE && K.n_rec) K.n_rec[i] = s.n_rec;
   }
   flush_scores(K, sc, s.c);
 }
@@ -183,8 +208,10 @@ __global__ void k_sample_sources(DevProblem P, DevSources S, uint64_t first, int
                                  unsigned long long* err_hist) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  Trk t;
+  bind_lds_cursor(t, P);
   abx_particle a;
-  const int rc = sample_source(P, S, first + (uint64_t)i, a);
+  const int rc = sample_source(P, S, first + (uint64_t)i, t, a);
   if (rc) {
     if (atomicCAS(err, 0, rc) == 0) *err_hist = first + (uint64_t)i;
     return;
@@ -226,8 +253,6 @@ __global__ void k_add_block_offsets(uint64_t* out, const uint64_t* block_off, in
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] += block_off[blockIdx.x];
 }
-
-__global__ void k_u64_to_u32_dummy() {}
 
 // scatter staged fission sites to parent order
 __global__ void k_order_bank(const abx_banked* stage, const uint32_t* parent,
@@ -298,4 +323,73 @@ __global__ void k_rebank(const abx_banked* b, int64_t n, double w_per_part, uint
   p.family_id = keep_family ? s.family_id : 0;
   p.rng_state = pcg_history_state(seed, stride, p.history_id);
   out[i] = p;
+}
+
+// ---------------------------------------------------------- cancellation
+// ApproximateMeshCancelator (src/approximate_mesh_cancelator.cpp:97-190) on a
+// device bank: key = (bin << 32) | index, sorted, then one lane per bin sums
+// the bin in bank order and, if the bin is mixed-sign with more than one
+// particle, writes the mean back.
+struct DevCancel {
+  double lx, ly, lz, dx, dy, dz;
+  int32_t nx, ny, nz, ne;  // ne = number of energy bins (1 without edges)
+  int32_t n_edges;         // 0 = no energy binning
+  const double* edges;
+};
+
+__global__ void k_cancel_keys(const abx_banked* b, int64_t n, DevCancel C, uint64_t* keys) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const abx_banked& p = b[t];
+  const int i = (int)floor((p.r[0] - C.lx) / C.dx);
+  const int j = (int)floor((p.r[1] - C.ly) / C.dy);
+  const int k = (int)floor((p.r[2] - C.lz) / C.dz);
+  int l = -1;
+  if (C.n_edges > 0) {
+    for (int e = 0; e < C.n_edges - 1; e++)
+      if (C.edges[e] <= p.E && p.E <= C.edges[e + 1]) {
+        l = e;
+        break;
+      }
+  } else {
+    l = 0;
+  }
+  uint64_t key = 0xffffffffull;  // outside: its own bin past the last one
+  if (!(i < 0 || j < 0 || k < 0 || l < 0) && !(i >= C.nx || j >= C.ny || k >= C.nz || l >= C.ne))
+    key = (uint64_t)(l + C.ne * (k + C.nz * (j + C.ny * i)));
+  keys[t] = (key << 32) | (uint64_t)t;
+}
+
+__global__ void k_cancel_apply(const uint64_t* keys, int64_t n, abx_banked* b) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const uint64_t bin = keys[t] >> 32;
+  if (bin == 0xffffffffull) return;
+  if (t > 0 && (keys[t - 1] >> 32) == bin) return;  // not the first of its bin
+  int64_t e = t;
+  bool p1 = false, n1 = false, p2 = false, n2 = false;
+  double s1 = 0., s2 = 0.;
+  while (e < n && (keys[e] >> 32) == bin) {
+    const abx_banked& q = b[keys[e] & 0xffffffffull];
+    if (q.wgt > 0.)
+      p1 = true;
+    else if (q.wgt < 0.)
+      n1 = true;
+    s1 += q.wgt;
+    if (q.wgt2 > 0.)
+      p2 = true;
+    else if (q.wgt2 < 0.)
+      n2 = true;
+    s2 += q.wgt2;
+    e++;
+  }
+  const int64_t cnt = e - t;
+  if (cnt <= 1) return;
+  const double N = (double)cnt;
+  const double a1 = s1 / N, a2 = s2 / N;
+  for (int64_t f = t; f < e; f++) {
+    abx_banked& q = b[keys[f] & 0xffffffffull];
+    if (p1 && n1) q.wgt = a1;
+    if (p2 && n2) q.wgt2 = a2;
+  }
 }

@@ -65,7 +65,7 @@ ABX_HD int energy_index(const DevProblem& P, double E) {
 }
 
 // Direction(mu, phi) with clamping, then normalisation
-ABX_HD V3 dir_mu_phi(double mu, double phi) {
+ABX_NOINL V3 dir_mu_phi(double mu, double phi) {
   if (mu < -1.)
     mu = -1.;
   else if (mu > 1.)
@@ -81,7 +81,7 @@ ABX_HD V3 dir_mu_phi(double mu, double phi) {
   return vdir(x, y, mu);
 }
 
-ABX_HD V3 rotate_dir(const V3& u, double mu, double phi) {
+ABX_NOINL V3 rotate_dir(const V3& u, double mu, double phi) {
   double sn, cs;
   dm_sincos(phi, &sn, &cs);
   const double sqrt_mu = sqrt(1. - mu * mu);
@@ -100,7 +100,7 @@ ABX_HD V3 rotate_dir(const V3& u, double mu, double phi) {
   return vdir(ux, uy, uz);
 }
 
-ABX_HD double sample_mu(const DevProblem& P, Part& p, int m, int gi, int go) {
+ABX_NOINL double sample_mu(const DevProblem& P, Part& p, int m, int gi, int go) {
   const int k = (m * P.G + gi) * P.G + go;
   const int b0 = P.ang_off[k], n = P.ang_off[k + 1] - b0;
   const double* mu = P.ang_mu + b0;
@@ -135,7 +135,7 @@ ABX_HD double quantity_factor(const DevProblem& P, const DTally& T, const Part& 
 }
 
 template <class Sink>
-ABX_HD void score_collision_tallies(const DevProblem& P, Sink& s, const Part& p, int m) {
+ABX_NOINL void score_collision_tallies(const DevProblem& P, Sink& s, const Part& p, int m) {
   const int mg = m * P.G + p.g;
   for (int ti = 0; ti < P.ntally; ti++) {
     const DTally& T = P.tally[ti];
@@ -210,7 +210,7 @@ ABX_HD bool tle_entry(const DTally& T, V3& r, const V3& u, double& d) {
 }
 
 template <class Sink>
-ABX_HD void score_flight_tallies(const DevProblem& P, Sink& s, const Part& p, double d,
+ABX_NOINL void score_flight_tallies(const DevProblem& P, Sink& s, const Part& p, double d,
                                  int m) {
   const int mg = m * P.G + p.g;
   for (int ti = 0; ti < P.ntally; ti++) {
@@ -286,7 +286,7 @@ ABX_HD void collide(const DevProblem& P, Sink& s, Part& p, int m, double kcol,
                     bool converged, Scores6& sc) {
   const int G = P.G;
   const int mg = m * G + p.g;
-  if (converged) score_collision_tallies(P, s, p, m);
+  if (converged && P.ntally) score_collision_tallies(P, s, p, m);
   {
     const double k_col_scr = p.w * P.vEf[mg] / P.Et[mg];
     const V3 dr = vsub(p.r, p.rb);
@@ -376,7 +376,7 @@ ABX_HD void leak(Part& p, const Bound& b, Scores6& sc) {
 
 // Tracker::do_reflection; the particle's position before the reflection
 // only feeds Particle::previous_r, which this path does not keep.
-ABX_HD void reflect(const DevProblem& P, Trk& t, Part& p, const Bound& b) {
+ABX_NOINL void reflect(const DevProblem& P, Trk& t, Part& p, const Bound& b) {
   if (b.sidx < 0) {
     t.err = -3;
     return;
@@ -388,12 +388,12 @@ ABX_HD void reflect(const DevProblem& P, Trk& t, Part& p, const Bound& b) {
   p.r = r_on;
   p.u = vdir(nd.x, nd.y, nd.z);
   t.token = 0;
-  t.lv[0] = p.r;
+  t.set_lv(0, p.r);
   trk_restart(P, t, p.u);
 }
 
 template <class Sink>
-ABX_HD bool try_resurrect(const DevProblem& P, Sink& s, Part& p, Trk& t, int& m) {
+ABX_NOINL bool try_resurrect(const DevProblem& P, Sink& s, Part& p, Trk& t, int& m) {
   SecRec q;
   if (!s.sec_pop(q)) return false;
   p.alive = 1;
@@ -405,7 +405,7 @@ ABX_HD bool try_resurrect(const DevProblem& P, Sink& s, Part& p, Trk& t, int& m)
   p.w2 = q.w2;
   p.sec_id++;
   t.token = 0;
-  t.lv[0] = p.r;
+  t.set_lv(0, p.r);
   trk_restart(P, t, p.u);
   if (t.cell < 0 || t.err) {
     s.error(-3, p.hid);
@@ -420,7 +420,7 @@ ABX_HD bool try_resurrect(const DevProblem& P, Sink& s, Part& p, Trk& t, int& m)
 
 // Carter-tracking weight split (src/carter_tracker.cpp:219-223)
 template <class Sink>
-ABX_HD void split(Sink& s, Part& p, const DevProblem& P) {
+ABX_NOINL void split(Sink& s, Part& p, const DevProblem& P) {
   if (p.alive && dabs(p.w) >= P.wgt_split) {
     const int n_new = (int)ceil(dabs(p.w));
     if (n_new > 1) {
@@ -460,17 +460,16 @@ ABX_HD bool start_history(const DevProblem& P, Sink& s, Part& p, Trk& t, int& m)
 // One flight of the transport loop up to the collision decision.  Returns
 // true when a real collision is pending; otherwise `ev` holds the event.
 // Sets p.alive = 0 and returns false after raising an error.
-template <class Sink>
+template <int TRK, class Sink>
 ABX_HD bool step_flight(const DevProblem& P, Sink& s, Part& p, Trk& t, int& m, bool converged,
                         Scores6& sc, int& ev) {
-  const int trk = P.tracking;
-  if (trk == 0) {
+  if (TRK == 0) {
     // ---------------- surface tracking (src/surface_tracker.cpp:70-151)
     const double d_coll = p_exponential(p, P.Et[m * P.G + p.g]);
     s.count_flight();
     const Bound b = trk_nearest(P, t, p.u);
     const double dmin = d_coll < b.d ? d_coll : b.d;
-    if (converged) score_flight_tallies(P, s, p, dmin, m);
+    if (converged && P.ntally) score_flight_tallies(P, s, p, dmin, m);
     sc.k_trk += p.w * dmin * P.vEf[m * P.G + p.g];
     ev = 2;
     if (b.d < d_coll || dabs(b.d - d_coll) < ABX_BOUNDRY_TOL) {
@@ -506,7 +505,7 @@ ABX_HD bool step_flight(const DevProblem& P, Sink& s, Part& p, Trk& t, int& m, b
   }
   // ---------------- delta / carter tracking (src/delta_tracker.cpp:105-195,
   // src/carter_tracker.cpp:124-216)
-  const bool carter = trk == 2;
+  const bool carter = TRK == 2;
   const double Emaj = (carter ? P.sampling[p.g] : P.majorant[p.g]) + 0.;
   const double d_coll = p_exponential(p, Emaj);
   s.count_flight();
@@ -517,15 +516,19 @@ ABX_HD bool step_flight(const DevProblem& P, Sink& s, Part& p, Trk& t, int& m, b
     p.alive = 0;
     return false;
   }
-  if (t.cell < 0) {
+  const bool lost = t.cell < 0;
+  Bound b{ABX_INF, -1, 2, 0};
+  if (lost) {
     trk_reset_position(P, t, p.r, p.u);
     if (t.err) {
       s.error(-3, p.hid);
       p.alive = 0;
       return false;
     }
-    const Bound b = trk_bc(P, t, p.u);
-    if (converged) score_flight_tallies(P, s, p, d_coll < b.d ? d_coll : b.d, m);
+    b = trk_bc(P, t, p.u);
+  }
+  if (converged && P.ntally) score_flight_tallies(P, s, p, d_coll < b.d ? d_coll : b.d, m);
+  if (lost) {
     s.count_crossing();
     if (b.btype == 0) {
       leak<Sink>(p, b, sc);
@@ -544,7 +547,6 @@ ABX_HD bool step_flight(const DevProblem& P, Sink& s, Part& p, Trk& t, int& m, b
     }
     return false;
   }
-  if (converged) score_flight_tallies(P, s, p, d_coll, m);
   ev = 0;
   p.r = vadd(p.r, vscale(p.u, d_coll));
   m = P.cell[t.cell].fill;
@@ -584,16 +586,16 @@ ABX_HD void step_collision(const DevProblem& P, Sink& s, Part& p, int m, double 
 
 // End of a loop iteration: carter split, trace, resurrection.  Returns
 // whether the slot still carries a live particle.
-template <class Sink>
+template <int TRK, class Sink>
 ABX_HD bool step_finish(const DevProblem& P, Sink& s, Part& p, Trk& t, int& m, int ev) {
-  if (P.tracking == 2) split(s, p, P);
+  if (TRK == 2) split(s, p, P);
   s.trace(p, t, ev);
   if (!p.alive) try_resurrect(P, s, p, t, m);
   return p.alive != 0;
 }
 
 // One whole history (plus its secondaries).
-template <class Sink>
+template <int TRK, class Sink>
 ABX_HD void run_loop(const DevProblem& P, Sink& s, Part& p, Trk& t, int& m, double kcol,
                      bool converged, Scores6& sc, bool collision_pending) {
   for (int guard = 0; p.alive; guard++) {
@@ -604,17 +606,16 @@ ABX_HD void run_loop(const DevProblem& P, Sink& s, Part& p, Trk& t, int& m, doub
     int ev = 0;
     bool hc = collision_pending;
     collision_pending = false;
-    if (!hc) hc = step_flight(P, s, p, t, m, converged, sc, ev);
+    if (!hc) hc = step_flight<TRK>(P, s, p, t, m, converged, sc, ev);
     if (p.alive && hc) step_collision(P, s, p, m, kcol, converged, sc, ev);
-    if (!step_finish(P, s, p, t, m, ev)) return;
+    if (!step_finish<TRK>(P, s, p, t, m, ev)) return;
   }
 }
 
-template <class Sink>
-ABX_HD void run_history(const DevProblem& P, Sink& s, Part& p, double kcol, bool converged,
-                        Scores6& sc) {
-  Trk t;
+template <int TRK, class Sink>
+ABX_HD void run_history(const DevProblem& P, Sink& s, Part& p, Trk& t, double kcol,
+                        bool converged, Scores6& sc) {
   int m = -1;
   if (!start_history(P, s, p, t, m)) return;
-  run_loop(P, s, p, t, m, kcol, converged, sc, false);
+  run_loop<TRK>(P, s, p, t, m, kcol, converged, sc, false);
 }

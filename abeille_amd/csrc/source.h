@@ -27,7 +27,7 @@ ABX_HD V3 source_position(const abx_source& s, uint64_t& rng) {
 }
 
 // Returns 0 or an ABX_E_* code.
-ABX_HD int sample_source(const DevProblem& P, const DevSources& S, uint64_t h,
+ABX_HD int sample_source(const DevProblem& P, const DevSources& S, uint64_t h, Trk& t,
                          abx_particle& out) {
   uint64_t rng = pcg_history_state(P.seed, P.stride, h);
   const int idx = S.n >= 2 ? rng_discrete(rng, S.cdf, S.n) : 0;
@@ -43,7 +43,6 @@ ABX_HD int sample_source(const DevProblem& P, const DevSources& S, uint64_t h,
   const double E = s.energy;
   if (E <= P.min_energy || P.max_energy <= E) return -1;  // 200 resamplings fail
   V3 r = source_position(s, rng);
-  Trk t;
   trk_init(P, t, r, u, 0);
   int guard = 0;
   while (t.cell < 0) {

@@ -441,6 +441,8 @@ static inline DevProblem dev_view(const HostTables& T, Map map, double* tally_da
   P.ntally = (int32_t)T.tally.size();
   P.n_dgrp_max = T.n_dgrp_max;
   P.precision = T.st.precision;
+  P.trk_D = T.max_pads;
+  P.trk_L = T.max_levels;
   P.wgt_cutoff = T.st.wgt_cutoff;
   P.wgt_survival = T.st.wgt_survival;
   P.wgt_split = T.st.wgt_split;

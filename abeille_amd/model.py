@@ -24,6 +24,7 @@ class Model:
         self.sources = []       # dict
         self.tallies = []       # dict
         self.root = None
+        self.cancel = None
         self.settings = dict(mode=abi.K_EIGENVALUE, tracking=abi.SURFACE_TRACKING,
                              seed=abi.DEFAULT_SEED, stride=abi.DEFAULT_STRIDE,
                              wgt_cutoff=0.25, wgt_survival=1.0, wgt_split=2.0,
@@ -108,6 +109,13 @@ class Model:
                                  ebounds=np.asarray(energy_bounds, np.float64)))
         return len(self.tallies) - 1
 
+    def cancellation_mesh(self, low, high, shape, energy_bounds=None):
+        """Approximate mesh cancellation of banked particles
+        (src/approximate_mesh_cancelator.cpp)."""
+        self.cancel = dict(low=list(low), high=list(high), shape=list(shape),
+                           ebounds=None if energy_bounds is None
+                           else np.asarray(energy_bounds, np.float64))
+
     def set(self, **kw):
         for k, v in kw.items():
             if k not in self.settings:
@@ -186,10 +194,25 @@ class Model:
             mats[i].n_delayed = n_d
             mats[i].delayed_probabilities = abi.ptr(m["dp"])
             mats[i].delayed_constants = abi.ptr(m["dc"])
-            mats[i].angle_offsets = abi.ptr(None, C.c_int32)
-            mats[i].angle_mu = abi.ptr(None)
-            mats[i].angle_pdf = abi.ptr(None)
-            mats[i].angle_cdf = abi.ptr(None)
+            if m.get("angles"):
+                # piecewise-linear cosine tables per (g_in, g_out)
+                offs = [0]
+                for (mu, _, _) in m["angles"]:
+                    offs.append(offs[-1] + len(mu))
+                offs = np.asarray(offs, np.int32)
+                mu_a = np.concatenate([a[0] for a in m["angles"]])
+                pdf_a = np.concatenate([a[1] for a in m["angles"]])
+                cdf_a = np.concatenate([a[2] for a in m["angles"]])
+                keep += [offs, mu_a, pdf_a, cdf_a]
+                mats[i].angle_offsets = abi.ptr(offs, C.c_int32)
+                mats[i].angle_mu = abi.ptr(mu_a)
+                mats[i].angle_pdf = abi.ptr(pdf_a)
+                mats[i].angle_cdf = abi.ptr(cdf_a)
+            else:
+                mats[i].angle_offsets = abi.ptr(None, C.c_int32)
+                mats[i].angle_mu = abi.ptr(None)
+                mats[i].angle_pdf = abi.ptr(None)
+                mats[i].angle_cdf = abi.ptr(None)
 
         srcs = (abi.Source * max(1, len(self.sources)))()
         for i, s in enumerate(self.sources):
@@ -250,7 +273,18 @@ class Model:
         s.precision = st["precision"]
         s.energy_bounds = abi.ptr(eb)
         s.sample_xs_ratio = abi.ptr(ratio)
+        cm = None
+        if self.cancel is not None:
+            cm = abi.CancelMesh()
+            for k in range(3):
+                cm.low[k] = self.cancel["low"][k]
+                cm.high[k] = self.cancel["high"][k]
+                cm.shape[k] = self.cancel["shape"][k]
+            ceb = self.cancel["ebounds"]
+            cm.n_ebounds = 0 if ceb is None else len(ceb)
+            cm.ebounds = abi.ptr(ceb)
+            d.cancel = C.pointer(cm)
         keep += [surfs, cells, rpn_arr, unis, ucell_arr, elem_arr, mats, srcs, tals, eb,
-                 ratio, d]
+                 ratio, d, cm]
         self._keep = keep
         return d

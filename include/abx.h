@@ -195,6 +195,18 @@ typedef struct abx_settings {
  * distances in FP32; positions, surface tests and tallies FP64. */
 enum abx_precision { ABX_FP64 = 0, ABX_SPLIT = 1 };
 
+/* ---- approximate mesh cancellation (src/approximate_mesh_cancelator.cpp)
+ * Applied to each generation's banked particles (k-eigenvalue: before
+ * weight normalisation, src/power_iterator.cpp:354-356; modified fixed
+ * source: before re-banking). */
+typedef struct abx_cancel_mesh {
+  double low[3];
+  double high[3];
+  int32_t shape[3];
+  int32_t n_ebounds; /* 0 = no energy binning */
+  const double* ebounds;
+} abx_cancel_mesh;
+
 typedef struct abx_problem_desc {
   int32_t n_surfaces;
   const abx_surface* surfaces;
@@ -216,6 +228,7 @@ typedef struct abx_problem_desc {
   int32_t n_tallies;
   const abx_mesh_tally* tallies;
   abx_settings settings;
+  const abx_cancel_mesh* cancel; /* NULL = no cancellation */
 } abx_problem_desc;
 
 /* ---- particles at the boundary ----------------------------------------- */
@@ -312,6 +325,12 @@ int abx_tally_size(abx_problem* p, int tally, size_t* n_bins);
 int abx_tally_fetch(abx_problem* p, int tally, double* gen_out);
 /* MeshTally::clear_generation / Tallies::clear_generation */
 int abx_tally_clear(abx_problem* p);
+
+/* ApproximateMeshCancelator::add_particle for every particle of `bank`, then
+ * perform_cancellation (src/approximate_mesh_cancelator.cpp:97-190), on the
+ * GPU; `bank` (host memory) is updated in place.  Needs a cancellation mesh
+ * in the description. */
+int abx_cancel_bank(abx_problem* p, abx_banked* bank, size_t n);
 
 /* ---- device-resident drivers (the reference's Simulation loops) ---------- */
 

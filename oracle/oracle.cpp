@@ -23,6 +23,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <random>
 #include <string>
@@ -527,6 +528,12 @@ struct Problem {
          leak_avg = 0, leak_var = 0, mig_avg = 0, mig_var = 0;
   std::vector<abx_particle> last_bank;
   abx_diag diag_total{};
+  // ApproximateMeshCancelator (src/approximate_mesh_cancelator.cpp:32-96)
+  bool cancel = false;
+  V3 c_low{0, 0, 0}, c_hi{0, 0, 0};
+  uint32_t c_shape[4] = {1, 1, 1, 1};
+  double c_dx = 0, c_dy = 0, c_dz = 0;
+  std::vector<double> c_edges;
   int universe_by_id(uint32_t id) const {
     for (size_t i = 0; i < universes.size(); i++)
       if (universes[i].id == id) return (int)i;
@@ -1809,6 +1816,23 @@ static void build(Problem& P, const abx_problem_desc* d) {
     for (int g = 0; g < G; g++) P.sampling[(size_t)g] = P.majorant[(size_t)g] * P.ratio[(size_t)g];
   }
   for (int i = 0; i < d->n_sources; i++) P.sources.push_back({d->sources[i]});
+  if (d->cancel) {
+    const abx_cancel_mesh& c = *d->cancel;
+    P.cancel = true;
+    P.c_low = {c.low[0], c.low[1], c.low[2]};
+    P.c_hi = {c.high[0], c.high[1], c.high[2]};
+    if (P.c_low.x >= P.c_hi.x || P.c_low.y >= P.c_hi.y || P.c_low.z >= P.c_hi.z)
+      fatal(ABX_E_INVALID, "Low position is not lower than hi position in ApproximateMeshCancelator.");
+    for (int k = 0; k < 3; k++) P.c_shape[k] = (uint32_t)c.shape[k];
+    P.c_dx = (P.c_hi.x - P.c_low.x) / (double)P.c_shape[0];
+    P.c_dy = (P.c_hi.y - P.c_low.y) / (double)P.c_shape[1];
+    P.c_dz = (P.c_hi.z - P.c_low.z) / (double)P.c_shape[2];
+    if (c.n_ebounds > 0) {
+      if (c.n_ebounds < 2) fatal(ABX_E_INVALID, "energy_edges must have at least two entries");
+      P.c_edges.assign(c.ebounds, c.ebounds + c.n_ebounds);
+      P.c_shape[3] = (uint32_t)(P.c_edges.size() - 1);
+    }
+  }
   for (int i = 0; i < d->n_tallies; i++) {
     const abx_mesh_tally& a = d->tallies[i];
     MeshTally T;
@@ -1979,6 +2003,62 @@ static void add_diag(abx_diag& a, const abx_diag& b) {
   a.secondaries += b.secondaries;
 }
 
+// ApproximateMeshCancelator::add_particle + perform_cancellation
+// (src/approximate_mesh_cancelator.cpp:97-190): bin by (i, j, k, e), and in
+// every bin holding more than one particle with weights of both signs set
+// each weight to the bin mean (w and w2 independently).  Bins keep the bank
+// order, so the sums run in bank order.
+static void approximate_cancellation(const Problem& P, std::vector<abx_banked>& bank) {
+  std::map<int, std::vector<size_t>> bins;
+  for (size_t n = 0; n < bank.size(); n++) {
+    const abx_banked& p = bank[n];
+    int i = (int)std::floor((p.r[0] - P.c_low.x) / P.c_dx);
+    int j = (int)std::floor((p.r[1] - P.c_low.y) / P.c_dy);
+    int k = (int)std::floor((p.r[2] - P.c_low.z) / P.c_dz);
+    int l = -1;
+    if (!P.c_edges.empty()) {
+      for (size_t e = 0; e < P.c_edges.size() - 1; e++)
+        if (P.c_edges[e] <= p.E && p.E <= P.c_edges[e + 1]) {
+          l = (int)e;
+          break;
+        }
+    } else {
+      l = 0;
+    }
+    if (i < 0 || j < 0 || k < 0 || l < 0) continue;
+    if (i >= (int)P.c_shape[0] || j >= (int)P.c_shape[1] || k >= (int)P.c_shape[2] ||
+        l >= (int)P.c_shape[3])
+      continue;
+    const int key = l + (int)P.c_shape[3] * (k + (int)P.c_shape[2] * (j + (int)P.c_shape[1] * i));
+    bins[key].push_back(n);
+  }
+  for (auto& kv : bins) {
+    auto& bin = kv.second;
+    if (bin.size() <= 1) continue;
+    bool pos1 = false, neg1 = false, pos2 = false, neg2 = false;
+    double sum1 = 0., sum2 = 0.;
+    for (size_t n : bin) {
+      const abx_banked& p = bank[n];
+      if (p.wgt > 0.)
+        pos1 = true;
+      else if (p.wgt < 0.)
+        neg1 = true;
+      sum1 += p.wgt;
+      if (p.wgt2 > 0.)
+        pos2 = true;
+      else if (p.wgt2 < 0.)
+        neg2 = true;
+      sum2 += p.wgt2;
+    }
+    const double N = (double)bin.size();
+    const double avg1 = sum1 / N, avg2 = sum2 / N;
+    for (size_t n : bin) {
+      if (pos1 && neg1) bank[n].wgt = avg1;
+      if (pos2 && neg2) bank[n].wgt2 = avg2;
+    }
+  }
+}
+
 // FixedSource::run, ModifiedFixedSource::run and PowerIterator::run
 static void run(Problem& P, int ngen, int nignored, int nthreads,
                 double* kcol_hist, abx_run_stats& S) {
@@ -2007,6 +2087,7 @@ static void run(Problem& P, int ngen, int nignored, int nthreads,
       transport(P, bank.data(), bank.size(), kcol, true, nthreads, next, sc, dg,
                 nullptr, 0, nullptr);
       while (P.st.mode == ABX_MODIFIED_FIXED_SOURCE && !next.empty()) {
+        if (P.cancel) approximate_cancellation(P, next);
         bank.clear();
         for (auto& b : next) {  // src/modified_fixed_source.cpp:88-93
           abx_particle a{};
@@ -2041,6 +2122,7 @@ static void run(Problem& P, int ngen, int nignored, int nthreads,
       if (next.empty()) fatal(ABX_E_INVALID, "No fission neutrons were produced.");
       double kc = sc.k_col / W;
       if (kcol_hist) kcol_hist[g - 1] = kc;
+      if (P.cancel) approximate_cancellation(P, next);
       // normalize_weights (src/power_iterator.cpp:538-586)
       double W_pos = 0., W_neg = 0.;
       for (auto& b : next) {
@@ -2255,6 +2337,15 @@ int orc_nearest_boundary(orc_problem* p, const double* r, const double* u,
       surface[i] = b.surface_index;
       btype[i] = b.btype;
     }
+  });
+}
+
+int orc_cancel(orc_problem* p, abx_banked* bank, size_t n) {
+  ORC_TRY({
+    if (!p->P.cancel) fatal(ABX_E_INVALID, "no cancellation mesh");
+    std::vector<abx_banked> b(bank, bank + n);
+    approximate_cancellation(p->P, b);
+    std::copy(b.begin(), b.end(), bank);
   });
 }
 

@@ -56,6 +56,9 @@ int orc_nearest_boundary(orc_problem* p, const double* r, const double* u,
                          size_t n, double* dist, int32_t* surface,
                          int32_t* btype);
 
+/* Approximate mesh cancellation of a bank, in place. */
+int orc_cancel(orc_problem* p, abx_banked* bank, size_t n);
+
 /* Majorant per group (src/majorant.cpp:133-176). */
 int orc_majorant(orc_problem* p, double* maj);
 

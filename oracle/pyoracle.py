@@ -49,6 +49,7 @@ def lib():
         L.orc_nearest_boundary.argtypes = [P, C.c_void_p, C.c_void_p, C.c_size_t,
                                            C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_majorant.argtypes = [P, C.c_void_p]
+        L.orc_cancel.argtypes = [P, C.c_void_p, C.c_size_t]
         for f in ("orc_log", "orc_sin", "orc_cos"):
             getattr(L, f).argtypes = [C.c_int, C.c_double]
             getattr(L, f).restype = C.c_double
@@ -154,6 +155,11 @@ class Oracle:
         _check(lib().orc_nearest_boundary(self.h, r.ctypes.data, u.ctypes.data, n,
                                           d.ctypes.data, s.ctypes.data, b.ctypes.data))
         return d, s, b
+
+    def cancel(self, bank):
+        b = np.ascontiguousarray(bank, dtype=abi.BANKED_DTYPE).copy()
+        _check(lib().orc_cancel(self.h, b.ctypes.data, len(b)))
+        return b
 
     def majorant(self):
         out = np.zeros(self.model.ngroups)

@@ -109,9 +109,18 @@ int hd_create(const abx_problem_desc* d, hd_problem** out) {
 
 void hd_destroy(hd_problem* h) { delete h; }
 
+struct HostCursor {
+  std::vector<unsigned char> buf;
+  Trk t;
+  explicit HostCursor(const DevProblem& P) : buf((size_t)trk_bytes(P.trk_D, P.trk_L) + 16) {
+    trk_bind(t, buf.data(), 1, 0, P.trk_D, P.trk_L);
+  }
+};
+
 int hd_sample_sources(hd_problem* h, uint64_t first, size_t n, abx_particle* out) {
+  HostCursor c(h->P);
   for (size_t i = 0; i < n; i++) {
-    int rc = sample_source(h->P, h->S, first + i, out[i]);
+    int rc = sample_source(h->P, h->S, first + i, c.t, out[i]);
     if (rc) return rc;
   }
   return 0;
@@ -144,7 +153,12 @@ int hd_transport(hd_problem* h, const abx_particle* bank, size_t n, double kcol,
     p.hid = a.history_id;
     p.fam = a.family_id;
     p.alive = 1;
-    run_history(h->P, s, p, kcol, converged != 0, sc);
+    HostCursor cur(h->P);
+    switch (h->P.tracking) {
+      case 0: run_history<0>(h->P, s, p, cur.t, kcol, converged != 0, sc); break;
+      case 1: run_history<1>(h->P, s, p, cur.t, kcol, converged != 0, sc); break;
+      default: run_history<2>(h->P, s, p, cur.t, kcol, converged != 0, sc); break;
+    }
     if (s.err) {
       g_err = "history " + std::to_string(s.err_hist) + " failed";
       return s.err;
