@@ -73,7 +73,8 @@ struct abx_problem {
   int tally_g = 0;
   // transport scratch
   DBuf in, sec, stage, stage_parent, stage_daughter, n_stage, daughters, offsets,
-      block_sums, block_sums2, ordered, scores, counters, err, err_hist, trace, n_rec, wsum;
+      block_sums, block_sums2, ordered, scores, counters, err, err_hist, trace, n_rec, wsum,
+      pool_next;
   EventBuffers ev;
   // cancellation
   bool cancel = false;
@@ -112,7 +113,7 @@ struct abx_problem {
     DBuf* all[] = {&tally_gen, &tally_avg, &tally_var, &in, &sec, &stage, &stage_parent,
                    &stage_daughter, &n_stage, &daughters, &offsets, &block_sums, &block_sums2,
                    &ordered, &scores, &counters, &err, &err_hist, &trace, &n_rec, &wsum,
-                   &bank_a, &bank_b, &ckeys, &ckeys2, &ctemp};
+                   &bank_a, &bank_b, &ckeys, &ckeys2, &ctemp, &pool_next};
     for (DBuf* b : all) b->release();
     ev.release();
     if (ke0) (void)hipEventDestroy(ke0);
@@ -189,6 +190,8 @@ TransportOut transport_device(abx_problem* p, const abx_particle* d_in, int64_t 
                               int32_t* d_nrec) {
   TransportOut res;
   const HostTables& T = p->T;
+  if (d_trace && T.st.precision == ABX_SPLIT)
+    fail(ABX_E_INVALID, "event traces are recorded in FP64 precision only");
   p->scores.reserve(8 * sizeof(double));
   p->counters.reserve(8 * sizeof(unsigned long long));
   p->err.reserve(sizeof(int32_t));
@@ -236,7 +239,8 @@ TransportOut transport_device(abx_problem* p, const abx_particle* d_in, int64_t 
     }
     if (p->engine == ABX_ENGINE_HISTORY) {
       if (p->timing) HIPC(hipEventRecord(p->ke0, p->stream));
-      HIPC(history_transport(p->P, K, p->stream, sec_cap > 0));
+      p->pool_next.reserve(sizeof(unsigned long long));
+      HIPC(history_transport(p->P, K, p->stream, sec_cap > 0, p->pool_next.as<unsigned long long>()));
       if (p->timing) {
         HIPC(hipEventRecord(p->ke1, p->stream));
         HIPC(hipEventSynchronize(p->ke1));
@@ -603,8 +607,12 @@ static void run_generations(abx_problem* p, int ngenerations, int nignored, doub
   HIPC(hipEventCreate(&e1));
   float ms_total = 0.f;
   int64_t histories = 0;
-  p->bank_a.reserve((size_t)N * sizeof(abx_particle));
-  if (T.st.mode == ABX_K_EIGENVALUE && !p->run_started) {
+  // bank_a holds the current k-eigenvalue source between calls: it is only
+  // (re)allocated when sources are sampled into it.
+  if (T.st.mode != ABX_K_EIGENVALUE) {
+    p->bank_a.reserve((size_t)N * sizeof(abx_particle));
+  } else if (!p->run_started) {
+    p->bank_a.reserve((size_t)N * sizeof(abx_particle));
     sample_sources_device(p, 0, N, p->bank_a.as<abx_particle>());
     p->run_hist = (uint64_t)N;
     p->bank_n = (size_t)N;

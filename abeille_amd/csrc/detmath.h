@@ -20,11 +20,16 @@
 // the ranges the transport path uses).
 #pragma once
 
+#include <math.h>
 #include <stdint.h>
 
 #if defined(__HIPCC__)
 #define DM_HD __host__ __device__ __forceinline__
+#if defined(ABX_OUTLINE) && ABX_OUTLINE
 #define DM_NOINL __host__ __device__ __attribute__((noinline))
+#else
+#define DM_NOINL __host__ __device__ __forceinline__
+#endif
 #else
 #define DM_HD static inline
 #define DM_NOINL static
@@ -202,3 +207,26 @@ DM_HD double dm_cos(double x) {
   dm_sincos(x, &s, &c);
   return c;
 }
+
+// Single-precision helpers of the split-precision path.  On the device they
+// map to the hardware's fast FP32 instructions (log2 / sin / cos / rcp /
+// sqrt with ~1 ulp error); on the host to libm's float functions.  Results
+// differ between the two in the last bits, which the split path tolerates.
+#if defined(__HIP_DEVICE_COMPILE__)
+DM_HD float fp_log(float x) { return __logf(x); }
+DM_HD void fp_sincos(float x, float* s, float* c) { __sincosf(x, s, c); }
+DM_HD float fp_div(float a, float b) { return __fdividef(a, b); }
+DM_HD float fp_rcp(float x) { return __frcp_rn(x); }
+DM_HD float fp_sqrt(float x) { return __fsqrt_rn(x); }
+DM_HD float fp_rsqrt(float x) { return __frsqrt_rn(x); }
+#else
+DM_HD float fp_log(float x) { return logf(x); }
+DM_HD void fp_sincos(float x, float* s, float* c) {
+  *s = sinf(x);
+  *c = cosf(x);
+}
+DM_HD float fp_div(float a, float b) { return a / b; }
+DM_HD float fp_rcp(float x) { return 1.0f / x; }
+DM_HD float fp_sqrt(float x) { return sqrtf(x); }
+DM_HD float fp_rsqrt(float x) { return 1.0f / sqrtf(x); }
+#endif

@@ -15,12 +15,22 @@
 
 #if defined(__HIPCC__)
 #define ABX_HD __host__ __device__ __forceinline__
-// Out of line on the device: rare or bulky paths kept out of the hot loop so
-// the kernels' code stays within the instruction cache.
+// Bulky device paths.  Inlined by default: an out-of-line call on AMDGPU
+// follows the call ABI (callee-saved VGPRs spilled to a scratch frame), which
+// cost more than the instruction-cache misses it saved.  -DABX_OUTLINE=1
+// builds them out of line.
+#if defined(ABX_OUTLINE) && ABX_OUTLINE
 #define ABX_NOINL __host__ __device__ __attribute__((noinline))
+#else
+#define ABX_NOINL __host__ __device__ __forceinline__
+#endif
+// Rarely executed paths: always out of line on the device, so that their
+// code is not copied into every inlined caller.
+#define ABX_COLD __host__ __device__ __attribute__((noinline))
 #else
 #define ABX_HD static inline
 #define ABX_NOINL static
+#define ABX_COLD static
 #endif
 
 // Member functions (no `static` on the host).
@@ -28,6 +38,14 @@
 #define ABX_HDM __host__ __device__ __forceinline__
 #else
 #define ABX_HDM inline
+#endif
+
+// Transport kernels: 256-lane blocks; -DABX_WAVES_PER_EU=n asks the compiler
+// for n resident waves per SIMD (fewer VGPRs, possibly spills).
+#if defined(ABX_WAVES_PER_EU)
+#define ABX_TKERNEL __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ABX_WAVES_PER_EU)))
+#else
+#define ABX_TKERNEL __launch_bounds__(256)
 #endif
 
 #define ABX_MAXPADS 10   // lily pads per tracker (C3 needs 6)

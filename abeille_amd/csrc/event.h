@@ -19,6 +19,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
 #include <vector>
 
 #include "kernels.h"
@@ -141,6 +142,7 @@ __device__ __forceinline__ void ev_finalize(const KCtx& K, const Sink& s, int64_
 template <class Sink>
 __device__ __forceinline__ void ev_resume(Sink& s, const EvState& S, int64_t i) {
   s.slot = i;
+  s.sslot = i;
   s.nsec = S.nsec[i];
   s.n_rec = S.nrec[i];
 }
@@ -183,14 +185,14 @@ __global__ void __launch_bounds__(256) k_ev_init(DevProblem P, KCtx K, EvState S
   flush_scores(K, sc, s.c);
 }
 
-template <int TRK, bool TRACE, bool SEC>
-__global__ void __launch_bounds__(256) k_ev_flight(DevProblem P, KCtx K, EvState S,
+template <int TRK, bool TRACE, bool SEC, bool SPLIT>
+__global__ void ABX_TKERNEL k_ev_flight(DevProblem P, KCtx K, EvState S,
                                                    const int32_t* qin, const unsigned* nqin,
                                                    int32_t* qf, unsigned* nqf, int32_t* qc,
                                                    unsigned* nqc) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   Scores6 sc{0, 0, 0, 0, 0, 0};
-  GpuSink<TRACE, SEC> s;
+  GpuSink<TRACE, SEC, SPLIT> s;
   s.init(&K, &P, 0);
   bool to_f = false, to_c = false;
   int32_t slot = 0;
@@ -221,13 +223,13 @@ __global__ void __launch_bounds__(256) k_ev_flight(DevProblem P, KCtx K, EvState
   flush_scores(K, sc, s.c);
 }
 
-template <int TRK, bool TRACE, bool SEC>
-__global__ void __launch_bounds__(256) k_ev_collide(DevProblem P, KCtx K, EvState S,
+template <int TRK, bool TRACE, bool SEC, bool SPLIT>
+__global__ void ABX_TKERNEL k_ev_collide(DevProblem P, KCtx K, EvState S,
                                                     const int32_t* qin, const unsigned* nqin,
                                                     int32_t* qf, unsigned* nqf) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   Scores6 sc{0, 0, 0, 0, 0, 0};
-  GpuSink<TRACE, SEC> s;
+  GpuSink<TRACE, SEC, SPLIT> s;
   s.init(&K, &P, 0);
   bool to_f = false;
   int32_t slot = 0;
@@ -255,13 +257,13 @@ __global__ void __launch_bounds__(256) k_ev_collide(DevProblem P, KCtx K, EvStat
 
 // Finish the remaining histories in place (flight queue first, then the
 // collision queue whose particles have a collision pending).
-template <int TRK, bool TRACE, bool SEC>
-__global__ void __launch_bounds__(256) k_ev_tail(DevProblem P, KCtx K, EvState S,
+template <int TRK, bool TRACE, bool SEC, bool SPLIT>
+__global__ void ABX_TKERNEL k_ev_tail(DevProblem P, KCtx K, EvState S,
                                                  const int32_t* qf, const unsigned* nqf,
                                                  const int32_t* qc, const unsigned* nqc) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   Scores6 sc{0, 0, 0, 0, 0, 0};
-  GpuSink<TRACE, SEC> s;
+  GpuSink<TRACE, SEC, SPLIT> s;
   s.init(&K, &P, 0);
   const int64_t nf = *nqf, nc = *nqc;
   if (k < nf + nc) {
@@ -278,6 +280,10 @@ __global__ void __launch_bounds__(256) k_ev_tail(DevProblem P, KCtx K, EvState S
   }
   flush_scores(K, sc, s.c);
 }
+
+// Host-side launch code follows; ABX_KERNELS_ONLY leaves it out (a
+// translation unit that only instantiates chosen kernels).
+#ifndef ABX_KERNELS_ONLY
 
 // Device buffers of the engine (grow-only).
 struct EventBuffers {
@@ -338,7 +344,7 @@ static inline EvState ev_carve(void* base, int64_t n, int D, int L) {
 // Runs one transport() with the event engine.  `kernel_ms` receives the
 // summed duration of the flight kernels (the dominant kernel), measured with
 // HIP events on `s`, and `launches` their count, when `timing` is set.
-template <int TRK, bool TRACE, bool SEC>
+template <int TRK, bool TRACE, bool SEC, bool SPLIT>
 static hipError_t event_transport_t(const DevProblem& P, const KCtx& K, EventBuffers& B,
                                     hipStream_t s, bool timing, double* kernel_ms,
                                     int64_t* launches) {
@@ -396,7 +402,7 @@ static hipError_t event_transport_t(const DevProblem& P, const KCtx& K, EventBuf
   std::vector<std::pair<hipEvent_t, hipEvent_t>> timed;
   while (nf + nc > 0) {
     if (nf + nc <= tail) {
-      hipLaunchKernelGGL((k_ev_tail<TRK, TRACE, SEC>), dim3((unsigned)((nf + nc + bs - 1) / bs)),
+      hipLaunchKernelGGL((k_ev_tail<TRK, TRACE, SEC, SPLIT>), dim3((unsigned)((nf + nc + bs - 1) / bs)),
                          dim3(bs), lds, s, P, K, S, qa, ca, qc, cc);
       if ((e = hipGetLastError()) != hipSuccess) return e;
       break;
@@ -404,7 +410,7 @@ static hipError_t event_transport_t(const DevProblem& P, const KCtx& K, EventBuf
     // collisions first (they refill the flight queue), then flights
     if ((e = hipMemsetAsync(cb, 0, sizeof(unsigned), s)) != hipSuccess) return e;
     if (nc > 0) {
-      hipLaunchKernelGGL((k_ev_collide<TRK, TRACE, SEC>), dim3((unsigned)((nc + bs - 1) / bs)),
+      hipLaunchKernelGGL((k_ev_collide<TRK, TRACE, SEC, SPLIT>), dim3((unsigned)((nc + bs - 1) / bs)),
                          dim3(bs), lds, s, P, K, S, qc, cc, qa, ca);
       if ((e = hipGetLastError()) != hipSuccess) return e;
     }
@@ -416,7 +422,7 @@ static hipError_t event_transport_t(const DevProblem& P, const KCtx& K, EventBuf
       t1 = get_event();
       (void)hipEventRecord(t0, s);
     }
-    hipLaunchKernelGGL((k_ev_flight<TRK, TRACE, SEC>), dim3((unsigned)((nf_all + bs - 1) / bs)),
+    hipLaunchKernelGGL((k_ev_flight<TRK, TRACE, SEC, SPLIT>), dim3((unsigned)((nf_all + bs - 1) / bs)),
                        dim3(bs), lds, s, P, K, S, qa, ca, qb, cb, qc, cc);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (timing) {
@@ -448,34 +454,89 @@ static hipError_t event_transport_t(const DevProblem& P, const KCtx& K, EventBuf
   return hipSuccess;
 }
 
-template <int TRK, bool TRACE, bool SEC>
-static hipError_t history_transport_t(const DevProblem& P, const KCtx& K, hipStream_t s) {
+// Resident blocks of a transport kernel on this device (persistent grid).
+template <class Kern>
+static int64_t resident_blocks(Kern kernel, int bs, size_t lds) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 1024;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 1024;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, bs, lds) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  return (int64_t)cus * per_cu;
+}
+
+// `next` is a device counter (zeroed here) from which lanes take histories.
+template <int TRK, bool TRACE, bool SEC, bool SPLIT>
+static hipError_t history_transport_t(const DevProblem& P, const KCtx& K, hipStream_t s,
+                                      unsigned long long* next) {
   const int bs = 256;
   const size_t lds = (size_t)bs * trk_bytes(P.trk_D, P.trk_L);
-  hipLaunchKernelGGL((k_history<TRK, TRACE, SEC>), dim3((unsigned)((K.n + bs - 1) / bs)), dim3(bs),
-                     lds, s, P, K);
+  hipError_t e;
+  if ((e = hipMemsetAsync(next, 0, sizeof(unsigned long long), s)) != hipSuccess) return e;
+  const int64_t want = (K.n + bs - 1) / bs;
+  const int64_t blocks = std::min<int64_t>(want, resident_blocks(k_history<TRK, TRACE, SEC, SPLIT>, bs, lds));
+  hipLaunchKernelGGL((k_history<TRK, TRACE, SEC, SPLIT>), dim3((unsigned)blocks), dim3(bs), lds, s, P, K,
+                     next);
   return hipGetLastError();
 }
 
 // Runtime dispatch over the compiled variants (tracking mode x trace x
-// secondary stack).  Carter tracking always carries the stack (splitting).
-#define ABX_DISPATCH(FN, TRK, TRACE, SEC, ...)                                   \
-  (TRK == 0 ? (TRACE ? (SEC ? FN<0, true, true>(__VA_ARGS__) : FN<0, true, false>(__VA_ARGS__))   \
-                     : (SEC ? FN<0, false, true>(__VA_ARGS__) : FN<0, false, false>(__VA_ARGS__))) \
-   : TRK == 1 ? (TRACE ? (SEC ? FN<1, true, true>(__VA_ARGS__) : FN<1, true, false>(__VA_ARGS__)) \
-                       : (SEC ? FN<1, false, true>(__VA_ARGS__) : FN<1, false, false>(__VA_ARGS__))) \
-              : (TRACE ? FN<2, true, true>(__VA_ARGS__) : FN<2, false, true>(__VA_ARGS__)))
+// secondary stack x precision).  Carter tracking always carries the stack
+// (splitting); traces are recorded in FP64 mode only.
+template <int V>
+using IntC = std::integral_constant<int, V>;
+template <bool V>
+using BoolC = std::integral_constant<bool, V>;
+
+template <class Fn>
+static hipError_t dispatch_variant(int trk, bool trace, bool sec, bool split, Fn&& fn) {
+  if (split) {
+    if (trace) return hipErrorInvalidValue;
+    if (trk == 0)
+      return sec ? fn(IntC<0>{}, BoolC<false>{}, BoolC<true>{}, BoolC<true>{})
+                 : fn(IntC<0>{}, BoolC<false>{}, BoolC<false>{}, BoolC<true>{});
+    if (trk == 1)
+      return sec ? fn(IntC<1>{}, BoolC<false>{}, BoolC<true>{}, BoolC<true>{})
+                 : fn(IntC<1>{}, BoolC<false>{}, BoolC<false>{}, BoolC<true>{});
+    return fn(IntC<2>{}, BoolC<false>{}, BoolC<true>{}, BoolC<true>{});
+  }
+  if (trk == 2)
+    return trace ? fn(IntC<2>{}, BoolC<true>{}, BoolC<true>{}, BoolC<false>{})
+                 : fn(IntC<2>{}, BoolC<false>{}, BoolC<true>{}, BoolC<false>{});
+  if (trk == 0) {
+    if (trace)
+      return sec ? fn(IntC<0>{}, BoolC<true>{}, BoolC<true>{}, BoolC<false>{})
+                 : fn(IntC<0>{}, BoolC<true>{}, BoolC<false>{}, BoolC<false>{});
+    return sec ? fn(IntC<0>{}, BoolC<false>{}, BoolC<true>{}, BoolC<false>{})
+               : fn(IntC<0>{}, BoolC<false>{}, BoolC<false>{}, BoolC<false>{});
+  }
+  if (trace)
+    return sec ? fn(IntC<1>{}, BoolC<true>{}, BoolC<true>{}, BoolC<false>{})
+               : fn(IntC<1>{}, BoolC<true>{}, BoolC<false>{}, BoolC<false>{});
+  return sec ? fn(IntC<1>{}, BoolC<false>{}, BoolC<true>{}, BoolC<false>{})
+             : fn(IntC<1>{}, BoolC<false>{}, BoolC<false>{}, BoolC<false>{});
+}
 
 static inline hipError_t event_transport(const DevProblem& P, const KCtx& K, EventBuffers& B,
                                          hipStream_t s, bool sec, bool timing, double* kernel_ms,
                                          int64_t* launches) {
-  const bool trace = K.trace != nullptr;
-  return ABX_DISPATCH(event_transport_t, P.tracking, trace, sec, P, K, B, s, timing, kernel_ms,
-                      launches);
+  return dispatch_variant(P.tracking, K.trace != nullptr, sec, P.precision == ABX_SPLIT,
+                          [&](auto T, auto TR, auto SE, auto SP) {
+                            return event_transport_t<decltype(T)::value, decltype(TR)::value,
+                                                     decltype(SE)::value, decltype(SP)::value>(
+                                P, K, B, s, timing, kernel_ms, launches);
+                          });
 }
 
 static inline hipError_t history_transport(const DevProblem& P, const KCtx& K, hipStream_t s,
-                                           bool sec) {
-  const bool trace = K.trace != nullptr;
-  return ABX_DISPATCH(history_transport_t, P.tracking, trace, sec, P, K, s);
+                                           bool sec, unsigned long long* next) {
+  return dispatch_variant(P.tracking, K.trace != nullptr, sec, P.precision == ABX_SPLIT,
+                          [&](auto T, auto TR, auto SE, auto SP) {
+                            return history_transport_t<decltype(T)::value, decltype(TR)::value,
+                                                       decltype(SE)::value, decltype(SP)::value>(
+                                P, K, s, next);
+                          });
 }
+
+#endif  // ABX_KERNELS_ONLY

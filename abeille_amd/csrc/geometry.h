@@ -53,6 +53,11 @@ ABX_HD V3 surf_norm(const DSurf& s, const V3& r) {
   }
 }
 
+// A point on the surface (|eval| <= 1e-12) takes the sign of u . n.
+ABX_COLD int surf_sign_on(const DSurf& s, const V3& r, const V3& u) {
+  return vdot(u, surf_norm(s, r)) > 0. ? 1 : -1;
+}
+
 ABX_HD int surf_sign(const DSurf& s, const V3& r, const V3& u) {
   double eval;
   switch (s.type) {
@@ -83,7 +88,7 @@ ABX_HD int surf_sign(const DSurf& s, const V3& r, const V3& u) {
   }
   if (eval > ABX_SURFACE_COINCIDENT) return 1;
   if (eval < -ABX_SURFACE_COINCIDENT) return -1;
-  return vdot(u, surf_norm(s, r)) > 0. ? 1 : -1;
+  return surf_sign_on(s, r, u);
 }
 
 ABX_HD double plane_axis_distance(double x0, double r0, double u0, bool on) {
@@ -413,8 +418,10 @@ ABX_NOINL int trk_descend(const DevProblem& P, Trk& t, int ui, int lvl, V3 r, co
   return -1;
 }
 
-// Tracker::restart_get_current from the global position lv[0].
-ABX_NOINL void trk_restart(const DevProblem& P, Trk& t, const V3& u) {
+// Tracker::restart_get_current from the global position lv[0].  Out of
+// line: it runs at birth, after reflections and resurrections, and when the
+// root pad itself went stale.
+ABX_COLD void trk_restart(const DevProblem& P, Trk& t, const V3& u) {
   t.np = 0;
   t.cell = trk_descend(P, t, P.root, 0, t.lv(0), u);
 }
@@ -468,7 +475,7 @@ ABX_HD void trk_get_current(const DevProblem& P, Trk& t, const V3& u) {
 // Tracker::set_r(r) followed by get_current(): the stale tree forces a
 // restart, after which the reference still compares against the old tree
 // size (tracker.hpp:237-278).
-ABX_NOINL void trk_reset_position(const DevProblem& P, Trk& t, const V3& r, const V3& u) {
+ABX_COLD void trk_reset_position(const DevProblem& P, Trk& t, const V3& r, const V3& u) {
   const int old_np = t.np;
   t.token = 0;
   t.set_lv(0, r);

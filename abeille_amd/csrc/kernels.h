@@ -66,12 +66,15 @@ __device__ __forceinline__ void raise_error(const KCtx& K, int code, uint64_t h)
 }
 
 // Global-memory side of the physics for one lane.  TRACE compiles the
-// per-event trace records in; SEC the per-history secondary stack.
-template <bool TRACE, bool SEC>
+// per-event trace records in; SEC the per-history secondary stack; SPLIT
+// selects the split-precision physics.
+template <bool TRACE, bool SEC, bool SPLIT = false>
 struct GpuSink {
+  static constexpr bool split = SPLIT;
   const KCtx* K;
   const DevProblem* P;
-  int64_t slot;
+  int64_t slot;   // history index in the input bank
+  int64_t sslot;  // secondary-stack slot (the history, or the persistent lane)
   int32_t nsec;
   int32_t n_rec;
   unsigned long long c[8];
@@ -85,7 +88,7 @@ struct GpuSink {
       return false;
     } else {
       if (nsec >= K->sec_cap) return false;
-      K->sec[slot * K->sec_cap + nsec] = q;
+      K->sec[sslot * K->sec_cap + nsec] = q;
       nsec++;
       return true;
     }
@@ -96,7 +99,7 @@ struct GpuSink {
     } else {
       if (nsec == 0) return false;
       nsec--;
-      q = K->sec[slot * K->sec_cap + nsec];
+      q = K->sec[sslot * K->sec_cap + nsec];
       return true;
     }
   }
@@ -154,6 +157,7 @@ struct GpuSink {
     K = k;
     P = p;
     slot = s;
+    sslot = s;
     nsec = 0;
     n_rec = 0;
     for (int i = 0; i < 8; i++) c[i] = 0;
@@ -184,21 +188,70 @@ __device__ __forceinline__ void bind_lds_cursor(Trk& t, const DevProblem& P) {
   trk_bind(t, abx_lds, blockDim.x, threadIdx.x, P.trk_D, P.trk_L);
 }
 
-template <int TRK, bool TRACE, bool SEC>
-__global__ void __launch_bounds__(256) k_history(DevProblem P, KCtx K) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// History-based transport on a persistent grid.  Each lane carries one
+// history at a time and, when it ends, takes the next one from a global
+// counter (one atomic per wavefront for all its idle lanes), so lanes stay
+// busy until the bank is drained instead of idling behind the longest
+// history of their wavefront.  Per-history results do not depend on which
+// lane runs a history: the RNG stream is the history's own and fission
+// sites are ordered afterwards by (parent, daughter).
+template <int TRK, bool TRACE, bool SEC, bool SPLIT>
+__global__ void ABX_TKERNEL k_history(DevProblem P, KCtx K, unsigned long long* next) {
+  const int lane = threadIdx.x & 63;
   Scores6 sc{0, 0, 0, 0, 0, 0};
-  GpuSink<TRACE, SEC> s;
-  s.init(&K, &P, i);
-  if (i < K.n) {
-    Part p;
-    load_part(P, K.in[i], p);
-    Trk t;
-    bind_lds_cursor(t, P);
-    run_history<TRK>(P, s, p, t, K.kcol, K.converged != 0, sc);
-    s.c[0] = 1;
-    if (K.daughters) K.daughters[i] = p.daughters;
-    if (TRACE && K.n_rec) K.n_rec[i] = s.n_rec;
+  GpuSink<TRACE, SEC, SPLIT> s;
+  s.init(&K, &P, 0);
+  s.sslot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  Trk t;
+  bind_lds_cursor(t, P);
+  Part p;
+  p.alive = 0;
+  int m = -1;
+  int64_t steps = 0;
+  bool done = false;
+  const bool converged = K.converged != 0;
+  for (;;) {
+    const bool need = !p.alive && !done;
+    const unsigned long long mask = __ballot(need);
+    if (mask) {
+      const int leader = __ffsll((long long)mask) - 1;
+      unsigned long long base = 0;
+      if (lane == leader) base = atomicAdd(next, (unsigned long long)__popcll(mask));
+      base = __shfl(base, leader, 64);
+      if (need) {
+        const int64_t i = (int64_t)(base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull)));
+        if (i < K.n) {
+          s.slot = i;
+          s.nsec = 0;
+          s.n_rec = 0;
+          steps = 0;
+          load_part(P, K.in[i], p);
+          s.c[0]++;
+          if (!start_history(P, s, p, t, m)) {
+            if (K.daughters) K.daughters[i] = 0;
+            if (TRACE && K.n_rec) K.n_rec[i] = s.n_rec;
+          }
+        } else {
+          done = true;
+        }
+      }
+    }
+    if (__ballot(!done) == 0ull) break;
+    if (p.alive) {
+      int ev = 0;
+      bool hc = step_flight<TRK>(P, s, p, t, m, converged, sc, ev);
+      if (p.alive && hc) step_collision(P, s, p, m, K.kcol, converged, sc, ev);
+      bool live = step_finish<TRK>(P, s, p, t, m, ev);
+      if (live && ++steps > 100000000) {
+        s.error(-3, p.hid);
+        p.alive = 0;
+        live = false;
+      }
+      if (!live) {
+        if (K.daughters) K.daughters[s.slot] = p.daughters;
+        if (TRACE && K.n_rec) K.n_rec[s.slot] = s.n_rec;
+      }
+    }
   }
   flush_scores(K, sc, s.c);
 }

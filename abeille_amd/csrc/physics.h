@@ -18,6 +18,13 @@
 // secondary stack, fission-bank appends, traces) goes through the `Sink`
 // template parameter, so the same code runs in the history kernel and in
 // the event kernels.
+//
+// Sink::split selects the split-precision variant: sampled flight
+// distances, cross-section ratios, scattering / emission directions and
+// tally score factors in FP32 (hardware log / sincos / rcp / sqrt);
+// positions, surface and lattice tests, weights, k scores and tally
+// accumulators stay FP64.  With split == false every operation is the
+// reference's FP64 one, in its order.
 #pragma once
 
 #include "detmath.h"
@@ -55,6 +62,19 @@ ABX_HD int p_discrete(Part& p, const double* cdf, int n) {
   if (n < 2) return 0;
   p.calls += 2;
   return rng_discrete(p.rng, cdf, n);
+}
+
+// Flight distance -log(1 - xi) / lambda (RNG::exponential), FP32 in split mode.
+template <bool SPLIT>
+ABX_HD double flight_distance(Part& p, double lambda) {
+  if constexpr (!SPLIT) {
+    return p_exponential(p, lambda);
+  } else {
+    if (lambda == 0.0) return ABX_INF;
+    p.calls += 2;
+    const double xi = rng_rand(p.rng);
+    return (double)fp_div(-fp_log((float)(1.0 - xi)), (float)lambda);
+  }
 }
 
 ABX_HD int energy_index(const DevProblem& P, double E) {
@@ -100,6 +120,44 @@ ABX_NOINL V3 rotate_dir(const V3& u, double mu, double phi) {
   return vdir(ux, uy, uz);
 }
 
+// rotate_direction in FP32, normalised with a reciprocal square root.  The
+// branch threshold is the FP32 analogue of the reference's 1e-10 on
+// sqrt(1 - w^2); both branches are the same rotation up to the azimuth
+// origin, which is uniform.
+ABX_HD V3 rotate_dir_fp(const V3& ud, double mu_d, double phi_d) {
+  const float mu = (float)mu_d;
+  float sn, cs;
+  fp_sincos((float)phi_d, &sn, &cs);
+  const float ux = (float)ud.x, uy = (float)ud.y, uz = (float)ud.z;
+  const float m2 = 1.f - mu * mu;
+  const float sqrt_mu = m2 > 0.f ? fp_sqrt(m2) : 0.f;
+  const float w2 = 1.f - uz * uz;
+  float x, y, z;
+  if (w2 > 1e-6f) {
+    const float sqrt_w = fp_sqrt(w2);
+    const float iw = fp_rcp(sqrt_w);
+    x = mu * ux + sqrt_mu * (ux * uz * cs - uy * sn) * iw;
+    y = mu * uy + sqrt_mu * (uy * uz * cs + ux * sn) * iw;
+    z = mu * uz - sqrt_mu * sqrt_w * cs;
+  } else {
+    const float sqrt_v = fp_sqrt(1.f - uy * uy);
+    const float iv = fp_rcp(sqrt_v);
+    x = mu * ux + sqrt_mu * (ux * uy * cs + uz * sn) * iv;
+    y = mu * uy - sqrt_mu * sqrt_v * cs;
+    z = mu * uz + sqrt_mu * (uy * uz * cs - ux * sn) * iv;
+  }
+  const float inv = fp_rsqrt(x * x + y * y + z * z);
+  return V3{(double)(x * inv), (double)(y * inv), (double)(z * inv)};
+}
+
+template <bool SPLIT>
+ABX_HD V3 rotate(const V3& u, double mu, double phi) {
+  if constexpr (SPLIT)
+    return rotate_dir_fp(u, mu, phi);
+  else
+    return rotate_dir(u, mu, phi);
+}
+
 ABX_NOINL double sample_mu(const DevProblem& P, Part& p, int m, int gi, int go) {
   const int k = (m * P.G + gi) * P.G + go;
   const int b0 = P.ang_off[k], n = P.ang_off[k + 1] - b0;
@@ -141,7 +199,11 @@ ABX_NOINL void score_collision_tallies(const DevProblem& P, Sink& s, const Part&
     const DTally& T = P.tally[ti];
     if (T.estimator != 0) continue;
     const double Et = P.Et[mg];
-    double scr = 1. / (Et * T.net_weight);
+    double scr;
+    if constexpr (Sink::split)
+      scr = (double)fp_rcp((float)Et * (float)T.net_weight);
+    else
+      scr = 1. / (Et * T.net_weight);
     const int i = (int)floor((p.r.x - T.lx) * T.dxi);
     const int j = (int)floor((p.r.y - T.ly) * T.dyi);
     const int k = (int)floor((p.r.z - T.lz) * T.dzi);
@@ -186,8 +248,8 @@ ABX_HD void tle_init(const DTally& T, const V3& r, const V3& u, int& i, int& j, 
 }
 
 // TrackLengthMeshTally::find_entry_point (slab method)
-ABX_HD bool tle_entry(const DTally& T, V3& r, const V3& u, double& d) {
-  const double ux_inv = 1. / u.x, uy_inv = 1. / u.y, uz_inv = 1. / u.z;
+ABX_HD bool tle_entry(const DTally& T, V3& r, const V3& u, const V3& ui, double& d) {
+  const double ux_inv = ui.x, uy_inv = ui.y, uz_inv = ui.z;
   double d_min = (T.lx - r.x) * ux_inv, d_max = (T.hx - r.x) * ux_inv;
   double t;
   if (d_min > d_max) { t = d_min; d_min = d_max; d_max = t; }
@@ -218,12 +280,17 @@ ABX_NOINL void score_flight_tallies(const DevProblem& P, Sink& s, const Part& p,
     if (T.estimator != 1) continue;
     V3 r = p.r;
     const V3 u = p.u;
+    V3 ui;  // 1 / u per axis
+    if constexpr (Sink::split)
+      ui = V3{(double)fp_rcp((float)u.x), (double)fp_rcp((float)u.y), (double)fp_rcp((float)u.z)};
+    else
+      ui = V3{1. / u.x, 1. / u.y, 1. / u.z};
     double dd = d;
     int i, j, k, on[3];
     tle_init(T, r, u, i, j, k, on);
     const bool inside = i >= 0 && i < T.Nx && j >= 0 && j < T.Ny && k >= 0 && k < T.Nz;
     if (!inside) {
-      if (!tle_entry(T, r, u, dd)) continue;
+      if (!tle_entry(T, r, u, ui, dd)) continue;
       tle_init(T, r, u, i, j, k, on);
     }
     double base = 1. / T.net_weight;
@@ -242,7 +309,7 @@ ABX_NOINL void score_flight_tallies(const DevProblem& P, Sink& s, const Part& p,
       const double diff_xl = -T.dx * 0.5 - rtx, diff_xh = T.dx * 0.5 - rtx;
       const double diff_yl = -T.dy * 0.5 - rty, diff_yh = T.dy * 0.5 - rty;
       const double diff_zl = -T.dz * 0.5 - rtz, diff_zh = T.dz * 0.5 - rtz;
-      const double ux_inv = 1. / u.x, uy_inv = 1. / u.y, uz_inv = 1. / u.z;
+      const double ux_inv = ui.x, uy_inv = ui.y, uz_inv = ui.z;
       const double d_xl = diff_xl * ux_inv, d_xh = diff_xh * ux_inv;
       const double d_yl = diff_yl * uy_inv, d_yh = diff_yh * uy_inv;
       const double d_zl = diff_zl * uz_inv, d_zh = diff_zh * uz_inv;
@@ -284,24 +351,42 @@ struct Scores6 {
 template <class Sink>
 ABX_HD void collide(const DevProblem& P, Sink& s, Part& p, int m, double kcol,
                     bool converged, Scores6& sc) {
+  constexpr bool SPLIT = Sink::split;
   const int G = P.G;
   const int mg = m * G + p.g;
   if (converged && P.ntally) score_collision_tallies(P, s, p, m);
-  {
-    const double k_col_scr = p.w * P.vEf[mg] / P.Et[mg];
-    const V3 dr = vsub(p.r, p.rb);
-    const double mig_dist = vnorm(dr);
-    const double mig_area_scr = p.w * P.Eabs[mg] / P.Et[mg] * mig_dist * mig_dist;
-    sc.k_col += k_col_scr;
-    sc.mig += mig_area_scr;
-  }
-  (void)p_rand(p);  // MaterialHelper::sample_nuclide (single component)
   const double total = P.Et[mg], fission = P.Ef[mg], absorption = P.Eabs[mg];
   const double nu_total = P.nu_tot[mg], nu_delayed = P.nu_d[mg];
-  sc.k_abs += p.w * nu_total * fission / total;
+  // ratios to the total cross section: nu-fission, absorption, nu * fission
+  double r_vf, r_abs, r_nuf;
+  if constexpr (SPLIT) {
+    const float it = fp_rcp((float)total);
+    r_vf = (double)((float)P.vEf[mg] * it);
+    r_abs = (double)((float)absorption * it);
+    r_nuf = (double)((float)(nu_total * fission) * it);
+  }
+  {
+    const V3 dr = vsub(p.r, p.rb);
+    if constexpr (SPLIT) {
+      sc.k_col += p.w * r_vf;
+      sc.mig += p.w * r_abs * vdot(dr, dr);
+    } else {
+      const double k_col_scr = p.w * P.vEf[mg] / P.Et[mg];
+      const double mig_dist = vnorm(dr);
+      const double mig_area_scr = p.w * P.Eabs[mg] / P.Et[mg] * mig_dist * mig_dist;
+      sc.k_col += k_col_scr;
+      sc.mig += mig_area_scr;
+    }
+  }
+  (void)p_rand(p);  // MaterialHelper::sample_nuclide (single component)
+  double k_abs_scr;
+  if constexpr (SPLIT)
+    k_abs_scr = p.w * r_nuf;
+  else
+    k_abs_scr = p.w * nu_total * fission / total;
+  sc.k_abs += k_abs_scr;
   // make_fission_neutrons
   {
-    const double k_abs_scr = p.w * nu_total * fission / total;
     int n_new;
     if (P.mode == 0)
       n_new = (int)floor(dabs(k_abs_scr) / kcol + p_rand(p));
@@ -314,7 +399,7 @@ ABX_HD void collide(const DevProblem& P, Sink& s, Part& p, int m, double kcol,
       const double E_out = P.emid[ei];
       const double mu = 2. * p_rand(p) - 1.;
       const double phi = 2. * ABX_PI * p_rand(p);
-      const V3 uout = rotate_dir(p.u, mu, phi);
+      const V3 uout = rotate<SPLIT>(p.u, mu, phi);
       if (p_rand(p) < P_delayed) {
         (void)p_discrete(p, P.dgrp_cdf + (size_t)m * P.n_dgrp_max, P.dgrp_n[m]);
       }
@@ -332,8 +417,14 @@ ABX_HD void collide(const DevProblem& P, Sink& s, Part& p, int m, double kcol,
       }
     }
   }
-  p.w = p.w * (1. - (absorption + 0.) / total);
-  p.w2 = p.w2 * (1. - (absorption + 0.) / total);
+  if constexpr (SPLIT) {
+    const double keep = 1. - r_abs;
+    p.w = p.w * keep;
+    p.w2 = p.w2 * keep;
+  } else {
+    p.w = p.w * (1. - (absorption + 0.) / total);
+    p.w2 = p.w2 * (1. - (absorption + 0.) / total);
+  }
   // russian_roulette
   if (dabs(p.w) < P.wgt_cutoff) {
     const double P_kill = 1.0 - (dabs(p.w) / P.wgt_survival);
@@ -356,7 +447,7 @@ ABX_HD void collide(const DevProblem& P, Sink& s, Part& p, int m, double kcol,
     const double E_out = P.emid[ei];
     const double mu = sample_mu(P, p, m, p.g, ei);
     const double phi = 2. * ABX_PI * p_rand(p);
-    p.u = rotate_dir(p.u, mu, phi);
+    p.u = rotate<SPLIT>(p.u, mu, phi);
     p.E = E_out;
     p.g = energy_index(P, E_out);
     p.w = p.w * 1.;
@@ -376,7 +467,7 @@ ABX_HD void leak(Part& p, const Bound& b, Scores6& sc) {
 
 // Tracker::do_reflection; the particle's position before the reflection
 // only feeds Particle::previous_r, which this path does not keep.
-ABX_NOINL void reflect(const DevProblem& P, Trk& t, Part& p, const Bound& b) {
+ABX_COLD void reflect(const DevProblem& P, Trk& t, Part& p, const Bound& b) {
   if (b.sidx < 0) {
     t.err = -3;
     return;
@@ -393,7 +484,7 @@ ABX_NOINL void reflect(const DevProblem& P, Trk& t, Part& p, const Bound& b) {
 }
 
 template <class Sink>
-ABX_NOINL bool try_resurrect(const DevProblem& P, Sink& s, Part& p, Trk& t, int& m) {
+ABX_COLD bool try_resurrect(const DevProblem& P, Sink& s, Part& p, Trk& t, int& m) {
   SecRec q;
   if (!s.sec_pop(q)) return false;
   p.alive = 1;
@@ -420,22 +511,25 @@ ABX_NOINL bool try_resurrect(const DevProblem& P, Sink& s, Part& p, Trk& t, int&
 
 // Carter-tracking weight split (src/carter_tracker.cpp:219-223)
 template <class Sink>
-ABX_NOINL void split(Sink& s, Part& p, const DevProblem& P) {
-  if (p.alive && dabs(p.w) >= P.wgt_split) {
-    const int n_new = (int)ceil(dabs(p.w));
-    if (n_new > 1) {
-      p.w = p.w / (double)n_new;
-      p.w2 = p.w2 / (double)n_new;
-      for (int np = 0; np < n_new - 1; np++) {
-        SecRec q{{p.r.x, p.r.y, p.r.z}, {p.u.x, p.u.y, p.u.z}, p.E, p.w, p.w2};
-        if (!s.sec_push(q)) {
-          s.error(-5, p.hid);
-          p.alive = 0;
-          return;
-        }
+ABX_COLD void split_weight(Sink& s, Part& p) {
+  const int n_new = (int)ceil(dabs(p.w));
+  if (n_new > 1) {
+    p.w = p.w / (double)n_new;
+    p.w2 = p.w2 / (double)n_new;
+    for (int np = 0; np < n_new - 1; np++) {
+      SecRec q{{p.r.x, p.r.y, p.r.z}, {p.u.x, p.u.y, p.u.z}, p.E, p.w, p.w2};
+      if (!s.sec_push(q)) {
+        s.error(-5, p.hid);
+        p.alive = 0;
+        return;
       }
     }
   }
+}
+
+template <class Sink>
+ABX_HD void split(Sink& s, Part& p, const DevProblem& P) {
+  if (p.alive && dabs(p.w) >= P.wgt_split) split_weight(s, p);
 }
 
 // Start of a history: locate; returns false if the particle is dead
@@ -465,7 +559,7 @@ ABX_HD bool step_flight(const DevProblem& P, Sink& s, Part& p, Trk& t, int& m, b
                         Scores6& sc, int& ev) {
   if (TRK == 0) {
     // ---------------- surface tracking (src/surface_tracker.cpp:70-151)
-    const double d_coll = p_exponential(p, P.Et[m * P.G + p.g]);
+    const double d_coll = flight_distance<Sink::split>(p, P.Et[m * P.G + p.g]);
     s.count_flight();
     const Bound b = trk_nearest(P, t, p.u);
     const double dmin = d_coll < b.d ? d_coll : b.d;
@@ -507,7 +601,7 @@ ABX_HD bool step_flight(const DevProblem& P, Sink& s, Part& p, Trk& t, int& m, b
   // src/carter_tracker.cpp:124-216)
   const bool carter = TRK == 2;
   const double Emaj = (carter ? P.sampling[p.g] : P.majorant[p.g]) + 0.;
-  const double d_coll = p_exponential(p, Emaj);
+  const double d_coll = flight_distance<Sink::split>(p, Emaj);
   s.count_flight();
   trk_move(t, p.u, d_coll);
   trk_get_current(P, t, p.u);
@@ -558,7 +652,11 @@ ABX_HD bool step_flight(const DevProblem& P, Sink& s, Part& p, Trk& t, int& m, b
       p.alive = 0;
       return false;
     }
-    if (p_rand(p) < (Et / Emaj)) had_collision = true;
+    if constexpr (Sink::split) {
+      if ((float)p_rand(p) < fp_div((float)Et, (float)Emaj)) had_collision = true;
+    } else {
+      if (p_rand(p) < (Et / Emaj)) had_collision = true;
+    }
   } else if (Emaj >= Et) {
     if (p_rand(p) < (Et / Emaj)) had_collision = true;
   } else {

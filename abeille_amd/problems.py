@@ -10,6 +10,8 @@
                       0-150 cm axially (vacuum top/bottom), surface tracking,
                       153x153x30x7 track-length tally.
   slab_problem        one random fixed-source slab problem (batched form).
+  subcritical_slab    modified-fixed-source one-group slab with carter
+                      tracking and approximate mesh cancellation.
   la13511             analytic one-group cases (LA-13511) with their published
                       k values, used as known answers.
 
@@ -281,3 +283,23 @@ def slab_problem(rng, G=None, tracking=None, nhist=None, fission_free=None):
     m.set(mode=abi.FIXED_SOURCE, nparticles=nhist, energy_bounds=np.arange(G + 1.0),
           tracking=_tracking(tracking))
     return m, nhist
+
+
+def subcritical_slab(nparticles=20000, tracking="carter", cancel=True, half=1.0):
+    """Modified-fixed-source run in a subcritical one-group box of the
+    PUa-1-0-SL material (2 half x 10 x 10 cm, vacuum on every face): a point
+    source at the centre, fission generations followed until the chain dies.
+    Carter tracking samples flights with 0.8 x the majorant, so weights go
+    negative and split; `cancel` adds approximate mesh cancellation on a
+    10 x 10 x 10 mesh.  A collision flux tally on 20 x 1 x 1."""
+    m, _ = la13511("PUa-1-0-SL", nparticles, tracking="surface")
+    m.surfaces = [(t, [(-half if p[0] < 0 else half)] if t == abi.XPLANE else p, abi.VACUUM, sid)
+                  for (t, p, bc, sid) in m.surfaces]
+    m.mesh_tally([-half, -5., -5.], [half, 5., 5.], [20, 1, 1], [0., 1.],
+                 estimator="collision")
+    m.set(mode=abi.MODIFIED_FIXED_SOURCE, tracking=_tracking(tracking))
+    if tracking == "carter":
+        m.set(sample_xs_ratio=[0.8])
+    if cancel:
+        m.cancellation_mesh([-half, -5., -5.], [half, 5., 5.], [10, 10, 10])
+    return m

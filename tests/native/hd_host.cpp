@@ -11,6 +11,7 @@
 #include "../../abeille_amd/csrc/tables.h"
 
 struct HostSink {
+  static constexpr bool split = false;
   std::vector<SecRec> stack;
   std::vector<abx_banked>* bank;
   std::vector<abx_trace_rec>* trace_buf;

@@ -19,6 +19,8 @@ CASES = {
     "core_dt": lambda: problems.core_3x3(600, tracking="delta"),
     "la_slab": lambda: problems.la13511("PUb-1-0-SL", 600)[0],
     "la_inf": lambda: problems.la13511("Ua-1-1-IN", 600)[0],
+    "mfs_carter": lambda: problems.subcritical_slab(600, tracking="carter", cancel=False),
+    "mfs_delta": lambda: problems.subcritical_slab(600, tracking="delta", cancel=False),
 }
 
 

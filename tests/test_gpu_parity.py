@@ -23,6 +23,8 @@ CASES = {
     "core_st": lambda: problems.core_3x3(2000, tracking="surface"),
     "core_dt": lambda: problems.core_3x3(2000, tracking="delta"),
     "slab_PUa_1_0_SL": lambda: problems.la13511("PUa-1-0-SL", 2000)[0],
+    "slab_mfs_carter": lambda: problems.subcritical_slab(2000, tracking="carter", cancel=False),
+    "slab_mfs_delta": lambda: problems.subcritical_slab(2000, tracking="delta", cancel=False),
 }
 
 
@@ -78,3 +80,48 @@ def test_cancellation_bit_exact():
     a_g = g.cancel_bank(b)
     assert a_o.tobytes() == a_g.tobytes()
     assert not np.array_equal(a_o["wgt"], b["wgt"])
+
+
+RUNS = {
+    "lattice_dt": (lambda: problems.pin_lattice_2d(4000, tracking="delta"), 4, 1),
+    "slab_PUa_1_0_SL": (lambda: problems.la13511("PUa-1-0-SL", 4000)[0], 4, 2),
+    "cube_st": (lambda: problems.fixed_source_cube(4000, tracking="surface"), 3, 0),
+    "slab_mfs_carter_cancel": (lambda: problems.subcritical_slab(4000, tracking="carter"), 3, 0),
+}
+
+
+@pytest.mark.parametrize("name", sorted(RUNS))
+@pytest.mark.parametrize("engine", [abi.ENGINE_EVENT, abi.ENGINE_HISTORY], ids=["event", "history"])
+def test_run_matches_oracle(name, engine):
+    """Whole device-resident runs (generation loop, weight normalisation,
+    re-banking, cancellation, statistics) against the CPU restatement's
+    run: per-generation k_col and every tally bin's mean and standard error
+    agree up to summation order."""
+    print(f"[run] case={name} engine={engine}", flush=True)
+    make, ngen, nign = RUNS[name]
+    m = make()
+    o = Oracle(m)
+    g = Problem(m, engine=engine)
+    so, ko = o.run(ngen, nign, nthreads=1)
+    sg, kg = g.run(ngen, nign)
+    np.testing.assert_allclose(kg, ko, rtol=1e-9)
+    assert sg.active_generations == so.active_generations
+    assert sg.diag.collisions == so.diag.collisions
+    for t in range(len(m.tallies)):
+        ao, eo = o.tally_stats(t)
+        ag, eg = g.tally_stats(t)
+        np.testing.assert_allclose(ag, ao, rtol=1e-8, atol=1e-14)
+        np.testing.assert_allclose(eg, eo, rtol=1e-6, atol=1e-12)
+
+
+@pytest.mark.parametrize("engine", [abi.ENGINE_EVENT, abi.ENGINE_HISTORY], ids=["event", "history"])
+def test_run_continue_keeps_the_source_bank(engine):
+    """run(1) followed by run_continue(3) transports the same generations as
+    run(4): the k-eigenvalue source survives between calls."""
+    m = problems.pin_lattice_2d(4000, tracking="delta", tally=False)
+    a = Problem(m, engine=engine)
+    _, k1 = a.run(1, nignored=1)
+    _, k2 = a.run_continue(3, nignored=0)
+    b = Problem(m, engine=engine)
+    _, k = b.run(4, nignored=1)
+    np.testing.assert_allclose(np.concatenate([k1, k2]), k, rtol=1e-9)

@@ -1,0 +1,83 @@
+"""Split precision (FP32 flight distances, cross-section ratios, directions
+and score factors; FP64 positions, surface tests, weights and tally
+accumulators) against the FP64 path on the same problems and seeds.
+
+The two paths sample different random walks after the first FP32 rounding,
+so they are compared statistically: k and every tally bin within their
+combined standard errors, and analytic answers where the problem has one."""
+import numpy as np
+import pytest
+
+from abeille_amd import Problem, abi, problems
+from helpers import assert_within_sigma
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(make, ngen, nign, engine):
+    out = []
+    for prec in (abi.FP64, abi.SPLIT):
+        m = make()
+        m.set(precision=prec)
+        p = Problem(m, engine=engine)
+        st, _ = p.run(ngen, nign)
+        tallies = [p.tally_stats(t) for t in range(len(m.tallies))]
+        out.append((st, tallies))
+    return out
+
+
+@pytest.mark.parametrize("engine", [abi.ENGINE_EVENT, abi.ENGINE_HISTORY], ids=["event", "history"])
+def test_lattice_split_matches_fp64(engine):
+    """17x17 lattice, delta tracking, track-length tally 51x51x7: k and the
+    tally agree within statistics, and the bins the FP64 run resolves to 1 %
+    agree with it to 1 % on average."""
+    print(f"[split] lattice engine={engine}", flush=True)
+    (s64, t64), (s32, t32) = _pair(lambda: problems.pin_lattice_2d(100_000), 12, 4, engine)
+    sig = np.hypot(s64.kcol_err, s32.kcol_err)
+    assert abs(s64.kcol_avg - s32.kcol_avg) < 4 * sig, (s64.kcol_avg, s32.kcol_avg, sig)
+    (a64, e64), (a32, e32) = t64[0], t32[0]
+    assert_within_sigma(a64, e64, a32, e32, nsig=4.0, min_frac=0.99)
+    good = (a64 > 0) & (e64 < 0.01 * a64)
+    assert good.sum() > 100
+    dev = np.abs(a32[good] - a64[good]) / a64[good]
+    assert dev.mean() < 0.01, dev.mean()
+
+
+def test_cube_split_matches_fp64():
+    """Fixed-source cube, collision-estimator tally 20^3 x 2."""
+    (s64, t64), (s32, t32) = _pair(lambda: problems.fixed_source_cube(20000, "delta"), 10, 0,
+                                   abi.ENGINE_HISTORY)
+    (a64, e64), (a32, e32) = t64[0], t32[0]
+    assert_within_sigma(a64, e64, a32, e32, nsig=4.0, min_frac=0.99)
+    assert abs(s64.leak_avg - s32.leak_avg) < 4 * np.hypot(s64.leak_err, s32.leak_err)
+
+
+@pytest.mark.parametrize("tracking", ["surface", "delta"])
+def test_split_critical_slab_k_is_one(tracking):
+    """LA-13511 PUa-1-0-SL is exactly critical."""
+    m, k_ref = problems.la13511("PUa-1-0-SL", 20000, tracking=tracking)
+    m.set(precision=abi.SPLIT)
+    st, _ = Problem(m).run(40, 10)
+    assert abs(st.kcol_avg - k_ref) < 4 * st.kcol_err, (st.kcol_avg, st.kcol_err)
+
+
+def test_split_infinite_medium_flux():
+    """Reflective 2-group box: volume-integrated collision and track-length
+    fluxes per source particle are 5.0 and 3.75 cm."""
+    m = problems.infinite_medium_2g(20000, tracking="delta")
+    m.set(precision=abi.SPLIT)
+    p = Problem(m)
+    p.run(10, 0)
+    for t in range(2):
+        avg, err = p.tally_stats(t)
+        assert abs(avg[0] - 5.0) < 4 * err[0] + 1e-9, (t, avg, err)
+        assert abs(avg[1] - 3.75) < 4 * err[1] + 1e-9, (t, avg, err)
+
+
+def test_split_rejects_traces():
+    m = problems.fixed_source_cube(100)
+    m.set(precision=abi.SPLIT)
+    p = Problem(m)
+    bank = p.sample_sources(0, 10)
+    with pytest.raises(Exception):
+        p.transport(bank, trace_cap=8)
