@@ -56,6 +56,15 @@ def lib():
         L.abx_cancel_bank.argtypes = [P, C.c_void_p, C.c_size_t]
         L.abx_bank_size.argtypes = [P, C.POINTER(C.c_size_t)]
         L.abx_bank_fetch.argtypes = [P, C.c_void_p]
+        L.abx_batch_create.argtypes = [C.c_size_t, C.POINTER(P)]
+        L.abx_batch_destroy.argtypes = [P]
+        L.abx_batch_destroy.restype = None
+        L.abx_problem_footprint.argtypes = [P, C.POINTER(C.c_size_t)]
+        L.abx_batch_add.argtypes = [P, P, C.c_int]
+        L.abx_batch_step.argtypes = [P, C.POINTER(P), C.c_int, C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int)]
+        L.abx_batch_kernel_seconds.argtypes = [P, C.POINTER(C.c_double)]
+        L.abx_problem_stats.argtypes = [P, C.POINTER(abi.RunStats)]
         _lib = L
     return _lib
 
@@ -68,7 +77,7 @@ def _check(rc):
 class Problem:
     """One problem staged on the GPU (abx_create)."""
 
-    def __init__(self, model, engine=abi.ENGINE_EVENT):
+    def __init__(self, model, engine=abi.ENGINE_HISTORY):
         self.model = model
         self.desc = model.desc()
         h = C.c_void_p()
@@ -160,3 +169,60 @@ class Problem:
         out = np.zeros((n.value, 9))
         _check(lib().abx_bank_fetch(self.h, out.ctypes.data))
         return out
+
+    def footprint(self):
+        """Device bytes the problem holds while it runs (abx_problem_footprint)."""
+        n = C.c_size_t()
+        _check(lib().abx_problem_footprint(self.h, C.byref(n)))
+        return n.value
+
+    def stats(self):
+        """Statistics of the problem's last batch-engine run (abx_problem_stats)."""
+        st = abi.RunStats()
+        _check(lib().abx_problem_stats(self.h, C.byref(st)))
+        return st
+
+
+class Batch:
+    """Many fixed-source / modified-fixed-source problems advanced together
+    (abx_batch_*).  `budget` caps the device bytes of attached problems."""
+
+    def __init__(self, budget=0):
+        h = C.c_void_p()
+        _check(lib().abx_batch_create(int(budget), C.byref(h)))
+        self.h = h
+        self.attached = {}
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().abx_batch_destroy(self.h)
+            self.h = None
+
+    def add(self, problem, nbatches):
+        """Attach; returns False (and attaches nothing) if it does not fit the
+        budget."""
+        rc = lib().abx_batch_add(self.h, problem.h, int(nbatches))
+        if rc == abi.ERRORS_BY_NAME["ABX_E_BUDGET"]:
+            return False
+        _check(rc)
+        self.attached[problem.h.value] = problem
+        return True
+
+    def step(self):
+        """One step; returns the problems that finished in it."""
+        cap = max(1, len(self.attached))
+        out = (C.c_void_p * cap)()
+        nf = C.c_int()
+        na = C.c_int()
+        _check(lib().abx_batch_step(self.h, out, cap, C.byref(nf), C.byref(na)))
+        done = [self.attached.pop(out[k]) for k in range(nf.value)]
+        return done
+
+    def __len__(self):
+        return len(self.attached)
+
+    def kernel_seconds(self):
+        x = C.c_double()
+        _check(lib().abx_batch_kernel_seconds(self.h, C.byref(x)))
+        return x.value
+

@@ -18,6 +18,7 @@ ERRORS = {
     -6: "ABX_E_NO_DEVICE",
     -7: "ABX_E_BUDGET",
 }
+ERRORS_BY_NAME = {v: k for k, v in ERRORS.items()}
 
 # surfaces
 XPLANE, YPLANE, ZPLANE, PLANE, XCYL, YCYL, ZCYL, SPHERE = range(8)

@@ -6,9 +6,11 @@
 // with banks and tallies resident in HBM.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -66,7 +68,7 @@ struct abx_problem {
   DevProblem P{};
   DevSources S{};
   hipStream_t stream = nullptr;
-  int engine = ABX_ENGINE_EVENT;
+  int engine = ABX_ENGINE_HISTORY;
   int device = 0;
   // tallies
   DBuf tally_gen, tally_avg, tally_var;
@@ -97,6 +99,8 @@ struct abx_problem {
   double kc_avg = 0, kc_var = 0, kt_avg = 0, kt_var = 0, lk_avg = 0, lk_var = 0, mg_avg = 0,
          mg_var = 0;
   abx_scores gen_scores{};
+  abx_diag run_diag{};     // summed over batch-engine steps since attach
+  int64_t run_histories = 0;
 
   template <class V>
   const typename V::value_type* upload(const V& v) {
@@ -183,12 +187,103 @@ struct TransportOut {
   abx_diag diag{};
 };
 
+// Device scalars of one transport call (scores, counters, error, staging
+// count): a problem's own block, or a slice of a batch-wide block.
+struct CallScalars {
+  double* scores;               // [8]
+  unsigned long long* counters; // [8]
+  int32_t* err;
+  unsigned long long* err_hist;
+  unsigned long long* n_stage;
+};
+
+// Grows the problem's transport scratch for a call over n histories and
+// fills the launch context (the scalars must be zeroed by the caller).
+KCtx make_kctx(abx_problem* p, const abx_particle* d_in, int64_t n, double kcol, int converged,
+               const CallScalars& cs, SecRec* sec, int sec_cap) {
+  const HostTables& T = p->T;
+  const bool banks = T.st.mode != ABX_FIXED_SOURCE;
+  const int64_t stage_cap = banks ? 4 * n + 4096 : 0;
+  if (banks) {
+    p->stage.reserve((size_t)stage_cap * sizeof(abx_banked));
+    p->stage_parent.reserve((size_t)stage_cap * sizeof(uint32_t));
+    p->stage_daughter.reserve((size_t)stage_cap * sizeof(uint32_t));
+    p->daughters.reserve((size_t)std::max<int64_t>(1, n) * sizeof(uint32_t));
+  }
+  KCtx K{};
+  K.in = d_in;
+  K.n = n;
+  K.kcol = kcol;
+  K.converged = converged;
+  K.sec_cap = sec_cap;
+  K.sec = sec_cap ? sec : nullptr;
+  K.stage = banks ? p->stage.as<abx_banked>() : nullptr;
+  K.stage_parent = banks ? p->stage_parent.as<uint32_t>() : nullptr;
+  K.stage_daughter = banks ? p->stage_daughter.as<uint32_t>() : nullptr;
+  K.n_stage = cs.n_stage;
+  K.stage_cap = stage_cap;
+  K.daughters = banks ? p->daughters.as<uint32_t>() : nullptr;
+  K.scores = cs.scores;
+  K.counters = cs.counters;
+  K.err = cs.err;
+  K.err_hist = cs.err_hist;
+  return K;
+}
+
+// Host copies of a call's scalars -> result; raises on a device error.
+TransportOut collect_scalars(const double* sc, const unsigned long long* cn, int32_t err,
+                             unsigned long long err_hist) {
+  TransportOut res;
+  res.scores = abx_scores{sc[0], sc[1], sc[2], sc[3], sc[4], sc[5]};
+  res.diag.histories = cn[0];
+  res.diag.collisions = cn[1];
+  res.diag.virtual_collisions = cn[2];
+  res.diag.flights = cn[3];
+  res.diag.crossings = cn[4];
+  res.diag.tally_updates = cn[5];
+  res.diag.lost = cn[6];
+  res.diag.secondaries = cn[7];
+  if (err) {
+    res.diag.error = err;
+    res.diag.error_history = err_hist;
+    fail(err, "transport failed at history " + std::to_string(err_hist));
+  }
+  return res;
+}
+
+// Orders the staged fission sites of a call over n parents into p->ordered
+// (parent order, then push order); returns their number.
+int64_t order_fission_bank(abx_problem* p, int64_t n, uint64_t n_staged) {
+  const int64_t stage_cap = 4 * n + 4096;
+  if ((int64_t)n_staged > stage_cap) fail(ABX_E_CAPACITY, "fission staging overflow");
+  p->offsets.reserve((size_t)std::max<int64_t>(1, n) * sizeof(uint64_t));
+  const uint64_t total = device_scan(p, p->daughters.as<uint32_t>(), p->offsets.as<uint64_t>(), n);
+  if (total != n_staged) fail(ABX_E_INVALID, "fission site count mismatch");
+  p->ordered.reserve((size_t)std::max<uint64_t>(1, total) * sizeof(abx_banked));
+  if (total) {
+    hipLaunchKernelGGL(k_order_bank, dim3((unsigned)blocks_for((int64_t)total, 256)), dim3(256), 0,
+                       p->stream, p->stage.as<abx_banked>(), p->stage_parent.as<uint32_t>(),
+                       p->stage_daughter.as<uint32_t>(), p->offsets.as<uint64_t>(), (int64_t)total,
+                       p->ordered.as<abx_banked>());
+    HIPC(hipGetLastError());
+  }
+  return (int64_t)total;
+}
+
+void add_scores(abx_scores& a, const abx_scores& b) {
+  a.k_col += b.k_col;
+  a.k_abs += b.k_abs;
+  a.k_trk += b.k_trk;
+  a.k_tot += b.k_tot;
+  a.leakage += b.leakage;
+  a.mig += b.mig;
+}
+
 // One transport() over a device-resident bank; ordered fission sites are
 // left in p->ordered.  Tallies accumulate into tally_gen.
 TransportOut transport_device(abx_problem* p, const abx_particle* d_in, int64_t n, double kcol,
                               int converged, abx_trace_rec* d_trace, int trace_cap,
                               int32_t* d_nrec) {
-  TransportOut res;
   const HostTables& T = p->T;
   if (d_trace && T.st.precision == ABX_SPLIT)
     fail(ABX_E_INVALID, "event traces are recorded in FP64 precision only");
@@ -202,33 +297,12 @@ TransportOut transport_device(abx_problem* p, const abx_particle* d_in, int64_t 
   HIPC(hipMemsetAsync(p->err.p, 0, sizeof(int32_t), p->stream));
   HIPC(hipMemsetAsync(p->n_stage.p, 0, sizeof(unsigned long long), p->stream));
   const bool banks = T.st.mode != ABX_FIXED_SOURCE;
-  const int64_t stage_cap = banks ? 4 * n + 4096 : 0;
-  if (banks) {
-    p->stage.reserve((size_t)stage_cap * sizeof(abx_banked));
-    p->stage_parent.reserve((size_t)stage_cap * sizeof(uint32_t));
-    p->stage_daughter.reserve((size_t)stage_cap * sizeof(uint32_t));
-    p->daughters.reserve((size_t)std::max<int64_t>(1, n) * sizeof(uint32_t));
-  }
   const int sec_cap = needs_secondaries(T) ? 16 : 0;
   if (sec_cap) p->sec.reserve((size_t)std::max<int64_t>(1, n) * sec_cap * sizeof(SecRec));
-
-  KCtx K{};
-  K.in = d_in;
-  K.n = n;
-  K.kcol = kcol;
-  K.converged = converged;
-  K.sec_cap = sec_cap;
-  K.sec = sec_cap ? p->sec.as<SecRec>() : nullptr;
-  K.stage = banks ? p->stage.as<abx_banked>() : nullptr;
-  K.stage_parent = banks ? p->stage_parent.as<uint32_t>() : nullptr;
-  K.stage_daughter = banks ? p->stage_daughter.as<uint32_t>() : nullptr;
-  K.n_stage = p->n_stage.as<unsigned long long>();
-  K.stage_cap = stage_cap;
-  K.daughters = banks ? p->daughters.as<uint32_t>() : nullptr;
-  K.scores = p->scores.as<double>();
-  K.counters = p->counters.as<unsigned long long>();
-  K.err = p->err.as<int32_t>();
-  K.err_hist = p->err_hist.as<unsigned long long>();
+  const CallScalars cs{p->scores.as<double>(), p->counters.as<unsigned long long>(),
+                       p->err.as<int32_t>(), p->err_hist.as<unsigned long long>(),
+                       p->n_stage.as<unsigned long long>()};
+  KCtx K = make_kctx(p, d_in, n, kcol, converged, cs, p->sec.as<SecRec>(), sec_cap);
   K.trace = d_trace;
   K.trace_cap = trace_cap;
   K.n_rec = d_nrec;
@@ -270,41 +344,9 @@ TransportOut transport_device(abx_problem* p, const abx_particle* d_in, int64_t 
   HIPC(hipMemcpyAsync(&err_hist, p->err_hist.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, p->stream));
   HIPC(hipMemcpyAsync(&nst, p->n_stage.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, p->stream));
   HIPC(hipStreamSynchronize(p->stream));
-  res.scores = abx_scores{sc[0], sc[1], sc[2], sc[3], sc[4], sc[5]};
-  res.diag.histories = cn[0];
-  res.diag.collisions = cn[1];
-  res.diag.virtual_collisions = cn[2];
-  res.diag.flights = cn[3];
-  res.diag.crossings = cn[4];
-  res.diag.tally_updates = cn[5];
-  res.diag.lost = cn[6];
-  res.diag.secondaries = cn[7];
-  if (err) {
-    res.diag.error = err;
-    res.diag.error_history = err_hist;
-    fail(err, "transport failed at history " + std::to_string(err_hist));
-  }
-  if (banks) {
-    if ((int64_t)nst > stage_cap) fail(ABX_E_CAPACITY, "fission staging overflow");
-    p->offsets.reserve((size_t)std::max<int64_t>(1, n) * sizeof(uint64_t));
-    const uint64_t total = device_scan(p, p->daughters.as<uint32_t>(), p->offsets.as<uint64_t>(), n);
-    if (total != nst) fail(ABX_E_INVALID, "fission site count mismatch");
-    p->ordered.reserve((size_t)std::max<uint64_t>(1, total) * sizeof(abx_banked));
-    if (total) {
-      hipLaunchKernelGGL(k_order_bank, dim3((unsigned)blocks_for((int64_t)total, 256)), dim3(256), 0,
-                         p->stream, p->stage.as<abx_banked>(), p->stage_parent.as<uint32_t>(),
-                         p->stage_daughter.as<uint32_t>(), p->offsets.as<uint64_t>(), (int64_t)total,
-                         p->ordered.as<abx_banked>());
-      HIPC(hipGetLastError());
-    }
-    res.n_sites = (int64_t)total;
-  }
-  p->gen_scores.k_col += res.scores.k_col;
-  p->gen_scores.k_abs += res.scores.k_abs;
-  p->gen_scores.k_trk += res.scores.k_trk;
-  p->gen_scores.k_tot += res.scores.k_tot;
-  p->gen_scores.leakage += res.scores.leakage;
-  p->gen_scores.mig += res.scores.mig;
+  TransportOut res = collect_scalars(sc, cn, err, err_hist);
+  if (banks) res.n_sites = order_fission_bank(p, n, nst);
+  add_scores(p->gen_scores, res.scores);
   return res;
 }
 
@@ -588,6 +630,20 @@ static void run_reset(abx_problem* p) {
   p->bank_n = 0;
 }
 
+// Tallies' running means and standard errors (src/tallies.cpp:268-280).
+static void fill_statistics(const abx_problem* p, abx_run_stats* stats) {
+  const double dg = (double)std::max(1, p->gen);
+  stats->kcol_avg = p->kc_avg;
+  stats->kcol_err = std::sqrt(p->kc_var / dg);
+  stats->ktrk_avg = p->kt_avg;
+  stats->ktrk_err = std::sqrt(p->kt_var / dg);
+  stats->leak_avg = p->lk_avg;
+  stats->leak_err = std::sqrt(p->lk_var / dg);
+  stats->mig_avg = p->mg_avg;
+  stats->mig_err = std::sqrt(p->mg_var / dg);
+  stats->active_generations = p->gen;
+}
+
 // FixedSource::run / ModifiedFixedSource::run / PowerIterator::run
 // (src/fixed_source.cpp:81-173, src/modified_fixed_source.cpp:59-143,
 // src/power_iterator.cpp:305-429) with every bank resident in HBM.
@@ -700,16 +756,7 @@ static void run_generations(abx_problem* p, int ngenerations, int nignored, doub
   p->timing = false;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  const double dg = (double)std::max(1, p->gen);
-  stats->kcol_avg = p->kc_avg;
-  stats->kcol_err = std::sqrt(p->kc_var / dg);
-  stats->ktrk_avg = p->kt_avg;
-  stats->ktrk_err = std::sqrt(p->kt_var / dg);
-  stats->leak_avg = p->lk_avg;
-  stats->leak_err = std::sqrt(p->lk_var / dg);
-  stats->mig_avg = p->mg_avg;
-  stats->mig_err = std::sqrt(p->mg_var / dg);
-  stats->active_generations = p->gen;
+  fill_statistics(p, stats);
   stats->transport_seconds = ms_total * 1e-3;
   stats->kernel_seconds = p->kernel_ms * 1e-3;
   stats->kernel_launches = p->kernel_launches;
@@ -774,6 +821,312 @@ int abx_bank_fetch(abx_problem* p, double* out9) {
       o[7] = b[i].wgt;
       o[8] = b[i].wgt2;
     }
+  });
+}
+
+}  // extern "C"
+
+// ============================================================== batches
+// Many fixed-source / modified-fixed-source problems advanced together: one
+// sampling launch per problem, then ONE persistent transport launch per
+// kernel variant (tracking x secondaries x precision) over the histories of
+// every attached problem, then per-problem bookkeeping.  A problem's results
+// are those of abx_run on it alone (same histories, same RNG streams, same
+// ordered banks); only the summation order of tallies and scores differs.
+
+namespace {
+
+struct BatchEntry {
+  abx_problem* p;
+  int batches_left;
+  bool mid_chain;     // a modified-fixed-source generation chain is running
+  int64_t n;          // histories of the current call
+  abx_scores acc{};   // scores of the current batch (chain summed)
+};
+
+// Per-entry device scalars in the batch-wide block.
+struct BatchScalars {
+  double sc[8];
+  unsigned long long cn[8];
+  unsigned long long err_hist;
+  unsigned long long n_stage;
+  int32_t err;
+  int32_t pad_;
+};
+
+int variant_key(const abx_problem* p) {
+  const HostTables& T = p->T;
+  const int sec = needs_secondaries(T) ? 1 : 0;
+  return T.st.tracking * 4 + sec * 2 + (T.st.precision == ABX_SPLIT ? 1 : 0);
+}
+
+template <int TRK, bool SEC, bool SPLIT>
+hipError_t launch_batch(const DevProblem* probs, const KCtx* ctxs, const int64_t* offsets, int n_items,
+                        int64_t total, int D, int L, unsigned long long* next, hipStream_t s) {
+  const int bs = 256;
+  const size_t lds = (size_t)bs * trk_bytes(D, L);
+  hipError_t e;
+  if ((e = hipMemsetAsync(next, 0, sizeof(unsigned long long), s)) != hipSuccess) return e;
+  const int64_t want = (total + bs - 1) / bs;
+  const int64_t blocks =
+      std::min<int64_t>(want, resident_blocks(k_batch_history<TRK, SEC, SPLIT>, bs, lds));
+  hipLaunchKernelGGL((k_batch_history<TRK, SEC, SPLIT>), dim3((unsigned)blocks), dim3(bs), lds, s,
+                     probs, ctxs, offsets, n_items, D, L, next);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_variant(int key, const DevProblem* probs, const KCtx* ctxs,
+                                const int64_t* offsets, int n_items, int64_t total, int D, int L,
+                                unsigned long long* next, hipStream_t s) {
+  const int trk = key / 4;
+  const bool sec = (key / 2) & 1, split = key & 1;
+  return dispatch_variant(trk, false, sec || trk == 2, split, [&](auto T, auto, auto SE, auto SP) {
+    return launch_batch<decltype(T)::value, decltype(SE)::value, decltype(SP)::value>(
+        probs, ctxs, offsets, n_items, total, D, L, next, s);
+  });
+}
+
+// Bytes a problem holds on the device while it runs (tables, tallies,
+// banks, fission staging) - what admission checks against a budget.
+size_t problem_footprint(const abx_problem* p) {
+  const HostTables& T = p->T;
+  const size_t N = (size_t)T.st.nparticles;
+  size_t b = 0;
+  for (const auto& a : p->arrays) b += a.bytes;
+  b += 3 * (size_t)std::max<int64_t>(1, T.tally_bins) * sizeof(double);
+  b += 2 * N * sizeof(abx_particle);
+  if (T.st.mode != ABX_FIXED_SOURCE)
+    b += (4 * N + 4096) * (sizeof(abx_banked) + 2 * sizeof(uint32_t)) +
+         N * (sizeof(uint32_t) + sizeof(uint64_t) + sizeof(abx_banked));
+  return b;
+}
+
+}  // namespace
+
+struct abx_batch {
+  std::vector<BatchEntry> items;
+  hipStream_t stream = nullptr;
+  DBuf probs, ctxs, offsets, scal, next, sec;
+  std::vector<BatchScalars> h_scal;
+  size_t budget = 0;  // device bytes admitted problems may hold (0 = no limit)
+  size_t used = 0;
+  double kernel_ms = 0.0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  ~abx_batch() {
+    DBuf* all[] = {&probs, &ctxs, &offsets, &scal, &next, &sec};
+    for (DBuf* b : all) b->release();
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+void batch_step(abx_batch* b, std::vector<abx_problem*>& finished) {
+  const int ni = (int)b->items.size();
+  if (ni == 0) return;
+  b->scal.reserve((size_t)ni * sizeof(BatchScalars));
+  HIPC(hipMemsetAsync(b->scal.p, 0, (size_t)ni * sizeof(BatchScalars), b->stream));
+  BatchScalars* ds = b->scal.as<BatchScalars>();
+  // 1. inputs: a new batch of sources, or the chain's re-banked generation
+  for (int j = 0; j < ni; j++) {
+    BatchEntry& it = b->items[j];
+    abx_problem* p = it.p;
+    if (it.mid_chain) continue;
+    const int64_t N = p->T.st.nparticles;
+    p->bank_a.reserve((size_t)N * sizeof(abx_particle));
+    hipLaunchKernelGGL(k_sample_sources, dim3((unsigned)blocks_for(N, 256)), dim3(256),
+                       (size_t)256 * trk_bytes(p->P.trk_D, p->P.trk_L), b->stream, p->P, p->S,
+                       p->run_hist, N, p->bank_a.as<abx_particle>(), &ds[j].err, &ds[j].err_hist);
+    HIPC(hipGetLastError());
+    p->run_hist += (uint64_t)N;
+    it.n = N;
+    it.acc = abx_scores{};
+  }
+  // 2. one launch per kernel variant
+  std::vector<int> order(ni);
+  for (int j = 0; j < ni; j++) order[j] = j;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+    return variant_key(b->items[a].p) < variant_key(b->items[c].p);
+  });
+  int max_sec = 0;
+  for (auto& it : b->items) max_sec = std::max(max_sec, needs_secondaries(it.p->T) ? 16 : 0);
+  int dev = 0, cus = 0;
+  HIPC(hipGetDevice(&dev));
+  HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const size_t lanes = (size_t)cus * 8 * 256;  // upper bound on resident lanes
+  if (max_sec) b->sec.reserve(lanes * max_sec * sizeof(SecRec));
+  std::vector<DevProblem> hp(ni);
+  std::vector<KCtx> hk(ni);
+  std::vector<int64_t> ho(ni + 1);
+  b->probs.reserve((size_t)ni * sizeof(DevProblem));
+  b->ctxs.reserve((size_t)ni * sizeof(KCtx));
+  b->offsets.reserve((size_t)(ni + 16) * sizeof(int64_t));  // + one entry per variant
+  b->next.reserve(8 * sizeof(unsigned long long));
+  if (!b->e0) {
+    HIPC(hipEventCreate(&b->e0));
+    HIPC(hipEventCreate(&b->e1));
+  }
+  HIPC(hipEventRecord(b->e0, b->stream));
+  int g0 = 0, launches = 0;
+  while (g0 < ni) {
+    const int key = variant_key(b->items[order[g0]].p);
+    int g1 = g0;
+    int D = 1, L = 1;
+    int64_t total = 0;
+    while (g1 < ni && variant_key(b->items[order[g1]].p) == key) {
+      const int j = order[g1];
+      BatchEntry& it = b->items[j];
+      abx_problem* p = it.p;
+      const CallScalars cs{ds[j].sc, ds[j].cn, &ds[j].err, &ds[j].err_hist, &ds[j].n_stage};
+      const int sec_cap = needs_secondaries(p->T) ? 16 : 0;
+      hk[g1] = make_kctx(p, p->bank_a.as<abx_particle>(), it.n, p->run_kcol, 1, cs,
+                         b->sec.as<SecRec>(), sec_cap);
+      hp[g1] = p->P;
+      ho[g1] = total;
+      total += it.n;
+      D = std::max(D, p->P.trk_D);
+      L = std::max(L, p->P.trk_L);
+      g1++;
+    }
+    ho[g1] = total;
+    const int cnt = g1 - g0;
+    // per-group slices of the device arrays (offsets rebased to the group)
+    std::vector<int64_t> off(cnt + 1);
+    for (int k = 0; k <= cnt; k++) off[k] = ho[g0 + k] - ho[g0];
+    HIPC(hipMemcpyAsync(b->probs.as<DevProblem>() + g0, hp.data() + g0, cnt * sizeof(DevProblem),
+                        hipMemcpyHostToDevice, b->stream));
+    HIPC(hipMemcpyAsync(b->ctxs.as<KCtx>() + g0, hk.data() + g0, cnt * sizeof(KCtx),
+                        hipMemcpyHostToDevice, b->stream));
+    HIPC(hipMemcpyAsync(b->offsets.as<int64_t>() + g0 + launches, off.data(), (cnt + 1) * sizeof(int64_t),
+                        hipMemcpyHostToDevice, b->stream));
+    if (total > 0)
+      HIPC(launch_batch_variant(key, b->probs.as<DevProblem>() + g0, b->ctxs.as<KCtx>() + g0,
+                                b->offsets.as<int64_t>() + g0 + launches, cnt, total, D, L,
+                                b->next.as<unsigned long long>() + (launches % 8), b->stream));
+    launches++;
+    g0 = g1;
+  }
+  HIPC(hipEventRecord(b->e1, b->stream));
+  b->h_scal.resize(ni);
+  HIPC(hipMemcpyAsync(b->h_scal.data(), b->scal.p, (size_t)ni * sizeof(BatchScalars),
+                      hipMemcpyDeviceToHost, b->stream));
+  HIPC(hipStreamSynchronize(b->stream));
+  float ms = 0.f;
+  HIPC(hipEventElapsedTime(&ms, b->e0, b->e1));
+  b->kernel_ms += ms;
+  // 3. per-problem bookkeeping
+  std::vector<BatchEntry> keep;
+  keep.reserve(ni);
+  for (int j = 0; j < ni; j++) {
+    BatchEntry& it = b->items[j];
+    abx_problem* p = it.p;
+    const HostTables& T = p->T;
+    const BatchScalars& h = b->h_scal[j];
+    TransportOut r = collect_scalars(h.sc, h.cn, h.err, h.err_hist);
+    add_diag(p->run_diag, r.diag);
+    p->run_histories += it.n;
+    add_scores(it.acc, r.scores);
+    int64_t n_sites = 0;
+    if (T.st.mode == ABX_MODIFIED_FIXED_SOURCE) n_sites = order_fission_bank(p, it.n, h.n_stage);
+    if (n_sites > 0) {
+      if (p->cancel) cancel_device(p, p->ordered.as<abx_banked>(), n_sites);
+      p->bank_b.reserve((size_t)n_sites * sizeof(abx_particle));
+      hipLaunchKernelGGL(k_rebank, dim3((unsigned)blocks_for(n_sites, 256)), dim3(256), 0, p->stream,
+                         p->ordered.as<abx_banked>(), n_sites, 1.0, p->run_hist, T.st.seed,
+                         T.st.stride, 1, 0, p->bank_b.as<abx_particle>());
+      HIPC(hipGetLastError());
+      p->run_hist += (uint64_t)n_sites;
+      std::swap(p->bank_a, p->bank_b);
+      p->bank_n = (size_t)n_sites;
+      it.n = n_sites;
+      it.mid_chain = true;
+    } else {
+      it.mid_chain = false;
+      record_generation(p, it.acc, (double)T.st.nparticles);
+      clear_generation(p);
+      it.batches_left--;
+    }
+    HIPC(hipStreamSynchronize(p->stream));
+    if (it.batches_left > 0) {
+      keep.push_back(it);
+    } else {
+      b->used -= std::min(b->used, problem_footprint(p));
+      finished.push_back(p);
+    }
+  }
+  b->items.swap(keep);
+}
+
+}  // namespace
+
+extern "C" {
+
+int abx_batch_create(size_t budget_bytes, abx_batch** out) {
+  ABX_TRY({
+    if (!out) fail(ABX_E_INVALID, "null argument");
+    auto b = std::make_unique<abx_batch>();
+    b->budget = budget_bytes;
+    HIPC(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    *out = b.release();
+  });
+}
+
+void abx_batch_destroy(abx_batch* b) { delete b; }
+
+int abx_problem_footprint(abx_problem* p, size_t* bytes) {
+  ABX_TRY({
+    if (!p || !bytes) fail(ABX_E_INVALID, "null argument");
+    *bytes = problem_footprint(p);
+  });
+}
+
+int abx_batch_add(abx_batch* b, abx_problem* p, int nbatches) {
+  ABX_TRY({
+    if (!b || !p || nbatches < 1) fail(ABX_E_INVALID, "batch arguments");
+    if (p->T.st.mode == ABX_K_EIGENVALUE)
+      fail(ABX_E_INVALID, "k-eigenvalue problems run with abx_run, not in a batch");
+    for (const auto& it : b->items)
+      if (it.p == p) fail(ABX_E_INVALID, "problem already attached");
+    const size_t need = problem_footprint(p);
+    if (b->budget && b->used + need > b->budget)
+      fail(ABX_E_BUDGET, "problem does not fit the batch's device budget");
+    run_reset(p);
+    p->run_diag = abx_diag{};
+    p->run_histories = 0;
+    b->used += need;
+    b->items.push_back(BatchEntry{p, nbatches, false, 0, abx_scores{}});
+  });
+}
+
+int abx_batch_step(abx_batch* b, abx_problem** finished, int cap, int* n_finished,
+                   int* n_active) {
+  ABX_TRY({
+    if (!b) fail(ABX_E_INVALID, "null argument");
+    std::vector<abx_problem*> done;
+    batch_step(b, done);
+    if (n_finished) *n_finished = (int)done.size();
+    if ((int)done.size() > cap && finished) fail(ABX_E_CAPACITY, "finished list too small");
+    for (size_t k = 0; k < done.size() && finished; k++) finished[k] = done[k];
+    if (n_active) *n_active = (int)b->items.size();
+  });
+}
+
+int abx_batch_kernel_seconds(abx_batch* b, double* seconds) {
+  ABX_TRY({
+    if (!b || !seconds) fail(ABX_E_INVALID, "null argument");
+    *seconds = b->kernel_ms * 1e-3;
+  });
+}
+
+int abx_problem_stats(abx_problem* p, abx_run_stats* stats) {
+  ABX_TRY({
+    if (!p || !stats) fail(ABX_E_INVALID, "null argument");
+    *stats = abx_run_stats{};
+    fill_statistics(p, stats);
+    stats->diag = p->run_diag;
+    stats->histories = p->run_histories;
   });
 }
 

@@ -568,6 +568,12 @@ ABX_NOINL Bound trk_bc(const DevProblem& P, const Trk& t, const V3& u) {
   return b;
 }
 
+// The same out of line, for delta / carter tracking, which only needs it
+// when a flight left the geometry.
+ABX_COLD Bound trk_bc_cold(const DevProblem& P, const Trk& t, const V3& u) {
+  return trk_bc(P, t, u);
+}
+
 // Tracker::get_nearest_boundary
 ABX_NOINL Bound trk_nearest(const DevProblem& P, const Trk& t, const V3& u) {
   Bound b = trk_bc(P, t, u);

@@ -619,7 +619,7 @@ ABX_HD bool step_flight(const DevProblem& P, Sink& s, Part& p, Trk& t, int& m, b
       p.alive = 0;
       return false;
     }
-    b = trk_bc(P, t, p.u);
+    b = trk_bc_cold(P, t, p.u);
   }
   if (converged && P.ntally) score_flight_tallies(P, s, p, d_coll < b.d ? d_coll : b.d, m);
   if (lost) {

@@ -89,7 +89,7 @@ def main():
     ap.add_argument("--workload", default="lattice", choices=["lattice", "core", "cube"])
     ap.add_argument("--particles", type=float, default=1e6)
     ap.add_argument("--tracking", default="delta", choices=["delta", "surface"])
-    ap.add_argument("--engine", default="event", choices=["event", "history"])
+    ap.add_argument("--engine", default="history", choices=["event", "history"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "split"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")

@@ -287,8 +287,9 @@ typedef struct abx_problem abx_problem;
 
 /* Which kernel family transport uses. */
 enum abx_engine {
-  ABX_ENGINE_EVENT = 0,   /* event-based: SoA banks, per-event kernels */
-  ABX_ENGINE_HISTORY = 1  /* one lane per history (parity vehicle) */
+  ABX_ENGINE_EVENT = 0,   /* event-based: SoA banks, one kernel per event kind */
+  ABX_ENGINE_HISTORY = 1  /* persistent grid, one history per lane at a time,
+                             lanes refilled from a queue (the default) */
 };
 
 const char* abx_last_error(void);
@@ -405,6 +406,34 @@ int abx_transport_trace(abx_problem* p, const abx_particle* bank, size_t n,
                         size_t cap, size_t* n_out, abx_scores* scores,
                         abx_trace_rec* trace, int cap_per_history,
                         int32_t* n_rec);
+
+/* ---- batches: many problems per launch ----------------------------------- */
+/* Fixed-source and modified-fixed-source problems advanced together.  Each
+ * step gives every attached problem one transport call - a new batch of
+ * sources (FixedSource::run, src/fixed_source.cpp:81-173) or the next
+ * generation of its fission chain (ModifiedFixedSource::run,
+ * src/modified_fixed_source.cpp:59-143) - and all those histories share one
+ * persistent launch per kernel variant.  Per-problem results equal abx_run
+ * on the problem alone up to the summation order of tallies and scores. */
+typedef struct abx_batch abx_batch;
+
+/* budget_bytes: device bytes the attached problems may hold (0 = no limit). */
+int abx_batch_create(size_t budget_bytes, abx_batch** out);
+void abx_batch_destroy(abx_batch* b);
+/* Device bytes a problem holds while attached (tables, tallies, banks). */
+int abx_problem_footprint(abx_problem* p, size_t* bytes);
+/* Attach p to run `nbatches` batches of its nparticles (statistics and
+ * tallies restart, as in abx_run).  ABX_E_BUDGET if it does not fit. */
+int abx_batch_add(abx_batch* b, abx_problem* p, int nbatches);
+/* One step.  Problems whose last batch was recorded are detached and listed
+ * in `finished` (capacity cap, count in *n_finished); *n_active receives the
+ * number still attached. */
+int abx_batch_step(abx_batch* b, abx_problem** finished, int cap, int* n_finished,
+                   int* n_active);
+/* GPU time of the batch's transport launches so far. */
+int abx_batch_kernel_seconds(abx_batch* b, double* seconds);
+/* Statistics of a problem's batch-engine run (or of its last abx_run). */
+int abx_problem_stats(abx_problem* p, abx_run_stats* stats);
 
 #ifdef __cplusplus
 }
