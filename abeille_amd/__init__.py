@@ -38,6 +38,8 @@ def lib():
         L.abx_destroy.argtypes = [P]
         L.abx_destroy.restype = None
         L.abx_set_engine.argtypes = [P, C.c_int]
+        L.abx_set_tuning.argtypes = [P, C.POINTER(abi.Tuning)]
+        L.abx_get_tuning.argtypes = [P, C.POINTER(abi.Tuning)]
         L.abx_sample_sources.argtypes = [P, C.c_uint64, C.c_size_t, C.c_void_p]
         L.abx_transport.argtypes = [P, C.c_void_p, C.c_size_t, C.c_double, C.c_int, C.c_void_p,
                                     C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(abi.Scores),
@@ -56,6 +58,7 @@ def lib():
         L.abx_cancel_bank.argtypes = [P, C.c_void_p, C.c_size_t]
         L.abx_bank_size.argtypes = [P, C.POINTER(C.c_size_t)]
         L.abx_bank_fetch.argtypes = [P, C.c_void_p]
+        L.abx_bank_load.argtypes = [P, C.c_void_p, C.c_size_t]
         L.abx_batch_create.argtypes = [C.c_size_t, C.POINTER(P)]
         L.abx_batch_destroy.argtypes = [P]
         L.abx_batch_destroy.restype = None
@@ -169,6 +172,22 @@ class Problem:
         out = np.zeros((n.value, 9))
         _check(lib().abx_bank_fetch(self.h, out.ctypes.data))
         return out
+
+    def load_bank(self, bank9):
+        """Seed the next generation with a [n][9] source (abx_bank_load)."""
+        b = np.ascontiguousarray(bank9, dtype=np.float64)
+        assert b.ndim == 2 and b.shape[1] == 9
+        _check(lib().abx_bank_load(self.h, b.ctypes.data, len(b)))
+
+    def set_tuning(self, block_size=256, blocks_per_cu=0, stage_tables=1, tail_threshold=0):
+        t = abi.Tuning(block_size, blocks_per_cu, stage_tables, tail_threshold)
+        _check(lib().abx_set_tuning(self.h, C.byref(t)))
+
+    def tuning(self):
+        t = abi.Tuning()
+        _check(lib().abx_get_tuning(self.h, C.byref(t)))
+        return dict(block_size=t.block_size, blocks_per_cu=t.blocks_per_cu,
+                    stage_tables=t.stage_tables, tail_threshold=t.tail_threshold)
 
     def footprint(self):
         """Device bytes the problem holds while it runs (abx_problem_footprint)."""

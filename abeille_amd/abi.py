@@ -244,3 +244,9 @@ def tokenize_region(region, surface_index_of_id):
             raise ValueError("Invalid character in cell region definition.")
     flush()
     return out
+
+
+class Tuning(C.Structure):
+    _fields_ = [("block_size", C.c_int32), ("blocks_per_cu", C.c_int32),
+                ("stage_tables", C.c_int32), ("tail_threshold", C.c_int32)]
+

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -69,6 +70,7 @@ struct abx_problem {
   DevSources S{};
   hipStream_t stream = nullptr;
   int engine = ABX_ENGINE_HISTORY;
+  abx_tuning tuning{256, 0, 1, 0};
   int device = 0;
   // tallies
   DBuf tally_gen, tally_avg, tally_var;
@@ -76,7 +78,7 @@ struct abx_problem {
   // transport scratch
   DBuf in, sec, stage, stage_parent, stage_daughter, n_stage, daughters, offsets,
       block_sums, block_sums2, ordered, scores, counters, err, err_hist, trace, n_rec, wsum,
-      pool_next;
+      pool_next, blob;
   EventBuffers ev;
   // cancellation
   bool cancel = false;
@@ -117,7 +119,7 @@ struct abx_problem {
     DBuf* all[] = {&tally_gen, &tally_avg, &tally_var, &in, &sec, &stage, &stage_parent,
                    &stage_daughter, &n_stage, &daughters, &offsets, &block_sums, &block_sums2,
                    &ordered, &scores, &counters, &err, &err_hist, &trace, &n_rec, &wsum,
-                   &bank_a, &bank_b, &ckeys, &ckeys2, &ctemp, &pool_next};
+                   &bank_a, &bank_b, &ckeys, &ckeys2, &ctemp, &pool_next, &blob};
     for (DBuf* b : all) b->release();
     ev.release();
     if (ke0) (void)hipEventDestroy(ke0);
@@ -314,7 +316,8 @@ TransportOut transport_device(abx_problem* p, const abx_particle* d_in, int64_t 
     if (p->engine == ABX_ENGINE_HISTORY) {
       if (p->timing) HIPC(hipEventRecord(p->ke0, p->stream));
       p->pool_next.reserve(sizeof(unsigned long long));
-      HIPC(history_transport(p->P, K, p->stream, sec_cap > 0, p->pool_next.as<unsigned long long>()));
+      HIPC(history_transport(p->P, K, p->stream, sec_cap > 0, p->pool_next.as<unsigned long long>(),
+                             p->tuning));
       if (p->timing) {
         HIPC(hipEventRecord(p->ke1, p->stream));
         HIPC(hipEventSynchronize(p->ke1));
@@ -451,8 +454,33 @@ int abx_create(const abx_problem_desc* desc, abx_problem** out) {
     HIPC(hipMemset(p->tally_gen.p, 0, tb));
     HIPC(hipMemset(p->tally_avg.p, 0, tb));
     HIPC(hipMemset(p->tally_var.p, 0, tb));
-    auto map = [p](const auto& v) { return p->upload(v); };
-    p->P = dev_view(T, map, p->tally_gen.as<double>());
+    // all tables in one 16-byte-aligned block: pass 1 sizes it, pass 2
+    // fills a host image that is uploaded in one copy
+    size_t blob_bytes = 0;
+    auto size_map = [&blob_bytes](const auto& v) {
+      using E = typename std::decay_t<decltype(v)>::value_type;
+      blob_bytes = (blob_bytes + 15) & ~(size_t)15;
+      blob_bytes += std::max<size_t>(1, v.size()) * sizeof(E);
+      return (const E*)nullptr;
+    };
+    (void)dev_view(T, size_map, nullptr);
+    blob_bytes = (blob_bytes + 15) & ~(size_t)15;
+    p->blob.reserve(std::max<size_t>(16, blob_bytes));
+    std::vector<unsigned char> image(std::max<size_t>(16, blob_bytes), 0);
+    size_t off = 0;
+    unsigned char* dbase = p->blob.as<unsigned char>();
+    auto fill_map = [&](const auto& v) {
+      using E = typename std::decay_t<decltype(v)>::value_type;
+      off = (off + 15) & ~(size_t)15;
+      const E* d = (const E*)(dbase + off);
+      if (!v.empty()) std::memcpy(image.data() + off, v.data(), v.size() * sizeof(E));
+      off += std::max<size_t>(1, v.size()) * sizeof(E);
+      return d;
+    };
+    p->P = dev_view(T, fill_map, p->tally_gen.as<double>());
+    HIPC(hipMemcpy(p->blob.p, image.data(), image.size(), hipMemcpyHostToDevice));
+    p->P.blob = dbase;
+    p->P.blob_bytes = (int64_t)blob_bytes;
     std::vector<double> w, cdf;
     for (auto& s : T.sources) w.push_back(s.weight);
     cdf.resize(w.size());
@@ -504,6 +532,24 @@ int abx_set_engine(abx_problem* p, int engine) {
   ABX_TRY({
     if (!p || (engine != ABX_ENGINE_EVENT && engine != ABX_ENGINE_HISTORY)) fail(ABX_E_INVALID, "engine");
     p->engine = engine;
+  });
+}
+
+int abx_set_tuning(abx_problem* p, const abx_tuning* t) {
+  ABX_TRY({
+    if (!p || !t) fail(ABX_E_INVALID, "null argument");
+    if (t->block_size != 64 && t->block_size != 128 && t->block_size != 256)
+      fail(ABX_E_INVALID, "block_size must be 64, 128 or 256");
+    if (t->blocks_per_cu < 0 || t->tail_threshold < 0) fail(ABX_E_INVALID, "tuning values");
+    p->tuning = *t;
+    p->ev.tail_threshold = t->tail_threshold;
+  });
+}
+
+int abx_get_tuning(abx_problem* p, abx_tuning* t) {
+  ABX_TRY({
+    if (!p || !t) fail(ABX_E_INVALID, "null argument");
+    *t = p->tuning;
   });
 }
 
@@ -800,6 +846,40 @@ int abx_bank_size(abx_problem* p, size_t* n) {
   ABX_TRY({
     if (!p || !n) fail(ABX_E_INVALID, "null argument");
     *n = p->bank_n;
+  });
+}
+
+int abx_bank_load(abx_problem* p, const double* in9, size_t n) {
+  ABX_TRY({
+    if (!p || (!in9 && n)) fail(ABX_E_INVALID, "null argument");
+    if (p->T.st.mode != ABX_K_EIGENVALUE) fail(ABX_E_INVALID, "a source bank seeds k-eigenvalue runs");
+    if (n == 0) fail(ABX_E_INVALID, "empty source bank");
+    std::vector<abx_particle> b(n);
+    double tot = 0.0;
+    for (size_t i = 0; i < n; i++) {
+      const double* s9 = in9 + 9 * i;
+      abx_particle& a = b[i];
+      const double m = std::sqrt(s9[3] * s9[3] + s9[4] * s9[4] + s9[5] * s9[5]);
+      for (int k = 0; k < 3; k++) {
+        a.r[k] = s9[k];
+        a.u[k] = s9[3 + k] / m;  // Direction's normalising constructor
+      }
+      a.E = s9[6];
+      a.wgt = s9[7];
+      a.wgt2 = s9[8];
+      a.history_id = i;
+      a.family_id = 0;
+      a.rng_state = pcg_history_state(p->T.st.seed, p->T.st.stride, i);
+      tot += s9[7];
+    }
+    run_reset(p);
+    p->bank_a.reserve(n * sizeof(abx_particle));
+    HIPC(hipMemcpyAsync(p->bank_a.p, b.data(), n * sizeof(abx_particle), hipMemcpyHostToDevice, p->stream));
+    HIPC(hipStreamSynchronize(p->stream));
+    p->bank_n = n;
+    p->run_hist = n;
+    p->run_started = true;
+    p->T.st.nparticles = std::max<int64_t>(1, std::llround(tot));
   });
 }
 

@@ -141,4 +141,53 @@ struct DevProblem {
   double* tally_data;
   // per-group tally energy bin of the group midpoint, [ntally][G]
   const int32_t* tally_gbin;
+  // Every table above except tally_data lives in one contiguous block
+  // [blob, blob + blob_bytes) so that a kernel can stage it in LDS.
+  const unsigned char* blob;
+  int64_t blob_bytes;
 };
+
+// The same problem with its tables read from `base`, a copy of the blob
+// (in LDS): every table pointer of dev_view() points into the blob.
+template <class T>
+ABX_HD const T* rebase_ptr(const T* p, int64_t delta) {
+  return (const T*)((const unsigned char*)p + delta);
+}
+
+ABX_HD DevProblem rebase_problem(const DevProblem& P, const unsigned char* base) {
+  DevProblem Q = P;
+  const int64_t d = (int64_t)(base - P.blob);
+#define ABX_REBASE(f) Q.f = rebase_ptr(P.f, d)
+  ABX_REBASE(surf);
+  ABX_REBASE(cell);
+  ABX_REBASE(rpn);
+  ABX_REBASE(univ);
+  ABX_REBASE(ucells);
+  ABX_REBASE(elems);
+  ABX_REBASE(ebounds);
+  ABX_REBASE(emid);
+  ABX_REBASE(majorant);
+  ABX_REBASE(sampling);
+  ABX_REBASE(Et);
+  ABX_REBASE(Eabs);
+  ABX_REBASE(Ef);
+  ABX_REBASE(nu_tot);
+  ABX_REBASE(nu_d);
+  ABX_REBASE(Eel);
+  ABX_REBASE(vEf);
+  ABX_REBASE(fissile);
+  ABX_REBASE(scat_cdf);
+  ABX_REBASE(chi_cdf);
+  ABX_REBASE(ang_off);
+  ABX_REBASE(ang_mu);
+  ABX_REBASE(ang_pdf);
+  ABX_REBASE(ang_cdf);
+  ABX_REBASE(dgrp_n);
+  ABX_REBASE(dgrp_cdf);
+  ABX_REBASE(tally);
+  ABX_REBASE(tally_eb);
+  ABX_REBASE(tally_gbin);
+#undef ABX_REBASE
+  Q.blob = base;
+  return Q;
+}

@@ -469,14 +469,26 @@ static int64_t resident_blocks(Kern kernel, int bs, size_t lds) {
 // `next` is a device counter (zeroed here) from which lanes take histories.
 template <int TRK, bool TRACE, bool SEC, bool SPLIT>
 static hipError_t history_transport_t(const DevProblem& P, const KCtx& K, hipStream_t s,
-                                      unsigned long long* next) {
-  const int bs = 256;
-  const size_t lds = (size_t)bs * trk_bytes(P.trk_D, P.trk_L);
+                                      unsigned long long* next, const abx_tuning& tu) {
+  const int bs = tu.block_size;
+  // Stage the tables in LDS when they fit next to the cursors (64 KB per block).
+  const size_t cur = cursor_lds_bytes(bs, P.trk_D, P.trk_L);
+  KCtx Ks = K;
+  Ks.stage_tables = tu.stage_tables && cur + (size_t)P.blob_bytes <= 65536 ? 1 : 0;
+  const size_t lds = Ks.stage_tables ? cur + (size_t)P.blob_bytes : cur;
   hipError_t e;
   if ((e = hipMemsetAsync(next, 0, sizeof(unsigned long long), s)) != hipSuccess) return e;
   const int64_t want = (K.n + bs - 1) / bs;
-  const int64_t blocks = std::min<int64_t>(want, resident_blocks(k_history<TRK, TRACE, SEC, SPLIT>, bs, lds));
-  hipLaunchKernelGGL((k_history<TRK, TRACE, SEC, SPLIT>), dim3((unsigned)blocks), dim3(bs), lds, s, P, K,
+  int64_t grid = resident_blocks(k_history<TRK, TRACE, SEC, SPLIT>, bs, lds);
+  if (tu.blocks_per_cu > 0) {
+    int dev = 0, cus = 0;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
+      return e;
+    grid = std::min<int64_t>(grid, (int64_t)cus * tu.blocks_per_cu);
+  }
+  const int64_t blocks = std::min<int64_t>(want, grid);
+  hipLaunchKernelGGL((k_history<TRK, TRACE, SEC, SPLIT>), dim3((unsigned)blocks), dim3(bs), lds, s, P, Ks,
                      next);
   return hipGetLastError();
 }
@@ -530,12 +542,13 @@ static inline hipError_t event_transport(const DevProblem& P, const KCtx& K, Eve
 }
 
 static inline hipError_t history_transport(const DevProblem& P, const KCtx& K, hipStream_t s,
-                                           bool sec, unsigned long long* next) {
+                                           bool sec, unsigned long long* next,
+                                           const abx_tuning& tu) {
   return dispatch_variant(P.tracking, K.trace != nullptr, sec, P.precision == ABX_SPLIT,
                           [&](auto T, auto TR, auto SE, auto SP) {
                             return history_transport_t<decltype(T)::value, decltype(TR)::value,
                                                        decltype(SE)::value, decltype(SP)::value>(
-                                P, K, s, next);
+                                P, K, s, next, tu);
                           });
 }
 

@@ -54,7 +54,7 @@ ABX_HD V3 surf_norm(const DSurf& s, const V3& r) {
 }
 
 // A point on the surface (|eval| <= 1e-12) takes the sign of u . n.
-ABX_COLD int surf_sign_on(const DSurf& s, const V3& r, const V3& u) {
+ABX_COLD int surf_sign_on(const DSurf& s, V3 r, V3 u) {
   return vdot(u, surf_norm(s, r)) > 0. ? 1 : -1;
 }
 
@@ -418,30 +418,38 @@ ABX_NOINL int trk_descend(const DevProblem& P, Trk& t, int ui, int lvl, V3 r, co
   return -1;
 }
 
-// Tracker::restart_get_current from the global position lv[0].  Out of
-// line: it runs at birth, after reflections and resurrections, and when the
-// root pad itself went stale.
-ABX_COLD void trk_restart(const DevProblem& P, Trk& t, const V3& u) {
-  t.np = 0;
-  t.cell = trk_descend(P, t, P.root, 0, t.lv(0), u);
+// Descend from pad `first_bad` (0 = from the root universe), the lily pads
+// below it being stale: Tracker::restart_get_current (first_bad == 0) and
+// the partial re-descent of Tracker::get_current (tracker.hpp:235-306),
+// which falls back to a restart when the partial descent finds no cell.
+// One descent site per call: the fallback loops instead of repeating it.
+ABX_HD void trk_locate(const DevProblem& P, Trk& t, const V3& u, int first_bad) {
+  for (;;) {
+    int ui, lvl;
+    if (first_bad > 0) {
+      const int32_t back = t.pad(first_bad - 1);
+      lvl = pad_lvl(back);
+      ui = pad_type(back) == PAD_C ? P.cell[pad_idx(back)].redescend : pad_idx(back);
+      t.np = first_bad - 1;
+    } else {
+      ui = P.root;
+      lvl = 0;
+      t.np = 0;
+    }
+    t.cell = trk_descend(P, t, ui, lvl, t.lv(lvl), u);
+    if (t.cell >= 0 || first_bad == 0) return;
+    first_bad = 0;
+  }
 }
+
+// Tracker::restart_get_current from the global position lv[0].
+ABX_HD void trk_restart(const DevProblem& P, Trk& t, const V3& u) { trk_locate(P, t, u, 0); }
 
 ABX_HD void trk_init(const DevProblem& P, Trk& t, const V3& r, const V3& u, int32_t token) {
   t.err = 0;
   t.token = token;
   t.set_lv(0, r);
   trk_restart(P, t, u);
-}
-
-// Re-descend after pad `first_bad` (>= 1) went stale (tracker.hpp:276-305).
-ABX_NOINL void trk_redescend(const DevProblem& P, Trk& t, const V3& u, int first_bad) {
-  t.np = first_bad;
-  const int32_t back = t.pad(t.np - 1);
-  const int lvl = pad_lvl(back);
-  const int ui = pad_type(back) == PAD_C ? P.cell[pad_idx(back)].redescend : pad_idx(back);
-  t.np--;
-  t.cell = trk_descend(P, t, ui, lvl, t.lv(lvl), u);
-  if (t.cell < 0) trk_restart(P, t, u);
 }
 
 // Tracker::get_current after a move (tree consistent with lv[0]).
@@ -465,27 +473,27 @@ ABX_HD void trk_get_current(const DevProblem& P, Trk& t, const V3& u) {
     }
   }
   if (first_bad == t.np) return;
-  if (first_bad == 0) {
-    trk_restart(P, t, u);
-    return;
-  }
-  trk_redescend(P, t, u, first_bad);
+  trk_locate(P, t, u, first_bad);
 }
 
 // Tracker::set_r(r) followed by get_current(): the stale tree forces a
 // restart, after which the reference still compares against the old tree
-// size (tracker.hpp:237-278).
-ABX_COLD void trk_reset_position(const DevProblem& P, Trk& t, const V3& r, const V3& u) {
+// size (tracker.hpp:237-278) and re-descends below it if the new tree is
+// deeper.  Both descents go through the one trk_locate site of the loop.
+ABX_HD void trk_reset_position(const DevProblem& P, Trk& t, const V3& r, const V3& u) {
   const int old_np = t.np;
   t.token = 0;
   t.set_lv(0, r);
-  trk_restart(P, t, u);
-  if (old_np == t.np) return;
-  if (old_np > t.np) {
-    t.err = -3;  // reference would resize into default pads
-    return;
+  int from = 0;
+  for (int pass = 0; pass < 2; pass++) {
+    trk_locate(P, t, u, from);
+    if (pass == 1 || old_np == t.np) return;
+    if (old_np > t.np) {
+      t.err = -3;  // reference would resize into default pads
+      return;
+    }
+    from = old_np;
   }
-  trk_redescend(P, t, u, old_np);
 }
 
 ABX_HD void trk_move(Trk& t, const V3& u, double d) {
@@ -570,9 +578,7 @@ ABX_NOINL Bound trk_bc(const DevProblem& P, const Trk& t, const V3& u) {
 
 // The same out of line, for delta / carter tracking, which only needs it
 // when a flight left the geometry.
-ABX_COLD Bound trk_bc_cold(const DevProblem& P, const Trk& t, const V3& u) {
-  return trk_bc(P, t, u);
-}
+ABX_HD Bound trk_bc_cold(const DevProblem& P, Trk t, V3 u) { return trk_bc(P, t, u); }
 
 // Tracker::get_nearest_boundary
 ABX_NOINL Bound trk_nearest(const DevProblem& P, const Trk& t, const V3& u) {

@@ -33,6 +33,7 @@ struct KCtx {
   abx_trace_rec* trace;
   int32_t trace_cap;
   int32_t* n_rec;
+  int32_t stage_tables;   // persistent kernels: copy the table blob into LDS
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -195,8 +196,26 @@ __device__ __forceinline__ void bind_lds_cursor(Trk& t, const DevProblem& P) {
 // history of their wavefront.  Per-history results do not depend on which
 // lane runs a history: the RNG stream is the history's own and fission
 // sites are ordered afterwards by (parent, daughter).
+// LDS bytes of the cursors of a block (the staged tables follow them).
+__host__ __device__ inline size_t cursor_lds_bytes(int bs, int D, int L) {
+  return ((size_t)bs * trk_bytes(D, L) + 15) & ~(size_t)15;
+}
+
+// Copies the problem's table blob into LDS after the cursors (whole block)
+// and returns the problem reading its tables from there.
+__device__ __forceinline__ DevProblem stage_tables(const DevProblem& P) {
+  unsigned char* dst = abx_lds + cursor_lds_bytes(blockDim.x, P.trk_D, P.trk_L);
+  const int64_t n16 = P.blob_bytes >> 4;
+  const uint4* src = (const uint4*)P.blob;
+  uint4* d4 = (uint4*)dst;
+  for (int64_t i = threadIdx.x; i < n16; i += blockDim.x) d4[i] = src[i];
+  __syncthreads();
+  return rebase_problem(P, dst);
+}
+
 template <int TRK, bool TRACE, bool SEC, bool SPLIT>
-__global__ void ABX_TKERNEL k_history(DevProblem P, KCtx K, unsigned long long* next) {
+__global__ void ABX_TKERNEL k_history(DevProblem P0, KCtx K, unsigned long long* next) {
+  const DevProblem P = K.stage_tables ? stage_tables(P0) : P0;
   const int lane = threadIdx.x & 63;
   Scores6 sc{0, 0, 0, 0, 0, 0};
   GpuSink<TRACE, SEC, SPLIT> s;

@@ -467,7 +467,7 @@ ABX_HD void leak(Part& p, const Bound& b, Scores6& sc) {
 
 // Tracker::do_reflection; the particle's position before the reflection
 // only feeds Particle::previous_r, which this path does not keep.
-ABX_COLD void reflect(const DevProblem& P, Trk& t, Part& p, const Bound& b) {
+ABX_HD void reflect(const DevProblem& P, Trk& t, Part& p, const Bound& b) {
   if (b.sidx < 0) {
     t.err = -3;
     return;
@@ -484,7 +484,7 @@ ABX_COLD void reflect(const DevProblem& P, Trk& t, Part& p, const Bound& b) {
 }
 
 template <class Sink>
-ABX_COLD bool try_resurrect(const DevProblem& P, Sink& s, Part& p, Trk& t, int& m) {
+ABX_HD bool try_resurrect(const DevProblem& P, Sink& s, Part& p, Trk& t, int& m) {
   SecRec q;
   if (!s.sec_pop(q)) return false;
   p.alive = 1;
@@ -511,7 +511,7 @@ ABX_COLD bool try_resurrect(const DevProblem& P, Sink& s, Part& p, Trk& t, int& 
 
 // Carter-tracking weight split (src/carter_tracker.cpp:219-223)
 template <class Sink>
-ABX_COLD void split_weight(Sink& s, Part& p) {
+ABX_HD void split_weight(Sink& s, Part& p) {
   const int n_new = (int)ceil(dabs(p.w));
   if (n_new > 1) {
     p.w = p.w / (double)n_new;

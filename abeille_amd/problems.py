@@ -236,11 +236,15 @@ def infinite_medium_2g(nparticles=20000, tracking="delta", mode=abi.FIXED_SOURCE
     return m
 
 
-def slab_problem(rng, G=None, tracking=None, nhist=None, fission_free=None):
+def slab_problem(rng, G=None, tracking=None, nhist=None, fission_free=None,
+                 nbatch_hist=None):
     """One batched-form problem: a box of 1-8 x-stacked slab regions with a
     vacuum outer boundary, G in {2,4,7,16,33}, per-group XS from the lattice
-    distributions (fission-free subset), a point source in the first slab,
-    and a 16^3 collision flux tally.  Returns (model, histories)."""
+    distributions, a point source in the first slab, and a 16^3 collision
+    flux tally.  Fission-free by default; `fission_free=False` gives every
+    region a subcritical fuel (nu Sigma_f = 0.3 Sigma_a, chi into the first
+    group), so a fixed-source run carries fission secondaries.
+    Returns (model, histories)."""
     if G is None:
         G = int(rng.choice([2, 4, 7, 16, 33]))
     nreg = int(rng.integers(1, 9))
@@ -252,9 +256,18 @@ def slab_problem(rng, G=None, tracking=None, nhist=None, fission_free=None):
         fission_free = True
     m = Model()
     width = rng.uniform(2.0, 10.0, nreg)
+    hy = float(rng.uniform(5.0, 20.0))
+    _slab_geometry(m, rng, G, width, hy, fission_free)
+    m.set(mode=abi.FIXED_SOURCE, nparticles=nbatch_hist or nhist,
+          energy_bounds=np.arange(G + 1.0), tracking=_tracking(tracking))
+    return m, nhist
+
+
+def _slab_geometry(m, rng, G, width, hy, fission_free, nu_fission_ratio=0.3,
+                   c_range=(0.5, 0.99)):
+    nreg = len(width)
     x = np.concatenate([[0.0], np.cumsum(width)])
     x -= x[-1] / 2.0
-    hy = float(rng.uniform(5.0, 20.0))
     sx = [m.surface("xplane", [float(v)], "vacuum" if i in (0, nreg) else "normal")
           for i, v in enumerate(x)]
     sy = [m.surface("yplane", [-hy], "vacuum"), m.surface("yplane", [hy], "vacuum")]
@@ -262,7 +275,7 @@ def slab_problem(rng, G=None, tracking=None, nhist=None, fission_free=None):
     cells = []
     for r in range(nreg):
         st = rng.uniform(0.2, 2.0, G)
-        c = rng.uniform(0.5, 0.99, G)
+        c = rng.uniform(c_range[0], c_range[1], G)
         sa = st * (1 - c)
         scat = np.zeros((G, G))
         for g in range(G):
@@ -273,16 +286,51 @@ def slab_problem(rng, G=None, tracking=None, nhist=None, fission_free=None):
             w /= w.sum()
             scat[g] = (st[g] - sa[g]) * w
         st = sa + scat.sum(axis=1)
-        m.material(r + 1, total=st, absorption=sa, scatter=scat)
+        kw = {}
+        if not fission_free:
+            chi = np.zeros(G)
+            chi[0] = 1.0
+            kw = dict(fission=nu_fission_ratio * sa / 2.5, nu=np.full(G, 2.5), chi=chi)
+        m.material(r + 1, total=st, absorption=sa, scatter=scat, **kw)
         cells.append(m.cell(f"+{sx[r]} & -{sx[r + 1]} & +{sy[0]} & -{sy[1]} & "
                             f"+{sz[0]} & -{sz[1]}", material=r + 1, id=r + 1))
     m.root = m.cell_universe(cells, id=1)
     m.point_source([float(0.5 * (x[0] + x[1])), 0., 0.], energy=0.5)
     m.mesh_tally([float(x[0]), -hy, -hy], [float(x[-1]), hy, hy], [16, 16, 16],
                  np.arange(G + 1.0), estimator="collision")
-    m.set(mode=abi.FIXED_SOURCE, nparticles=nhist, energy_bounds=np.arange(G + 1.0),
-          tracking=_tracking(tracking))
-    return m, nhist
+    return x
+
+
+def served_request(rng, nbatches=2):
+    """One request of the served stream: a slab problem from the batched-form
+    generator with 1e4 - 5e6 histories (log-uniform) split over `nbatches`
+    batches.  One request in five is a modified-fixed-source chain in a thin
+    fissile box (1-3 slabs of 1-3 cm, |y|, |z| < 2-6 cm) with carter
+    tracking on a sampling cross section of 0.8 x the majorant and
+    approximate mesh cancellation on 10 x 10 x 10; the rest are fission-free
+    fixed-source problems with delta or surface tracking.  The fissile boxes
+    have nu Sigma_f = 0.05 Sigma_a and scattering ratios in [0.3, 0.6]: in
+    the negative-weight branch a flight multiplies |w| by 1.5 and a real
+    collision by the scattering ratio, so with ratios near one weights (and,
+    through splitting, particle counts) grow without bound, and the chain's
+    particle count multiplies by roughly the |w|-weighted k per generation.
+    Returns (model, total histories, kind)."""
+    nhist = int(np.exp(rng.uniform(np.log(1e4), np.log(5e6))))
+    per_batch = max(1, nhist // nbatches)
+    if rng.random() < 0.2:
+        G = int(rng.choice([2, 4, 7, 16, 33]))
+        m = Model()
+        width = rng.uniform(1.0, 3.0, int(rng.integers(1, 4)))
+        hy = float(rng.uniform(2.0, 6.0))
+        x = _slab_geometry(m, rng, G, width, hy, fission_free=False, nu_fission_ratio=0.05,
+                           c_range=(0.3, 0.6))
+        m.set(mode=abi.MODIFIED_FIXED_SOURCE, nparticles=per_batch,
+              energy_bounds=np.arange(G + 1.0), tracking=abi.CARTER_TRACKING,
+              sample_xs_ratio=[0.8] * G)
+        m.cancellation_mesh([float(x[0]), -hy, -hy], [float(x[-1]), hy, hy], [10, 10, 10])
+        return m, per_batch * nbatches, "carter"
+    m, _ = slab_problem(rng, nhist=nhist, nbatch_hist=per_batch)
+    return m, per_batch * nbatches, "fixed"
 
 
 def subcritical_slab(nparticles=20000, tracking="carter", cancel=True, half=1.0):

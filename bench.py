@@ -64,12 +64,12 @@ def cpu_baseline(name, tracking, seconds, nthreads):
     """Time the CPU restatement (oracle/) on a bounded sample: generations of
     a scaled-down bank of the same problem until about `seconds` of work."""
     from oracle.pyoracle import Oracle
-    n = 20000
+    n = 100000
     m, _ = workload(name, n, tracking, "fp64")
     o = Oracle(m, portable_math=False)
     o.run(2, 2, nthreads)  # warm up (threads, source)
     gens, coll, hist, t_used = 0, 0, 0, 0.0
-    while t_used < seconds and gens < 200:
+    while t_used < seconds and gens < 400:
         st, _ = o.run(2, 1, nthreads)
         t_used += st.transport_seconds
         coll += st.diag.collisions
@@ -81,12 +81,148 @@ def cpu_baseline(name, tracking, seconds, nthreads):
             "histories_per_s": hist / t_used}
 
 
+def _common(args, metric_value, unit, ms_per_step, dtype, config, extra):
+    out = {"metric": METRIC, "value": metric_value, "unit": unit, "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+           "data": "synthetic (seeded cross sections and geometry, no external data)",
+           "config": config}
+    out.update(extra)
+    return out
+
+
+def _c4_problems(count, precision, seed=512):
+    """The batched-form problem set: `count` seeded slab problems, histories
+    log-uniform in [1e3, 1e6] run as 2 batches each."""
+    from abeille_amd import abi, problems
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        nhist = int(np.exp(rng.uniform(np.log(1e3), np.log(1e6))))
+        m, _ = problems.slab_problem(rng, nhist=nhist, nbatch_hist=max(1, nhist // 2))
+        m.set(precision=precision)
+        out.append(m)
+    return out
+
+
+def bench_batch(args):
+    """512 independent fixed-source slab problems: the batch engine (split
+    precision, shared persistent launches) against the same problems run one
+    at a time through the FP64 device loop."""
+    from abeille_amd import Batch, Problem, abi
+    nb = 2
+    split = [Problem(m) for m in _c4_problems(args.problems, abi.SPLIT)]
+    # warm-up: a few problems through the batch (module load, grid sizing)
+    w = Batch()
+    for p in split[:8]:
+        w.add(p, 1)
+    while len(w):
+        w.step()
+    b = Batch()
+    t0 = time.perf_counter()
+    for p in split:
+        b.add(p, nb)
+    steps = 0
+    while len(b):
+        b.step()
+        steps += 1
+    wall = time.perf_counter() - t0
+    coll = sum(p.stats().diag.collisions for p in split)
+    hist = sum(p.stats().histories for p in split)
+    kernel_s = b.kernel_seconds()
+    del split
+    # plain path: FP64, one problem at a time
+    plain = [Problem(m) for m in _c4_problems(args.problems, abi.FP64)]
+    plain[0].run(1, 0)
+    t0 = time.perf_counter()
+    pc = 0
+    for p in plain:
+        st, _ = p.run(nb, 0)
+        pc += st.diag.collisions
+    pwall = time.perf_counter() - t0
+    out = _common(args, coll / wall, "collisions/s", 1e3 * wall / max(1, steps), "f64/f32 split",
+                  {"workload": f"{args.problems} slab fixed-source problems (1-8 regions, G in "
+                               "{2,4,7,16,33}, 1e3-1e6 histories, DT/ST, 16^3 collision "
+                               "tally), 2 batches each, batched split precision",
+                   "problems": args.problems},
+                  {"histories_per_s": hist / wall, "wall_s": wall, "batch_steps": steps,
+                   "transport_kernel_s": kernel_s,
+                   "plain_fp64_one_at_a_time": {"collisions_per_s": pc / pwall, "wall_s": pwall},
+                   "speedup_vs_plain": (coll / wall) / (pc / pwall)})
+    print(json.dumps(out), flush=True)
+
+
+def bench_serve(args):
+    """Served stream: Poisson arrivals of slab requests (20 % carter tracking
+    with a 0.8 x majorant sampling cross section and approximate mesh
+    cancellation), continuous batching under an HBM budget, split precision;
+    against the same requests served one at a time, FP64, first come first
+    served."""
+    from abeille_amd import Problem, abi, problems
+    from abeille_amd.serve import Request, Server, latency_percentiles, serial_latencies
+    rng = np.random.default_rng(2025)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, args.requests))
+    reqs, plain_models, kinds = [], [], []
+    for i in range(args.requests):
+        state = rng.bit_generator.state
+        m, nh, kind = problems.served_request(rng)
+        rng2 = np.random.default_rng()
+        rng2.bit_generator.state = state
+        mp, _, _ = problems.served_request(rng2)
+        m.set(precision=abi.SPLIT)
+        reqs.append(Request(m, 2, float(arrivals[i]), name=f"r{i}"))
+        plain_models.append(mp)
+        kinds.append(kind)
+    # warm-up
+    Server(keep_tallies=False).run([Request(reqs[0].model, 1, 0.0)])
+    srv = Server(budget=64 << 30, keep_tallies=False)
+    t0 = time.perf_counter()
+    srv.run(reqs)
+    wall = time.perf_counter() - t0
+    coll = sum(r.stats.diag.collisions for r in reqs)
+    hist = sum(r.stats.histories for r in reqs)
+    pct = latency_percentiles(reqs)
+    service = np.zeros(len(reqs))
+    pc = 0
+    for i, m in enumerate(plain_models):
+        p = Problem(m)
+        t = time.perf_counter()
+        st, _ = p.run(2, 0)
+        service[i] = time.perf_counter() - t
+        pc += st.diag.collisions
+        del p
+    lat = serial_latencies(arrivals, service)
+    span = float(arrivals[-1])
+    t = float(np.max(arrivals + lat))  # makespan of the one-at-a-time service
+    out = _common(args, coll / wall, "collisions/s", 1e3 * wall / max(1, srv.steps),
+                  "f64/f32 split",
+                  {"workload": f"{args.requests} slab requests, Poisson {args.rate}/s, 1e4-5e6 "
+                               "histories, 20% carter (0.8 x majorant) + 10^3 approximate "
+                               "cancellation, continuous batching, 64 GiB budget, split precision",
+                   "requests": args.requests},
+                  {"histories_per_s": hist / wall, "wall_s": wall, "arrival_span_s": span,
+                   "latency_s": pct, "steps": srv.steps,
+                   "carter_requests": int(sum(k == "carter" for k in kinds)),
+                   "plain_fp64_one_at_a_time": {
+                       "collisions_per_s": pc / t,
+                       "saturated_collisions_per_s": pc / float(service.sum()),
+                       "makespan_s": t,
+                       "latency_s": {"p50": float(np.percentile(lat, 50)),
+                                     "p99": float(np.percentile(lat, 99))}},
+                   "speedup_vs_plain": (coll / wall) / (pc / t)})
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="lattice", choices=["lattice", "core", "cube"])
+    ap.add_argument("--workload", default="lattice",
+                    choices=["lattice", "core", "cube", "batch", "serve"])
+    ap.add_argument("--problems", type=int, default=512, help="batch: number of problems")
+    ap.add_argument("--requests", type=int, default=100, help="serve: number of requests")
+    ap.add_argument("--rate", type=float, default=50.0, help="serve: arrivals per second")
     ap.add_argument("--particles", type=float, default=1e6)
     ap.add_argument("--tracking", default="delta", choices=["delta", "surface"])
     ap.add_argument("--engine", default="history", choices=["event", "history"])
@@ -113,6 +249,10 @@ def main():
                                    "reduction), --gpus must be 1"}))
         sys.exit(2)
 
+    if args.workload == "batch":
+        return bench_batch(args)
+    if args.workload == "serve":
+        return bench_serve(args)
     from abeille_amd import Problem, abi
     nparticles = int(args.particles)
     m, desc = workload(args.workload, nparticles, args.tracking, args.precision)

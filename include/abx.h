@@ -292,6 +292,19 @@ enum abx_engine {
                              lanes refilled from a queue (the default) */
 };
 
+/* Launch shape of the transport kernels; what the autotuner
+ * (abeille_amd/autotune.py) searches. */
+typedef struct abx_tuning {
+  int32_t block_size;     /* threads per block of the persistent transport
+                             kernels: 64, 128 or 256 (default 256) */
+  int32_t blocks_per_cu;  /* persistent grid size in resident blocks per CU
+                             (0 = as many as fit, the default) */
+  int32_t stage_tables;   /* 1 = copy the problem tables into LDS when they
+                             fit (default), 0 = read them through L2 */
+  int32_t tail_threshold; /* event engine: particles below which the rest of
+                             their histories run in-lane (0 = automatic) */
+} abx_tuning;
+
 const char* abx_last_error(void);
 const char* abx_version(void);
 
@@ -302,6 +315,8 @@ const char* abx_version(void);
 int abx_create(const abx_problem_desc* desc, abx_problem** out);
 void abx_destroy(abx_problem* p);
 int abx_set_engine(abx_problem* p, int engine);
+int abx_set_tuning(abx_problem* p, const abx_tuning* t);
+int abx_get_tuning(abx_problem* p, abx_tuning* t);
 
 /* Simulation::sample_sources (src/simulation.cpp:55-77): n source particles
  * for histories first_history .. first_history+n-1, sampled on the GPU. */
@@ -371,6 +386,12 @@ int abx_tally_stats(abx_problem* p, int tally, double* avg, double* std_err);
  * (src/simulation.cpp:137-175). */
 int abx_bank_size(abx_problem* p, size_t* n);
 int abx_bank_fetch(abx_problem* p, double* out9);
+/* PowerIterator::load_source_from_file (src/power_iterator.cpp:60-133): a
+ * [n][9] source becomes the first generation's bank - history ids 0..n-1,
+ * streams seeded from them, directions normalised - and nparticles (the
+ * tally normalisation) becomes the rounded total weight.  Statistics
+ * restart; the next abx_run_continue transports this bank. */
+int abx_bank_load(abx_problem* p, const double* in9, size_t n);
 
 /* ---- per-history traces (parity) ----------------------------------------- */
 /* One record per loop iteration of a history (after the flight / collision

@@ -35,7 +35,8 @@ STRUCTS = {"abx_surface": abi.Surface, "abx_cell": abi.Cell, "abx_universe": abi
            "abx_mesh_tally": abi.MeshTally, "abx_settings": abi.Settings,
            "abx_problem_desc": abi.ProblemDesc, "abx_particle": abi.Particle,
            "abx_banked": abi.Banked, "abx_scores": abi.Scores, "abx_diag": abi.Diag,
-           "abx_run_stats": abi.RunStats, "abx_trace_rec": abi.TraceRec}
+           "abx_run_stats": abi.RunStats, "abx_trace_rec": abi.TraceRec,
+           "abx_cancel_mesh": abi.CancelMesh, "abx_tuning": abi.Tuning}
 
 
 def test_record_layouts_match_c(tmp_path):
