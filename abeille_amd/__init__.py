@@ -40,6 +40,7 @@ def lib():
         L.abx_set_engine.argtypes = [P, C.c_int]
         L.abx_set_tuning.argtypes = [P, C.POINTER(abi.Tuning)]
         L.abx_get_tuning.argtypes = [P, C.POINTER(abi.Tuning)]
+        L.abx_time_transport.argtypes = [P, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
         L.abx_sample_sources.argtypes = [P, C.c_uint64, C.c_size_t, C.c_void_p]
         L.abx_transport.argtypes = [P, C.c_void_p, C.c_size_t, C.c_double, C.c_int, C.c_void_p,
                                     C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(abi.Scores),
@@ -188,6 +189,12 @@ class Problem:
         _check(lib().abx_get_tuning(self.h, C.byref(t)))
         return dict(block_size=t.block_size, blocks_per_cu=t.blocks_per_cu,
                     stage_tables=t.stage_tables, tail_threshold=t.tail_threshold)
+
+    def time_transport(self, n, converged=True):
+        """Device seconds of one transport of n fresh histories (abx_time_transport)."""
+        x = C.c_double()
+        _check(lib().abx_time_transport(self.h, int(n), 1 if converged else 0, C.byref(x)))
+        return x.value
 
     def footprint(self):
         """Device bytes the problem holds while it runs (abx_problem_footprint)."""

@@ -1,0 +1,62 @@
+"""Project autotuner for the transport launch shape (abx_tuning): block
+size, persistent-grid size and LDS staging of the tables, per problem
+class.
+
+A candidate is timed with abx_time_transport on a sample of fresh source
+histories (the same kernels and tables as the run, none of its state); the
+fastest is applied and remembered in a JSON cache keyed by the problem's
+class (tracking, mode, precision, groups, geometry and tally sizes), so a
+class is searched once.
+"""
+import hashlib
+import json
+import os
+
+DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "abeille_amd", "tuning.json")
+
+CANDIDATES = [dict(block_size=bs, blocks_per_cu=bpc, stage_tables=st, tail_threshold=0)
+              for bs in (256, 128, 64) for bpc in (0, 4, 2) for st in (1, 0)]
+
+
+def problem_class(model):
+    s = model.settings
+    sig = dict(tracking=int(s["tracking"]), mode=int(s["mode"]), precision=int(s["precision"]),
+               G=model.ngroups, surfaces=len(model.surfaces), cells=len(model.cells),
+               universes=len(model.universes), materials=len(model.materials),
+               tallies=[(t["estimator"], list(t["shape"])) for t in model.tallies])
+    return hashlib.sha1(json.dumps(sig, sort_keys=True).encode()).hexdigest()[:16]
+
+
+def _load(path):
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def autotune(problem, model, sample=200_000, repeats=2, cache=DEFAULT_CACHE,
+             candidates=CANDIDATES, force=False):
+    """Pick and apply the fastest tuning for `problem`; returns (tuning,
+    {candidate index: seconds})."""
+    key = problem_class(model)
+    db = _load(cache) if cache else {}
+    if key in db and not force:
+        problem.set_tuning(**db[key]["tuning"])
+        return db[key]["tuning"], {}
+    n = max(1, min(int(sample), int(model.settings["nparticles"])))
+    times = {}
+    for i, c in enumerate(candidates):
+        problem.set_tuning(**c)
+        problem.time_transport(min(n, 4096))          # warm the variant
+        times[i] = min(problem.time_transport(n) for _ in range(repeats))
+    best = min(times, key=times.get)
+    problem.set_tuning(**candidates[best])
+    if cache:
+        db[key] = {"tuning": candidates[best], "seconds": times[best], "sample": n}
+        os.makedirs(os.path.dirname(cache), exist_ok=True)
+        tmp = cache + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(db, f, indent=1, sort_keys=True)
+        os.replace(tmp, cache)
+    return candidates[best], times

@@ -78,8 +78,9 @@ struct abx_problem {
   // transport scratch
   DBuf in, sec, stage, stage_parent, stage_daughter, n_stage, daughters, offsets,
       block_sums, block_sums2, ordered, scores, counters, err, err_hist, trace, n_rec, wsum,
-      pool_next, blob;
+      pool_next, blob, gdev, khist_dev;
   EventBuffers ev;
+  std::vector<hipEvent_t> gen_events;  // graph runs: 2 per generation
   // cancellation
   bool cancel = false;
   DevCancel cm{};
@@ -88,6 +89,7 @@ struct abx_problem {
   DBuf bank_a, bank_b;  // abx_particle banks
   size_t bank_n = 0;
   bool run_started = false;
+  bool use_graphs = true;   // k-eigenvalue runs replay captured generations
   uint64_t run_hist = 0;  // histories_counter
   double run_kcol = 1.0;  // Tallies::kcol() of the previous generation
   // per-kernel timing of the transport kernels (HIP events on p->stream)
@@ -119,7 +121,9 @@ struct abx_problem {
     DBuf* all[] = {&tally_gen, &tally_avg, &tally_var, &in, &sec, &stage, &stage_parent,
                    &stage_daughter, &n_stage, &daughters, &offsets, &block_sums, &block_sums2,
                    &ordered, &scores, &counters, &err, &err_hist, &trace, &n_rec, &wsum,
-                   &bank_a, &bank_b, &ckeys, &ckeys2, &ctemp, &pool_next, &blob};
+                   &bank_a, &bank_b, &ckeys, &ckeys2, &ctemp, &pool_next, &blob, &gdev,
+                   &khist_dev};
+    for (auto e : gen_events) (void)hipEventDestroy(e);
     for (DBuf* b : all) b->release();
     ev.release();
     if (ke0) (void)hipEventDestroy(ke0);
@@ -546,6 +550,35 @@ int abx_set_tuning(abx_problem* p, const abx_tuning* t) {
   });
 }
 
+int abx_time_transport(abx_problem* p, size_t n, int converged, double* seconds) {
+  ABX_TRY({
+    if (!p || !seconds || n == 0) fail(ABX_E_INVALID, "timing arguments");
+    // a scratch bank of fresh sources from history ids the runs do not use
+    p->in.reserve(n * sizeof(abx_particle));
+    sample_sources_device(p, (uint64_t)1 << 62, (int64_t)n, p->in.as<abx_particle>());
+    const abx_scores saved = p->gen_scores;
+    if (!p->ke0) {
+      HIPC(hipEventCreate(&p->ke0));
+      HIPC(hipEventCreate(&p->ke1));
+    }
+    const bool timing = p->timing;
+    p->timing = false;
+    HIPC(hipEventRecord(p->ke0, p->stream));
+    (void)transport_device(p, p->in.as<abx_particle>(), (int64_t)n, p->run_kcol, converged, nullptr,
+                           0, nullptr);
+    HIPC(hipEventRecord(p->ke1, p->stream));
+    HIPC(hipEventSynchronize(p->ke1));
+    float ms = 0.f;
+    HIPC(hipEventElapsedTime(&ms, p->ke0, p->ke1));
+    p->timing = timing;
+    *seconds = ms * 1e-3;
+    // leave the current generation's tallies and scores as they were
+    clear_generation(p);
+    p->gen_scores = saved;
+    HIPC(hipStreamSynchronize(p->stream));
+  });
+}
+
 int abx_get_tuning(abx_problem* p, abx_tuning* t) {
   ABX_TRY({
     if (!p || !t) fail(ABX_E_INVALID, "null argument");
@@ -690,6 +723,220 @@ static void fill_statistics(const abx_problem* p, abx_run_stats* stats) {
   stats->active_generations = p->gen;
 }
 
+// Grows a buffer keeping its first `keep` bytes.
+static void grow_keep(DBuf& b, size_t bytes, size_t keep, hipStream_t s) {
+  if (bytes <= b.bytes) return;
+  DBuf n;
+  n.reserve(bytes);
+  if (keep) HIPC(hipMemcpyAsync(n.p, b.p, keep, hipMemcpyDeviceToDevice, s));
+  HIPC(hipStreamSynchronize(s));
+  b.release();
+  b = n;
+}
+
+// PowerIterator::run with every generation a replay of captured graphs:
+// graph 1 = zero the generation's scalars + the persistent transport kernel
+// (bank and k read from GenDev), graph 2 = device scan of the daughter
+// counts, ordered fission bank, weight sums, statistics, re-bank into the
+// other buffer, tally statistics.  No host round trip inside a generation;
+// HIP events between the two graphs time the transport kernel.  Used for
+// the history engine without cancellation or traces; returns false
+// otherwise.
+static bool run_generations_graph(abx_problem* p, int ngenerations, int nignored,
+                                  double* kcol_hist, abx_run_stats* stats) {
+  const HostTables& T = p->T;
+  if (!p->use_graphs || T.st.mode != ABX_K_EIGENVALUE || p->engine != ABX_ENGINE_HISTORY ||
+      p->cancel || !p->run_started)
+    return false;
+  const int64_t N = T.st.nparticles;
+  const double W = (double)N;
+  const uint64_t cap = (uint64_t)(4 * N + 4096);
+  if (p->bank_n > cap) return false;
+  hipStream_t s = p->stream;
+  // buffers at capacity (the live bank is kept)
+  grow_keep(p->bank_a, cap * sizeof(abx_particle), p->bank_n * sizeof(abx_particle), s);
+  p->bank_b.reserve(cap * sizeof(abx_particle));
+  const int64_t nb = (int64_t)((cap + 1023) / 1024);
+  p->offsets.reserve(cap * sizeof(uint64_t));
+  p->block_sums.reserve((size_t)(nb + 1) * sizeof(uint64_t));
+  p->ordered.reserve(cap * sizeof(abx_banked));
+  p->gdev.reserve(sizeof(GenDev));
+  p->khist_dev.reserve((size_t)std::max(1, ngenerations) * sizeof(double));
+  p->pool_next.reserve(sizeof(unsigned long long));
+  GenDev* g = p->gdev.as<GenDev>();
+  GenDev h{};
+  h.n = p->bank_n;
+  h.run_hist = p->run_hist;
+  h.cap = cap;
+  h.kcol = p->run_kcol;
+  h.gen = p->gen;
+  h.kc_avg = p->kc_avg;
+  h.kc_var = p->kc_var;
+  h.kt_avg = p->kt_avg;
+  h.kt_var = p->kt_var;
+  h.lk_avg = p->lk_avg;
+  h.lk_var = p->lk_var;
+  h.mg_avg = p->mg_avg;
+  h.mg_var = p->mg_var;
+  HIPC(hipMemcpyAsync(g, &h, sizeof(GenDev), hipMemcpyHostToDevice, s));
+  HIPC(hipStreamSynchronize(s));
+  // transport context: counts and scalars live in GenDev
+  const CallScalars cs{g->scores, g->counters, &g->err, &g->err_hist, &g->n_stage};
+  KCtx K = make_kctx(p, p->bank_a.as<abx_particle>(), N, 1.0, 1, cs, nullptr, 0);
+  p->daughters.reserve(cap * sizeof(uint32_t));
+  K.daughters = p->daughters.as<uint32_t>();
+  K.stage_cap = (int64_t)cap;
+  K.n_dev = &g->n;
+  K.kcol_dev = &g->kcol;
+  const abx_tuning tu = p->tuning;
+  const int bs = tu.block_size;
+  const size_t cur = cursor_lds_bytes(bs, p->P.trk_D, p->P.trk_L);
+  K.stage_tables = tu.stage_tables && cur + (size_t)p->P.blob_bytes <= 65536 ? 1 : 0;
+  const size_t lds = K.stage_tables ? cur + (size_t)p->P.blob_bytes : cur;
+  const int trk = T.st.tracking;
+  const bool split = T.st.precision == ABX_SPLIT;
+  const bool sec = needs_secondaries(T);
+  if (sec) return false;  // k-eigenvalue without carter tracking only
+  int64_t grid = 0;
+  HIPC(dispatch_variant(trk, false, false, split, [&](auto TT, auto, auto, auto SP) {
+    grid = resident_blocks(k_history<decltype(TT)::value, false, false, decltype(SP)::value>, bs, lds);
+    return hipSuccess;
+  }));
+  if (tu.blocks_per_cu > 0) {
+    int dev = 0, cus = 0;
+    HIPC(hipGetDevice(&dev));
+    HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    grid = std::min<int64_t>(grid, (int64_t)cus * tu.blocks_per_cu);
+  }
+  grid = std::min<int64_t>(grid, (int64_t)((cap + bs - 1) / bs));
+  const int64_t tb = T.tally_bins;
+  // capture: [parity][converged] x {transport, bookkeeping}
+  hipGraphExec_t ex[2][2][2] = {};
+  hipGraph_t gr[2][2][2] = {};
+  auto capture = [&](int par, int conv) {
+    abx_particle* in = par ? p->bank_b.as<abx_particle>() : p->bank_a.as<abx_particle>();
+    abx_particle* out = par ? p->bank_a.as<abx_particle>() : p->bank_b.as<abx_particle>();
+    KCtx Kc = K;
+    Kc.in = in;
+    Kc.converged = conv;
+    HIPC(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(k_gen_begin, dim3(1), dim3(64), 0, s, g);
+    HIPC(dispatch_variant(trk, false, false, split, [&](auto TT, auto, auto, auto SP) {
+      hipLaunchKernelGGL((k_history<decltype(TT)::value, false, false, decltype(SP)::value>),
+                         dim3((unsigned)grid), dim3(bs), lds, s, p->P, Kc, &g->next);
+      return hipSuccess;
+    }));
+    HIPC(hipStreamEndCapture(s, &gr[par][conv][0]));
+    HIPC(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(k_scan_blocks_dev, dim3((unsigned)nb), dim3(1024), 0, s,
+                       p->daughters.as<uint32_t>(), p->offsets.as<uint64_t>(),
+                       p->block_sums.as<uint64_t>(), &g->n);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, p->block_sums.as<uint64_t>(), nb,
+                       &g->n_sites);
+    hipLaunchKernelGGL(k_add_block_offsets_dev, dim3((unsigned)nb), dim3(1024), 0, s,
+                       p->offsets.as<uint64_t>(), p->block_sums.as<uint64_t>(), &g->n);
+    const unsigned gcap = (unsigned)((cap + 255) / 256);
+    hipLaunchKernelGGL(k_order_bank_dev, dim3(gcap), dim3(256), 0, s, p->stage.as<abx_banked>(),
+                       p->stage_parent.as<uint32_t>(), p->stage_daughter.as<uint32_t>(),
+                       p->offsets.as<uint64_t>(), g, p->ordered.as<abx_banked>());
+    hipLaunchKernelGGL(k_weight_sums_dev, dim3(gcap), dim3(256), 0, s, p->ordered.as<abx_banked>(), g);
+    hipLaunchKernelGGL(k_gen_end, dim3(1), dim3(64), 0, s, g, W, conv, p->khist_dev.as<double>());
+    hipLaunchKernelGGL(k_rebank_dev, dim3(gcap), dim3(256), 0, s, p->ordered.as<abx_banked>(), g,
+                       T.st.seed, T.st.stride, out);
+    if (conv && tb > 0)
+      hipLaunchKernelGGL(k_record_generation_dev, dim3((unsigned)blocks_for(tb, 256)), dim3(256), 0, s,
+                         p->tally_gen.as<double>(), p->tally_avg.as<double>(),
+                         p->tally_var.as<double>(), tb, g, 0);
+    HIPC(hipStreamEndCapture(s, &gr[par][conv][1]));
+    for (int k = 0; k < 2; k++)
+      HIPC(hipGraphInstantiate(&ex[par][conv][k], gr[par][conv][k], nullptr, nullptr, 0));
+  };
+  auto cleanup = [&]() {
+    for (auto& a : ex)
+      for (auto& b : a)
+        for (auto& c : b)
+          if (c) (void)hipGraphExecDestroy(c);
+    for (auto& a : gr)
+      for (auto& b : a)
+        for (auto& c : b)
+          if (c) (void)hipGraphDestroy(c);
+  };
+  try {
+    while ((int)p->gen_events.size() < 2 * ngenerations + 2) {
+      hipEvent_t e;
+      HIPC(hipEventCreate(&e));
+      p->gen_events.push_back(e);
+    }
+    bool converged = nignored == 0;
+    int par = 0;
+    for (int gi = 0; gi < ngenerations; gi++) {
+      const int conv = converged ? 1 : 0;
+      if (!ex[par][conv][0]) capture(par, conv);
+      HIPC(hipEventRecord(p->gen_events[2 * gi], s));
+      HIPC(hipGraphLaunch(ex[par][conv][0], s));
+      HIPC(hipEventRecord(p->gen_events[2 * gi + 1], s));
+      HIPC(hipGraphLaunch(ex[par][conv][1], s));
+      par ^= 1;
+      if (gi + 1 == nignored) converged = true;
+    }
+    HIPC(hipEventRecord(p->gen_events[2 * ngenerations], s));
+    HIPC(hipMemcpyAsync(&h, g, sizeof(GenDev), hipMemcpyDeviceToHost, s));
+    if (kcol_hist)
+      HIPC(hipMemcpyAsync(kcol_hist, p->khist_dev.p, (size_t)ngenerations * sizeof(double),
+                          hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    // the latest bank is in bank_b after an odd number of generations
+    if (par) std::swap(p->bank_a, p->bank_b);
+    float total = 0.f;
+    double kms = 0.0;
+    for (int gi = 0; gi < ngenerations; gi++) {
+      float x = 0.f;
+      HIPC(hipEventElapsedTime(&x, p->gen_events[2 * gi], p->gen_events[2 * gi + 1]));
+      kms += x;
+    }
+    HIPC(hipEventElapsedTime(&total, p->gen_events[0], p->gen_events[2 * ngenerations]));
+    cleanup();
+    if (h.err) {
+      if (h.err == -8) fail(ABX_E_INVALID, "No fission neutrons were produced.");
+      fail(h.err, "transport failed at history " + std::to_string(h.err_hist));
+    }
+    p->gen = h.gen;
+    p->tally_g = h.gen;
+    p->kc_avg = h.kc_avg;
+    p->kc_var = h.kc_var;
+    p->kt_avg = h.kt_avg;
+    p->kt_var = h.kt_var;
+    p->lk_avg = h.lk_avg;
+    p->lk_var = h.lk_var;
+    p->mg_avg = h.mg_avg;
+    p->mg_var = h.mg_var;
+    p->bank_n = (size_t)h.n;
+    p->run_hist = h.run_hist;
+    p->run_kcol = h.kcol;
+    p->gen_scores = abx_scores{};
+    *stats = abx_run_stats{};
+    fill_statistics(p, stats);
+    abx_diag& d = stats->diag;
+    d.histories = h.diag[0];
+    d.collisions = h.diag[1];
+    d.virtual_collisions = h.diag[2];
+    d.flights = h.diag[3];
+    d.crossings = h.diag[4];
+    d.tally_updates = h.diag[5];
+    d.lost = h.diag[6];
+    d.secondaries = h.diag[7];
+    stats->transport_seconds = total * 1e-3;
+    stats->kernel_seconds = kms * 1e-3;
+    stats->kernel_launches = ngenerations;
+    stats->kernel_items = (int64_t)(d.flights + d.collisions);
+    stats->histories = (int64_t)d.histories;
+  } catch (...) {
+    cleanup();
+    throw;
+  }
+  return true;
+}
+
 // FixedSource::run / ModifiedFixedSource::run / PowerIterator::run
 // (src/fixed_source.cpp:81-173, src/modified_fixed_source.cpp:59-143,
 // src/power_iterator.cpp:305-429) with every bank resident in HBM.
@@ -720,6 +967,12 @@ static void run_generations(abx_problem* p, int ngenerations, int nignored, doub
     p->bank_n = (size_t)N;
   }
   p->run_started = true;
+  if (run_generations_graph(p, ngenerations, nignored, kcol_hist, stats)) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    p->timing = false;
+    return;
+  }
   for (int g = 1; g <= ngenerations; g++) {
     abx_diag dg{};
     abx_scores sc{};
@@ -773,7 +1026,9 @@ static void run_generations(abx_problem* p, int ngenerations, int nignored, doub
       if (r.n_sites == 0) fail(ABX_E_INVALID, "No fission neutrons were produced.");
       const double kc = sc.k_col / W;
       if (kcol_hist) kcol_hist[g - 1] = kc;
-      // normalize_weights: W = W_pos - W_neg, w *= nparticles / W
+      // perform_regional_cancellation, then normalize_weights: W = W_pos -
+      // W_neg, w *= nparticles / W (src/power_iterator.cpp:354-359)
+      if (p->cancel) cancel_device(p, p->ordered.as<abx_banked>(), r.n_sites);
       p->wsum.reserve(2 * sizeof(double));
       HIPC(hipMemsetAsync(p->wsum.p, 0, 2 * sizeof(double), p->stream));
       hipLaunchKernelGGL(k_weight_sums, dim3((unsigned)blocks_for(r.n_sites, 256)), dim3(256), 0,

@@ -148,16 +148,20 @@ struct DevProblem {
 };
 
 // The same problem with its tables read from `base`, a copy of the blob
-// (in LDS): every table pointer of dev_view() points into the blob.
+// (in LDS): every table pointer of dev_view() points into the blob.  The new
+// pointers are formed from `base` plus an integer offset, never by adding
+// to the old pointer: a pointer member of a kernel argument is assumed to
+// address global memory, and arithmetic on it would keep that assumption
+// (global loads of LDS addresses fault).
 template <class T>
-ABX_HD const T* rebase_ptr(const T* p, int64_t delta) {
-  return (const T*)((const unsigned char*)p + delta);
+ABX_HD const T* rebase_ptr(const T* p, const unsigned char* blob, const unsigned char* base) {
+  const uint64_t off = (uint64_t)((const unsigned char*)p - blob);
+  return (const T*)(base + off);
 }
 
 ABX_HD DevProblem rebase_problem(const DevProblem& P, const unsigned char* base) {
   DevProblem Q = P;
-  const int64_t d = (int64_t)(base - P.blob);
-#define ABX_REBASE(f) Q.f = rebase_ptr(P.f, d)
+#define ABX_REBASE(f) Q.f = rebase_ptr(P.f, P.blob, base)
   ABX_REBASE(surf);
   ABX_REBASE(cell);
   ABX_REBASE(rpn);

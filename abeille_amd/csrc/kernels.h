@@ -34,6 +34,10 @@ struct KCtx {
   int32_t trace_cap;
   int32_t* n_rec;
   int32_t stage_tables;   // persistent kernels: copy the table blob into LDS
+  // device-resident generation loop: bank size and k of the previous
+  // generation read from device memory (null = use n / kcol)
+  const unsigned long long* n_dev;
+  const double* kcol_dev;
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -216,6 +220,8 @@ __device__ __forceinline__ DevProblem stage_tables(const DevProblem& P) {
 template <int TRK, bool TRACE, bool SEC, bool SPLIT>
 __global__ void ABX_TKERNEL k_history(DevProblem P0, KCtx K, unsigned long long* next) {
   const DevProblem P = K.stage_tables ? stage_tables(P0) : P0;
+  const int64_t n_hist = K.n_dev ? (int64_t)*K.n_dev : K.n;
+  const double kcol = K.kcol_dev ? *K.kcol_dev : K.kcol;
   const int lane = threadIdx.x & 63;
   Scores6 sc{0, 0, 0, 0, 0, 0};
   GpuSink<TRACE, SEC, SPLIT> s;
@@ -239,7 +245,7 @@ __global__ void ABX_TKERNEL k_history(DevProblem P0, KCtx K, unsigned long long*
       base = __shfl(base, leader, 64);
       if (need) {
         const int64_t i = (int64_t)(base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull)));
-        if (i < K.n) {
+        if (i < n_hist) {
           s.slot = i;
           s.nsec = 0;
           s.n_rec = 0;
@@ -259,7 +265,7 @@ __global__ void ABX_TKERNEL k_history(DevProblem P0, KCtx K, unsigned long long*
     if (p.alive) {
       int ev = 0;
       bool hc = step_flight<TRK>(P, s, p, t, m, converged, sc, ev);
-      if (p.alive && hc) step_collision(P, s, p, m, K.kcol, converged, sc, ev);
+      if (p.alive && hc) step_collision(P, s, p, m, kcol, converged, sc, ev);
       bool live = step_finish<TRK>(P, s, p, t, m, ev);
       if (live && ++steps > 100000000) {
         s.error(-3, p.hid);
@@ -586,3 +592,204 @@ __global__ void k_cancel_apply(const uint64_t* keys, int64_t n, abx_banked* b) {
     if (p2 && n2) q.wgt2 = a2;
   }
 }
+
+// ------------------------------------------- device-resident generations
+// The k-eigenvalue generation loop (PowerIterator::run,
+// src/power_iterator.cpp:305-429) with every count and scalar kept on the
+// device, so that a generation is a fixed sequence of launches with no host
+// round trip and can be captured once as a graph and replayed.  Grids are
+// sized for the bank capacity; kernels read the live counts from GenDev.
+struct GenDev {
+  unsigned long long n;           // histories of the current generation
+  unsigned long long n_sites;     // fission sites it banked (after the scan)
+  unsigned long long n_stage;     // sites staged by the transport kernel
+  unsigned long long first_hist;  // first history id of the re-banked generation
+  unsigned long long run_hist;    // Simulation::histories_counter
+  unsigned long long next;        // persistent-kernel history counter
+  unsigned long long err_hist;
+  unsigned long long cap;         // bank / staging capacity
+  unsigned long long counters[8]; // this generation's event counters
+  unsigned long long diag[8];     // summed over the run
+  double scores[8];               // this generation's k_col k_abs k_trk k_tot leak mig
+  double wsum[2];                 // positive / negative banked weight
+  double kcol;                    // Tallies::kcol() for the next transport
+  double wpp;                     // nparticles / total weight of the re-bank
+  double kc_avg, kc_var, kt_avg, kt_var, lk_avg, lk_var, mg_avg, mg_var;
+  int32_t err;
+  int32_t gen;                    // recorded (active) generations
+  int32_t hist_idx;               // generations run (index into the k history)
+  int32_t pad_;
+};
+
+__global__ void k_gen_begin(GenDev* g) {
+  const int t = threadIdx.x;
+  if (t < 8) {
+    g->scores[t] = 0.0;
+    g->counters[t] = 0;
+  }
+  if (t < 2) g->wsum[t] = 0.0;
+  if (t == 0) {
+    g->n_stage = 0;
+    g->next = 0;
+  }
+}
+
+// Exclusive scan of daughters[0 .. *n) into offsets, first phase (1024 per
+// block; blocks past the live count write zero block sums).
+__global__ void k_scan_blocks_dev(const uint32_t* in, uint64_t* out, uint64_t* block_sums,
+                                  const unsigned long long* n_dev) {
+  __shared__ uint64_t wsum[16];
+  const int64_t n = (int64_t)*n_dev;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const uint64_t v = i < n ? in[i] : 0;
+  uint64_t x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint64_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[wid] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t acc = 0;
+    const int nw = blockDim.x >> 6;
+    for (int w = 0; w < nw; w++) {
+      const uint64_t t = wsum[w];
+      wsum[w] = acc;
+      acc += t;
+    }
+    block_sums[blockIdx.x] = acc;
+  }
+  __syncthreads();
+  if (i < n) out[i] = x - v + wsum[wid];
+}
+
+// Second phase: one block scans the block sums in place (exclusive) and
+// writes the total to *total.
+__global__ void k_scan_sums(uint64_t* block_sums, int64_t nb, unsigned long long* total) {
+  __shared__ uint64_t part[1024];
+  uint64_t carry = 0;
+  for (int64_t base = 0; base < nb; base += blockDim.x) {
+    const int64_t i = base + threadIdx.x;
+    const uint64_t v = i < nb ? block_sums[i] : 0;
+    part[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < (int)blockDim.x; o <<= 1) {
+      const uint64_t y = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0;
+      __syncthreads();
+      part[threadIdx.x] += y;
+      __syncthreads();
+    }
+    if (i < nb) block_sums[i] = carry + part[threadIdx.x] - v;
+    carry += part[blockDim.x - 1];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ void k_add_block_offsets_dev(uint64_t* out, const uint64_t* block_off,
+                                        const unsigned long long* n_dev) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < (int64_t)*n_dev) out[i] += block_off[blockIdx.x];
+}
+
+__global__ void k_order_bank_dev(const abx_banked* stage, const uint32_t* parent,
+                                 const uint32_t* daughter, const uint64_t* offsets,
+                                 const GenDev* g, abx_banked* out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t m = (int64_t)(g->n_stage < g->cap ? g->n_stage : g->cap);
+  if (k >= m || g->err) return;
+  const uint64_t dst = offsets[parent[k]] + daughter[k];
+  if (dst < g->cap) out[dst] = stage[k];  // beyond: k_gen_end reports the overflow
+}
+
+__global__ void k_weight_sums_dev(const abx_banked* b, GenDev* g) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = (int64_t)(g->n_sites < g->cap ? g->n_sites : g->cap);
+  double wp = 0., wn = 0.;
+  if (i < n) {
+    const double w = b[i].wgt;
+    if (w > 0.)
+      wp = w;
+    else
+      wn = -w;
+  }
+  wp = wave_sum(wp);
+  wn = wave_sum(wn);
+  if ((threadIdx.x & 63) == 0) {
+    if (wp != 0.) unsafeAtomicAdd(g->wsum, wp);
+    if (wn != 0.) unsafeAtomicAdd(g->wsum + 1, wn);
+  }
+}
+
+__device__ __forceinline__ void gen_upd(int gen, double x, double& avg, double& var) {
+  const double dgen = (double)gen;
+  const double avg_old = avg, var_old = var;
+  avg = avg_old + (x - avg_old) / dgen;
+  if (gen > 1) var = var_old + ((x - avg_old) * (x - avg_old) / dgen) - (var_old / (dgen - 1.));
+}
+
+// End of a generation (one thread): k statistics (Tallies::calc_gen_values
+// / record_generation, src/tallies.cpp:159-203), the re-bank's weight
+// factor (normalize_weights, src/power_iterator.cpp:538-586), history ids,
+// the next bank size.  A device error or an empty bank stops the run: the
+// next generations transport nothing.
+__global__ void k_gen_end(GenDev* g, double W, int converged, double* kcol_hist) {
+  if (threadIdx.x != 0) return;
+  if (!g->err && g->n_sites != g->n_stage) g->err = -1;
+  if (!g->err && g->n_sites > g->cap) g->err = -5;
+  if (!g->err && g->n_sites == 0) g->err = -8;
+  for (int k = 0; k < 8; k++) g->diag[k] += g->counters[k];
+  const double kc = g->scores[0] / W;
+  if (kcol_hist) kcol_hist[g->hist_idx] = kc;
+  g->hist_idx++;
+  if (converged && !g->err) {
+    g->gen++;
+    gen_upd(g->gen, kc, g->kc_avg, g->kc_var);
+    gen_upd(g->gen, g->scores[2] / W, g->kt_avg, g->kt_var);
+    gen_upd(g->gen, g->scores[4] / W, g->lk_avg, g->lk_var);
+    gen_upd(g->gen, g->scores[5] / W, g->mg_avg, g->mg_var);
+  }
+  g->wpp = W / (g->wsum[0] - g->wsum[1]);
+  g->first_hist = g->run_hist;
+  g->run_hist += g->n_sites;
+  g->kcol = kc;
+  g->n = g->err ? 0 : g->n_sites;
+}
+
+__global__ void k_rebank_dev(const abx_banked* b, const GenDev* g, uint64_t seed, uint64_t stride,
+                             abx_particle* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)g->n) return;
+  const abx_banked& s = b[i];
+  abx_particle p;
+  for (int k = 0; k < 3; k++) {
+    p.r[k] = s.r[k];
+    p.u[k] = s.u[k];
+  }
+  p.E = s.E;
+  p.wgt = s.wgt * g->wpp;
+  p.wgt2 = 0.;
+  p.history_id = g->first_hist + (uint64_t)i;
+  p.family_id = s.family_id;
+  p.rng_state = pcg_history_state(seed, stride, p.history_id);
+  out[i] = p;
+}
+
+// MeshTally::record_generation + clear_generation with the generation
+// count read on the device (after k_gen_end incremented it).
+__global__ void k_record_generation_dev(double* gen, double* avg, double* var, int64_t n,
+                                        const GenDev* g, int32_t g0) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double dg = (double)(g0 + g->gen);
+  const double old_avg = avg[i];
+  const double val = gen[i] * 1.0;
+  const double a = old_avg + (val - old_avg) / dg;
+  avg[i] = a;
+  double v = var[i];
+  v = v + (((val - old_avg) * (val - a) - (v)) / dg);
+  var[i] = v;
+  gen[i] = 0.0;
+}
+

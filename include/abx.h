@@ -317,6 +317,10 @@ void abx_destroy(abx_problem* p);
 int abx_set_engine(abx_problem* p, int engine);
 int abx_set_tuning(abx_problem* p, const abx_tuning* t);
 int abx_get_tuning(abx_problem* p, abx_tuning* t);
+/* Device time of one transport() of n fresh source histories under the
+ * current tuning (the autotuner's probe).  The current generation's tallies
+ * are cleared afterwards: call it between runs, not inside one. */
+int abx_time_transport(abx_problem* p, size_t n, int converged, double* seconds);
 
 /* Simulation::sample_sources (src/simulation.cpp:55-77): n source particles
  * for histories first_history .. first_history+n-1, sampled on the GPU. */
