@@ -34,6 +34,7 @@ def lib():
         P = C.c_void_p
         L.abx_last_error.restype = C.c_char_p
         L.abx_version.restype = C.c_char_p
+        L.abx_synchronize.argtypes = []
         L.abx_create.argtypes = [C.POINTER(abi.ProblemDesc), C.POINTER(P)]
         L.abx_destroy.argtypes = [P]
         L.abx_destroy.restype = None
@@ -71,6 +72,11 @@ def lib():
         L.abx_problem_stats.argtypes = [P, C.POINTER(abi.RunStats)]
         _lib = L
     return _lib
+
+
+def synchronize():
+    """Wait for all work on the device (abx_synchronize)."""
+    _check(lib().abx_synchronize())
 
 
 def _check(rc):

@@ -304,7 +304,11 @@ TransportOut transport_device(abx_problem* p, const abx_particle* d_in, int64_t 
   HIPC(hipMemsetAsync(p->n_stage.p, 0, sizeof(unsigned long long), p->stream));
   const bool banks = T.st.mode != ABX_FIXED_SOURCE;
   const int sec_cap = needs_secondaries(T) ? 16 : 0;
-  if (sec_cap) p->sec.reserve((size_t)std::max<int64_t>(1, n) * sec_cap * sizeof(SecRec));
+  // The stack is indexed by persistent lane (history engine) or by history
+  // (event engine).  The persistent grid has at most ceil(n / bs) blocks of
+  // bs <= 256 lanes, and any lane may take any history, so size it for
+  // n + 256 slots.
+  if (sec_cap) p->sec.reserve((size_t)(std::max<int64_t>(1, n) + 256) * sec_cap * sizeof(SecRec));
   const CallScalars cs{p->scores.as<double>(), p->counters.as<unsigned long long>(),
                        p->err.as<int32_t>(), p->err_hist.as<unsigned long long>(),
                        p->n_stage.as<unsigned long long>()};
@@ -537,6 +541,10 @@ int abx_set_engine(abx_problem* p, int engine) {
     if (!p || (engine != ABX_ENGINE_EVENT && engine != ABX_ENGINE_HISTORY)) fail(ABX_E_INVALID, "engine");
     p->engine = engine;
   });
+}
+
+int abx_synchronize(void) {
+  ABX_TRY({ HIPC(hipDeviceSynchronize()); });
 }
 
 int abx_set_tuning(abx_problem* p, const abx_tuning* t) {
@@ -791,15 +799,18 @@ static bool run_generations_graph(abx_problem* p, int ngenerations, int nignored
   const abx_tuning tu = p->tuning;
   const int bs = tu.block_size;
   const size_t cur = cursor_lds_bytes(bs, p->P.trk_D, p->P.trk_L);
-  K.stage_tables = tu.stage_tables && cur + (size_t)p->P.blob_bytes <= 65536 ? 1 : 0;
-  const size_t lds = K.stage_tables ? cur + (size_t)p->P.blob_bytes : cur;
+  const bool stage = tu.stage_tables && cur + (size_t)p->P.blob_bytes <= 65536;
+  const size_t lds = stage ? cur + (size_t)p->P.blob_bytes : cur;
   const int trk = T.st.tracking;
   const bool split = T.st.precision == ABX_SPLIT;
   const bool sec = needs_secondaries(T);
   if (sec) return false;  // k-eigenvalue without carter tracking only
   int64_t grid = 0;
   HIPC(dispatch_variant(trk, false, false, split, [&](auto TT, auto, auto, auto SP) {
-    grid = resident_blocks(k_history<decltype(TT)::value, false, false, decltype(SP)::value>, bs, lds);
+    grid = stage ? resident_blocks(k_history<decltype(TT)::value, false, false, decltype(SP)::value, true>,
+                                   bs, lds)
+                 : resident_blocks(k_history<decltype(TT)::value, false, false, decltype(SP)::value, false>,
+                                   bs, lds);
     return hipSuccess;
   }));
   if (tu.blocks_per_cu > 0) {
@@ -822,8 +833,12 @@ static bool run_generations_graph(abx_problem* p, int ngenerations, int nignored
     HIPC(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(k_gen_begin, dim3(1), dim3(64), 0, s, g);
     HIPC(dispatch_variant(trk, false, false, split, [&](auto TT, auto, auto, auto SP) {
-      hipLaunchKernelGGL((k_history<decltype(TT)::value, false, false, decltype(SP)::value>),
-                         dim3((unsigned)grid), dim3(bs), lds, s, p->P, Kc, &g->next);
+      if (stage)
+        hipLaunchKernelGGL((k_history<decltype(TT)::value, false, false, decltype(SP)::value, true>),
+                           dim3((unsigned)grid), dim3(bs), lds, s, p->P, Kc, &g->next);
+      else
+        hipLaunchKernelGGL((k_history<decltype(TT)::value, false, false, decltype(SP)::value, false>),
+                           dim3((unsigned)grid), dim3(bs), lds, s, p->P, Kc, &g->next);
       return hipSuccess;
     }));
     HIPC(hipStreamEndCapture(s, &gr[par][conv][0]));

@@ -473,13 +473,13 @@ static hipError_t history_transport_t(const DevProblem& P, const KCtx& K, hipStr
   const int bs = tu.block_size;
   // Stage the tables in LDS when they fit next to the cursors (64 KB per block).
   const size_t cur = cursor_lds_bytes(bs, P.trk_D, P.trk_L);
-  KCtx Ks = K;
-  Ks.stage_tables = tu.stage_tables && cur + (size_t)P.blob_bytes <= 65536 ? 1 : 0;
-  const size_t lds = Ks.stage_tables ? cur + (size_t)P.blob_bytes : cur;
+  const bool stage = tu.stage_tables && cur + (size_t)P.blob_bytes <= 65536;
+  const size_t lds = stage ? cur + (size_t)P.blob_bytes : cur;
   hipError_t e;
   if ((e = hipMemsetAsync(next, 0, sizeof(unsigned long long), s)) != hipSuccess) return e;
   const int64_t want = (K.n + bs - 1) / bs;
-  int64_t grid = resident_blocks(k_history<TRK, TRACE, SEC, SPLIT>, bs, lds);
+  int64_t grid = stage ? resident_blocks(k_history<TRK, TRACE, SEC, SPLIT, true>, bs, lds)
+                       : resident_blocks(k_history<TRK, TRACE, SEC, SPLIT, false>, bs, lds);
   if (tu.blocks_per_cu > 0) {
     int dev = 0, cus = 0;
     if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
@@ -488,8 +488,12 @@ static hipError_t history_transport_t(const DevProblem& P, const KCtx& K, hipStr
     grid = std::min<int64_t>(grid, (int64_t)cus * tu.blocks_per_cu);
   }
   const int64_t blocks = std::min<int64_t>(want, grid);
-  hipLaunchKernelGGL((k_history<TRK, TRACE, SEC, SPLIT>), dim3((unsigned)blocks), dim3(bs), lds, s, P, Ks,
-                     next);
+  if (stage)
+    hipLaunchKernelGGL((k_history<TRK, TRACE, SEC, SPLIT, true>), dim3((unsigned)blocks), dim3(bs),
+                       lds, s, P, K, next);
+  else
+    hipLaunchKernelGGL((k_history<TRK, TRACE, SEC, SPLIT, false>), dim3((unsigned)blocks), dim3(bs),
+                       lds, s, P, K, next);
   return hipGetLastError();
 }
 

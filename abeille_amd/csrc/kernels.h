@@ -33,7 +33,6 @@ struct KCtx {
   abx_trace_rec* trace;
   int32_t trace_cap;
   int32_t* n_rec;
-  int32_t stage_tables;   // persistent kernels: copy the table blob into LDS
   // device-resident generation loop: bank size and k of the previous
   // generation read from device memory (null = use n / kcol)
   const unsigned long long* n_dev;
@@ -217,9 +216,12 @@ __device__ __forceinline__ DevProblem stage_tables(const DevProblem& P) {
   return rebase_problem(P, dst);
 }
 
-template <int TRK, bool TRACE, bool SEC, bool SPLIT>
+// STAGE: the tables are copied into LDS and read with LDS instructions;
+// otherwise read in place (global loads through L2).  A compile-time choice,
+// so that each variant's loads carry their address space.
+template <int TRK, bool TRACE, bool SEC, bool SPLIT, bool STAGE>
 __global__ void ABX_TKERNEL k_history(DevProblem P0, KCtx K, unsigned long long* next) {
-  const DevProblem P = K.stage_tables ? stage_tables(P0) : P0;
+  const DevProblem P = STAGE ? stage_tables(P0) : P0;
   const int64_t n_hist = K.n_dev ? (int64_t)*K.n_dev : K.n;
   const double kcol = K.kcol_dev ? *K.kcol_dev : K.kcol;
   const int lane = threadIdx.x & 63;

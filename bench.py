@@ -54,6 +54,22 @@ def workload(name, nparticles, tracking, precision):
     return m, desc
 
 
+def measured_traffic(kernel, workload_name, precision):
+    """HBM bytes per launch of `kernel` on this workload from the committed
+    rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE per dispatch, each
+    scaled by the copy-kernel calibration written in the same file), or
+    None when no summary exists for it."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload_name}_{precision}_pmc.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1].split()[0])
+    except OSError:
+        pass
+    return None
+
+
 def algorithmic_bytes(diag):
     events = diag.flights + diag.collisions
     return (events * BYTES_PER_EVENT + diag.tally_updates * BYTES_PER_TALLY_UPDATE +
@@ -105,11 +121,52 @@ def _c4_problems(count, precision, seed=512):
     return out
 
 
+def bench_precision(args):
+    """The workload once in FP64 and once in split precision: collisions/s of
+    both and, per tally, the distribution of the per-bin differences in
+    units of their combined standard error (z), and the mean relative
+    deviation over the bins FP64 resolves to 1 %."""
+    from abeille_amd import Problem, abi, synchronize
+    nparticles = int(args.particles)
+    res = {}
+    for prec in ("fp64", "split"):
+        m, desc = workload(args.workload, nparticles, args.tracking, prec)
+        p = Problem(m)
+        p.run(max(1, args.warmup), nignored=max(1, args.warmup))
+        synchronize()
+        t0 = time.perf_counter()
+        st, _ = p.run_continue(args.steps, nignored=0)
+        synchronize()
+        wall = time.perf_counter() - t0
+        res[prec] = dict(rate=st.diag.collisions / wall, ms=1e3 * wall / args.steps,
+                         k=(st.kcol_avg, st.kcol_err),
+                         tallies=[p.tally_stats(t) for t in range(len(m.tallies))])
+        del p
+    dev = []
+    for (a64, e64), (a32, e32) in zip(res["fp64"]["tallies"], res["split"]["tallies"]):
+        sig = np.hypot(e64, e32)
+        ok = sig > 0
+        z = (a32[ok] - a64[ok]) / sig[ok]
+        good = (a64 > 0) & (e64 < 0.01 * a64)
+        rel = np.abs(a32[good] - a64[good]) / a64[good] if good.any() else np.array([np.nan])
+        dev.append({"bins": int(ok.sum()), "z_mean": float(z.mean()), "z_std": float(z.std()),
+                    "frac_within_3sigma": float((np.abs(z) <= 3).mean()),
+                    "bins_below_1pct_err": int(good.sum()),
+                    "mean_rel_dev_those_bins": float(rel.mean())})
+    out = _common(args, res["split"]["rate"], "collisions/s", res["split"]["ms"], "f64/f32 split",
+                  {"workload": desc + ", FP64 vs split precision", "nparticles": nparticles},
+                  {"fp64_collisions_per_s": res["fp64"]["rate"],
+                   "split_vs_fp64": res["split"]["rate"] / res["fp64"]["rate"],
+                   "k_fp64": res["fp64"]["k"], "k_split": res["split"]["k"],
+                   "tally_deviation": dev})
+    print(json.dumps(out), flush=True)
+
+
 def bench_batch(args):
     """512 independent fixed-source slab problems: the batch engine (split
     precision, shared persistent launches) against the same problems run one
     at a time through the FP64 device loop."""
-    from abeille_amd import Batch, Problem, abi
+    from abeille_amd import Batch, Problem, abi, synchronize
     nb = 2
     split = [Problem(m) for m in _c4_problems(args.problems, abi.SPLIT)]
     # warm-up: a few problems through the batch (module load, grid sizing)
@@ -119,6 +176,7 @@ def bench_batch(args):
     while len(w):
         w.step()
     b = Batch()
+    synchronize()
     t0 = time.perf_counter()
     for p in split:
         b.add(p, nb)
@@ -126,6 +184,7 @@ def bench_batch(args):
     while len(b):
         b.step()
         steps += 1
+    synchronize()
     wall = time.perf_counter() - t0
     coll = sum(p.stats().diag.collisions for p in split)
     hist = sum(p.stats().histories for p in split)
@@ -185,6 +244,8 @@ def bench_serve(args):
     service = np.zeros(len(reqs))
     pc = 0
     for i, m in enumerate(plain_models):
+        print(f"[serve] plain request {i} ({kinds[i]}, {m.settings['nparticles']} histories/batch)",
+              file=sys.stderr, flush=True)
         p = Problem(m)
         t = time.perf_counter()
         st, _ = p.run(2, 0)
@@ -220,6 +281,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="lattice",
                     choices=["lattice", "core", "cube", "batch", "serve"])
+    ap.add_argument("--compare-precision", action="store_true",
+                    help="run the workload in FP64 and in split precision and report the "
+                         "rates and every tally bin's deviation")
     ap.add_argument("--problems", type=int, default=512, help="batch: number of problems")
     ap.add_argument("--requests", type=int, default=100, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=50.0, help="serve: arrivals per second")
@@ -249,11 +313,13 @@ def main():
                                    "reduction), --gpus must be 1"}))
         sys.exit(2)
 
+    if args.compare_precision:
+        return bench_precision(args)
     if args.workload == "batch":
         return bench_batch(args)
     if args.workload == "serve":
         return bench_serve(args)
-    from abeille_amd import Problem, abi
+    from abeille_amd import Problem, abi, synchronize
     nparticles = int(args.particles)
     m, desc = workload(args.workload, nparticles, args.tracking, args.precision)
     engine = abi.ENGINE_EVENT if args.engine == "event" else abi.ENGINE_HISTORY
@@ -261,8 +327,10 @@ def main():
     kmode = m.settings["mode"] == abi.K_EIGENVALUE
     # warmup: inactive generations (also what converges the fission source)
     prob.run(max(1, args.warmup), nignored=max(1, args.warmup) if kmode else 0)
+    synchronize()
     t0 = time.perf_counter()
     st, kh = prob.run_continue(args.steps, nignored=0)
+    synchronize()
     wall = time.perf_counter() - t0
     d = st.diag
     coll_s = d.collisions / wall
@@ -270,8 +338,16 @@ def main():
     kernel_avg = st.kernel_seconds / max(1, st.kernel_launches)
     # algorithmic bytes of the dominant kernel per launch: the events it
     # processed (all flights) over its launches
-    bytes_per_launch = st.kernel_items * BYTES_PER_EVENT / max(1, st.kernel_launches)
+    if args.engine == "history":
+        # one launch transports one generation: all its events, tally updates
+        # and histories
+        bytes_per_launch = algorithmic_bytes(d) / max(1, st.kernel_launches)
+    else:
+        # the flight kernel: the events it processed
+        bytes_per_launch = st.kernel_items * BYTES_PER_EVENT / max(1, st.kernel_launches)
     achieved = bytes_per_launch / kernel_avg / 1e9 if kernel_avg > 0 else 0.0
+    kname = "k_ev_flight" if args.engine == "event" else "k_history"
+    traffic = measured_traffic(kname, args.workload, args.precision)
     out = {
         "metric": METRIC,
         "value": coll_s,
@@ -294,12 +370,12 @@ def main():
         "kcol_err": st.kcol_err,
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_ev_flight" if args.engine == "event" else "k_history",
+            "kernel": kname,
             "achieved": achieved,
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
-            "traffic": None,
+            "traffic": traffic,
             "avg_launch_ms": kernel_avg * 1e3,
             "launches": st.kernel_launches,
             "algorithmic_bytes_per_launch": bytes_per_launch,

@@ -307,6 +307,8 @@ typedef struct abx_tuning {
 
 const char* abx_last_error(void);
 const char* abx_version(void);
+/* Wait for all work on the current device (hipDeviceSynchronize). */
+int abx_synchronize(void);
 
 /* Replaces parse_input_file()'s object graph (src/parser.cpp:77-137) and the
  * DeltaTracker/CarterTracker majorant set-up (src/delta_tracker.cpp:50-70,
