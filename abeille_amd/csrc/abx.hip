@@ -78,7 +78,7 @@ struct abx_problem {
   // transport scratch
   DBuf in, sec, stage, stage_parent, stage_daughter, n_stage, daughters, offsets,
       block_sums, block_sums2, ordered, scores, counters, err, err_hist, trace, n_rec, wsum,
-      pool_next, blob, gdev, khist_dev;
+      pool_next, blob, gdev, khist_dev, one_prob, one_ctx, one_off;
   EventBuffers ev;
   std::vector<hipEvent_t> gen_events;  // graph runs: 2 per generation
   // cancellation
@@ -122,7 +122,7 @@ struct abx_problem {
                    &stage_daughter, &n_stage, &daughters, &offsets, &block_sums, &block_sums2,
                    &ordered, &scores, &counters, &err, &err_hist, &trace, &n_rec, &wsum,
                    &bank_a, &bank_b, &ckeys, &ckeys2, &ctemp, &pool_next, &blob, &gdev,
-                   &khist_dev};
+                   &khist_dev, &one_prob, &one_ctx, &one_off};
     for (auto e : gen_events) (void)hipEventDestroy(e);
     for (DBuf* b : all) b->release();
     ev.release();
@@ -285,6 +285,8 @@ void add_scores(abx_scores& a, const abx_scores& b) {
   a.mig += b.mig;
 }
 
+hipError_t launch_one_as_batch(abx_problem* p, const KCtx& K);
+
 // One transport() over a device-resident bank; ordered fission sites are
 // left in p->ordered.  Tallies accumulate into tally_gen.
 TransportOut transport_device(abx_problem* p, const abx_particle* d_in, int64_t n, double kcol,
@@ -324,8 +326,16 @@ TransportOut transport_device(abx_problem* p, const abx_particle* d_in, int64_t 
     if (p->engine == ABX_ENGINE_HISTORY) {
       if (p->timing) HIPC(hipEventRecord(p->ke0, p->stream));
       p->pool_next.reserve(sizeof(unsigned long long));
-      HIPC(history_transport(p->P, K, p->stream, sec_cap > 0, p->pool_next.as<unsigned long long>(),
-                             p->tuning));
+      if (sec_cap > 0 && !d_trace) {
+        // Histories that carry secondaries (carter splitting, fixed-source
+        // fission) run in the batch kernel with this one problem: the
+        // single-problem kernel's secondary-stack variant faulted on large
+        // modified-fixed-source carter runs and is kept for traced runs only.
+        HIPC(launch_one_as_batch(p, K));
+      } else {
+        HIPC(history_transport(p->P, K, p->stream, sec_cap > 0,
+                               p->pool_next.as<unsigned long long>(), p->tuning));
+      }
       if (p->timing) {
         HIPC(hipEventRecord(p->ke1, p->stream));
         HIPC(hipEventSynchronize(p->ke1));
@@ -1234,6 +1244,31 @@ hipError_t launch_batch_variant(int key, const DevProblem* probs, const KCtx* ct
     return launch_batch<decltype(T)::value, decltype(SE)::value, decltype(SP)::value>(
         probs, ctxs, offsets, n_items, total, D, L, next, s);
   });
+}
+
+// The batch kernel over one problem (see transport_device).  Its lanes index
+// the secondary stack up to ceil(n / 256) * 256, within the n + 256 slots
+// transport_device reserves.
+hipError_t launch_one_as_batch(abx_problem* p, const KCtx& K) {
+  p->one_prob.reserve(sizeof(DevProblem));
+  p->one_ctx.reserve(sizeof(KCtx));
+  p->one_off.reserve(2 * sizeof(int64_t));
+  const int64_t off[2] = {0, K.n};
+  hipError_t e;
+  if ((e = hipMemcpyAsync(p->one_prob.p, &p->P, sizeof(DevProblem), hipMemcpyHostToDevice,
+                          p->stream)) != hipSuccess)
+    return e;
+  if ((e = hipMemcpyAsync(p->one_ctx.p, &K, sizeof(KCtx), hipMemcpyHostToDevice, p->stream)) !=
+      hipSuccess)
+    return e;
+  if ((e = hipMemcpyAsync(p->one_off.p, off, sizeof(off), hipMemcpyHostToDevice, p->stream)) !=
+      hipSuccess)
+    return e;
+  // the host copies above are read before this call returns
+  if ((e = hipStreamSynchronize(p->stream)) != hipSuccess) return e;
+  return launch_batch_variant(variant_key(p), p->one_prob.as<DevProblem>(), p->one_ctx.as<KCtx>(),
+                              p->one_off.as<int64_t>(), 1, K.n, p->P.trk_D, p->P.trk_L,
+                              p->pool_next.as<unsigned long long>(), p->stream);
 }
 
 // Bytes a problem holds on the device while it runs (tables, tallies,
