@@ -860,7 +860,8 @@ static bool run_generations_graph(abx_problem* p, int ngenerations, int nignored
                        &g->n_sites);
     hipLaunchKernelGGL(k_add_block_offsets_dev, dim3((unsigned)nb), dim3(1024), 0, s,
                        p->offsets.as<uint64_t>(), p->block_sums.as<uint64_t>(), &g->n);
-    const unsigned gcap = (unsigned)((cap + 255) / 256);
+    // grid-stride bookkeeping kernels: enough blocks to fill the GPU, not the capacity
+    const unsigned gcap = (unsigned)std::min<uint64_t>((cap + 255) / 256, 2048);
     hipLaunchKernelGGL(k_order_bank_dev, dim3(gcap), dim3(256), 0, s, p->stage.as<abx_banked>(),
                        p->stage_parent.as<uint32_t>(), p->stage_daughter.as<uint32_t>(),
                        p->offsets.as<uint64_t>(), g, p->ordered.as<abx_banked>());
@@ -1220,40 +1221,66 @@ int variant_key(const abx_problem* p) {
   return T.st.tracking * 4 + sec * 2 + (T.st.precision == ABX_SPLIT ? 1 : 0);
 }
 
-template <int TRK, bool SEC, bool SPLIT>
-hipError_t launch_batch(const DevProblem* probs, const KCtx* ctxs, const int64_t* offsets, int n_items,
-                        int64_t total, int D, int L, unsigned long long* next, hipStream_t s) {
+// Chunks of `chunk` histories over the problems of one launch (item indices
+// local to the launch's arrays).
+std::vector<BatchChunk> make_chunks(const std::vector<int64_t>& n, int64_t chunk) {
+  std::vector<BatchChunk> c;
+  for (size_t j = 0; j < n.size(); j++)
+    for (int64_t a = 0; a < n[j]; a += chunk)
+      c.push_back(BatchChunk{(int32_t)j, 0, a, std::min(n[j], a + chunk)});
+  return c;
+}
+
+// Chunk length: about eight chunks per CU over the launch, 256 - 8192
+// histories (at least one block's lanes, so that lanes <= histories + 255).
+int64_t chunk_length(int64_t total) {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    cus = 256;
+  return std::max<int64_t>(256, std::min<int64_t>(8192, total / std::max(1, 8 * cus)));
+}
+
+template <int TRK, bool SEC, bool SPLIT, bool STAGE>
+hipError_t launch_chunks(const DevProblem* probs, const KCtx* ctxs, const BatchChunk* chunks,
+                         int n_chunks, int D, int L, size_t blob_max, unsigned* next,
+                         hipStream_t s) {
   const int bs = 256;
-  const size_t lds = (size_t)bs * trk_bytes(D, L);
+  const size_t lds = cursor_lds_bytes(bs, D, L) + (STAGE ? blob_max : 0);
   hipError_t e;
-  if ((e = hipMemsetAsync(next, 0, sizeof(unsigned long long), s)) != hipSuccess) return e;
-  const int64_t want = (total + bs - 1) / bs;
-  const int64_t blocks =
-      std::min<int64_t>(want, resident_blocks(k_batch_history<TRK, SEC, SPLIT>, bs, lds));
-  hipLaunchKernelGGL((k_batch_history<TRK, SEC, SPLIT>), dim3((unsigned)blocks), dim3(bs), lds, s,
-                     probs, ctxs, offsets, n_items, D, L, next);
+  if ((e = hipMemsetAsync(next, 0, sizeof(unsigned), s)) != hipSuccess) return e;
+  const int64_t blocks = std::min<int64_t>(
+      n_chunks, resident_blocks(k_batch_chunks<TRK, SEC, SPLIT, STAGE>, bs, lds));
+  hipLaunchKernelGGL((k_batch_chunks<TRK, SEC, SPLIT, STAGE>), dim3((unsigned)blocks), dim3(bs),
+                     lds, s, probs, ctxs, chunks, n_chunks, D, L, next);
   return hipGetLastError();
 }
 
-hipError_t launch_batch_variant(int key, const DevProblem* probs, const KCtx* ctxs,
-                                const int64_t* offsets, int n_items, int64_t total, int D, int L,
-                                unsigned long long* next, hipStream_t s) {
+// One launch over the chunks of problems of one kernel variant; the tables
+// are staged in LDS when the largest blob fits next to the cursors.
+hipError_t launch_chunks_variant(int key, const DevProblem* probs, const KCtx* ctxs,
+                                 const BatchChunk* chunks, int n_chunks, int D, int L,
+                                 size_t blob_max, unsigned* next, hipStream_t s) {
   const int trk = key / 4;
   const bool sec = (key / 2) & 1, split = key & 1;
+  const bool stage = cursor_lds_bytes(256, D, L) + blob_max <= 65536;
   return dispatch_variant(trk, false, sec || trk == 2, split, [&](auto T, auto, auto SE, auto SP) {
-    return launch_batch<decltype(T)::value, decltype(SE)::value, decltype(SP)::value>(
-        probs, ctxs, offsets, n_items, total, D, L, next, s);
+    if (stage)
+      return launch_chunks<decltype(T)::value, decltype(SE)::value, decltype(SP)::value, true>(
+          probs, ctxs, chunks, n_chunks, D, L, blob_max, next, s);
+    return launch_chunks<decltype(T)::value, decltype(SE)::value, decltype(SP)::value, false>(
+        probs, ctxs, chunks, n_chunks, D, L, blob_max, next, s);
   });
 }
 
-// The batch kernel over one problem (see transport_device).  Its lanes index
-// the secondary stack up to ceil(n / 256) * 256, within the n + 256 slots
-// transport_device reserves.
+// The batch kernel over one problem (see transport_device).  Chunks hold at
+// least 256 histories, so its lanes index the secondary stack below
+// ceil(n / 256) * 256, within the n + 256 slots transport_device reserves.
 hipError_t launch_one_as_batch(abx_problem* p, const KCtx& K) {
+  const std::vector<BatchChunk> ch = make_chunks({K.n}, chunk_length(K.n));
   p->one_prob.reserve(sizeof(DevProblem));
   p->one_ctx.reserve(sizeof(KCtx));
-  p->one_off.reserve(2 * sizeof(int64_t));
-  const int64_t off[2] = {0, K.n};
+  p->one_off.reserve(std::max<size_t>(1, ch.size()) * sizeof(BatchChunk));
   hipError_t e;
   if ((e = hipMemcpyAsync(p->one_prob.p, &p->P, sizeof(DevProblem), hipMemcpyHostToDevice,
                           p->stream)) != hipSuccess)
@@ -1261,14 +1288,15 @@ hipError_t launch_one_as_batch(abx_problem* p, const KCtx& K) {
   if ((e = hipMemcpyAsync(p->one_ctx.p, &K, sizeof(KCtx), hipMemcpyHostToDevice, p->stream)) !=
       hipSuccess)
     return e;
-  if ((e = hipMemcpyAsync(p->one_off.p, off, sizeof(off), hipMemcpyHostToDevice, p->stream)) !=
-      hipSuccess)
+  if ((e = hipMemcpyAsync(p->one_off.p, ch.data(), ch.size() * sizeof(BatchChunk),
+                          hipMemcpyHostToDevice, p->stream)) != hipSuccess)
     return e;
   // the host copies above are read before this call returns
   if ((e = hipStreamSynchronize(p->stream)) != hipSuccess) return e;
-  return launch_batch_variant(variant_key(p), p->one_prob.as<DevProblem>(), p->one_ctx.as<KCtx>(),
-                              p->one_off.as<int64_t>(), 1, K.n, p->P.trk_D, p->P.trk_L,
-                              p->pool_next.as<unsigned long long>(), p->stream);
+  return launch_chunks_variant(variant_key(p), p->one_prob.as<DevProblem>(), p->one_ctx.as<KCtx>(),
+                               p->one_off.as<BatchChunk>(), (int)ch.size(), p->P.trk_D,
+                               p->P.trk_L, (size_t)p->P.blob_bytes,
+                               reinterpret_cast<unsigned*>(p->pool_next.p), p->stream);
 }
 
 // Bytes a problem holds on the device while it runs (tables, tallies,
@@ -1291,14 +1319,14 @@ size_t problem_footprint(const abx_problem* p) {
 struct abx_batch {
   std::vector<BatchEntry> items;
   hipStream_t stream = nullptr;
-  DBuf probs, ctxs, offsets, scal, next, sec;
+  DBuf probs, ctxs, offsets, scal, next, sec, chunks;
   std::vector<BatchScalars> h_scal;
   size_t budget = 0;  // device bytes admitted problems may hold (0 = no limit)
   size_t used = 0;
   double kernel_ms = 0.0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   ~abx_batch() {
-    DBuf* all[] = {&probs, &ctxs, &offsets, &scal, &next, &sec};
+    DBuf* all[] = {&probs, &ctxs, &offsets, &scal, &next, &sec, &chunks};
     for (DBuf* b : all) b->release();
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
@@ -1344,22 +1372,30 @@ void batch_step(abx_batch* b, std::vector<abx_problem*>& finished) {
   if (max_sec) b->sec.reserve(lanes * max_sec * sizeof(SecRec));
   std::vector<DevProblem> hp(ni);
   std::vector<KCtx> hk(ni);
-  std::vector<int64_t> ho(ni + 1);
   b->probs.reserve((size_t)ni * sizeof(DevProblem));
   b->ctxs.reserve((size_t)ni * sizeof(KCtx));
-  b->offsets.reserve((size_t)(ni + 16) * sizeof(int64_t));  // + one entry per variant
   b->next.reserve(8 * sizeof(unsigned long long));
   if (!b->e0) {
     HIPC(hipEventCreate(&b->e0));
     HIPC(hipEventCreate(&b->e1));
   }
-  HIPC(hipEventRecord(b->e0, b->stream));
-  int g0 = 0, launches = 0;
+  // chunks of every variant group, uploaded once
+  struct Group {
+    int key, g0, cnt, D, L;
+    size_t blob_max;
+    int64_t c0, nc;
+  };
+  std::vector<Group> groups;
+  std::vector<BatchChunk> all_chunks;
+  int64_t total_all = 0;
+  for (auto& it : b->items) total_all += it.n;
+  const int64_t clen = chunk_length(total_all);
+  int g0 = 0;
   while (g0 < ni) {
     const int key = variant_key(b->items[order[g0]].p);
     int g1 = g0;
-    int D = 1, L = 1;
-    int64_t total = 0;
+    Group gr{key, g0, 0, 1, 1, 16, (int64_t)all_chunks.size(), 0};
+    std::vector<int64_t> n;
     while (g1 < ni && variant_key(b->items[order[g1]].p) == key) {
       const int j = order[g1];
       BatchEntry& it = b->items[j];
@@ -1369,30 +1405,40 @@ void batch_step(abx_batch* b, std::vector<abx_problem*>& finished) {
       hk[g1] = make_kctx(p, p->bank_a.as<abx_particle>(), it.n, p->run_kcol, 1, cs,
                          b->sec.as<SecRec>(), sec_cap);
       hp[g1] = p->P;
-      ho[g1] = total;
-      total += it.n;
-      D = std::max(D, p->P.trk_D);
-      L = std::max(L, p->P.trk_L);
+      n.push_back(it.n);
+      gr.D = std::max(gr.D, p->P.trk_D);
+      gr.L = std::max(gr.L, p->P.trk_L);
+      gr.blob_max = std::max(gr.blob_max, (size_t)p->P.blob_bytes);
       g1++;
     }
-    ho[g1] = total;
-    const int cnt = g1 - g0;
-    // per-group slices of the device arrays (offsets rebased to the group)
-    std::vector<int64_t> off(cnt + 1);
-    for (int k = 0; k <= cnt; k++) off[k] = ho[g0 + k] - ho[g0];
-    HIPC(hipMemcpyAsync(b->probs.as<DevProblem>() + g0, hp.data() + g0, cnt * sizeof(DevProblem),
-                        hipMemcpyHostToDevice, b->stream));
-    HIPC(hipMemcpyAsync(b->ctxs.as<KCtx>() + g0, hk.data() + g0, cnt * sizeof(KCtx),
-                        hipMemcpyHostToDevice, b->stream));
-    HIPC(hipMemcpyAsync(b->offsets.as<int64_t>() + g0 + launches, off.data(), (cnt + 1) * sizeof(int64_t),
-                        hipMemcpyHostToDevice, b->stream));
-    if (total > 0)
-      HIPC(launch_batch_variant(key, b->probs.as<DevProblem>() + g0, b->ctxs.as<KCtx>() + g0,
-                                b->offsets.as<int64_t>() + g0 + launches, cnt, total, D, L,
-                                b->next.as<unsigned long long>() + (launches % 8), b->stream));
-    launches++;
+    gr.cnt = g1 - g0;
+    const std::vector<BatchChunk> c = make_chunks(n, clen);
+    all_chunks.insert(all_chunks.end(), c.begin(), c.end());
+    gr.nc = (int64_t)c.size();
+    groups.push_back(gr);
     g0 = g1;
   }
+  b->chunks.reserve(std::max<size_t>(1, all_chunks.size()) * sizeof(BatchChunk));
+  HIPC(hipMemcpyAsync(b->probs.p, hp.data(), ni * sizeof(DevProblem), hipMemcpyHostToDevice,
+                      b->stream));
+  HIPC(hipMemcpyAsync(b->ctxs.p, hk.data(), ni * sizeof(KCtx), hipMemcpyHostToDevice, b->stream));
+  if (!all_chunks.empty())
+    HIPC(hipMemcpyAsync(b->chunks.p, all_chunks.data(), all_chunks.size() * sizeof(BatchChunk),
+                        hipMemcpyHostToDevice, b->stream));
+  HIPC(hipEventRecord(b->e0, b->stream));
+  int launches = 0;
+  for (const Group& gr : groups) {
+    if (gr.nc > 0)
+      HIPC(launch_chunks_variant(gr.key, b->probs.as<DevProblem>() + gr.g0,
+                                 b->ctxs.as<KCtx>() + gr.g0, b->chunks.as<BatchChunk>() + gr.c0,
+                                 (int)gr.nc, gr.D, gr.L, gr.blob_max,
+                                 reinterpret_cast<unsigned*>(b->next.as<unsigned long long>() +
+                                                             (launches % 8)),
+                                 b->stream));
+    launches++;
+  }
+  // the host arrays above are read by the copies before they go out of scope
+  HIPC(hipStreamSynchronize(b->stream));
   HIPC(hipEventRecord(b->e1, b->stream));
   b->h_scal.resize(ni);
   HIPC(hipMemcpyAsync(b->h_scal.data(), b->scal.p, (size_t)ni * sizeof(BatchScalars),

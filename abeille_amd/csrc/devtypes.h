@@ -129,7 +129,8 @@ struct DevProblem {
   const int32_t* fissile;  // [nmat]
   const double* scat_cdf;  // [nmat][G][G]
   const double* chi_cdf;   // [nmat][G][G]
-  const int32_t* ang_off;  // [nmat][G][G] + 1 into ang_mu/pdf/cdf
+  const int32_t* ang_base; // [nmat] into ang_off, -1 = isotropic material
+  const int32_t* ang_off;  // per anisotropic material [G][G] + 1 into ang_mu/pdf/cdf
   const double* ang_mu;
   const double* ang_pdf;
   const double* ang_cdf;
@@ -182,6 +183,7 @@ ABX_HD DevProblem rebase_problem(const DevProblem& P, const unsigned char* base)
   ABX_REBASE(fissile);
   ABX_REBASE(scat_cdf);
   ABX_REBASE(chi_cdf);
+  ABX_REBASE(ang_base);
   ABX_REBASE(ang_off);
   ABX_REBASE(ang_mu);
   ABX_REBASE(ang_pdf);

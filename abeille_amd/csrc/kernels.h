@@ -404,6 +404,109 @@ __global__ void ABX_TKERNEL k_batch_history(const DevProblem* __restrict__ probs
   flush_grouped(ctxs, acc, sc, s.c);
 }
 
+// Work item of the block-affine batch kernel: histories [first, last) of
+// problem `item` (local history ids, read from ctxs[item].in).
+struct BatchChunk {
+  int32_t item;
+  int32_t pad_;
+  int64_t first, last;
+};
+
+// Many problems in one persistent launch, one problem per block at a time:
+// a block takes a chunk of one problem's histories from a global counter,
+// stages that problem's tables in LDS when it changes (STAGE), and its lanes
+// take the chunk's histories from an LDS counter as in k_history.  The
+// problem's tables and context are block-uniform, so they sit in scalar
+// registers and the table reads are LDS reads.  A block waits at a chunk's
+// end for its longest history; chunks of several thousand histories keep
+// that wait small.
+template <int TRK, bool SEC, bool SPLIT, bool STAGE>
+__global__ void ABX_TKERNEL k_batch_chunks(const DevProblem* __restrict__ probs,
+                                           const KCtx* __restrict__ ctxs,
+                                           const BatchChunk* __restrict__ chunks, int n_chunks,
+                                           int D, int L, unsigned* next_chunk) {
+  __shared__ int s_chunk;
+  __shared__ unsigned long long s_next;
+  const int lane = threadIdx.x & 63;
+  Trk t;
+  trk_bind(t, abx_lds, blockDim.x, threadIdx.x, D, L);
+  unsigned char* tables = abx_lds + cursor_lds_bytes(blockDim.x, D, L);
+  int staged = -1;
+  GpuSink<false, SEC, SPLIT> s;
+  s.init(nullptr, nullptr, 0);
+  s.sslot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (;;) {
+    if (threadIdx.x == 0) s_chunk = (int)atomicAdd(next_chunk, 1u);
+    __syncthreads();
+    const int c = s_chunk;
+    if (c >= n_chunks) break;
+    const BatchChunk ch = chunks[c];
+    const int j = ch.item;
+    if (threadIdx.x == 0) s_next = (unsigned long long)ch.first;
+    if (STAGE && j != staged) {
+      const int64_t n16 = probs[j].blob_bytes >> 4;
+      const uint4* src = (const uint4*)probs[j].blob;
+      uint4* d4 = (uint4*)tables;
+      for (int64_t k = threadIdx.x; k < n16; k += blockDim.x) d4[k] = src[k];
+      staged = j;
+    }
+    __syncthreads();
+    const DevProblem P = STAGE ? rebase_problem(probs[j], tables) : probs[j];
+    const KCtx& K = ctxs[j];
+    s.K = &K;
+    s.P = &P;
+    Scores6 sc{0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < 8; k++) s.c[k] = 0;
+    const bool converged = K.converged != 0;
+    Part p;
+    p.alive = 0;
+    int m = -1;
+    int64_t steps = 0;
+    bool done = false;
+    for (;;) {
+      const bool need = !p.alive && !done;
+      const unsigned long long mask = __ballot(need);
+      if (mask) {
+        const int leader = __ffsll((long long)mask) - 1;
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(&s_next, (unsigned long long)__popcll(mask));
+        base = __shfl(base, leader, 64);
+        if (need) {
+          const int64_t i =
+              (int64_t)(base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull)));
+          if (i < ch.last) {
+            s.slot = i;
+            s.nsec = 0;
+            steps = 0;
+            load_part(P, K.in[i], p);
+            s.c[0]++;
+            if (!start_history(P, s, p, t, m)) {
+              if (K.daughters) K.daughters[i] = 0;
+            }
+          } else {
+            done = true;
+          }
+        }
+      }
+      if (__ballot(!done) == 0ull) break;
+      if (p.alive) {
+        int ev = 0;
+        bool hc = step_flight<TRK>(P, s, p, t, m, converged, sc, ev);
+        if (p.alive && hc) step_collision(P, s, p, m, K.kcol, converged, sc, ev);
+        bool live = step_finish<TRK>(P, s, p, t, m, ev);
+        if (live && ++steps > 100000000) {
+          s.error(-3, p.hid);
+          p.alive = 0;
+          live = false;
+        }
+        if (!live && K.daughters) K.daughters[s.slot] = p.daughters;
+      }
+    }
+    flush_scores(K, sc, s.c);
+    __syncthreads();
+  }
+}
+
 __global__ void k_sample_sources(DevProblem P, DevSources S, uint64_t first, int64_t n,
                                  abx_particle* out, int32_t* err,
                                  unsigned long long* err_hist) {
@@ -698,29 +801,44 @@ __global__ void k_add_block_offsets_dev(uint64_t* out, const uint64_t* block_off
 __global__ void k_order_bank_dev(const abx_banked* stage, const uint32_t* parent,
                                  const uint32_t* daughter, const uint64_t* offsets,
                                  const GenDev* g, abx_banked* out) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t m = (int64_t)(g->n_stage < g->cap ? g->n_stage : g->cap);
-  if (k >= m || g->err) return;
-  const uint64_t dst = offsets[parent[k]] + daughter[k];
-  if (dst < g->cap) out[dst] = stage[k];  // beyond: k_gen_end reports the overflow
+  if (g->err) return;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t dst = offsets[parent[k]] + daughter[k];
+    if (dst < g->cap) out[dst] = stage[k];  // beyond: k_gen_end reports the overflow
+  }
 }
 
+// Grid-stride over the live sites; one pair of atomics per block.
 __global__ void k_weight_sums_dev(const abx_banked* b, GenDev* g) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double part[2][16];
   const int64_t n = (int64_t)(g->n_sites < g->cap ? g->n_sites : g->cap);
   double wp = 0., wn = 0.;
-  if (i < n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
     const double w = b[i].wgt;
     if (w > 0.)
-      wp = w;
+      wp += w;
     else
-      wn = -w;
+      wn -= w;
   }
   wp = wave_sum(wp);
   wn = wave_sum(wn);
-  if ((threadIdx.x & 63) == 0) {
-    if (wp != 0.) unsafeAtomicAdd(g->wsum, wp);
-    if (wn != 0.) unsafeAtomicAdd(g->wsum + 1, wn);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    part[0][wid] = wp;
+    part[1][wid] = wn;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0., c = 0.;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+      a += part[0][w];
+      c += part[1][w];
+    }
+    if (a != 0.) unsafeAtomicAdd(g->wsum, a);
+    if (c != 0.) unsafeAtomicAdd(g->wsum + 1, c);
   }
 }
 
@@ -759,11 +877,8 @@ __global__ void k_gen_end(GenDev* g, double W, int converged, double* kcol_hist)
   g->n = g->err ? 0 : g->n_sites;
 }
 
-__global__ void k_rebank_dev(const abx_banked* b, const GenDev* g, uint64_t seed, uint64_t stride,
-                             abx_particle* out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int64_t)g->n) return;
-  const abx_banked& s = b[i];
+__device__ __forceinline__ void rebank_one(const abx_banked& s, int64_t i, const GenDev* g,
+                                           uint64_t seed, uint64_t stride, abx_particle* out) {
   abx_particle p;
   for (int k = 0; k < 3; k++) {
     p.r[k] = s.r[k];
@@ -776,6 +891,14 @@ __global__ void k_rebank_dev(const abx_banked* b, const GenDev* g, uint64_t seed
   p.family_id = s.family_id;
   p.rng_state = pcg_history_state(seed, stride, p.history_id);
   out[i] = p;
+}
+
+__global__ void k_rebank_dev(const abx_banked* b, const GenDev* g, uint64_t seed, uint64_t stride,
+                             abx_particle* out) {
+  const int64_t n = (int64_t)g->n;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    rebank_one(b[i], i, g, seed, stride, out);
 }
 
 // MeshTally::record_generation + clear_generation with the generation

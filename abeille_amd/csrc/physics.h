@@ -159,12 +159,18 @@ ABX_HD V3 rotate(const V3& u, double mu, double phi) {
 }
 
 ABX_NOINL double sample_mu(const DevProblem& P, Part& p, int m, int gi, int go) {
-  const int k = (m * P.G + gi) * P.G + go;
+  const double xi = p_rand(p);
+  const int base = P.ang_base[m];
+  if (base < 0) {
+    // the isotropic table {mu -1, 1; pdf 0.5, 0.5; cdf 0, 1} evaluated below
+    // reduces to this (xi == 0 returns mu[0] = -1, the same value)
+    return -1. + ((xi - 0.) / 0.5);
+  }
+  const int k = base + gi * P.G + go;
   const int b0 = P.ang_off[k], n = P.ang_off[k + 1] - b0;
   const double* mu = P.ang_mu + b0;
   const double* pdf = P.ang_pdf + b0;
   const double* cdf = P.ang_cdf + b0;
-  const double xi = p_rand(p);
   int l = cdf_lower_bound(cdf, n, xi);
   if (xi == cdf[l]) return mu[l];
   l--;

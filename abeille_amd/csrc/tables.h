@@ -41,7 +41,8 @@ struct HostTables {
   std::vector<double> Et, Eabs, Ef, nu_tot, nu_d, Eel, vEf;
   std::vector<int32_t> fissile;
   std::vector<double> scat_cdf, chi_cdf;
-  std::vector<int32_t> ang_off;
+  std::vector<int32_t> ang_base;  // [nmat] start in ang_off, -1 = isotropic everywhere
+  std::vector<int32_t> ang_off;   // per anisotropic material: [G][G] + 1 into ang_mu/pdf/cdf
   std::vector<double> ang_mu, ang_pdf, ang_cdf;
   std::vector<int32_t> dgrp_n;
   std::vector<double> dgrp_cdf;
@@ -260,7 +261,8 @@ static inline void build_tables(const abx_problem_desc* d, HostTables& T) {
   T.fissile.assign(M, 0);
   T.scat_cdf.assign(MGG, 0.);
   T.chi_cdf.assign(MGG, 0.);
-  T.ang_off.assign(MGG + 1, 0);
+  T.ang_base.assign(M, -1);
+  T.ang_off.clear();
   T.dgrp_n.assign(M, 0);
   int ndmax = 1;
   for (int m = 0; m < M; m++) ndmax = std::max(ndmax, d->materials[m].n_delayed);
@@ -324,36 +326,32 @@ static inline void build_tables(const abx_problem_desc* d, HostTables& T) {
       build_discrete_cdf(&chi[(size_t)g * G], G, &T.chi_cdf[k * G]);
       if (Ef[g] != 0. && (nup[g] + nud[g]) > 0.) T.fissile[m] = 1;
     }
-    // angle tables
-    for (int gi = 0; gi < G; gi++)
-      for (int go = 0; go < G; go++) {
-        const size_t k = ((size_t)m * G + gi) * G + go;
-        T.ang_off[k] = (int32_t)T.ang_mu.size();
-        if (a.angle_offsets) {
+    // angle tables.  A material without tables is isotropic everywhere
+    // (MGAngleDistribution(), src/mg_angle_distribution.cpp:31-32): its
+    // cosine is sampled in closed form, so it stores nothing.
+    if (a.angle_offsets) {
+      T.ang_base[m] = (int32_t)T.ang_off.size();
+      for (int gi = 0; gi < G; gi++)
+        for (int go = 0; go < G; go++) {
+          T.ang_off.push_back((int32_t)T.ang_mu.size());
           const int b0 = a.angle_offsets[gi * G + go], b1 = a.angle_offsets[gi * G + go + 1];
           tables_require(b1 - b0 >= 2, "angle table too short");
+          const size_t first = T.ang_mu.size();
           for (int q = b0; q < b1; q++) {
             T.ang_mu.push_back(a.angle_mu[q]);
             T.ang_pdf.push_back(a.angle_pdf[q]);
             T.ang_cdf.push_back(a.angle_cdf[q]);
           }
-          tables_require(T.ang_cdf[T.ang_off[k]] == 0. && T.ang_cdf.back() == 1., "angle cdf must run 0..1");
-        } else {  // MGAngleDistribution() (src/mg_angle_distribution.cpp:31-32)
-          T.ang_mu.push_back(-1.);
-          T.ang_mu.push_back(1.);
-          T.ang_pdf.push_back(0.5);
-          T.ang_pdf.push_back(0.5);
-          T.ang_cdf.push_back(0.);
-          T.ang_cdf.push_back(1.);
+          tables_require(T.ang_cdf[first] == 0. && T.ang_cdf.back() == 1., "angle cdf must run 0..1");
         }
-      }
+      T.ang_off.push_back((int32_t)T.ang_mu.size());
+    }
     T.dgrp_n[m] = a.n_delayed;
     if (a.n_delayed > 0) {
       tables_require(a.delayed_probabilities != nullptr, "delayed probabilities");
       build_discrete_cdf(a.delayed_probabilities, a.n_delayed, &T.dgrp_cdf[(size_t)m * ndmax]);
     }
   }
-  T.ang_off[MGG] = (int32_t)T.ang_mu.size();
   // energy limits (src/parser.cpp:152-167)
   T.min_energy = std::max(0., T.ebounds.front());
   T.max_energy = std::min(100000., T.ebounds.back());
@@ -464,6 +462,7 @@ static inline DevProblem dev_view(const HostTables& T, Map map, double* tally_da
   P.fissile = map(T.fissile);
   P.scat_cdf = map(T.scat_cdf);
   P.chi_cdf = map(T.chi_cdf);
+  P.ang_base = map(T.ang_base);
   P.ang_off = map(T.ang_off);
   P.ang_mu = map(T.ang_mu);
   P.ang_pdf = map(T.ang_pdf);

@@ -351,3 +351,13 @@ def subcritical_slab(nparticles=20000, tracking="carter", cancel=True, half=1.0)
     if cancel:
         m.cancellation_mesh([-half, -5., -5.], [half, 5., 5.], [10, 10, 10])
     return m
+
+
+def anisotropic_slab(nparticles=20000, tracking="surface", moments=(0.3, 0.1)):
+    """The PUa-1-0-SL critical slab with P1/P2 Legendre scattering moments,
+    linearised like LegendreDistribution::linearize (the scattering tables
+    of an anisotropic multigroup material)."""
+    from .deck import linearize_legendre
+    m, _ = la13511("PUa-1-0-SL", nparticles, tracking=tracking)
+    m.materials[0]["angles"] = [linearize_legendre(list(moments))]
+    return m

@@ -251,8 +251,8 @@ def bench_serve(args):
         m = plain_models[i]
         print(f"[serve] plain request {i} ({kinds[i]}, {m.settings['nparticles']} histories/batch)",
               file=sys.stderr, flush=True)
-        p = Problem(m)
         t = time.perf_counter()
+        p = Problem(m)          # staging the tables is part of serving a request
         st, _ = p.run(2, 0)
         service[k] = time.perf_counter() - t
         pc += st.diag.collisions
@@ -302,6 +302,10 @@ def main():
     ap.add_argument("--engine", default="history", choices=["event", "history"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "split"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shape instead of the autotuner's choice")
+    ap.add_argument("--tuning", default=None,
+                    help="fixed launch shape block_size,blocks_per_cu,stage_tables (no autotuning)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (calibrates the "
@@ -335,6 +339,14 @@ def main():
     engine = abi.ENGINE_EVENT if args.engine == "event" else abi.ENGINE_HISTORY
     prob = Problem(m, engine=engine)
     kmode = m.settings["mode"] == abi.K_EIGENVALUE
+    tuning = prob.tuning()
+    if args.tuning:
+        bs, bpc, st = (int(x) for x in args.tuning.split(","))
+        prob.set_tuning(block_size=bs, blocks_per_cu=bpc, stage_tables=st)
+        tuning = prob.tuning()
+    elif not args.no_autotune and engine == abi.ENGINE_HISTORY:
+        from abeille_amd.autotune import autotune
+        tuning, _ = autotune(prob, m, sample=nparticles, cache=None)
     # warmup: inactive generations (also what converges the fission source)
     prob.run(max(1, args.warmup), nignored=max(1, args.warmup) if kmode else 0)
     synchronize()
@@ -372,7 +384,7 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f64/f32 split",
         "data": "synthetic (seeded cross sections and geometry, no external data)",
         "config": {"workload": desc, "engine": args.engine, "nparticles": nparticles,
-                   "generations_timed": args.steps},
+                   "generations_timed": args.steps, "tuning": tuning},
         "histories_per_s": hist_s,
         "events_per_s": (d.flights + d.collisions) / wall,
         "virtual_collisions_per_s": d.virtual_collisions / wall,

@@ -21,6 +21,8 @@ CASES = {
     "la_inf": lambda: problems.la13511("Ua-1-1-IN", 600)[0],
     "mfs_carter": lambda: problems.subcritical_slab(600, tracking="carter", cancel=False),
     "mfs_delta": lambda: problems.subcritical_slab(600, tracking="delta", cancel=False),
+    "aniso_st": lambda: problems.anisotropic_slab(600, tracking="surface"),
+    "aniso_dt": lambda: problems.anisotropic_slab(600, tracking="delta"),
 }
 
 

@@ -25,6 +25,7 @@ CASES = {
     "slab_PUa_1_0_SL": lambda: problems.la13511("PUa-1-0-SL", 2000)[0],
     "slab_mfs_carter": lambda: problems.subcritical_slab(2000, tracking="carter", cancel=False),
     "slab_mfs_delta": lambda: problems.subcritical_slab(2000, tracking="delta", cancel=False),
+    "slab_p2_scatter": lambda: problems.anisotropic_slab(2000, tracking="surface"),
 }
 
 
