@@ -283,127 +283,6 @@ __global__ void ABX_TKERNEL k_history(DevProblem P0, KCtx K, unsigned long long*
   flush_scores(K, sc, s.c);
 }
 
-// Scores and counters of lanes that may belong to different problems of a
-// batch: one wave reduction and one set of atomics per distinct key.  Every
-// lane of the wave must call this (the reductions shuffle across the wave).
-__device__ __forceinline__ void flush_grouped(const KCtx* ctxs, int key, const Scores6& sc,
-                                              const uint32_t c[8]) {
-  unsigned long long todo = __ballot(key >= 0);
-  while (todo) {
-    const int leader = __ffsll((long long)todo) - 1;
-    const int kk = __builtin_amdgcn_readfirstlane(__shfl(key, leader, 64));
-    const bool mine = key == kk;
-    todo &= ~__ballot(mine);
-    const Scores6 z = mine ? sc : Scores6{0, 0, 0, 0, 0, 0};
-    uint32_t cz[8];
-#pragma unroll
-    for (int i = 0; i < 8; i++) cz[i] = mine ? c[i] : 0u;
-    flush_scores(ctxs[kk], z, cz);
-  }
-}
-
-// Many independent problems in one persistent launch.  `offsets` [n + 1]
-// splits the global history range among the problems (problem j owns
-// offsets[j] .. offsets[j+1]-1, its local history i - offsets[j] read from
-// ctxs[j].in).  Lanes take histories as in k_history; each loop iteration
-// then runs one step for every distinct problem held by the wavefront's
-// lanes, with that problem's tables and context loaded as wave-uniform
-// (scalar) data.  Histories are taken in order, so a wavefront usually holds
-// one or two problems.  D, L: largest cursor capacities in the batch.
-template <int TRK, bool SEC, bool SPLIT>
-__global__ void ABX_TKERNEL k_batch_history(const DevProblem* __restrict__ probs,
-                                            const KCtx* __restrict__ ctxs,
-                                            const int64_t* __restrict__ offsets, int n_items,
-                                            int D, int L, unsigned long long* next) {
-  const int lane = threadIdx.x & 63;
-  const int64_t total = offsets[n_items];
-  Scores6 sc{0, 0, 0, 0, 0, 0};
-  GpuSink<false, SEC, SPLIT> s;
-  s.init(nullptr, nullptr, 0);
-  s.sslot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  Trk t;
-  trk_bind(t, abx_lds, blockDim.x, threadIdx.x, D, L);
-  Part p;
-  p.alive = 0;
-  int m = -1, q = -1, acc = -1;
-  int64_t steps = 0, gi = 0;
-  bool done = false;
-  for (;;) {
-    const bool need = !p.alive && !done;
-    const unsigned long long mask = __ballot(need);
-    bool fresh = false;
-    if (mask) {
-      const int leader = __ffsll((long long)mask) - 1;
-      unsigned long long base = 0;
-      if (lane == leader) base = atomicAdd(next, (unsigned long long)__popcll(mask));
-      base = __shfl(base, leader, 64);
-      if (need) {
-        gi = (int64_t)(base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull)));
-        if (gi < total) {
-          int lo = 0, hi = n_items;  // last j with offsets[j] <= gi
-          while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (offsets[mid] <= gi) lo = mid; else hi = mid;
-          }
-          q = lo;
-          fresh = true;
-        } else {
-          done = true;
-        }
-      }
-    }
-    if (__ballot(!done) == 0ull) break;
-    unsigned long long todo = __ballot(fresh || p.alive);
-    while (todo) {
-      const int leader = __ffsll((long long)todo) - 1;
-      const int qq = __builtin_amdgcn_readfirstlane(__shfl(q, leader, 64));
-      const bool mine = (fresh || p.alive) && q == qq;
-      todo &= ~__ballot(mine);
-      if (!mine) continue;
-      const DevProblem& P = probs[qq];
-      const KCtx& K = ctxs[qq];
-      s.K = &K;
-      s.P = &P;
-      if (acc != qq) {  // scores belong to one problem at a time
-        if (acc >= 0) {
-          const double v[6] = {sc.k_col, sc.k_abs, sc.k_trk, sc.k_tot, sc.leak, sc.mig};
-          for (int k = 0; k < 6; k++)
-            if (v[k] != 0.0) unsafeAtomicAdd(ctxs[acc].scores + k, v[k]);
-          for (int k = 0; k < 8; k++)
-            if (s.c[k]) atomicAdd(ctxs[acc].counters + k, (unsigned long long)s.c[k]);
-          sc = Scores6{0, 0, 0, 0, 0, 0};
-          for (int k = 0; k < 8; k++) s.c[k] = 0;
-        }
-        acc = qq;
-      }
-      if (fresh) {
-        const int64_t li = gi - offsets[qq];
-        s.slot = li;
-        s.nsec = 0;
-        steps = 0;
-        load_part(P, K.in[li], p);
-        s.c[0]++;
-        if (!start_history(P, s, p, t, m)) {
-          if (K.daughters) K.daughters[li] = 0;
-        }
-        continue;
-      }
-      const bool converged = K.converged != 0;
-      int ev = 0;
-      bool hc = step_flight<TRK>(P, s, p, t, m, converged, sc, ev);
-      if (p.alive && hc) step_collision(P, s, p, m, K.kcol, converged, sc, ev);
-      bool live = step_finish<TRK>(P, s, p, t, m, ev);
-      if (live && ++steps > 100000000) {
-        s.error(-3, p.hid);
-        p.alive = 0;
-        live = false;
-      }
-      if (!live && K.daughters) K.daughters[s.slot] = p.daughters;
-    }
-  }
-  flush_grouped(ctxs, acc, sc, s.c);
-}
-
 // Work item of the block-affine batch kernel: histories [first, last) of
 // problem `item` (local history ids, read from ctxs[item].in).
 struct BatchChunk {
