@@ -69,6 +69,7 @@ def lib():
         L.abx_batch_step.argtypes = [P, C.POINTER(P), C.c_int, C.POINTER(C.c_int),
                                      C.POINTER(C.c_int)]
         L.abx_batch_kernel_seconds.argtypes = [P, C.POINTER(C.c_double)]
+        L.abx_batch_set_step_histories.argtypes = [P, C.c_int64]
         L.abx_problem_stats.argtypes = [P, C.POINTER(abi.RunStats)]
         _lib = L
     return _lib
@@ -219,11 +220,18 @@ class Batch:
     """Many fixed-source / modified-fixed-source problems advanced together
     (abx_batch_*).  `budget` caps the device bytes of attached problems."""
 
-    def __init__(self, budget=0):
+    def __init__(self, budget=0, step_histories=0):
         h = C.c_void_p()
         _check(lib().abx_batch_create(int(budget), C.byref(h)))
         self.h = h
         self.attached = {}
+        if step_histories:
+            self.set_step_histories(step_histories)
+
+    def set_step_histories(self, n):
+        """Cap the histories of one step over all problems (0 = no cap); a
+        fixed-source batch above its share is split over several steps."""
+        _check(lib().abx_batch_set_step_histories(self.h, C.c_int64(int(n))))
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -378,7 +378,8 @@ void upd(int gen, double x, double& avg, double& var) {
   if (gen > 1) var = var_old + ((x - avg_old) * (x - avg_old) / dgen) - (var_old / (dgen - 1.));
 }
 
-void record_generation(abx_problem* p, const abx_scores& s, double W) {
+void record_generation(abx_problem* p, const abx_scores& s, double W,
+                       hipStream_t st = nullptr) {
   p->gen++;
   upd(p->gen, s.k_col / W, p->kc_avg, p->kc_var);
   upd(p->gen, s.k_trk / W, p->kt_avg, p->kt_var);
@@ -388,17 +389,18 @@ void record_generation(abx_problem* p, const abx_scores& s, double W) {
   const int64_t nb = p->T.tally_bins;
   if (nb > 0) {
     hipLaunchKernelGGL(k_record_generation, dim3((unsigned)blocks_for(nb, 256)), dim3(256), 0,
-                       p->stream, p->tally_gen.as<double>(), p->tally_avg.as<double>(),
+                       st ? st : p->stream, p->tally_gen.as<double>(), p->tally_avg.as<double>(),
                        p->tally_var.as<double>(), nb, (double)p->tally_g, 0);
     HIPC(hipGetLastError());
   }
 }
 
-void clear_generation(abx_problem* p) {
+void clear_generation(abx_problem* p, hipStream_t st = nullptr) {
   p->gen_scores = abx_scores{};
   const int64_t nb = p->T.tally_bins;
   if (nb > 0) {
-    hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks_for(nb, 256)), dim3(256), 0, p->stream,
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks_for(nb, 256)), dim3(256), 0,
+                       st ? st : p->stream,
                        p->tally_gen.as<double>(), nb, 0.0);
     HIPC(hipGetLastError());
   }
@@ -809,20 +811,24 @@ static bool run_generations_graph(abx_problem* p, int ngenerations, int nignored
   const abx_tuning tu = p->tuning;
   const int bs = tu.block_size;
   const size_t cur = cursor_lds_bytes(bs, p->P.trk_D, p->P.trk_L);
-  const bool stage = tu.stage_tables && cur + (size_t)p->P.blob_bytes <= 65536;
-  const size_t lds = stage ? cur + (size_t)p->P.blob_bytes : cur;
   const int trk = T.st.tracking;
   const bool split = T.st.precision == ABX_SPLIT;
   const bool sec = needs_secondaries(T);
   if (sec) return false;  // k-eigenvalue without carter tracking only
+  bool stage = false;
   int64_t grid = 0;
   HIPC(dispatch_variant(trk, false, false, split, [&](auto TT, auto, auto, auto SP) {
-    grid = stage ? resident_blocks(k_history<decltype(TT)::value, false, false, decltype(SP)::value, true>,
-                                   bs, lds)
-                 : resident_blocks(k_history<decltype(TT)::value, false, false, decltype(SP)::value, false>,
-                                   bs, lds);
+    constexpr int TK = decltype(TT)::value;
+    constexpr bool SPL = decltype(SP)::value;
+    stage = tu.stage_tables && stage_pays(k_history<TK, false, false, SPL, true>,
+                                          k_history<TK, false, false, SPL, false>, bs, cur,
+                                          (size_t)p->P.blob_bytes);
+    grid = stage ? resident_blocks(k_history<TK, false, false, SPL, true>, bs,
+                                   cur + (size_t)p->P.blob_bytes)
+                 : resident_blocks(k_history<TK, false, false, SPL, false>, bs, cur);
     return hipSuccess;
   }));
+  const size_t lds = stage ? cur + (size_t)p->P.blob_bytes : cur;
   if (tu.blocks_per_cu > 0) {
     int dev = 0, cus = 0;
     HIPC(hipGetDevice(&dev));
@@ -1202,8 +1208,49 @@ struct BatchEntry {
   int batches_left;
   bool mid_chain;     // a modified-fixed-source generation chain is running
   int64_t n;          // histories of the current call
-  abx_scores acc{};   // scores of the current batch (chain summed)
+  abx_scores acc{};   // scores of the current batch (chain or slices summed)
+  int64_t done = 0;   // histories of the current fixed-source batch already run
 };
+
+// A fixed-source batch may be split over several steps (its histories are
+// independent and its tallies accumulate until the batch is recorded); a
+// modified-fixed-source generation chain runs one whole generation per step.
+bool sliceable(const abx_problem* p) { return p->T.st.mode == ABX_FIXED_SOURCE; }
+
+// Histories each entry runs this step.  With a step budget, entries whose
+// work cannot be split take what they need and the sliceable ones share the
+// rest by water-filling (smallest remainder first), each getting at least
+// `floor` histories so that every problem advances.
+std::vector<int64_t> step_allocation(const std::vector<BatchEntry>& items, int64_t budget) {
+  const int ni = (int)items.size();
+  const int64_t floor = 8192;
+  std::vector<int64_t> give(ni, 0), idx;
+  int64_t left = budget;
+  for (int j = 0; j < ni; j++) {
+    const BatchEntry& it = items[j];
+    const int64_t N = it.p->T.st.nparticles;
+    if (it.mid_chain) {
+      give[j] = it.n;
+    } else if (!sliceable(it.p)) {
+      give[j] = N;
+    } else {
+      give[j] = N - it.done;
+      idx.push_back(j);
+      continue;
+    }
+    left -= give[j];
+  }
+  if (budget <= 0) return give;
+  std::stable_sort(idx.begin(), idx.end(), [&](int64_t a, int64_t c) { return give[a] < give[c]; });
+  int64_t k = (int64_t)idx.size();
+  for (int64_t j : idx) {
+    const int64_t share = std::max<int64_t>(floor, left / std::max<int64_t>(1, k));
+    give[j] = std::min(give[j], share);
+    left -= give[j];
+    k--;
+  }
+  return give;
+}
 
 // Per-entry device scalars in the batch-wide block.
 struct BatchScalars {
@@ -1215,10 +1262,19 @@ struct BatchScalars {
   int32_t pad_;
 };
 
+// Launch group of a problem in the batch kernel: tracking mode, secondaries,
+// precision, and whether its tables fit in LDS next to 256 cursors (problems
+// whose tables fit are launched apart from those whose tables do not, so
+// that the former run the staged variant).
 int variant_key(const abx_problem* p) {
   const HostTables& T = p->T;
   const int sec = needs_secondaries(T) ? 1 : 0;
-  return T.st.tracking * 4 + sec * 2 + (T.st.precision == ABX_SPLIT ? 1 : 0);
+  const int split = T.st.precision == ABX_SPLIT ? 1 : 0;
+  const int fits = cursor_lds_bytes(256, p->P.trk_D, p->P.trk_L) + (size_t)p->P.blob_bytes <=
+                           lds_block_limit()
+                       ? 1
+                       : 0;
+  return ((T.st.tracking * 2 + sec) * 2 + split) * 2 + fits;
 }
 
 // Chunks of `chunk` histories over the problems of one launch (item indices
@@ -1261,15 +1317,18 @@ hipError_t launch_chunks(const DevProblem* probs, const KCtx* ctxs, const BatchC
 hipError_t launch_chunks_variant(int key, const DevProblem* probs, const KCtx* ctxs,
                                  const BatchChunk* chunks, int n_chunks, int D, int L,
                                  size_t blob_max, unsigned* next, hipStream_t s) {
-  const int trk = key / 4;
-  const bool sec = (key / 2) & 1, split = key & 1;
-  const bool stage = cursor_lds_bytes(256, D, L) + blob_max <= 65536;
+  const int trk = key / 8;
+  const bool sec = (key / 4) & 1, split = (key / 2) & 1, fits = key & 1;
+  const size_t cur = cursor_lds_bytes(256, D, L);
   return dispatch_variant(trk, false, sec || trk == 2, split, [&](auto T, auto, auto SE, auto SP) {
-    if (stage)
-      return launch_chunks<decltype(T)::value, decltype(SE)::value, decltype(SP)::value, true>(
-          probs, ctxs, chunks, n_chunks, D, L, blob_max, next, s);
-    return launch_chunks<decltype(T)::value, decltype(SE)::value, decltype(SP)::value, false>(
-        probs, ctxs, chunks, n_chunks, D, L, blob_max, next, s);
+    constexpr int TK = decltype(T)::value;
+    constexpr bool SEC = decltype(SE)::value, SPL = decltype(SP)::value;
+    if (fits && stage_pays(k_batch_chunks<TK, SEC, SPL, true>, k_batch_chunks<TK, SEC, SPL, false>,
+                           256, cur, blob_max))
+      return launch_chunks<TK, SEC, SPL, true>(probs, ctxs, chunks, n_chunks, D, L, blob_max,
+                                               next, s);
+    return launch_chunks<TK, SEC, SPL, false>(probs, ctxs, chunks, n_chunks, D, L, blob_max, next,
+                                              s);
   });
 }
 
@@ -1323,13 +1382,25 @@ struct abx_batch {
   std::vector<BatchScalars> h_scal;
   size_t budget = 0;  // device bytes admitted problems may hold (0 = no limit)
   size_t used = 0;
+  int64_t step_hist = 0;  // histories per step over all problems (0 = no limit)
   double kernel_ms = 0.0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  // Launch groups (kernel variants) run on these streams side by side, so
+  // that one group's tail (its last, longest chunks) overlaps the next
+  // group's work.
+  static constexpr int NS = 4;
+  hipStream_t aux[NS] = {};
+  hipEvent_t ready = nullptr, done[NS] = {};
   ~abx_batch() {
     DBuf* all[] = {&probs, &ctxs, &offsets, &scal, &next, &sec, &chunks};
     for (DBuf* b : all) b->release();
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
+    if (ready) (void)hipEventDestroy(ready);
+    for (int k = 0; k < NS; k++) {
+      if (done[k]) (void)hipEventDestroy(done[k]);
+      if (aux[k]) (void)hipStreamDestroy(aux[k]);
+    }
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -1342,20 +1413,22 @@ void batch_step(abx_batch* b, std::vector<abx_problem*>& finished) {
   b->scal.reserve((size_t)ni * sizeof(BatchScalars));
   HIPC(hipMemsetAsync(b->scal.p, 0, (size_t)ni * sizeof(BatchScalars), b->stream));
   BatchScalars* ds = b->scal.as<BatchScalars>();
-  // 1. inputs: a new batch of sources, or the chain's re-banked generation
+  // 1. inputs: the next sources of a batch (all of it, or this step's slice
+  // of a fixed-source batch), or the chain's re-banked generation
+  const std::vector<int64_t> give = step_allocation(b->items, b->step_hist);
   for (int j = 0; j < ni; j++) {
     BatchEntry& it = b->items[j];
     abx_problem* p = it.p;
     if (it.mid_chain) continue;
-    const int64_t N = p->T.st.nparticles;
-    p->bank_a.reserve((size_t)N * sizeof(abx_particle));
-    hipLaunchKernelGGL(k_sample_sources, dim3((unsigned)blocks_for(N, 256)), dim3(256),
+    const int64_t n = give[j];
+    p->bank_a.reserve((size_t)std::max<int64_t>(1, n) * sizeof(abx_particle));
+    hipLaunchKernelGGL(k_sample_sources, dim3((unsigned)blocks_for(n, 256)), dim3(256),
                        (size_t)256 * trk_bytes(p->P.trk_D, p->P.trk_L), b->stream, p->P, p->S,
-                       p->run_hist, N, p->bank_a.as<abx_particle>(), &ds[j].err, &ds[j].err_hist);
+                       p->run_hist, n, p->bank_a.as<abx_particle>(), &ds[j].err, &ds[j].err_hist);
     HIPC(hipGetLastError());
-    p->run_hist += (uint64_t)N;
-    it.n = N;
-    it.acc = abx_scores{};
+    p->run_hist += (uint64_t)n;
+    if (it.done == 0) it.acc = abx_scores{};
+    it.n = n;
   }
   // 2. one launch per kernel variant
   std::vector<int> order(ni);
@@ -1369,15 +1442,22 @@ void batch_step(abx_batch* b, std::vector<abx_problem*>& finished) {
   HIPC(hipGetDevice(&dev));
   HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   const size_t lanes = (size_t)cus * 8 * 256;  // upper bound on resident lanes
-  if (max_sec) b->sec.reserve(lanes * max_sec * sizeof(SecRec));
+  // one secondary stack per stream slot: groups on different streams run
+  // at the same time and index their stacks by lane
+  const int NS = abx_batch::NS;
+  if (max_sec) b->sec.reserve((size_t)NS * lanes * max_sec * sizeof(SecRec));
   std::vector<DevProblem> hp(ni);
   std::vector<KCtx> hk(ni);
   b->probs.reserve((size_t)ni * sizeof(DevProblem));
   b->ctxs.reserve((size_t)ni * sizeof(KCtx));
-  b->next.reserve(8 * sizeof(unsigned long long));
   if (!b->e0) {
     HIPC(hipEventCreate(&b->e0));
     HIPC(hipEventCreate(&b->e1));
+    HIPC(hipEventCreateWithFlags(&b->ready, hipEventDisableTiming));
+    for (int k = 0; k < NS; k++) {
+      HIPC(hipStreamCreateWithFlags(&b->aux[k], hipStreamNonBlocking));
+      HIPC(hipEventCreateWithFlags(&b->done[k], hipEventDisableTiming));
+    }
   }
   // chunks of every variant group, uploaded once
   struct Group {
@@ -1402,8 +1482,9 @@ void batch_step(abx_batch* b, std::vector<abx_problem*>& finished) {
       abx_problem* p = it.p;
       const CallScalars cs{ds[j].sc, ds[j].cn, &ds[j].err, &ds[j].err_hist, &ds[j].n_stage};
       const int sec_cap = needs_secondaries(p->T) ? 16 : 0;
-      hk[g1] = make_kctx(p, p->bank_a.as<abx_particle>(), it.n, p->run_kcol, 1, cs,
-                         b->sec.as<SecRec>(), sec_cap);
+      SecRec* sec = max_sec ? b->sec.as<SecRec>() + (groups.size() % NS) * lanes * max_sec
+                            : nullptr;
+      hk[g1] = make_kctx(p, p->bank_a.as<abx_particle>(), it.n, p->run_kcol, 1, cs, sec, sec_cap);
       hp[g1] = p->P;
       n.push_back(it.n);
       gr.D = std::max(gr.D, p->P.trk_D);
@@ -1425,17 +1506,24 @@ void batch_step(abx_batch* b, std::vector<abx_problem*>& finished) {
   if (!all_chunks.empty())
     HIPC(hipMemcpyAsync(b->chunks.p, all_chunks.data(), all_chunks.size() * sizeof(BatchChunk),
                         hipMemcpyHostToDevice, b->stream));
+  // one chunk counter per group (groups run concurrently)
+  b->next.reserve(std::max<size_t>(8, groups.size()) * sizeof(unsigned long long));
   HIPC(hipEventRecord(b->e0, b->stream));
-  int launches = 0;
-  for (const Group& gr : groups) {
+  HIPC(hipEventRecord(b->ready, b->stream));
+  const int used = (int)std::min<size_t>(NS, groups.size());
+  for (int k = 0; k < used; k++) HIPC(hipStreamWaitEvent(b->aux[k], b->ready, 0));
+  for (size_t gi = 0; gi < groups.size(); gi++) {
+    const Group& gr = groups[gi];
     if (gr.nc > 0)
       HIPC(launch_chunks_variant(gr.key, b->probs.as<DevProblem>() + gr.g0,
                                  b->ctxs.as<KCtx>() + gr.g0, b->chunks.as<BatchChunk>() + gr.c0,
                                  (int)gr.nc, gr.D, gr.L, gr.blob_max,
-                                 reinterpret_cast<unsigned*>(b->next.as<unsigned long long>() +
-                                                             (launches % 8)),
-                                 b->stream));
-    launches++;
+                                 reinterpret_cast<unsigned*>(b->next.as<unsigned long long>() + gi),
+                                 b->aux[gi % NS]));
+  }
+  for (int k = 0; k < used; k++) {
+    HIPC(hipEventRecord(b->done[k], b->aux[k]));
+    HIPC(hipStreamWaitEvent(b->stream, b->done[k], 0));
   }
   // the host arrays above are read by the copies before they go out of scope
   HIPC(hipStreamSynchronize(b->stream));
@@ -1473,13 +1561,18 @@ void batch_step(abx_batch* b, std::vector<abx_problem*>& finished) {
       p->bank_n = (size_t)n_sites;
       it.n = n_sites;
       it.mid_chain = true;
+    } else if (sliceable(p) && (it.done += it.n) < T.st.nparticles) {
+      // more slices of this batch follow
     } else {
       it.mid_chain = false;
-      record_generation(p, it.acc, (double)T.st.nparticles);
-      clear_generation(p);
+      it.done = 0;
+      record_generation(p, it.acc, (double)T.st.nparticles, b->stream);
+      clear_generation(p, b->stream);
       it.batches_left--;
     }
-    HIPC(hipStreamSynchronize(p->stream));
+    // fission-bank ordering, cancellation and re-banking ran on the
+    // problem's own stream; the next step's launches must see them
+    if (!sliceable(p)) HIPC(hipStreamSynchronize(p->stream));
     if (it.batches_left > 0) {
       keep.push_back(it);
     } else {
@@ -1488,6 +1581,9 @@ void batch_step(abx_batch* b, std::vector<abx_problem*>& finished) {
     }
   }
   b->items.swap(keep);
+  // the records above ran on the batch stream; finished problems are read
+  // on their own streams
+  HIPC(hipStreamSynchronize(b->stream));
 }
 
 }  // namespace
@@ -1527,7 +1623,7 @@ int abx_batch_add(abx_batch* b, abx_problem* p, int nbatches) {
     p->run_diag = abx_diag{};
     p->run_histories = 0;
     b->used += need;
-    b->items.push_back(BatchEntry{p, nbatches, false, 0, abx_scores{}});
+    b->items.push_back(BatchEntry{p, nbatches, false, 0, abx_scores{}, 0});
   });
 }
 
@@ -1541,6 +1637,13 @@ int abx_batch_step(abx_batch* b, abx_problem** finished, int cap, int* n_finishe
     if ((int)done.size() > cap && finished) fail(ABX_E_CAPACITY, "finished list too small");
     for (size_t k = 0; k < done.size() && finished; k++) finished[k] = done[k];
     if (n_active) *n_active = (int)b->items.size();
+  });
+}
+
+int abx_batch_set_step_histories(abx_batch* b, int64_t histories) {
+  ABX_TRY({
+    if (!b || histories < 0) fail(ABX_E_INVALID, "step histories");
+    b->step_hist = histories;
   });
 }
 

@@ -466,14 +466,42 @@ static int64_t resident_blocks(Kern kernel, int bs, size_t lds) {
   return (int64_t)cus * per_cu;
 }
 
+// LDS one block may allocate on the current device (160 KB on gfx950).
+static size_t lds_block_limit() {
+  static size_t cached[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 65536;
+  if (!cached[dev]) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+        v <= 0)
+      v = 65536;
+    cached[dev] = (size_t)v;
+  }
+  return cached[dev];
+}
+
+// Whether a kernel's LDS-staged variant is worth launching: the tables fit
+// next to the cursors, and the extra LDS costs no resident blocks (the
+// staged variant gains ~10 % from LDS reads, losing a block per CU costs
+// more).
+template <class KS, class KU>
+static bool stage_pays(KS staged, KU unstaged, int bs, size_t cur, size_t blob) {
+  if (cur + blob > lds_block_limit()) return false;
+  return resident_blocks(staged, bs, cur + blob) >= resident_blocks(unstaged, bs, cur);
+}
+
 // `next` is a device counter (zeroed here) from which lanes take histories.
 template <int TRK, bool TRACE, bool SEC, bool SPLIT>
 static hipError_t history_transport_t(const DevProblem& P, const KCtx& K, hipStream_t s,
                                       unsigned long long* next, const abx_tuning& tu) {
   const int bs = tu.block_size;
-  // Stage the tables in LDS when they fit next to the cursors (64 KB per block).
+  // Stage the tables in LDS when they fit next to the cursors.
   const size_t cur = cursor_lds_bytes(bs, P.trk_D, P.trk_L);
-  const bool stage = tu.stage_tables && cur + (size_t)P.blob_bytes <= 65536;
+  const bool stage = tu.stage_tables &&
+                     stage_pays(k_history<TRK, TRACE, SEC, SPLIT, true>,
+                                k_history<TRK, TRACE, SEC, SPLIT, false>, bs, cur,
+                                (size_t)P.blob_bytes);
   const size_t lds = stage ? cur + (size_t)P.blob_bytes : cur;
   hipError_t e;
   if ((e = hipMemsetAsync(next, 0, sizeof(unsigned long long), s)) != hipSuccess) return e;

@@ -38,13 +38,18 @@ class Request:
 
 
 class Server:
-    """`budget` caps the device bytes of admitted problems (0 = no cap)."""
+    """`budget` caps the device bytes of admitted problems (0 = no cap).
+    `step_histories` caps the histories of one step over all problems (0 =
+    no cap): larger fixed-source batches are then split over several steps,
+    so that a request arriving while a large one runs starts within one
+    short step instead of after the large batch."""
 
-    def __init__(self, budget=0, clock=time.perf_counter, keep_tallies=True):
-        self.batch = Batch(budget)
+    def __init__(self, budget=0, clock=time.perf_counter, keep_tallies=True, step_histories=0):
+        self.batch = Batch(budget, step_histories)
         self.clock = clock
         self.keep_tallies = keep_tallies
         self.steps = 0
+        self.step_seconds = 0.0     # wall time inside batch steps
 
     def run(self, requests):
         """Serve `requests` (any order; arrival times are honoured) and return
@@ -69,7 +74,10 @@ class Server:
                 running[r._problem.h.value] = r
                 waiting.popleft()
             if running:
-                for p in self.batch.step():
+                ts = self.clock()
+                done = self.batch.step()
+                self.step_seconds += self.clock() - ts
+                for p in done:
                     r = running.pop(p.h.value)
                     r.stats = p.stats()
                     if self.keep_tallies:

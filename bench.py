@@ -129,8 +129,14 @@ def bench_precision(args):
     from abeille_amd import Problem, abi, synchronize
     nparticles = int(args.particles)
     res = {}
-    for prec in ("fp64", "split"):
-        m, desc = workload(args.workload, nparticles, args.tracking, prec)
+    # "fp64_seed2" is FP64 with another RNG seed: its z against FP64 is the
+    # spread the statistics alone give (generation-to-generation correlation
+    # of k-eigenvalue tallies makes it wider than 1), the yardstick for the
+    # split run's z
+    for prec in ("fp64", "split", "fp64_seed2"):
+        m, desc = workload(args.workload, nparticles, args.tracking, prec.split("_")[0])
+        if prec == "fp64_seed2":
+            m.set(seed=m.settings["seed"] + 7919)
         p = Problem(m)
         p.run(max(1, args.warmup), nignored=max(1, args.warmup))
         synchronize()
@@ -142,23 +148,29 @@ def bench_precision(args):
                          k=(st.kcol_avg, st.kcol_err),
                          tallies=[p.tally_stats(t) for t in range(len(m.tallies))])
         del p
-    dev = []
-    for (a64, e64), (a32, e32) in zip(res["fp64"]["tallies"], res["split"]["tallies"]):
-        sig = np.hypot(e64, e32)
-        ok = sig > 0
-        z = (a32[ok] - a64[ok]) / sig[ok]
-        good = (a64 > 0) & (e64 < 0.01 * a64)
-        rel = np.abs(a32[good] - a64[good]) / a64[good] if good.any() else np.array([np.nan])
-        dev.append({"bins": int(ok.sum()), "z_mean": float(z.mean()), "z_std": float(z.std()),
-                    "frac_within_3sigma": float((np.abs(z) <= 3).mean()),
-                    "bins_below_1pct_err": int(good.sum()),
-                    "mean_rel_dev_those_bins": float(rel.mean())})
+    def deviation(other):
+        dev = []
+        for (a64, e64), (a32, e32) in zip(res["fp64"]["tallies"], res[other]["tallies"]):
+            sig = np.hypot(e64, e32)
+            ok = sig > 0
+            z = (a32[ok] - a64[ok]) / sig[ok]
+            good = (a64 > 0) & (e64 < 0.01 * a64)
+            rel = np.abs(a32[good] - a64[good]) / a64[good] if good.any() else np.array([np.nan])
+            dev.append({"bins": int(ok.sum()), "z_mean": float(z.mean()),
+                        "z_std": float(z.std()),
+                        "frac_within_3sigma": float((np.abs(z) <= 3).mean()),
+                        "bins_below_1pct_err": int(good.sum()),
+                        "mean_rel_dev_those_bins": float(rel.mean())})
+        return dev
+    dev = deviation("split")
     out = _common(args, res["split"]["rate"], "collisions/s", res["split"]["ms"], "f64/f32 split",
                   {"workload": desc + ", FP64 vs split precision", "nparticles": nparticles},
                   {"fp64_collisions_per_s": res["fp64"]["rate"],
                    "split_vs_fp64": res["split"]["rate"] / res["fp64"]["rate"],
                    "k_fp64": res["fp64"]["k"], "k_split": res["split"]["k"],
-                   "tally_deviation": dev})
+                   "k_fp64_seed2": res["fp64_seed2"]["k"],
+                   "tally_deviation": dev,
+                   "fp64_other_seed_deviation": deviation("fp64_seed2")})
     print(json.dumps(out), flush=True)
 
 
@@ -234,7 +246,7 @@ def bench_serve(args):
         kinds.append(kind)
     # warm-up
     Server(keep_tallies=False).run([Request(reqs[0].model, 1, 0.0)])
-    srv = Server(budget=64 << 30, keep_tallies=False)
+    srv = Server(budget=64 << 30, keep_tallies=False, step_histories=args.step_histories)
     t0 = time.perf_counter()
     srv.run(reqs)
     wall = time.perf_counter() - t0
@@ -269,12 +281,15 @@ def bench_serve(args):
                   {"workload": f"{args.requests} slab requests, Poisson {args.rate}/s, 1e4-5e6 "
                                "histories, 20% carter (0.8 x majorant) + 10^3 approximate "
                                "cancellation, continuous batching, 64 GiB budget, split precision",
-                   "requests": args.requests},
+                   "requests": args.requests, "step_histories": args.step_histories},
                   {"histories_per_s": hist / wall, "wall_s": wall, "arrival_span_s": span,
                    "latency_s": pct, "steps": srv.steps,
+                   "step_wall_s": srv.step_seconds,
+                   "transport_kernel_s": srv.batch.kernel_seconds(),
                    "carter_requests": int(sum(k == "carter" for k in kinds)),
                    "fixed_source_requests": {
                        "served_latency_s": pct_fixed, "served_collisions": coll_fixed,
+                       "served_makespan_s": float(max(r.finished for r in served_fixed)),
                        "plain_fp64_one_at_a_time": {
                            "collisions_per_s": pc / t,
                            "saturated_collisions_per_s": pc / float(service.sum()),
@@ -297,6 +312,8 @@ def main():
     ap.add_argument("--problems", type=int, default=512, help="batch: number of problems")
     ap.add_argument("--requests", type=int, default=100, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=50.0, help="serve: arrivals per second")
+    ap.add_argument("--step-histories", type=int, default=0,
+                    help="serve: histories per step over all requests (0 = whole batches)")
     ap.add_argument("--particles", type=float, default=1e6)
     ap.add_argument("--tracking", default="delta", choices=["delta", "surface"])
     ap.add_argument("--engine", default="history", choices=["event", "history"])

@@ -457,6 +457,13 @@ int abx_batch_add(abx_batch* b, abx_problem* p, int nbatches);
  * number still attached. */
 int abx_batch_step(abx_batch* b, abx_problem** finished, int cap, int* n_finished,
                    int* n_active);
+/* Caps the histories one step runs over all problems (0 = no cap, the
+ * default).  A fixed-source batch larger than its share is then split over
+ * several steps - its histories, RNG streams and recorded statistics are
+ * unchanged - so that short steps let newly attached problems start soon.
+ * Every problem still gets at least 8192 histories per step, and a
+ * modified-fixed-source generation always runs whole. */
+int abx_batch_set_step_histories(abx_batch* b, int64_t histories);
 /* GPU time of the batch's transport launches so far. */
 int abx_batch_kernel_seconds(abx_batch* b, double* seconds);
 /* Statistics of a problem's batch-engine run (or of its last abx_run). */

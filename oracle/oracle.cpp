@@ -46,7 +46,6 @@ constexpr double BOUNDRY_TOL = 500. * SURFACE_COINCIDENT;
 static thread_local std::string g_err;
 static bool g_portable = true;
 
-static double LOG(double x) { return g_portable ? dm_log(x) : std::log(x); }
 static double SIN(double x) { return g_portable ? dm_sin(x) : std::sin(x); }
 static double COS(double x) { return g_portable ? dm_cos(x) : std::cos(x); }
 

@@ -79,3 +79,42 @@ def test_batch_rejects_k_eigenvalue():
     b = Batch()
     with pytest.raises(Exception):
         b.add(Problem(problems.la13511("PUa-1-0-SL", 1000)[0]), 1)
+
+
+def test_sliced_batches_equal_alone():
+    """With a step cap, fixed-source batches larger than their share run over
+    several steps; results are those of the whole batch run alone.  The
+    modified-fixed-source problems in the mix run whole generations."""
+    rng = np.random.default_rng(77)
+    cases = [("cube_dt", problems.fixed_source_cube(30000, "delta"), 2),
+             ("cube_st", problems.fixed_source_cube(20000, "surface"), 2),
+             ("small", problems.fixed_source_cube(3000, "delta"), 3),
+             ("mfs_carter_cancel", problems.subcritical_slab(2000, tracking="carter"), 2)]
+    m, _ = problems.slab_problem(rng, nhist=25000)
+    m.set(precision=abi.SPLIT)
+    cases.append(("slab_split", m, 2))
+    b = Batch(step_histories=16384)
+    probs = {}
+    for name, m, nb in cases:
+        probs[name] = Problem(m)
+        assert b.add(probs[name], nb)
+    steps = 0
+    while len(b):
+        b.step()
+        steps += 1
+        assert steps < 1000
+    # the 30000-history batches need at least 4 slices of at most 8192
+    assert steps >= 8
+    for name, m, nb in cases:
+        print(f"[sliced] {name}", flush=True)
+        sa, ta = _alone(m, nb)
+        sb = probs[name].stats()
+        assert sb.active_generations == sa.active_generations == nb
+        assert sb.diag.collisions == sa.diag.collisions, name
+        assert sb.histories == sa.histories, name
+        for f in ("kcol_avg", "leak_avg", "mig_avg"):
+            assert getattr(sb, f) == pytest.approx(getattr(sa, f), rel=1e-9, abs=1e-14), (name, f)
+        for t, (aa, ea) in enumerate(ta):
+            ab, eb = probs[name].tally_stats(t)
+            np.testing.assert_allclose(ab, aa, rtol=1e-8, atol=1e-14, err_msg=name)
+            np.testing.assert_allclose(eb, ea, rtol=1e-6, atol=1e-12, err_msg=name)
