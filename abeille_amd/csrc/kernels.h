@@ -199,6 +199,10 @@ __device__ __forceinline__ void bind_lds_cursor(Trk& t, const DevProblem& P) {
 // history of their wavefront.  Per-history results do not depend on which
 // lane runs a history: the RNG stream is the history's own and fission
 // sites are ordered afterwards by (parent, daughter).
+#ifndef ABX_WAVE_GRAB
+#define ABX_WAVE_GRAB 64ull
+#endif
+
 // LDS bytes of the cursors of a block (the staged tables follow them).
 __host__ __device__ inline size_t cursor_lds_bytes(int bs, int D, int L) {
   return ((size_t)bs * trk_bytes(D, L) + 15) & ~(size_t)15;
@@ -237,16 +241,33 @@ __global__ void ABX_TKERNEL k_history(DevProblem P0, KCtx K, unsigned long long*
   int64_t steps = 0;
   bool done = false;
   const bool converged = K.converged != 0;
+  // The wave's private range [wb, we) of history ids: the global counter is
+  // touched once per ABX_WAVE_GRAB histories, not at every refill (a
+  // counter shared by every wave of the grid is a serialising hot spot).
+  unsigned long long wb = 0, we = 0;
   for (;;) {
     const bool need = !p.alive && !done;
     const unsigned long long mask = __ballot(need);
     if (mask) {
-      const int leader = __ffsll((long long)mask) - 1;
-      unsigned long long base = 0;
-      if (lane == leader) base = atomicAdd(next, (unsigned long long)__popcll(mask));
-      base = __shfl(base, leader, 64);
+      const unsigned long long cnt = (unsigned long long)__popcll(mask);
+      const unsigned long long rank = (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull));
+      const unsigned long long avail = we - wb;
+      unsigned long long i64;
+      if (cnt <= avail) {
+        i64 = wb + rank;
+        wb += cnt;
+      } else {
+        const unsigned long long grab = cnt - avail > ABX_WAVE_GRAB ? cnt - avail : ABX_WAVE_GRAB;
+        const int leader = __ffsll((long long)mask) - 1;
+        unsigned long long nb = 0;
+        if (lane == leader) nb = atomicAdd(next, grab);
+        nb = __shfl(nb, leader, 64);
+        i64 = rank < avail ? wb + rank : nb + (rank - avail);
+        wb = nb + (cnt - avail);
+        we = nb + grab;
+      }
       if (need) {
-        const int64_t i = (int64_t)(base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull)));
+        const int64_t i = (int64_t)i64;
         if (i < n_hist) {
           s.slot = i;
           s.nsec = 0;
