@@ -201,6 +201,7 @@ struct CallScalars {
   int32_t* err;
   unsigned long long* err_hist;
   unsigned long long* n_stage;
+  unsigned long long* n_hole;
 };
 
 // Grows the problem's transport scratch for a call over n histories and
@@ -227,6 +228,7 @@ KCtx make_kctx(abx_problem* p, const abx_particle* d_in, int64_t n, double kcol,
   K.stage_parent = banks ? p->stage_parent.as<uint32_t>() : nullptr;
   K.stage_daughter = banks ? p->stage_daughter.as<uint32_t>() : nullptr;
   K.n_stage = cs.n_stage;
+  K.n_hole = cs.n_hole;
   K.stage_cap = stage_cap;
   K.daughters = banks ? p->daughters.as<uint32_t>() : nullptr;
   K.scores = cs.scores;
@@ -258,19 +260,21 @@ TransportOut collect_scalars(const double* sc, const unsigned long long* cn, int
 }
 
 // Orders the staged fission sites of a call over n parents into p->ordered
-// (parent order, then push order); returns their number.
-int64_t order_fission_bank(abx_problem* p, int64_t n, uint64_t n_staged) {
+// (parent order, then push order); returns their number.  n_staged staging
+// slots were handed out, n_holes of them hold no site.
+int64_t order_fission_bank(abx_problem* p, int64_t n, uint64_t n_staged, uint64_t n_holes) {
   const int64_t stage_cap = 4 * n + 4096;
   if ((int64_t)n_staged > stage_cap) fail(ABX_E_CAPACITY, "fission staging overflow");
   p->offsets.reserve((size_t)std::max<int64_t>(1, n) * sizeof(uint64_t));
   const uint64_t total = device_scan(p, p->daughters.as<uint32_t>(), p->offsets.as<uint64_t>(), n);
-  if (total != n_staged) fail(ABX_E_INVALID, "fission site count mismatch");
+  if (n_holes > n_staged || total != n_staged - n_holes)
+    fail(ABX_E_INVALID, "fission site count mismatch");
   p->ordered.reserve((size_t)std::max<uint64_t>(1, total) * sizeof(abx_banked));
   if (total) {
-    hipLaunchKernelGGL(k_order_bank, dim3((unsigned)blocks_for((int64_t)total, 256)), dim3(256), 0,
-                       p->stream, p->stage.as<abx_banked>(), p->stage_parent.as<uint32_t>(),
-                       p->stage_daughter.as<uint32_t>(), p->offsets.as<uint64_t>(), (int64_t)total,
-                       p->ordered.as<abx_banked>());
+    hipLaunchKernelGGL(k_order_bank, dim3((unsigned)blocks_for((int64_t)n_staged, 256)), dim3(256),
+                       0, p->stream, p->stage.as<abx_banked>(), p->stage_parent.as<uint32_t>(),
+                       p->stage_daughter.as<uint32_t>(), p->offsets.as<uint64_t>(),
+                       (int64_t)n_staged, p->ordered.as<abx_banked>());
     HIPC(hipGetLastError());
   }
   return (int64_t)total;
@@ -299,11 +303,11 @@ TransportOut transport_device(abx_problem* p, const abx_particle* d_in, int64_t 
   p->counters.reserve(8 * sizeof(unsigned long long));
   p->err.reserve(sizeof(int32_t));
   p->err_hist.reserve(sizeof(unsigned long long));
-  p->n_stage.reserve(sizeof(unsigned long long));
+  p->n_stage.reserve(2 * sizeof(unsigned long long));  // slots handed out, holes
   HIPC(hipMemsetAsync(p->scores.p, 0, 8 * sizeof(double), p->stream));
   HIPC(hipMemsetAsync(p->counters.p, 0, 8 * sizeof(unsigned long long), p->stream));
   HIPC(hipMemsetAsync(p->err.p, 0, sizeof(int32_t), p->stream));
-  HIPC(hipMemsetAsync(p->n_stage.p, 0, sizeof(unsigned long long), p->stream));
+  HIPC(hipMemsetAsync(p->n_stage.p, 0, 2 * sizeof(unsigned long long), p->stream));
   const bool banks = T.st.mode != ABX_FIXED_SOURCE;
   const int sec_cap = needs_secondaries(T) ? 16 : 0;
   // The stack is indexed by persistent lane (history engine) or by history
@@ -313,7 +317,8 @@ TransportOut transport_device(abx_problem* p, const abx_particle* d_in, int64_t 
   if (sec_cap) p->sec.reserve((size_t)(std::max<int64_t>(1, n) + 256) * sec_cap * sizeof(SecRec));
   const CallScalars cs{p->scores.as<double>(), p->counters.as<unsigned long long>(),
                        p->err.as<int32_t>(), p->err_hist.as<unsigned long long>(),
-                       p->n_stage.as<unsigned long long>()};
+                       p->n_stage.as<unsigned long long>(),
+                       p->n_stage.as<unsigned long long>() + 1};
   KCtx K = make_kctx(p, d_in, n, kcol, converged, cs, p->sec.as<SecRec>(), sec_cap);
   K.trace = d_trace;
   K.trace_cap = trace_cap;
@@ -358,15 +363,16 @@ TransportOut transport_device(abx_problem* p, const abx_particle* d_in, int64_t 
   double sc[8];
   unsigned long long cn[8];
   int32_t err = 0;
-  unsigned long long err_hist = 0, nst = 0;
+  unsigned long long err_hist = 0, nst[2] = {0, 0};
   HIPC(hipMemcpyAsync(sc, p->scores.p, 8 * sizeof(double), hipMemcpyDeviceToHost, p->stream));
   HIPC(hipMemcpyAsync(cn, p->counters.p, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, p->stream));
   HIPC(hipMemcpyAsync(&err, p->err.p, sizeof(int32_t), hipMemcpyDeviceToHost, p->stream));
   HIPC(hipMemcpyAsync(&err_hist, p->err_hist.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, p->stream));
-  HIPC(hipMemcpyAsync(&nst, p->n_stage.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, p->stream));
+  HIPC(hipMemcpyAsync(nst, p->n_stage.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                      p->stream));
   HIPC(hipStreamSynchronize(p->stream));
   TransportOut res = collect_scalars(sc, cn, err, err_hist);
-  if (banks) res.n_sites = order_fission_bank(p, n, nst);
+  if (banks) res.n_sites = order_fission_bank(p, n, nst[0], nst[1]);
   add_scores(p->gen_scores, res.scores);
   return res;
 }
@@ -801,7 +807,7 @@ static bool run_generations_graph(abx_problem* p, int ngenerations, int nignored
   HIPC(hipMemcpyAsync(g, &h, sizeof(GenDev), hipMemcpyHostToDevice, s));
   HIPC(hipStreamSynchronize(s));
   // transport context: counts and scalars live in GenDev
-  const CallScalars cs{g->scores, g->counters, &g->err, &g->err_hist, &g->n_stage};
+  const CallScalars cs{g->scores, g->counters, &g->err, &g->err_hist, &g->n_stage, &g->n_hole};
   KCtx K = make_kctx(p, p->bank_a.as<abx_particle>(), N, 1.0, 1, cs, nullptr, 0);
   p->daughters.reserve(cap * sizeof(uint32_t));
   K.daughters = p->daughters.as<uint32_t>();
@@ -1258,6 +1264,7 @@ struct BatchScalars {
   unsigned long long cn[8];
   unsigned long long err_hist;
   unsigned long long n_stage;
+  unsigned long long n_hole;
   int32_t err;
   int32_t pad_;
 };
@@ -1480,7 +1487,8 @@ void batch_step(abx_batch* b, std::vector<abx_problem*>& finished) {
       const int j = order[g1];
       BatchEntry& it = b->items[j];
       abx_problem* p = it.p;
-      const CallScalars cs{ds[j].sc, ds[j].cn, &ds[j].err, &ds[j].err_hist, &ds[j].n_stage};
+      const CallScalars cs{ds[j].sc, ds[j].cn, &ds[j].err, &ds[j].err_hist, &ds[j].n_stage,
+                           &ds[j].n_hole};
       const int sec_cap = needs_secondaries(p->T) ? 16 : 0;
       SecRec* sec = max_sec ? b->sec.as<SecRec>() + (groups.size() % NS) * lanes * max_sec
                             : nullptr;
@@ -1548,7 +1556,7 @@ void batch_step(abx_batch* b, std::vector<abx_problem*>& finished) {
     p->run_histories += it.n;
     add_scores(it.acc, r.scores);
     int64_t n_sites = 0;
-    if (T.st.mode == ABX_MODIFIED_FIXED_SOURCE) n_sites = order_fission_bank(p, it.n, h.n_stage);
+    if (T.st.mode == ABX_MODIFIED_FIXED_SOURCE) n_sites = order_fission_bank(p, it.n, h.n_stage, h.n_hole);
     if (n_sites > 0) {
       if (p->cancel) cancel_device(p, p->ordered.as<abx_banked>(), n_sites);
       p->bank_b.reserve((size_t)n_sites * sizeof(abx_particle));

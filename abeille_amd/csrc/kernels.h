@@ -23,7 +23,8 @@ struct KCtx {
   abx_banked* stage;
   uint32_t* stage_parent;
   uint32_t* stage_daughter;
-  unsigned long long* n_stage;
+  unsigned long long* n_stage;   // staging slots handed out (sites + unused slots)
+  unsigned long long* n_hole;    // slots handed out but left unused (marked kHole)
   int64_t stage_cap;
   uint32_t* daughters;    // [n] sites per parent
   double* scores;         // [6] k_col k_abs k_trk k_tot leak mig
@@ -69,6 +70,9 @@ __device__ __forceinline__ void raise_error(const KCtx& K, int code, uint64_t h)
   if (atomicCAS(K.err, 0, code) == 0) *K.err_hist = (unsigned long long)h;
 }
 
+// stage_parent value of a staging slot that holds no site.
+constexpr uint32_t kHole = 0xFFFFFFFFu;
+
 // Global-memory side of the physics for one lane.  TRACE compiles the
 // per-event trace records in; SEC the per-history secondary stack; SPLIT
 // selects the split-precision physics.
@@ -82,6 +86,54 @@ struct GpuSink {
   int32_t nsec;
   int32_t n_rec;
   uint32_t c[8];  // per-lane event counters (a lane's counts fit 32 bits)
+  // This wave's range [srange[0], srange[1]) of staging slots (LDS), or
+  // null: one counter atomic per banking call of the wave.
+  unsigned long long* srange;
+
+  // Staging slot for a fission site.  With a wave range, the shared counter
+  // is touched once per 64 slots: a counter that every wave of the grid
+  // increments at nearly every collision serialises at the memory-side
+  // atomic unit.  Called by the lanes that bank now (any subset).
+  __device__ unsigned long long stage_slot() {
+    if (!srange) return atomicAdd(K->n_stage, 1ull);
+    const int lane = threadIdx.x & 63;
+    const unsigned long long mask = __ballot(1);
+    const int leader = __ffsll((long long)mask) - 1;
+    const unsigned long long cnt = (unsigned long long)__popcll(mask);
+    const unsigned long long rank = (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull));
+    const unsigned long long wb =
+        __hip_atomic_load(srange, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    const unsigned long long we =
+        __hip_atomic_load(srange + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    const unsigned long long avail = we - wb;
+    if (cnt <= avail) {
+      if (lane == leader)
+        __hip_atomic_store(srange, wb + cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+      return wb + rank;
+    }
+    const unsigned long long grab = cnt - avail > 64ull ? cnt - avail : 64ull;
+    unsigned long long nb = 0;
+    if (lane == leader) nb = atomicAdd(K->n_stage, grab);
+    nb = __shfl(nb, leader, 64);
+    if (lane == leader) {
+      __hip_atomic_store(srange, nb + (cnt - avail), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_WAVEFRONT);
+      __hip_atomic_store(srange + 1, nb + grab, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    }
+    return rank < avail ? wb + rank : nb + (rank - avail);
+  }
+  // At the wave's end (all lanes): mark the unused rest of its range.
+  __device__ void close_stage_range() {
+    if (!srange) return;
+    const int lane = threadIdx.x & 63;
+    const unsigned long long wb =
+        __hip_atomic_load(srange, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    const unsigned long long we =
+        __hip_atomic_load(srange + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    for (unsigned long long j = wb + (unsigned long long)lane; j < we; j += 64ull)
+      if ((int64_t)j < K->stage_cap) K->stage_parent[j] = kHole;
+    if (lane == 0 && we > wb) atomicAdd(K->n_hole, we - wb);
+  }
 
   __device__ void tally_add(double* a, double v) {
     unsafeAtomicAdd(a, v);
@@ -108,7 +160,7 @@ struct GpuSink {
     }
   }
   __device__ void bank_fission(const Part& p, const V3& u, double E, double w) {
-    const unsigned long long i = atomicAdd(K->n_stage, 1ull);
+    const unsigned long long i = stage_slot();
     if ((int64_t)i >= K->stage_cap) return;  // counted; host reports capacity
     abx_banked b;
     b.r[0] = p.r.x;
@@ -165,6 +217,7 @@ struct GpuSink {
     nsec = 0;
     n_rec = 0;
     for (int i = 0; i < 8; i++) c[i] = 0;
+    srange = nullptr;
   }
 };
 
@@ -233,6 +286,15 @@ __global__ void ABX_TKERNEL k_history(DevProblem P0, KCtx K, unsigned long long*
   GpuSink<TRACE, SEC, SPLIT> s;
   s.init(&K, &P, 0);
   s.sslot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // per-wave fission staging ranges (blocks of at most 1024 lanes)
+  __shared__ unsigned long long s_srange[32];
+  if (K.stage && K.n_hole) {
+    s.srange = s_srange + 2 * (threadIdx.x >> 6);
+    if (lane == 0) {
+      __hip_atomic_store(s.srange, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+      __hip_atomic_store(s.srange + 1, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    }
+  }
   Trk t;
   bind_lds_cursor(t, P);
   Part p;
@@ -301,6 +363,7 @@ __global__ void ABX_TKERNEL k_history(DevProblem P0, KCtx K, unsigned long long*
       }
     }
   }
+  s.close_stage_range();
   flush_scores(K, sc, s.c);
 }
 
@@ -464,7 +527,8 @@ __global__ void k_order_bank(const abx_banked* stage, const uint32_t* parent,
                              abx_banked* out) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= m) return;
-  out[offsets[parent[k]] + daughter[k]] = stage[k];
+  const uint32_t q = parent[k];
+  if (q != kHole) out[offsets[q] + daughter[k]] = stage[k];
 }
 
 // MeshTally::record_generation (src/mesh_tally.cpp:121-150), then zero the
@@ -607,7 +671,8 @@ __global__ void k_cancel_apply(const uint64_t* keys, int64_t n, abx_banked* b) {
 struct GenDev {
   unsigned long long n;           // histories of the current generation
   unsigned long long n_sites;     // fission sites it banked (after the scan)
-  unsigned long long n_stage;     // sites staged by the transport kernel
+  unsigned long long n_stage;     // staging slots handed out by the transport kernel
+  unsigned long long n_hole;      // of which left unused
   unsigned long long first_hist;  // first history id of the re-banked generation
   unsigned long long run_hist;    // Simulation::histories_counter
   unsigned long long next;        // persistent-kernel history counter
@@ -635,6 +700,7 @@ __global__ void k_gen_begin(GenDev* g) {
   if (t < 2) g->wsum[t] = 0.0;
   if (t == 0) {
     g->n_stage = 0;
+    g->n_hole = 0;
     g->next = 0;
   }
 }
@@ -705,7 +771,9 @@ __global__ void k_order_bank_dev(const abx_banked* stage, const uint32_t* parent
   if (g->err) return;
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m;
        k += (int64_t)gridDim.x * blockDim.x) {
-    const uint64_t dst = offsets[parent[k]] + daughter[k];
+    const uint32_t q = parent[k];
+    if (q == kHole) continue;
+    const uint64_t dst = offsets[q] + daughter[k];
     if (dst < g->cap) out[dst] = stage[k];  // beyond: k_gen_end reports the overflow
   }
 }
@@ -756,7 +824,8 @@ __device__ __forceinline__ void gen_upd(int gen, double x, double& avg, double& 
 // next generations transport nothing.
 __global__ void k_gen_end(GenDev* g, double W, int converged, double* kcol_hist) {
   if (threadIdx.x != 0) return;
-  if (!g->err && g->n_sites != g->n_stage) g->err = -1;
+  if (!g->err && g->n_sites != g->n_stage - g->n_hole) g->err = -1;
+  if (!g->err && g->n_stage > g->cap) g->err = -5;
   if (!g->err && g->n_sites > g->cap) g->err = -5;
   if (!g->err && g->n_sites == 0) g->err = -8;
   for (int k = 0; k < 8; k++) g->diag[k] += g->counters[k];
