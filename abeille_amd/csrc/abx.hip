@@ -74,6 +74,7 @@ struct abx_problem {
   int device = 0;
   // tallies
   DBuf tally_gen, tally_avg, tally_var;
+  DBuf tally_more;  // batch engine: generation arrays of concurrently run batches 2, 3, ...
   int tally_g = 0;
   // transport scratch
   DBuf in, sec, stage, stage_parent, stage_daughter, n_stage, daughters, offsets,
@@ -83,6 +84,7 @@ struct abx_problem {
   std::vector<hipEvent_t> gen_events;  // graph runs: 2 per generation
   // cancellation
   bool cancel = false;
+  int batch_stage = -1;  // batch kernel: tables staged in LDS pay off (-1 = not yet known)
   DevCancel cm{};
   DBuf ckeys, ckeys2, ctemp;
   // driver state
@@ -118,7 +120,7 @@ struct abx_problem {
 
   ~abx_problem() {
     for (auto& b : arrays) b.release();
-    DBuf* all[] = {&tally_gen, &tally_avg, &tally_var, &in, &sec, &stage, &stage_parent,
+    DBuf* all[] = {&tally_gen, &tally_avg, &tally_var, &tally_more, &in, &sec, &stage, &stage_parent,
                    &stage_daughter, &n_stage, &daughters, &offsets, &block_sums, &block_sums2,
                    &ordered, &scores, &counters, &err, &err_hist, &trace, &n_rec, &wsum,
                    &bank_a, &bank_b, &ckeys, &ckeys2, &ctemp, &pool_next, &blob, &gdev,
@@ -385,7 +387,7 @@ void upd(int gen, double x, double& avg, double& var) {
 }
 
 void record_generation(abx_problem* p, const abx_scores& s, double W,
-                       hipStream_t st = nullptr) {
+                       hipStream_t st = nullptr, double* gen = nullptr) {
   p->gen++;
   upd(p->gen, s.k_col / W, p->kc_avg, p->kc_var);
   upd(p->gen, s.k_trk / W, p->kt_avg, p->kt_var);
@@ -395,19 +397,19 @@ void record_generation(abx_problem* p, const abx_scores& s, double W,
   const int64_t nb = p->T.tally_bins;
   if (nb > 0) {
     hipLaunchKernelGGL(k_record_generation, dim3((unsigned)blocks_for(nb, 256)), dim3(256), 0,
-                       st ? st : p->stream, p->tally_gen.as<double>(), p->tally_avg.as<double>(),
+                       st ? st : p->stream, gen ? gen : p->tally_gen.as<double>(),
+                       p->tally_avg.as<double>(),
                        p->tally_var.as<double>(), nb, (double)p->tally_g, 0);
     HIPC(hipGetLastError());
   }
 }
 
-void clear_generation(abx_problem* p, hipStream_t st = nullptr) {
+void clear_generation(abx_problem* p, hipStream_t st = nullptr, double* gen = nullptr) {
   p->gen_scores = abx_scores{};
   const int64_t nb = p->T.tally_bins;
   if (nb > 0) {
     hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks_for(nb, 256)), dim3(256), 0,
-                       st ? st : p->stream,
-                       p->tally_gen.as<double>(), nb, 0.0);
+                       st ? st : p->stream, gen ? gen : p->tally_gen.as<double>(), nb, 0.0);
     HIPC(hipGetLastError());
   }
 }
@@ -1216,6 +1218,7 @@ struct BatchEntry {
   int64_t n;          // histories of the current call
   abx_scores acc{};   // scores of the current batch (chain or slices summed)
   int64_t done = 0;   // histories of the current fixed-source batch already run
+  size_t foot = 0;    // device bytes counted against the batch's budget
 };
 
 // A fixed-source batch may be split over several steps (its histories are
@@ -1269,19 +1272,34 @@ struct BatchScalars {
   int32_t pad_;
 };
 
+// Whether the batch kernel would run this problem with its tables staged in
+// LDS: they fit next to 256 cursors and cost no resident block.
+bool batch_stage_fits(const abx_problem* p) {
+  const HostTables& T = p->T;
+  const int trk = T.st.tracking;
+  const bool sec = needs_secondaries(T), split = T.st.precision == ABX_SPLIT;
+  const size_t cur = cursor_lds_bytes(256, p->P.trk_D, p->P.trk_L);
+  bool ok = false;
+  (void)dispatch_variant(trk, false, sec || trk == 2, split, [&](auto TT, auto, auto SE, auto SP) {
+    constexpr int TK = decltype(TT)::value;
+    constexpr bool SEC = decltype(SE)::value, SPL = decltype(SP)::value;
+    ok = stage_pays(k_batch_chunks<TK, SEC, SPL, true>, k_batch_chunks<TK, SEC, SPL, false>, 256,
+                    cur, (size_t)p->P.blob_bytes);
+    return hipSuccess;
+  });
+  return ok;
+}
+
 // Launch group of a problem in the batch kernel: tracking mode, secondaries,
-// precision, and whether its tables fit in LDS next to 256 cursors (problems
-// whose tables fit are launched apart from those whose tables do not, so
-// that the former run the staged variant).
-int variant_key(const abx_problem* p) {
+// precision, and whether its tables are worth staging in LDS (problems whose
+// tables are launched apart from those whose tables are not, so that the
+// former run the staged variant).
+int variant_key(abx_problem* p) {
   const HostTables& T = p->T;
   const int sec = needs_secondaries(T) ? 1 : 0;
   const int split = T.st.precision == ABX_SPLIT ? 1 : 0;
-  const int fits = cursor_lds_bytes(256, p->P.trk_D, p->P.trk_L) + (size_t)p->P.blob_bytes <=
-                           lds_block_limit()
-                       ? 1
-                       : 0;
-  return ((T.st.tracking * 2 + sec) * 2 + split) * 2 + fits;
+  if (p->batch_stage < 0) p->batch_stage = batch_stage_fits(p) ? 1 : 0;
+  return ((T.st.tracking * 2 + sec) * 2 + split) * 2 + p->batch_stage;
 }
 
 // Chunks of `chunk` histories over the problems of one launch (item indices
@@ -1387,6 +1405,10 @@ struct abx_batch {
   hipStream_t stream = nullptr;
   DBuf probs, ctxs, offsets, scal, next, sec, chunks;
   std::vector<BatchScalars> h_scal;
+  // host images of a step's launch arrays (read by the uploads)
+  std::vector<DevProblem> hp;
+  std::vector<KCtx> hk;
+  std::vector<BatchChunk> all_chunks;
   size_t budget = 0;  // device bytes admitted problems may hold (0 = no limit)
   size_t used = 0;
   int64_t step_hist = 0;  // histories per step over all problems (0 = no limit)
@@ -1414,34 +1436,89 @@ struct abx_batch {
 
 namespace {
 
+// Concurrent batches of one fixed-source problem in a step: up to
+// kMultiMax, and at most kMultiHist histories together (a problem with
+// large batches runs one at a time).
+constexpr int kMultiMax = 8;
+constexpr int64_t kMultiHist = 1 << 20;
+
+int concurrent_batches(const abx_problem* p, int batches_left, int64_t step_hist) {
+  const int64_t N = p->T.st.nparticles;
+  if (!sliceable(p) || step_hist > 0 || N <= 0) return 1;
+  const int64_t k = std::min<int64_t>({(int64_t)batches_left, (int64_t)kMultiMax,
+                                       std::max<int64_t>(1, kMultiHist / N)});
+  return (int)std::max<int64_t>(1, k);
+}
+
+// One launch item: histories [0, n) of bank `in` scoring into `gen`.
+struct LaunchItem {
+  int entry;
+  int k;  // concurrent batch index within the entry
+  int64_t n;
+  const abx_particle* in;
+  double* gen;
+};
+
 void batch_step(abx_batch* b, std::vector<abx_problem*>& finished) {
-  const int ni = (int)b->items.size();
-  if (ni == 0) return;
-  b->scal.reserve((size_t)ni * sizeof(BatchScalars));
-  HIPC(hipMemsetAsync(b->scal.p, 0, (size_t)ni * sizeof(BatchScalars), b->stream));
-  BatchScalars* ds = b->scal.as<BatchScalars>();
-  // 1. inputs: the next sources of a batch (all of it, or this step's slice
-  // of a fixed-source batch), or the chain's re-banked generation
+  const int ne = (int)b->items.size();
+  if (ne == 0) return;
+  // 1. launch items and their sources: the next batches (several at once
+  // for small fixed-source problems, or this step's slice of a large one),
+  // or the chain's re-banked generation
   const std::vector<int64_t> give = step_allocation(b->items, b->step_hist);
-  for (int j = 0; j < ni; j++) {
+  std::vector<LaunchItem> li;
+  std::vector<int> first_item(ne);
+  for (int j = 0; j < ne; j++) {
     BatchEntry& it = b->items[j];
     abx_problem* p = it.p;
-    if (it.mid_chain) continue;
-    const int64_t n = give[j];
-    p->bank_a.reserve((size_t)std::max<int64_t>(1, n) * sizeof(abx_particle));
-    hipLaunchKernelGGL(k_sample_sources, dim3((unsigned)blocks_for(n, 256)), dim3(256),
-                       (size_t)256 * trk_bytes(p->P.trk_D, p->P.trk_L), b->stream, p->P, p->S,
-                       p->run_hist, n, p->bank_a.as<abx_particle>(), &ds[j].err, &ds[j].err_hist);
-    HIPC(hipGetLastError());
-    p->run_hist += (uint64_t)n;
+    first_item[j] = (int)li.size();
+    if (it.mid_chain) {
+      li.push_back(LaunchItem{j, 0, it.n, p->bank_a.as<abx_particle>(), p->tally_gen.as<double>()});
+      continue;
+    }
+    const int kb = concurrent_batches(p, it.batches_left, b->step_hist);
+    const int64_t n = kb > 1 ? p->T.st.nparticles : give[j];
+    const int64_t nb = std::max<int64_t>(1, p->T.tally_bins);
+    p->bank_a.reserve((size_t)std::max<int64_t>(1, kb * n) * sizeof(abx_particle));
+    if (kb > 1 && p->tally_more.bytes < (size_t)(kb - 1) * nb * sizeof(double)) {
+      // generation arrays are zero between steps; a new one starts zeroed
+      p->tally_more.reserve((size_t)(kb - 1) * nb * sizeof(double));
+      hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks_for((kb - 1) * nb, 256)), dim3(256), 0,
+                         b->stream, p->tally_more.as<double>(), (int64_t)(kb - 1) * nb, 0.0);
+      HIPC(hipGetLastError());
+    }
+    for (int k = 0; k < kb; k++)
+      li.push_back(LaunchItem{j, k, n, p->bank_a.as<abx_particle>() + (size_t)k * n,
+                              k == 0 ? p->tally_gen.as<double>()
+                                     : p->tally_more.as<double>() + (size_t)(k - 1) * nb});
     if (it.done == 0) it.acc = abx_scores{};
     it.n = n;
   }
+  const int ni = (int)li.size();
+  b->scal.reserve((size_t)ni * sizeof(BatchScalars));
+  HIPC(hipMemsetAsync(b->scal.p, 0, (size_t)ni * sizeof(BatchScalars), b->stream));
+  BatchScalars* ds = b->scal.as<BatchScalars>();
+  for (int j = 0; j < ne; j++) {
+    BatchEntry& it = b->items[j];
+    if (it.mid_chain) continue;
+    abx_problem* p = it.p;
+    const int f = first_item[j];
+    const int64_t n = (j + 1 < ne ? first_item[j + 1] : ni) - f;  // items of the entry
+    const int64_t total = n * it.n;
+    if (total > 0) {
+      hipLaunchKernelGGL(k_sample_sources, dim3((unsigned)blocks_for(total, 256)), dim3(256),
+                         (size_t)256 * trk_bytes(p->P.trk_D, p->P.trk_L), b->stream, p->P, p->S,
+                         p->run_hist, total, p->bank_a.as<abx_particle>(), &ds[f].err,
+                         &ds[f].err_hist);
+      HIPC(hipGetLastError());
+    }
+    p->run_hist += (uint64_t)total;
+  }
   // 2. one launch per kernel variant
   std::vector<int> order(ni);
-  for (int j = 0; j < ni; j++) order[j] = j;
+  for (int i = 0; i < ni; i++) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
-    return variant_key(b->items[a].p) < variant_key(b->items[c].p);
+    return variant_key(b->items[li[a].entry].p) < variant_key(b->items[li[c].entry].p);
   });
   int max_sec = 0;
   for (auto& it : b->items) max_sec = std::max(max_sec, needs_secondaries(it.p->T) ? 16 : 0);
@@ -1453,8 +1530,8 @@ void batch_step(abx_batch* b, std::vector<abx_problem*>& finished) {
   // at the same time and index their stacks by lane
   const int NS = abx_batch::NS;
   if (max_sec) b->sec.reserve((size_t)NS * lanes * max_sec * sizeof(SecRec));
-  std::vector<DevProblem> hp(ni);
-  std::vector<KCtx> hk(ni);
+  b->hp.resize(ni);
+  b->hk.resize(ni);
   b->probs.reserve((size_t)ni * sizeof(DevProblem));
   b->ctxs.reserve((size_t)ni * sizeof(KCtx));
   if (!b->e0) {
@@ -1473,28 +1550,29 @@ void batch_step(abx_batch* b, std::vector<abx_problem*>& finished) {
     int64_t c0, nc;
   };
   std::vector<Group> groups;
-  std::vector<BatchChunk> all_chunks;
+  b->all_chunks.clear();
   int64_t total_all = 0;
-  for (auto& it : b->items) total_all += it.n;
+  for (const auto& x : li) total_all += x.n;
   const int64_t clen = chunk_length(total_all);
   int g0 = 0;
   while (g0 < ni) {
-    const int key = variant_key(b->items[order[g0]].p);
+    const int key = variant_key(b->items[li[order[g0]].entry].p);
     int g1 = g0;
-    Group gr{key, g0, 0, 1, 1, 16, (int64_t)all_chunks.size(), 0};
+    Group gr{key, g0, 0, 1, 1, 16, (int64_t)b->all_chunks.size(), 0};
     std::vector<int64_t> n;
-    while (g1 < ni && variant_key(b->items[order[g1]].p) == key) {
-      const int j = order[g1];
-      BatchEntry& it = b->items[j];
-      abx_problem* p = it.p;
-      const CallScalars cs{ds[j].sc, ds[j].cn, &ds[j].err, &ds[j].err_hist, &ds[j].n_stage,
-                           &ds[j].n_hole};
+    while (g1 < ni && variant_key(b->items[li[order[g1]].entry].p) == key) {
+      const int i = order[g1];
+      const LaunchItem& x = li[i];
+      abx_problem* p = b->items[x.entry].p;
+      const CallScalars cs{ds[i].sc, ds[i].cn, &ds[i].err, &ds[i].err_hist, &ds[i].n_stage,
+                           &ds[i].n_hole};
       const int sec_cap = needs_secondaries(p->T) ? 16 : 0;
       SecRec* sec = max_sec ? b->sec.as<SecRec>() + (groups.size() % NS) * lanes * max_sec
                             : nullptr;
-      hk[g1] = make_kctx(p, p->bank_a.as<abx_particle>(), it.n, p->run_kcol, 1, cs, sec, sec_cap);
-      hp[g1] = p->P;
-      n.push_back(it.n);
+      b->hk[g1] = make_kctx(p, x.in, x.n, p->run_kcol, 1, cs, sec, sec_cap);
+      b->hp[g1] = p->P;
+      b->hp[g1].tally_data = x.gen;
+      n.push_back(x.n);
       gr.D = std::max(gr.D, p->P.trk_D);
       gr.L = std::max(gr.L, p->P.trk_L);
       gr.blob_max = std::max(gr.blob_max, (size_t)p->P.blob_bytes);
@@ -1502,18 +1580,20 @@ void batch_step(abx_batch* b, std::vector<abx_problem*>& finished) {
     }
     gr.cnt = g1 - g0;
     const std::vector<BatchChunk> c = make_chunks(n, clen);
-    all_chunks.insert(all_chunks.end(), c.begin(), c.end());
+    b->all_chunks.insert(b->all_chunks.end(), c.begin(), c.end());
     gr.nc = (int64_t)c.size();
     groups.push_back(gr);
     g0 = g1;
   }
-  b->chunks.reserve(std::max<size_t>(1, all_chunks.size()) * sizeof(BatchChunk));
-  HIPC(hipMemcpyAsync(b->probs.p, hp.data(), ni * sizeof(DevProblem), hipMemcpyHostToDevice,
+  b->chunks.reserve(std::max<size_t>(1, b->all_chunks.size()) * sizeof(BatchChunk));
+  HIPC(hipMemcpyAsync(b->probs.p, b->hp.data(), ni * sizeof(DevProblem), hipMemcpyHostToDevice,
                       b->stream));
-  HIPC(hipMemcpyAsync(b->ctxs.p, hk.data(), ni * sizeof(KCtx), hipMemcpyHostToDevice, b->stream));
-  if (!all_chunks.empty())
-    HIPC(hipMemcpyAsync(b->chunks.p, all_chunks.data(), all_chunks.size() * sizeof(BatchChunk),
-                        hipMemcpyHostToDevice, b->stream));
+  HIPC(hipMemcpyAsync(b->ctxs.p, b->hk.data(), ni * sizeof(KCtx), hipMemcpyHostToDevice,
+                      b->stream));
+  if (!b->all_chunks.empty())
+    HIPC(hipMemcpyAsync(b->chunks.p, b->all_chunks.data(),
+                        b->all_chunks.size() * sizeof(BatchChunk), hipMemcpyHostToDevice,
+                        b->stream));
   // one chunk counter per group (groups run concurrently)
   b->next.reserve(std::max<size_t>(8, groups.size()) * sizeof(unsigned long long));
   HIPC(hipEventRecord(b->e0, b->stream));
@@ -1533,8 +1613,6 @@ void batch_step(abx_batch* b, std::vector<abx_problem*>& finished) {
     HIPC(hipEventRecord(b->done[k], b->aux[k]));
     HIPC(hipStreamWaitEvent(b->stream, b->done[k], 0));
   }
-  // the host arrays above are read by the copies before they go out of scope
-  HIPC(hipStreamSynchronize(b->stream));
   HIPC(hipEventRecord(b->e1, b->stream));
   b->h_scal.resize(ni);
   HIPC(hipMemcpyAsync(b->h_scal.data(), b->scal.p, (size_t)ni * sizeof(BatchScalars),
@@ -1543,48 +1621,62 @@ void batch_step(abx_batch* b, std::vector<abx_problem*>& finished) {
   float ms = 0.f;
   HIPC(hipEventElapsedTime(&ms, b->e0, b->e1));
   b->kernel_ms += ms;
-  // 3. per-problem bookkeeping
+  // 3. per-problem bookkeeping, the concurrent batches of an entry recorded
+  // in history order
   std::vector<BatchEntry> keep;
-  keep.reserve(ni);
-  for (int j = 0; j < ni; j++) {
+  keep.reserve(ne);
+  for (int j = 0; j < ne; j++) {
     BatchEntry& it = b->items[j];
     abx_problem* p = it.p;
     const HostTables& T = p->T;
-    const BatchScalars& h = b->h_scal[j];
-    TransportOut r = collect_scalars(h.sc, h.cn, h.err, h.err_hist);
-    add_diag(p->run_diag, r.diag);
-    p->run_histories += it.n;
-    add_scores(it.acc, r.scores);
-    int64_t n_sites = 0;
-    if (T.st.mode == ABX_MODIFIED_FIXED_SOURCE) n_sites = order_fission_bank(p, it.n, h.n_stage, h.n_hole);
-    if (n_sites > 0) {
-      if (p->cancel) cancel_device(p, p->ordered.as<abx_banked>(), n_sites);
-      p->bank_b.reserve((size_t)n_sites * sizeof(abx_particle));
-      hipLaunchKernelGGL(k_rebank, dim3((unsigned)blocks_for(n_sites, 256)), dim3(256), 0, p->stream,
-                         p->ordered.as<abx_banked>(), n_sites, 1.0, p->run_hist, T.st.seed,
-                         T.st.stride, 1, 0, p->bank_b.as<abx_particle>());
-      HIPC(hipGetLastError());
-      p->run_hist += (uint64_t)n_sites;
-      std::swap(p->bank_a, p->bank_b);
-      p->bank_n = (size_t)n_sites;
-      it.n = n_sites;
-      it.mid_chain = true;
-    } else if (sliceable(p) && (it.done += it.n) < T.st.nparticles) {
-      // more slices of this batch follow
-    } else {
-      it.mid_chain = false;
-      it.done = 0;
-      record_generation(p, it.acc, (double)T.st.nparticles, b->stream);
-      clear_generation(p, b->stream);
-      it.batches_left--;
+    const int f = first_item[j];
+    const int nk = (j + 1 < ne ? first_item[j + 1] : ni) - f;
+    for (int k = 0; k < nk; k++) {
+      const LaunchItem& x = li[f + k];
+      const BatchScalars& h = b->h_scal[f + k];
+      TransportOut r = collect_scalars(h.sc, h.cn, h.err, h.err_hist);
+      add_diag(p->run_diag, r.diag);
+      p->run_histories += x.n;
+      if (nk > 1) {
+        // a whole batch of its own
+        record_generation(p, r.scores, (double)T.st.nparticles, b->stream, x.gen);
+        clear_generation(p, b->stream, x.gen);
+        it.batches_left--;
+        continue;
+      }
+      add_scores(it.acc, r.scores);
+      int64_t n_sites = 0;
+      if (T.st.mode == ABX_MODIFIED_FIXED_SOURCE)
+        n_sites = order_fission_bank(p, it.n, h.n_stage, h.n_hole);
+      if (n_sites > 0) {
+        if (p->cancel) cancel_device(p, p->ordered.as<abx_banked>(), n_sites);
+        p->bank_b.reserve((size_t)n_sites * sizeof(abx_particle));
+        hipLaunchKernelGGL(k_rebank, dim3((unsigned)blocks_for(n_sites, 256)), dim3(256), 0,
+                           p->stream, p->ordered.as<abx_banked>(), n_sites, 1.0, p->run_hist,
+                           T.st.seed, T.st.stride, 1, 0, p->bank_b.as<abx_particle>());
+        HIPC(hipGetLastError());
+        p->run_hist += (uint64_t)n_sites;
+        std::swap(p->bank_a, p->bank_b);
+        p->bank_n = (size_t)n_sites;
+        it.n = n_sites;
+        it.mid_chain = true;
+      } else if (sliceable(p) && (it.done += it.n) < T.st.nparticles) {
+        // more slices of this batch follow
+      } else {
+        it.mid_chain = false;
+        it.done = 0;
+        record_generation(p, it.acc, (double)T.st.nparticles, b->stream);
+        clear_generation(p, b->stream);
+        it.batches_left--;
+      }
+      // fission-bank ordering, cancellation and re-banking ran on the
+      // problem's own stream; the next step's launches must see them
+      if (!sliceable(p)) HIPC(hipStreamSynchronize(p->stream));
     }
-    // fission-bank ordering, cancellation and re-banking ran on the
-    // problem's own stream; the next step's launches must see them
-    if (!sliceable(p)) HIPC(hipStreamSynchronize(p->stream));
     if (it.batches_left > 0) {
       keep.push_back(it);
     } else {
-      b->used -= std::min(b->used, problem_footprint(p));
+      b->used -= std::min(b->used, it.foot);
       finished.push_back(p);
     }
   }
@@ -1624,14 +1716,20 @@ int abx_batch_add(abx_batch* b, abx_problem* p, int nbatches) {
       fail(ABX_E_INVALID, "k-eigenvalue problems run with abx_run, not in a batch");
     for (const auto& it : b->items)
       if (it.p == p) fail(ABX_E_INVALID, "problem already attached");
-    const size_t need = problem_footprint(p);
+    // the problem alone, plus the banks and generation arrays of batches
+    // run side by side
+    const int kb = concurrent_batches(p, nbatches, b->step_hist);
+    size_t need = problem_footprint(p);
+    if (kb > 1)
+      need += (size_t)(kb - 1) * ((size_t)p->T.st.nparticles * sizeof(abx_particle) +
+                                  (size_t)std::max<int64_t>(1, p->T.tally_bins) * sizeof(double));
     if (b->budget && b->used + need > b->budget)
       fail(ABX_E_BUDGET, "problem does not fit the batch's device budget");
     run_reset(p);
     p->run_diag = abx_diag{};
     p->run_histories = 0;
     b->used += need;
-    b->items.push_back(BatchEntry{p, nbatches, false, 0, abx_scores{}, 0});
+    b->items.push_back(BatchEntry{p, nbatches, false, 0, abx_scores{}, 0, need});
   });
 }
 
