@@ -1440,7 +1440,7 @@ namespace {
 // kMultiMax, and at most kMultiHist histories together (a problem with
 // large batches runs one at a time).
 constexpr int kMultiMax = 8;
-constexpr int64_t kMultiHist = 1 << 20;
+constexpr int64_t kMultiHist = 1 << 23;
 
 int concurrent_batches(const abx_problem* p, int batches_left, int64_t step_hist) {
   const int64_t N = p->T.st.nparticles;

@@ -276,6 +276,14 @@ def bench_serve(args):
     served_fixed = [reqs[i] for i in plain_idx]
     pct_fixed = latency_percentiles(served_fixed)
     coll_fixed = sum(r.stats.diag.collisions for r in served_fixed)
+    # the fixed-source requests served on their own (the stream the plain
+    # leg serves): same arrivals, no carter chains sharing the steps
+    alone = [Request(reqs[i].model, 2, float(arrivals[i]), name=f"r{i}") for i in plain_idx]
+    srv2 = Server(budget=64 << 30, keep_tallies=False, step_histories=args.step_histories)
+    t0 = time.perf_counter()
+    srv2.run(alone)
+    wall2 = time.perf_counter() - t0
+    coll2 = sum(r.stats.diag.collisions for r in alone)
     out = _common(args, coll / wall, "collisions/s", 1e3 * wall / max(1, srv.steps),
                   "f64/f32 split",
                   {"workload": f"{args.requests} slab requests, Poisson {args.rate}/s, 1e4-5e6 "
@@ -290,6 +298,10 @@ def bench_serve(args):
                    "fixed_source_requests": {
                        "served_latency_s": pct_fixed, "served_collisions": coll_fixed,
                        "served_makespan_s": float(max(r.finished for r in served_fixed)),
+                       "served_without_carter": {
+                           "collisions_per_s": coll2 / wall2,
+                           "makespan_s": float(max(r.finished for r in alone)),
+                           "latency_s": latency_percentiles(alone)},
                        "plain_fp64_one_at_a_time": {
                            "collisions_per_s": pc / t,
                            "saturated_collisions_per_s": pc / float(service.sum()),
