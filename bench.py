@@ -155,12 +155,12 @@ def bench_precision(args):
             ok = sig > 0
             z = (a32[ok] - a64[ok]) / sig[ok]
             good = (a64 > 0) & (e64 < 0.01 * a64)
-            rel = np.abs(a32[good] - a64[good]) / a64[good] if good.any() else np.array([np.nan])
+            rel = np.abs(a32[good] - a64[good]) / a64[good] if good.any() else None
             dev.append({"bins": int(ok.sum()), "z_mean": float(z.mean()),
                         "z_std": float(z.std()),
                         "frac_within_3sigma": float((np.abs(z) <= 3).mean()),
                         "bins_below_1pct_err": int(good.sum()),
-                        "mean_rel_dev_those_bins": float(rel.mean())})
+                        "mean_rel_dev_those_bins": None if rel is None else float(rel.mean())})
         return dev
     dev = deviation("split")
     out = _common(args, res["split"]["rate"], "collisions/s", res["split"]["ms"], "f64/f32 split",
