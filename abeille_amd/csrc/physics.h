@@ -223,6 +223,10 @@ ABX_NOINL void score_collision_tallies(const DevProblem& P, Sink& s, const Part&
   }
 }
 
+#ifndef ABX_TLE_DDA
+#define ABX_TLE_DDA 1
+#endif
+
 // TrackLengthMeshTally::initialize_indices
 ABX_HD void tle_init(const DTally& T, const V3& r, const V3& u, int& i, int& j, int& k,
                      int on[3]) {
@@ -305,6 +309,47 @@ ABX_NOINL void score_flight_tallies(const DevProblem& P, Sink& s, const Part& p,
     if (l == -1) continue;
     double remaining = dd;
     double* data = P.tally_data + T.data_off + (int64_t)l * T.Nx * T.Ny * T.Nz;
+#if ABX_TLE_DDA
+    // The walk of TrackLengthMeshTally::score_flight
+    // (src/track_length_mesh_tally.cpp:34-435) as a digital differential
+    // analyser: the distance along the flight to the next plane of each
+    // axis is advanced by one cell pitch per crossing instead of being
+    // recomputed from an advanced position, and the nearest plane wins in
+    // the reference's order (x, then y, then z).  Same cells, same order;
+    // segment lengths agree to rounding.
+    const int sx = u.x > 0. ? 1 : (u.x < 0. ? -1 : 0);
+    const int sy = u.y > 0. ? 1 : (u.y < 0. ? -1 : 0);
+    const int sz = u.z > 0. ? 1 : (u.z < 0. ? -1 : 0);
+    double tx = sx ? (T.lx + (i + (sx > 0)) * T.dx - r.x) * ui.x : ABX_INF;
+    double ty = sy ? (T.ly + (j + (sy > 0)) * T.dy - r.y) * ui.y : ABX_INF;
+    double tz = sz ? (T.lz + (k + (sz > 0)) * T.dz - r.z) * ui.z : ABX_INF;
+    const double px = sx ? T.dx * dabs(ui.x) : 0., py = sy ? T.dy * dabs(ui.y) : 0.,
+                 pz = sz ? T.dz * dabs(ui.z) : 0.;
+    double t = 0.;
+    while (remaining > 0.) {
+      int axis = 0;
+      double tn = tx;
+      if (ty < tn) { tn = ty; axis = 1; }
+      if (tz < tn) { tn = tz; axis = 2; }
+      if (tn == ABX_INF) break;
+      const double step = tn - t;
+      const double d_tile = step < remaining ? step : remaining;
+      if (i >= 0 && i < T.Nx && j >= 0 && j < T.Ny && k >= 0 && k < T.Nz) {
+        s.tally_add(data + ((int64_t)i * T.Ny + j) * T.Nz + k, d_tile * base);
+      } else {
+        break;
+      }
+      remaining -= d_tile;
+      if (remaining <= 0.) break;
+      t = tn;
+      if (axis == 0) { i += sx; tx += px; }
+      else if (axis == 1) { j += sy; ty += py; }
+      else { k += sz; tz += pz; }
+    }
+  }
+}
+
+#else
     while (remaining > 0.) {
       const double xc = T.lx + i * T.dx + 0.5 * T.dx;
       const double yc = T.ly + j * T.dy + 0.5 * T.dy;
@@ -348,6 +393,7 @@ ABX_NOINL void score_flight_tallies(const DevProblem& P, Sink& s, const Part& p,
     }
   }
 }
+#endif
 
 struct Scores6 {
   double k_col, k_abs, k_trk, k_tot, leak, mig;
