@@ -255,6 +255,9 @@ __device__ __forceinline__ void bind_lds_cursor(Trk& t, const DevProblem& P) {
 #ifndef ABX_WAVE_GRAB
 #define ABX_WAVE_GRAB 64ull
 #endif
+#ifndef ABX_REFILL_MIN
+#define ABX_REFILL_MIN 8
+#endif
 
 // LDS bytes of the cursors of a block (the staged tables follow them).
 __host__ __device__ inline size_t cursor_lds_bytes(int bs, int D, int L) {
@@ -310,7 +313,10 @@ __global__ void ABX_TKERNEL k_history(DevProblem P0, KCtx K, unsigned long long*
   for (;;) {
     const bool need = !p.alive && !done;
     const unsigned long long mask = __ballot(need);
-    if (mask) {
+    // Refill once enough lanes are idle (or none has work): starting a
+    // history is a long divergent stretch (source particle, descent from
+    // the root), paid once per refill whatever the number of lanes.
+    if (mask && (__popcll(mask) >= ABX_REFILL_MIN || __ballot(p.alive) == 0ull)) {
       const unsigned long long cnt = (unsigned long long)__popcll(mask);
       const unsigned long long rank = (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull));
       const unsigned long long avail = we - wb;
@@ -429,7 +435,7 @@ __global__ void ABX_TKERNEL k_batch_chunks(const DevProblem* __restrict__ probs,
     for (;;) {
       const bool need = !p.alive && !done;
       const unsigned long long mask = __ballot(need);
-      if (mask) {
+      if (mask && (__popcll(mask) >= ABX_REFILL_MIN || __ballot(p.alive) == 0ull)) {
         const int leader = __ffsll((long long)mask) - 1;
         unsigned long long base = 0;
         if (lane == leader) base = atomicAdd(&s_next, (unsigned long long)__popcll(mask));
