@@ -70,6 +70,7 @@ def lib():
                                      C.POINTER(C.c_int)]
         L.abx_batch_kernel_seconds.argtypes = [P, C.POINTER(C.c_double)]
         L.abx_batch_set_step_histories.argtypes = [P, C.c_int64]
+        L.abx_batch_set_params.argtypes = [P, C.c_int64, C.c_int64]
         L.abx_problem_stats.argtypes = [P, C.POINTER(abi.RunStats)]
         _lib = L
     return _lib
@@ -227,6 +228,12 @@ class Batch:
         self.attached = {}
         if step_histories:
             self.set_step_histories(step_histories)
+
+    def set_params(self, chunk=0, multi_hist=0):
+        """Batch-kernel chunk length and the cap on a problem's side-by-side
+        batches (0 = automatic / default)."""
+        _check(lib().abx_batch_set_params(self.h, C.c_int64(int(chunk)),
+                                          C.c_int64(int(multi_hist))))
 
     def set_step_histories(self, n):
         """Cap the histories of one step over all problems (0 = no cap); a

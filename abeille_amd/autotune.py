@@ -1,16 +1,21 @@
-"""Project autotuner for the transport launch shape (abx_tuning): block
-size, persistent-grid size and LDS staging of the tables, per problem
-class.
+"""Project autotuner.
 
-A candidate is timed with abx_time_transport on a sample of fresh source
+Single problems: the transport launch shape (abx_tuning) - block size,
+persistent-grid size and LDS staging of the tables - per problem class. A
+candidate is timed with abx_time_transport on a sample of fresh source
 histories (the same kernels and tables as the run, none of its state); the
 fastest is applied and remembered in a JSON cache keyed by the problem's
 class (tracking, mode, precision, groups, geometry and tally sizes), so a
 class is searched once.
+
+Batches (autotune_batch): the batch engine's chunk length (a block's unit
+of work) and the cap on one problem's side-by-side batches, timed by
+running a sample of the batch's problems through it.
 """
 import hashlib
 import json
 import os
+import time
 
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "abeille_amd", "tuning.json")
 
@@ -60,3 +65,34 @@ def autotune(problem, model, sample=200_000, repeats=2, cache=DEFAULT_CACHE,
             json.dump(db, f, indent=1, sort_keys=True)
         os.replace(tmp, cache)
     return candidates[best], times
+
+
+BATCH_CANDIDATES = [dict(chunk=c, multi_hist=m) for c in (0, 1024, 4096, 16384)
+                    for m in (1 << 20, 1 << 23)]
+
+
+def autotune_batch(batch, sample, nbatches=1, candidates=BATCH_CANDIDATES, repeats=1):
+    """Pick and apply (batch.set_params) the fastest batch-engine parameters
+    for problems like `sample` (a list of Problems, run `nbatches` batches
+    each through a scratch Batch per candidate; their run statistics are
+    reset when they are attached again).  Returns (params, {index: s})."""
+    from . import Batch, synchronize
+    times = {}
+    for i, c in enumerate(candidates):
+        best = None
+        for _ in range(repeats):
+            b = Batch()
+            b.set_params(**c)
+            synchronize()
+            t0 = time.perf_counter()
+            for p in sample:
+                b.add(p, nbatches)
+            while len(b):
+                b.step()
+            synchronize()
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        times[i] = best
+    k = min(times, key=times.get)
+    batch.set_params(**candidates[k])
+    return candidates[k], times

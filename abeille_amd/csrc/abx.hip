@@ -1412,6 +1412,8 @@ struct abx_batch {
   size_t budget = 0;  // device bytes admitted problems may hold (0 = no limit)
   size_t used = 0;
   int64_t step_hist = 0;  // histories per step over all problems (0 = no limit)
+  int64_t chunk = 0;      // histories per chunk of the batch kernel (0 = automatic)
+  int64_t multi_hist = int64_t(1) << 23;  // cap on side-by-side batches of one problem
   double kernel_ms = 0.0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   // Launch groups (kernel variants) run on these streams side by side, so
@@ -1437,16 +1439,16 @@ struct abx_batch {
 namespace {
 
 // Concurrent batches of one fixed-source problem in a step: up to
-// kMultiMax, and at most kMultiHist histories together (a problem with
+// kMultiMax, and at most multi_hist histories together (a problem with
 // large batches runs one at a time).
 constexpr int kMultiMax = 8;
-constexpr int64_t kMultiHist = 1 << 23;
 
-int concurrent_batches(const abx_problem* p, int batches_left, int64_t step_hist) {
+int concurrent_batches(const abx_problem* p, int batches_left, int64_t step_hist,
+                       int64_t multi_hist) {
   const int64_t N = p->T.st.nparticles;
   if (!sliceable(p) || step_hist > 0 || N <= 0) return 1;
   const int64_t k = std::min<int64_t>({(int64_t)batches_left, (int64_t)kMultiMax,
-                                       std::max<int64_t>(1, kMultiHist / N)});
+                                       std::max<int64_t>(1, multi_hist / N)});
   return (int)std::max<int64_t>(1, k);
 }
 
@@ -1476,7 +1478,7 @@ void batch_step(abx_batch* b, std::vector<abx_problem*>& finished) {
       li.push_back(LaunchItem{j, 0, it.n, p->bank_a.as<abx_particle>(), p->tally_gen.as<double>()});
       continue;
     }
-    const int kb = concurrent_batches(p, it.batches_left, b->step_hist);
+    const int kb = concurrent_batches(p, it.batches_left, b->step_hist, b->multi_hist);
     const int64_t n = kb > 1 ? p->T.st.nparticles : give[j];
     const int64_t nb = std::max<int64_t>(1, p->T.tally_bins);
     p->bank_a.reserve((size_t)std::max<int64_t>(1, kb * n) * sizeof(abx_particle));
@@ -1553,7 +1555,7 @@ void batch_step(abx_batch* b, std::vector<abx_problem*>& finished) {
   b->all_chunks.clear();
   int64_t total_all = 0;
   for (const auto& x : li) total_all += x.n;
-  const int64_t clen = chunk_length(total_all);
+  const int64_t clen = b->chunk > 0 ? std::max<int64_t>(64, b->chunk) : chunk_length(total_all);
   int g0 = 0;
   while (g0 < ni) {
     const int key = variant_key(b->items[li[order[g0]].entry].p);
@@ -1718,7 +1720,7 @@ int abx_batch_add(abx_batch* b, abx_problem* p, int nbatches) {
       if (it.p == p) fail(ABX_E_INVALID, "problem already attached");
     // the problem alone, plus the banks and generation arrays of batches
     // run side by side
-    const int kb = concurrent_batches(p, nbatches, b->step_hist);
+    const int kb = concurrent_batches(p, nbatches, b->step_hist, b->multi_hist);
     size_t need = problem_footprint(p);
     if (kb > 1)
       need += (size_t)(kb - 1) * ((size_t)p->T.st.nparticles * sizeof(abx_particle) +
@@ -1750,6 +1752,14 @@ int abx_batch_set_step_histories(abx_batch* b, int64_t histories) {
   ABX_TRY({
     if (!b || histories < 0) fail(ABX_E_INVALID, "step histories");
     b->step_hist = histories;
+  });
+}
+
+int abx_batch_set_params(abx_batch* b, int64_t chunk, int64_t multi_hist) {
+  ABX_TRY({
+    if (!b || chunk < 0 || multi_hist < 0) fail(ABX_E_INVALID, "batch parameters");
+    b->chunk = chunk;
+    b->multi_hist = multi_hist > 0 ? multi_hist : (int64_t(1) << 23);
   });
 }
 

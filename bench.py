@@ -179,6 +179,7 @@ def bench_batch(args):
     precision, shared persistent launches) against the same problems run one
     at a time through the FP64 device loop."""
     from abeille_amd import Batch, Problem, abi, synchronize
+    from abeille_amd.autotune import autotune_batch
     nb = 2
     split = [Problem(m) for m in _c4_problems(args.problems, abi.SPLIT)]
     # warm-up: a few problems through the batch (module load, grid sizing)
@@ -188,6 +189,13 @@ def bench_batch(args):
     while len(w):
         w.step()
     b = Batch()
+    # chunk length and side-by-side cap from the autotuner, on a sample of
+    # problems drawn from the same generator (not the timed set)
+    params = None
+    if not args.no_autotune:
+        sample = [Problem(m) for m in _c4_problems(64, abi.SPLIT, seed=4096)]
+        params, _ = autotune_batch(b, sample, nbatches=nb)
+        del sample
     synchronize()
     t0 = time.perf_counter()
     for p in split:
@@ -215,7 +223,7 @@ def bench_batch(args):
                   {"workload": f"{args.problems} slab fixed-source problems (1-8 regions, G in "
                                "{2,4,7,16,33}, 1e3-1e6 histories, DT/ST, 16^3 collision "
                                "tally), 2 batches each, batched split precision",
-                   "problems": args.problems},
+                   "problems": args.problems, "batch_params": params},
                   {"histories_per_s": hist / wall, "wall_s": wall, "batch_steps": steps,
                    "transport_kernel_s": kernel_s,
                    "plain_fp64_one_at_a_time": {"collisions_per_s": pc / pwall, "wall_s": pwall},

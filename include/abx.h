@@ -464,6 +464,14 @@ int abx_batch_step(abx_batch* b, abx_problem** finished, int cap, int* n_finishe
  * Every problem still gets at least 8192 histories per step, and a
  * modified-fixed-source generation always runs whole. */
 int abx_batch_set_step_histories(abx_batch* b, int64_t histories);
+/* Launch parameters of the batch engine, for the autotuner: histories per
+ * chunk of the batch kernel (a block's unit of work; 0 = automatic, about
+ * eight chunks per CU) and the histories that the side-by-side batches of
+ * one fixed-source problem may hold together (0 = 2^23; a problem whose
+ * batch alone exceeds it runs one batch per step).  The reference has no
+ * counterpart: its batches run one after another (FixedSource::run,
+ * src/fixed_source.cpp:81-173). */
+int abx_batch_set_params(abx_batch* b, int64_t chunk, int64_t multi_hist);
 /* GPU time of the batch's transport launches so far. */
 int abx_batch_kernel_seconds(abx_batch* b, double* seconds);
 /* Statistics of a problem's batch-engine run (or of its last abx_run). */

@@ -34,3 +34,34 @@ def test_autotune_caches_and_keeps_results(tmp_path):
     r = Problem(m)
     _, k_plain = r.run(4, 1)
     np.testing.assert_allclose(k_tuned, k_plain, rtol=1e-9)
+
+
+@pytest.mark.gpu
+def test_autotune_batch_picks_and_keeps_results():
+    """The batch-engine parameters change speed only: a batch run with the
+    chosen chunk length and side-by-side cap gives each problem its results
+    alone."""
+    from abeille_amd import Batch
+    from abeille_amd.autotune import BATCH_CANDIDATES, autotune_batch
+    rng = np.random.default_rng(5)
+    models = [problems.slab_problem(rng, nhist=int(rng.integers(500, 3000)))[0] for _ in range(6)]
+    sample = [Problem(m) for m in models[:3]]
+    cands = [BATCH_CANDIDATES[0], dict(chunk=300, multi_hist=1), dict(chunk=64, multi_hist=1 << 23)]
+    b = Batch()
+    params, times = autotune_batch(b, sample, nbatches=1, candidates=cands)
+    assert params in cands and len(times) == len(cands)
+    for c in cands:                      # every candidate keeps the results
+        b = Batch()
+        b.set_params(**c)
+        probs = [Problem(m) for m in models]
+        for p in probs:
+            assert b.add(p, 2)
+        while len(b):
+            b.step()
+        for m, p in zip(models, probs):
+            q = Problem(m)
+            st, _ = q.run(2, 0)
+            assert p.stats().diag.collisions == st.diag.collisions
+            for t in range(len(m.tallies)):
+                np.testing.assert_allclose(p.tally_stats(t)[0], q.tally_stats(t)[0], rtol=1e-8,
+                                           atol=1e-14)
