@@ -498,11 +498,9 @@ ABX_HD void trk_reset_position(const DevProblem& P, Trk& t, const V3& r, const V
 
 ABX_HD void trk_move(Trk& t, const V3& u, double d) {
   const V3 du = vscale(u, d);
-  int maxl = 0;
-  for (int i = 0; i < t.np; i++) {
-    const int l = pad_lvl(t.pad(i));
-    maxl = l > maxl ? l : maxl;
-  }
+  // pads are pushed in descent order, so their frame levels never decrease:
+  // the deepest frame in use is the last pad's
+  const int maxl = t.np > 0 ? pad_lvl(t.pad(t.np - 1)) : 0;
   for (int l = 0; l <= maxl && l < t.L; l++) t.set_lv(l, vadd(t.lv(l), du));
   t.token = 0;
 }
