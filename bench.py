@@ -26,9 +26,10 @@ sys.path.insert(0, ROOT)
 METRIC = "collisions/s & histories/s vs plain GPU path at 1% tally err; p50/p99 latency"
 HBM_PEAK_GBPS = 8000.0
 # algorithmic bytes (per event = read + write of the 80 B FP64 particle
-# record; per tally update = one FP64 read-modify-write; per history = the
-# 80 B source record)
+# record, 68 B in split precision with the direction in FP32; per tally
+# update = one FP64 read-modify-write; per history = the 80 B source record)
 BYTES_PER_EVENT = 160
+BYTES_PER_EVENT_SPLIT = 136
 BYTES_PER_TALLY_UPDATE = 16
 BYTES_PER_HISTORY = 80
 
@@ -70,9 +71,10 @@ def measured_traffic(kernel, workload_name, precision):
     return None
 
 
-def algorithmic_bytes(diag):
+def algorithmic_bytes(diag, split=False):
     events = diag.flights + diag.collisions
-    return (events * BYTES_PER_EVENT + diag.tally_updates * BYTES_PER_TALLY_UPDATE +
+    per_event = BYTES_PER_EVENT_SPLIT if split else BYTES_PER_EVENT
+    return (events * per_event + diag.tally_updates * BYTES_PER_TALLY_UPDATE +
             diag.histories * BYTES_PER_HISTORY)
 
 
@@ -91,10 +93,18 @@ def cpu_baseline(name, tracking, seconds, nthreads):
         coll += st.diag.collisions
         hist += st.diag.histories
         gens += 2
+    # one thread, on a shorter sample
+    g1, c1, t1 = 0, 0, 0.0
+    while t1 < max(1.0, seconds / 5) and g1 < 40:
+        st, _ = o.run(1, 0, 1)
+        t1 += st.transport_seconds
+        c1 += st.diag.collisions
+        g1 += 1
     return {"value": coll / t_used, "unit": "collisions/s", "cores": nthreads, "kind": "port",
             "sample": f"{gens} generations x {n} particles of the same problem "
                       f"({t_used:.1f} s of transport, glibc libm)",
-            "histories_per_s": hist / t_used}
+            "histories_per_s": hist / t_used,
+            "one_thread_collisions_per_s": c1 / t1 if t1 > 0 else None}
 
 
 def _common(args, metric_value, unit, ms_per_step, dtype, config, extra):
@@ -395,15 +405,17 @@ def main():
     coll_s = d.collisions / wall
     hist_s = st.histories / wall
     kernel_avg = st.kernel_seconds / max(1, st.kernel_launches)
+    split = args.precision == "split"
     # algorithmic bytes of the dominant kernel per launch: the events it
     # processed (all flights) over its launches
     if args.engine == "history":
         # one launch transports one generation: all its events, tally updates
         # and histories
-        bytes_per_launch = algorithmic_bytes(d) / max(1, st.kernel_launches)
+        bytes_per_launch = algorithmic_bytes(d, split) / max(1, st.kernel_launches)
     else:
         # the flight kernel: the events it processed
-        bytes_per_launch = st.kernel_items * BYTES_PER_EVENT / max(1, st.kernel_launches)
+        bytes_per_launch = (st.kernel_items * (BYTES_PER_EVENT_SPLIT if split else BYTES_PER_EVENT)
+                            / max(1, st.kernel_launches))
     achieved = bytes_per_launch / kernel_avg / 1e9 if kernel_avg > 0 else 0.0
     kname = "k_ev_flight" if args.engine == "event" else "k_history"
     traffic = measured_traffic(kname, args.workload, args.precision)
@@ -439,7 +451,7 @@ def main():
             "launches": st.kernel_launches,
             "algorithmic_bytes_per_launch": bytes_per_launch,
         },
-        "whole_job_algorithmic_GBps": algorithmic_bytes(d) / wall / 1e9,
+        "whole_job_algorithmic_GBps": algorithmic_bytes(d, split) / wall / 1e9,
     }
     if not args.no_cpu:
         nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
