@@ -246,12 +246,14 @@ __device__ __forceinline__ void bind_lds_cursor(Trk& t, const DevProblem& P) {
 }
 
 // History-based transport on a persistent grid.  Each lane carries one
-// history at a time and, when it ends, takes the next one from a global
-// counter (one atomic per wavefront for all its idle lanes), so lanes stay
-// busy until the bank is drained instead of idling behind the longest
-// history of their wavefront.  Per-history results do not depend on which
-// lane runs a history: the RNG stream is the history's own and fission
-// sites are ordered afterwards by (parent, daughter).
+// history at a time and, when it ends, takes the next one from its
+// wavefront's private range of history ids, which the wavefront refills
+// ABX_WAVE_GRAB ids at a time from a global counter; idle lanes are refilled
+// together once ABX_REFILL_MIN of them wait.  Lanes stay busy until the
+// bank is drained instead of idling behind the longest history of their
+// wavefront.  Per-history results do not depend on which lane runs a
+// history: the RNG stream is the history's own and fission sites are
+// ordered afterwards by (parent, daughter).
 #ifndef ABX_WAVE_GRAB
 #define ABX_WAVE_GRAB 64ull
 #endif
