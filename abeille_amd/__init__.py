@@ -71,6 +71,7 @@ def lib():
         L.abx_batch_kernel_seconds.argtypes = [P, C.POINTER(C.c_double)]
         L.abx_batch_set_step_histories.argtypes = [P, C.c_int64]
         L.abx_batch_set_params.argtypes = [P, C.c_int64, C.c_int64]
+        L.abx_batch_need.argtypes = [P, P, C.c_int, C.POINTER(C.c_size_t)]
         L.abx_problem_stats.argtypes = [P, C.POINTER(abi.RunStats)]
         _lib = L
     return _lib
@@ -254,6 +255,12 @@ class Batch:
         _check(rc)
         self.attached[problem.h.value] = problem
         return True
+
+    def need(self, problem, nbatches):
+        """Device bytes `add` would count for `problem` run `nbatches` batches."""
+        x = C.c_size_t()
+        _check(lib().abx_batch_need(self.h, problem.h, int(nbatches), C.byref(x)))
+        return x.value
 
     def step(self):
         """One step; returns the problems that finished in it."""

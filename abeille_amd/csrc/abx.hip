@@ -1461,6 +1461,18 @@ struct LaunchItem {
   double* gen;
 };
 
+// Device bytes a problem holds while attached to b for `nbatches` batches:
+// the problem alone, plus the banks and generation arrays of the batches it
+// runs side by side.
+size_t batch_need(const abx_batch* b, const abx_problem* p, int nbatches) {
+  const int kb = concurrent_batches(p, nbatches, b->step_hist, b->multi_hist);
+  size_t need = problem_footprint(p);
+  if (kb > 1)
+    need += (size_t)(kb - 1) * ((size_t)p->T.st.nparticles * sizeof(abx_particle) +
+                                (size_t)std::max<int64_t>(1, p->T.tally_bins) * sizeof(double));
+  return need;
+}
+
 void batch_step(abx_batch* b, std::vector<abx_problem*>& finished) {
   const int ne = (int)b->items.size();
   if (ne == 0) return;
@@ -1711,6 +1723,13 @@ int abx_problem_footprint(abx_problem* p, size_t* bytes) {
   });
 }
 
+int abx_batch_need(abx_batch* b, abx_problem* p, int nbatches, size_t* bytes) {
+  ABX_TRY({
+    if (!b || !p || !bytes || nbatches < 1) fail(ABX_E_INVALID, "batch arguments");
+    *bytes = batch_need(b, p, nbatches);
+  });
+}
+
 int abx_batch_add(abx_batch* b, abx_problem* p, int nbatches) {
   ABX_TRY({
     if (!b || !p || nbatches < 1) fail(ABX_E_INVALID, "batch arguments");
@@ -1718,13 +1737,7 @@ int abx_batch_add(abx_batch* b, abx_problem* p, int nbatches) {
       fail(ABX_E_INVALID, "k-eigenvalue problems run with abx_run, not in a batch");
     for (const auto& it : b->items)
       if (it.p == p) fail(ABX_E_INVALID, "problem already attached");
-    // the problem alone, plus the banks and generation arrays of batches
-    // run side by side
-    const int kb = concurrent_batches(p, nbatches, b->step_hist, b->multi_hist);
-    size_t need = problem_footprint(p);
-    if (kb > 1)
-      need += (size_t)(kb - 1) * ((size_t)p->T.st.nparticles * sizeof(abx_particle) +
-                                  (size_t)std::max<int64_t>(1, p->T.tally_bins) * sizeof(double));
+    const size_t need = batch_need(b, p, nbatches);
     if (b->budget && b->used + need > b->budget)
       fail(ABX_E_BUDGET, "problem does not fit the batch's device budget");
     run_reset(p);
@@ -1751,6 +1764,7 @@ int abx_batch_step(abx_batch* b, abx_problem** finished, int cap, int* n_finishe
 int abx_batch_set_step_histories(abx_batch* b, int64_t histories) {
   ABX_TRY({
     if (!b || histories < 0) fail(ABX_E_INVALID, "step histories");
+    if (!b->items.empty()) fail(ABX_E_INVALID, "set the step cap before attaching problems");
     b->step_hist = histories;
   });
 }
@@ -1758,6 +1772,9 @@ int abx_batch_set_step_histories(abx_batch* b, int64_t histories) {
 int abx_batch_set_params(abx_batch* b, int64_t chunk, int64_t multi_hist) {
   ABX_TRY({
     if (!b || chunk < 0 || multi_hist < 0) fail(ABX_E_INVALID, "batch parameters");
+    // attached problems were admitted (budget) with the concurrency these
+    // parameters decide
+    if (!b->items.empty()) fail(ABX_E_INVALID, "set batch parameters before attaching problems");
     b->chunk = chunk;
     b->multi_hist = multi_hist > 0 ? multi_hist : (int64_t(1) << 23);
   });

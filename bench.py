@@ -311,7 +311,8 @@ def bench_serve(args):
                   {"histories_per_s": hist / wall, "wall_s": wall, "arrival_span_s": span,
                    "latency_s": pct, "steps": srv.steps,
                    "step_wall_s": srv.step_seconds,
-                   "transport_kernel_s": srv.batch.kernel_seconds(),
+                   "free_s": {"total": srv.free_seconds, "max": srv.free_max},
+                   "transport_kernel_s": srv.kernel_seconds(),
                    "carter_requests": int(sum(k == "carter" for k in kinds)),
                    "fixed_source_requests": {
                        "served_latency_s": pct_fixed, "served_collisions": coll_fixed,

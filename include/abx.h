@@ -452,6 +452,10 @@ int abx_problem_footprint(abx_problem* p, size_t* bytes);
 /* Attach p to run `nbatches` batches of its nparticles (statistics and
  * tallies restart, as in abx_run).  ABX_E_BUDGET if it does not fit. */
 int abx_batch_add(abx_batch* b, abx_problem* p, int nbatches);
+/* Device bytes abx_batch_add would count against the budget for p (the
+ * problem's footprint plus its side-by-side batches), for callers that
+ * share one budget between several batches. */
+int abx_batch_need(abx_batch* b, abx_problem* p, int nbatches, size_t* bytes);
 /* One step.  Problems whose last batch was recorded are detached and listed
  * in `finished` (capacity cap, count in *n_finished); *n_active receives the
  * number still attached. */
@@ -463,7 +467,7 @@ int abx_batch_step(abx_batch* b, abx_problem** finished, int cap, int* n_finishe
  * unchanged - so that short steps let newly attached problems start soon.
  * Every problem still gets at least 8192 histories per step, and a
  * modified-fixed-source generation always runs whole. */
-int abx_batch_set_step_histories(abx_batch* b, int64_t histories);
+int abx_batch_set_step_histories(abx_batch* b, int64_t histories);  /* before any abx_batch_add */
 /* Launch parameters of the batch engine, for the autotuner: histories per
  * chunk of the batch kernel (a block's unit of work; 0 = automatic, about
  * eight chunks per CU) and the histories that the side-by-side batches of
@@ -471,7 +475,7 @@ int abx_batch_set_step_histories(abx_batch* b, int64_t histories);
  * batch alone exceeds it runs one batch per step).  The reference has no
  * counterpart: its batches run one after another (FixedSource::run,
  * src/fixed_source.cpp:81-173). */
-int abx_batch_set_params(abx_batch* b, int64_t chunk, int64_t multi_hist);
+int abx_batch_set_params(abx_batch* b, int64_t chunk, int64_t multi_hist);  /* before any add */
 /* GPU time of the batch's transport launches so far. */
 int abx_batch_kernel_seconds(abx_batch* b, double* seconds);
 /* Statistics of a problem's batch-engine run (or of its last abx_run). */

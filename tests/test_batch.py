@@ -118,3 +118,17 @@ def test_sliced_batches_equal_alone():
             ab, eb = probs[name].tally_stats(t)
             np.testing.assert_allclose(ab, aa, rtol=1e-8, atol=1e-14, err_msg=name)
             np.testing.assert_allclose(eb, ea, rtol=1e-6, atol=1e-12, err_msg=name)
+
+
+def test_batch_parameters_only_before_attaching():
+    b = Batch()
+    b.set_params(chunk=2048, multi_hist=1 << 20)
+    b.set_step_histories(0)
+    p = Problem(problems.fixed_source_cube(1000))
+    assert b.add(p, 1)
+    with pytest.raises(Exception):
+        b.set_params(chunk=1024)
+    with pytest.raises(Exception):
+        b.set_step_histories(4096)
+    while len(b):
+        b.step()

@@ -1,7 +1,9 @@
 """Served stream (abeille_amd.serve): continuous batching under an HBM
-budget.  Each request's result in the mix equals its result alone (same
-precision, abx_run on a fresh problem) up to tally summation order, and the
-budget is honoured (requests wait and are admitted as memory frees up)."""
+budget, in one lane or in several (fixed-source lanes and a chain lane,
+each stepped by a thread of its own).  Each request's result in the mix equals its result
+alone (same precision, abx_run on a fresh problem) up to tally summation
+order, and the budget is honoured (requests wait and are admitted as
+memory frees up)."""
 import numpy as np
 import pytest
 
@@ -10,7 +12,8 @@ from abeille_amd.serve import Request, Server, serial_latencies
 
 
 @pytest.mark.gpu
-def test_served_results_equal_alone():
+@pytest.mark.parametrize("lanes", [3, 1])
+def test_served_results_equal_alone(lanes):
     rng = np.random.default_rng(99)
     reqs = []
     t = 0.0
@@ -24,7 +27,7 @@ def test_served_results_equal_alone():
         reqs.append(Request(m, 2, t, name=f"{kind}{len(reqs)}"))
     assert any(r.name.startswith("carter") for r in reqs)
     foot = max(Problem(r.model).footprint() for r in reqs)
-    srv = Server(budget=3 * foot)
+    srv = Server(budget=3 * foot, lanes=lanes)
     srv.run(reqs)
     waited = sum(r.admitted > r.arrival + 1e-3 for r in reqs)
     assert waited > 0, "the budget never made a request wait"
