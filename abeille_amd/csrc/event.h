@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
+#include <atomic>
 #include <vector>
 
 #include "kernels.h"
@@ -468,17 +469,19 @@ static int64_t resident_blocks(Kern kernel, int bs, size_t lds) {
 
 // LDS one block may allocate on the current device (160 KB on gfx950).
 static size_t lds_block_limit() {
-  static size_t cached[64] = {};
+  static std::atomic<size_t> cached[64];  // per device; batches may step from several threads
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 65536;
-  if (!cached[dev]) {
+  size_t c = cached[dev].load(std::memory_order_relaxed);
+  if (!c) {
     int v = 0;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
         v <= 0)
       v = 65536;
-    cached[dev] = (size_t)v;
+    c = (size_t)v;
+    cached[dev].store(c, std::memory_order_relaxed);
   }
-  return cached[dev];
+  return c;
 }
 
 // Whether a kernel's LDS-staged variant is worth launching: the tables fit
