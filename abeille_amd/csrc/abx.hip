@@ -879,7 +879,6 @@ static bool run_generations_graph(abx_problem* p, int ngenerations, int nignored
     hipLaunchKernelGGL(k_order_bank_dev, dim3(gcap), dim3(256), 0, s, p->stage.as<abx_banked>(),
                        p->stage_parent.as<uint32_t>(), p->stage_daughter.as<uint32_t>(),
                        p->offsets.as<uint64_t>(), g, p->ordered.as<abx_banked>());
-    hipLaunchKernelGGL(k_weight_sums_dev, dim3(gcap), dim3(256), 0, s, p->ordered.as<abx_banked>(), g);
     hipLaunchKernelGGL(k_gen_end, dim3(1), dim3(64), 0, s, g, W, conv, p->khist_dev.as<double>());
     hipLaunchKernelGGL(k_rebank_dev, dim3(gcap), dim3(256), 0, s, p->ordered.as<abx_banked>(), g,
                        T.st.seed, T.st.stride, out);

@@ -772,32 +772,29 @@ __global__ void k_add_block_offsets_dev(uint64_t* out, const uint64_t* block_off
   if (i < (int64_t)*n_dev) out[i] += block_off[blockIdx.x];
 }
 
+// Scatters the staged sites into parent order and sums their positive and
+// negative weights (normalize_weights, src/power_iterator.cpp:538-586):
+// grid-stride over the staging slots, one pair of atomics per block.
 __global__ void k_order_bank_dev(const abx_banked* stage, const uint32_t* parent,
                                  const uint32_t* daughter, const uint64_t* offsets,
-                                 const GenDev* g, abx_banked* out) {
-  const int64_t m = (int64_t)(g->n_stage < g->cap ? g->n_stage : g->cap);
-  if (g->err) return;
-  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m;
-       k += (int64_t)gridDim.x * blockDim.x) {
-    const uint32_t q = parent[k];
-    if (q == kHole) continue;
-    const uint64_t dst = offsets[q] + daughter[k];
-    if (dst < g->cap) out[dst] = stage[k];  // beyond: k_gen_end reports the overflow
-  }
-}
-
-// Grid-stride over the live sites; one pair of atomics per block.
-__global__ void k_weight_sums_dev(const abx_banked* b, GenDev* g) {
+                                 GenDev* g, abx_banked* out) {
   __shared__ double part[2][16];
-  const int64_t n = (int64_t)(g->n_sites < g->cap ? g->n_sites : g->cap);
+  const int64_t m = (int64_t)(g->n_stage < g->cap ? g->n_stage : g->cap);
   double wp = 0., wn = 0.;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const double w = b[i].wgt;
-    if (w > 0.)
-      wp += w;
-    else
-      wn -= w;
+  if (!g->err) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m;
+         k += (int64_t)gridDim.x * blockDim.x) {
+      const uint32_t q = parent[k];
+      if (q == kHole) continue;
+      const uint64_t dst = offsets[q] + daughter[k];
+      if (dst >= g->cap) continue;  // k_gen_end reports the overflow
+      const abx_banked b = stage[k];
+      out[dst] = b;
+      if (b.wgt > 0.)
+        wp += b.wgt;
+      else
+        wn -= b.wgt;
+    }
   }
   wp = wave_sum(wp);
   wn = wave_sum(wn);
