@@ -80,7 +80,10 @@ def algorithmic_bytes(diag, split=False):
 
 def cpu_baseline(name, tracking, seconds, nthreads):
     """Time the CPU restatement (oracle/) on a bounded sample: generations of
-    a scaled-down bank of the same problem until about `seconds` of work."""
+    a scaled-down bank of the same problem until about `seconds` of work.
+    Threads are pinned close to each other (OMP_PROC_BIND=close, read when
+    the OpenMP runtime loads with the oracle)."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
     from oracle.pyoracle import Oracle
     n = 100000
     m, _ = workload(name, n, tracking, "fp64")
@@ -104,7 +107,19 @@ def cpu_baseline(name, tracking, seconds, nthreads):
             "sample": f"{gens} generations x {n} particles of the same problem "
                       f"({t_used:.1f} s of transport, glibc libm)",
             "histories_per_s": hist / t_used,
-            "one_thread_collisions_per_s": c1 / t1 if t1 > 0 else None}
+            "one_thread_collisions_per_s": c1 / t1 if t1 > 0 else None,
+            "cpu_model": _cpu_model(), "omp_proc_bind": os.environ.get("OMP_PROC_BIND")}
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def _common(args, metric_value, unit, ms_per_step, dtype, config, extra):
