@@ -1516,8 +1516,8 @@ void batch_step(abx_batch* b, std::vector<abx_problem*>& finished) {
     if (it.mid_chain) continue;
     abx_problem* p = it.p;
     const int f = first_item[j];
-    const int64_t n = (j + 1 < ne ? first_item[j + 1] : ni) - f;  // items of the entry
-    const int64_t total = n * it.n;
+    const int64_t items = (j + 1 < ne ? first_item[j + 1] : ni) - f;  // the entry's batches
+    const int64_t total = items * it.n;
     if (total > 0) {
       hipLaunchKernelGGL(k_sample_sources, dim3((unsigned)blocks_for(total, 256)), dim3(256),
                          (size_t)256 * trk_bytes(p->P.trk_D, p->P.trk_L), b->stream, p->P, p->S,
