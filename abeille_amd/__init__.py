@@ -189,15 +189,17 @@ class Problem:
         assert b.ndim == 2 and b.shape[1] == 9
         _check(lib().abx_bank_load(self.h, b.ctypes.data, len(b)))
 
-    def set_tuning(self, block_size=256, blocks_per_cu=0, stage_tables=1, tail_threshold=0):
-        t = abi.Tuning(block_size, blocks_per_cu, stage_tables, tail_threshold)
+    def set_tuning(self, block_size=256, blocks_per_cu=0, stage_tables=1, tail_threshold=0,
+                   sort_every=0):
+        t = abi.Tuning(block_size, blocks_per_cu, stage_tables, tail_threshold, sort_every)
         _check(lib().abx_set_tuning(self.h, C.byref(t)))
 
     def tuning(self):
         t = abi.Tuning()
         _check(lib().abx_get_tuning(self.h, C.byref(t)))
         return dict(block_size=t.block_size, blocks_per_cu=t.blocks_per_cu,
-                    stage_tables=t.stage_tables, tail_threshold=t.tail_threshold)
+                    stage_tables=t.stage_tables, tail_threshold=t.tail_threshold,
+                    sort_every=t.sort_every)
 
     def time_transport(self, n, converged=True):
         """Device seconds of one transport of n fresh histories (abx_time_transport)."""

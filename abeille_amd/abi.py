@@ -248,5 +248,6 @@ def tokenize_region(region, surface_index_of_id):
 
 class Tuning(C.Structure):
     _fields_ = [("block_size", C.c_int32), ("blocks_per_cu", C.c_int32),
-                ("stage_tables", C.c_int32), ("tail_threshold", C.c_int32)]
+                ("stage_tables", C.c_int32), ("tail_threshold", C.c_int32),
+                ("sort_every", C.c_int32)]
 

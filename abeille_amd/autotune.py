@@ -19,8 +19,13 @@ import time
 
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "abeille_amd", "tuning.json")
 
-CANDIDATES = [dict(block_size=bs, blocks_per_cu=bpc, stage_tables=st, tail_threshold=0)
+CANDIDATES = [dict(block_size=bs, blocks_per_cu=bpc, stage_tables=st, tail_threshold=0,
+                   sort_every=0)
               for bs in (256, 128, 64) for bpc in (0, 4, 2) for st in (1, 0)]
+# the event engine: sort cadence of the collision queue x tail threshold
+EVENT_CANDIDATES = [dict(block_size=256, blocks_per_cu=0, stage_tables=1, tail_threshold=tt,
+                         sort_every=se)
+                    for se in (0, 1, 2, 4) for tt in (0, 65536)]
 
 
 def problem_class(model):
@@ -45,6 +50,8 @@ def autotune(problem, model, sample=200_000, repeats=2, cache=DEFAULT_CACHE,
     """Pick and apply the fastest tuning for `problem`; returns (tuning,
     {candidate index: seconds})."""
     key = problem_class(model)
+    if candidates is not CANDIDATES:   # another search space, another entry
+        key += "-" + hashlib.sha1(json.dumps(candidates, sort_keys=True).encode()).hexdigest()[:8]
     db = _load(cache) if cache else {}
     if key in db and not force:
         problem.set_tuning(**db[key]["tuning"])

@@ -70,7 +70,7 @@ struct abx_problem {
   DevSources S{};
   hipStream_t stream = nullptr;
   int engine = ABX_ENGINE_HISTORY;
-  abx_tuning tuning{256, 0, 1, 0};
+  abx_tuning tuning{256, 0, 1, 0, 0};
   int device = 0;
   // tallies
   DBuf tally_gen, tally_avg, tally_var;
@@ -572,9 +572,11 @@ int abx_set_tuning(abx_problem* p, const abx_tuning* t) {
     if (!p || !t) fail(ABX_E_INVALID, "null argument");
     if (t->block_size != 64 && t->block_size != 128 && t->block_size != 256)
       fail(ABX_E_INVALID, "block_size must be 64, 128 or 256");
-    if (t->blocks_per_cu < 0 || t->tail_threshold < 0) fail(ABX_E_INVALID, "tuning values");
+    if (t->blocks_per_cu < 0 || t->tail_threshold < 0 || t->sort_every < 0)
+      fail(ABX_E_INVALID, "tuning values");
     p->tuning = *t;
     p->ev.tail_threshold = t->tail_threshold;
+    p->ev.sort_every = t->sort_every;
   });
 }
 

@@ -25,6 +25,10 @@
 
 #include "kernels.h"
 
+#ifndef ABX_KERNELS_ONLY
+#include <rocprim/device/device_radix_sort.hpp>
+#endif
+
 // Structure-of-arrays particle + cursor state, one slot per history.
 struct EvState {
   double *rx, *ry, *rz, *ux, *uy, *uz, *E, *w, *w2, *bx, *by, *bz;
@@ -282,6 +286,17 @@ __global__ void ABX_TKERNEL k_ev_tail(DevProblem P, KCtx K, EvState S,
   flush_scores(K, sc, s.c);
 }
 
+// Sort keys of the collision queue: (material, group) of each queued slot,
+// so that a sorted queue hands each wavefront particles of one material
+// and group (the same table rows, the same fissile / non-fissile branch).
+__global__ void __launch_bounds__(256) k_ev_sort_keys(EvState S, const int32_t* q, int64_t n,
+                                                      uint16_t* keys) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int32_t slot = q[k];
+  keys[k] = (uint16_t)(((S.m[slot] & 0xff) << 8) | (S.g[slot] & 0xff));
+}
+
 // Host-side launch code follows; ABX_KERNELS_ONLY leaves it out (a
 // translation unit that only instantiates chosen kernels).
 #ifndef ABX_KERNELS_ONLY
@@ -296,6 +311,11 @@ struct EventBuffers {
   unsigned* h_counts = nullptr;   // pinned host mirror
   std::vector<hipEvent_t> evs;
   int64_t tail_threshold = 0;     // 0 = automatic
+  int32_t sort_every = 0;         // sort the collision queue every n-th iteration (0 = never)
+  uint16_t* keys[2] = {nullptr, nullptr};
+  int32_t* q_sorted = nullptr;
+  void* sort_temp = nullptr;
+  size_t sort_temp_bytes = 0;
   int64_t iterations = 0;
   int64_t flight_items = 0;       // particles processed by k_ev_flight (last call)
   void release() {
@@ -304,6 +324,14 @@ struct EventBuffers {
       if (x) (void)hipFree(x);
     if (counts) (void)hipFree(counts);
     if (h_counts) (void)hipHostFree(h_counts);
+    for (auto& x : keys)
+      if (x) (void)hipFree(x);
+    if (q_sorted) (void)hipFree(q_sorted);
+    if (sort_temp) (void)hipFree(sort_temp);
+    keys[0] = keys[1] = nullptr;
+    q_sorted = nullptr;
+    sort_temp = nullptr;
+    sort_temp_bytes = 0;
     for (auto e : evs) (void)hipEventDestroy(e);
     evs.clear();
     state = nullptr;
@@ -367,7 +395,28 @@ static hipError_t event_transport_t(const DevProblem& P, const KCtx& K, EventBuf
       x = nullptr;
       if ((e = hipMalloc(&x, (size_t)n * sizeof(int32_t))) != hipSuccess) return e;
     }
+    for (auto& x : B.keys) {
+      if (x) (void)hipFree(x);
+      x = nullptr;
+    }
+    if (B.q_sorted) (void)hipFree(B.q_sorted);
+    B.q_sorted = nullptr;
     B.q_cap = (size_t)n;
+  }
+  if (B.sort_every > 0 && !B.q_sorted) {
+    for (auto& x : B.keys)
+      if ((e = hipMalloc(&x, B.q_cap * sizeof(uint16_t))) != hipSuccess) return e;
+    if ((e = hipMalloc(&B.q_sorted, B.q_cap * sizeof(int32_t))) != hipSuccess) return e;
+    size_t tb = 0;
+    if ((e = rocprim::radix_sort_pairs(nullptr, tb, B.keys[0], B.keys[1], B.q[2], B.q_sorted,
+                                       B.q_cap, 0, 16, s)) != hipSuccess)
+      return e;
+    if (tb > B.sort_temp_bytes) {
+      if (B.sort_temp) (void)hipFree(B.sort_temp);
+      B.sort_temp = nullptr;
+      if ((e = hipMalloc(&B.sort_temp, tb)) != hipSuccess) return e;
+      B.sort_temp_bytes = tb;
+    }
   }
   if (!B.counts) {
     if ((e = hipMalloc(&B.counts, 8 * sizeof(unsigned))) != hipSuccess) return e;
@@ -411,8 +460,20 @@ static hipError_t event_transport_t(const DevProblem& P, const KCtx& K, EventBuf
     // collisions first (they refill the flight queue), then flights
     if ((e = hipMemsetAsync(cb, 0, sizeof(unsigned), s)) != hipSuccess) return e;
     if (nc > 0) {
+      // the collision queue, sorted by (material, group) on the chosen cadence
+      const int32_t* qcol = qc;
+      if (B.sort_every > 0 && iters % B.sort_every == 0 && nc > 1) {
+        hipLaunchKernelGGL(k_ev_sort_keys, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, S,
+                           qc, nc, B.keys[0]);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        size_t tb = B.sort_temp_bytes;
+        if ((e = rocprim::radix_sort_pairs(B.sort_temp, tb, B.keys[0], B.keys[1], qc, B.q_sorted,
+                                           (size_t)nc, 0, 16, s)) != hipSuccess)
+          return e;
+        qcol = B.q_sorted;
+      }
       hipLaunchKernelGGL((k_ev_collide<TRK, TRACE, SEC, SPLIT>), dim3((unsigned)((nc + bs - 1) / bs)),
-                         dim3(bs), lds, s, P, K, S, qc, cc, qa, ca);
+                         dim3(bs), lds, s, P, K, S, qcol, cc, qa, ca);
       if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     const int64_t nf_all = nf + nc;  // upper bound on the flight queue

@@ -407,9 +407,13 @@ def main():
         bs, bpc, st = (int(x) for x in args.tuning.split(","))
         prob.set_tuning(block_size=bs, blocks_per_cu=bpc, stage_tables=st)
         tuning = prob.tuning()
-    elif not args.no_autotune and engine == abi.ENGINE_HISTORY:
-        from abeille_amd.autotune import autotune
-        tuning, _ = autotune(prob, m, sample=nparticles, cache=None)
+    elif not args.no_autotune:
+        from abeille_amd.autotune import EVENT_CANDIDATES, autotune
+        if engine == abi.ENGINE_HISTORY:
+            tuning, _ = autotune(prob, m, sample=nparticles, cache=None)
+        else:   # the event engine: collision-queue sort cadence and tail threshold
+            tuning, _ = autotune(prob, m, sample=nparticles, cache=None,
+                                 candidates=EVENT_CANDIDATES)
     # warmup: inactive generations (also what converges the fission source)
     prob.run(max(1, args.warmup), nignored=max(1, args.warmup) if kmode else 0)
     synchronize()

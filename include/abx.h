@@ -303,6 +303,9 @@ typedef struct abx_tuning {
                              fit (default), 0 = read them through L2 */
   int32_t tail_threshold; /* event engine: particles below which the rest of
                              their histories run in-lane (0 = automatic) */
+  int32_t sort_every;     /* event engine: sort the collision queue by
+                             (material, group) before every n-th collision
+                             launch (0 = never, the default) */
 } abx_tuning;
 
 const char* abx_last_error(void);

@@ -25,7 +25,7 @@ def test_autotune_caches_and_keeps_results(tmp_path):
     cands = CANDIDATES[:4]
     best, times = autotune(p, m, sample=20000, repeats=1, cache=str(cache), candidates=cands)
     assert best in cands and len(times) == len(cands)
-    assert problem_class(m) in json.loads(cache.read_text())
+    assert any(k.startswith(problem_class(m)) for k in json.loads(cache.read_text()))
     q = Problem(m)
     best2, times2 = autotune(q, m, cache=str(cache), candidates=cands)
     assert best2 == best and times2 == {}

@@ -33,10 +33,25 @@ CASES = {
 @pytest.mark.parametrize("engine", [abi.ENGINE_EVENT, abi.ENGINE_HISTORY], ids=["event", "history"])
 def test_trace_bit_exact(name, engine):
     print(f"[parity] case={name} engine={engine}", flush=True)
-    m = CASES[name]()
+    _trace_bit_exact(CASES[name](), engine)
+
+
+@pytest.mark.parametrize("name", ["lattice_dt", "core_st", "slab_mfs_carter", "cube_st"])
+def test_trace_bit_exact_event_sorted(name):
+    """The event engine with its collision queue sorted by (material, group)
+    before every collision launch, and no in-lane tail: the order in which
+    particles are processed does not change any history."""
+    print(f"[parity] case={name} event engine, sorted queue", flush=True)
+    _trace_bit_exact(CASES[name](), abi.ENGINE_EVENT,
+                     dict(tail_threshold=1, sort_every=1))
+
+
+def _trace_bit_exact(m, engine, tuning=None):
     n, cap = 2000, 512
     o = Oracle(m)
     g = Problem(m, engine=engine)
+    if tuning:
+        g.set_tuning(**tuning)
     so = o.sample_sources(0, n)
     sg = g.sample_sources(0, n)
     assert so.tobytes() == sg.tobytes(), "source particles differ"
