@@ -21,10 +21,11 @@ ERRORS = {
 ERRORS_BY_NAME = {v: k for k, v in ERRORS.items()}
 
 # surfaces
-XPLANE, YPLANE, ZPLANE, PLANE, XCYL, YCYL, ZCYL, SPHERE = range(8)
+XPLANE, YPLANE, ZPLANE, PLANE, XCYL, YCYL, ZCYL, SPHERE, CYL = range(9)
 SURFACE_TYPES = {
     "xplane": XPLANE, "yplane": YPLANE, "zplane": ZPLANE, "plane": PLANE,
     "xcylinder": XCYL, "ycylinder": YCYL, "zcylinder": ZCYL, "sphere": SPHERE,
+    "cylinder": CYL,
 }
 VACUUM, REFLECTIVE, NORMAL = 0, 1, 2
 OP_UNIN = 2147483647 - 4
@@ -53,7 +54,7 @@ PI32 = C.POINTER(C.c_int32)
 
 class Surface(C.Structure):
     _fields_ = [("type", C.c_int32), ("boundary", C.c_int32), ("id", C.c_uint32),
-                ("pad_", C.c_int32), ("p", C.c_double * 4)]
+                ("pad_", C.c_int32), ("p", C.c_double * 8)]
 
 
 class Cell(C.Structure):

@@ -82,11 +82,40 @@ static inline void build_tables(const abx_problem_desc* d, HostTables& T) {
   tables_require(d->n_surfaces >= 1, "no surfaces");
   for (int i = 0; i < d->n_surfaces; i++) {
     const abx_surface& a = d->surfaces[i];
-    tables_require(a.type >= 0 && a.type <= 7, "surface type");
+    tables_require(a.type >= 0 && a.type <= 8, "surface type");
     tables_require(a.boundary >= 0 && a.boundary <= 2, "surface boundary");
     DSurf S{};
     for (int k = 0; k < 4; k++) S.p[k] = a.p[k];
     S.type = a.type;
+    if (a.type == ABX_CYL) {
+      // Cylinder (src/cylinder.cpp:30-117) is the quadric
+      // alpha x^2 + beta y^2 + gamma z^2 = R^2 with alpha = 1 - u0^2 etc. of
+      // the normalised axis.  Along a coordinate axis two of alpha, beta,
+      // gamma are exactly 1 and the third exactly 0, and its sign, distance
+      // and normal evaluate bit for bit as the x/y/z-cylinder's (the zero
+      // terms add nothing, the others add in either order), so it is
+      // stored as one.  Off the axes the reference's quadric is not a
+      // circular cylinder; the device path does not carry it.
+      double u0 = a.p[3], v0 = a.p[4], w0 = a.p[5];
+      const double mag = sqrt(u0 * u0 + v0 * v0 + w0 * w0);
+      tables_require(mag != 0., "Direction vector of Cylinder can not have a mangitude of 0.");
+      u0 /= mag;
+      v0 /= mag;
+      w0 /= mag;
+      const double R = a.p[6];
+      if (v0 == 0. && w0 == 0.) {
+        S.type = ABX_XCYL;
+        S.p[0] = a.p[1], S.p[1] = a.p[2], S.p[2] = R, S.p[3] = 0.;
+      } else if (u0 == 0. && w0 == 0.) {
+        S.type = ABX_YCYL;
+        S.p[0] = a.p[0], S.p[1] = a.p[2], S.p[2] = R, S.p[3] = 0.;
+      } else if (u0 == 0. && v0 == 0.) {
+        S.type = ABX_ZCYL;
+        S.p[0] = a.p[0], S.p[1] = a.p[1], S.p[2] = R, S.p[3] = 0.;
+      } else {
+        tables_require(false, "Cylinder: only an axis along x, y or z is supported on the GPU path");
+      }
+    }
     S.bc = a.boundary;
     T.surf.push_back(S);
   }

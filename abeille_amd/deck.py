@@ -114,6 +114,7 @@ SURFACE_PARAMS = {
     "plane": ["A", "B", "C", "D"],
     "xcylinder": ["y0", "z0", "r"], "ycylinder": ["x0", "z0", "r"],
     "zcylinder": ["x0", "y0", "r"], "sphere": ["x0", "y0", "z0", "r"],
+    "cylinder": ["x0", "y0", "z0", "u0", "v0", "w0", "r"],
 }
 
 

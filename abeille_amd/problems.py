@@ -93,6 +93,33 @@ def fixed_source_cube(nparticles=25000, tracking="delta"):
     return m
 
 
+def cylinder_cube(nparticles=25000, tracking="surface", axis=(0., 2., 0.), radius=4.,
+                  centre=(0.5, 3.0, -0.5)):
+    """The 20 cm vacuum cube with a general Cylinder (src/cylinder.cpp) of
+    another, more absorbing material through it; `axis` need not be
+    normalised (the GPU path takes coordinate axes only).  Point source at
+    the centre, inside the cylinder; collision-estimator flux on 20^3 x 2."""
+    m = Model()
+    s = [m.surface("xplane", [-10.], "vacuum"), m.surface("xplane", [10.], "vacuum"),
+         m.surface("yplane", [-10.], "vacuum"), m.surface("yplane", [10.], "vacuum"),
+         m.surface("zplane", [-10.], "vacuum"), m.surface("zplane", [10.], "vacuum")]
+    cyl = m.surface("cylinder", [*centre, *axis, radius])
+    m.material(1, total=[0.5, 1.2], absorption=[0.05, 0.20],
+               scatter=[[0.30, 0.15], [0.00, 1.00]])
+    m.material(2, total=[0.8, 1.5], absorption=[0.30, 0.60],
+               scatter=[[0.35, 0.15], [0.00, 0.90]])
+    box = f"+{s[0]} & -{s[1]} & +{s[2]} & -{s[3]} & +{s[4]} & -{s[5]}"
+    m.cell(f"-{cyl} & {box}", material=2, id=1)
+    m.cell(f"+{cyl} & {box}", material=1, id=2)
+    m.root = m.cell_universe([1, 2], id=1)
+    m.point_source([0., 0., 0.], energy=0.5)
+    m.mesh_tally([-10., -10., -10.], [10., 10., 10.], [20, 20, 20], [0., 1., 2.],
+                 estimator="collision")
+    m.set(mode=abi.FIXED_SOURCE, nparticles=nparticles, energy_bounds=[0., 1., 2.],
+          tracking=_tracking(tracking))
+    return m
+
+
 def _tracking(name):
     return {"delta": abi.DELTA_TRACKING, "surface": abi.SURFACE_TRACKING,
             "carter": abi.CARTER_TRACKING}[name]

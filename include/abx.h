@@ -40,7 +40,7 @@ extern "C" {
 /* ---- geometry (include/geometry/) ------------------------------------ */
 
 /* surface types: src/{x,y,z}plane.cpp, src/plane.cpp, src/{x,y,z}cylinder.cpp,
- * src/sphere.cpp */
+ * src/sphere.cpp, src/cylinder.cpp */
 enum abx_surface_type {
   ABX_XPLANE = 0, /* p = {x0}            */
   ABX_YPLANE = 1, /* p = {y0}            */
@@ -49,7 +49,14 @@ enum abx_surface_type {
   ABX_XCYL = 4,   /* p = {y0, z0, R}     */
   ABX_YCYL = 5,   /* p = {x0, z0, R}     */
   ABX_ZCYL = 6,   /* p = {x0, y0, R}     */
-  ABX_SPHERE = 7  /* p = {x0, y0, z0, R} */
+  ABX_SPHERE = 7, /* p = {x0, y0, z0, R} */
+  ABX_CYL = 8     /* p = {x0, y0, z0, u0, v0, w0, R}: Cylinder, src/cylinder.cpp:30-57.
+                     The reference's surface is the quadric
+                     a x^2 + b y^2 + c z^2 = R^2 with a = 1 - u^2, b = 1 - v^2,
+                     c = 1 - w^2 of the normalised axis (u, v, w): a circular
+                     cylinder when the axis is a coordinate axis, which is the
+                     case abx_create accepts (stored as the x/y/z-cylinder it
+                     equals bit for bit); other axes give ABX_E_INVALID. */
 };
 
 /* boundary types: include/geometry/surfaces/surface.hpp:34 (same order) */
@@ -60,7 +67,7 @@ typedef struct abx_surface {
   int32_t boundary; /* abx_boundary */
   uint32_t id;
   int32_t pad_;
-  double p[4];
+  double p[8];
 } abx_surface;
 
 /* Cell region tokens in reverse Polish notation (src/cell.cpp:248-294):

@@ -179,12 +179,14 @@ struct RNG {
 
 //==========================================================================
 // Surfaces (src/{x,y,z}plane.cpp, src/plane.cpp, src/{x,y,z}cylinder.cpp,
-// src/sphere.cpp)
+// src/sphere.cpp, src/cylinder.cpp)
 struct Surface {
   int type;
   int bc;
   uint32_t id;
   double p[4];
+  // Cylinder (src/cylinder.cpp:30-57): point, normalised-axis coefficients
+  double x0 = 0., y0 = 0., z0 = 0., alpha = 0., beta = 0., gamma = 0., R = 0.;
 
   V3 norm(const V3& r) const {
     switch (type) {
@@ -202,6 +204,8 @@ struct Surface {
         return direction(r.x - p[0], 0., r.z - p[1]);
       case ABX_ZCYL:
         return direction(r.x - p[0], r.y - p[1], 0.);
+      case ABX_CYL:  // Cylinder::norm, src/cylinder.cpp:115-117
+        return direction(alpha * (r.x - x0), beta * (r.y - y0), gamma * (r.z - z0));
       default:
         return direction(r.x - p[0], r.y - p[1], r.z - p[2]);
     }
@@ -235,6 +239,11 @@ struct Surface {
       case ABX_ZCYL: {
         const double x = r.x - p[0], y = r.y - p[1];
         eval = y * y + x * x - p[2] * p[2];
+        break;
+      }
+      case ABX_CYL: {  // Cylinder::sign, src/cylinder.cpp:59-72
+        const double x = r.x - x0, y = r.y - y0, z = r.z - z0;
+        eval = alpha * x * x + beta * y * y + gamma * z * z - R * R;
         break;
       }
       default: {
@@ -323,6 +332,14 @@ struct Surface {
         const double x = r.x - p[0], y = r.y - p[1];
         const double k = y * u.y + x * u.x;
         const double c = y * y + x * x - p[2] * p[2];
+        return cyl(a, k, c, on_surf);
+      }
+      case ABX_CYL: {  // Cylinder::distance, src/cylinder.cpp:74-113
+        const double a = alpha * u.x * u.x + beta * u.y * u.y + gamma * u.z * u.z;
+        if (a == 0.) return INF;
+        const double x = r.x - x0, y = r.y - y0, z = r.z - z0;
+        const double k = alpha * x * u.x + beta * y * u.y + gamma * z * u.z;
+        const double c = alpha * x * x + beta * y * y + gamma * z * z - R * R;
         return cyl(a, k, c, on_surf);
       }
       default: {  // sphere (src/sphere.cpp:51-76)
@@ -1646,6 +1663,25 @@ static void build(Problem& P, const abx_problem_desc* d) {
     S.bc = s.boundary;
     S.id = s.id;
     for (int k = 0; k < 4; k++) S.p[k] = s.p[k];
+    if (s.type == ABX_CYL) {
+      // Cylinder::Cylinder, src/cylinder.cpp:42-57
+      S.x0 = s.p[0];
+      S.y0 = s.p[1];
+      S.z0 = s.p[2];
+      double u0 = s.p[3], v0 = s.p[4], w0 = s.p[5];
+      const double mag = std::sqrt(u0 * u0 + v0 * v0 + w0 * w0);
+      if (mag == 0.)
+        fatal(ABX_E_INVALID, "Direction vector of Cylinder can not have a mangitude of 0.");
+      u0 /= mag;
+      v0 /= mag;
+      w0 /= mag;
+      S.alpha = 1. - u0 * u0;
+      S.beta = 1. - v0 * v0;
+      S.gamma = 1. - w0 * w0;
+      S.R = s.p[6];
+    } else if (s.type < ABX_XPLANE || s.type > ABX_SPHERE) {
+      fatal(ABX_E_INVALID, "surface type");
+    }
     P.surfaces.push_back(S);
   }
   for (int i = 0; i < d->n_cells; i++) {

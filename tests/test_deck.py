@@ -57,3 +57,23 @@ def test_reference_decks_load(deck):
     assert d.n_materials >= 1 and d.n_universes >= 1
     m.set(nparticles=200)
     Oracle(m).run(2, 1)
+
+
+@pytest.mark.skipif(not os.path.isdir(REF), reason="reference decks not present")
+def test_general_cylinder_surface_in_a_deck():
+    """`type: cylinder` (x0, y0, z0, u0, v0, w0, r; src/cylinder.cpp:121-203):
+    the Ua-1-1-CY deck with its z-cylinder written as a general Cylinder
+    along z gives the same critical system."""
+    import yaml
+    with open(os.path.join(REF, "Ua-1-1-CY.yaml")) as f:
+        deck = yaml.safe_load(f)
+    s = deck["surfaces"][0]
+    assert s["type"] == "zcylinder"
+    deck["surfaces"][0] = {"type": "cylinder", "x0": s["x0"], "y0": s["y0"], "z0": 0.,
+                           "u0": 0., "v0": 0., "w0": 2., "r": s["r"],
+                           "boundary": s["boundary"], "id": s["id"]}
+    m = model_from_deck(deck)
+    assert m.surfaces[0][0] == abi.CYL and m.surfaces[0][1][5] == 2.
+    m.set(nparticles=4000)
+    st, _ = Oracle(m).run(40, 10)
+    assert abs(st.kcol_avg - 1.0) < 4 * st.kcol_err

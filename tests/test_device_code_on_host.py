@@ -23,6 +23,12 @@ CASES = {
     "mfs_delta": lambda: problems.subcritical_slab(600, tracking="delta", cancel=False),
     "aniso_st": lambda: problems.anisotropic_slab(600, tracking="surface"),
     "aniso_dt": lambda: problems.anisotropic_slab(600, tracking="delta"),
+    "cyl_y_st": lambda: problems.cylinder_cube(600, tracking="surface"),
+    "cyl_y_dt": lambda: problems.cylinder_cube(600, tracking="delta"),
+    "cyl_x_st": lambda: problems.cylinder_cube(600, tracking="surface", axis=(3., 0., 0.),
+                                               centre=(0., 0.5, -0.5)),
+    "cyl_z_st": lambda: problems.cylinder_cube(600, tracking="surface", axis=(0., 0., -1.),
+                                               centre=(0.5, -0.5, 2.)),
 }
 
 

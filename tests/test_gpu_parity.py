@@ -26,7 +26,15 @@ CASES = {
     "slab_mfs_carter": lambda: problems.subcritical_slab(2000, tracking="carter", cancel=False),
     "slab_mfs_delta": lambda: problems.subcritical_slab(2000, tracking="delta", cancel=False),
     "slab_p2_scatter": lambda: problems.anisotropic_slab(2000, tracking="surface"),
+    "cube_cylinder_st": lambda: problems.cylinder_cube(2000, tracking="surface"),
 }
+
+
+def test_oblique_cylinder_rejected():
+    """A general Cylinder off the coordinate axes is the reference's
+    non-circular quadric; the device path refuses it at creation."""
+    with pytest.raises(Exception, match="Cylinder"):
+        Problem(problems.cylinder_cube(10, axis=(1., 1., 2.)))
 
 
 @pytest.mark.parametrize("name", sorted(CASES))

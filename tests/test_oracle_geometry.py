@@ -9,7 +9,7 @@
 """
 import numpy as np
 
-from abeille_amd import problems
+from abeille_amd import abi, problems
 from abeille_amd.problems import GT_POSITIONS
 from oracle.pyoracle import Oracle
 
@@ -110,3 +110,79 @@ def test_tile_face_tie_break():
     assert list(cell) == [1, 1]
     d, _, _ = o.nearest_boundary(r, u)
     assert np.all(d > 0.5)  # the face it sits on is skipped (100 * 1e-12)
+
+
+# ------------------------------------------------ Cylinder (src/cylinder.cpp)
+def _cyl_points(n, seed):
+    rng = np.random.default_rng(seed)
+    r = rng.uniform(-9.5, 9.5, (n, 3))
+    u = rng.normal(size=(n, 3))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    return r, u
+
+
+def test_general_cylinder_on_a_coordinate_axis_is_the_z_cylinder():
+    """With its axis along z (given unnormalised) the general Cylinder is the
+    z-cylinder: the same cells, and nearest boundaries to rounding."""
+    from abeille_amd.model import Model
+
+    def cube(kind, params):
+        m = Model()
+        s = [m.surface("xplane", [-10.], "vacuum"), m.surface("xplane", [10.], "vacuum"),
+             m.surface("yplane", [-10.], "vacuum"), m.surface("yplane", [10.], "vacuum"),
+             m.surface("zplane", [-10.], "vacuum"), m.surface("zplane", [10.], "vacuum")]
+        c = m.surface(kind, params)
+        m.material(1, total=[0.5, 1.2], absorption=[0.05, 0.20],
+                   scatter=[[0.30, 0.15], [0.00, 1.00]])
+        box = f"+{s[0]} & -{s[1]} & +{s[2]} & -{s[3]} & +{s[4]} & -{s[5]}"
+        m.cell(f"-{c} & {box}", material=1, id=1)
+        m.cell(f"+{c} & {box}", material=1, id=2)
+        m.root = m.cell_universe([1, 2], id=1)
+        m.point_source([0., 0., 0.], energy=0.5)
+        m.set(mode=abi.FIXED_SOURCE, nparticles=10, energy_bounds=[0., 1., 2.],
+              tracking=abi.SURFACE_TRACKING)
+        return Oracle(m)
+
+    g = cube("cylinder", [1.5, -0.5, 7.0, 0., 0., 3.0, 4.0])
+    z = cube("zcylinder", [1.5, -0.5, 4.0])
+    r, u = _cyl_points(4000, 11)
+    np.testing.assert_array_equal(g.locate(r, u)[0], z.locate(r, u)[0])
+    dg, sg, _ = g.nearest_boundary(r, u)
+    dz, sz, _ = z.nearest_boundary(r, u)
+    np.testing.assert_array_equal(sg, sz)
+    np.testing.assert_allclose(dg, dz, rtol=1e-12, atol=1e-12)
+    # and the z-cylinder's distances are the circle's: |(r + d u)_xy - c| = R
+    hit = sz == 6
+    p = r[hit] + dz[hit, None] * u[hit]
+    np.testing.assert_allclose(np.hypot(p[:, 0] - 1.5, p[:, 1] + 0.5), 4.0, rtol=1e-10)
+
+
+def test_general_cylinder_oblique_axis_roots_and_sides():
+    """An oblique axis: the surface is the quadric alpha x^2 + beta y^2 +
+    gamma z^2 = R^2 of the normalised axis (the reference's Cylinder); the
+    nearest boundary found on it is a root of that quadric, and a point's
+    cell is the side the quadric puts it on."""
+    axis = np.array([1., 1., 2.])
+    a = axis / np.linalg.norm(axis)
+    coef = 1.0 - a * a
+    c0 = np.array([0.5, -0.5, 0.])
+    R = 4.0
+
+    def quadric(p):
+        x = p - c0
+        return coef[0] * x[:, 0] ** 2 + coef[1] * x[:, 1] ** 2 + coef[2] * x[:, 2] ** 2 - R * R
+
+    o = Oracle(problems.cylinder_cube(10, tracking="surface", axis=tuple(axis), radius=R,
+                                      centre=tuple(c0)))   # the oracle takes any axis
+    r, u = _cyl_points(4000, 12)
+    cell, _ = o.locate(r, u)
+    q = quadric(r)
+    clear = np.abs(q) > 1e-9
+    np.testing.assert_array_equal(cell[clear], np.where(q[clear] < 0, 0, 1))
+    d, s, _ = o.nearest_boundary(r, u)
+    on_cyl = s == 6
+    assert on_cyl.sum() > 500
+    np.testing.assert_allclose(quadric(r[on_cyl] + d[on_cyl, None] * u[on_cyl]), 0.0, atol=1e-8)
+    on_box = ~on_cyl
+    p = r[on_box] + d[on_box, None] * u[on_box]
+    np.testing.assert_allclose(np.abs(p).max(axis=1), 10.0, rtol=1e-12)
