@@ -11,11 +11,12 @@
 //  * log: argument reduction x = 2^k * m, m in [sqrt(2)/2, sqrt(2)); with
 //    f = m - 1 and s = f / (2 + f), log(m) = 2 atanh(s) evaluated as
 //    f - hfsq + s (hfsq + R(s^2)), hfsq = f^2/2, R(z) = sum_{n>=1} 2 z^n/(2n+1)
-//    (the Sun fdlibm / Tang formulation of the atanh series).  ln 2 is split
+//    (the Sun fdlibm / Tang formulation of the atanh series), the sum in
+//    Estrin's form.  ln 2 is split
 //    into a 32-bit head and a tail so that k * ln2_hi is exact.
 //  * sin/cos: Cody-Waite reduction by pi/2 with a three-term split of pi/2,
 //    then Taylor polynomials on |r| <= pi/4 (through r^19 for sin and r^20
-//    for cos) with the compensated 1 - r^2/2 step for cos.
+//    for cos, in Estrin's form) with the compensated 1 - r^2/2 step for cos.
 // Accuracy is checked against glibc by tests/test_detmath.py (<= 1 ulp over
 // the ranges the transport path uses).
 #pragma once
@@ -86,52 +87,46 @@ DM_HD double dm_log(double x) {
   const double f = m - 1.0;
   const double s = f / (2.0 + f);
   const double z = s * s;
-  // R(z) = 2z/3 + 2z^2/5 + ... + 2z^12/25, Horner form.
-  double R = 2.0 / 25.0;
-  R = 2.0 / 23.0 + z * R;
-  R = 2.0 / 21.0 + z * R;
-  R = 2.0 / 19.0 + z * R;
-  R = 2.0 / 17.0 + z * R;
-  R = 2.0 / 15.0 + z * R;
-  R = 2.0 / 13.0 + z * R;
-  R = 2.0 / 11.0 + z * R;
-  R = 2.0 / 9.0 + z * R;
-  R = 2.0 / 7.0 + z * R;
-  R = 2.0 / 5.0 + z * R;
-  R = 2.0 / 3.0 + z * R;
-  R = z * R;
+  // R(z) = 2z/3 + 2z^2/5 + ... + 2z^12/25, in Estrin's form (pairs of
+  // terms joined by powers z^2, z^4, z^8): eight dependent steps instead of
+  // Horner's twenty-four.
+  const double z2 = z * z, z4 = z2 * z2;
+  const double p01 = 2.0 / 3.0 + z * (2.0 / 5.0), p23 = 2.0 / 7.0 + z * (2.0 / 9.0);
+  const double p45 = 2.0 / 11.0 + z * (2.0 / 13.0), p67 = 2.0 / 15.0 + z * (2.0 / 17.0);
+  const double p89 = 2.0 / 19.0 + z * (2.0 / 21.0), pab = 2.0 / 23.0 + z * (2.0 / 25.0);
+  const double q0 = p01 + z2 * p23, q1 = p45 + z2 * p67, q2 = p89 + z2 * pab;
+  const double R = z * (q0 + z4 * (q1 + z4 * q2));
   const double hfsq = 0.5 * f * f;
   const double dk = (double)k;
   return dk * ln2_hi - ((hfsq - (s * (hfsq + R) + dk * ln2_lo)) - f);
 }
 
 // sin and cos of r, |r| <= pi/4 (Taylor kernels).
+// Both polynomials are evaluated in Estrin's form (pairs of terms joined by
+// z^2, z^4, z^8) for a short dependency chain.
 DM_HD double dm_ksin(double r) {
   const double z = r * r;
-  double p = -1.0 / 121645100408832000.0;  // -1/19!
-  p = 1.0 / 355687428096000.0 + z * p;     //  1/17!
-  p = -1.0 / 1307674368000.0 + z * p;      // -1/15!
-  p = 1.0 / 6227020800.0 + z * p;          //  1/13!
-  p = -1.0 / 39916800.0 + z * p;           // -1/11!
-  p = 1.0 / 362880.0 + z * p;              //  1/9!
-  p = -1.0 / 5040.0 + z * p;               // -1/7!
-  p = 1.0 / 120.0 + z * p;                 //  1/5!
-  p = -1.0 / 6.0 + z * p;                  // -1/3!
+  const double z2 = z * z, z4 = z2 * z2, z8 = z4 * z4;
+  // P(z) = -1/3! + z/5! - ... - z^8/19!
+  const double p01 = -1.0 / 6.0 + z * (1.0 / 120.0);
+  const double p23 = -1.0 / 5040.0 + z * (1.0 / 362880.0);
+  const double p45 = -1.0 / 39916800.0 + z * (1.0 / 6227020800.0);
+  const double p67 = -1.0 / 1307674368000.0 + z * (1.0 / 355687428096000.0);
+  const double p8 = -1.0 / 121645100408832000.0;
+  const double p = (p01 + z2 * p23) + z4 * (p45 + z2 * p67) + z8 * p8;
   return r + (r * z) * p;
 }
 
 DM_HD double dm_kcos(double r) {
   const double z = r * r;
   // C(z) = 1/4! - z/6! + ... + z^8/20!
-  double p = 1.0 / 2432902008176640000.0;  //  1/20!
-  p = -1.0 / 6402373705728000.0 + z * p;   // -1/18!
-  p = 1.0 / 20922789888000.0 + z * p;      //  1/16!
-  p = -1.0 / 87178291200.0 + z * p;        // -1/14!
-  p = 1.0 / 479001600.0 + z * p;           //  1/12!
-  p = -1.0 / 3628800.0 + z * p;            // -1/10!
-  p = 1.0 / 40320.0 + z * p;               //  1/8!
-  p = -1.0 / 720.0 + z * p;                // -1/6!
-  p = 1.0 / 24.0 + z * p;                  //  1/4!
+  const double z2 = z * z, z4 = z2 * z2, z8 = z4 * z4;
+  const double p01 = 1.0 / 24.0 + z * (-1.0 / 720.0);
+  const double p23 = 1.0 / 40320.0 + z * (-1.0 / 3628800.0);
+  const double p45 = 1.0 / 479001600.0 + z * (-1.0 / 87178291200.0);
+  const double p67 = 1.0 / 20922789888000.0 + z * (-1.0 / 6402373705728000.0);
+  const double p8 = 1.0 / 2432902008176640000.0;
+  const double p = (p01 + z2 * p23) + z4 * (p45 + z2 * p67) + z8 * p8;
   const double hz = 0.5 * z;
   const double w = 1.0 - hz;
   return w + (((1.0 - w) - hz) + (z * z) * p);
