@@ -233,8 +233,9 @@ class Batch:
             self.set_step_histories(step_histories)
 
     def set_params(self, chunk=0, multi_hist=0):
-        """Batch-kernel chunk length and the cap on a problem's side-by-side
-        batches (0 = automatic / default)."""
+        """Batch-kernel chunk length (histories; 0 = about 8 chunks per CU,
+        -f = about f per CU) and the cap on a problem's side-by-side batches
+        (0 = default)."""
         _check(lib().abx_batch_set_params(self.h, C.c_int64(int(chunk)),
                                           C.c_int64(int(multi_hist))))
 

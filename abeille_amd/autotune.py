@@ -74,7 +74,9 @@ def autotune(problem, model, sample=200_000, repeats=2, cache=DEFAULT_CACHE,
     return candidates[best], times
 
 
-BATCH_CANDIDATES = [dict(chunk=c, multi_hist=m) for c in (0, 1024, 4096, 16384)
+# chunk lengths as chunks per CU (negative), so that a choice made on a
+# sample carries over to the full set of problems
+BATCH_CANDIDATES = [dict(chunk=c, multi_hist=m) for c in (0, -2, -4, -16)
                     for m in (1 << 20, 1 << 23)]
 
 

@@ -386,14 +386,20 @@ void upd(int gen, double x, double& avg, double& var) {
   if (gen > 1) var = var_old + ((x - avg_old) * (x - avg_old) / dgen) - (var_old / (dgen - 1.));
 }
 
-void record_generation(abx_problem* p, const abx_scores& s, double W,
-                       hipStream_t st = nullptr, double* gen = nullptr) {
+// The scalar part of a record (Tallies::record_generation,
+// src/tallies.cpp:159-203) and the tally generation count.
+void record_scores(abx_problem* p, const abx_scores& s, double W) {
   p->gen++;
   upd(p->gen, s.k_col / W, p->kc_avg, p->kc_var);
   upd(p->gen, s.k_trk / W, p->kt_avg, p->kt_var);
   upd(p->gen, s.leakage / W, p->lk_avg, p->lk_var);
   upd(p->gen, s.mig / W, p->mg_avg, p->mg_var);
   p->tally_g++;
+}
+
+void record_generation(abx_problem* p, const abx_scores& s, double W,
+                       hipStream_t st = nullptr, double* gen = nullptr) {
+  record_scores(p, s, W);
   const int64_t nb = p->T.tally_bins;
   if (nb > 0) {
     hipLaunchKernelGGL(k_record_generation, dim3((unsigned)blocks_for(nb, 256)), dim3(256), 0,
@@ -1313,14 +1319,14 @@ std::vector<BatchChunk> make_chunks(const std::vector<int64_t>& n, int64_t chunk
   return c;
 }
 
-// Chunk length: about eight chunks per CU over the launch, 256 - 8192
+// Chunk length: about `per_cu` chunks per CU over the launch, 256 - 8192
 // histories (at least one block's lanes, so that lanes <= histories + 255).
-int64_t chunk_length(int64_t total) {
+int64_t chunk_length(int64_t total, int per_cu = 8) {
   int dev = 0, cus = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     cus = 256;
-  return std::max<int64_t>(256, std::min<int64_t>(8192, total / std::max(1, 8 * cus)));
+  return std::max<int64_t>(256, std::min<int64_t>(8192, total / std::max(1, per_cu * cus)));
 }
 
 template <int TRK, bool SEC, bool SPLIT, bool STAGE>
@@ -1410,10 +1416,14 @@ struct abx_batch {
   std::vector<DevProblem> hp;
   std::vector<KCtx> hk;
   std::vector<BatchChunk> all_chunks;
+  std::vector<SampleJob> sjobs;
+  std::vector<DevSources> ssrc;
+  std::vector<RecordJob> rjobs;
+  DBuf sjob_d, ssrc_d, rjob_d;
   size_t budget = 0;  // device bytes admitted problems may hold (0 = no limit)
   size_t used = 0;
   int64_t step_hist = 0;  // histories per step over all problems (0 = no limit)
-  int64_t chunk = 0;      // histories per chunk of the batch kernel (0 = automatic)
+  int64_t chunk = 0;      // histories per chunk (0 = about 8 per CU, -f = about f per CU)
   int64_t multi_hist = int64_t(1) << 23;  // cap on side-by-side batches of one problem
   double kernel_ms = 0.0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1424,7 +1434,7 @@ struct abx_batch {
   hipStream_t aux[NS] = {};
   hipEvent_t ready = nullptr, done[NS] = {};
   ~abx_batch() {
-    DBuf* all[] = {&probs, &ctxs, &offsets, &scal, &next, &sec, &chunks};
+    DBuf* all[] = {&probs, &ctxs, &offsets, &scal, &next, &sec, &chunks, &sjob_d, &ssrc_d, &rjob_d};
     for (DBuf* b : all) b->release();
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
@@ -1513,22 +1523,6 @@ void batch_step(abx_batch* b, std::vector<abx_problem*>& finished) {
   b->scal.reserve((size_t)ni * sizeof(BatchScalars));
   HIPC(hipMemsetAsync(b->scal.p, 0, (size_t)ni * sizeof(BatchScalars), b->stream));
   BatchScalars* ds = b->scal.as<BatchScalars>();
-  for (int j = 0; j < ne; j++) {
-    BatchEntry& it = b->items[j];
-    if (it.mid_chain) continue;
-    abx_problem* p = it.p;
-    const int f = first_item[j];
-    const int64_t items = (j + 1 < ne ? first_item[j + 1] : ni) - f;  // the entry's batches
-    const int64_t total = items * it.n;
-    if (total > 0) {
-      hipLaunchKernelGGL(k_sample_sources, dim3((unsigned)blocks_for(total, 256)), dim3(256),
-                         (size_t)256 * trk_bytes(p->P.trk_D, p->P.trk_L), b->stream, p->P, p->S,
-                         p->run_hist, total, p->bank_a.as<abx_particle>(), &ds[f].err,
-                         &ds[f].err_hist);
-      HIPC(hipGetLastError());
-    }
-    p->run_hist += (uint64_t)total;
-  }
   // 2. one launch per kernel variant
   std::vector<int> order(ni);
   for (int i = 0; i < ni; i++) order[i] = i;
@@ -1565,10 +1559,13 @@ void batch_step(abx_batch* b, std::vector<abx_problem*>& finished) {
     int64_t c0, nc;
   };
   std::vector<Group> groups;
+  std::vector<int> pos(ni);  // item -> its position in the launch arrays
   b->all_chunks.clear();
   int64_t total_all = 0;
   for (const auto& x : li) total_all += x.n;
-  const int64_t clen = b->chunk > 0 ? std::max<int64_t>(64, b->chunk) : chunk_length(total_all);
+  const int64_t clen = b->chunk > 0   ? std::max<int64_t>(64, b->chunk)
+                       : b->chunk < 0 ? chunk_length(total_all, (int)std::min<int64_t>(1024, -b->chunk))
+                                      : chunk_length(total_all);
   int g0 = 0;
   while (g0 < ni) {
     const int key = variant_key(b->items[li[order[g0]].entry].p);
@@ -1586,6 +1583,7 @@ void batch_step(abx_batch* b, std::vector<abx_problem*>& finished) {
                             : nullptr;
       b->hk[g1] = make_kctx(p, x.in, x.n, p->run_kcol, 1, cs, sec, sec_cap);
       b->hp[g1] = p->P;
+      pos[i] = g1;
       b->hp[g1].tally_data = x.gen;
       n.push_back(x.n);
       gr.D = std::max(gr.D, p->P.trk_D);
@@ -1609,6 +1607,41 @@ void batch_step(abx_batch* b, std::vector<abx_problem*>& finished) {
     HIPC(hipMemcpyAsync(b->chunks.p, b->all_chunks.data(),
                         b->all_chunks.size() * sizeof(BatchChunk), hipMemcpyHostToDevice,
                         b->stream));
+  // the sources of every entry that starts batches, in one launch
+  b->sjobs.clear();
+  b->ssrc.clear();
+  int64_t stotal = 0;
+  int sD = 1, sL = 1;
+  for (int j = 0; j < ne; j++) {
+    BatchEntry& it = b->items[j];
+    if (it.mid_chain) continue;
+    abx_problem* p = it.p;
+    const int f = first_item[j];
+    const int64_t items = (j + 1 < ne ? first_item[j + 1] : ni) - f;  // the entry's batches
+    const int64_t total = items * it.n;
+    if (total > 0) {
+      b->sjobs.push_back(SampleJob{stotal, total, p->run_hist, p->bank_a.as<abx_particle>(),
+                                   &ds[f].err, &ds[f].err_hist, pos[f], 0});
+      b->ssrc.push_back(p->S);
+      stotal += total;
+      sD = std::max(sD, p->P.trk_D);
+      sL = std::max(sL, p->P.trk_L);
+    }
+    p->run_hist += (uint64_t)total;
+  }
+  if (!b->sjobs.empty()) {
+    b->sjob_d.reserve(b->sjobs.size() * sizeof(SampleJob));
+    b->ssrc_d.reserve(b->ssrc.size() * sizeof(DevSources));
+    HIPC(hipMemcpyAsync(b->sjob_d.p, b->sjobs.data(), b->sjobs.size() * sizeof(SampleJob),
+                        hipMemcpyHostToDevice, b->stream));
+    HIPC(hipMemcpyAsync(b->ssrc_d.p, b->ssrc.data(), b->ssrc.size() * sizeof(DevSources),
+                        hipMemcpyHostToDevice, b->stream));
+    hipLaunchKernelGGL(k_sample_jobs, dim3((unsigned)blocks_for(stotal, 256)), dim3(256),
+                       (size_t)256 * trk_bytes(sD, sL), b->stream, b->probs.as<DevProblem>(),
+                       b->ssrc_d.as<DevSources>(), b->sjob_d.as<SampleJob>(),
+                       (int)b->sjobs.size(), stotal, sD, sL);
+    HIPC(hipGetLastError());
+  }
   // one chunk counter per group (groups run concurrently)
   b->next.reserve(std::max<size_t>(8, groups.size()) * sizeof(unsigned long long));
   HIPC(hipEventRecord(b->e0, b->stream));
@@ -1640,12 +1673,24 @@ void batch_step(abx_batch* b, std::vector<abx_problem*>& finished) {
   // in history order
   std::vector<BatchEntry> keep;
   keep.reserve(ne);
+  b->rjobs.clear();
+  int64_t rtotal = 0;
+  // the tally part of a record: gathered into one launch after the loop
+  auto record_tallies = [&](abx_problem* p, double* gen0, int kb, int g0) {
+    const int64_t nb = p->T.tally_bins;
+    if (nb <= 0) return;
+    b->rjobs.push_back(RecordJob{rtotal, nb, gen0, p->tally_more.as<double>(),
+                                 p->tally_avg.as<double>(), p->tally_var.as<double>(),
+                                 (double)g0, kb, 0});
+    rtotal += nb;
+  };
   for (int j = 0; j < ne; j++) {
     BatchEntry& it = b->items[j];
     abx_problem* p = it.p;
     const HostTables& T = p->T;
     const int f = first_item[j];
     const int nk = (j + 1 < ne ? first_item[j + 1] : ni) - f;
+    if (nk > 1) record_tallies(p, li[f].gen, nk, p->tally_g);
     for (int k = 0; k < nk; k++) {
       const LaunchItem& x = li[f + k];
       const BatchScalars& h = b->h_scal[f + k];
@@ -1653,9 +1698,10 @@ void batch_step(abx_batch* b, std::vector<abx_problem*>& finished) {
       add_diag(p->run_diag, r.diag);
       p->run_histories += x.n;
       if (nk > 1) {
-        // a whole batch of its own
-        record_generation(p, r.scores, (double)T.st.nparticles, b->stream, x.gen);
-        clear_generation(p, b->stream, x.gen);
+        // a whole batch of its own; its tallies are folded in by the
+        // record job registered above
+        record_scores(p, r.scores, (double)T.st.nparticles);
+        p->gen_scores = abx_scores{};
         it.batches_left--;
         continue;
       }
@@ -1680,8 +1726,9 @@ void batch_step(abx_batch* b, std::vector<abx_problem*>& finished) {
       } else {
         it.mid_chain = false;
         it.done = 0;
-        record_generation(p, it.acc, (double)T.st.nparticles, b->stream);
-        clear_generation(p, b->stream);
+        record_tallies(p, p->tally_gen.as<double>(), 1, p->tally_g);
+        record_scores(p, it.acc, (double)T.st.nparticles);
+        p->gen_scores = abx_scores{};
         it.batches_left--;
       }
       // fission-bank ordering, cancellation and re-banking ran on the
@@ -1696,6 +1743,14 @@ void batch_step(abx_batch* b, std::vector<abx_problem*>& finished) {
     }
   }
   b->items.swap(keep);
+  if (!b->rjobs.empty()) {
+    b->rjob_d.reserve(b->rjobs.size() * sizeof(RecordJob));
+    HIPC(hipMemcpyAsync(b->rjob_d.p, b->rjobs.data(), b->rjobs.size() * sizeof(RecordJob),
+                        hipMemcpyHostToDevice, b->stream));
+    hipLaunchKernelGGL(k_record_jobs, dim3((unsigned)blocks_for(rtotal, 256)), dim3(256), 0,
+                       b->stream, b->rjob_d.as<RecordJob>(), (int)b->rjobs.size(), rtotal);
+    HIPC(hipGetLastError());
+  }
   // the records above ran on the batch stream; finished problems are read
   // on their own streams
   HIPC(hipStreamSynchronize(b->stream));
@@ -1772,7 +1827,7 @@ int abx_batch_set_step_histories(abx_batch* b, int64_t histories) {
 
 int abx_batch_set_params(abx_batch* b, int64_t chunk, int64_t multi_hist) {
   ABX_TRY({
-    if (!b || chunk < 0 || multi_hist < 0) fail(ABX_E_INVALID, "batch parameters");
+    if (!b || multi_hist < 0) fail(ABX_E_INVALID, "batch parameters");
     // attached problems were admitted (budget) with the concurrency these
     // parameters decide
     if (!b->items.empty()) fail(ABX_E_INVALID, "set batch parameters before attaching problems");

@@ -494,6 +494,85 @@ __global__ void k_sample_sources(DevProblem P, DevSources S, uint64_t first, int
   out[i] = a;
 }
 
+// A batch step's source sampling in one launch: job j samples n histories
+// from id `first` into `out` with problem probs[prob] and sources srcs[j]
+// (SourceDistribution sampling, src/source.cpp:44-90, as k_sample_sources).
+// Cursor slices are laid out for the launch's largest capacities (D, L).
+struct SampleJob {
+  int64_t start;  // prefix over the jobs' histories
+  int64_t n;
+  uint64_t first;
+  abx_particle* out;
+  int32_t* err;
+  unsigned long long* err_hist;
+  int32_t prob;
+  int32_t pad_;
+};
+
+__global__ void k_sample_jobs(const DevProblem* probs, const DevSources* srcs,
+                              const SampleJob* jobs, int njobs, int64_t total, int D, int L) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  int lo = 0, hi = njobs - 1;  // the last job starting at or before i
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].start <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  const SampleJob& J = jobs[lo];
+  const int64_t k = i - J.start;
+  Trk t;
+  trk_bind(t, abx_lds, blockDim.x, threadIdx.x, D, L);
+  abx_particle a;
+  const int rc = sample_source(probs[J.prob], srcs[lo], J.first + (uint64_t)k, t, a);
+  if (rc) {
+    if (atomicCAS(J.err, 0, rc) == 0) *J.err_hist = J.first + (uint64_t)k;
+    return;
+  }
+  J.out[k] = a;
+}
+
+// A batch step's tally records in one launch (MeshTally::record_generation
+// then clear_generation, src/mesh_tally.cpp:121-152, as k_record_generation):
+// job j folds kb generation arrays of one problem into its mean and variance
+// in order, the k-th as record number g0 + k + 1, and zeroes them.
+struct RecordJob {
+  int64_t start;  // prefix over the jobs' bins
+  int64_t nb;
+  double* gen0;   // first generation array
+  double* more;   // the others, nb apart
+  double* avg;
+  double* var;
+  double g0;
+  int32_t kb;
+  int32_t pad_;
+};
+
+__global__ void k_record_jobs(const RecordJob* jobs, int njobs, int64_t total) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].start <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  const RecordJob& J = jobs[lo];
+  const int64_t b = i - J.start;
+  double a = J.avg[b], v = J.var[b];
+  for (int k = 0; k < J.kb; k++) {
+    double* gen = k == 0 ? J.gen0 : J.more + (int64_t)(k - 1) * J.nb;
+    const double dg = J.g0 + (double)(k + 1);
+    const double old_avg = a;
+    const double val = gen[b] * 1.0;
+    a = old_avg + (val - old_avg) / dg;
+    v = v + (((val - old_avg) * (val - a) - (v)) / dg);
+    gen[b] = 0.0;
+  }
+  J.avg[b] = a;
+  J.var[b] = v;
+}
+
 // ---------------------------------------------------------------- scans
 // Exclusive scan of uint32 counts into uint64 offsets, three phases.
 __global__ void k_scan_blocks(const uint32_t* in, uint64_t* out, uint64_t* block_sums,

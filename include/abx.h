@@ -480,7 +480,8 @@ int abx_batch_step(abx_batch* b, abx_problem** finished, int cap, int* n_finishe
 int abx_batch_set_step_histories(abx_batch* b, int64_t histories);  /* before any abx_batch_add */
 /* Launch parameters of the batch engine, for the autotuner: histories per
  * chunk of the batch kernel (a block's unit of work; 0 = automatic, about
- * eight chunks per CU) and the histories that the side-by-side batches of
+ * eight chunks per CU; -f = about f chunks per CU, which scales with the
+ * step's histories) and the histories that the side-by-side batches of
  * one fixed-source problem may hold together (0 = 2^23; a problem whose
  * batch alone exceeds it runs one batch per step).  The reference has no
  * counterpart: its batches run one after another (FixedSource::run,

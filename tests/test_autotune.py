@@ -46,7 +46,8 @@ def test_autotune_batch_picks_and_keeps_results():
     rng = np.random.default_rng(5)
     models = [problems.slab_problem(rng, nhist=int(rng.integers(500, 3000)))[0] for _ in range(6)]
     sample = [Problem(m) for m in models[:3]]
-    cands = [BATCH_CANDIDATES[0], dict(chunk=300, multi_hist=1), dict(chunk=64, multi_hist=1 << 23)]
+    cands = [BATCH_CANDIDATES[0], dict(chunk=300, multi_hist=1), dict(chunk=64, multi_hist=1 << 23),
+             dict(chunk=-32, multi_hist=1 << 20)]
     b = Batch()
     params, times = autotune_batch(b, sample, nbatches=1, candidates=cands)
     assert params in cands and len(times) == len(cands)
