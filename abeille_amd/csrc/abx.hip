@@ -138,6 +138,7 @@ namespace {
 
 // Exclusive scan of n uint32 counts (device) into uint64 offsets; returns total.
 uint64_t device_scan(abx_problem* p, const uint32_t* in, uint64_t* out, int64_t n) {
+  if (n <= 0) return 0;  // an empty bank: no launch (a grid of 0 blocks is an error)
   const int bs = 1024;
   const int64_t nb = blocks_for(n, bs);
   p->block_sums.reserve((size_t)(nb + 1) * sizeof(uint64_t));

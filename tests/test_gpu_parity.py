@@ -149,3 +149,46 @@ def test_run_continue_keeps_the_source_bank(engine):
     b = Problem(m, engine=engine)
     _, k = b.run(4, nignored=1)
     np.testing.assert_allclose(np.concatenate([k1, k2]), k, rtol=1e-9)
+
+
+@pytest.mark.parametrize("name", ["lattice_dt", "cube_st", "slab_mfs_carter"])
+@pytest.mark.parametrize("engine", [abi.ENGINE_EVENT, abi.ENGINE_HISTORY], ids=["event", "history"])
+def test_empty_bank(name, engine):
+    """An empty bank in every mode: no sites, no scores, no tally, no
+    launch error; the next transport on the same problem still matches the
+    CPU restatement bit for bit."""
+    print(f"[parity] case={name} engine={engine} empty bank", flush=True)
+    m = CASES[name]()
+    o = Oracle(m)
+    g = Problem(m, engine=engine)
+    g.tally_clear()
+    r = g.transport(g.sample_sources(0, 0), kcol=0.9)
+    assert len(r["fission"]) == 0
+    s = r["scores"]
+    assert (s.k_col, s.k_abs, s.k_trk, s.leakage, s.mig) == (0.0,) * 5
+    assert r["diag"].histories == 0 and r["diag"].collisions == 0
+    assert not g.tally(0).any()
+    so = o.sample_sources(0, 500)
+    ro = o.transport(so, kcol=0.9, nthreads=1)
+    rg = g.transport(so, kcol=0.9)
+    assert ro["fission"].tobytes() == rg["fission"].tobytes()
+    assert ro["diag"].collisions == rg["diag"].collisions
+
+
+def test_fission_capacity_error_then_recovers():
+    """A fission_out buffer smaller than the generation's sites is refused
+    with ABX_E_CAPACITY (the library cannot grow the caller's buffer as the
+    reference's vector grows); the problem stays usable and the next
+    transport matches the CPU restatement bit for bit."""
+    from abeille_amd import AbxError
+    m = CASES["lattice_dt"]()
+    o = Oracle(m)
+    g = Problem(m)
+    so = o.sample_sources(0, 500)
+    with pytest.raises(AbxError, match="capacity") as e:
+        g.transport(so, kcol=0.9, cap=1)
+    assert e.value.code == -5   # ABX_E_CAPACITY
+    ro = o.transport(so, kcol=0.9, nthreads=1)
+    rg = g.transport(so, kcol=0.9)
+    assert len(ro["fission"]) > 1
+    assert ro["fission"].tobytes() == rg["fission"].tobytes()

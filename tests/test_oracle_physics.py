@@ -63,3 +63,20 @@ def test_glibc_and_portable_math_agree_statistically():
     sb, _ = b.run(6, 0)
     err = np.hypot(sa.leak_err, sb.leak_err)
     assert abs(sa.leak_avg - sb.leak_avg) < 4 * err + 1e-12
+
+
+@pytest.mark.parametrize("make", [
+    lambda: problems.pin_lattice_2d(100),
+    lambda: problems.fixed_source_cube(100, tracking="surface"),
+    lambda: problems.subcritical_slab(100, tracking="carter", cancel=False),
+], ids=["k_eigenvalue", "fixed_source", "modified_fixed_source"])
+def test_empty_bank(make):
+    """transport() of an empty bank returns an empty fission bank and
+    scores nothing (the reference's loop over the bank does no work)."""
+    o = Oracle(make())
+    r = o.transport(o.sample_sources(0, 0), kcol=0.9, nthreads=1, trace_cap=4)
+    assert len(r["fission"]) == 0 and r["n_rec"].shape == (0,)
+    s = r["scores"]
+    assert (s.k_col, s.k_abs, s.k_trk, s.leakage, s.mig) == (0.0,) * 5
+    assert r["diag"].histories == 0 and r["diag"].collisions == 0
+    assert not o.tally(0).any()
