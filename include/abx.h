@@ -346,7 +346,10 @@ int abx_sample_sources(abx_problem* p, uint64_t first_history, size_t n,
  * the reference's order to `fission_out` (capacity `cap`, count in *n_out).
  * `kcol` is Tallies::kcol() of the previous generation; `converged` gates
  * the mesh tallies (Tallies::score_collision / score_flight).  Scores add to
- * the problem's current-generation tallies. */
+ * the problem's current-generation tallies.  n may be 0 (no sites, no
+ * scores).  If the sites outnumber `cap`, *n_out still receives their
+ * count and the call returns ABX_E_CAPACITY; the generation's scores have
+ * been added by then. */
 int abx_transport(abx_problem* p, const abx_particle* bank, size_t n,
                   double kcol, int converged, abx_banked* fission_out,
                   size_t cap, size_t* n_out, abx_scores* scores,
