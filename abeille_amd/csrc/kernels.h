@@ -73,6 +73,15 @@ __device__ __forceinline__ void raise_error(const KCtx& K, int code, uint64_t h)
 // stage_parent value of a staging slot that holds no site.
 constexpr uint32_t kHole = 0xFFFFFFFFu;
 
+// 1: mesh-tally atomics are aggregated over the lanes of a wavefront that
+// score the same bin (GpuSink::tally_add); 0: one atomic per lane.  Off by
+// default: on the lattice a wave's lanes almost never score one bin at the
+// same time (vector stores per wave 1080 with it off, 1079 on), and the
+// match costs ~1 % of kernel time.
+#ifndef ABX_TALLY_AGG
+#define ABX_TALLY_AGG 0
+#endif
+
 // Global-memory side of the physics for one lane.  TRACE compiles the
 // per-event trace records in; SEC the per-history secondary stack; SPLIT
 // selects the split-precision physics.
@@ -136,7 +145,28 @@ struct GpuSink {
   }
 
   __device__ void tally_add(double* a, double v) {
+#if ABX_TALLY_AGG
+    // Wavefront aggregation: the lanes scoring the same bin as the first
+    // active lane add their scores together (in lane order) and that lane
+    // issues one atomic for them; every other lane adds its own.
+    const uintptr_t ai = (uintptr_t)a;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)ai);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(ai >> 32));
+    const bool same = ai == (((uintptr_t)hi << 32) | lo);
+    const unsigned long long grp = __ballot(same);
+    if (__popcll(grp) > 1) {
+      double s = 0.0;
+      // grp is uniform and its lanes are all active, so every shuffle
+      // reads an active lane
+      for (unsigned long long g = grp; g; g &= g - 1ull) s += __shfl(v, __ffsll(g) - 1);
+      if (!same) unsafeAtomicAdd(a, v);
+      else if ((threadIdx.x & 63) == __ffsll(grp) - 1) unsafeAtomicAdd(a, s);
+    } else {
+      unsafeAtomicAdd(a, v);
+    }
+#else
     unsafeAtomicAdd(a, v);
+#endif
     c[5]++;
   }
   __device__ bool sec_push(const SecRec& q) {
