@@ -73,6 +73,8 @@ def lib():
         L.abx_batch_set_params.argtypes = [P, C.c_int64, C.c_int64]
         L.abx_batch_need.argtypes = [P, P, C.c_int, C.POINTER(C.c_size_t)]
         L.abx_problem_stats.argtypes = [P, C.POINTER(abi.RunStats)]
+        L.abx_fission_fetch.argtypes = [P, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]
+        L.abx_gen_values.argtypes = [P, C.c_int, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]
         _lib = L
     return _lib
 
@@ -170,6 +172,22 @@ class Problem:
         _check(lib().abx_run_continue(self.h, ngenerations, nignored, kh.ctypes.data,
                                       C.byref(st)))
         return st, kh
+
+    def fission_fetch(self, cap):
+        """The ordered fission sites of the last transport (abx_fission_fetch),
+        e.g. after a transport whose output buffer was too small."""
+        out = np.zeros(max(1, cap), abi.BANKED_DTYPE)
+        n = C.c_size_t()
+        _check(lib().abx_fission_fetch(self.h, out.ctypes.data, cap, C.byref(n)))
+        return out[:n.value].copy()
+
+    def gen_values(self, quantity):
+        """Per-generation values since the run started (abx_gen_values; abi.GEN_*)."""
+        n = C.c_size_t()
+        _check(lib().abx_gen_values(self.h, quantity, None, 0, C.byref(n)))
+        out = np.zeros(max(1, n.value))
+        _check(lib().abx_gen_values(self.h, quantity, out.ctypes.data, len(out), C.byref(n)))
+        return out[:n.value].copy()
 
     def cancel_bank(self, bank):
         b = np.ascontiguousarray(bank, dtype=abi.BANKED_DTYPE).copy()

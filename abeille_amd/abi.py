@@ -147,7 +147,9 @@ class Diag(C.Structure):
 
 class RunStats(C.Structure):
     _fields_ = [("kcol_avg", C.c_double), ("kcol_err", C.c_double),
+                ("kabs_avg", C.c_double), ("kabs_err", C.c_double),
                 ("ktrk_avg", C.c_double), ("ktrk_err", C.c_double),
+                ("ktot_avg", C.c_double), ("ktot_err", C.c_double),
                 ("leak_avg", C.c_double), ("leak_err", C.c_double),
                 ("mig_avg", C.c_double), ("mig_err", C.c_double),
                 ("active_generations", C.c_int32), ("pad_", C.c_int32),
@@ -252,3 +254,6 @@ class Tuning(C.Structure):
                 ("stage_tables", C.c_int32), ("tail_threshold", C.c_int32),
                 ("sort_every", C.c_int32)]
 
+# abx_gen_quantity: per-generation vectors (abx_gen_values)
+GEN_KCOL, GEN_KABS, GEN_KTRK, GEN_KTOT, GEN_LEAKAGE, GEN_MIG = range(6)
+GEN_NAMES = ("kcol", "kabs", "ktrk", "ktot", "leakage", "mig-area")

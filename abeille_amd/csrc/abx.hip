@@ -61,6 +61,15 @@ struct DBuf {
 
 int64_t blocks_for(int64_t n, int bs) { return (n + bs - 1) / bs; }
 
+// A kernel is about to read or write n records of `rec` bytes in b: refuse
+// to launch it over a buffer that is too small.
+void require_records(const DBuf& b, int64_t n, size_t rec, const char* what) {
+  if (n > 0 && b.bytes < (size_t)n * rec)
+    fail(ABX_E_INVALID, std::string("internal: ") + what + " buffer holds " +
+                            std::to_string(b.bytes / rec) + " records, " + std::to_string(n) +
+                            " needed");
+}
+
 }  // namespace
 
 struct abx_problem {
@@ -79,7 +88,7 @@ struct abx_problem {
   // transport scratch
   DBuf in, sec, stage, stage_parent, stage_daughter, n_stage, daughters, offsets,
       block_sums, block_sums2, ordered, scores, counters, err, err_hist, trace, n_rec, wsum,
-      pool_next, blob, gdev, khist_dev, one_prob, one_ctx, one_off;
+      pool_next, blob, gdev, khist_dev;
   EventBuffers ev;
   std::vector<hipEvent_t> gen_events;  // graph runs: 2 per generation
   // cancellation
@@ -102,9 +111,13 @@ struct abx_problem {
   hipEvent_t ke0 = nullptr, ke1 = nullptr;
   // scalar statistics (src/tallies.cpp:268-280)
   int gen = 0;
-  double kc_avg = 0, kc_var = 0, kt_avg = 0, kt_var = 0, lk_avg = 0, lk_var = 0, mg_avg = 0,
-         mg_var = 0;
+  // running means and variances of k_col k_abs k_trk k_tot leakage mig
+  double s_avg[6] = {0, 0, 0, 0, 0, 0}, s_var[6] = {0, 0, 0, 0, 0, 0};
+  // every generation's six values since the run started (the reference's
+  // k_col_vec ... mig_vec, inactive generations included)
+  std::vector<double> gen_vals[6];
   abx_scores gen_scores{};
+  int64_t last_sites = 0;  // ordered fission sites of the last transport (in `ordered`)
   abx_diag run_diag{};     // summed over batch-engine steps since attach
   int64_t run_histories = 0;
 
@@ -124,7 +137,7 @@ struct abx_problem {
                    &stage_daughter, &n_stage, &daughters, &offsets, &block_sums, &block_sums2,
                    &ordered, &scores, &counters, &err, &err_hist, &trace, &n_rec, &wsum,
                    &bank_a, &bank_b, &ckeys, &ckeys2, &ctemp, &pool_next, &blob, &gdev,
-                   &khist_dev, &one_prob, &one_ctx, &one_off};
+                   &khist_dev};
     for (auto e : gen_events) (void)hipEventDestroy(e);
     for (DBuf* b : all) b->release();
     ev.release();
@@ -274,12 +287,20 @@ int64_t order_fission_bank(abx_problem* p, int64_t n, uint64_t n_staged, uint64_
     fail(ABX_E_INVALID, "fission site count mismatch");
   p->ordered.reserve((size_t)std::max<uint64_t>(1, total) * sizeof(abx_banked));
   if (total) {
+    p->err.reserve(sizeof(int32_t));
+    HIPC(hipMemsetAsync(p->err.p, 0, sizeof(int32_t), p->stream));
     hipLaunchKernelGGL(k_order_bank, dim3((unsigned)blocks_for((int64_t)n_staged, 256)), dim3(256),
                        0, p->stream, p->stage.as<abx_banked>(), p->stage_parent.as<uint32_t>(),
                        p->stage_daughter.as<uint32_t>(), p->offsets.as<uint64_t>(),
-                       (int64_t)n_staged, p->ordered.as<abx_banked>());
+                       (int64_t)n_staged, n, total, p->ordered.as<abx_banked>(),
+                       p->err.as<int32_t>());
     HIPC(hipGetLastError());
+    int32_t err = 0;
+    HIPC(hipMemcpyAsync(&err, p->err.p, sizeof(int32_t), hipMemcpyDeviceToHost, p->stream));
+    HIPC(hipStreamSynchronize(p->stream));
+    if (err) fail(err, "fission bank ordering: a staging slot names no parent of this call");
   }
+  p->last_sites = (int64_t)total;
   return (int64_t)total;
 }
 
@@ -291,8 +312,6 @@ void add_scores(abx_scores& a, const abx_scores& b) {
   a.leakage += b.leakage;
   a.mig += b.mig;
 }
-
-hipError_t launch_one_as_batch(abx_problem* p, const KCtx& K);
 
 // One transport() over a device-resident bank; ordered fission sites are
 // left in p->ordered.  Tallies accumulate into tally_gen.
@@ -311,6 +330,7 @@ TransportOut transport_device(abx_problem* p, const abx_particle* d_in, int64_t 
   HIPC(hipMemsetAsync(p->counters.p, 0, 8 * sizeof(unsigned long long), p->stream));
   HIPC(hipMemsetAsync(p->err.p, 0, sizeof(int32_t), p->stream));
   HIPC(hipMemsetAsync(p->n_stage.p, 0, 2 * sizeof(unsigned long long), p->stream));
+  p->last_sites = 0;
   const bool banks = T.st.mode != ABX_FIXED_SOURCE;
   const int sec_cap = needs_secondaries(T) ? 16 : 0;
   // The stack is indexed by persistent lane (history engine) or by history
@@ -334,16 +354,8 @@ TransportOut transport_device(abx_problem* p, const abx_particle* d_in, int64_t 
     if (p->engine == ABX_ENGINE_HISTORY) {
       if (p->timing) HIPC(hipEventRecord(p->ke0, p->stream));
       p->pool_next.reserve(sizeof(unsigned long long));
-      if (sec_cap > 0 && !d_trace) {
-        // Histories that carry secondaries (carter splitting, fixed-source
-        // fission) run in the batch kernel with this one problem: the
-        // single-problem kernel's secondary-stack variant faulted on large
-        // modified-fixed-source carter runs and is kept for traced runs only.
-        HIPC(launch_one_as_batch(p, K));
-      } else {
-        HIPC(history_transport(p->P, K, p->stream, sec_cap > 0,
-                               p->pool_next.as<unsigned long long>(), p->tuning));
-      }
+      HIPC(history_transport(p->P, K, p->stream, sec_cap > 0,
+                             p->pool_next.as<unsigned long long>(), p->tuning));
       if (p->timing) {
         HIPC(hipEventRecord(p->ke1, p->stream));
         HIPC(hipEventSynchronize(p->ke1));
@@ -389,26 +401,49 @@ void upd(int gen, double x, double& avg, double& var) {
 
 // The scalar part of a record (Tallies::record_generation,
 // src/tallies.cpp:159-203) and the tally generation count.
+void score_values(const abx_scores& s, double W, double x[6]) {
+  x[0] = s.k_col / W;
+  x[1] = s.k_abs / W;
+  x[2] = s.k_trk / W;
+  x[3] = s.k_tot / W;
+  x[4] = s.leakage / W;
+  x[5] = s.mig / W;
+}
+
+// Tallies::calc_gen_values: the generation's values join the per-generation
+// vectors (every generation, recorded or not).
+void push_gen_values(abx_problem* p, const abx_scores& s, double W) {
+  double x[6];
+  score_values(s, W, x);
+  for (int q = 0; q < 6; q++) p->gen_vals[q].push_back(x[q]);
+}
+
 void record_scores(abx_problem* p, const abx_scores& s, double W) {
   p->gen++;
-  upd(p->gen, s.k_col / W, p->kc_avg, p->kc_var);
-  upd(p->gen, s.k_trk / W, p->kt_avg, p->kt_var);
-  upd(p->gen, s.leakage / W, p->lk_avg, p->lk_var);
-  upd(p->gen, s.mig / W, p->mg_avg, p->mg_var);
+  double x[6];
+  score_values(s, W, x);
+  for (int q = 0; q < 6; q++) upd(p->gen, x[q], p->s_avg[q], p->s_var[q]);
   p->tally_g++;
+}
+
+// MeshTally::record_generation (src/mesh_tally.cpp:121-150) of every mesh
+// tally, as record number p->tally_g.
+void record_tallies(abx_problem* p, double multiplier, hipStream_t st = nullptr,
+                    double* gen = nullptr) {
+  const int64_t nb = p->T.tally_bins;
+  if (nb > 0) {
+    hipLaunchKernelGGL(k_record_generation, dim3((unsigned)blocks_for(nb, 256)), dim3(256), 0,
+                       st ? st : p->stream, gen ? gen : p->tally_gen.as<double>(),
+                       p->tally_avg.as<double>(), p->tally_var.as<double>(), nb,
+                       (double)p->tally_g, 0, multiplier);
+    HIPC(hipGetLastError());
+  }
 }
 
 void record_generation(abx_problem* p, const abx_scores& s, double W,
                        hipStream_t st = nullptr, double* gen = nullptr) {
   record_scores(p, s, W);
-  const int64_t nb = p->T.tally_bins;
-  if (nb > 0) {
-    hipLaunchKernelGGL(k_record_generation, dim3((unsigned)blocks_for(nb, 256)), dim3(256), 0,
-                       st ? st : p->stream, gen ? gen : p->tally_gen.as<double>(),
-                       p->tally_avg.as<double>(),
-                       p->tally_var.as<double>(), nb, (double)p->tally_g, 0);
-    HIPC(hipGetLastError());
-  }
+  record_tallies(p, 1.0, st, gen);
 }
 
 void clear_generation(abx_problem* p, hipStream_t st = nullptr, double* gen = nullptr) {
@@ -654,8 +689,15 @@ static int transport_host(abx_problem* p, const abx_particle* bank, size_t n, do
     TransportOut r = transport_device(p, p->in.as<abx_particle>(), (int64_t)n, kcol, converged,
                                       d_trace, cap_per_history, d_nrec);
     if (n_out) *n_out = (size_t)r.n_sites;
+    // the scores are the call's whatever happens to the sites below
+    if (scores) *scores = r.scores;
+    if (diag) *diag = r.diag;
     if (r.n_sites > 0) {
-      if (!fission_out || (size_t)r.n_sites > cap) fail(ABX_E_CAPACITY, "fission_out capacity");
+      // the ordered sites stay on the device: abx_fission_fetch can still
+      // hand them over after this capacity error
+      if (!fission_out || (size_t)r.n_sites > cap)
+        fail(ABX_E_CAPACITY, "fission_out holds " + std::to_string(cap) + " sites, " +
+                                 std::to_string(r.n_sites) + " were banked");
       HIPC(hipMemcpyAsync(fission_out, p->ordered.p, (size_t)r.n_sites * sizeof(abx_banked),
                           hipMemcpyDeviceToHost, p->stream));
     }
@@ -665,8 +707,6 @@ static int transport_host(abx_problem* p, const abx_particle* bank, size_t n, do
       HIPC(hipMemcpyAsync(n_rec, d_nrec, n * sizeof(int32_t), hipMemcpyDeviceToHost, p->stream));
     }
     HIPC(hipStreamSynchronize(p->stream));
-    if (scores) *scores = r.scores;
-    if (diag) *diag = r.diag;
   });
 }
 
@@ -689,6 +729,22 @@ int abx_transport_trace(abx_problem* p, const abx_particle* bank, size_t n, doub
                         trace, cap_per_history, n_rec);
 }
 
+int abx_fission_fetch(abx_problem* p, abx_banked* out, size_t cap, size_t* n_out) {
+  ABX_TRY({
+    if (!p) fail(ABX_E_INVALID, "null problem");
+    const size_t n = (size_t)p->last_sites;
+    if (n_out) *n_out = n;
+    if (n > 0) {
+      if (!out || n > cap)
+        fail(ABX_E_CAPACITY, "fission_out holds " + std::to_string(cap) + " sites, " +
+                                 std::to_string(n) + " were banked");
+      HIPC(hipMemcpyAsync(out, p->ordered.p, n * sizeof(abx_banked), hipMemcpyDeviceToHost,
+                          p->stream));
+      HIPC(hipStreamSynchronize(p->stream));
+    }
+  });
+}
+
 int abx_tally_size(abx_problem* p, int tally, size_t* n_bins) {
   ABX_TRY({
     if (!p || !n_bins || tally < 0 || tally >= (int)p->T.tally.size()) fail(ABX_E_INVALID, "tally index");
@@ -704,6 +760,15 @@ int abx_tally_fetch(abx_problem* p, int tally, double* gen_out) {
     const DTally& t = p->T.tally[tally];
     HIPC(hipMemcpyAsync(gen_out, p->tally_gen.as<double>() + t.data_off, nb * sizeof(double),
                         hipMemcpyDeviceToHost, p->stream));
+    HIPC(hipStreamSynchronize(p->stream));
+  });
+}
+
+int abx_tally_record(abx_problem* p, double multiplier) {
+  ABX_TRY({
+    if (!p) fail(ABX_E_INVALID, "null problem");
+    p->tally_g++;
+    record_tallies(p, multiplier);
     HIPC(hipStreamSynchronize(p->stream));
   });
 }
@@ -734,7 +799,10 @@ int abx_tally_stats(abx_problem* p, int tally, double* avg, double* std_err) {
 static void run_reset(abx_problem* p) {
   const HostTables& T = p->T;
   p->gen = 0;
-  p->kc_avg = p->kc_var = p->kt_avg = p->kt_var = p->lk_avg = p->lk_var = p->mg_avg = p->mg_var = 0;
+  for (int q = 0; q < 6; q++) {
+    p->s_avg[q] = p->s_var[q] = 0.0;
+    p->gen_vals[q].clear();
+  }
   p->tally_g = 0;
   const size_t tb = (size_t)std::max<int64_t>(1, T.tally_bins) * sizeof(double);
   HIPC(hipMemsetAsync(p->tally_avg.p, 0, tb, p->stream));
@@ -749,14 +817,20 @@ static void run_reset(abx_problem* p) {
 // Tallies' running means and standard errors (src/tallies.cpp:268-280).
 static void fill_statistics(const abx_problem* p, abx_run_stats* stats) {
   const double dg = (double)std::max(1, p->gen);
-  stats->kcol_avg = p->kc_avg;
-  stats->kcol_err = std::sqrt(p->kc_var / dg);
-  stats->ktrk_avg = p->kt_avg;
-  stats->ktrk_err = std::sqrt(p->kt_var / dg);
-  stats->leak_avg = p->lk_avg;
-  stats->leak_err = std::sqrt(p->lk_var / dg);
-  stats->mig_avg = p->mg_avg;
-  stats->mig_err = std::sqrt(p->mg_var / dg);
+  double err[6];
+  for (int q = 0; q < 6; q++) err[q] = std::sqrt(p->s_var[q] / dg);
+  stats->kcol_avg = p->s_avg[0];
+  stats->kcol_err = err[0];
+  stats->kabs_avg = p->s_avg[1];
+  stats->kabs_err = err[1];
+  stats->ktrk_avg = p->s_avg[2];
+  stats->ktrk_err = err[2];
+  stats->ktot_avg = p->s_avg[3];
+  stats->ktot_err = err[3];
+  stats->leak_avg = p->s_avg[4];
+  stats->leak_err = err[4];
+  stats->mig_avg = p->s_avg[5];
+  stats->mig_err = err[5];
   stats->active_generations = p->gen;
 }
 
@@ -798,7 +872,7 @@ static bool run_generations_graph(abx_problem* p, int ngenerations, int nignored
   p->block_sums.reserve((size_t)(nb + 1) * sizeof(uint64_t));
   p->ordered.reserve(cap * sizeof(abx_banked));
   p->gdev.reserve(sizeof(GenDev));
-  p->khist_dev.reserve((size_t)std::max(1, ngenerations) * sizeof(double));
+  p->khist_dev.reserve((size_t)6 * std::max(1, ngenerations) * sizeof(double));
   p->pool_next.reserve(sizeof(unsigned long long));
   GenDev* g = p->gdev.as<GenDev>();
   GenDev h{};
@@ -807,14 +881,11 @@ static bool run_generations_graph(abx_problem* p, int ngenerations, int nignored
   h.cap = cap;
   h.kcol = p->run_kcol;
   h.gen = p->gen;
-  h.kc_avg = p->kc_avg;
-  h.kc_var = p->kc_var;
-  h.kt_avg = p->kt_avg;
-  h.kt_var = p->kt_var;
-  h.lk_avg = p->lk_avg;
-  h.lk_var = p->lk_var;
-  h.mg_avg = p->mg_avg;
-  h.mg_var = p->mg_var;
+  for (int q = 0; q < 6; q++) {
+    h.s_avg[q] = p->s_avg[q];
+    h.s_var[q] = p->s_var[q];
+  }
+  h.hist_cap = std::max(1, ngenerations);
   HIPC(hipMemcpyAsync(g, &h, sizeof(GenDev), hipMemcpyHostToDevice, s));
   HIPC(hipStreamSynchronize(s));
   // transport context: counts and scalars live in GenDev
@@ -834,15 +905,16 @@ static bool run_generations_graph(abx_problem* p, int ngenerations, int nignored
   if (sec) return false;  // k-eigenvalue without carter tracking only
   bool stage = false;
   int64_t grid = 0;
-  HIPC(dispatch_variant(trk, false, false, split, [&](auto TT, auto, auto, auto SP) {
+  // (carter tracking never gets here: it carries the secondary stack)
+  HIPC(dispatch_variant(trk, false, false, split, [&](auto TT, auto, auto SE, auto SP) {
     constexpr int TK = decltype(TT)::value;
-    constexpr bool SPL = decltype(SP)::value;
-    stage = tu.stage_tables && stage_pays(k_history<TK, false, false, SPL, true>,
-                                          k_history<TK, false, false, SPL, false>, bs, cur,
+    constexpr bool SEC = decltype(SE)::value, SPL = decltype(SP)::value;
+    stage = tu.stage_tables && stage_pays(k_history<TK, false, SEC, SPL, true>,
+                                          k_history<TK, false, SEC, SPL, false>, bs, cur,
                                           (size_t)p->P.blob_bytes);
-    grid = stage ? resident_blocks(k_history<TK, false, false, SPL, true>, bs,
+    grid = stage ? resident_blocks(k_history<TK, false, SEC, SPL, true>, bs,
                                    cur + (size_t)p->P.blob_bytes)
-                 : resident_blocks(k_history<TK, false, false, SPL, false>, bs, cur);
+                 : resident_blocks(k_history<TK, false, SEC, SPL, false>, bs, cur);
     return hipSuccess;
   }));
   const size_t lds = stage ? cur + (size_t)p->P.blob_bytes : cur;
@@ -865,12 +937,14 @@ static bool run_generations_graph(abx_problem* p, int ngenerations, int nignored
     Kc.converged = conv;
     HIPC(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(k_gen_begin, dim3(1), dim3(64), 0, s, g);
-    HIPC(dispatch_variant(trk, false, false, split, [&](auto TT, auto, auto, auto SP) {
+    HIPC(dispatch_variant(trk, false, false, split, [&](auto TT, auto, auto SE, auto SP) {
+      constexpr int TK = decltype(TT)::value;
+      constexpr bool SEC = decltype(SE)::value, SPL = decltype(SP)::value;
       if (stage)
-        hipLaunchKernelGGL((k_history<decltype(TT)::value, false, false, decltype(SP)::value, true>),
+        hipLaunchKernelGGL((k_history<TK, false, SEC, SPL, true>),
                            dim3((unsigned)grid), dim3(bs), lds, s, p->P, Kc, &g->next);
       else
-        hipLaunchKernelGGL((k_history<decltype(TT)::value, false, false, decltype(SP)::value, false>),
+        hipLaunchKernelGGL((k_history<TK, false, SEC, SPL, false>),
                            dim3((unsigned)grid), dim3(bs), lds, s, p->P, Kc, &g->next);
       return hipSuccess;
     }));
@@ -929,9 +1003,9 @@ static bool run_generations_graph(abx_problem* p, int ngenerations, int nignored
     }
     HIPC(hipEventRecord(p->gen_events[2 * ngenerations], s));
     HIPC(hipMemcpyAsync(&h, g, sizeof(GenDev), hipMemcpyDeviceToHost, s));
-    if (kcol_hist)
-      HIPC(hipMemcpyAsync(kcol_hist, p->khist_dev.p, (size_t)ngenerations * sizeof(double),
-                          hipMemcpyDeviceToHost, s));
+    std::vector<double> hist((size_t)6 * ngenerations);
+    HIPC(hipMemcpyAsync(hist.data(), p->khist_dev.p, hist.size() * sizeof(double),
+                        hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
     // the latest bank is in bank_b after an odd number of generations
     if (par) std::swap(p->bank_a, p->bank_b);
@@ -950,14 +1024,17 @@ static bool run_generations_graph(abx_problem* p, int ngenerations, int nignored
     }
     p->gen = h.gen;
     p->tally_g = h.gen;
-    p->kc_avg = h.kc_avg;
-    p->kc_var = h.kc_var;
-    p->kt_avg = h.kt_avg;
-    p->kt_var = h.kt_var;
-    p->lk_avg = h.lk_avg;
-    p->lk_var = h.lk_var;
-    p->mg_avg = h.mg_avg;
-    p->mg_var = h.mg_var;
+    for (int q = 0; q < 6; q++) {
+      p->s_avg[q] = h.s_avg[q];
+      p->s_var[q] = h.s_var[q];
+    }
+    // the generations that ran (a device error stops the run early)
+    const int ran = std::min(ngenerations, (int)h.hist_idx);
+    for (int q = 0; q < 6; q++)
+      p->gen_vals[q].insert(p->gen_vals[q].end(), hist.begin() + (size_t)q * ngenerations,
+                            hist.begin() + (size_t)q * ngenerations + ran);
+    if (kcol_hist) std::copy(hist.begin(), hist.begin() + ngenerations, kcol_hist);
+    p->last_sites = (int64_t)h.n_sites;
     p->bank_n = (size_t)h.n;
     p->run_hist = h.run_hist;
     p->run_kcol = h.kcol;
@@ -1025,6 +1102,11 @@ static void run_generations(abx_problem* p, int ngenerations, int nignored, doub
     abx_diag dg{};
     abx_scores sc{};
     if (T.st.mode != ABX_K_EIGENVALUE) {
+      // A modified-fixed-source chain swaps bank_a with bank_b, which was
+      // sized for the chain's last re-banked generation: the next batch's
+      // N sources need bank_a grown back to N records.
+      p->bank_a.reserve((size_t)N * sizeof(abx_particle));
+      require_records(p->bank_a, N, sizeof(abx_particle), "source bank");
       sample_sources_device(p, p->run_hist, N, p->bank_a.as<abx_particle>());
       p->run_hist += (uint64_t)N;
       int64_t n = N;
@@ -1037,6 +1119,7 @@ static void run_generations(abx_problem* p, int ngenerations, int nignored, doub
         n = r.n_sites;
         if (p->cancel) cancel_device(p, p->ordered.as<abx_banked>(), n);
         p->bank_b.reserve((size_t)n * sizeof(abx_particle));
+        require_records(p->bank_b, n, sizeof(abx_particle), "re-banked generation");
         hipLaunchKernelGGL(k_rebank, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0, p->stream,
                            p->ordered.as<abx_banked>(), n, 1.0, p->run_hist, T.st.seed, T.st.stride, 1, 0,
                            p->bank_b.as<abx_particle>());
@@ -1046,11 +1129,7 @@ static void run_generations(abx_problem* p, int ngenerations, int nignored, doub
         r = transport_device(p, p->bank_a.as<abx_particle>(), n, p->run_kcol, 1, nullptr, 0, nullptr);
         histories += n;
         add_diag(dg, r.diag);
-        sc.k_col += r.scores.k_col;
-        sc.k_abs += r.scores.k_abs;
-        sc.k_trk += r.scores.k_trk;
-        sc.leakage += r.scores.leakage;
-        sc.mig += r.scores.mig;
+        add_scores(sc, r.scores);
       }
       HIPC(hipEventRecord(e1, p->stream));
       HIPC(hipEventSynchronize(e1));
@@ -1058,6 +1137,7 @@ static void run_generations(abx_problem* p, int ngenerations, int nignored, doub
       HIPC(hipEventElapsedTime(&ms, e0, e1));
       ms_total += ms;
       if (kcol_hist) kcol_hist[g - 1] = sc.k_col / W;
+      push_gen_values(p, sc, W);
       record_generation(p, sc, W);
     } else {
       HIPC(hipEventRecord(e0, p->stream));
@@ -1074,6 +1154,7 @@ static void run_generations(abx_problem* p, int ngenerations, int nignored, doub
       if (r.n_sites == 0) fail(ABX_E_INVALID, "No fission neutrons were produced.");
       const double kc = sc.k_col / W;
       if (kcol_hist) kcol_hist[g - 1] = kc;
+      push_gen_values(p, sc, W);
       // perform_regional_cancellation, then normalize_weights: W = W_pos -
       // W_neg, w *= nparticles / W (src/power_iterator.cpp:354-359)
       if (p->cancel) cancel_device(p, p->ordered.as<abx_banked>(), r.n_sites);
@@ -1129,6 +1210,18 @@ int abx_run_continue(abx_problem* p, int ngenerations, int nignored, double* kco
   ABX_TRY({
     if (!p || !stats || ngenerations < 1 || nignored < 0) fail(ABX_E_INVALID, "run arguments");
     run_generations(p, ngenerations, nignored, kcol_hist, stats);
+  });
+}
+
+int abx_gen_values(abx_problem* p, int quantity, double* out, size_t cap, size_t* n) {
+  ABX_TRY({
+    if (!p || quantity < 0 || quantity > 5) fail(ABX_E_INVALID, "quantity");
+    const std::vector<double>& v = p->gen_vals[quantity];
+    if (n) *n = v.size();
+    if (out && !v.empty()) {  // out == NULL: the count only
+      if (v.size() > cap) fail(ABX_E_CAPACITY, "per-generation buffer too small");
+      std::copy(v.begin(), v.end(), out);
+    }
   });
 }
 
@@ -1363,32 +1456,6 @@ hipError_t launch_chunks_variant(int key, const DevProblem* probs, const KCtx* c
     return launch_chunks<TK, SEC, SPL, false>(probs, ctxs, chunks, n_chunks, D, L, blob_max, next,
                                               s);
   });
-}
-
-// The batch kernel over one problem (see transport_device).  Chunks hold at
-// least 256 histories, so its lanes index the secondary stack below
-// ceil(n / 256) * 256, within the n + 256 slots transport_device reserves.
-hipError_t launch_one_as_batch(abx_problem* p, const KCtx& K) {
-  const std::vector<BatchChunk> ch = make_chunks({K.n}, chunk_length(K.n));
-  p->one_prob.reserve(sizeof(DevProblem));
-  p->one_ctx.reserve(sizeof(KCtx));
-  p->one_off.reserve(std::max<size_t>(1, ch.size()) * sizeof(BatchChunk));
-  hipError_t e;
-  if ((e = hipMemcpyAsync(p->one_prob.p, &p->P, sizeof(DevProblem), hipMemcpyHostToDevice,
-                          p->stream)) != hipSuccess)
-    return e;
-  if ((e = hipMemcpyAsync(p->one_ctx.p, &K, sizeof(KCtx), hipMemcpyHostToDevice, p->stream)) !=
-      hipSuccess)
-    return e;
-  if ((e = hipMemcpyAsync(p->one_off.p, ch.data(), ch.size() * sizeof(BatchChunk),
-                          hipMemcpyHostToDevice, p->stream)) != hipSuccess)
-    return e;
-  // the host copies above are read before this call returns
-  if ((e = hipStreamSynchronize(p->stream)) != hipSuccess) return e;
-  return launch_chunks_variant(variant_key(p), p->one_prob.as<DevProblem>(), p->one_ctx.as<KCtx>(),
-                               p->one_off.as<BatchChunk>(), (int)ch.size(), p->P.trk_D,
-                               p->P.trk_L, (size_t)p->P.blob_bytes,
-                               reinterpret_cast<unsigned*>(p->pool_next.p), p->stream);
 }
 
 // Bytes a problem holds on the device while it runs (tables, tallies,
@@ -1701,6 +1768,7 @@ void batch_step(abx_batch* b, std::vector<abx_problem*>& finished) {
       if (nk > 1) {
         // a whole batch of its own; its tallies are folded in by the
         // record job registered above
+        push_gen_values(p, r.scores, (double)T.st.nparticles);
         record_scores(p, r.scores, (double)T.st.nparticles);
         p->gen_scores = abx_scores{};
         it.batches_left--;
@@ -1728,6 +1796,7 @@ void batch_step(abx_batch* b, std::vector<abx_problem*>& finished) {
         it.mid_chain = false;
         it.done = 0;
         record_tallies(p, p->tally_gen.as<double>(), 1, p->tally_g);
+        push_gen_values(p, it.acc, (double)T.st.nparticles);
         record_scores(p, it.acc, (double)T.st.nparticles);
         p->gen_scores = abx_scores{};
         it.batches_left--;

@@ -20,13 +20,13 @@
 // cost more than the instruction-cache misses it saved.  -DABX_OUTLINE=1
 // builds them out of line.
 #if defined(ABX_OUTLINE) && ABX_OUTLINE
-#define ABX_NOINL __host__ __device__ __attribute__((noinline))
+#define ABX_NOINL static __host__ __device__ __attribute__((noinline))
 #else
 #define ABX_NOINL __host__ __device__ __forceinline__
 #endif
 // Rarely executed paths: always out of line on the device, so that their
 // code is not copied into every inlined caller.
-#define ABX_COLD __host__ __device__ __attribute__((noinline))
+#define ABX_COLD static __host__ __device__ __attribute__((noinline))
 #else
 #define ABX_HD static inline
 #define ABX_NOINL static

@@ -286,6 +286,7 @@ __global__ void ABX_TKERNEL k_ev_tail(DevProblem P, KCtx K, EvState S,
   flush_scores(K, sc, s.c);
 }
 
+#ifndef ABX_KERNELS_ONLY
 // Sort keys of the collision queue: (material, group) of each queued slot,
 // so that a sorted queue hands each wavefront particles of one material
 // and group (the same table rows, the same fissile / non-fissile branch).
@@ -296,6 +297,10 @@ __global__ void __launch_bounds__(256) k_ev_sort_keys(EvState S, const int32_t* 
   const int32_t slot = q[k];
   keys[k] = (uint16_t)(((S.m[slot] & 0xff) << 8) | (S.g[slot] & 0xff));
 }
+#endif  // ABX_KERNELS_ONLY
+
+// The transport kernels' variants are compiled in instances.hip.
+#include "instances.h"
 
 // Host-side launch code follows; ABX_KERNELS_ONLY leaves it out (a
 // translation unit that only instantiates chosen kernels).

@@ -1,0 +1,35 @@
+// instances.hip - device code of the transport kernel variants, one part per
+// translation unit: built with -DABX_INST_PART=0..7 (abeille_amd/build.py),
+// the parts compile in parallel and link into libabx.so beside abx.hip.
+#define ABX_KERNELS_ONLY 1
+#include "event.h"
+
+#ifndef ABX_INST_PART
+#error "ABX_INST_PART must be defined (0..7)"
+#endif
+
+#define ABX_IH(T, TR, SE, SP) ABX_HIST_DECL(, T, TR, SE, SP)
+#define ABX_IB(T, SE, SP) ABX_BATCH_DECL(, T, SE, SP)
+#define ABX_IE(T, TR, SE, SP) ABX_EVENT_DECL(, T, TR, SE, SP)
+
+#if ABX_INST_PART == 0
+ABX_VARIANTS_T0(ABX_IH)
+#elif ABX_INST_PART == 1
+ABX_VARIANTS_T1(ABX_IH)
+#elif ABX_INST_PART == 2
+ABX_VARIANTS_T2(ABX_IH)
+ABX_BATCH_T2(ABX_IB)
+#elif ABX_INST_PART == 3
+ABX_BATCH_T0(ABX_IB)
+#elif ABX_INST_PART == 4
+ABX_BATCH_T1(ABX_IB)
+ABX_INIT_DECL()
+#elif ABX_INST_PART == 5
+ABX_VARIANTS_T0(ABX_IE)
+#elif ABX_INST_PART == 6
+ABX_VARIANTS_T1(ABX_IE)
+#elif ABX_INST_PART == 7
+ABX_VARIANTS_T2(ABX_IE)
+#else
+#error "ABX_INST_PART out of range"
+#endif

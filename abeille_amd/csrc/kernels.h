@@ -508,6 +508,10 @@ __global__ void ABX_TKERNEL k_batch_chunks(const DevProblem* __restrict__ probs,
   }
 }
 
+// The bookkeeping kernels below are compiled with the host code (abx.hip)
+// only, not in the parts of instances.hip.
+#ifndef ABX_KERNELS_ONLY
+
 __global__ void k_sample_sources(DevProblem P, DevSources S, uint64_t first, int64_t n,
                                  abx_particle* out, int32_t* err,
                                  unsigned long long* err_hist) {
@@ -638,24 +642,37 @@ __global__ void k_add_block_offsets(uint64_t* out, const uint64_t* block_off, in
   if (i < n) out[i] += block_off[blockIdx.x];
 }
 
-// scatter staged fission sites to parent order
+// Scatters staged fission sites to parent order.  A slot whose parent is
+// not one of the n parents, or whose destination lies past the `total`
+// sites, would be a staging slot handed out but never written: it raises
+// ABX_E_INVALID in *err instead of writing.
 __global__ void k_order_bank(const abx_banked* stage, const uint32_t* parent,
                              const uint32_t* daughter, const uint64_t* offsets, int64_t m,
-                             abx_banked* out) {
+                             int64_t n, uint64_t total, abx_banked* out, int32_t* err) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= m) return;
   const uint32_t q = parent[k];
-  if (q != kHole) out[offsets[q] + daughter[k]] = stage[k];
+  if (q == kHole) return;
+  if ((int64_t)q >= n) {
+    atomicCAS(err, 0, ABX_E_INVALID);
+    return;
+  }
+  const uint64_t dst = offsets[q] + daughter[k];
+  if (dst >= total) {
+    atomicCAS(err, 0, ABX_E_INVALID);
+    return;
+  }
+  out[dst] = stage[k];
 }
 
 // MeshTally::record_generation (src/mesh_tally.cpp:121-150), then zero the
 // generation array (clear_generation).
 __global__ void k_record_generation(double* gen, double* avg, double* var, int64_t n,
-                                    double dg, int clear) {
+                                    double dg, int clear, double multiplier) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double old_avg = avg[i];
-  const double val = gen[i] * 1.0;
+  const double val = gen[i] * multiplier;
   const double a = old_avg + (val - old_avg) / dg;
   avg[i] = a;
   double v = var[i];
@@ -801,11 +818,12 @@ struct GenDev {
   double wsum[2];                 // positive / negative banked weight
   double kcol;                    // Tallies::kcol() for the next transport
   double wpp;                     // nparticles / total weight of the re-bank
-  double kc_avg, kc_var, kt_avg, kt_var, lk_avg, lk_var, mg_avg, mg_var;
+  // running statistics of k_col k_abs k_trk k_tot leakage mig (score order)
+  double s_avg[6], s_var[6];
+  int32_t hist_cap;               // generations the per-generation arrays hold
   int32_t err;
   int32_t gen;                    // recorded (active) generations
-  int32_t hist_idx;               // generations run (index into the k history)
-  int32_t pad_;
+  int32_t hist_idx;               // generations run (index into the per-generation arrays)
 };
 
 __global__ void k_gen_begin(GenDev* g) {
@@ -895,6 +913,10 @@ __global__ void k_order_bank_dev(const abx_banked* stage, const uint32_t* parent
          k += (int64_t)gridDim.x * blockDim.x) {
       const uint32_t q = parent[k];
       if (q == kHole) continue;
+      if ((unsigned long long)q >= g->n) {  // a slot handed out but never written
+        atomicCAS(&g->err, 0, ABX_E_INVALID);
+        continue;
+      }
       const uint64_t dst = offsets[q] + daughter[k];
       if (dst >= g->cap) continue;  // k_gen_end reports the overflow
       const abx_banked b = stage[k];
@@ -931,32 +953,33 @@ __device__ __forceinline__ void gen_upd(int gen, double x, double& avg, double& 
   if (gen > 1) var = var_old + ((x - avg_old) * (x - avg_old) / dgen) - (var_old / (dgen - 1.));
 }
 
-// End of a generation (one thread): k statistics (Tallies::calc_gen_values
-// / record_generation, src/tallies.cpp:159-203), the re-bank's weight
-// factor (normalize_weights, src/power_iterator.cpp:538-586), history ids,
-// the next bank size.  A device error or an empty bank stops the run: the
-// next generations transport nothing.
-__global__ void k_gen_end(GenDev* g, double W, int converged, double* kcol_hist) {
+// End of a generation (one thread): the generation's values and the running
+// statistics of the six scores (Tallies::calc_gen_values / record_generation,
+// src/tallies.cpp:159-203), the re-bank's weight factor (normalize_weights,
+// src/power_iterator.cpp:538-586), history ids, the next bank size.  `hist`
+// ([6][hist_cap], optional) receives every generation's six values, the
+// reference's k_col_vec ... mig_vec.  A device error or an empty bank stops
+// the run: the next generations transport nothing.
+__global__ void k_gen_end(GenDev* g, double W, int converged, double* hist) {
   if (threadIdx.x != 0) return;
   if (!g->err && g->n_sites != g->n_stage - g->n_hole) g->err = -1;
   if (!g->err && g->n_stage > g->cap) g->err = -5;
   if (!g->err && g->n_sites > g->cap) g->err = -5;
   if (!g->err && g->n_sites == 0) g->err = -8;
   for (int k = 0; k < 8; k++) g->diag[k] += g->counters[k];
-  const double kc = g->scores[0] / W;
-  if (kcol_hist) kcol_hist[g->hist_idx] = kc;
+  double x[6];
+  for (int q = 0; q < 6; q++) x[q] = g->scores[q] / W;
+  if (hist && g->hist_idx < g->hist_cap)
+    for (int q = 0; q < 6; q++) hist[(int64_t)q * g->hist_cap + g->hist_idx] = x[q];
   g->hist_idx++;
   if (converged && !g->err) {
     g->gen++;
-    gen_upd(g->gen, kc, g->kc_avg, g->kc_var);
-    gen_upd(g->gen, g->scores[2] / W, g->kt_avg, g->kt_var);
-    gen_upd(g->gen, g->scores[4] / W, g->lk_avg, g->lk_var);
-    gen_upd(g->gen, g->scores[5] / W, g->mg_avg, g->mg_var);
+    for (int q = 0; q < 6; q++) gen_upd(g->gen, x[q], g->s_avg[q], g->s_var[q]);
   }
   g->wpp = W / (g->wsum[0] - g->wsum[1]);
   g->first_hist = g->run_hist;
   g->run_hist += g->n_sites;
-  g->kcol = kc;
+  g->kcol = x[0];
   g->n = g->err ? 0 : g->n_sites;
 }
 
@@ -1001,3 +1024,4 @@ __global__ void k_record_generation_dev(double* gen, double* avg, double* var, i
   gen[i] = 0.0;
 }
 
+#endif  // ABX_KERNELS_ONLY

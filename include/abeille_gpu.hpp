@@ -96,8 +96,16 @@ class Problem {
   abx_problem* p_ = nullptr;
 };
 
+// settings::converged (src/settings.cpp): set by the driver once the
+// inactive generations are done; transport() reads it to gate the mesh
+// tallies, as the reference's trackers pass it to Tallies::score_collision /
+// score_flight.
+namespace settings {
+inline bool converged = false;
+}
+
 // Tallies (tallies.hpp:38-186): scalar scores on the host, mesh tallies
-// resident on the device.
+// resident on the device (their generation arrays, means and variances).
 class Tallies {
  public:
   Tallies(std::shared_ptr<Problem> prob, double tot_wgt) : prob_(std::move(prob)), W_(tot_wgt) {}
@@ -107,6 +115,7 @@ class Tallies {
   void score_k_tot(double s) { k_tot_s_ += s; }
   void score_leak(double s) { leak_s_ += s; }
   void score_mig_area(double s) { mig_s_ += s; }
+  // Tallies::clear_generation (tallies.cpp:141-157)
   void clear_generation() {
     k_col_s_ = k_abs_s_ = k_trk_s_ = k_tot_s_ = leak_s_ = mig_s_ = 0.;
     check(abx_tally_clear(prob_->get()));
@@ -114,52 +123,93 @@ class Tallies {
   // Tallies::calc_gen_values (tallies.cpp:159-181)
   void calc_gen_values() {
     k_col_ = k_col_s_ / W_;
+    k_abs_ = k_abs_s_ / W_;
     k_trk_ = k_trk_s_ / W_;
     leak_ = leak_s_ / W_;
+    k_tot_ = k_tot_s_ / W_;
     mig_ = mig_s_ / W_;
     k_col_vec.push_back(k_col_);
+    k_abs_vec.push_back(k_abs_);
+    k_trk_vec.push_back(k_trk_);
+    leak_vec.push_back(leak_);
+    mig_vec.push_back(mig_);
   }
-  // Tallies::record_generation (tallies.cpp:183-203) for the scalars; mesh
-  // statistics are kept by abx_run on the device.
-  void record_generation() {
+  // Tallies::record_generation (tallies.cpp:183-203): the scalars here, the
+  // mesh tallies on the device (MeshTally::record_generation).
+  void record_generation(double multiplier = 1.) {
     gen_++;
     upd(k_col_, k_col_avg_, k_col_var_);
+    upd(k_abs_, k_abs_avg_, k_abs_var_);
     upd(k_trk_, k_trk_avg_, k_trk_var_);
     upd(leak_, leak_avg_, leak_var_);
+    upd(k_tot_, k_tot_avg_, k_tot_var_);
     upd(mig_, mig_avg_, mig_var_);
+    check(abx_tally_record(prob_->get(), multiplier));
   }
+  void set_total_weight(double W) { W_ = W; }
+  double total_weight() const { return W_; }
+
   double kcol() const { return k_col_; }
   double kcol_avg() const { return k_col_avg_; }
-  double kcol_err() const { return std::sqrt(k_col_var_ / gen_); }
+  double kcol_err() const { return err(k_col_var_); }
+  double kabs() const { return k_abs_; }
+  double kabs_avg() const { return k_abs_avg_; }
+  double kabs_err() const { return err(k_abs_var_); }
+  double ktrk() const { return k_trk_; }
   double ktrk_avg() const { return k_trk_avg_; }
+  double ktrk_err() const { return err(k_trk_var_); }
+  double ktot() const { return k_tot_; }
+  double ktot_avg() const { return k_tot_avg_; }
+  double ktot_err() const { return err(k_tot_var_); }
+  double leakage() const { return leak_; }
   double leakage_avg() const { return leak_avg_; }
-  double leakage_err() const { return std::sqrt(leak_var_ / gen_); }
+  double leakage_err() const { return err(leak_var_); }
+  double mig_area() const { return mig_; }
+  double mig_area_avg() const { return mig_avg_; }
+  double mig_area_err() const { return err(mig_var_); }
   int generations() const { return gen_; }
-  std::vector<double> k_col_vec;
+  // A mesh tally's mean and error of the mean, [Ne][Nx][Ny][Nz]
+  // (MeshTally::write_tally, mesh_tally.cpp:191-197).
+  void mesh_tally_stats(int tally, std::vector<double>& avg, std::vector<double>& err) const {
+    size_t nb = 0;
+    check(abx_tally_size(prob_->get(), tally, &nb));
+    avg.resize(nb);
+    err.resize(nb);
+    check(abx_tally_stats(prob_->get(), tally, avg.data(), err.data()));
+  }
+  std::vector<double> k_col_vec, k_abs_vec, k_trk_vec, leak_vec, mig_vec;
   Problem& problem() { return *prob_; }
 
  private:
+  // Tallies::update_avg_and_var (tallies.cpp:268-280)
   void upd(double x, double& avg, double& var) {
     const double dg = (double)gen_, old = avg, vold = var;
     avg = old + (x - old) / dg;
     if (gen_ > 1) var = vold + ((x - old) * (x - old) / dg) - (vold / (dg - 1.));
   }
+  double err(double var) const { return std::sqrt(var / (double)gen_); }
   std::shared_ptr<Problem> prob_;
   double W_;
   double k_col_s_ = 0, k_abs_s_ = 0, k_trk_s_ = 0, k_tot_s_ = 0, leak_s_ = 0, mig_s_ = 0;
-  double k_col_ = 1.0, k_trk_ = 1.0, leak_ = 0., mig_ = 0.;
-  double k_col_avg_ = 0, k_col_var_ = 0, k_trk_avg_ = 0, k_trk_var_ = 0, leak_avg_ = 0,
-         leak_var_ = 0, mig_avg_ = 0, mig_var_ = 0;
+  double k_col_ = 1.0, k_abs_ = 1.0, k_trk_ = 1.0, k_tot_ = 1.0, leak_ = 0., mig_ = 0.;
+  double k_col_avg_ = 0, k_col_var_ = 0, k_abs_avg_ = 0, k_abs_var_ = 0, k_trk_avg_ = 0,
+         k_trk_var_ = 0, k_tot_avg_ = 0, k_tot_var_ = 0, leak_avg_ = 0, leak_var_ = 0,
+         mig_avg_ = 0, mig_var_ = 0;
   int gen_ = 0;
 };
+
+// The noise source of the reference's noise transport (out of this path:
+// transport() with noise == true is refused).
+class NoiseMaker;
 
 // Transporter (transporter.hpp:39-87).
 class Transporter {
  public:
   explicit Transporter(std::shared_ptr<Tallies> t) : tallies(std::move(t)) {}
   virtual ~Transporter() = default;
-  virtual std::vector<BankedParticle> transport(std::vector<Particle>& bank,
-                                                bool converged) = 0;
+  virtual std::vector<BankedParticle> transport(std::vector<Particle>& bank, bool noise = false,
+                                                std::vector<BankedParticle>* noise_bank = nullptr,
+                                                const NoiseMaker* noise_maker = nullptr) = 0;
 
  protected:
   std::shared_ptr<Tallies> tallies;
@@ -167,11 +217,17 @@ class Transporter {
 
 // The GPU transporter: whichever tracker the problem's settings name
 // (delta / surface / carter) runs on the device.  `bank` is consumed, the
-// fission bank comes back in parent order then push order.
+// fission bank comes back in parent order then push order, the scores go
+// to the Tallies, and the mesh tallies score when settings::converged.
 class GpuTransporter : public Transporter {
  public:
   using Transporter::Transporter;
-  std::vector<BankedParticle> transport(std::vector<Particle>& bank, bool converged) override {
+  std::vector<BankedParticle> transport(std::vector<Particle>& bank, bool noise = false,
+                                        std::vector<BankedParticle>* noise_bank = nullptr,
+                                        const NoiseMaker* noise_maker = nullptr) override {
+    (void)noise_bank;
+    (void)noise_maker;
+    if (noise) throw Error(ABX_E_INVALID, "noise transport is not part of the GPU path");
     std::vector<abx_particle> in(bank.size());
     for (size_t i = 0; i < bank.size(); i++) {
       const Particle& p = bank[i];
@@ -189,12 +245,21 @@ class GpuTransporter : public Transporter {
       a.family_id = p.family_id();
       a.rng_state = p.rng_state;
     }
-    std::vector<abx_banked> out(4 * bank.size() + 1024);
+    abx_problem* prob = tallies->problem().get();
+    std::vector<abx_banked> out(2 * bank.size() + 1024);
     size_t n_out = 0;
     abx_scores sc{};
     abx_diag dg{};
-    check(abx_transport(tallies->problem().get(), in.data(), in.size(), tallies->kcol(),
-                        converged ? 1 : 0, out.data(), out.size(), &n_out, &sc, &dg));
+    const int rc = abx_transport(prob, in.data(), in.size(), tallies->kcol(),
+                                 settings::converged ? 1 : 0, out.data(), out.size(), &n_out,
+                                 &sc, &dg);
+    if (rc == ABX_E_CAPACITY && n_out > out.size()) {
+      // more sites than the guess: the ordered bank waits on the device
+      out.resize(n_out);
+      check(abx_fission_fetch(prob, out.data(), out.size(), &n_out));
+    } else {
+      check(rc);
+    }
     tallies->score_k_col(sc.k_col);
     tallies->score_k_abs(sc.k_abs);
     tallies->score_k_trk(sc.k_trk);

@@ -347,19 +347,34 @@ int abx_sample_sources(abx_problem* p, uint64_t first_history, size_t n,
  * `kcol` is Tallies::kcol() of the previous generation; `converged` gates
  * the mesh tallies (Tallies::score_collision / score_flight).  Scores add to
  * the problem's current-generation tallies.  n may be 0 (no sites, no
- * scores).  If the sites outnumber `cap`, *n_out still receives their
- * count and the call returns ABX_E_CAPACITY; the generation's scores have
- * been added by then. */
+ * scores).  If the sites outnumber `cap` (or fission_out is NULL), *n_out,
+ * *scores and *diag still receive the call's values, the call returns
+ * ABX_E_CAPACITY, and the ordered sites stay on the device, where
+ * abx_fission_fetch hands them over: the transport is not repeated and
+ * nothing is scored twice. */
 int abx_transport(abx_problem* p, const abx_particle* bank, size_t n,
                   double kcol, int converged, abx_banked* fission_out,
                   size_t cap, size_t* n_out, abx_scores* scores,
                   abx_diag* diag);
+
+/* The ordered fission sites of the last transport (abx_transport, or the
+ * last generation of abx_run) into `out` (capacity cap; count in *n_out) -
+ * the vector transport() returns in the reference
+ * (include/simulation/transporter.hpp:44-47).  For a caller whose
+ * fission_out was too small: ABX_E_CAPACITY again if cap still is. */
+int abx_fission_fetch(abx_problem* p, abx_banked* out, size_t cap, size_t* n_out);
 
 /* MeshTally generation array [Ne][Nx][Ny][Nz] (src/mesh_tally.cpp:106). */
 int abx_tally_size(abx_problem* p, int tally, size_t* n_bins);
 int abx_tally_fetch(abx_problem* p, int tally, double* gen_out);
 /* MeshTally::clear_generation / Tallies::clear_generation */
 int abx_tally_clear(abx_problem* p);
+/* MeshTally::record_generation(multiplier) of every mesh tally
+ * (src/mesh_tally.cpp:121-150): the generation arrays join the running mean
+ * and variance that abx_tally_stats reads - for a host driver that calls
+ * abx_transport itself (abx_run records on its own).  The generation arrays
+ * are kept; abx_tally_clear zeroes them (Tallies::clear_generation). */
+int abx_tally_record(abx_problem* p, double multiplier);
 
 /* ApproximateMeshCancelator::add_particle for every particle of `bank`, then
  * perform_cancellation (src/approximate_mesh_cancelator.cpp:97-190), on the
@@ -371,8 +386,13 @@ int abx_cancel_bank(abx_problem* p, abx_banked* bank, size_t n);
 
 /* Result of a whole run: Tallies' running statistics. */
 typedef struct abx_run_stats {
+  /* Tallies::{kcol,kabs,ktrk,ktot,leakage,mig_area}_{avg,err}
+   * (include/simulation/tallies.hpp:103-151): running mean and standard
+   * error of the mean over the recorded generations (src/tallies.cpp:268-280) */
   double kcol_avg, kcol_err;
+  double kabs_avg, kabs_err;
   double ktrk_avg, ktrk_err;
+  double ktot_avg, ktot_err;
   double leak_avg, leak_err;
   double mig_avg, mig_err;
   int32_t active_generations;
@@ -399,6 +419,21 @@ int abx_run(abx_problem* p, int ngenerations, int nignored,
 int abx_run_continue(abx_problem* p, int ngenerations, int nignored,
                      double* kcol_hist, abx_run_stats* stats);
 int abx_tally_stats(abx_problem* p, int tally, double* avg, double* std_err);
+
+/* The per-generation vectors of Tallies (k_col_vec, k_abs_vec, k_trk_vec,
+ * leak_vec, mig_vec; src/tallies.cpp:159-181), every generation since the
+ * last abx_run / abx_batch_add, inactive ones included.  `quantity` is an
+ * abx_gen_quantity; count in *n (out == NULL asks for the count only),
+ * ABX_E_CAPACITY if cap is too small. */
+enum abx_gen_quantity {
+  ABX_GEN_KCOL = 0,
+  ABX_GEN_KABS = 1,
+  ABX_GEN_KTRK = 2,
+  ABX_GEN_KTOT = 3,
+  ABX_GEN_LEAKAGE = 4,
+  ABX_GEN_MIG = 5
+};
+int abx_gen_values(abx_problem* p, int quantity, double* out, size_t cap, size_t* n);
 
 /* Device bank of the last k-eigenvalue generation in the reference's
  * source-file layout [N][9] = x,y,z,ux,uy,uz,E,w,w2

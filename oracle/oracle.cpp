@@ -540,9 +540,12 @@ struct Problem {
   Scores gen_scores;
   double kcol_prev = 1.0;
   int gen = 0;
-  double k_col_avg = 0, k_col_var = 0, k_trk_avg = 0, k_trk_var = 0,
-         leak_avg = 0, leak_var = 0, mig_avg = 0, mig_var = 0;
+  double k_col_avg = 0, k_col_var = 0, k_abs_avg = 0, k_abs_var = 0, k_trk_avg = 0,
+         k_trk_var = 0, k_tot_avg = 0, k_tot_var = 0, leak_avg = 0, leak_var = 0, mig_avg = 0,
+         mig_var = 0;
   std::vector<abx_particle> last_bank;
+  // Tallies::{k_col,k_abs,k_trk,k_tot,leak,mig}_vec: every generation's values
+  std::vector<double> gen_vals[6];
   abx_diag diag_total{};
   // ApproximateMeshCancelator (src/approximate_mesh_cancelator.cpp:32-96)
   bool cancel = false;
@@ -2005,13 +2008,25 @@ static void upd(int gen, double x, double& avg, double& var) {
   if (gen > 1) var = var_old + ((x - avg_old) * (x - avg_old) / dgen) - (var_old / (dgen - 1.));
 }
 
-static void record_generation(Problem& P, double kc, double kt, double lk,
-                              double mg) {
+// Tallies::calc_gen_values (src/tallies.cpp:159-181): the generation's
+// values join the per-generation vectors, recorded or not.
+static void push_gen_values(Problem& P, const Scores& sc, double W) {
+  const double x[6] = {sc.k_col / W, sc.k_abs / W, sc.k_trk / W,
+                       sc.k_tot / W, sc.leak / W,  sc.mig / W};
+  for (int q = 0; q < 6; q++) P.gen_vals[q].push_back(x[q]);
+}
+
+// Tallies::record_generation (src/tallies.cpp:183-203): the six scalar
+// scores of the generation (already divided by the total weight, as
+// calc_gen_values does, :159-181), then every mesh tally.
+static void record_generation(Problem& P, const Scores& sc, double W) {
   P.gen++;
-  upd(P.gen, kc, P.k_col_avg, P.k_col_var);
-  upd(P.gen, kt, P.k_trk_avg, P.k_trk_var);
-  upd(P.gen, lk, P.leak_avg, P.leak_var);
-  upd(P.gen, mg, P.mig_avg, P.mig_var);
+  upd(P.gen, sc.k_col / W, P.k_col_avg, P.k_col_var);
+  upd(P.gen, sc.k_abs / W, P.k_abs_avg, P.k_abs_var);
+  upd(P.gen, sc.k_trk / W, P.k_trk_avg, P.k_trk_var);
+  upd(P.gen, sc.leak / W, P.leak_avg, P.leak_var);
+  upd(P.gen, sc.k_tot / W, P.k_tot_avg, P.k_tot_var);
+  upd(P.gen, sc.mig / W, P.mig_avg, P.mig_var);
   for (auto& t : P.tallies) {  // MeshTally::record_generation (:121-150)
     t.g++;
     const double dg = (double)t.g;
@@ -2105,6 +2120,16 @@ static void run(Problem& P, int ngen, int nignored, int nthreads,
   bool converged = P.st.mode != ABX_K_EIGENVALUE || nignored == 0;
   S = abx_run_stats{};
   double seconds = 0;
+  // a run starts its statistics afresh (as abx_run does)
+  P.gen = 0;
+  P.k_col_avg = P.k_col_var = P.k_abs_avg = P.k_abs_var = P.k_trk_avg = P.k_trk_var = 0;
+  P.k_tot_avg = P.k_tot_var = P.leak_avg = P.leak_var = P.mig_avg = P.mig_var = 0;
+  for (auto& v : P.gen_vals) v.clear();
+  for (auto& t : P.tallies) {
+    t.g = 0;
+    std::fill(t.avg.begin(), t.avg.end(), 0.);
+    std::fill(t.var.begin(), t.var.end(), 0.);
+  }
   if (P.st.mode == ABX_K_EIGENVALUE) {
     bank.resize(N);
     for (uint64_t i = 0; i < N; i++) bank[i] = sample_one(P, histories_counter++);
@@ -2148,7 +2173,8 @@ static void run(Problem& P, int ngen, int nignored, int nthreads,
       seconds += omp_get_wtime() - t0;
       double kc = sc.k_col / W;
       if (kcol_hist) kcol_hist[g - 1] = kc;
-      record_generation(P, kc, sc.k_trk / W, sc.leak / W, sc.mig / W);
+      push_gen_values(P, sc, W);
+      record_generation(P, sc, W);
     } else {
       double t0 = omp_get_wtime();
       transport(P, bank.data(), bank.size(), kcol, converged, nthreads, next, sc,
@@ -2157,6 +2183,7 @@ static void run(Problem& P, int ngen, int nignored, int nthreads,
       if (next.empty()) fatal(ABX_E_INVALID, "No fission neutrons were produced.");
       double kc = sc.k_col / W;
       if (kcol_hist) kcol_hist[g - 1] = kc;
+      push_gen_values(P, sc, W);
       if (P.cancel) approximate_cancellation(P, next);
       // normalize_weights (src/power_iterator.cpp:538-586)
       double W_pos = 0., W_neg = 0.;
@@ -2169,7 +2196,7 @@ static void run(Problem& P, int ngen, int nignored, int nthreads,
       double Wn = W_pos - W_neg;
       double w_per_part = (double)P.st.nparticles / Wn;
       for (auto& b : next) b.wgt *= w_per_part;
-      if (converged) record_generation(P, kc, sc.k_trk / W, sc.leak / W, sc.mig / W);
+      if (converged) record_generation(P, sc, W);
       // rebank (:394-404)
       histories_counter = global_counter;
       bank.clear();
@@ -2199,6 +2226,10 @@ static void run(Problem& P, int ngen, int nignored, int nthreads,
   P.last_bank = bank;
   S.kcol_avg = P.k_col_avg;
   S.kcol_err = std::sqrt(P.k_col_var / (double)P.gen);
+  S.kabs_avg = P.k_abs_avg;
+  S.kabs_err = std::sqrt(P.k_abs_var / (double)P.gen);
+  S.ktot_avg = P.k_tot_avg;
+  S.ktot_err = std::sqrt(P.k_tot_var / (double)P.gen);
   S.ktrk_avg = P.k_trk_avg;
   S.ktrk_err = std::sqrt(P.k_trk_var / (double)P.gen);
   S.leak_avg = P.leak_avg;
@@ -2319,6 +2350,16 @@ int orc_tally_stats(orc_problem* p, int tally, double* avg, double* std_err) {
       avg[i] = t.avg[i];
       std_err[i] = std::sqrt(t.var[i] / (double)t.g);
     }
+  });
+}
+
+int orc_gen_values(orc_problem* p, int quantity, double* out, size_t cap, size_t* n) {
+  ORC_TRY({
+    if (quantity < 0 || quantity > 5) fatal(ABX_E_INVALID, "quantity");
+    const auto& v = p->P.gen_vals[quantity];
+    *n = v.size();
+    if (v.size() > cap) fatal(ABX_E_CAPACITY, "per-generation buffer too small");
+    std::copy(v.begin(), v.end(), out);
   });
 }
 

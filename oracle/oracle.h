@@ -44,6 +44,8 @@ int orc_tally_clear(orc_problem* p);
 int orc_run(orc_problem* p, int ngenerations, int nignored, int nthreads,
             double* kcol_hist, abx_run_stats* stats);
 int orc_tally_stats(orc_problem* p, int tally, double* avg, double* std_err);
+/* Per-generation values of the last run (abx_gen_quantity order). */
+int orc_gen_values(orc_problem* p, int quantity, double* out, size_t cap, size_t* n);
 int orc_bank_size(orc_problem* p, size_t* n);
 int orc_bank_fetch(orc_problem* p, double* out9);
 

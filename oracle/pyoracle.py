@@ -42,6 +42,8 @@ def lib():
         L.orc_run.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_void_p,
                               C.POINTER(abi.RunStats)]
         L.orc_tally_stats.argtypes = [P, C.c_int, C.c_void_p, C.c_void_p]
+        L.orc_gen_values.argtypes = [P, C.c_int, C.c_void_p, C.c_size_t,
+                                     C.POINTER(C.c_size_t)]
         L.orc_bank_size.argtypes = [P, C.POINTER(C.c_size_t)]
         L.orc_bank_fetch.argtypes = [P, C.c_void_p]
         L.orc_locate.argtypes = [P, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
@@ -134,6 +136,14 @@ class Oracle:
         err = np.zeros(nb.value)
         _check(lib().orc_tally_stats(self.h, i, avg.ctypes.data, err.ctypes.data))
         return avg, err
+
+    def gen_values(self, quantity):
+        """Per-generation values of the last run (abi.GEN_*)."""
+        n = C.c_size_t()
+        cap = 1 << 16
+        out = np.zeros(cap)
+        _check(lib().orc_gen_values(self.h, quantity, out.ctypes.data, cap, C.byref(n)))
+        return out[:n.value].copy()
 
     def locate(self, r, u):
         r = np.ascontiguousarray(r, np.float64)

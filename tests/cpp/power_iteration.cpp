@@ -76,7 +76,6 @@ int main(int argc, char** argv) {
   try {
     auto prob = std::make_shared<abx::Problem>(d);
     auto tallies = std::make_shared<abx::Tallies>(prob, (double)N);
-    abx::GpuTransporter transporter(tallies);
     // PowerIterator::sample_source_from_sources
     std::vector<abx_particle> src_bank(N);
     abx::check(abx_sample_sources(prob->get(), 0, N, src_bank.data()));
@@ -88,12 +87,16 @@ int main(int argc, char** argv) {
       bank.push_back(p);
     }
     uint64_t histories_counter = (uint64_t)N;
-    bool converged = false;
+    std::shared_ptr<abx::Transporter> transporter =
+        std::make_shared<abx::GpuTransporter>(tallies);
+    // PowerIterator::run (src/power_iterator.cpp:305-429), without the
+    // entropy, family, MPI and output bookkeeping
+    if (nignored == 0) abx::settings::converged = true;
     for (int g = 1; g <= ngen; g++) {
-      auto next_gen = transporter.transport(bank, converged);
-      if (next_gen.empty()) throw abx::Error(ABX_E_INVALID, "No fission neutrons were produced.");
+      std::vector<abx::BankedParticle> next_gen = transporter->transport(bank);
+      if (next_gen.size() == 0) throw abx::Error(ABX_E_INVALID, "No fission neutrons were produced.");
       tallies->calc_gen_values();
-      // normalize_weights
+      // normalize_weights (src/power_iterator.cpp:538-586)
       double W_pos = 0., W_neg = 0.;
       for (auto& b : next_gen) {
         if (b.wgt > 0.)
@@ -103,19 +106,21 @@ int main(int argc, char** argv) {
       }
       const double w_per_part = (double)N / (W_pos - W_neg);
       for (auto& b : next_gen) b.wgt *= w_per_part;
-      if (converged) tallies->record_generation();
+      if (abx::settings::converged) tallies->record_generation();
       tallies->clear_generation();
       bank.clear();
       for (auto& b : next_gen) {
-        abx::Particle p(b.r, b.u, b.E, b.wgt, histories_counter++);
-        p.initialize_rng(d.settings.seed, d.settings.stride);
-        p.set_family_id(b.family_id);
-        bank.push_back(p);
+        bank.push_back(abx::Particle(b.r, b.u, b.E, b.wgt, histories_counter++));
+        bank.back().initialize_rng(d.settings.seed, d.settings.stride);
+        bank.back().set_family_id(b.family_id);
       }
-      if (g == nignored) converged = true;
+      if (g == nignored) abx::settings::converged = true;
     }
     std::printf("kcol %.6f %.6f %d\n", tallies->kcol_avg(), tallies->kcol_err(),
                 tallies->generations());
+    std::printf("kabs %.6f %.6f\n", tallies->kabs_avg(), tallies->kabs_err());
+    std::printf("leakage %.6f %.6f\n", tallies->leakage_avg(), tallies->leakage_err());
+    std::printf("gens %zu\n", tallies->k_col_vec.size());
   } catch (const abx::Error& e) {
     std::fprintf(stderr, "error %d: %s\n", e.code, e.what());
     return 1;

@@ -29,7 +29,29 @@ def test_cpp_power_iteration_critical_slab(tmp_path):
     out = subprocess.run([exe, "20000", "70", "10"], capture_output=True, text=True,
                          timeout=300)
     assert out.returncode == 0, out.stderr
-    _, k, err, gens = out.stdout.split()
+    lines = {ln.split()[0]: ln.split()[1:] for ln in out.stdout.splitlines()}
+    k, err, gens = lines["kcol"]
     k, err = float(k), float(err)
     assert int(gens) == 60
     assert abs(k - 1.0) < 4 * err, (k, err)
+    # multigroup k_abs is k_col's estimator in another arithmetic order
+    kabs, kabs_err = (float(x) for x in lines["kabs"])
+    assert kabs == pytest.approx(k, abs=2e-6) and kabs_err == pytest.approx(err, rel=1e-3)
+    assert int(lines["gens"][0]) == 70   # calc_gen_values: every generation
+
+
+def test_transport_signature_is_the_references(tmp_path):
+    """A driver that calls transport(bank) - one argument, as
+    PowerIterator::run does (src/power_iterator.cpp:333) - and the
+    reference's four-argument form both compile against the mirror."""
+    src = tmp_path / "sig.cpp"
+    src.write_text('#include "abeille_gpu.hpp"\n'
+                   'std::vector<abx::BankedParticle> f(abx::Transporter& t, '
+                   'std::vector<abx::Particle>& b) {\n'
+                   '  auto a = t.transport(b);\n'
+                   '  auto c = t.transport(b, false, nullptr, nullptr);\n'
+                   '  a.insert(a.end(), c.begin(), c.end());\n'
+                   '  abx::settings::converged = true;\n'
+                   '  return a;\n}\n')
+    subprocess.run(["g++", "-std=c++17", "-fsyntax-only", str(src), "-I",
+                    os.path.join(ROOT, "include")], check=True)
