@@ -185,10 +185,13 @@ def test_fission_capacity_error_then_recovers():
     o = Oracle(m)
     g = Problem(m)
     so = o.sample_sources(0, 500)
-    with pytest.raises(AbxError, match="capacity") as e:
+    with pytest.raises(AbxError, match="ABX_E_CAPACITY") as e:
         g.transport(so, kcol=0.9, cap=1)
     assert e.value.code == -5   # ABX_E_CAPACITY
     ro = o.transport(so, kcol=0.9, nthreads=1)
-    rg = g.transport(so, kcol=0.9)
     assert len(ro["fission"]) > 1
+    # the refused call's ordered sites are still on the device
+    fetched = g.fission_fetch(len(ro["fission"]))
+    assert ro["fission"].tobytes() == fetched.tobytes()
+    rg = g.transport(so, kcol=0.9)
     assert ro["fission"].tobytes() == rg["fission"].tobytes()
