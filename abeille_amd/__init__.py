@@ -53,6 +53,7 @@ def lib():
         L.abx_tally_size.argtypes = [P, C.c_int, C.POINTER(C.c_size_t)]
         L.abx_tally_fetch.argtypes = [P, C.c_int, C.c_void_p]
         L.abx_tally_clear.argtypes = [P]
+        L.abx_tally_record.argtypes = [P, C.c_double]
         L.abx_tally_stats.argtypes = [P, C.c_int, C.c_void_p, C.c_void_p]
         L.abx_run.argtypes = [P, C.c_int, C.c_int, C.c_void_p, C.POINTER(abi.RunStats)]
         L.abx_run_continue.argtypes = [P, C.c_int, C.c_int, C.c_void_p,
@@ -148,6 +149,11 @@ class Problem:
         out = np.zeros(nb.value)
         _check(lib().abx_tally_fetch(self.h, i, out.ctypes.data))
         return out
+
+    def tally_record(self, multiplier=1.0):
+        """abx_tally_record: the generation arrays join the running mean and
+        variance (MeshTally::record_generation)."""
+        _check(lib().abx_tally_record(self.h, multiplier))
 
     def tally_clear(self):
         _check(lib().abx_tally_clear(self.h))

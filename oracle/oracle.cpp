@@ -13,9 +13,16 @@
 //  * log/sin/cos: glibc, or the portable detmath.h checked <= 1 ulp to glibc.
 //  * surfaces, cells, lattices, tracker: closed-form distances, brute-force
 //    point location, and face tie-break cases (tests/test_oracle_geometry.py).
-//  * collision physics and tallies: closed forms (infinite-medium fluxes,
-//    k-infinity, leakage + absorption balance, exact ray/mesh segment
-//    lengths) and the LA-13511 analytic k values.
+//  * collision physics: closed forms (infinite-medium fluxes for delta,
+//    surface and carter tracking - carter on both of its branches -,
+//    k-infinity, leakage + absorption balance) and the LA-13511 analytic k
+//    values (tests/test_oracle_physics.py, tests/test_carter_closed_form.py).
+//  * mesh tallies: track-length scores of fixed rays through asymmetric
+//    meshes against per-cell lengths from sorted plane crossings, and
+//    collision scores against the sites in the event traces, bin by bin
+//    (tests/test_tally_pinning.py).
+//  * generation statistics: running means and errors against numpy over the
+//    same per-generation values (tests/test_statistics_recurrences.py).
 #include "oracle.h"
 
 #include <algorithm>
@@ -2336,6 +2343,26 @@ int orc_tally_fetch(orc_problem* p, int tally, double* gen_out) {
 }
 
 int orc_tally_clear(orc_problem* p) { ORC_TRY(tally_clear(p->P)); }
+
+// MeshTally::record_generation(multiplier) of every mesh tally
+// (src/mesh_tally.cpp:121-150), for a driver that calls transport itself.
+int orc_tally_record(orc_problem* p, double multiplier) {
+  ORC_TRY({
+    for (auto& t : p->P.tallies) {
+      t.g++;
+      const double dg = (double)t.g;
+      for (size_t i = 0; i < t.gen.size(); i++) {
+        double old_avg = t.avg[i];
+        double val = t.gen[i] * multiplier;
+        double avg = old_avg + (val - old_avg) / dg;
+        t.avg[i] = avg;
+        double var = t.var[i];
+        var = var + (((val - old_avg) * (val - avg) - (var)) / dg);
+        t.var[i] = var;
+      }
+    }
+  });
+}
 
 int orc_run(orc_problem* p, int ngenerations, int nignored, int nthreads,
             double* kcol_hist, abx_run_stats* stats) {

@@ -39,6 +39,8 @@ int orc_transport(orc_problem* p, const abx_particle* bank, size_t n,
 int orc_tally_size(orc_problem* p, int tally, size_t* n_bins);
 int orc_tally_fetch(orc_problem* p, int tally, double* gen_out);
 int orc_tally_clear(orc_problem* p);
+/* MeshTally::record_generation(multiplier) of every mesh tally. */
+int orc_tally_record(orc_problem* p, double multiplier);
 
 /* Whole FixedSource / PowerIterator run. */
 int orc_run(orc_problem* p, int ngenerations, int nignored, int nthreads,

@@ -39,6 +39,7 @@ def lib():
         L.orc_tally_size.argtypes = [P, C.c_int, C.POINTER(C.c_size_t)]
         L.orc_tally_fetch.argtypes = [P, C.c_int, C.c_void_p]
         L.orc_tally_clear.argtypes = [P]
+        L.orc_tally_record.argtypes = [P, C.c_double]
         L.orc_run.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_void_p,
                               C.POINTER(abi.RunStats)]
         L.orc_tally_stats.argtypes = [P, C.c_int, C.c_void_p, C.c_void_p]
@@ -121,6 +122,9 @@ class Oracle:
 
     def tally_clear(self):
         _check(lib().orc_tally_clear(self.h))
+
+    def tally_record(self, multiplier=1.0):
+        _check(lib().orc_tally_record(self.h, multiplier))
 
     def run(self, ngenerations, nignored=0, nthreads=0):
         st = abi.RunStats()
