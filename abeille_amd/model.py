@@ -126,6 +126,59 @@ class Model:
     def ngroups(self):
         return len(self.settings["energy_bounds"]) - 1
 
+    # ---- flat JSON form ------------------------------------------------
+    def to_dict(self):
+        """Plain lists and numbers (JSON-ready): the model as built, ids and
+        region strings included, so that `Model.from_dict` rebuilds the same
+        description without the deck it was read from."""
+        def arr(v):
+            return None if v is None else np.asarray(v, np.float64).tolist()
+
+        mats = []
+        for m in self.materials:
+            d = {k: (arr(v) if k not in ("id", "angles") else v) for k, v in m.items()}
+            if m.get("angles"):
+                d["angles"] = [[arr(a) for a in t] for t in m["angles"]]
+            mats.append(d)
+        tallies = [dict(t, ebounds=arr(t["ebounds"])) for t in self.tallies]
+        st = dict(self.settings)
+        st["energy_bounds"] = arr(st["energy_bounds"])
+        st["sample_xs_ratio"] = arr(st["sample_xs_ratio"])
+        cancel = None
+        if self.cancel is not None:
+            cancel = dict(self.cancel, ebounds=arr(self.cancel["ebounds"]))
+        return dict(surfaces=[list(s) for s in self.surfaces], cells=self.cells,
+                    universes=self.universes, materials=mats, sources=self.sources,
+                    tallies=tallies, root=self.root, cancel=cancel, settings=st)
+
+    @classmethod
+    def from_dict(cls, d):
+        m = cls()
+        m.surfaces = [(int(t), list(p), int(bc), sid) for t, p, bc, sid in d["surfaces"]]
+        m.cells = [dict(c) for c in d["cells"]]
+        m.universes = [dict(u) for u in d["universes"]]
+        for md in d["materials"]:
+            mm = {k: (None if v is None else np.asarray(v, np.float64))
+                  for k, v in md.items() if k not in ("id", "angles")}
+            mm["id"] = md["id"]
+            if mm.get("scatter") is not None:
+                G = len(mm["total"])
+                mm["scatter"] = np.ascontiguousarray(mm["scatter"].reshape(G, G))
+            if mm.get("chi") is not None:
+                mm["chi"] = np.ascontiguousarray(mm["chi"].reshape(len(mm["total"]), -1))
+            if md.get("angles"):
+                mm["angles"] = [tuple(np.asarray(a, np.float64) for a in t) for t in md["angles"]]
+            m.materials.append(mm)
+        m.sources = [dict(s) for s in d["sources"]]
+        m.tallies = [dict(t, ebounds=np.asarray(t["ebounds"], np.float64)) for t in d["tallies"]]
+        m.root = d["root"]
+        if d.get("cancel"):
+            c = dict(d["cancel"])
+            c["ebounds"] = None if c["ebounds"] is None else np.asarray(c["ebounds"], np.float64)
+            m.cancel = c
+        m.settings.update(d["settings"])
+        return m
+
     # ---- flattening ----------------------------------------------------
     def desc(self):
         keep = []

@@ -1,0 +1,204 @@
+"""Every benchmark configuration at its full size against the CPU
+restatement (run on all host cores), through the C ABI.
+
+* 20 cm cube, 2 groups, delta tracking: 1e6 histories as 40 batches of
+  25,000 - the whole run;
+* 17x17 lattice, 7 groups, delta tracking: one generation of 1e6 source
+  particles (ordered fission bank bit for bit, event counts exact, every
+  track-length bin to 1e-9), then a three-generation device-resident run;
+* 3x3 core, surface tracking: one generation of 2e6;
+* 3x3 core in split precision against FP64 at 2e6 per generation: per-bin
+  z-scores of the difference, against the spread of FP64 against FP64 with
+  another seed;
+* 64 batched slab problems (G in {2, 4, 7, 16, 33}, delta and surface
+  tracking, some with fission secondaries) through the batch kernel;
+* served requests of the served-stream generator - carter tracking with
+  approximate cancellation, and fixed-source ones - through the serve loop.
+
+Tallies are FP64 atomic sums, so bins agree up to summation order; counts,
+banks and histories are exact.
+"""
+import numpy as np
+import pytest
+
+from abeille_amd import Batch, Problem, abi, problems
+from abeille_amd.serve import Request, Server
+from oracle.pyoracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+COUNTERS = ("histories", "collisions", "virtual_collisions", "flights", "crossings",
+            "tally_updates", "lost", "secondaries")
+
+
+def _bins_close(a, b, rtol=1e-9):
+    scale = max(np.abs(a).max(), 1e-300)
+    return np.allclose(a, b, rtol=rtol, atol=1e-14 * scale)
+
+
+def _generation_vs_oracle(m, n, kcol=1.0):
+    o, g = Oracle(m), Problem(m)
+    so = o.sample_sources(0, n)
+    sg = g.sample_sources(0, n)
+    assert so.tobytes() == sg.tobytes(), "source particles differ"
+    ro = o.transport(so, kcol=kcol, nthreads=0)
+    rg = g.transport(so, kcol=kcol)
+    assert len(ro["fission"]) == len(rg["fission"]) > n // 4
+    assert ro["fission"].tobytes() == rg["fission"].tobytes(), "ordered fission banks differ"
+    for f in COUNTERS:
+        assert getattr(ro["diag"], f) == getattr(rg["diag"], f), f
+    for f in ("k_col", "k_abs", "k_trk", "k_tot", "leakage", "mig"):
+        a, b = getattr(ro["scores"], f), getattr(rg["scores"], f)
+        assert abs(a - b) <= 1e-12 * max(1.0, abs(a)), (f, a, b)
+    for t in range(len(m.tallies)):
+        assert _bins_close(o.tally(t), g.tally(t)), f"tally {t}"
+    return o, g
+
+
+def _runs_equal(m, ngen, nign, what):
+    o, g = Oracle(m), Problem(m)
+    so, ko = o.run(ngen, nign, nthreads=0)
+    sg, kg = g.run(ngen, nign)
+    assert np.allclose(ko, kg, rtol=1e-9, atol=0), (what, ko, kg)
+    assert so.diag.collisions == sg.diag.collisions, what
+    assert so.histories == sg.histories, what
+    for f in ("kcol_avg", "leak_avg", "mig_avg", "ktrk_avg", "kabs_avg", "ktot_avg"):
+        assert getattr(sg, f) == pytest.approx(getattr(so, f), rel=1e-9, abs=1e-14), (what, f)
+    for t in range(len(m.tallies)):
+        ao, eo = o.tally_stats(t)
+        ag, eg = g.tally_stats(t)
+        assert _bins_close(ao, ag, 1e-8), (what, t)
+        assert _bins_close(eo, eg, 1e-6), (what, t)
+    return so, sg
+
+
+def test_cube_full_run_vs_oracle():
+    print("[at-size] cube 40 x 25,000", flush=True)
+    _runs_equal(problems.fixed_source_cube(25_000), 40, 0, "cube")
+
+
+def test_lattice_generation_full_size_vs_oracle():
+    print("[at-size] lattice one generation of 1e6", flush=True)
+    _generation_vs_oracle(problems.pin_lattice_2d(1_000_000), 1_000_000)
+
+
+def test_lattice_run_full_size_vs_oracle():
+    print("[at-size] lattice 3 generations of 1e6", flush=True)
+    _runs_equal(problems.pin_lattice_2d(1_000_000), 3, 1, "lattice")
+
+
+def test_core_generation_full_size_vs_oracle():
+    print("[at-size] core one generation of 2e6", flush=True)
+    _generation_vs_oracle(problems.core_3x3(2_000_000), 2_000_000)
+
+
+def _core_run(precision, seed, ngen=30, nign=10):
+    m = problems.core_3x3(2_000_000)
+    m.set(precision=precision, seed=seed)
+    p = Problem(m)
+    st, _ = p.run(ngen, nign)
+    return st, p.tally_stats(0)
+
+
+def _z(a, b):
+    (aa, ea), (ab, eb) = a, b
+    sig = np.hypot(ea, eb)
+    ok = sig > 0
+    return (ab[ok] - aa[ok]) / sig[ok]
+
+
+def test_core_split_vs_fp64_full_size():
+    """Split precision against FP64 on the 3x3 core at 2e6 per generation
+    (10 inactive, 20 active): the per-bin z-scores are centred on zero and
+    spread no more than FP64 against itself with another seed (correlation
+    between generations widens both beyond 1)."""
+    print("[at-size] core split vs fp64", flush=True)
+    s64, t64 = _core_run(abi.FP64, abi.DEFAULT_SEED)
+    s32, t32 = _core_run(abi.SPLIT, abi.DEFAULT_SEED)
+    s64b, t64b = _core_run(abi.FP64, abi.DEFAULT_SEED + 7)
+    z = _z(t64, t32)
+    zref = _z(t64, t64b)
+    print(f"[at-size] split-vs-fp64 z mean {z.mean():+.4f} std {z.std():.3f} "
+          f"within3 {np.mean(np.abs(z) < 3):.4f}; fp64-vs-fp64 z mean {zref.mean():+.4f} "
+          f"std {zref.std():.3f}", flush=True)
+    assert len(z) > 1_000_000
+    assert abs(z.mean()) < 0.05, z.mean()
+    assert z.std() < 1.1 * zref.std() + 0.05, (z.std(), zref.std())
+    sig = np.hypot(s64.kcol_err, s32.kcol_err)
+    assert abs(s64.kcol_avg - s32.kcol_avg) < 4 * sig
+
+
+def _slab_set(n=64, seed=5151):
+    rng = np.random.default_rng(seed)
+    groups = [16, 33, 2, 4, 7, 16, 33, 7]
+    out = []
+    for k in range(n):
+        G = groups[k % len(groups)]
+        trk = "delta" if k % 2 else "surface"
+        m, nh = problems.slab_problem(rng, G=G, tracking=trk,
+                                      nhist=int(rng.integers(1000, 6000)),
+                                      fission_free=(k % 4 != 3))
+        out.append((f"slab{k}_G{G}_{trk}", m))
+    return out
+
+
+def test_batch_64_slab_problems_vs_oracle():
+    """The batched form's generator through the batch kernel (FP64), each
+    problem against the CPU restatement run alone."""
+    cases = _slab_set()
+    b = Batch()
+    probs = []
+    for name, m in cases:
+        p = Problem(m)
+        assert b.add(p, 2)
+        probs.append(p)
+    steps = 0
+    while len(b):
+        b.step()
+        steps += 1
+        assert steps < 1000
+    assert {int(m.ngroups) for _, m in cases} >= {16, 33}
+    for (name, m), p in zip(cases, probs):
+        o = Oracle(m)
+        so, _ = o.run(2, 0, nthreads=0)
+        sb = p.stats()
+        assert sb.diag.collisions == so.diag.collisions, name
+        assert sb.diag.secondaries == so.diag.secondaries, name
+        assert sb.histories == so.histories, name
+        assert sb.leak_avg == pytest.approx(so.leak_avg, rel=1e-9, abs=1e-14), name
+        ao, eo = o.tally_stats(0)
+        ab, eb = p.tally_stats(0)
+        assert _bins_close(ao, ab, 1e-8), name
+        assert _bins_close(eo, eb, 1e-6), name
+
+
+def test_served_carter_and_fixed_requests_vs_oracle():
+    """Served-stream requests (carter tracking at 0.8 of the majorant with
+    approximate mesh cancellation on 10x10x10, and fixed-source ones),
+    served together in FP64, each against the CPU restatement alone."""
+    rng = np.random.default_rng(2024)
+    reqs = []
+    t = 0.0
+    while len(reqs) < 12:
+        m, nh, kind = problems.served_request(rng)
+        if nh > 300_000:
+            continue
+        if kind == "fixed" and sum(r.name.startswith("fixed") for r in reqs) >= 4:
+            continue
+        t += rng.exponential(0.005)
+        reqs.append(Request(m, 2, t, name=f"{kind}{len(reqs)}"))
+    assert sum(r.name.startswith("carter") for r in reqs) >= 8
+    srv = Server(lanes=3)
+    srv.run(reqs)
+    for r in reqs:
+        print(f"[at-size] served {r.name}", flush=True)
+        o = Oracle(r.model)
+        so, _ = o.run(r.nbatches, 0, nthreads=0)
+        assert r.stats.diag.collisions == so.diag.collisions, r.name
+        assert r.stats.diag.secondaries == so.diag.secondaries, r.name
+        assert r.stats.histories == so.histories, r.name
+        assert r.stats.leak_avg == pytest.approx(so.leak_avg, rel=1e-9, abs=1e-14), r.name
+        for t_idx, (a, e) in enumerate(r.tallies):
+            ao, eo = o.tally_stats(t_idx)
+            assert _bins_close(ao, a, 1e-8), r.name
+            assert _bins_close(eo, e, 1e-6), r.name
