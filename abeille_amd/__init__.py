@@ -76,6 +76,9 @@ def lib():
         L.abx_problem_stats.argtypes = [P, C.POINTER(abi.RunStats)]
         L.abx_fission_fetch.argtypes = [P, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]
         L.abx_gen_values.argtypes = [P, C.c_int, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]
+        L.abx_transport_noalloc.argtypes = [P, C.c_void_p, C.c_size_t, C.c_double, C.c_int,
+                                            C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t),
+                                            C.POINTER(abi.Scores), C.POINTER(abi.Diag)]
         _lib = L
     return _lib
 
@@ -142,6 +145,18 @@ class Problem:
         res["fission"] = fis[:n_out.value].copy()
         res["scores"] = sc
         return res
+
+    def transport_noalloc(self, d_bank, n, d_fission_out, cap, kcol=1.0, converged=True):
+        """abx_transport_noalloc: d_bank (n abx_particle records) and
+        d_fission_out (cap abx_banked records) are device addresses (ints).
+        Returns (sites written, scores, diag)."""
+        n_out = C.c_size_t()
+        sc = abi.Scores()
+        dg = abi.Diag()
+        _check(lib().abx_transport_noalloc(self.h, d_bank, n, kcol, 1 if converged else 0,
+                                           d_fission_out, cap, C.byref(n_out), C.byref(sc),
+                                           C.byref(dg)))
+        return n_out.value, sc, dg
 
     def tally(self, i):
         nb = C.c_size_t()

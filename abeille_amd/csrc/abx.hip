@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <type_traits>
 #include <string>
 #include <vector>
@@ -1924,3 +1925,183 @@ int abx_problem_stats(abx_problem* p, abx_run_stats* stats) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------ transport without allocation
+// abx_transport_noalloc keeps every device buffer of a transport call beyond
+// the caller's bank and output (scores, counters, the history counter,
+// fission staging, per-parent counts and offsets, the secondary stack) in
+// one file-scope device array.  Each GPU holds its own copy at its own
+// address, taken once per device with hipGetSymbolAddress; host code only
+// ever passes that address, never the array's name.
+namespace {
+
+constexpr size_t kNoAllocBytes = (size_t)64 << 20;
+constexpr int64_t kNoAllocMaxHistories = ABX_NOALLOC_MAX_HISTORIES;
+static_assert(kNoAllocBytes == ABX_NOALLOC_SCRATCH_BYTES, "scratch size");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on device `dev` (taken with that device
+// current, then restored).
+unsigned char* scratch_on(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) fail(ABX_E_INVALID, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int cur = 0;
+    HIPC(hipGetDevice(&cur));
+    if (cur != dev) HIPC(hipSetDevice(dev));
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != dev) HIPC(hipSetDevice(cur));
+    hip_check(e, "hipGetSymbolAddress(scratch)");
+    if (!ptr) fail(ABX_E_HIP, "hipGetSymbolAddress(scratch) returned no address");
+    g_scratch_ptr[dev] = static_cast<unsigned char*>(ptr);
+  }
+  return g_scratch_ptr[dev];
+}
+
+// Slices of the scratch array for one call over n histories.
+struct NoAllocLayout {
+  size_t scalars, daughters, offsets, block_sums, stage, stage_parent, stage_daughter, sec, end;
+};
+
+constexpr size_t kScalarBytes = 256;  // scores[8] counters[8] err err_hist n_stage[2] next total
+
+NoAllocLayout noalloc_layout(int64_t n, int64_t stage_cap, int sec_cap) {
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const int64_t nb = (n + 1023) / 1024;
+  NoAllocLayout L{};
+  size_t o = 0;
+  L.scalars = o;
+  o += up(kScalarBytes);
+  L.daughters = o;
+  o += up((size_t)n * sizeof(uint32_t));
+  L.offsets = o;
+  o += up((size_t)n * sizeof(uint64_t));
+  L.block_sums = o;
+  o += up((size_t)(nb + 1) * sizeof(uint64_t));
+  L.stage = o;
+  o += up((size_t)stage_cap * sizeof(abx_banked));
+  L.stage_parent = o;
+  o += up((size_t)stage_cap * sizeof(uint32_t));
+  L.stage_daughter = o;
+  o += up((size_t)stage_cap * sizeof(uint32_t));
+  L.sec = o;
+  o += up((size_t)(sec_cap ? n + 256 : 0) * sec_cap * sizeof(SecRec));
+  L.end = o;
+  return L;
+}
+
+// The worst case (a banking problem with the secondary stack) at the
+// largest accepted call fits the array.
+static_assert(kNoAllocMaxHistories * (int64_t)(sizeof(uint32_t) + sizeof(uint64_t)) +
+                      (4 * kNoAllocMaxHistories + 4096) *
+                          (int64_t)(sizeof(abx_banked) + 2 * sizeof(uint32_t)) +
+                      (kNoAllocMaxHistories + 256) * 16 * (int64_t)sizeof(SecRec) + (1 << 20) <=
+                  (int64_t)kNoAllocBytes,
+              "scratch too small for the largest call");
+
+}  // namespace
+
+extern "C" int abx_transport_noalloc(abx_problem* p, const abx_particle* d_bank, size_t n,
+                                     double kcol, int converged, abx_banked* d_fission_out,
+                                     size_t cap, size_t* n_out, abx_scores* scores,
+                                     abx_diag* diag) {
+  ABX_TRY({
+    if (!p || (!d_bank && n)) fail(ABX_E_INVALID, "null argument");
+    if ((int64_t)n > kNoAllocMaxHistories)
+      fail(ABX_E_INVALID, "abx_transport_noalloc takes at most " +
+                              std::to_string(kNoAllocMaxHistories) + " histories per call, got " +
+                              std::to_string(n));
+    const HostTables& T = p->T;
+    const int64_t nn = (int64_t)n;
+    const bool banks = T.st.mode != ABX_FIXED_SOURCE;
+    const int sec_cap = needs_secondaries(T) ? 16 : 0;
+    const int64_t stage_cap = banks ? 4 * nn + 4096 : 0;
+    const NoAllocLayout L = noalloc_layout(nn, stage_cap, sec_cap);
+    if (L.end > kNoAllocBytes) fail(ABX_E_INVALID, "scratch layout exceeds the scratch array");
+    unsigned char* base = scratch_on(p->device);
+    double* d_scores = reinterpret_cast<double*>(base + L.scalars);
+    unsigned long long* d_counters = reinterpret_cast<unsigned long long*>(base + L.scalars + 64);
+    int32_t* d_err = reinterpret_cast<int32_t*>(base + L.scalars + 128);
+    unsigned long long* d_err_hist = reinterpret_cast<unsigned long long*>(base + L.scalars + 136);
+    unsigned long long* d_nst = reinterpret_cast<unsigned long long*>(base + L.scalars + 144);
+    unsigned long long* d_next = reinterpret_cast<unsigned long long*>(base + L.scalars + 160);
+    unsigned long long* d_total = reinterpret_cast<unsigned long long*>(base + L.scalars + 168);
+    uint32_t* d_daughters = reinterpret_cast<uint32_t*>(base + L.daughters);
+    uint64_t* d_offsets = reinterpret_cast<uint64_t*>(base + L.offsets);
+    uint64_t* d_bsums = reinterpret_cast<uint64_t*>(base + L.block_sums);
+    // the previous call's scalars are still there: clear them with a kernel
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, p->stream, d_scores,
+                       (int64_t)(kScalarBytes / sizeof(double)), 0.0);
+    HIPC(hipGetLastError());
+    p->last_sites = 0;
+    KCtx K{};
+    K.in = d_bank;
+    K.n = nn;
+    K.kcol = kcol;
+    K.converged = converged;
+    K.sec_cap = sec_cap;
+    K.sec = sec_cap ? reinterpret_cast<SecRec*>(base + L.sec) : nullptr;
+    K.stage = banks ? reinterpret_cast<abx_banked*>(base + L.stage) : nullptr;
+    K.stage_parent = banks ? reinterpret_cast<uint32_t*>(base + L.stage_parent) : nullptr;
+    K.stage_daughter = banks ? reinterpret_cast<uint32_t*>(base + L.stage_daughter) : nullptr;
+    K.n_stage = d_nst;
+    K.n_hole = d_nst + 1;
+    K.stage_cap = stage_cap;
+    K.daughters = banks ? d_daughters : nullptr;
+    K.scores = d_scores;
+    K.counters = d_counters;
+    K.err = d_err;
+    K.err_hist = d_err_hist;
+    if (nn > 0) HIPC(history_transport(p->P, K, p->stream, sec_cap > 0, d_next, p->tuning));
+    double sc[8];
+    unsigned long long cn[8], nst[2] = {0, 0};
+    int32_t err = 0;
+    unsigned long long err_hist = 0;
+    HIPC(hipMemcpyAsync(sc, d_scores, sizeof(sc), hipMemcpyDeviceToHost, p->stream));
+    HIPC(hipMemcpyAsync(cn, d_counters, sizeof(cn), hipMemcpyDeviceToHost, p->stream));
+    HIPC(hipMemcpyAsync(&err, d_err, sizeof(err), hipMemcpyDeviceToHost, p->stream));
+    HIPC(hipMemcpyAsync(&err_hist, d_err_hist, sizeof(err_hist), hipMemcpyDeviceToHost, p->stream));
+    HIPC(hipMemcpyAsync(nst, d_nst, sizeof(nst), hipMemcpyDeviceToHost, p->stream));
+    HIPC(hipStreamSynchronize(p->stream));
+    TransportOut r = collect_scalars(sc, cn, err, err_hist);
+    add_scores(p->gen_scores, r.scores);
+    if (scores) *scores = r.scores;
+    if (diag) *diag = r.diag;
+    if (n_out) *n_out = 0;
+    if (!banks || nn == 0) return ABX_OK;
+    // order the staged sites (parent order, then push order) into the
+    // caller's buffer: scan of the per-parent counts, all on the device
+    if ((int64_t)nst[0] > stage_cap) fail(ABX_E_CAPACITY, "fission staging overflow");
+    const int64_t nb = (nn + 1023) / 1024;
+    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(1024), 0, p->stream, d_daughters,
+                       d_offsets, d_bsums, nn);
+    HIPC(hipGetLastError());
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, p->stream, d_bsums, nb, d_total);
+    HIPC(hipGetLastError());
+    hipLaunchKernelGGL(k_add_block_offsets, dim3((unsigned)nb), dim3(1024), 0, p->stream,
+                       d_offsets, d_bsums, nn);
+    HIPC(hipGetLastError());
+    unsigned long long total = 0;
+    HIPC(hipMemcpyAsync(&total, d_total, sizeof(total), hipMemcpyDeviceToHost, p->stream));
+    HIPC(hipStreamSynchronize(p->stream));
+    if (nst[1] > nst[0] || total != nst[0] - nst[1]) fail(ABX_E_INVALID, "fission site count mismatch");
+    if (n_out) *n_out = (size_t)total;
+    if (total == 0) return ABX_OK;
+    if (!d_fission_out || total > cap)
+      fail(ABX_E_CAPACITY, "fission_out holds " + std::to_string(cap) + " sites, " +
+                               std::to_string(total) + " were banked");
+    hipLaunchKernelGGL(k_order_bank, dim3((unsigned)blocks_for((int64_t)nst[0], 256)), dim3(256), 0,
+                       p->stream, K.stage, K.stage_parent, K.stage_daughter, d_offsets,
+                       (int64_t)nst[0], nn, (uint64_t)total, d_fission_out, d_err);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(&err, d_err, sizeof(err), hipMemcpyDeviceToHost, p->stream));
+    HIPC(hipStreamSynchronize(p->stream));
+    if (err) fail(err, "fission bank ordering: a staging slot names no parent of this call");
+  });
+}

@@ -364,6 +364,28 @@ int abx_transport(abx_problem* p, const abx_particle* bank, size_t n,
  * fission_out was too small: ABX_E_CAPACITY again if cap still is. */
 int abx_fission_fetch(abx_problem* p, abx_banked* out, size_t cap, size_t* n_out);
 
+/* Transport that allocates no device memory.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * meaning as abx_transport, except that `d_bank` and `d_fission_out` are
+ * DEVICE pointers (on the problem's device), the history kernel runs
+ * whatever abx_set_engine chose, and the ordered sites go straight to
+ * d_fission_out (abx_fission_fetch does not see them).  Every buffer beyond
+ * the caller's bank and output lives in one static device array of
+ * ABX_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; a call
+ * takes at most ABX_NOALLOC_MAX_HISTORIES histories (more: ABX_E_INVALID),
+ * and a d_fission_out of 4n + 4096 sites never overflows.  All calls on one
+ * GPU share that array: calls that overlap on one GPU (from several threads,
+ * or several problems) race, and must be serialised by the caller.  Splitting
+ * a bank into consecutive slices of histories gives the same fission bank
+ * (the slices' outputs concatenated) and the same tallies as one call. */
+#define ABX_NOALLOC_MAX_HISTORIES 32768
+#define ABX_NOALLOC_SCRATCH_BYTES (64u << 20)
+int abx_transport_noalloc(abx_problem* p, const abx_particle* d_bank, size_t n,
+                          double kcol, int converged, abx_banked* d_fission_out,
+                          size_t cap, size_t* n_out, abx_scores* scores,
+                          abx_diag* diag);
+
 /* MeshTally generation array [Ne][Nx][Ny][Nz] (src/mesh_tally.cpp:106). */
 int abx_tally_size(abx_problem* p, int tally, size_t* n_bins);
 int abx_tally_fetch(abx_problem* p, int tally, double* gen_out);
