@@ -30,3 +30,14 @@ def assert_within_sigma(avg_a, err_a, avg_b, err_b, nsig=3.0, min_frac=0.99):
     frac = ok.mean()
     assert frac >= min_frac, f"only {frac:.4f} of bins within {nsig} sigma"
     return frac
+
+
+def bins_close(a, b, rtol=1e-9):
+    """Tally bins of two runs of the same histories: FP64 atomics add in
+    another order, and the GPU's track-length walk accumulates the distance
+    to the next mesh plane per axis while the reference recomputes it from
+    an advanced position, so a segment's length differs by ~1e-15 cm per
+    crossing.  Each bin must agree to `rtol` relative, or to 1e-9 of the
+    largest bin (a bin holding only a sub-micron corner clip)."""
+    scale = max(float(np.abs(a).max()), 1e-300)
+    return bool(np.all(np.abs(a - b) <= rtol * np.abs(a) + 1e-9 * scale))

@@ -20,7 +20,10 @@ struct HostSink {
   int err = 0;
   uint64_t err_hist = 0;
   abx_diag diag{};
-  void tally_add(double* a, double v) { *a += v; }
+  void tally_add(double* a, double v) {
+    *a += v;
+    diag.tally_updates++;
+  }
   bool sec_push(const SecRec& q) {
     if (stack.size() >= ABX_SEC_CAP) return false;
     stack.push_back(q);
@@ -171,6 +174,7 @@ int hd_transport(hd_problem* h, const abx_particle* bank, size_t n, double kcol,
     dg.virtual_collisions += s.diag.virtual_collisions;
     dg.lost += s.diag.lost;
     dg.secondaries += s.diag.secondaries;
+    dg.tally_updates += s.diag.tally_updates;
     if (trace) {
       std::copy(tr.begin(), tr.end(), trace + i * (size_t)trace_cap);
       n_rec[i] = s.n_rec;

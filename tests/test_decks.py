@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 from abeille_amd.model import Model
-from helpers import assert_traces_equal
+from helpers import assert_traces_equal, bins_close
 from oracle.pyoracle import Oracle
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -84,7 +84,7 @@ def test_gpu_deck_run_matches_oracle(name):
     for t in range(len(m.tallies)):
         ao, _ = o.tally_stats(t)
         ag, _ = g.tally_stats(t)
-        assert np.allclose(ao, ag, rtol=1e-8, atol=1e-14 * max(np.abs(ao).max(), 1e-300))
+        assert bins_close(ao, ag, 1e-8)
 
 
 @pytest.mark.gpu

@@ -15,14 +15,17 @@ restatement (run on all host cores), through the C ABI.
 * served requests of the served-stream generator - carter tracking with
   approximate cancellation, and fixed-source ones - through the serve loop.
 
-Tallies are FP64 atomic sums, so bins agree up to summation order; counts,
-banks and histories are exact.
+Tallies are FP64 atomic sums, so bins agree up to summation order and the
+track-length walk's rounding (helpers.bins_close); counts, banks and
+histories are exact, except the count of track-length segments, which may
+differ by sub-ulp segments at flights that end on a mesh plane.
 """
 import numpy as np
 import pytest
 
 from abeille_amd import Batch, Problem, abi, problems
 from abeille_amd.serve import Request, Server
+from helpers import bins_close
 from oracle.pyoracle import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -32,8 +35,7 @@ COUNTERS = ("histories", "collisions", "virtual_collisions", "flights", "crossin
 
 
 def _bins_close(a, b, rtol=1e-9):
-    scale = max(np.abs(a).max(), 1e-300)
-    return np.allclose(a, b, rtol=rtol, atol=1e-14 * scale)
+    return bins_close(a, b, rtol)
 
 
 def _generation_vs_oracle(m, n, kcol=1.0):
@@ -46,7 +48,14 @@ def _generation_vs_oracle(m, n, kcol=1.0):
     assert len(ro["fission"]) == len(rg["fission"]) > n // 4
     assert ro["fission"].tobytes() == rg["fission"].tobytes(), "ordered fission banks differ"
     for f in COUNTERS:
+        if f == "tally_updates":
+            continue
         assert getattr(ro["diag"], f) == getattr(rg["diag"], f), f
+    # a flight that ends on a mesh plane (surface tracking stops at lattice
+    # faces, which the meshes share) leaves a segment of ~1e-16 cm that one
+    # walk scores in the next cell and the other does not, either way
+    a, b = ro["diag"].tally_updates, rg["diag"].tally_updates
+    assert abs(a - b) <= 0.01 * a, ("tally_updates", a, b)
     for f in ("k_col", "k_abs", "k_trk", "k_tot", "leakage", "mig"):
         a, b = getattr(ro["scores"], f), getattr(rg["scores"], f)
         assert abs(a - b) <= 1e-12 * max(1.0, abs(a)), (f, a, b)
@@ -61,7 +70,7 @@ def _runs_equal(m, ngen, nign, what):
     sg, kg = g.run(ngen, nign)
     assert np.allclose(ko, kg, rtol=1e-9, atol=0), (what, ko, kg)
     assert so.diag.collisions == sg.diag.collisions, what
-    assert so.histories == sg.histories, what
+    assert so.diag.histories == sg.diag.histories, what
     for f in ("kcol_avg", "leak_avg", "mig_avg", "ktrk_avg", "kabs_avg", "ktot_avg"):
         assert getattr(sg, f) == pytest.approx(getattr(so, f), rel=1e-9, abs=1e-14), (what, f)
     for t in range(len(m.tallies)):
@@ -164,7 +173,7 @@ def test_batch_64_slab_problems_vs_oracle():
         sb = p.stats()
         assert sb.diag.collisions == so.diag.collisions, name
         assert sb.diag.secondaries == so.diag.secondaries, name
-        assert sb.histories == so.histories, name
+        assert sb.histories == so.diag.histories, name
         assert sb.leak_avg == pytest.approx(so.leak_avg, rel=1e-9, abs=1e-14), name
         ao, eo = o.tally_stats(0)
         ab, eb = p.tally_stats(0)
@@ -196,7 +205,7 @@ def test_served_carter_and_fixed_requests_vs_oracle():
         so, _ = o.run(r.nbatches, 0, nthreads=0)
         assert r.stats.diag.collisions == so.diag.collisions, r.name
         assert r.stats.diag.secondaries == so.diag.secondaries, r.name
-        assert r.stats.histories == so.histories, r.name
+        assert r.stats.histories == so.diag.histories, r.name
         assert r.stats.leak_avg == pytest.approx(so.leak_avg, rel=1e-9, abs=1e-14), r.name
         for t_idx, (a, e) in enumerate(r.tallies):
             ao, eo = o.tally_stats(t_idx)
