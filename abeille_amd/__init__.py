@@ -76,6 +76,8 @@ def lib():
         L.abx_problem_stats.argtypes = [P, C.POINTER(abi.RunStats)]
         L.abx_fission_fetch.argtypes = [P, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]
         L.abx_gen_values.argtypes = [P, C.c_int, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]
+        L.abx_run_replicas.argtypes = [P, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                       C.POINTER(abi.RunStats)]
         L.abx_transport_noalloc.argtypes = [P, C.c_void_p, C.c_size_t, C.c_double, C.c_int,
                                             C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t),
                                             C.POINTER(abi.Scores), C.POINTER(abi.Diag)]
@@ -192,6 +194,21 @@ class Problem:
         kh = np.zeros(ngenerations)
         _check(lib().abx_run_continue(self.h, ngenerations, nignored, kh.ctypes.data,
                                       C.byref(st)))
+        return st, kh
+
+    def run_replicas(self, replicas, ngenerations, nignored=0, seeds=None):
+        """abx_run_replicas: `replicas` chains of this k-eigenvalue problem
+        (seeds[r], default settings.seed + r) run together.  Returns the
+        statistics of the chains' mean and k_col per chain and generation
+        ([replicas][ngenerations])."""
+        st = abi.RunStats()
+        kh = np.zeros((replicas, ngenerations))
+        sd = None
+        if seeds is not None:
+            sd = np.ascontiguousarray(seeds, dtype=np.uint64)
+            assert len(sd) == replicas
+        _check(lib().abx_run_replicas(self.h, replicas, None if sd is None else sd.ctypes.data,
+                                      ngenerations, nignored, kh.ctypes.data, C.byref(st)))
         return st, kh
 
     def fission_fetch(self, cap):

@@ -974,15 +974,23 @@ static bool run_generations_graph(abx_problem* p, int ngenerations, int nignored
     for (int k = 0; k < 2; k++)
       HIPC(hipGraphInstantiate(&ex[par][conv][k], gr[par][conv][k], nullptr, nullptr, 0));
   };
+  // (destroys each graph once: an error after the first call lands in the
+  // catch below, which calls it again)
   auto cleanup = [&]() {
     for (auto& a : ex)
       for (auto& b : a)
         for (auto& c : b)
-          if (c) (void)hipGraphExecDestroy(c);
+          if (c) {
+            (void)hipGraphExecDestroy(c);
+            c = nullptr;
+          }
     for (auto& a : gr)
       for (auto& b : a)
         for (auto& c : b)
-          if (c) (void)hipGraphDestroy(c);
+          if (c) {
+            (void)hipGraphDestroy(c);
+            c = nullptr;
+          }
   };
   try {
     while ((int)p->gen_events.size() < 2 * ngenerations + 2) {
@@ -1061,6 +1069,260 @@ static bool run_generations_graph(abx_problem* p, int ngenerations, int nignored
     throw;
   }
   return true;
+}
+
+// PowerIterator::run (src/power_iterator.cpp:305-429) for `R` independent
+// chains of one k-eigenvalue problem that differ only in their seed, in one
+// bank: every generation is one transport launch over all chains'
+// histories plus the chain-aware bookkeeping, captured as two graphs and
+// replayed.  Chain r samples histories 0..N-1 with seeds[r] and then runs
+// exactly the histories abx_run with that seed runs; the mesh tallies and
+// the running statistics follow the chains' mean per generation.
+// The replica-mode kernel variants: surface or delta tracking, FP64 or
+// split precision (no trace, no secondary stack).
+extern "C++" {
+template <class Fn>
+static hipError_t rep_dispatch(int trk, bool split, Fn&& fn) {
+  if (trk == 0) return split ? fn(IntC<0>{}, BoolC<true>{}) : fn(IntC<0>{}, BoolC<false>{});
+  if (trk == 1) return split ? fn(IntC<1>{}, BoolC<true>{}) : fn(IntC<1>{}, BoolC<false>{});
+  return hipErrorInvalidValue;
+}
+}
+
+static void run_replicas(abx_problem* p, int R, const uint64_t* seeds, int ngenerations,
+                         int nignored, double* kcol_hist, abx_run_stats* stats) {
+  const HostTables& T = p->T;
+  if (T.st.mode != ABX_K_EIGENVALUE) fail(ABX_E_INVALID, "replicas are k-eigenvalue chains");
+  if (p->cancel) fail(ABX_E_INVALID, "replicas: approximate cancellation is not supported");
+  if (needs_secondaries(T)) fail(ABX_E_INVALID, "replicas: carter tracking is not supported");
+  run_reset(p);
+  const int64_t N = T.st.nparticles;
+  if (N < 1) fail(ABX_E_INVALID, "nparticles must be positive");
+  const double W = (double)N;
+  const uint64_t NR = (uint64_t)N * (uint64_t)R;
+  const uint64_t cap = 4 * NR + 4096;
+  hipStream_t s = p->stream;
+  p->bank_a.reserve(cap * sizeof(abx_particle));
+  p->bank_b.reserve(cap * sizeof(abx_particle));
+  const int64_t nb = (int64_t)((cap + 1023) / 1024);
+  p->offsets.reserve(cap * sizeof(uint64_t));
+  p->block_sums.reserve((size_t)(nb + 1) * sizeof(uint64_t));
+  p->ordered.reserve(cap * sizeof(abx_banked));
+  p->gdev.reserve(sizeof(GenRep));
+  p->khist_dev.reserve((size_t)(6 + R) * ngenerations * sizeof(double));
+  uint64_t sd[kMaxReplicas];
+  for (int r = 0; r < R; r++) sd[r] = seeds ? seeds[r] : T.st.seed + (uint64_t)r;
+  // each chain's first generation: histories 0..N-1 sampled with its seed
+  HIPC(hipMemsetAsync(p->err.p, 0, sizeof(int32_t), s));
+  for (int r = 0; r < R; r++) {
+    DevProblem Pr = p->P;
+    Pr.seed = sd[r];
+    hipLaunchKernelGGL(k_sample_sources, dim3((unsigned)blocks_for(N, 256)), dim3(256),
+                       (size_t)256 * trk_bytes(p->P.trk_D, p->P.trk_L), s, Pr, p->S, 0, N,
+                       p->bank_a.as<abx_particle>() + (size_t)r * N, p->err.as<int32_t>(),
+                       p->err_hist.as<unsigned long long>());
+    HIPC(hipGetLastError());
+  }
+  {
+    int32_t err = 0;
+    HIPC(hipMemcpyAsync(&err, p->err.p, sizeof(err), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    if (err) fail(err, "source sampling failed");
+  }
+  GenRep* g = p->gdev.as<GenRep>();
+  GenRep h{};
+  h.n = NR;
+  h.cap = cap;
+  h.nrep = R;
+  for (int r = 0; r < R; r++) {
+    h.seg[r] = (uint64_t)r * N;
+    h.run_hist[r] = (uint64_t)N;
+    h.seed[r] = sd[r];
+    h.kcol[r] = 1.0;
+  }
+  h.seg[R] = NR;
+  h.hist_cap = ngenerations;
+  HIPC(hipMemcpyAsync(g, &h, sizeof(GenRep), hipMemcpyHostToDevice, s));
+  HIPC(hipStreamSynchronize(s));
+  const CallScalars cs{&g->scores[0][0], g->counters, &g->err, &g->err_hist, &g->n_stage,
+                       &g->n_hole};
+  KCtx K = make_kctx(p, p->bank_a.as<abx_particle>(), (int64_t)NR, 1.0, 1, cs, nullptr, 0);
+  p->daughters.reserve(cap * sizeof(uint32_t));
+  K.daughters = p->daughters.as<uint32_t>();
+  K.stage_cap = (int64_t)cap;
+  K.n_dev = &g->n;
+  K.kcol_dev = nullptr;
+  K.rep_start = g->seg;
+  K.rep_kcol = g->kcol;
+  K.rep_scores = &g->scores[0][0];
+  K.nrep = R;
+  const abx_tuning tu = p->tuning;
+  const int bs = tu.block_size;
+  const size_t cur = cursor_lds_bytes(bs, p->P.trk_D, p->P.trk_L);
+  const int trk = T.st.tracking;
+  const bool split = T.st.precision == ABX_SPLIT;
+  bool stage = false;
+  int64_t grid = 0;
+  HIPC(rep_dispatch(trk, split, [&](auto TT, auto SP) {
+    constexpr int TK = decltype(TT)::value;
+    constexpr bool SPL = decltype(SP)::value;
+    stage = tu.stage_tables && stage_pays(k_history<TK, false, false, SPL, true, true>,
+                                          k_history<TK, false, false, SPL, false, true>, bs, cur,
+                                          (size_t)p->P.blob_bytes);
+    grid = stage ? resident_blocks(k_history<TK, false, false, SPL, true, true>, bs,
+                                   cur + (size_t)p->P.blob_bytes)
+                 : resident_blocks(k_history<TK, false, false, SPL, false, true>, bs, cur);
+    return hipSuccess;
+  }));
+  const size_t lds = stage ? cur + (size_t)p->P.blob_bytes : cur;
+  if (tu.blocks_per_cu > 0) {
+    int dev = 0, cus = 0;
+    HIPC(hipGetDevice(&dev));
+    HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    grid = std::min<int64_t>(grid, (int64_t)cus * tu.blocks_per_cu);
+  }
+  grid = std::min<int64_t>(grid, (int64_t)((cap + bs - 1) / bs));
+  const int64_t tb = T.tally_bins;
+  double* hist_dev = p->khist_dev.as<double>();
+  double* khist_dev = hist_dev + (size_t)6 * ngenerations;
+  hipGraphExec_t ex[2][2][2] = {};
+  hipGraph_t gr[2][2][2] = {};
+  auto capture = [&](int par, int conv) {
+    abx_particle* in = par ? p->bank_b.as<abx_particle>() : p->bank_a.as<abx_particle>();
+    abx_particle* out = par ? p->bank_a.as<abx_particle>() : p->bank_b.as<abx_particle>();
+    KCtx Kc = K;
+    Kc.in = in;
+    Kc.converged = conv;
+    HIPC(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(k_rep_begin, dim3(1), dim3(256), 0, s, g);
+    HIPC(rep_dispatch(trk, split, [&](auto TT, auto SP) {
+      constexpr int TK = decltype(TT)::value;
+      constexpr bool SPL = decltype(SP)::value;
+      if (stage)
+        hipLaunchKernelGGL((k_history<TK, false, false, SPL, true, true>),
+                           dim3((unsigned)grid), dim3(bs), lds, s, p->P, Kc, &g->next);
+      else
+        hipLaunchKernelGGL((k_history<TK, false, false, SPL, false, true>),
+                           dim3((unsigned)grid), dim3(bs), lds, s, p->P, Kc, &g->next);
+      return hipSuccess;
+    }));
+    HIPC(hipStreamEndCapture(s, &gr[par][conv][0]));
+    HIPC(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(k_scan_blocks_dev, dim3((unsigned)nb), dim3(1024), 0, s,
+                       p->daughters.as<uint32_t>(), p->offsets.as<uint64_t>(),
+                       p->block_sums.as<uint64_t>(), &g->n);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, p->block_sums.as<uint64_t>(), nb,
+                       &g->n_sites);
+    hipLaunchKernelGGL(k_add_block_offsets_dev, dim3((unsigned)nb), dim3(1024), 0, s,
+                       p->offsets.as<uint64_t>(), p->block_sums.as<uint64_t>(), &g->n);
+    hipLaunchKernelGGL(k_rep_sites, dim3(1), dim3(64), 0, s, g, p->offsets.as<uint64_t>());
+    const unsigned gcap = (unsigned)std::min<uint64_t>((cap + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_rep_order, dim3(gcap), dim3(256), 0, s, p->stage.as<abx_banked>(),
+                       p->stage_parent.as<uint32_t>(), p->stage_daughter.as<uint32_t>(),
+                       p->offsets.as<uint64_t>(), g, p->ordered.as<abx_banked>());
+    hipLaunchKernelGGL(k_rep_end, dim3(1), dim3(64), 0, s, g, W, conv, hist_dev, khist_dev);
+    hipLaunchKernelGGL(k_rebank_rep, dim3(gcap), dim3(256), 0, s, p->ordered.as<abx_banked>(),
+                       g, T.st.stride, out);
+    if (conv && tb > 0)
+      hipLaunchKernelGGL(k_record_rep, dim3((unsigned)blocks_for(tb, 256)), dim3(256), 0, s,
+                         p->tally_gen.as<double>(), p->tally_avg.as<double>(),
+                         p->tally_var.as<double>(), tb, g, 1.0 / (double)R);
+    HIPC(hipStreamEndCapture(s, &gr[par][conv][1]));
+    for (int k = 0; k < 2; k++)
+      HIPC(hipGraphInstantiate(&ex[par][conv][k], gr[par][conv][k], nullptr, nullptr, 0));
+  };
+  // (destroys each graph once: an error after the first call lands in the
+  // catch below, which calls it again)
+  auto cleanup = [&]() {
+    for (auto& a : ex)
+      for (auto& b : a)
+        for (auto& c : b)
+          if (c) {
+            (void)hipGraphExecDestroy(c);
+            c = nullptr;
+          }
+    for (auto& a : gr)
+      for (auto& b : a)
+        for (auto& c : b)
+          if (c) {
+            (void)hipGraphDestroy(c);
+            c = nullptr;
+          }
+  };
+  try {
+    while ((int)p->gen_events.size() < 2 * ngenerations + 2) {
+      hipEvent_t e;
+      HIPC(hipEventCreate(&e));
+      p->gen_events.push_back(e);
+    }
+    bool converged = nignored == 0;
+    int par = 0;
+    for (int gi = 0; gi < ngenerations; gi++) {
+      const int conv = converged ? 1 : 0;
+      if (!ex[par][conv][0]) capture(par, conv);
+      HIPC(hipEventRecord(p->gen_events[2 * gi], s));
+      HIPC(hipGraphLaunch(ex[par][conv][0], s));
+      HIPC(hipEventRecord(p->gen_events[2 * gi + 1], s));
+      HIPC(hipGraphLaunch(ex[par][conv][1], s));
+      par ^= 1;
+      if (gi + 1 == nignored) converged = true;
+    }
+    HIPC(hipEventRecord(p->gen_events[2 * ngenerations], s));
+    HIPC(hipMemcpyAsync(&h, g, sizeof(GenRep), hipMemcpyDeviceToHost, s));
+    std::vector<double> hist((size_t)(6 + R) * ngenerations);
+    HIPC(hipMemcpyAsync(hist.data(), hist_dev, hist.size() * sizeof(double),
+                        hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    float total = 0.f;
+    double kms = 0.0;
+    for (int gi = 0; gi < ngenerations; gi++) {
+      float x = 0.f;
+      HIPC(hipEventElapsedTime(&x, p->gen_events[2 * gi], p->gen_events[2 * gi + 1]));
+      kms += x;
+    }
+    HIPC(hipEventElapsedTime(&total, p->gen_events[0], p->gen_events[2 * ngenerations]));
+    cleanup();
+    if (h.err) {
+      if (h.err == -8) fail(ABX_E_INVALID, "No fission neutrons were produced.");
+      fail(h.err, "transport failed at history " + std::to_string(h.err_hist));
+    }
+    p->gen = h.gen;
+    p->tally_g = h.gen;
+    for (int q = 0; q < 6; q++) {
+      p->s_avg[q] = h.s_avg[q];
+      p->s_var[q] = h.s_var[q];
+    }
+    const int ran = std::min(ngenerations, (int)h.hist_idx);
+    for (int q = 0; q < 6; q++)
+      p->gen_vals[q].insert(p->gen_vals[q].end(), hist.begin() + (size_t)q * ngenerations,
+                            hist.begin() + (size_t)q * ngenerations + ran);
+    if (kcol_hist)
+      std::copy(hist.begin() + (size_t)6 * ngenerations,
+                hist.begin() + (size_t)(6 + R) * ngenerations, kcol_hist);
+    p->gen_scores = abx_scores{};
+    p->run_started = false;  // the next abx_run starts afresh
+    p->bank_n = 0;
+    p->last_sites = 0;
+    *stats = abx_run_stats{};
+    fill_statistics(p, stats);
+    abx_diag& d = stats->diag;
+    d.histories = h.diag[0];
+    d.collisions = h.diag[1];
+    d.virtual_collisions = h.diag[2];
+    d.flights = h.diag[3];
+    d.crossings = h.diag[4];
+    d.tally_updates = h.diag[5];
+    d.lost = h.diag[6];
+    d.secondaries = h.diag[7];
+    stats->transport_seconds = total * 1e-3;
+    stats->kernel_seconds = kms * 1e-3;
+    stats->kernel_launches = ngenerations;
+    stats->kernel_items = (int64_t)(d.flights + d.collisions);
+    stats->histories = (int64_t)d.histories;
+  } catch (...) {
+    cleanup();
+    throw;
+  }
 }
 
 // FixedSource::run / ModifiedFixedSource::run / PowerIterator::run
@@ -1211,6 +1473,16 @@ int abx_run_continue(abx_problem* p, int ngenerations, int nignored, double* kco
   ABX_TRY({
     if (!p || !stats || ngenerations < 1 || nignored < 0) fail(ABX_E_INVALID, "run arguments");
     run_generations(p, ngenerations, nignored, kcol_hist, stats);
+  });
+}
+
+int abx_run_replicas(abx_problem* p, int replicas, const uint64_t* seeds, int ngenerations,
+                     int nignored, double* kcol_hist, abx_run_stats* stats) {
+  ABX_TRY({
+    if (!p || !stats || ngenerations < 1 || nignored < 0) fail(ABX_E_INVALID, "run arguments");
+    if (replicas < 1 || replicas > kMaxReplicas)
+      fail(ABX_E_INVALID, "replicas must be 1.." + std::to_string(kMaxReplicas));
+    run_replicas(p, replicas, seeds, ngenerations, nignored, kcol_hist, stats);
   });
 }
 

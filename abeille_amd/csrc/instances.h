@@ -29,6 +29,13 @@
                                                               unsigned long long*);           \
   EXT template __global__ void k_history<T, TR, SE, SP, false>(DevProblem, KCtx,              \
                                                                unsigned long long*);
+// replica mode: k-eigenvalue chains (no trace, no secondary stack)
+#define ABX_REP_DECL(EXT, T, SP)                                                              \
+  EXT template __global__ void k_history<T, false, false, SP, true, true>(DevProblem, KCtx,   \
+                                                                      unsigned long long*); \
+  EXT template __global__ void k_history<T, false, false, SP, false, true>(DevProblem, KCtx,  \
+                                                                       unsigned long long*);
+#define ABX_REP_ALL(X) X(0, false) X(0, true) X(1, false) X(1, true)
 #define ABX_BATCH_DECL(EXT, T, SE, SP)                                                        \
   EXT template __global__ void k_batch_chunks<T, SE, SP, true>(                               \
       const DevProblem*, const KCtx*, const BatchChunk*, int, int, int, unsigned*);           \
@@ -68,6 +75,9 @@ ABX_VARIANTS_T0(ABX_XE)
 ABX_VARIANTS_T1(ABX_XE)
 ABX_VARIANTS_T2(ABX_XE)
 ABX_INIT_DECL(extern)
+#define ABX_XR(T, SP) ABX_REP_DECL(extern, T, SP)
+ABX_REP_ALL(ABX_XR)
+#undef ABX_XR
 #undef ABX_XH
 #undef ABX_XB
 #undef ABX_XE

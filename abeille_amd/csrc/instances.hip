@@ -24,6 +24,8 @@ ABX_BATCH_T0(ABX_IB)
 #elif ABX_INST_PART == 4
 ABX_BATCH_T1(ABX_IB)
 ABX_INIT_DECL()
+#define ABX_IR(T, SP) ABX_REP_DECL(, T, SP)
+ABX_REP_ALL(ABX_IR)
 #elif ABX_INST_PART == 5
 ABX_VARIANTS_T0(ABX_IE)
 #elif ABX_INST_PART == 6

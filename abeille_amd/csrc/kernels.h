@@ -38,6 +38,14 @@ struct KCtx {
   // generation read from device memory (null = use n / kcol)
   const unsigned long long* n_dev;
   const double* kcol_dev;
+  // replica mode: the bank holds `nrep` independent k-eigenvalue chains in
+  // consecutive history segments rep_start[0..nrep]; a history uses its
+  // chain's k (rep_kcol) and scores into its chain's sums (rep_scores,
+  // [nrep][8]).  Null outside replica mode.
+  const unsigned long long* rep_start;
+  const double* rep_kcol;
+  double* rep_scores;
+  int32_t nrep;
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -64,6 +72,30 @@ __device__ __forceinline__ void flush_scores(const KCtx& K, const Scores6& sc,
     const unsigned long long t = wave_sum_u64((unsigned long long)c[i]);
     if (lane == 0 && t) atomicAdd(K.counters + i, t);
   }
+}
+
+// Replica mode: at most this many chains share a bank; a history's chain
+// is the segment of the bank that holds it.
+constexpr int kMaxReplicas = 16;
+
+__device__ __forceinline__ int chain_of(const unsigned long long* seg, int nrep,
+                                        unsigned long long i) {
+  int r = 0;
+  while (r + 1 < nrep && i >= seg[r + 1]) r++;
+  return r;
+}
+
+// Replica mode: a lane's scores go to its block's per-chain sums in LDS
+// (`acc`, [kMaxReplicas][6]) when the lane moves on to another chain and at
+// the end.  Every lane on the GPU moves on to the next chain at about the
+// same time (histories are handed out in order), so per-lane device
+// atomics on the six sums of one chain would all serialise at the memory
+// side; LDS atomics stay inside the CU.
+__device__ __forceinline__ void rep_scores_to_lds(double* acc, int rep, const Scores6& sc) {
+  const double s[6] = {sc.k_col, sc.k_abs, sc.k_trk, sc.k_tot, sc.leak, sc.mig};
+#pragma unroll
+  for (int i = 0; i < 6; i++)
+    if (s[i] != 0.0) atomicAdd(acc + 6 * rep + i, s[i]);
 }
 
 __device__ __forceinline__ void raise_error(const KCtx& K, int code, uint64_t h) {
@@ -311,11 +343,24 @@ __device__ __forceinline__ DevProblem stage_tables(const DevProblem& P) {
 // STAGE: the tables are copied into LDS and read with LDS instructions;
 // otherwise read in place (global loads through L2).  A compile-time choice,
 // so that each variant's loads carry their address space.
-template <int TRK, bool TRACE, bool SEC, bool SPLIT, bool STAGE>
+// REP: replica mode (K.rep_start and friends set), a variant of its own so
+// that the plain kernels carry none of its code.
+template <int TRK, bool TRACE, bool SEC, bool SPLIT, bool STAGE, bool REP = false>
 __global__ void ABX_TKERNEL k_history(DevProblem P0, KCtx K, unsigned long long* next) {
   const DevProblem P = STAGE ? stage_tables(P0) : P0;
   const int64_t n_hist = K.n_dev ? (int64_t)*K.n_dev : K.n;
   const double kcol = K.kcol_dev ? *K.kcol_dev : K.kcol;
+  // replica mode: the chains' segments and k, read per history from LDS
+  // (nothing per lane is kept: a lane's chain is that of its slot)
+  __shared__ unsigned long long s_seg[kMaxReplicas + 1];
+  __shared__ double s_kcol[kMaxReplicas];
+  __shared__ double s_racc[6 * kMaxReplicas];  // the block's score sums per chain
+  if constexpr (REP) {
+    if (threadIdx.x <= (unsigned)K.nrep) s_seg[threadIdx.x] = K.rep_start[threadIdx.x];
+    if (threadIdx.x < (unsigned)K.nrep) s_kcol[threadIdx.x] = K.rep_kcol[threadIdx.x];
+    if (threadIdx.x < 6u * kMaxReplicas) s_racc[threadIdx.x] = 0.0;
+    __syncthreads();
+  }
   const int lane = threadIdx.x & 63;
   Scores6 sc{0, 0, 0, 0, 0, 0};
   GpuSink<TRACE, SEC, SPLIT> s;
@@ -369,6 +414,15 @@ __global__ void ABX_TKERNEL k_history(DevProblem P0, KCtx K, unsigned long long*
       if (need) {
         const int64_t i = (int64_t)i64;
         if (i < n_hist) {
+          if constexpr (REP) {
+            // the lane's scores so far belong to its last history's chain
+            // (slot 0 and no scores before its first history)
+            const int r_old = chain_of(s_seg, K.nrep, (unsigned long long)s.slot);
+            if (chain_of(s_seg, K.nrep, (unsigned long long)i) != r_old) {
+              rep_scores_to_lds(s_racc, r_old, sc);
+              sc = Scores6{0, 0, 0, 0, 0, 0};
+            }
+          }
           s.slot = i;
           s.nsec = 0;
           s.n_rec = 0;
@@ -388,7 +442,13 @@ __global__ void ABX_TKERNEL k_history(DevProblem P0, KCtx K, unsigned long long*
     if (p.alive) {
       int ev = 0;
       bool hc = step_flight<TRK>(P, s, p, t, m, converged, sc, ev);
-      if (p.alive && hc) step_collision(P, s, p, m, kcol, converged, sc, ev);
+      if (p.alive && hc) {
+        if constexpr (REP)
+          step_collision(P, s, p, m, s_kcol[chain_of(s_seg, K.nrep, (unsigned long long)s.slot)],
+                         converged, sc, ev);
+        else
+          step_collision(P, s, p, m, kcol, converged, sc, ev);
+      }
       bool live = step_finish<TRK>(P, s, p, t, m, ev);
       if (live && ++steps > 100000000) {
         s.error(-3, p.hid);
@@ -402,6 +462,14 @@ __global__ void ABX_TKERNEL k_history(DevProblem P0, KCtx K, unsigned long long*
     }
   }
   s.close_stage_range();
+  if constexpr (REP) {
+    rep_scores_to_lds(s_racc, chain_of(s_seg, K.nrep, (unsigned long long)s.slot), sc);
+    sc = Scores6{0, 0, 0, 0, 0, 0};
+    __syncthreads();  // every wave of the block has added its sums
+    if (threadIdx.x < 6u * (unsigned)K.nrep && s_racc[threadIdx.x] != 0.0)
+      unsafeAtomicAdd(K.rep_scores + 8 * (threadIdx.x / 6) + threadIdx.x % 6,
+                      s_racc[threadIdx.x]);
+  }
   flush_scores(K, sc, s.c);
 }
 
@@ -1016,6 +1084,165 @@ __global__ void k_record_generation_dev(double* gen, double* avg, double* var, i
   const double dg = (double)(g0 + g->gen);
   const double old_avg = avg[i];
   const double val = gen[i] * 1.0;
+  const double a = old_avg + (val - old_avg) / dg;
+  avg[i] = a;
+  double v = var[i];
+  v = v + (((val - old_avg) * (val - a) - (v)) / dg);
+  var[i] = v;
+  gen[i] = 0.0;
+}
+
+// ------------------------------------------- replicas: chains in one bank
+// `nrep` independent k-eigenvalue chains of one problem (PowerIterator::run,
+// src/power_iterator.cpp:305-429, once per chain with its own seed) share
+// each generation's launches: their banks are consecutive segments of one
+// bank, the transport kernel runs all their histories, and the bookkeeping
+// below keeps every chain's k, weight normalisation, history ids and RNG
+// seed apart, so that each chain transports exactly the histories it
+// transports alone.  The mesh tallies hold the sum of the chains; a
+// generation records their mean.
+struct GenRep {
+  unsigned long long n;          // histories of the generation, all chains
+  unsigned long long n_sites;    // fission sites banked, all chains
+  unsigned long long n_stage, n_hole, next, err_hist, cap;
+  unsigned long long counters[8], diag[8];
+  unsigned long long seg[kMaxReplicas + 1];       // chain segments of the bank
+  unsigned long long site_seg[kMaxReplicas + 1];  // chain segments of the sites
+  unsigned long long first_hist[kMaxReplicas], run_hist[kMaxReplicas];
+  unsigned long long seed[kMaxReplicas];
+  double scores[kMaxReplicas][8];  // per chain: k_col k_abs k_trk k_tot leak mig
+  double wsum[kMaxReplicas][2];    // per chain: positive / negative banked weight
+  double kcol[kMaxReplicas];       // per chain: Tallies::kcol() for the next transport
+  double wpp[kMaxReplicas];        // per chain: nparticles / total weight
+  double s_avg[6], s_var[6];       // running statistics of the chains' mean
+  int32_t nrep, hist_cap, err, gen, hist_idx, pad_;
+};
+
+__global__ void k_rep_begin(GenRep* g) {
+  const int t = threadIdx.x;
+  if (t < 8 * kMaxReplicas) (&g->scores[0][0])[t] = 0.0;
+  if (t < 2 * kMaxReplicas) (&g->wsum[0][0])[t] = 0.0;
+  if (t < 8) g->counters[t] = 0;
+  if (t == 0) {
+    g->n_stage = 0;
+    g->n_hole = 0;
+    g->next = 0;
+  }
+}
+
+// Chain segments of the ordered sites: the exclusive scan of the per-parent
+// counts at each chain's first parent (the total past the last).
+__global__ void k_rep_sites(GenRep* g, const uint64_t* offsets) {
+  if (threadIdx.x != 0) return;
+  for (int r = 0; r <= g->nrep; r++)
+    g->site_seg[r] = g->seg[r] < g->n ? offsets[g->seg[r]] : g->n_sites;
+}
+
+// k_order_bank_dev with the weight sums kept per chain.
+__global__ void k_rep_order(const abx_banked* stage, const uint32_t* parent,
+                            const uint32_t* daughter, const uint64_t* offsets, GenRep* g,
+                            abx_banked* out) {
+  __shared__ double part[kMaxReplicas][2];
+  if (threadIdx.x < 2 * kMaxReplicas) (&part[0][0])[threadIdx.x] = 0.0;
+  __syncthreads();
+  const int64_t m = (int64_t)(g->n_stage < g->cap ? g->n_stage : g->cap);
+  if (!g->err) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m;
+         k += (int64_t)gridDim.x * blockDim.x) {
+      const uint32_t q = parent[k];
+      if (q == kHole) continue;
+      if ((unsigned long long)q >= g->n) {  // a slot handed out but never written
+        atomicCAS(&g->err, 0, ABX_E_INVALID);
+        continue;
+      }
+      const uint64_t dst = offsets[q] + daughter[k];
+      if (dst >= g->cap) continue;  // k_rep_end reports the overflow
+      const abx_banked b = stage[k];
+      out[dst] = b;
+      const int r = chain_of(g->seg, g->nrep, q);
+      if (b.wgt > 0.)
+        atomicAdd(&part[r][0], b.wgt);
+      else
+        atomicAdd(&part[r][1], -b.wgt);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 2 * g->nrep) {
+    const double v = (&part[0][0])[threadIdx.x];
+    if (v != 0.) unsafeAtomicAdd(&g->wsum[0][0] + threadIdx.x, v);
+  }
+}
+
+// End of a generation for every chain (one thread): each chain's values,
+// k, weight factor and history ids; the mean of the chains' six values joins
+// the per-generation vectors (`hist`, [6][hist_cap]) and the running
+// statistics; `khist` ([nrep][hist_cap]) receives every chain's k_col.
+__global__ void k_rep_end(GenRep* g, double W, int converged, double* hist, double* khist) {
+  if (threadIdx.x != 0) return;
+  const int R = g->nrep;
+  if (!g->err && g->n_sites != g->n_stage - g->n_hole) g->err = -1;
+  if (!g->err && g->n_stage > g->cap) g->err = -5;
+  if (!g->err && g->n_sites > g->cap) g->err = -5;
+  for (int r = 0; r < R; r++)
+    if (!g->err && g->site_seg[r + 1] == g->site_seg[r]) g->err = -8;
+  for (int k = 0; k < 8; k++) g->diag[k] += g->counters[k];
+  double x[6] = {0, 0, 0, 0, 0, 0};
+  for (int r = 0; r < R; r++) {
+    for (int q = 0; q < 6; q++) x[q] += g->scores[r][q] / W;
+    if (khist && g->hist_idx < g->hist_cap)
+      khist[(int64_t)r * g->hist_cap + g->hist_idx] = g->scores[r][0] / W;
+  }
+  for (int q = 0; q < 6; q++) x[q] /= (double)R;
+  if (hist && g->hist_idx < g->hist_cap)
+    for (int q = 0; q < 6; q++) hist[(int64_t)q * g->hist_cap + g->hist_idx] = x[q];
+  g->hist_idx++;
+  if (converged && !g->err) {
+    g->gen++;
+    for (int q = 0; q < 6; q++) gen_upd(g->gen, x[q], g->s_avg[q], g->s_var[q]);
+  }
+  for (int r = 0; r < R; r++) {
+    g->wpp[r] = W / (g->wsum[r][0] - g->wsum[r][1]);
+    g->first_hist[r] = g->run_hist[r];
+    g->run_hist[r] += g->site_seg[r + 1] - g->site_seg[r];
+    g->kcol[r] = g->scores[r][0] / W;
+  }
+  for (int r = 0; r <= R; r++) g->seg[r] = g->site_seg[r];
+  g->n = g->err ? 0 : g->n_sites;
+}
+
+// Re-bank of every chain's sites with the chain's weight factor, history
+// ids and seed.
+__global__ void k_rebank_rep(const abx_banked* b, const GenRep* g, uint64_t stride,
+                             abx_particle* out) {
+  const int64_t n = (int64_t)g->n;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int r = chain_of(g->seg, g->nrep, (unsigned long long)i);
+    const abx_banked& s = b[i];
+    abx_particle p;
+    for (int k = 0; k < 3; k++) {
+      p.r[k] = s.r[k];
+      p.u[k] = s.u[k];
+    }
+    p.E = s.E;
+    p.wgt = s.wgt * g->wpp[r];
+    p.wgt2 = 0.;
+    p.history_id = g->first_hist[r] + (uint64_t)(i - (int64_t)g->seg[r]);
+    p.family_id = s.family_id;
+    p.rng_state = pcg_history_state(g->seed[r], stride, p.history_id);
+    out[i] = p;
+  }
+}
+
+// MeshTally::record_generation of the chains' mean (the generation array
+// holds their sum), with the generation count read on the device.
+__global__ void k_record_rep(double* gen, double* avg, double* var, int64_t n, const GenRep* g,
+                             double mult) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double dg = (double)g->gen;
+  const double old_avg = avg[i];
+  const double val = gen[i] * mult;
   const double a = old_avg + (val - old_avg) / dg;
   avg[i] = a;
   double v = var[i];

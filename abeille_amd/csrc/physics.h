@@ -73,7 +73,13 @@ ABX_HD double flight_distance(Part& p, double lambda) {
     if (lambda == 0.0) return ABX_INF;
     p.calls += 2;
     const double xi = rng_rand(p.rng);
-    return (double)fp_div(-fp_log((float)(1.0 - xi)), (float)lambda);
+    // For xi below 2^-25, (float)(1 - xi) is 1.0f and the FP32 log would
+    // give a flight of zero length (one flight in 3e7): a particle that has
+    // just reflected would collide on the reflecting surface and could leave
+    // the geometry with its new direction.  Below 2^-12 the series
+    // xi + xi^2 / 2 is -log(1 - xi) to FP32 precision.
+    const float t = xi < 0x1p-12 ? (float)(xi + 0.5 * xi * xi) : -fp_log((float)(1.0 - xi));
+    return (double)fp_div(t, (float)lambda);
   }
 }
 

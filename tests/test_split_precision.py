@@ -89,3 +89,24 @@ def test_split_rejects_traces():
     bank = p.sample_sources(0, 10)
     with pytest.raises(Exception):
         p.transport(bank, trace_cap=8)
+
+
+def test_split_flight_never_zero_length():
+    """A history of the 17x17 lattice (seed + 2, generation 10 of a 1e6 run)
+    that once drew a flight of zero length in FP32 (1 - xi rounds to 1.0f
+    for xi < 2^-25), collided on the reflective plane it had just reflected
+    from, and left the geometry (ABX_E_LOST).  The FP32 flight distance now
+    takes -log(1 - xi) from its series for small xi, and the history ends
+    normally."""
+    m = problems.pin_lattice_2d(1_000_000)
+    m.set(precision=abi.SPLIT, seed=abi.DEFAULT_SEED + 2)
+    p = Problem(m)
+    b = np.zeros(1, abi.PARTICLE_DTYPE)
+    b[0]["r"] = [-1.3622847318790436, 1.5491719473775523, -9.360116872407609]
+    b[0]["u"] = [0.826570451259613, -0.5488812327384949, -0.12454143911600113]
+    b[0]["E"] = 1.5
+    b[0]["wgt"] = 1.0003661340050458
+    b[0]["history_id"] = 8701559
+    b[0]["rng_state"] = 10619179219843491746
+    res = p.transport(b, kcol=0.2942699356771855, converged=True)
+    assert res["diag"].histories == 1 and res["diag"].lost == 0
