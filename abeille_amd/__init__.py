@@ -76,8 +76,10 @@ def lib():
         L.abx_problem_stats.argtypes = [P, C.POINTER(abi.RunStats)]
         L.abx_fission_fetch.argtypes = [P, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]
         L.abx_gen_values.argtypes = [P, C.c_int, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]
-        L.abx_run_replicas.argtypes = [P, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
-                                       C.POINTER(abi.RunStats)]
+        L.abx_run_replicas.argtypes = [P, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                       C.c_void_p, C.POINTER(abi.RunStats)]
+        L.abx_run_replicas_continue.argtypes = [P, C.c_int, C.c_int, C.c_void_p,
+                                                C.POINTER(abi.RunStats)]
         L.abx_transport_noalloc.argtypes = [P, C.c_void_p, C.c_size_t, C.c_double, C.c_int,
                                             C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t),
                                             C.POINTER(abi.Scores), C.POINTER(abi.Diag)]
@@ -196,11 +198,12 @@ class Problem:
                                       C.byref(st)))
         return st, kh
 
-    def run_replicas(self, replicas, ngenerations, nignored=0, seeds=None):
+    def run_replicas(self, replicas, ngenerations, nignored=0, seeds=None, from_bank=False):
         """abx_run_replicas: `replicas` chains of this k-eigenvalue problem
-        (seeds[r], default settings.seed + r) run together.  Returns the
-        statistics of the chains' mean and k_col per chain and generation
-        ([replicas][ngenerations])."""
+        (seeds[r], default settings.seed + r) run together, each from its own
+        sampled source or (from_bank) from the bank of the problem's last
+        run.  Returns the statistics of the chains' mean and k_col per chain
+        and generation ([replicas][ngenerations])."""
         st = abi.RunStats()
         kh = np.zeros((replicas, ngenerations))
         sd = None
@@ -208,7 +211,17 @@ class Problem:
             sd = np.ascontiguousarray(seeds, dtype=np.uint64)
             assert len(sd) == replicas
         _check(lib().abx_run_replicas(self.h, replicas, None if sd is None else sd.ctypes.data,
-                                      ngenerations, nignored, kh.ctypes.data, C.byref(st)))
+                                      1 if from_bank else 0, ngenerations, nignored,
+                                      kh.ctypes.data, C.byref(st)))
+        self._replicas = replicas
+        return st, kh
+
+    def run_replicas_continue(self, ngenerations, nignored=0):
+        """abx_run_replicas_continue: more generations of the last replica run."""
+        st = abi.RunStats()
+        kh = np.zeros((getattr(self, "_replicas", 1), ngenerations))
+        _check(lib().abx_run_replicas_continue(self.h, ngenerations, nignored, kh.ctypes.data,
+                                               C.byref(st)))
         return st, kh
 
     def fission_fetch(self, cap):

@@ -101,6 +101,7 @@ struct abx_problem {
   DBuf bank_a, bank_b;  // abx_particle banks
   size_t bank_n = 0;
   bool run_started = false;
+  int rep_chains = 0;       // chains of the replica run that can be continued (0 = none)
   bool use_graphs = true;   // k-eigenvalue runs replay captured generations
   uint64_t run_hist = 0;  // histories_counter
   double run_kcol = 1.0;  // Tallies::kcol() of the previous generation
@@ -813,6 +814,7 @@ static void run_reset(abx_problem* p) {
   p->run_hist = 0;
   p->run_kcol = 1.0;
   p->bank_n = 0;
+  p->rep_chains = 0;
 }
 
 // Tallies' running means and standard errors (src/tallies.cpp:268-280).
@@ -1089,19 +1091,31 @@ static hipError_t rep_dispatch(int trk, bool split, Fn&& fn) {
 }
 }
 
-static void run_replicas(abx_problem* p, int R, const uint64_t* seeds, int ngenerations,
-                         int nignored, double* kcol_hist, abx_run_stats* stats) {
+static void run_replicas(abx_problem* p, int R, const uint64_t* seeds, bool from_bank,
+                         int ngenerations, int nignored, double* kcol_hist, abx_run_stats* stats,
+                         bool cont) {
   const HostTables& T = p->T;
   if (T.st.mode != ABX_K_EIGENVALUE) fail(ABX_E_INVALID, "replicas are k-eigenvalue chains");
   if (p->cancel) fail(ABX_E_INVALID, "replicas: approximate cancellation is not supported");
   if (needs_secondaries(T)) fail(ABX_E_INVALID, "replicas: carter tracking is not supported");
-  run_reset(p);
+  // the bank every chain starts from (from_bank): the problem's current one
+  const uint64_t n0 = from_bank ? (uint64_t)p->bank_n : 0;
+  const double kcol0 = p->run_kcol;
+  if (from_bank && (!p->run_started || n0 == 0))
+    fail(ABX_E_INVALID, "replicas: no k-eigenvalue bank to start from (abx_run first)");
+  if (cont) {
+    if (p->rep_chains < 1) fail(ABX_E_INVALID, "replicas: no replica run to continue");
+    R = p->rep_chains;
+  } else {
+    run_reset(p);
+  }
   const int64_t N = T.st.nparticles;
   if (N < 1) fail(ABX_E_INVALID, "nparticles must be positive");
   const double W = (double)N;
-  const uint64_t NR = (uint64_t)N * (uint64_t)R;
+  const uint64_t NR = (uint64_t)std::max<uint64_t>((uint64_t)N, n0) * (uint64_t)R;
   const uint64_t cap = 4 * NR + 4096;
   hipStream_t s = p->stream;
+  if (from_bank) grow_keep(p->bank_a, cap * sizeof(abx_particle), n0 * sizeof(abx_particle), s);
   p->bank_a.reserve(cap * sizeof(abx_particle));
   p->bank_b.reserve(cap * sizeof(abx_particle));
   const int64_t nb = (int64_t)((cap + 1023) / 1024);
@@ -1110,11 +1124,33 @@ static void run_replicas(abx_problem* p, int R, const uint64_t* seeds, int ngene
   p->ordered.reserve(cap * sizeof(abx_banked));
   p->gdev.reserve(sizeof(GenRep));
   p->khist_dev.reserve((size_t)(6 + R) * ngenerations * sizeof(double));
+  GenRep* g = p->gdev.as<GenRep>();
+  GenRep h{};
+  if (cont) {
+    // the chains' banks, k, history counters and statistics carry over;
+    // the per-generation arrays and diagnostics are this call's
+    HIPC(hipMemcpyAsync(&h, g, sizeof(GenRep), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    if (h.nrep != R) fail(ABX_E_INVALID, "replicas: device state does not match the run");
+    h.hist_cap = ngenerations;
+    h.hist_idx = 0;
+    for (int k = 0; k < 8; k++) h.diag[k] = 0;
+    HIPC(hipMemcpyAsync(g, &h, sizeof(GenRep), hipMemcpyHostToDevice, s));
+    HIPC(hipStreamSynchronize(s));
+  }
   uint64_t sd[kMaxReplicas];
   for (int r = 0; r < R; r++) sd[r] = seeds ? seeds[r] : T.st.seed + (uint64_t)r;
   // each chain's first generation: histories 0..N-1 sampled with its seed
-  HIPC(hipMemsetAsync(p->err.p, 0, sizeof(int32_t), s));
-  for (int r = 0; r < R; r++) {
+  if (!cont) HIPC(hipMemsetAsync(p->err.p, 0, sizeof(int32_t), s));
+  // from_bank: chain r's copy of the bank becomes its histories 0..n0-1 on
+  // its own stream (the last chain first: chain 0's copy is the bank itself)
+  for (int r = (cont || !from_bank) ? -1 : R - 1; r >= 0; r--) {
+    hipLaunchKernelGGL(k_fork_chain, dim3((unsigned)blocks_for((int64_t)n0, 256)), dim3(256), 0,
+                       s, p->bank_a.as<abx_particle>(), (int64_t)n0, sd[r], T.st.stride,
+                       p->bank_a.as<abx_particle>() + (size_t)r * n0);
+    HIPC(hipGetLastError());
+  }
+  for (int r = 0; r < (cont || from_bank ? 0 : R); r++) {
     DevProblem Pr = p->P;
     Pr.seed = sd[r];
     hipLaunchKernelGGL(k_sample_sources, dim3((unsigned)blocks_for(N, 256)), dim3(256),
@@ -1123,27 +1159,27 @@ static void run_replicas(abx_problem* p, int R, const uint64_t* seeds, int ngene
                        p->err_hist.as<unsigned long long>());
     HIPC(hipGetLastError());
   }
-  {
+  if (!cont) {
     int32_t err = 0;
     HIPC(hipMemcpyAsync(&err, p->err.p, sizeof(err), hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
     if (err) fail(err, "source sampling failed");
+    const uint64_t per = from_bank ? n0 : (uint64_t)N;  // histories per chain
+    h = GenRep{};
+    h.n = per * (uint64_t)R;
+    h.cap = cap;
+    h.nrep = R;
+    for (int r = 0; r < R; r++) {
+      h.seg[r] = (uint64_t)r * per;
+      h.run_hist[r] = per;
+      h.seed[r] = sd[r];
+      h.kcol[r] = from_bank ? kcol0 : 1.0;
+    }
+    h.seg[R] = per * (uint64_t)R;
+    h.hist_cap = ngenerations;
+    HIPC(hipMemcpyAsync(g, &h, sizeof(GenRep), hipMemcpyHostToDevice, s));
+    HIPC(hipStreamSynchronize(s));
   }
-  GenRep* g = p->gdev.as<GenRep>();
-  GenRep h{};
-  h.n = NR;
-  h.cap = cap;
-  h.nrep = R;
-  for (int r = 0; r < R; r++) {
-    h.seg[r] = (uint64_t)r * N;
-    h.run_hist[r] = (uint64_t)N;
-    h.seed[r] = sd[r];
-    h.kcol[r] = 1.0;
-  }
-  h.seg[R] = NR;
-  h.hist_cap = ngenerations;
-  HIPC(hipMemcpyAsync(g, &h, sizeof(GenRep), hipMemcpyHostToDevice, s));
-  HIPC(hipStreamSynchronize(s));
   const CallScalars cs{&g->scores[0][0], g->counters, &g->err, &g->err_hist, &g->n_stage,
                        &g->n_hole};
   KCtx K = make_kctx(p, p->bank_a.as<abx_particle>(), (int64_t)NR, 1.0, 1, cs, nullptr, 0);
@@ -1303,6 +1339,9 @@ static void run_replicas(abx_problem* p, int R, const uint64_t* seeds, int ngene
     p->run_started = false;  // the next abx_run starts afresh
     p->bank_n = 0;
     p->last_sites = 0;
+    // the latest bank is in bank_b after an odd number of generations
+    if (par) std::swap(p->bank_a, p->bank_b);
+    p->rep_chains = R;
     *stats = abx_run_stats{};
     fill_statistics(p, stats);
     abx_diag& d = stats->diag;
@@ -1476,13 +1515,23 @@ int abx_run_continue(abx_problem* p, int ngenerations, int nignored, double* kco
   });
 }
 
-int abx_run_replicas(abx_problem* p, int replicas, const uint64_t* seeds, int ngenerations,
-                     int nignored, double* kcol_hist, abx_run_stats* stats) {
+int abx_run_replicas(abx_problem* p, int replicas, const uint64_t* seeds, int from_bank,
+                     int ngenerations, int nignored, double* kcol_hist, abx_run_stats* stats) {
   ABX_TRY({
     if (!p || !stats || ngenerations < 1 || nignored < 0) fail(ABX_E_INVALID, "run arguments");
     if (replicas < 1 || replicas > kMaxReplicas)
       fail(ABX_E_INVALID, "replicas must be 1.." + std::to_string(kMaxReplicas));
-    run_replicas(p, replicas, seeds, ngenerations, nignored, kcol_hist, stats);
+    run_replicas(p, replicas, seeds, from_bank != 0, ngenerations, nignored, kcol_hist, stats,
+                 false);
+  });
+}
+
+int abx_run_replicas_continue(abx_problem* p, int ngenerations, int nignored,
+                              double* kcol_hist, abx_run_stats* stats) {
+  ABX_TRY({
+    if (!p || !stats || ngenerations < 1 || nignored < 0) fail(ABX_E_INVALID, "run arguments");
+    run_replicas(p, p->rep_chains, nullptr, false, ngenerations, nignored, kcol_hist, stats,
+                 true);
   });
 }
 

@@ -1234,6 +1234,18 @@ __global__ void k_rebank_rep(const abx_banked* b, const GenRep* g, uint64_t stri
   }
 }
 
+// A chain started from a shared bank: the bank's particles become the
+// chain's histories 0..n-1 on its own stream (`dst` may be `src`).
+__global__ void k_fork_chain(const abx_particle* src, int64_t n, uint64_t seed, uint64_t stride,
+                             abx_particle* dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  abx_particle q = src[i];
+  q.history_id = (uint64_t)i;
+  q.rng_state = pcg_history_state(seed, stride, q.history_id);
+  dst[i] = q;
+}
+
 // MeshTally::record_generation of the chains' mean (the generation array
 // holds their sum), with the generation count read on the device.
 __global__ void k_record_rep(double* gen, double* avg, double* var, int64_t n, const GenRep* g,

@@ -1,16 +1,30 @@
 """Benchmark of the transport hot path on one MI355X.
 
-Workload (default): the 17x17 pin-lattice k-eigenvalue problem - pitch
-1.26 cm, pin radius 0.54 cm, reflective, 7 groups, delta tracking, 1e6
-particles per generation, 51x51x7 track-length flux tally, FP64.  A step is
-one generation; `--warmup` inactive generations run first (untimed, they
-also converge the source), then exactly `--steps` active generations are
-timed with the device-resident generation loop.
+Default workload (the headline): the 17x17 pin-lattice k-eigenvalue problem
+- pitch 1.26 cm, pin radius 0.54 cm, reflective, 7 groups, delta tracking,
+1e6 particles per generation, 51x51x7 track-length flux tally - on two
+paths that run the same number of active histories (so their tallies reach
+the same per-bin relative error):
+  * plain: FP64, one chain, the device-resident generation loop;
+  * batched: `--chains` chains of the problem with their own seeds, split
+    precision, one transport launch per generation for all chains.
+`--warmup` inactive generations run first (untimed; they converge the
+source), then exactly `--steps` batched generations are timed (the plain
+leg runs chains x steps generations).  A short served stream (Poisson
+arrivals, continuous batching) gives p50/p99 latency against serving the
+same requests one at a time on the plain path.
 
-Prints one JSON line: collisions/s (value), histories/s, the roofline of the
-dominant transport kernel (algorithmic bytes per launch / HIP-event launch
-time, HBM-bound) and a CPU baseline (the CPU restatement under OpenMP on
-this host's cores, on a bounded sample of the same workload).
+Prints one JSON line: collisions/s of the batched path (value), the plain
+path's rate beside it, each path's per-bin relative errors and the
+histories and seconds each needs to bring 95 % of the bins to 1 %, the
+latencies, the roofline of the dominant transport kernel (algorithmic bytes
+per launch / HIP-event launch time, HBM-bound; issue rate from the committed
+counter profile) and a CPU baseline (the CPU restatement under OpenMP on this
+host's cores, on a bounded sample of the same workload).
+
+Other workloads: --workload lattice|core|cube (the plain path alone),
+batch (512 slab problems), serve (the full served stream),
+--compare-precision (FP64 against split precision, per-bin deviations).
 """
 import argparse
 import json
@@ -32,6 +46,8 @@ BYTES_PER_EVENT = 160
 BYTES_PER_EVENT_SPLIT = 136
 BYTES_PER_TALLY_UPDATE = 16
 BYTES_PER_HISTORY = 80
+# the 17x17 lattice's inactive generations (they converge the fission source)
+INACTIVE = 50
 
 
 def workload(name, nparticles, tracking, precision):
@@ -146,6 +162,133 @@ def _c4_problems(count, precision, seed=512):
     return out
 
 
+def rel_err_summary(avg, err, hist_active, hist_per_s, target=0.01):
+    """Per-bin relative error of a tally (bins with a nonzero mean) and the
+    active histories (and seconds at `hist_per_s`) that would bring 95 % of
+    those bins to `target`: the error falls as 1/sqrt(histories)."""
+    nz = avg > 0
+    rel = err[nz] / avg[nz]
+    q = {f"p{k}": float(np.percentile(rel, k)) for k in (50, 90, 95, 99)}
+    need = hist_active * (q["p95"] / target) ** 2
+    return {"bins": int(nz.sum()), "rel_err": q,
+            "histories_to_1pct_p95": need, "seconds_to_1pct_p95": need / hist_per_s}
+
+
+def bench_headline(args):
+    """The headline: the 17x17 lattice k-eigenvalue problem (1e6 particles
+    per generation, 7 groups, delta tracking, 51x51x7 track-length tally) on
+    the plain FP64 path (one chain, the device-resident generation loop)
+    and on the batched mixed-precision path (`--chains` chains of the same
+    problem with their own seeds, split precision, one transport launch per
+    generation for all chains).  Each path first converges its source with
+    INACTIVE inactive generations of one chain (untimed; the batched chains
+    all start from that bank), then `--warmup` untimed generations, then the
+    timed ones: `--steps` generations of the batched path, chains x steps of
+    the plain path, the same number of active histories, so that both
+    tallies reach the same per-bin relative error (both reported).  A
+    served-stream leg reports p50/p99 latency beside throughput."""
+    from abeille_amd import Problem, abi, synchronize
+    from abeille_amd.autotune import autotune
+    N = int(args.particles)
+    R = args.chains
+    K = args.steps
+    W = max(0, args.warmup)
+    # ---- plain FP64 GPU path: one chain
+    m, desc = workload("lattice", N, "delta", "fp64")
+    plain = Problem(m)
+    tuning = plain.tuning()
+    if not args.no_autotune:
+        tuning, _ = autotune(plain, m, sample=N, cache=None)
+    plain.run(INACTIVE, nignored=INACTIVE)
+    if W:
+        plain.run_continue(W, nignored=W)
+    synchronize()
+    t0 = time.perf_counter()
+    sp, _ = plain.run_continue(R * K, nignored=0)
+    synchronize()
+    wall_p = time.perf_counter() - t0
+    hist_p = sp.histories
+    avg_p, err_p = plain.tally_stats(0)
+    plain_out = {"collisions_per_s": sp.diag.collisions / wall_p,
+                 "histories_per_s": hist_p / wall_p,
+                 "ms_per_generation": 1e3 * wall_p / (R * K), "generations": R * K,
+                 "active_histories": hist_p, "kcol": [sp.kcol_avg, sp.kcol_err],
+                 "roofline": _roofline(sp, False, "k_history")}
+    plain_out.update(rel_err_summary(avg_p, err_p, hist_p, hist_p / wall_p))
+    del plain
+    # ---- batched split path: R chains started from one converged bank
+    mb, _ = workload("lattice", N, "delta", "split")
+    bat = Problem(mb)
+    bat.set_tuning(**{k: tuning[k] for k in ("block_size", "blocks_per_cu", "stage_tables")})
+    bat.run(INACTIVE, nignored=INACTIVE)
+    bat.run_replicas(R, max(1, W), max(1, W), from_bank=True)
+    synchronize()
+    t0 = time.perf_counter()
+    sb, _ = bat.run_replicas_continue(K, 0)
+    synchronize()
+    wall_b = time.perf_counter() - t0
+    hist_b = sb.histories
+    avg_b, err_b = bat.tally_stats(0)
+    coll_b = sb.diag.collisions / wall_b
+    bat_out = {"collisions_per_s": coll_b, "histories_per_s": hist_b / wall_b,
+               "ms_per_generation_of_all_chains": 1e3 * wall_b / K, "generations": K,
+               "chains": R, "active_histories": hist_b, "kcol": [sb.kcol_avg, sb.kcol_err]}
+    bat_out.update(rel_err_summary(avg_b, err_b, hist_b, hist_b / wall_b))
+    roof = _roofline(sb, True, "k_history (chains, split)")
+    del bat
+    # ---- served stream: p50 / p99 latency
+    served = None if args.no_serve else serve_legs(args, args.requests)
+    out = _common(args, coll_b, "collisions/s", 1e3 * wall_b / K, "f64/f32 split",
+                  {"workload": f"{desc}; batched form: {R} chains (own seeds) in one launch "
+                               "per generation, split precision; plain form: one chain, FP64",
+                   "nparticles_per_chain": N, "chains": R, "inactive_generations": INACTIVE,
+                   "tuning": tuning},
+                  {"histories_per_s": hist_b / wall_b,
+                   "speedup_vs_plain": coll_b / plain_out["collisions_per_s"],
+                   "time_to_1pct_ratio": plain_out["seconds_to_1pct_p95"]
+                   / bat_out["seconds_to_1pct_p95"],
+                   "plain_fp64": plain_out, "batched_split": bat_out,
+                   "latency_s": served,
+                   "roofline": roof,
+                   "issue": measured_issue("k_history", "lattice", "split_chains")})
+    if not args.no_cpu:
+        nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["cpu_baseline"] = cpu_baseline("lattice", "delta", args.cpu_seconds, nthreads)
+    print(json.dumps(out), flush=True)
+
+
+def _roofline(st, split, kname):
+    """HBM roofline of the transport kernel of a run: algorithmic bytes per
+    launch (one launch per generation) over the HIP-event launch time."""
+    launches = max(1, st.kernel_launches)
+    kernel_avg = st.kernel_seconds / launches
+    per_launch = algorithmic_bytes(st.diag, split) / launches
+    achieved = per_launch / kernel_avg / 1e9 if kernel_avg > 0 else 0.0
+    wl, prec = ("lattice", "split_chains") if split else ("lattice", "fp64")
+    return {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
+            "traffic": measured_traffic("k_history", wl, prec),
+            "avg_launch_ms": kernel_avg * 1e3, "launches": st.kernel_launches,
+            "algorithmic_bytes_per_launch": per_launch}
+
+
+def measured_issue(kernel, workload_name, precision):
+    """Instruction-issue roofline of `kernel` from the committed rocprofv3
+    counter summary: VALU instructions per event and the share of the
+    chip's VALU issue slots the kernel uses (or None without a summary)."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload_name}_{precision}_pmc.md")
+    out = {}
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("issue_") and ":" in line:
+                    k, v = line.split(":", 1)
+                    out[k[len("issue_"):].strip()] = float(v.split()[0])
+    except OSError:
+        return None
+    return out or None
+
+
 def bench_precision(args):
     """The workload once in FP64 and once in split precision: collisions/s of
     both and, per tally, the distribution of the per-bin differences in
@@ -256,6 +399,50 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def serve_legs(args, n_requests):
+    """The served stream in short (the headline's latency legs): Poisson
+    arrivals of `n_requests` requests of the served-stream generator, served
+    with continuous batching in split precision, and the same requests
+    served one at a time on the plain FP64 path, first come first served.
+    Returns p50 / p99 latency and throughput of both."""
+    from abeille_amd import Problem, abi, problems
+    from abeille_amd.serve import Request, Server, latency_percentiles, serial_latencies
+    rng = np.random.default_rng(2025)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, n_requests))
+    reqs, plain_models = [], []
+    for i in range(n_requests):
+        state = rng.bit_generator.state
+        m, _, kind = problems.served_request(rng)
+        rng2 = np.random.default_rng()
+        rng2.bit_generator.state = state
+        mp, _, _ = problems.served_request(rng2)
+        m.set(precision=abi.SPLIT)
+        reqs.append(Request(m, 2, float(arrivals[i]), name=f"{kind}{i}"))
+        plain_models.append(mp)
+    Server(keep_tallies=False).run([Request(reqs[0].model, 1, 0.0)])  # warm-up
+    srv = Server(budget=64 << 30, keep_tallies=False)
+    t0 = time.perf_counter()
+    srv.run(reqs)
+    wall = time.perf_counter() - t0
+    coll = sum(r.stats.diag.collisions for r in reqs)
+    service = np.zeros(n_requests)
+    pc = 0
+    for i, m in enumerate(plain_models):
+        t = time.perf_counter()
+        p = Problem(m)
+        st, _ = p.run(2, 0)
+        service[i] = time.perf_counter() - t
+        pc += st.diag.collisions
+        del p
+    lat = serial_latencies(arrivals, service)
+    makespan = float(np.max(arrivals + lat))
+    return {"requests": n_requests, "rate_per_s": args.rate,
+            "served_split": dict(latency_percentiles(reqs), collisions_per_s=coll / wall),
+            "one_at_a_time_fp64": {"p50": float(np.percentile(lat, 50)),
+                                   "p99": float(np.percentile(lat, 99)),
+                                   "collisions_per_s": pc / makespan}}
+
+
 def bench_serve(args):
     """Served stream: Poisson arrivals of slab requests (20 % carter tracking
     with a 0.8 x majorant sampling cross section and approximate mesh
@@ -286,9 +473,8 @@ def bench_serve(args):
     coll = sum(r.stats.diag.collisions for r in reqs)
     hist = sum(r.stats.histories for r in reqs)
     pct = latency_percentiles(reqs)
-    # One at a time on the plain FP64 path.  The carter requests are left
-    # out of this comparison: their single-problem FP64 runs faulted the GPU
-    # and stay off it until the cause is found (the served path runs them).
+    # One at a time on the plain FP64 path: the fixed-source requests here;
+    # every request, carter chains included, in the headline's serve_legs
     plain_idx = [i for i in range(len(reqs)) if kinds[i] != "carter"]
     service = np.zeros(len(plain_idx))
     pc = 0
@@ -350,13 +536,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="lattice",
-                    choices=["lattice", "core", "cube", "batch", "serve"])
+    ap.add_argument("--workload", default="headline",
+                    choices=["headline", "lattice", "core", "cube", "batch", "serve"])
+    ap.add_argument("--chains", type=int, default=8,
+                    help="headline: chains of the batched k-eigenvalue form (1..16)")
     ap.add_argument("--compare-precision", action="store_true",
                     help="run the workload in FP64 and in split precision and report the "
                          "rates and every tally bin's deviation")
     ap.add_argument("--problems", type=int, default=512, help="batch: number of problems")
-    ap.add_argument("--requests", type=int, default=100, help="serve: number of requests")
+    ap.add_argument("--requests", type=int, default=None,
+                    help="serve: number of requests (default 100; 40 in the headline)")
     ap.add_argument("--rate", type=float, default=50.0, help="serve: arrivals per second")
     ap.add_argument("--step-histories", type=int, default=0,
                     help="serve: histories per step over all requests (0 = whole batches)")
@@ -370,6 +559,7 @@ def main():
     ap.add_argument("--tuning", default=None,
                     help="fixed launch shape block_size,blocks_per_cu,stage_tables (no autotuning)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="headline: skip the served-stream leg")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -390,8 +580,12 @@ def main():
                                    "reduction), --gpus must be 1"}))
         sys.exit(2)
 
+    if args.requests is None:
+        args.requests = 40 if args.workload == "headline" else 100
     if args.compare_precision:
         return bench_precision(args)
+    if args.workload == "headline":
+        return bench_headline(args)
     if args.workload == "batch":
         return bench_batch(args)
     if args.workload == "serve":

@@ -446,17 +446,28 @@ int abx_tally_stats(abx_problem* p, int tally, double* avg, double* std_err);
  * (1..16) independent chains of one k-eigenvalue problem that differ only
  * in their seed (seeds[r], or NULL for settings.seed + r), run together:
  * every generation is one transport launch over all chains' histories and
- * one chain-aware bookkeeping pass, replayed from captured graphs.  Chain r
- * runs exactly the histories abx_run runs with seed seeds[r] (sources
- * 0..nparticles-1, its own k, weight normalisation and history ids).  The
+ * one chain-aware bookkeeping pass, replayed from captured graphs.  With
+ * from_bank == 0, chain r runs exactly the histories abx_run runs with seed
+ * seeds[r] (sources 0..nparticles-1, its own k, weight normalisation and
+ * history ids).  With from_bank != 0, every chain starts from the bank the
+ * problem holds after abx_run / abx_run_continue (a converged source): its
+ * particles become histories 0..n-1 of each chain's stream, with their
+ * weights and the run's last k.  The
  * mesh tallies and the statistics in *stats are those of the chains' mean
  * per generation (abx_tally_stats, abx_gen_values); kcol_hist (optional,
  * [replicas][ngenerations]) receives every chain's k_col.  The reference
  * runs one chain per process; this is the batched form of its k-eigenvalue
  * loop.  Not with carter tracking or cancellation.  The next abx_run starts
  * a new run. */
-int abx_run_replicas(abx_problem* p, int replicas, const uint64_t* seeds, int ngenerations,
-                     int nignored, double* kcol_hist, abx_run_stats* stats);
+int abx_run_replicas(abx_problem* p, int replicas, const uint64_t* seeds, int from_bank,
+                     int ngenerations, int nignored, double* kcol_hist, abx_run_stats* stats);
+/* Continue the last abx_run_replicas for `ngenerations` more generations
+ * (chains, banks, k and statistics carry over, as abx_run_continue does for
+ * one chain; `nignored` counts inactive generations from here on).  Stats
+ * cover these generations' timing and diagnostics; kcol_hist is
+ * [replicas][ngenerations]. */
+int abx_run_replicas_continue(abx_problem* p, int ngenerations, int nignored,
+                              double* kcol_hist, abx_run_stats* stats);
 
 /* The per-generation vectors of Tallies (k_col_vec, k_abs_vec, k_trk_vec,
  * leak_vec, mig_vec; src/tallies.cpp:159-181), every generation since the

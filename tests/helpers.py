@@ -1,4 +1,6 @@
 """Shared comparison helpers for the parity tests."""
+import ctypes as C
+
 import numpy as np
 
 
@@ -41,3 +43,51 @@ def bins_close(a, b, rtol=1e-9):
     largest bin (a bin holding only a sub-micron corner clip)."""
     scale = max(float(np.abs(a).max()), 1e-300)
     return bool(np.all(np.abs(a - b) <= rtol * np.abs(a) + 1e-9 * scale))
+
+
+_hip = None
+
+
+def hip():
+    """The HIP runtime libabx.so itself uses (same shared object), for
+    device buffers in tests of the entry points that take device pointers."""
+    global _hip
+    if _hip is None:
+        import abeille_amd
+        abeille_amd.lib()  # loads libamdhip64 as a dependency of libabx.so
+        _hip = C.CDLL("libamdhip64.so.7")
+        _hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        _hip.hipFree.argtypes = [C.c_void_p]
+        _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _hip.hipDeviceSynchronize.argtypes = []
+    return _hip
+
+
+class DeviceBuffer:
+    """hipMalloc'd bytes; upload / download of numpy arrays."""
+
+    def __init__(self, nbytes):
+        self.nbytes = max(1, int(nbytes))
+        self.ptr = C.c_void_p()
+        rc = hip().hipMalloc(C.byref(self.ptr), self.nbytes)
+        assert rc == 0, f"hipMalloc failed ({rc})"
+
+    @property
+    def addr(self):
+        return self.ptr.value
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr)
+        assert a.nbytes <= self.nbytes
+        assert hip().hipMemcpy(self.ptr, a.ctypes.data, a.nbytes, 1) == 0  # host to device
+
+    def download(self, nbytes, dtype):
+        out = np.zeros(max(1, nbytes), np.uint8)
+        if nbytes:
+            assert hip().hipMemcpy(out.ctypes.data, self.ptr, nbytes, 2) == 0  # device to host
+        return out[:nbytes].view(dtype)
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and self.ptr.value:
+            hip().hipFree(self.ptr)
+            self.ptr = C.c_void_p()

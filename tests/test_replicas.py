@@ -66,6 +66,45 @@ def test_replicas_equal_chains_alone(name, R):
         assert bins_close(mean_alone, avg, 1e-8), t
 
 
+def test_replicas_continue_equals_one_call():
+    """Two calls (inactive generations, then active ones) give the chains and
+    statistics of one call over all generations."""
+    m = problems.pin_lattice_2d(3000, tracking="delta")
+    a, b = Problem(m), Problem(m)
+    sa, ka = a.run_replicas(3, 7, 2, seeds=SEEDS[:3])
+    b.run_replicas(3, 2, 2, seeds=SEEDS[:3])
+    sb, kb = b.run_replicas_continue(5, 0)
+    assert np.allclose(ka[:, 2:], kb, rtol=1e-9, atol=0)
+    assert sb.active_generations == sa.active_generations == 5
+    assert sb.kcol_avg == pytest.approx(sa.kcol_avg, rel=1e-9)
+    assert sb.kcol_err == pytest.approx(sa.kcol_err, rel=1e-6)
+    for t in range(len(m.tallies)):
+        assert bins_close(a.tally_stats(t)[0], b.tally_stats(t)[0], 1e-8)
+    with pytest.raises(AbxError, match="no replica run"):
+        Problem(m).run_replicas_continue(2, 0)
+
+
+def test_replicas_from_bank():
+    """Chains started from the bank of a converged run: deterministic, every
+    chain's first generation transports that bank (on its own stream), and a
+    problem without a bank is refused."""
+    m = problems.pin_lattice_2d(3000, tracking="delta")
+    runs = []
+    for _ in range(2):
+        p = Problem(m)
+        p.run(4, 4)
+        n0 = len(p.bank())
+        st, kh = p.run_replicas(3, 4, 0, seeds=SEEDS[:3], from_bank=True)
+        runs.append((st, kh))
+    (s1, k1), (s2, k2) = runs
+    assert np.allclose(k1, k2, rtol=1e-12, atol=0)   # weight sums add in any order
+    assert len(set(k1[:, 0].tolist())) == 3      # same particles, different streams
+    assert s1.diag.histories > 3 * n0 * 3        # four generations of three chains
+    assert abs(s1.kcol_avg - 0.295) < 0.02
+    with pytest.raises(AbxError, match="no k-eigenvalue bank"):
+        Problem(m).run_replicas(2, 2, 0, from_bank=True)
+
+
 def test_replica_chain_matches_oracle():
     """Chain 1 of a 2-chain run against the CPU restatement with its seed."""
     m = problems.pin_lattice_2d(2000, tracking="delta")

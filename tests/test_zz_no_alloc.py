@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 from abeille_amd import AbxError, Problem, abi, problems
+from helpers import DeviceBuffer
 
 pytestmark = pytest.mark.gpu
 
@@ -22,8 +23,10 @@ REC_P = abi.PARTICLE_DTYPE.itemsize
 REC_B = abi.BANKED_DTYPE.itemsize
 
 
-def _to_device(torch, arr):
-    return torch.from_numpy(np.ascontiguousarray(arr).view(np.uint8).copy()).cuda()
+def _to_device(arr):
+    b = DeviceBuffer(arr.nbytes)
+    b.upload(arr)
+    return b
 
 
 def _close(a, b, tol):
@@ -32,7 +35,6 @@ def _close(a, b, tol):
 
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_noalloc_matches_transport(name):
-    torch = pytest.importorskip("torch")
     print(f"[noalloc] case={name}", flush=True)
     m = CASES[name]()
     ref = Problem(m)
@@ -45,12 +47,10 @@ def test_noalloc_matches_transport(name):
         dut.tally_clear()
         r = ref.transport(bank, kcol=0.9)
         cap = 4 * n + 4096
-        d_bank = _to_device(torch, bank)
-        d_out = torch.zeros(cap * REC_B, dtype=torch.uint8, device="cuda")
-        torch.cuda.synchronize()
-        n_out, sc, dg = dut.transport_noalloc(d_bank.data_ptr(), n, d_out.data_ptr(), cap,
-                                              kcol=0.9)
-        fis = d_out[:n_out * REC_B].cpu().numpy().view(abi.BANKED_DTYPE)
+        d_bank = _to_device(bank)
+        d_out = DeviceBuffer(cap * REC_B)
+        n_out, sc, dg = dut.transport_noalloc(d_bank.addr, n, d_out.addr, cap, kcol=0.9)
+        fis = d_out.download(n_out * REC_B, abi.BANKED_DTYPE)
         assert n_out == len(r["fission"]), (n, n_out, len(r["fission"]))
         assert fis.tobytes() == r["fission"].tobytes(), f"n={n}: fission banks differ"
         for f in ("histories", "collisions", "virtual_collisions", "flights", "crossings",
@@ -68,7 +68,6 @@ def test_noalloc_matches_transport(name):
 def test_noalloc_slices_equal_one_call():
     """A bank split into consecutive slices gives the concatenated fission
     bank and the tallies of one abx_transport call."""
-    torch = pytest.importorskip("torch")
     m = problems.pin_lattice_2d(2000, tracking="delta")
     ref = Problem(m)
     dut = Problem(m)
@@ -79,11 +78,10 @@ def test_noalloc_slices_equal_one_call():
     for a, b in ((0, 11111), (11111, n)):
         k = b - a
         cap = 4 * k + 4096
-        d_bank = _to_device(torch, bank[a:b])
-        d_out = torch.zeros(cap * REC_B, dtype=torch.uint8, device="cuda")
-        torch.cuda.synchronize()
-        n_out, _, _ = dut.transport_noalloc(d_bank.data_ptr(), k, d_out.data_ptr(), cap, kcol=0.9)
-        parts.append(d_out[:n_out * REC_B].cpu().numpy().view(abi.BANKED_DTYPE).copy())
+        d_bank = _to_device(bank[a:b])
+        d_out = DeviceBuffer(cap * REC_B)
+        n_out, _, _ = dut.transport_noalloc(d_bank.addr, k, d_out.addr, cap, kcol=0.9)
+        parts.append(d_out.download(n_out * REC_B, abi.BANKED_DTYPE).copy())
     assert np.concatenate(parts).tobytes() == r["fission"].tobytes()
     ta, tb = ref.tally(0), dut.tally(0)
     assert np.all(np.abs(ta - tb) <= 1e-9 * np.maximum(np.abs(ta), 1e-300))
