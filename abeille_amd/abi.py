@@ -34,7 +34,8 @@ OP_COMP = 2147483647 - 2
 OP_R_PAR = 2147483647 - 1
 OP_L_PAR = 2147483647
 FILL_MATERIAL, FILL_UNIVERSE = 0, 1
-CELL_UNIVERSE, RECT_LATTICE = 0, 1
+CELL_UNIVERSE, RECT_LATTICE, HEX_LATTICE = 0, 1, 2
+HEX_POINTY, HEX_FLAT = 0, 1
 SRC_POINT, SRC_BOX = 0, 1
 DIR_ISOTROPIC, DIR_MONO = 0, 1
 EST_COLLISION, EST_TRACK_LENGTH = 0, 1
@@ -65,7 +66,7 @@ class Cell(C.Structure):
 class Universe(C.Structure):
     _fields_ = [("id", C.c_uint32), ("type", C.c_int32), ("cells_offset", C.c_int32),
                 ("n_cells", C.c_int32), ("shape", C.c_int32 * 3),
-                ("elems_offset", C.c_int32), ("outer", C.c_int32), ("pad_", C.c_int32),
+                ("elems_offset", C.c_int32), ("outer", C.c_int32), ("hex_top", C.c_int32),
                 ("pitch", C.c_double * 3), ("origin", C.c_double * 3)]
 
 

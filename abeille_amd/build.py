@@ -1,6 +1,6 @@
 """Builds the gfx950 shared library libabx.so in-tree with hipcc.
 
-abx.hip (host side of the C ABI and the small kernels) and the eight parts of
+abx.hip (host side of the C ABI and the small kernels) and the twelve parts of
 instances.hip (the transport kernel variants) compile in parallel to objects
 under _build/, which are linked into abeille_amd/libabx.so.
 
@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libabx.so")
-N_PARTS = 8
+N_PARTS = 12
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
          "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]

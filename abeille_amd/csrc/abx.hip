@@ -356,7 +356,7 @@ TransportOut transport_device(abx_problem* p, const abx_particle* d_in, int64_t 
     if (p->engine == ABX_ENGINE_HISTORY) {
       if (p->timing) HIPC(hipEventRecord(p->ke0, p->stream));
       p->pool_next.reserve(sizeof(unsigned long long));
-      HIPC(history_transport(p->P, K, p->stream, sec_cap > 0,
+      HIPC(history_transport(p->P, p->T.has_hex, K, p->stream, sec_cap > 0,
                              p->pool_next.as<unsigned long long>(), p->tuning));
       if (p->timing) {
         HIPC(hipEventRecord(p->ke1, p->stream));
@@ -909,15 +909,16 @@ static bool run_generations_graph(abx_problem* p, int ngenerations, int nignored
   bool stage = false;
   int64_t grid = 0;
   // (carter tracking never gets here: it carries the secondary stack)
-  HIPC(dispatch_variant(trk, false, false, split, [&](auto TT, auto, auto SE, auto SP) {
+  const bool hex = p->T.has_hex;
+  HIPC(dispatch_variant(trk, false, false, split, hex, [&](auto TT, auto, auto SE, auto SP, auto HX) {
     constexpr int TK = decltype(TT)::value;
-    constexpr bool SEC = decltype(SE)::value, SPL = decltype(SP)::value;
-    stage = tu.stage_tables && stage_pays(k_history<TK, false, SEC, SPL, true>,
-                                          k_history<TK, false, SEC, SPL, false>, bs, cur,
-                                          (size_t)p->P.blob_bytes);
-    grid = stage ? resident_blocks(k_history<TK, false, SEC, SPL, true>, bs,
+    constexpr bool SEC = decltype(SE)::value, SPL = decltype(SP)::value, HEX = decltype(HX)::value;
+    stage = tu.stage_tables && stage_pays(k_history<TK, false, SEC, SPL, true, false, HEX>,
+                                          k_history<TK, false, SEC, SPL, false, false, HEX>, bs,
+                                          cur, (size_t)p->P.blob_bytes);
+    grid = stage ? resident_blocks(k_history<TK, false, SEC, SPL, true, false, HEX>, bs,
                                    cur + (size_t)p->P.blob_bytes)
-                 : resident_blocks(k_history<TK, false, SEC, SPL, false>, bs, cur);
+                 : resident_blocks(k_history<TK, false, SEC, SPL, false, false, HEX>, bs, cur);
     return hipSuccess;
   }));
   const size_t lds = stage ? cur + (size_t)p->P.blob_bytes : cur;
@@ -940,14 +941,16 @@ static bool run_generations_graph(abx_problem* p, int ngenerations, int nignored
     Kc.converged = conv;
     HIPC(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(k_gen_begin, dim3(1), dim3(64), 0, s, g);
-    HIPC(dispatch_variant(trk, false, false, split, [&](auto TT, auto, auto SE, auto SP) {
+    HIPC(dispatch_variant(trk, false, false, split, hex,
+                          [&](auto TT, auto, auto SE, auto SP, auto HX) {
       constexpr int TK = decltype(TT)::value;
       constexpr bool SEC = decltype(SE)::value, SPL = decltype(SP)::value;
+      constexpr bool HEX = decltype(HX)::value;
       if (stage)
-        hipLaunchKernelGGL((k_history<TK, false, SEC, SPL, true>),
+        hipLaunchKernelGGL((k_history<TK, false, SEC, SPL, true, false, HEX>),
                            dim3((unsigned)grid), dim3(bs), lds, s, p->P, Kc, &g->next);
       else
-        hipLaunchKernelGGL((k_history<TK, false, SEC, SPL, false>),
+        hipLaunchKernelGGL((k_history<TK, false, SEC, SPL, false, false, HEX>),
                            dim3((unsigned)grid), dim3(bs), lds, s, p->P, Kc, &g->next);
       return hipSuccess;
     }));
@@ -1083,11 +1086,19 @@ static bool run_generations_graph(abx_problem* p, int ngenerations, int nignored
 // The replica-mode kernel variants: surface or delta tracking, FP64 or
 // split precision (no trace, no secondary stack).
 extern "C++" {
-template <class Fn>
-static hipError_t rep_dispatch(int trk, bool split, Fn&& fn) {
-  if (trk == 0) return split ? fn(IntC<0>{}, BoolC<true>{}) : fn(IntC<0>{}, BoolC<false>{});
-  if (trk == 1) return split ? fn(IntC<1>{}, BoolC<true>{}) : fn(IntC<1>{}, BoolC<false>{});
+template <bool HEX, class Fn>
+static hipError_t rep_dispatch_t(int trk, bool split, Fn&& fn) {
+  const BoolC<HEX> hx{};
+  if (trk == 0)
+    return split ? fn(IntC<0>{}, BoolC<true>{}, hx) : fn(IntC<0>{}, BoolC<false>{}, hx);
+  if (trk == 1)
+    return split ? fn(IntC<1>{}, BoolC<true>{}, hx) : fn(IntC<1>{}, BoolC<false>{}, hx);
   return hipErrorInvalidValue;
+}
+// fn(TRK, SPLIT, HEX) over the replica variants
+template <class Fn>
+static hipError_t rep_dispatch(int trk, bool split, bool hex, Fn&& fn) {
+  return hex ? rep_dispatch_t<true>(trk, split, fn) : rep_dispatch_t<false>(trk, split, fn);
 }
 }
 
@@ -1199,15 +1210,16 @@ static void run_replicas(abx_problem* p, int R, const uint64_t* seeds, bool from
   const bool split = T.st.precision == ABX_SPLIT;
   bool stage = false;
   int64_t grid = 0;
-  HIPC(rep_dispatch(trk, split, [&](auto TT, auto SP) {
+  const bool hex = p->T.has_hex;
+  HIPC(rep_dispatch(trk, split, hex, [&](auto TT, auto SP, auto HX) {
     constexpr int TK = decltype(TT)::value;
-    constexpr bool SPL = decltype(SP)::value;
-    stage = tu.stage_tables && stage_pays(k_history<TK, false, false, SPL, true, true>,
-                                          k_history<TK, false, false, SPL, false, true>, bs, cur,
-                                          (size_t)p->P.blob_bytes);
-    grid = stage ? resident_blocks(k_history<TK, false, false, SPL, true, true>, bs,
+    constexpr bool SPL = decltype(SP)::value, HEX = decltype(HX)::value;
+    stage = tu.stage_tables && stage_pays(k_history<TK, false, false, SPL, true, true, HEX>,
+                                          k_history<TK, false, false, SPL, false, true, HEX>, bs,
+                                          cur, (size_t)p->P.blob_bytes);
+    grid = stage ? resident_blocks(k_history<TK, false, false, SPL, true, true, HEX>, bs,
                                    cur + (size_t)p->P.blob_bytes)
-                 : resident_blocks(k_history<TK, false, false, SPL, false, true>, bs, cur);
+                 : resident_blocks(k_history<TK, false, false, SPL, false, true, HEX>, bs, cur);
     return hipSuccess;
   }));
   const size_t lds = stage ? cur + (size_t)p->P.blob_bytes : cur;
@@ -1231,14 +1243,14 @@ static void run_replicas(abx_problem* p, int R, const uint64_t* seeds, bool from
     Kc.converged = conv;
     HIPC(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(k_rep_begin, dim3(1), dim3(256), 0, s, g);
-    HIPC(rep_dispatch(trk, split, [&](auto TT, auto SP) {
+    HIPC(rep_dispatch(trk, split, hex, [&](auto TT, auto SP, auto HX) {
       constexpr int TK = decltype(TT)::value;
-      constexpr bool SPL = decltype(SP)::value;
+      constexpr bool SPL = decltype(SP)::value, HEX = decltype(HX)::value;
       if (stage)
-        hipLaunchKernelGGL((k_history<TK, false, false, SPL, true, true>),
+        hipLaunchKernelGGL((k_history<TK, false, false, SPL, true, true, HEX>),
                            dim3((unsigned)grid), dim3(bs), lds, s, p->P, Kc, &g->next);
       else
-        hipLaunchKernelGGL((k_history<TK, false, false, SPL, false, true>),
+        hipLaunchKernelGGL((k_history<TK, false, false, SPL, false, true, HEX>),
                            dim3((unsigned)grid), dim3(bs), lds, s, p->P, Kc, &g->next);
       return hipSuccess;
     }));
@@ -1703,26 +1715,29 @@ bool batch_stage_fits(const abx_problem* p) {
   const bool sec = needs_secondaries(T), split = T.st.precision == ABX_SPLIT;
   const size_t cur = cursor_lds_bytes(256, p->P.trk_D, p->P.trk_L);
   bool ok = false;
-  (void)dispatch_variant(trk, false, sec || trk == 2, split, [&](auto TT, auto, auto SE, auto SP) {
+  (void)dispatch_variant(trk, false, sec || trk == 2, split, T.has_hex,
+                         [&](auto TT, auto, auto SE, auto SP, auto HX) {
     constexpr int TK = decltype(TT)::value;
     constexpr bool SEC = decltype(SE)::value, SPL = decltype(SP)::value;
-    ok = stage_pays(k_batch_chunks<TK, SEC, SPL, true>, k_batch_chunks<TK, SEC, SPL, false>, 256,
-                    cur, (size_t)p->P.blob_bytes);
+    constexpr bool HEX = decltype(HX)::value;
+    ok = stage_pays(k_batch_chunks<TK, SEC, SPL, true, HEX>,
+                    k_batch_chunks<TK, SEC, SPL, false, HEX>, 256, cur, (size_t)p->P.blob_bytes);
     return hipSuccess;
   });
   return ok;
 }
 
 // Launch group of a problem in the batch kernel: tracking mode, secondaries,
-// precision, and whether its tables are worth staging in LDS (problems whose
-// tables are launched apart from those whose tables are not, so that the
-// former run the staged variant).
+// precision, hexagonal lattices, and whether its tables are worth staging in
+// LDS (problems whose tables are launched apart from those whose tables are
+// not, so that the former run the staged variant).
 int variant_key(abx_problem* p) {
   const HostTables& T = p->T;
   const int sec = needs_secondaries(T) ? 1 : 0;
   const int split = T.st.precision == ABX_SPLIT ? 1 : 0;
+  const int hex = T.has_hex ? 1 : 0;
   if (p->batch_stage < 0) p->batch_stage = batch_stage_fits(p) ? 1 : 0;
-  return ((T.st.tracking * 2 + sec) * 2 + split) * 2 + p->batch_stage;
+  return (((T.st.tracking * 2 + sec) * 2 + split) * 2 + hex) * 2 + p->batch_stage;
 }
 
 // Chunks of `chunk` histories over the problems of one launch (item indices
@@ -1745,7 +1760,7 @@ int64_t chunk_length(int64_t total, int per_cu = 8) {
   return std::max<int64_t>(256, std::min<int64_t>(8192, total / std::max(1, per_cu * cus)));
 }
 
-template <int TRK, bool SEC, bool SPLIT, bool STAGE>
+template <int TRK, bool SEC, bool SPLIT, bool STAGE, bool HEX>
 hipError_t launch_chunks(const DevProblem* probs, const KCtx* ctxs, const BatchChunk* chunks,
                          int n_chunks, int D, int L, size_t blob_max, unsigned* next,
                          hipStream_t s) {
@@ -1754,9 +1769,9 @@ hipError_t launch_chunks(const DevProblem* probs, const KCtx* ctxs, const BatchC
   hipError_t e;
   if ((e = hipMemsetAsync(next, 0, sizeof(unsigned), s)) != hipSuccess) return e;
   const int64_t blocks = std::min<int64_t>(
-      n_chunks, resident_blocks(k_batch_chunks<TRK, SEC, SPLIT, STAGE>, bs, lds));
-  hipLaunchKernelGGL((k_batch_chunks<TRK, SEC, SPLIT, STAGE>), dim3((unsigned)blocks), dim3(bs),
-                     lds, s, probs, ctxs, chunks, n_chunks, D, L, next);
+      n_chunks, resident_blocks(k_batch_chunks<TRK, SEC, SPLIT, STAGE, HEX>, bs, lds));
+  hipLaunchKernelGGL((k_batch_chunks<TRK, SEC, SPLIT, STAGE, HEX>), dim3((unsigned)blocks),
+                     dim3(bs), lds, s, probs, ctxs, chunks, n_chunks, D, L, next);
   return hipGetLastError();
 }
 
@@ -1765,18 +1780,20 @@ hipError_t launch_chunks(const DevProblem* probs, const KCtx* ctxs, const BatchC
 hipError_t launch_chunks_variant(int key, const DevProblem* probs, const KCtx* ctxs,
                                  const BatchChunk* chunks, int n_chunks, int D, int L,
                                  size_t blob_max, unsigned* next, hipStream_t s) {
-  const int trk = key / 8;
-  const bool sec = (key / 4) & 1, split = (key / 2) & 1, fits = key & 1;
+  const int trk = key / 16;
+  const bool sec = (key / 8) & 1, split = (key / 4) & 1, hex = (key / 2) & 1, fits = key & 1;
   const size_t cur = cursor_lds_bytes(256, D, L);
-  return dispatch_variant(trk, false, sec || trk == 2, split, [&](auto T, auto, auto SE, auto SP) {
+  return dispatch_variant(trk, false, sec || trk == 2, split, hex,
+                          [&](auto T, auto, auto SE, auto SP, auto HX) {
     constexpr int TK = decltype(T)::value;
     constexpr bool SEC = decltype(SE)::value, SPL = decltype(SP)::value;
-    if (fits && stage_pays(k_batch_chunks<TK, SEC, SPL, true>, k_batch_chunks<TK, SEC, SPL, false>,
-                           256, cur, blob_max))
-      return launch_chunks<TK, SEC, SPL, true>(probs, ctxs, chunks, n_chunks, D, L, blob_max,
-                                               next, s);
-    return launch_chunks<TK, SEC, SPL, false>(probs, ctxs, chunks, n_chunks, D, L, blob_max, next,
-                                              s);
+    constexpr bool HEX = decltype(HX)::value;
+    if (fits && stage_pays(k_batch_chunks<TK, SEC, SPL, true, HEX>,
+                           k_batch_chunks<TK, SEC, SPL, false, HEX>, 256, cur, blob_max))
+      return launch_chunks<TK, SEC, SPL, true, HEX>(probs, ctxs, chunks, n_chunks, D, L, blob_max,
+                                                    next, s);
+    return launch_chunks<TK, SEC, SPL, false, HEX>(probs, ctxs, chunks, n_chunks, D, L, blob_max,
+                                                   next, s);
   });
 }
 
@@ -2379,7 +2396,7 @@ extern "C" int abx_transport_noalloc(abx_problem* p, const abx_particle* d_bank,
     K.counters = d_counters;
     K.err = d_err;
     K.err_hist = d_err_hist;
-    if (nn > 0) HIPC(history_transport(p->P, K, p->stream, sec_cap > 0, d_next, p->tuning));
+    if (nn > 0) HIPC(history_transport(p->P, p->T.has_hex, K, p->stream, sec_cap > 0, d_next, p->tuning));
     double sc[8];
     unsigned long long cn[8], nst[2] = {0, 0};
     int32_t err = 0;

@@ -74,6 +74,11 @@ struct DCell {
   uint32_t id;
 };
 
+// Universe: cell list, or lattice.  A rectilinear lattice keeps its lower
+// corner in Xl/Yl/Zl; a hexagonal one keeps its pitch in Px, axial pitch in
+// Pz, origin in Xl/Yl/Zl, tiles in an Nx x Ny (= width x width) x Nz array
+// of shifted axial coordinates, and the trigonometric constants of
+// include/geometry/hex_lattice.hpp:60-63 as the host computed them.
 struct DUniv {
   int32_t type;
   int32_t cells_off, ncells;
@@ -81,8 +86,11 @@ struct DUniv {
   int32_t Nx, Ny, Nz;
   int32_t elems_off;
   int32_t outer;
-  int32_t pad_;
+  int32_t top;    // hexagonal: abx_hex_top
   double Px, Py, Pz, Pxi, Pyi, Pzi, Xl, Yl, Zl;
+  double z_low, c6, s6, c3, s3;  // hexagonal
+  int32_t rings;  // hexagonal: rings; mid = rings - 1
+  int32_t pad_;
 };
 
 struct DTally {

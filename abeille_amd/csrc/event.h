@@ -42,7 +42,7 @@ struct EvState {
   int32_t D, L;
 };
 
-__device__ __forceinline__ void ev_load(const EvState& S, int64_t i, Part& p, Trk& t, int& m) {
+__device__ __forceinline__ void ev_load(const EvState& S, int64_t i, Part& p, TrkT<true>& t, int& m) {
   p.r = V3{S.rx[i], S.ry[i], S.rz[i]};
   p.u = V3{S.ux[i], S.uy[i], S.uz[i]};
   p.rb = V3{S.bx[i], S.by[i], S.bz[i]};
@@ -80,7 +80,7 @@ __device__ __forceinline__ void ev_load(const EvState& S, int64_t i, Part& p, Tr
     t.set_lv(l, V3{S.lv[(3 * l) * n + i], S.lv[(3 * l + 1) * n + i], S.lv[(3 * l + 2) * n + i]});
 }
 
-__device__ __forceinline__ void ev_store(const EvState& S, int64_t i, const Part& p, const Trk& t,
+__device__ __forceinline__ void ev_store(const EvState& S, int64_t i, const Part& p, const TrkT<true>& t,
                                          int m) {
   S.rx[i] = p.r.x;
   S.ry[i] = p.r.y;
@@ -169,7 +169,7 @@ __global__ void __launch_bounds__(256) k_ev_init(DevProblem P, KCtx K, EvState S
   if (i < K.n) {
     Part p;
     load_part(P, K.in[i], p);
-    Trk t;
+    TrkT<true> t;
     bind_lds_cursor(t, P);
     int m = -1;
     live = start_history(P, s, p, t, m);
@@ -205,7 +205,7 @@ __global__ void ABX_TKERNEL k_ev_flight(DevProblem P, KCtx K, EvState S,
     slot = qin[k];
     ev_resume(s, S, slot);
     Part p;
-    Trk t;
+    TrkT<true> t;
     bind_lds_cursor(t, P);
     int m;
     ev_load(S, slot, p, t, m);
@@ -242,7 +242,7 @@ __global__ void ABX_TKERNEL k_ev_collide(DevProblem P, KCtx K, EvState S,
     slot = qin[k];
     ev_resume(s, S, slot);
     Part p;
-    Trk t;
+    TrkT<true> t;
     bind_lds_cursor(t, P);
     int m;
     ev_load(S, slot, p, t, m);
@@ -276,7 +276,7 @@ __global__ void ABX_TKERNEL k_ev_tail(DevProblem P, KCtx K, EvState S,
     const int32_t slot = pending ? qc[k - nf] : qf[k];
     ev_resume(s, S, slot);
     Part p;
-    Trk t;
+    TrkT<true> t;
     bind_lds_cursor(t, P);
     int m;
     ev_load(S, slot, p, t, m);
@@ -561,22 +561,22 @@ static bool stage_pays(KS staged, KU unstaged, int bs, size_t cur, size_t blob) 
 }
 
 // `next` is a device counter (zeroed here) from which lanes take histories.
-template <int TRK, bool TRACE, bool SEC, bool SPLIT>
+template <int TRK, bool TRACE, bool SEC, bool SPLIT, bool HEX>
 static hipError_t history_transport_t(const DevProblem& P, const KCtx& K, hipStream_t s,
                                       unsigned long long* next, const abx_tuning& tu) {
   const int bs = tu.block_size;
   // Stage the tables in LDS when they fit next to the cursors.
   const size_t cur = cursor_lds_bytes(bs, P.trk_D, P.trk_L);
   const bool stage = tu.stage_tables &&
-                     stage_pays(k_history<TRK, TRACE, SEC, SPLIT, true>,
-                                k_history<TRK, TRACE, SEC, SPLIT, false>, bs, cur,
+                     stage_pays(k_history<TRK, TRACE, SEC, SPLIT, true, false, HEX>,
+                                k_history<TRK, TRACE, SEC, SPLIT, false, false, HEX>, bs, cur,
                                 (size_t)P.blob_bytes);
   const size_t lds = stage ? cur + (size_t)P.blob_bytes : cur;
   hipError_t e;
   if ((e = hipMemsetAsync(next, 0, sizeof(unsigned long long), s)) != hipSuccess) return e;
   const int64_t want = (K.n + bs - 1) / bs;
-  int64_t grid = stage ? resident_blocks(k_history<TRK, TRACE, SEC, SPLIT, true>, bs, lds)
-                       : resident_blocks(k_history<TRK, TRACE, SEC, SPLIT, false>, bs, lds);
+  int64_t grid = stage ? resident_blocks(k_history<TRK, TRACE, SEC, SPLIT, true, false, HEX>, bs, lds)
+                       : resident_blocks(k_history<TRK, TRACE, SEC, SPLIT, false, false, HEX>, bs, lds);
   if (tu.blocks_per_cu > 0) {
     int dev = 0, cus = 0;
     if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
@@ -586,10 +586,10 @@ static hipError_t history_transport_t(const DevProblem& P, const KCtx& K, hipStr
   }
   const int64_t blocks = std::min<int64_t>(want, grid);
   if (stage)
-    hipLaunchKernelGGL((k_history<TRK, TRACE, SEC, SPLIT, true>), dim3((unsigned)blocks), dim3(bs),
+    hipLaunchKernelGGL((k_history<TRK, TRACE, SEC, SPLIT, true, false, HEX>), dim3((unsigned)blocks), dim3(bs),
                        lds, s, P, K, next);
   else
-    hipLaunchKernelGGL((k_history<TRK, TRACE, SEC, SPLIT, false>), dim3((unsigned)blocks), dim3(bs),
+    hipLaunchKernelGGL((k_history<TRK, TRACE, SEC, SPLIT, false, false, HEX>), dim3((unsigned)blocks), dim3(bs),
                        lds, s, P, K, next);
   return hipGetLastError();
 }
@@ -603,7 +603,7 @@ template <bool V>
 using BoolC = std::integral_constant<bool, V>;
 
 template <class Fn>
-static hipError_t dispatch_variant(int trk, bool trace, bool sec, bool split, Fn&& fn) {
+static hipError_t dispatch_variant_t(int trk, bool trace, bool sec, bool split, Fn&& fn) {
   if (split) {
     if (trace) return hipErrorInvalidValue;
     if (trk == 0)
@@ -631,10 +631,23 @@ static hipError_t dispatch_variant(int trk, bool trace, bool sec, bool split, Fn
              : fn(IntC<1>{}, BoolC<false>{}, BoolC<false>{}, BoolC<false>{});
 }
 
+// ... x hexagonal lattices: fn(T, TR, SE, SP, HX).
+template <class Fn>
+static hipError_t dispatch_variant(int trk, bool trace, bool sec, bool split, bool hex, Fn&& fn) {
+  if (hex)
+    return dispatch_variant_t(trk, trace, sec, split, [&](auto T, auto TR, auto SE, auto SP) {
+      return fn(T, TR, SE, SP, BoolC<true>{});
+    });
+  return dispatch_variant_t(trk, trace, sec, split, [&](auto T, auto TR, auto SE, auto SP) {
+    return fn(T, TR, SE, SP, BoolC<false>{});
+  });
+}
+
 static inline hipError_t event_transport(const DevProblem& P, const KCtx& K, EventBuffers& B,
                                          hipStream_t s, bool sec, bool timing, double* kernel_ms,
                                          int64_t* launches) {
-  return dispatch_variant(P.tracking, K.trace != nullptr, sec, P.precision == ABX_SPLIT,
+  // the event kernels always carry the hexagonal-lattice navigation
+  return dispatch_variant_t(P.tracking, K.trace != nullptr, sec, P.precision == ABX_SPLIT,
                           [&](auto T, auto TR, auto SE, auto SP) {
                             return event_transport_t<decltype(T)::value, decltype(TR)::value,
                                                      decltype(SE)::value, decltype(SP)::value>(
@@ -642,14 +655,15 @@ static inline hipError_t event_transport(const DevProblem& P, const KCtx& K, Eve
                           });
 }
 
-static inline hipError_t history_transport(const DevProblem& P, const KCtx& K, hipStream_t s,
-                                           bool sec, unsigned long long* next,
+// hex: the problem has hexagonal lattices (its tables' has_hex)
+static inline hipError_t history_transport(const DevProblem& P, bool hex, const KCtx& K,
+                                           hipStream_t s, bool sec, unsigned long long* next,
                                            const abx_tuning& tu) {
-  return dispatch_variant(P.tracking, K.trace != nullptr, sec, P.precision == ABX_SPLIT,
-                          [&](auto T, auto TR, auto SE, auto SP) {
+  return dispatch_variant(P.tracking, K.trace != nullptr, sec, P.precision == ABX_SPLIT, hex,
+                          [&](auto T, auto TR, auto SE, auto SP, auto HX) {
                             return history_transport_t<decltype(T)::value, decltype(TR)::value,
-                                                       decltype(SE)::value, decltype(SP)::value>(
-                                P, K, s, next, tu);
+                                                       decltype(SE)::value, decltype(SP)::value,
+                                                       decltype(HX)::value>(P, K, s, next, tu);
                           });
 }
 

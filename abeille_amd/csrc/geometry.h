@@ -6,8 +6,9 @@
 //    src/sphere.cpp:33-80 - same operand order, same 1e-12 tolerances;
 //  * Cell::is_inside / distance_to_boundary(_condition): src/cell.cpp:71-193;
 //  * RectLattice::get_tile / distance_to_tile_boundary / tile_center:
-//    src/rect_lattice.cpp:209-310;
-//  * CellUniverse / RectLattice ::get_cell with the lily-pad stack and
+//    src/rect_lattice.cpp:209-310; HexLattice's: src/hex_lattice.cpp:245-452;
+//  * CellUniverse / RectLattice / HexLattice ::get_cell with the lily-pad
+//    stack (src/hex_lattice.cpp:143-217) and
 //    Tracker::{move,get_current,get_boundary_condition,get_nearest_boundary}:
 //    include/simulation/tracker.hpp:41-372, src/cell_universe.cpp:72-150,
 //    src/lattice.cpp:77-91.
@@ -21,6 +22,7 @@
 
 #include <math.h>
 
+#include "../../include/abx.h"
 #include "devtypes.h"
 
 struct V3 {
@@ -293,6 +295,158 @@ ABX_HD double tile_distance(const DUniv& L, const V3& rl, const V3& u, const int
   return dist;
 }
 
+// ------------------------------------------------------- hexagonal lattices
+// HexLattice (src/hex_lattice.cpp), expression for expression (the 0.* and
+// 1.* factors included) so that tiles and distances carry the host's bits.
+
+// HexLattice::get_nearest_hex + get_tile (src/hex_lattice.cpp:260-308): the
+// axial (q, r) of the nearest centre by cube rounding, the axial layer from
+// z_low; the direction plays no part.  With `elems`, also the tile's element
+// or -1 outside the rings / layers (get_ring, linear_index:
+// src/hex_lattice.cpp:245-258, 333-345).
+struct HexTile {
+  int32_t q, r, z, elem;
+};
+ABX_HD HexTile hex_tile(const DUniv& L, const int32_t* elems, double px, double py, double pz) {
+  const double pt = L.Px;
+  double q, r, det;
+  if (L.top == ABX_HEX_FLAT) {
+    det = -pt * pt * L.s3;
+    q = (pt / det) * (0. * px - 1. * py);
+    r = (pt / det) * (-L.s3 * px + L.c3 * py);
+  } else {
+    det = -pt * pt * L.c6;
+    q = (pt / det) * (L.s6 * px - L.c6 * py);
+    r = (pt / det) * (-1. * px - 0. * py);
+  }
+  const double cx = q, cz = r, cy = -cx - cz;
+  double rx = round(cx), ry = round(cy), rz = round(cz);
+  const double dx = fabs(rx - cx), dy = fabs(ry - cy), dz = fabs(rz - cz);
+  if (dx > dy && dx > dz)
+    rx = -ry - rz;
+  else if (dy > dx && dy > dz)
+    ry = -rx - rz;
+  else
+    rz = -rx - ry;
+  HexTile t;
+  t.q = (int32_t)rx;
+  t.r = (int32_t)rz;
+  t.z = (int32_t)floor((pz - L.z_low) / L.Pz);
+  t.elem = -1;
+  if (elems) {
+    const int32_t y = -t.q - t.r;
+    const uint32_t aq = (uint32_t)abs(t.q), ay = (uint32_t)abs(y), ar = (uint32_t)abs(t.r);
+    const uint32_t ring = (aq > ay ? aq : ay) > ar ? (aq > ay ? aq : ay) : ar;
+    if (ring < (uint32_t)L.rings && t.z >= 0 && t.z < L.Nz) {
+      const int mid = L.rings - 1;
+      t.elem = elems[L.elems_off + (t.z * L.Ny + (t.r + mid)) * L.Nx + (t.q + mid)];
+    }
+  }
+  return t;
+}
+
+// HexLattice::tile_center / get_hex_center (src/hex_lattice.cpp:310-331)
+ABX_HD V3 hex_tile_center(const DUniv& L, int q, int r, int nz) {
+  const double dq = (double)q, dr = (double)r;
+  V3 c;
+  if (L.top == ABX_HEX_FLAT) {
+    c.x = L.Px * (L.c3 * dq + 1. * dr);
+    c.y = L.Px * (L.s3 * dq + 0. * dr);
+  } else {
+    c.x = L.Px * (0. * dq + L.c6 * dr);
+    c.y = L.Px * (1. * dq + L.s6 * dr);
+  }
+  c.z = ((double)nz + 0.5) * L.Pz + L.z_low;
+  return c;
+}
+
+// HexLattice::distance_to_line (src/hex_lattice.cpp:435-452)
+ABX_HD double hex_line(const V3& r, const V3& u, double x1, double y1, double x2, double y2) {
+  const double A = y2 - y1;
+  const double B = x1 - x2;
+  const double D = (x2 - x1) * y1 - (y2 - y1) * x1;
+  const double num = D - A * r.x - B * r.y;
+  const double denom = A * u.x + B * u.y;
+  const double d = num / denom;
+  if (d < 0.) return HUGE_VAL;
+  if (fabs(num) <= 100 * ABX_SURFACE_COINCIDENT * sqrt(A * A + B * B)) return HUGE_VAL;
+  return d;
+}
+
+// HexLattice::distance_to_tile_boundary(_pointy / _flat)
+// (src/hex_lattice.cpp:347-433): the reference's six lines (which are not
+// the edges of the hexagon get_tile assigns) and two axial planes, in the
+// frame of the tile centre taken from r_local as given.  Lines and planes
+// within 100 * SURFACE_COINCIDENT of the point are skipped, RectLattice's
+// tolerance (src/rect_lattice.cpp:268-281): the reference takes any
+// positive distance, and since its lines are not tile edges a particle
+// stopped on one can find it again ~1e-16 cm ahead and cross it for ever
+// (src/surface_tracker.cpp:72-150).
+ABX_HD double hex_tile_distance(const DUniv& L, double rx, double ry, double rz, double ux,
+                                  double uy, double uz, int q, int r, int nz) {
+  const V3 u{ux, uy, uz};
+  const V3 rt = vsub(V3{rx, ry, rz}, hex_tile_center(L, q, r, nz));
+  const double pt = L.Px;
+  const bool pointy = L.top == ABX_HEX_POINTY;
+  // the six lines as (x1, y1) -> (x2, y2) in the reference's order, from the
+  // orientation's two points (a, b), (c, e)
+  const double a = pointy ? 0. : pt / (2. * L.s6);
+  const double b = pointy ? pt / (2. * L.c6) : pt / 2.;
+  const double c = pointy ? pt / 2. : pt / (2. * L.c6);
+  const double e = pointy ? pt / (2. * L.s6) : 0.;
+  double d = ABX_INF;
+  for (int k = 0; k < 6; k++) {
+    double x1, y1, x2, y2;
+    if (pointy) {
+      switch (k) {
+        case 0: x1 = a; y1 = b; x2 = c; y2 = b; break;
+        case 1: x1 = c; y1 = e; x2 = c; y2 = -e; break;
+        case 2: x1 = c; y1 = -e; x2 = a; y2 = -b; break;
+        case 3: x1 = a; y1 = -b; x2 = -c; y2 = -e; break;
+        case 4: x1 = -c; y1 = -e; x2 = -c; y2 = e; break;
+        default: x1 = -c; y1 = e; x2 = a; y2 = b; break;
+      }
+    } else {
+      switch (k) {
+        case 0: x1 = a; y1 = b; x2 = c; y2 = b; break;
+        case 1: x1 = c; y1 = e; x2 = a; y2 = -b; break;
+        case 2: x1 = a; y1 = -b; x2 = -a; y2 = -b; break;
+        case 3: x1 = -a; y1 = -b; x2 = -c; y2 = -e; break;
+        case 4: x1 = -c; y1 = -e; x2 = -a; y2 = b; break;
+        default: x1 = -a; y1 = b; x2 = a; y2 = b; break;
+      }
+    }
+    const double dk = hex_line(rt, u, x1, y1, x2, y2);
+    if (dk > 0. && dk < d) d = dk;
+  }
+  const double tol = 100 * ABX_SURFACE_COINCIDENT;
+  const double dzl = (-L.Pz * 0.5 - rt.z) / u.z;
+  const double dzu = (L.Pz * 0.5 - rt.z) / u.z;
+  if (dzl > 0. && dzl < d && fabs(-L.Pz * 0.5 - rt.z) > tol) d = dzl;
+  if (dzu > 0. && dzu < d && fabs(L.Pz * 0.5 - rt.z) > tol) d = dzu;
+  return d;
+}
+
+// Lattice::get_tile / distance_to_tile_boundary of either kind, from a
+// lattice pad's r_local (the frame the lattice was entered in).
+template <bool HEX>
+ABX_HD void lat_pad_tile(const DUniv& L, const V3& r, const V3& u, int t[3]) {
+  if (HEX && L.type == ABX_HEX_LATTICE) {
+    const HexTile h = hex_tile(L, nullptr, r.x, r.y, r.z);
+    t[0] = h.q;
+    t[1] = h.r;
+    t[2] = h.z;
+  } else {
+    get_tile(L, r, u, t);
+  }
+}
+template <bool HEX>
+ABX_HD double lat_pad_distance(const DUniv& L, const V3& rl, const V3& u, const int32_t t[3]) {
+  if (HEX && L.type == ABX_HEX_LATTICE)
+    return hex_tile_distance(L, rl.x, rl.y, rl.z, u.x, u.y, u.z, t[0], t[1], t[2]);
+  return tile_distance(L, rl, u, t);
+}
+
 // ------------------------------------------------------------------ cursor
 enum { PAD_U = 0, PAD_L = 1, PAD_C = 2 };
 
@@ -310,7 +464,12 @@ ABX_HD bool tile_eq(const int32_t a[3], const int b[3]) {
 // The cursor proper.  Pads and frames live outside the struct (per-thread
 // slices of LDS on the GPU, a small local buffer on the host) at a stride,
 // so that runtime nesting depth costs neither registers nor scratch.
-struct Trk {
+// HEX: whether the navigation built on this cursor handles hexagonal
+// lattices; kernels for problems without them are built without that code
+// (the host launches the HEX variants for every problem that has one).
+template <bool HEX>
+struct TrkT {
+  static constexpr bool kHex = HEX;
   int32_t np;     // pads in use
   int32_t cell;   // material cell index, -1 = lost
   int32_t token;  // surface token the particle sits on
@@ -340,7 +499,8 @@ struct Trk {
 ABX_HD int trk_bytes(int D, int L) { return 16 * D + 24 * L; }
 
 // Point a cursor at a block of `stride` lanes' storage (lane `lane`).
-ABX_HD void trk_bind(Trk& t, void* base, int stride, int lane, int D, int L) {
+template <class T>
+ABX_HD void trk_bind(T& t, void* base, int stride, int lane, int D, int L) {
   int32_t* b = (int32_t*)base;
   t.D = D;
   t.L = L;
@@ -355,7 +515,8 @@ ABX_HD bool tile_in_range(const DUniv& L, const int t[3]) {
            t[2] >= L.Nz);
 }
 
-ABX_HD bool trk_push(Trk& t, int32_t w, const int* tile) {
+template <class T>
+ABX_HD bool trk_push(T& t, int32_t w, const int* tile) {
   if (t.np >= t.D) {
     t.err = -3;
     return false;
@@ -371,10 +532,11 @@ ABX_HD bool trk_push(Trk& t, int32_t w, const int* tile) {
 }
 
 // Universe::get_cell with the stack, iteratively; r is frame `lvl`.
-ABX_NOINL int trk_descend(const DevProblem& P, Trk& t, int ui, int lvl, V3 r, const V3& u) {
+template <class T>
+ABX_NOINL int trk_descend(const DevProblem& P, T& t, int ui, int lvl, V3 r, const V3& u) {
   for (int guard = 0; guard < 4 * ABX_MAXPADS; guard++) {
     const DUniv& U = P.univ[ui];
-    if (U.type == 0) {
+    if (U.type == ABX_CELL_UNIVERSE) {
       if (!trk_push(t, pad_word(PAD_U, lvl, ui), nullptr)) return -1;
       int found = -1;
       for (int k = 0; k < U.ncells; k++) {
@@ -392,17 +554,34 @@ ABX_NOINL int trk_descend(const DevProblem& P, Trk& t, int ui, int lvl, V3 r, co
       continue;
     }
     int tl[3];
-    get_tile(U, r, u, tl);
     int elem = -1;
-    if (tile_in_range(U, tl))
-      elem = P.elems[U.elems_off + tl[2] * (U.Nx * U.Ny) + tl[0] * U.Ny + tl[1]];
+    bool hex = false;
+    if (T::kHex && U.type == ABX_HEX_LATTICE) {
+      // HexLattice::get_cell (src/hex_lattice.cpp:143-217): the tile of the
+      // origin-relative point; the pad and the outer universe keep r
+      const HexTile h = hex_tile(U, P.elems, r.x - U.Xl, r.y - U.Yl, r.z - U.Zl);
+      tl[0] = h.q;
+      tl[1] = h.r;
+      tl[2] = h.z;
+      elem = h.elem;
+      hex = true;
+    } else {
+      get_tile(U, r, u, tl);
+      if (tile_in_range(U, tl))
+        elem = P.elems[U.elems_off + tl[2] * (U.Nx * U.Ny) + tl[0] * U.Ny + tl[1]];
+    }
     if (!trk_push(t, pad_word(PAD_L, lvl, ui), tl)) return -1;
     if (elem >= 0) {
       if (lvl + 1 >= t.L) {
         t.err = -3;
         return -1;
       }
-      r = vsub(r, tile_center(U, tl[0], tl[1], tl[2]));
+      // the element's frame: (origin-relative point) - tile centre for a
+      // hexagonal lattice
+      if (T::kHex && hex)
+        r = vsub(vsub(r, V3{U.Xl, U.Yl, U.Zl}), hex_tile_center(U, tl[0], tl[1], tl[2]));
+      else
+        r = vsub(r, tile_center(U, tl[0], tl[1], tl[2]));
       lvl++;
       t.set_lv(lvl, r);
       ui = elem;
@@ -423,7 +602,8 @@ ABX_NOINL int trk_descend(const DevProblem& P, Trk& t, int ui, int lvl, V3 r, co
 // the partial re-descent of Tracker::get_current (tracker.hpp:235-306),
 // which falls back to a restart when the partial descent finds no cell.
 // One descent site per call: the fallback loops instead of repeating it.
-ABX_HD void trk_locate(const DevProblem& P, Trk& t, const V3& u, int first_bad) {
+template <class T>
+ABX_HD void trk_locate(const DevProblem& P, T& t, const V3& u, int first_bad) {
   for (;;) {
     int ui, lvl;
     if (first_bad > 0) {
@@ -443,9 +623,11 @@ ABX_HD void trk_locate(const DevProblem& P, Trk& t, const V3& u, int first_bad) 
 }
 
 // Tracker::restart_get_current from the global position lv[0].
-ABX_HD void trk_restart(const DevProblem& P, Trk& t, const V3& u) { trk_locate(P, t, u, 0); }
+template <class T>
+ABX_HD void trk_restart(const DevProblem& P, T& t, const V3& u) { trk_locate(P, t, u, 0); }
 
-ABX_HD void trk_init(const DevProblem& P, Trk& t, const V3& r, const V3& u, int32_t token) {
+template <class T>
+ABX_HD void trk_init(const DevProblem& P, T& t, const V3& r, const V3& u, int32_t token) {
   t.err = 0;
   t.token = token;
   t.set_lv(0, r);
@@ -453,7 +635,8 @@ ABX_HD void trk_init(const DevProblem& P, Trk& t, const V3& r, const V3& u, int3
 }
 
 // Tracker::get_current after a move (tree consistent with lv[0]).
-ABX_HD void trk_get_current(const DevProblem& P, Trk& t, const V3& u) {
+template <class T>
+ABX_HD void trk_get_current(const DevProblem& P, T& t, const V3& u) {
   int first_bad = t.np;
   for (int i = 0; i < t.np; i++) {
     const int32_t w = t.pad(i);
@@ -465,7 +648,7 @@ ABX_HD void trk_get_current(const DevProblem& P, Trk& t, const V3& u) {
       }
     } else if (ty == PAD_L) {
       int tl[3];
-      get_tile(P.univ[pad_idx(w)], t.lv(pad_lvl(w)), u, tl);
+      lat_pad_tile<T::kHex>(P.univ[pad_idx(w)], t.lv(pad_lvl(w)), u, tl);
       if (!t.tile_is(i, tl)) {
         first_bad = i;
         break;
@@ -480,7 +663,8 @@ ABX_HD void trk_get_current(const DevProblem& P, Trk& t, const V3& u) {
 // restart, after which the reference still compares against the old tree
 // size (tracker.hpp:237-278) and re-descends below it if the new tree is
 // deeper.  Both descents go through the one trk_locate site of the loop.
-ABX_HD void trk_reset_position(const DevProblem& P, Trk& t, const V3& r, const V3& u) {
+template <class T>
+ABX_HD void trk_reset_position(const DevProblem& P, T& t, const V3& r, const V3& u) {
   const int old_np = t.np;
   t.token = 0;
   t.set_lv(0, r);
@@ -496,7 +680,8 @@ ABX_HD void trk_reset_position(const DevProblem& P, Trk& t, const V3& r, const V
   }
 }
 
-ABX_HD void trk_move(Trk& t, const V3& u, double d) {
+template <class T>
+ABX_HD void trk_move(T& t, const V3& u, double d) {
   const V3 du = vscale(u, d);
   // pads are pushed in descent order, so their frame levels never decrease:
   // the deepest frame in use is the last pad's
@@ -513,11 +698,12 @@ struct Bound {
 };
 
 // Universe::get_boundary_condition (cell universe / lattice via outer)
+template <bool HEX>
 ABX_NOINL Bound univ_bc(const DevProblem& P, int ui, const V3& r, const V3& u, int32_t on) {
   Bound b{ABX_INF, -1, 0, 0};
   for (int guard = 0; guard < 16; guard++) {
     const DUniv& U = P.univ[ui];
-    if (U.type == 1) {
+    if (HEX ? U.type != ABX_CELL_UNIVERSE : U.type == ABX_RECT_LATTICE) {
       if (!U.has_bc) return b;
       ui = U.outer;
       continue;
@@ -544,8 +730,9 @@ ABX_NOINL Bound univ_bc(const DevProblem& P, int ui, const V3& r, const V3& u, i
 }
 
 // Tracker::get_boundary_condition
-ABX_NOINL Bound trk_bc(const DevProblem& P, const Trk& t, const V3& u) {
-  if (t.cell < 0) return univ_bc(P, P.root, t.lv(0), u, t.token);
+template <class T>
+ABX_NOINL Bound trk_bc(const DevProblem& P, const T& t, const V3& u) {
+  if (t.cell < 0) return univ_bc<T::kHex>(P, P.root, t.lv(0), u, t.token);
   Bound b{ABX_INF, -1, 0, 0};
   for (int i = 0; i < t.np; i++) {
     const int32_t w = t.pad(i);
@@ -566,7 +753,7 @@ ABX_NOINL Bound trk_bc(const DevProblem& P, const Trk& t, const V3& u) {
     } else {
       const int ui = pad_idx(w);
       if (P.univ[ui].has_bc) {
-        const Bound ub = univ_bc(P, ui, rl, u, t.token);
+        const Bound ub = univ_bc<T::kHex>(P, ui, rl, u, t.token);
         if (ub.d < b.d && dabs(ub.d - b.d) > ABX_BOUNDRY_TOL) b = ub;
       }
     }
@@ -576,10 +763,12 @@ ABX_NOINL Bound trk_bc(const DevProblem& P, const Trk& t, const V3& u) {
 
 // The same out of line, for delta / carter tracking, which only needs it
 // when a flight left the geometry.
-ABX_HD Bound trk_bc_cold(const DevProblem& P, Trk t, V3 u) { return trk_bc(P, t, u); }
+template <class T>
+ABX_HD Bound trk_bc_cold(const DevProblem& P, T t, V3 u) { return trk_bc(P, t, u); }
 
 // Tracker::get_nearest_boundary
-ABX_NOINL Bound trk_nearest(const DevProblem& P, const Trk& t, const V3& u) {
+template <class T>
+ABX_NOINL Bound trk_nearest(const DevProblem& P, const T& t, const V3& u) {
   Bound b = trk_bc(P, t, u);
   for (int i = 0; i < t.np; i++) {
     const int32_t w = t.pad(i);
@@ -587,7 +776,7 @@ ABX_NOINL Bound trk_nearest(const DevProblem& P, const Trk& t, const V3& u) {
     const int ty = pad_type(w);
     if (ty == PAD_L) {
       const int32_t tl[3] = {t.tile(i, 0), t.tile(i, 1), t.tile(i, 2)};
-      const double d = tile_distance(P.univ[pad_idx(w)], rl, u, tl);
+      const double d = lat_pad_distance<T::kHex>(P.univ[pad_idx(w)], rl, u, tl);
       if (d < b.d && dabs(d - b.d) > ABX_BOUNDRY_TOL) {
         b.d = d;
         b.btype = 2;

@@ -1,7 +1,8 @@
 // instances.h - the compiled variants of the transport kernels.
 //
-// Every (tracking, trace, secondary stack, precision, LDS staging) variant of
-// the big transport kernels is instantiated once, in one of the parts of
+// Every (tracking, trace, secondary stack, precision, LDS staging,
+// hexagonal lattices) variant of the big transport kernels is instantiated
+// once, in one of the parts of
 // instances.hip (each part is its own translation unit, compiled in
 // parallel).  Everywhere else the variants are `extern`, so the host code of
 // abx.hip launches them without compiling their device code again.
@@ -24,22 +25,22 @@
 #define ABX_BATCH_T1(X) X(1, false, false) X(1, true, false) X(1, false, true) X(1, true, true)
 #define ABX_BATCH_T2(X) X(2, true, false) X(2, true, true)
 
-#define ABX_HIST_DECL(EXT, T, TR, SE, SP)                                                     \
-  EXT template __global__ void k_history<T, TR, SE, SP, true>(DevProblem, KCtx,               \
-                                                              unsigned long long*);           \
-  EXT template __global__ void k_history<T, TR, SE, SP, false>(DevProblem, KCtx,              \
-                                                               unsigned long long*);
+#define ABX_HIST_DECL(EXT, T, TR, SE, SP, HX)                                                 \
+  EXT template __global__ void k_history<T, TR, SE, SP, true, false, HX>(DevProblem, KCtx,    \
+                                                                         unsigned long long*); \
+  EXT template __global__ void k_history<T, TR, SE, SP, false, false, HX>(DevProblem, KCtx,   \
+                                                                          unsigned long long*);
 // replica mode: k-eigenvalue chains (no trace, no secondary stack)
-#define ABX_REP_DECL(EXT, T, SP)                                                              \
-  EXT template __global__ void k_history<T, false, false, SP, true, true>(DevProblem, KCtx,   \
-                                                                      unsigned long long*); \
-  EXT template __global__ void k_history<T, false, false, SP, false, true>(DevProblem, KCtx,  \
-                                                                       unsigned long long*);
+#define ABX_REP_DECL(EXT, T, SP, HX)                                                          \
+  EXT template __global__ void k_history<T, false, false, SP, true, true, HX>(                \
+      DevProblem, KCtx, unsigned long long*);                                                 \
+  EXT template __global__ void k_history<T, false, false, SP, false, true, HX>(               \
+      DevProblem, KCtx, unsigned long long*);
 #define ABX_REP_ALL(X) X(0, false) X(0, true) X(1, false) X(1, true)
-#define ABX_BATCH_DECL(EXT, T, SE, SP)                                                        \
-  EXT template __global__ void k_batch_chunks<T, SE, SP, true>(                               \
+#define ABX_BATCH_DECL(EXT, T, SE, SP, HX)                                                    \
+  EXT template __global__ void k_batch_chunks<T, SE, SP, true, HX>(                           \
       const DevProblem*, const KCtx*, const BatchChunk*, int, int, int, unsigned*);           \
-  EXT template __global__ void k_batch_chunks<T, SE, SP, false>(                              \
+  EXT template __global__ void k_batch_chunks<T, SE, SP, false, HX>(                          \
       const DevProblem*, const KCtx*, const BatchChunk*, int, int, int, unsigned*);
 #define ABX_EVENT_DECL(EXT, T, TR, SE, SP)                                                    \
   EXT template __global__ void k_ev_flight<T, TR, SE, SP>(DevProblem, KCtx, EvState,          \
@@ -62,8 +63,10 @@
   EXT template __global__ void k_ev_init<true, true>(DevProblem, KCtx, EvState, int32_t*,     \
                                                      unsigned*);
 
-#define ABX_XH(T, TR, SE, SP) ABX_HIST_DECL(extern, T, TR, SE, SP)
-#define ABX_XB(T, SE, SP) ABX_BATCH_DECL(extern, T, SE, SP)
+#define ABX_XH(T, TR, SE, SP)                                                                 \
+  ABX_HIST_DECL(extern, T, TR, SE, SP, false) ABX_HIST_DECL(extern, T, TR, SE, SP, true)
+#define ABX_XB(T, SE, SP)                                                                     \
+  ABX_BATCH_DECL(extern, T, SE, SP, false) ABX_BATCH_DECL(extern, T, SE, SP, true)
 #define ABX_XE(T, TR, SE, SP) ABX_EVENT_DECL(extern, T, TR, SE, SP)
 ABX_VARIANTS_T0(ABX_XH)
 ABX_VARIANTS_T1(ABX_XH)
@@ -75,7 +78,7 @@ ABX_VARIANTS_T0(ABX_XE)
 ABX_VARIANTS_T1(ABX_XE)
 ABX_VARIANTS_T2(ABX_XE)
 ABX_INIT_DECL(extern)
-#define ABX_XR(T, SP) ABX_REP_DECL(extern, T, SP)
+#define ABX_XR(T, SP) ABX_REP_DECL(extern, T, SP, false) ABX_REP_DECL(extern, T, SP, true)
 ABX_REP_ALL(ABX_XR)
 #undef ABX_XR
 #undef ABX_XH

@@ -241,7 +241,8 @@ struct GpuSink {
     K->stage_parent[i] = (uint32_t)slot;
     K->stage_daughter[i] = p.daughters;
   }
-  __device__ void trace(const Part& p, const Trk& t, int ev) {
+  template <class TK>
+  __device__ void trace(const Part& p, const TK& t, int ev) {
     if constexpr (TRACE) {
       if (n_rec < K->trace_cap) {
         abx_trace_rec r;
@@ -303,7 +304,8 @@ __device__ __forceinline__ void load_part(const DevProblem& P, const abx_particl
 // Dynamic LDS: every lane's cursor pads / tiles / frames, lane-interleaved.
 extern __shared__ __align__(16) unsigned char abx_lds[];
 
-__device__ __forceinline__ void bind_lds_cursor(Trk& t, const DevProblem& P) {
+template <class TK>
+__device__ __forceinline__ void bind_lds_cursor(TK& t, const DevProblem& P) {
   trk_bind(t, abx_lds, blockDim.x, threadIdx.x, P.trk_D, P.trk_L);
 }
 
@@ -345,7 +347,10 @@ __device__ __forceinline__ DevProblem stage_tables(const DevProblem& P) {
 // so that each variant's loads carry their address space.
 // REP: replica mode (K.rep_start and friends set), a variant of its own so
 // that the plain kernels carry none of its code.
-template <int TRK, bool TRACE, bool SEC, bool SPLIT, bool STAGE, bool REP = false>
+// HEX: the problem has hexagonal lattices; likewise its own variant, so
+// that the kernels of other problems carry none of that navigation.
+template <int TRK, bool TRACE, bool SEC, bool SPLIT, bool STAGE, bool REP = false,
+          bool HEX = false>
 __global__ void ABX_TKERNEL k_history(DevProblem P0, KCtx K, unsigned long long* next) {
   const DevProblem P = STAGE ? stage_tables(P0) : P0;
   const int64_t n_hist = K.n_dev ? (int64_t)*K.n_dev : K.n;
@@ -375,7 +380,7 @@ __global__ void ABX_TKERNEL k_history(DevProblem P0, KCtx K, unsigned long long*
       __hip_atomic_store(s.srange + 1, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
     }
   }
-  Trk t;
+  TrkT<HEX> t;
   bind_lds_cursor(t, P);
   Part p;
   p.alive = 0;
@@ -489,7 +494,7 @@ struct BatchChunk {
 // registers and the table reads are LDS reads.  A block waits at a chunk's
 // end for its longest history; chunks of several thousand histories keep
 // that wait small.
-template <int TRK, bool SEC, bool SPLIT, bool STAGE>
+template <int TRK, bool SEC, bool SPLIT, bool STAGE, bool HEX = false>
 __global__ void ABX_TKERNEL k_batch_chunks(const DevProblem* __restrict__ probs,
                                            const KCtx* __restrict__ ctxs,
                                            const BatchChunk* __restrict__ chunks, int n_chunks,
@@ -497,7 +502,7 @@ __global__ void ABX_TKERNEL k_batch_chunks(const DevProblem* __restrict__ probs,
   __shared__ int s_chunk;
   __shared__ unsigned long long s_next;
   const int lane = threadIdx.x & 63;
-  Trk t;
+  TrkT<HEX> t;
   trk_bind(t, abx_lds, blockDim.x, threadIdx.x, D, L);
   unsigned char* tables = abx_lds + cursor_lds_bytes(blockDim.x, D, L);
   int staged = -1;
@@ -585,7 +590,7 @@ __global__ void k_sample_sources(DevProblem P, DevSources S, uint64_t first, int
                                  unsigned long long* err_hist) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  Trk t;
+  TrkT<true> t;
   bind_lds_cursor(t, P);
   abx_particle a;
   const int rc = sample_source(P, S, first + (uint64_t)i, t, a);
@@ -623,7 +628,7 @@ __global__ void k_sample_jobs(const DevProblem* probs, const DevSources* srcs,
   }
   const SampleJob& J = jobs[lo];
   const int64_t k = i - J.start;
-  Trk t;
+  TrkT<true> t;
   trk_bind(t, abx_lds, blockDim.x, threadIdx.x, D, L);
   abx_particle a;
   const int rc = sample_source(probs[J.prob], srcs[lo], J.first + (uint64_t)k, t, a);

@@ -525,7 +525,8 @@ ABX_HD void leak(Part& p, const Bound& b, Scores6& sc) {
 
 // Tracker::do_reflection; the particle's position before the reflection
 // only feeds Particle::previous_r, which this path does not keep.
-ABX_HD void reflect(const DevProblem& P, Trk& t, Part& p, const Bound& b) {
+template <class TK>
+ABX_HD void reflect(const DevProblem& P, TK& t, Part& p, const Bound& b) {
   if (b.sidx < 0) {
     t.err = -3;
     return;
@@ -541,8 +542,8 @@ ABX_HD void reflect(const DevProblem& P, Trk& t, Part& p, const Bound& b) {
   trk_restart(P, t, p.u);
 }
 
-template <class Sink>
-ABX_HD bool try_resurrect(const DevProblem& P, Sink& s, Part& p, Trk& t, int& m) {
+template <class Sink, class TK>
+ABX_HD bool try_resurrect(const DevProblem& P, Sink& s, Part& p, TK& t, int& m) {
   SecRec q;
   if (!s.sec_pop(q)) return false;
   p.alive = 1;
@@ -592,8 +593,8 @@ ABX_HD void split(Sink& s, Part& p, const DevProblem& P) {
 
 // Start of a history: locate; returns false if the particle is dead
 // (lost at birth, DeltaTracker::transport src/delta_tracker.cpp:88-98).
-template <class Sink>
-ABX_HD bool start_history(const DevProblem& P, Sink& s, Part& p, Trk& t, int& m) {
+template <class Sink, class TK>
+ABX_HD bool start_history(const DevProblem& P, Sink& s, Part& p, TK& t, int& m) {
   trk_init(P, t, p.r, p.u, 0);
   if (t.err) {
     s.error(-3, p.hid);
@@ -612,8 +613,8 @@ ABX_HD bool start_history(const DevProblem& P, Sink& s, Part& p, Trk& t, int& m)
 // One flight of the transport loop up to the collision decision.  Returns
 // true when a real collision is pending; otherwise `ev` holds the event.
 // Sets p.alive = 0 and returns false after raising an error.
-template <int TRK, class Sink>
-ABX_HD bool step_flight(const DevProblem& P, Sink& s, Part& p, Trk& t, int& m, bool converged,
+template <int TRK, class Sink, class TK>
+ABX_HD bool step_flight(const DevProblem& P, Sink& s, Part& p, TK& t, int& m, bool converged,
                         Scores6& sc, int& ev) {
   if (TRK == 0) {
     // ---------------- surface tracking (src/surface_tracker.cpp:70-151)
@@ -742,8 +743,8 @@ ABX_HD void step_collision(const DevProblem& P, Sink& s, Part& p, int m, double 
 
 // End of a loop iteration: carter split, trace, resurrection.  Returns
 // whether the slot still carries a live particle.
-template <int TRK, class Sink>
-ABX_HD bool step_finish(const DevProblem& P, Sink& s, Part& p, Trk& t, int& m, int ev) {
+template <int TRK, class Sink, class TK>
+ABX_HD bool step_finish(const DevProblem& P, Sink& s, Part& p, TK& t, int& m, int ev) {
   if (TRK == 2) split(s, p, P);
   s.trace(p, t, ev);
   if (!p.alive) try_resurrect(P, s, p, t, m);
@@ -751,8 +752,8 @@ ABX_HD bool step_finish(const DevProblem& P, Sink& s, Part& p, Trk& t, int& m, i
 }
 
 // One whole history (plus its secondaries).
-template <int TRK, class Sink>
-ABX_HD void run_loop(const DevProblem& P, Sink& s, Part& p, Trk& t, int& m, double kcol,
+template <int TRK, class Sink, class TK>
+ABX_HD void run_loop(const DevProblem& P, Sink& s, Part& p, TK& t, int& m, double kcol,
                      bool converged, Scores6& sc, bool collision_pending) {
   for (int guard = 0; p.alive; guard++) {
     if (guard > 100000000) {
@@ -768,8 +769,8 @@ ABX_HD void run_loop(const DevProblem& P, Sink& s, Part& p, Trk& t, int& m, doub
   }
 }
 
-template <int TRK, class Sink>
-ABX_HD void run_history(const DevProblem& P, Sink& s, Part& p, Trk& t, double kcol,
+template <int TRK, class Sink, class TK>
+ABX_HD void run_history(const DevProblem& P, Sink& s, Part& p, TK& t, double kcol,
                         bool converged, Scores6& sc) {
   int m = -1;
   if (!start_history(P, s, p, t, m)) return;

@@ -27,7 +27,8 @@ ABX_HD V3 source_position(const abx_source& s, uint64_t& rng) {
 }
 
 // Returns 0 or an ABX_E_* code.
-ABX_HD int sample_source(const DevProblem& P, const DevSources& S, uint64_t h, Trk& t,
+template <class TK>
+ABX_HD int sample_source(const DevProblem& P, const DevSources& S, uint64_t h, TK& t,
                          abx_particle& out) {
   uint64_t rng = pcg_history_state(P.seed, P.stride, h);
   const int idx = S.n >= 2 ? rng_discrete(rng, S.cdf, S.n) : 0;

@@ -52,6 +52,7 @@ struct HostTables {
   int64_t tally_bins = 0;
   std::vector<abx_source> sources;
   int32_t max_pads = 1, max_levels = 1;  // cursor depth bound over all descents
+  bool has_hex = false;                   // any hexagonal lattice (the kernels' HEX variants)
   abx_settings st{};
   double min_energy = 0., max_energy = 100000.;
 };
@@ -188,6 +189,50 @@ static inline void build_tables(const abx_problem_desc* d, HostTables& T) {
         tables_require(ci >= 0 && ci < d->n_cells, "universe cell index");
         T.ucells.push_back(ci);
       }
+    } else if (a.type == ABX_HEX_LATTICE) {
+      // HexLattice constructor and set_elements (src/hex_lattice.cpp:30-56,
+      // 219-243); make_hex_lattice's checks (src/hex_lattice.cpp:469-503)
+      tables_require(a.shape[0] >= 1 && a.shape[1] >= 1, "hex lattice shape");
+      tables_require(a.pitch[0] > 0. && a.pitch[1] > 0., "hex lattice pitch");
+      tables_require(a.hex_top == ABX_HEX_POINTY || a.hex_top == ABX_HEX_FLAT, "hex lattice top");
+      tables_require(a.shape[0] <= 4096 && a.shape[1] <= (1 << 20), "hex lattice size");
+      const uint32_t rings = (uint32_t)a.shape[0], width = 2 * (rings - 1) + 1;
+      const uint32_t mid = width / 2;
+      T.has_hex = true;
+      U.rings = (int32_t)rings;
+      U.top = a.hex_top;
+      U.Nx = U.Ny = (int32_t)width;
+      U.Nz = a.shape[1];
+      U.Px = a.pitch[0];
+      U.Pz = a.pitch[1];
+      U.Xl = a.origin[0];
+      U.Yl = a.origin[1];
+      U.Zl = a.origin[2];
+      U.z_low = U.Zl - 0.5 * (double)(uint32_t)U.Nz * U.Pz;
+      U.c6 = std::cos(ABX_PI / 6.0);
+      U.s6 = std::sin(ABX_PI / 6.0);
+      U.c3 = std::cos(ABX_PI / 3.0);
+      U.s3 = std::sin(ABX_PI / 3.0);
+      const int64_t nhex = 3 * (int64_t)rings * (rings - 1) + 1;
+      const int64_t ne = nhex * U.Nz;
+      tables_require(a.elems_offset >= 0 && a.elems_offset + ne <= d->n_lattice_elems,
+                     "Improper number of universes for HexLattice.");
+      U.elems_off = (int32_t)T.elems.size();
+      T.elems.resize(T.elems.size() + (size_t)width * width * U.Nz, -1);
+      int64_t k = 0;
+      for (uint32_t az = 0; az < (uint32_t)U.Nz; az++)
+        for (uint32_t ar = 0; ar < width; ar++)
+          for (uint32_t aq = 0; aq < width; aq++) {
+            const int32_t q = (int32_t)(aq - mid), r = (int32_t)(ar - mid), y = -q - r;
+            const uint32_t ring = std::max(std::max((uint32_t)std::abs(q), (uint32_t)std::abs(y)),
+                                           (uint32_t)std::abs(r));
+            if (ring >= rings) continue;
+            const int e = d->lattice_elems[a.elems_offset + k++];
+            tables_require(e >= -1 && e < d->n_universes, "lattice element index");
+            T.elems[U.elems_off + ((size_t)az * width + ar) * width + aq] = e;
+          }
+      tables_require(a.outer >= -1 && a.outer < d->n_universes, "lattice outer");
+      U.outer = a.outer;
     } else {
       tables_require(a.type == ABX_RECT_LATTICE, "universe type");
       for (int k = 0; k < 3; k++) tables_require(a.shape[k] >= 1 && a.pitch[k] > 0., "lattice shape/pitch");
@@ -222,7 +267,7 @@ static inline void build_tables(const abx_problem_desc* d, HostTables& T) {
         if (T.cell[T.ucells[U.cells_off + k]].has_bc) U.has_bc = 1;
   for (int pass = 0; pass < 8; pass++)
     for (auto& U : T.univ)
-      if (U.type == ABX_RECT_LATTICE && U.outer >= 0) U.has_bc = T.univ[U.outer].has_bc;
+      if (U.type != ABX_CELL_UNIVERSE && U.outer >= 0) U.has_bc = T.univ[U.outer].has_bc;
   tables_require(d->root_universe >= 0 && d->root_universe < d->n_universes, "root universe");
   T.root = d->root_universe;
   // cursor depth: pads along the deepest descent and coordinate frames

@@ -3,12 +3,12 @@
 Follows the reference's parser (src/parser.cpp:77-1006): `materials` with
 multigroup data (src/mg_nuclide.cpp:579-922, including P1..P5 Legendre
 moments linearised like src/legendre_distribution.cpp:105-155), `surfaces`,
-`cells` (region strings), `universes` (cell lists or `rectlinear`
-lattices), `root-universe`, `sources` (point / box, isotropic /
+`cells` (region strings), `universes` (cell lists, `rectlinear` or
+`hexagonal` lattices), `root-universe`, `sources` (point / box, isotropic /
 mono-directional, mono-energetic), `tallies` (mesh tallies with
 collision / track-length estimators), `settings`, `sampling-xs-ratio` and an
 approximate `cancelator`.  Keys outside that subset (continuous energy,
-hexagonal lattices, noise, exact cancelators, ...) raise ValueError.
+noise, exact cancelators, ...) raise ValueError.
 """
 import math
 
@@ -146,12 +146,25 @@ def model_from_deck(deck):
         if "cells" in node:
             m.cell_universe([int(c) for c in node["cells"]], id=int(node["id"]))
         else:
-            if node.get("type") != "rectlinear":
-                raise ValueError("only rectlinear lattices are supported")
             outer = node.get("outer", -1)
-            m.rect_lattice(node["shape"], node["pitch"], node["origin"],
-                           [int(u) for u in node["universes"]],
-                           outer=None if outer == -1 else int(outer), id=int(node["id"]))
+            outer = None if outer == -1 else int(outer)
+            unis = [int(u) for u in node["universes"]]
+            if node.get("type") == "rectlinear":
+                m.rect_lattice(node["shape"], node["pitch"], node["origin"], unis,
+                               outer=outer, id=int(node["id"]))
+            elif node.get("type") == "hexagonal":
+                # src/hex_lattice.cpp:454-570
+                shape, pitch = node["shape"], node["pitch"]
+                if len(shape) != 2:
+                    raise ValueError("Lattice must have a valid shape.")
+                if len(pitch) != 2:
+                    raise ValueError("Lattice must have a valid pitch.")
+                m.hex_lattice(int(shape[0]), int(shape[1]), float(pitch[0]), float(pitch[1]),
+                              [float(x) for x in node["origin"]], unis,
+                              top=str(node.get("top", "pointy")), outer=outer,
+                              id=int(node["id"]))
+            else:
+                raise ValueError(f"Unknown lattice type {node.get('type')}.")
     m.root = int(deck["root-universe"])
     for node in deck.get("sources", []):
         sp, di, en = node["spatial"], node["direction"], node["energy"]

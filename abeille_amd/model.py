@@ -2,9 +2,9 @@
 objects, flattened into the C description of include/abx.h.
 
 The object model follows the deck (src/parser.cpp:77-1006): surfaces with
-ids and boundary conditions, cells with region strings, cell universes and
-rectilinear lattices referring to each other by id, multigroup materials,
-sources, mesh tallies and settings.  `Model.desc()` resolves ids to indices
+ids and boundary conditions, cells with region strings, cell universes,
+rectilinear and hexagonal lattices referring to each other by id,
+multigroup materials, sources, mesh tallies and settings.  `Model.desc()` resolves ids to indices
 and returns an `abx_problem_desc` (ctypes) that stays valid while the model
 object is alive.
 """
@@ -13,6 +13,16 @@ import ctypes as C
 import numpy as np
 
 from . import abi
+
+
+def hex_tiles(rings):
+    """Axial (q, r) of one layer of a hexagonal lattice of `rings` rings in
+    the order its universes are listed: rows of r ascending, q ascending
+    within a row, tiles outside the rings skipped (src/hex_lattice.cpp:
+    219-243)."""
+    mid = rings - 1
+    return [(q, r) for r in range(-mid, mid + 1) for q in range(-mid, mid + 1)
+            if max(abs(q), abs(r), abs(q + r)) < rings]
 
 
 class Model:
@@ -62,6 +72,27 @@ class Model:
         self.universes.append(dict(id=uid, type=abi.RECT_LATTICE, shape=list(shape),
                                    pitch=list(pitch), origin=list(origin),
                                    elems=list(universes), outer=outer))
+        return uid
+
+    def hex_lattice(self, rings, nz, pitch, pitch_z, origin, universes, top="pointy",
+                    outer=None, id=None):
+        """Hexagonal lattice of `rings` rings and `nz` axial layers
+        (src/hex_lattice.cpp:454-570, deck `type: hexagonal`).  `universes`
+        lists universe ids (or -1) in the deck's order: layer by layer, each
+        layer's tiles in rows of axial r ascending, q ascending within a row,
+        tiles outside the rings skipped (src/hex_lattice.cpp:219-243)."""
+        uid = id if id is not None else len(self.universes) + 1
+        if rings < 1 or nz < 1:
+            raise ValueError("All elements of HexLattice shape must be >= 1.")
+        if pitch <= 0. or pitch_z <= 0.:
+            raise ValueError("All elements of HexLattice pitch must > 0.")
+        if top not in ("pointy", "flat"):
+            raise ValueError(f"Uknown top for hexagonal lattice {top}.")
+        if len(universes) != (3 * rings * (rings - 1) + 1) * nz:
+            raise ValueError("Improper number of universes for HexLattice.")
+        self.universes.append(dict(id=uid, type=abi.HEX_LATTICE, shape=[rings, nz, 1],
+                                   pitch=[pitch, pitch_z, 0.], origin=list(origin),
+                                   top=top, elems=list(universes), outer=outer))
         return uid
 
     def material(self, id, total, absorption, scatter, fission=None, nu=None,
@@ -227,6 +258,8 @@ class Model:
                     unis[i].shape[k] = u["shape"][k]
                     unis[i].pitch[k] = u["pitch"][k]
                     unis[i].origin[k] = u["origin"][k]
+                if u["type"] == abi.HEX_LATTICE:
+                    unis[i].hex_top = abi.HEX_FLAT if u["top"] == "flat" else abi.HEX_POINTY
                 unis[i].elems_offset = len(elems)
                 elems += [(-1 if e == -1 else uidx[e]) for e in u["elems"]]
                 unis[i].outer = -1 if u["outer"] is None else uidx[u["outer"]]

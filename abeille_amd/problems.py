@@ -14,6 +14,8 @@
                       tracking and approximate mesh cancellation.
   la13511             analytic one-group cases (LA-13511) with their published
                       k values, used as known answers.
+  hex_assembly        hexagonal pin lattice (pointy or flat) with empty
+                      corner tiles, two axial layers, in a moderator box.
 
 Cross sections of the lattice problems are drawn from a numpy Generator with
 a fixed seed: per group sigma_t ~ U[0.2, 2.0], scattering ratio ~ U[0.5, 0.99],
@@ -25,7 +27,7 @@ by raising Sigma_a (src/mg_nuclide.cpp:267-272 requires it).
 import numpy as np
 
 from . import abi
-from .model import Model
+from .model import Model, hex_tiles
 
 # The 25 guide-tube / instrument positions of a 17x17 PWR assembly
 # (row = x index, column = y index; the layout of input_files/c5g7.yaml:257-273).
@@ -217,6 +219,58 @@ def core_3x3(nparticles=2_000_000, tracking="surface", xs_seed=20240607,
           energy_bounds=np.arange(G + 1.0), tracking=_tracking(tracking),
           precision=precision)
     return m
+
+
+def hex_assembly(nparticles=20000, tracking="surface", top="pointy", rings=4, nz=2,
+                 origin=(0., 0., 0.), xs_seed=20240607, tally=True, mode=abi.K_EIGENVALUE):
+    """Hexagonal pin assembly: `rings` rings of pins (pitch 1.26 cm, pin
+    radius 0.41 cm) in `nz` axial layers of 10 cm, inside a moderator box
+    (vacuum x faces, reflective y and z faces) entered as the lattice's
+    outer universe.  Layer 0 has guide tubes in the centre and on every
+    third tile of the inner rings and leaves the outer ring's six corners
+    empty (-1: the outer universe); layer 1 swaps fuel and guide tubes.
+    Returns (model, {(q, r, layer): universe id or -1})."""
+    G = 7
+    m = Model()
+    xs = seven_group_xs(xs_seed, G)
+    fuel, mod, gt = _add_lattice_materials(m, xs)
+    cyl = m.surface("zcylinder", [0., 0., 0.41], id=1)
+    u_fuel, u_gt = _pin_universes(m, fuel, mod, gt, cyl)
+    p, pz = 1.26, 10.0
+    hx, hz = rings * p + 0.5, nz * pz / 2.0
+    b = [m.surface("xplane", [origin[0] - hx], "vacuum", id=11),
+         m.surface("xplane", [origin[0] + hx], "vacuum", id=12),
+         m.surface("yplane", [origin[1] - hx], "reflective", id=13),
+         m.surface("yplane", [origin[1] + hx], "reflective", id=14),
+         m.surface("zplane", [origin[2] - hz], "reflective", id=15),
+         m.surface("zplane", [origin[2] + hz], "reflective", id=16)]
+    cbox = m.cell(f"+{b[0]} & -{b[1]} & +{b[2]} & -{b[3]} & +{b[4]} & -{b[5]}",
+                  material=mod, id=105)
+    u_out = m.cell_universe([cbox], id=3)
+    layout, elems = {}, []
+    last = rings - 1
+    corners = {(last, 0), (-last, 0), (0, last), (0, -last), (last, -last), (-last, last)}
+    for iz in range(nz):
+        for (q, r) in hex_tiles(rings):
+            ring = max(abs(q), abs(r), abs(q + r))
+            u = u_gt if (ring < last and (q - r) % 3 == 0) else u_fuel
+            if iz % 2 == 1:
+                u = u_fuel if u == u_gt else u_gt
+            if iz % 2 == 0 and rings > 1 and (q, r) in corners:
+                u = -1
+            layout[(q, r, iz)] = u
+            elems.append(u)
+    m.root = m.hex_lattice(rings, nz, p, pz, list(origin), elems, top=top, outer=u_out, id=10)
+    m.box_source([origin[0] - hx, origin[1] - hx, origin[2] - hz],
+                 [origin[0] + hx, origin[1] + hx, origin[2] + hz], energy=0.5,
+                 fissile_only=True)
+    if tally:
+        m.mesh_tally([origin[0] - hx, origin[1] - hx, origin[2] - hz],
+                     [origin[0] + hx, origin[1] + hx, origin[2] + hz], [24, 24, 2],
+                     np.arange(G + 1.0), estimator="track-length")
+    m.set(mode=mode, nparticles=nparticles, energy_bounds=np.arange(G + 1.0),
+          tracking=_tracking(tracking))
+    return m, layout
 
 
 def la13511(case, nparticles=100_000, tracking="surface"):

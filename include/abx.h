@@ -87,7 +87,11 @@ typedef struct abx_cell {
   int32_t rpn_len;
 } abx_cell;
 
-enum abx_universe_type { ABX_CELL_UNIVERSE = 0, ABX_RECT_LATTICE = 1 };
+enum abx_universe_type { ABX_CELL_UNIVERSE = 0, ABX_RECT_LATTICE = 1, ABX_HEX_LATTICE = 2 };
+
+/* hexagonal lattice orientation (include/geometry/hex_lattice.hpp:37, same
+ * order; the deck's `top: pointy | flat`, pointy by default) */
+enum abx_hex_top { ABX_HEX_POINTY = 0, ABX_HEX_FLAT = 1 };
 
 typedef struct abx_universe {
   uint32_t id;
@@ -97,11 +101,17 @@ typedef struct abx_universe {
   int32_t n_cells;
   /* rectilinear lattice (src/rect_lattice.cpp:33-52); `origin` is the
    * lattice centre as in the deck, elements are x-major, y fastest:
-   * index = nz*Nx*Ny + nx*Ny + ny (src/rect_lattice.cpp:300) */
+   * index = nz*Nx*Ny + nx*Ny + ny (src/rect_lattice.cpp:300).
+   * hexagonal lattice (src/hex_lattice.cpp:30-56): shape = {rings, Nz, -},
+   * pitch = {flat-to-flat pitch, axial pitch, -}, origin = centre of the
+   * middle tile; elements are the deck's list of rings*(3 rings - 3) + 1
+   * tiles per axial layer, layer by layer, each layer in rows of axial r
+   * ascending and q ascending within a row, tiles outside the rings
+   * skipped (src/hex_lattice.cpp:219-243). */
   int32_t shape[3];
   int32_t elems_offset; /* into abx_problem_desc.lattice_elems; -1 = dummy */
   int32_t outer;        /* outer universe index or -1 */
-  int32_t pad_;
+  int32_t hex_top;      /* abx_hex_top (hexagonal lattices) */
   double pitch[3];
   double origin[3];
 } abx_universe;

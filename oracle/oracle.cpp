@@ -12,7 +12,10 @@
 //    libstdc++ (tests/golden/rng_golden.json) + O'Neill's demo output.
 //  * log/sin/cos: glibc, or the portable detmath.h checked <= 1 ulp to glibc.
 //  * surfaces, cells, lattices, tracker: closed-form distances, brute-force
-//    point location, and face tie-break cases (tests/test_oracle_geometry.py).
+//    point location, and face tie-break cases (tests/test_oracle_geometry.py);
+//    hexagonal lattices: brute-force nearest-centre location, tile distances
+//    against ray/line intersections and closed forms, k-infinity through
+//    the lattice (tests/test_hex_lattice.py).
 //  * collision physics: closed forms (infinite-medium fluxes for delta,
 //    surface and carter tracking - carter on both of its branches -,
 //    k-infinity, leakage + absorption balance) and the LA-13511 analytic k
@@ -414,6 +417,12 @@ struct Universe {
   double Xl = 0, Yl = 0, Zl = 0;
   std::vector<int32_t> elems;
   int outer = -1;
+  // hex lattice (src/hex_lattice.cpp:30-56; include/geometry/hex_lattice.hpp:
+  // 58-66): pitch in Px, axial pitch in Pz, origin X_o/Y_o/Z_o in Xl/Yl/Zl,
+  // tiles in a width x width x Nz array of axial (q, r) shifted by mid
+  uint32_t rings = 0, width = 0, mid = 0;
+  int top = ABX_HEX_POINTY;
+  double c6 = 0, s6 = 0, c3 = 0, s3 = 0;
 };
 
 //==========================================================================
@@ -708,6 +717,163 @@ static size_t lattice_linear_index(const Universe& L, int nx, int ny, int nz) {
                   (uint32_t)ny);
 }
 
+//--------------------------------------------------------------------------
+// HexLattice (src/hex_lattice.cpp).  The arithmetic keeps the reference's
+// expressions term by term (its 0.* and 1.* factors included), so that the
+// tiles and distances carry the same bits.
+
+// HexLattice::get_nearest_hex (src/hex_lattice.cpp:260-301): axial (q, r)
+// of the point in the lattice's centred frame, by cube rounding.
+static void hex_nearest(const Universe& L, const V3& p, int32_t qr[2]) {
+  const double pt = L.Px;
+  double q, r, det;
+  if (L.top == ABX_HEX_FLAT) {
+    det = -pt * pt * L.s3;
+    q = (pt / det) * (0. * p.x - 1. * p.y);
+    r = (pt / det) * (-L.s3 * p.x + L.c3 * p.y);
+  } else {
+    det = -pt * pt * L.c6;
+    q = (pt / det) * (L.s6 * p.x - L.c6 * p.y);
+    r = (pt / det) * (-1. * p.x - 0. * p.y);
+  }
+  const double cx = q, cz = r, cy = -cx - cz;
+  double rx = std::round(cx), ry = std::round(cy), rz = std::round(cz);
+  const double dx = std::abs(rx - cx), dy = std::abs(ry - cy), dz = std::abs(rz - cz);
+  if (dx > dy && dx > dz)
+    rx = -ry - rz;
+  else if (dy > dx && dy > dz)
+    ry = -rx - rz;
+  else
+    rz = -rx - ry;
+  qr[0] = (int32_t)rx;
+  qr[1] = (int32_t)rz;
+}
+
+// HexLattice::get_tile (src/hex_lattice.cpp:303-308): the direction plays
+// no part, and the axial layer is counted from Z_o - Nz pz / 2 in the frame
+// the caller passes.
+static void hex_get_tile(const Universe& L, const V3& p, int t[3]) {
+  int32_t qr[2];
+  hex_nearest(L, p, qr);
+  const double z_low = L.Zl - 0.5 * (double)L.Nz * L.Pz;
+  t[0] = qr[0];
+  t[1] = qr[1];
+  t[2] = (int32_t)std::floor((p.z - z_low) / L.Pz);
+}
+
+// HexLattice::tile_center / get_hex_center (src/hex_lattice.cpp:310-331)
+static V3 hex_tile_center(const Universe& L, int q, int r, int nz) {
+  const double z_low = L.Zl - 0.5 * (double)L.Nz * L.Pz;
+  const double dq = (double)q, dr = (double)r;
+  V3 c;
+  if (L.top == ABX_HEX_FLAT) {
+    c.x = L.Px * (L.c3 * dq + 1. * dr);
+    c.y = L.Px * (L.s3 * dq + 0. * dr);
+  } else {
+    c.x = L.Px * (0. * dq + L.c6 * dr);
+    c.y = L.Px * (1. * dq + L.s6 * dr);
+  }
+  c.z = ((double)nz + 0.5) * L.Pz + z_low;
+  return c;
+}
+
+// HexLattice::get_ring (src/hex_lattice.cpp:333-345)
+static uint32_t hex_ring(int32_t q, int32_t r) {
+  const int32_t y = -q - r;
+  return std::max(std::max((uint32_t)std::abs(q), (uint32_t)std::abs(y)),
+                  (uint32_t)std::abs(r));
+}
+
+// HexLattice::distance_to_line (src/hex_lattice.cpp:435-452)
+static double hex_line(const V3& r, const V3& u, double x1, double y1, double x2,
+                       double y2) {
+  const double A = y2 - y1;
+  const double B = x1 - x2;
+  const double D = (x2 - x1) * y1 - (y2 - y1) * x1;
+  const double num = D - A * r.x - B * r.y;
+  const double denom = A * u.x + B * u.y;
+  const double d = num / denom;
+  if (d < 0.) return HUGE_VAL;
+  // within the coincidence tolerance of the line (|num| / |(A, B)|)
+  if (std::abs(num) <= 100 * SURFACE_COINCIDENT * std::sqrt(A * A + B * B)) return HUGE_VAL;
+  return d;
+}
+
+// HexLattice::distance_to_tile_boundary and its pointy / flat parts
+// (src/hex_lattice.cpp:347-433): six edge lines and two axial planes in the
+// frame of the tile centre, taken from r_local as given.  The six lines are
+// the reference's, vertex values included; they are not the edges of the
+// hexagon get_nearest_hex assigns (a particle can stop short of, or run
+// past, its tile's true edge) and are kept as they are for equal results.
+//
+// One deviation.  The reference takes any positive distance, and its tiles
+// do not depend on the direction.  None of its six lines is an edge of the
+// tile, so crossing one keeps the tile: a particle stopped on a line finds
+// it again ~1e-16 cm ahead (half the time), too close to move it, and
+// SurfaceTracker::transport (src/surface_tracker.cpp:72-150) crosses it for
+// ever (the first histories of a surface-tracked hex lattice already do).
+// Lines and layer planes within 100 * SURFACE_COINCIDENT (1e-10 cm) of the
+// point are skipped here, the tolerance RectLattice applies to its own
+// planes (src/rect_lattice.cpp:268-281).  A history the reference completes
+// differs only if a flight starts within 1e-10 cm of a line or plane
+// without stopping on it.
+static double hex_tile_distance(const Universe& L, const V3& r_local, const V3& u,
+                                const int tile[3]) {
+  const V3 rt = sub(r_local, hex_tile_center(L, tile[0], tile[1], tile[2]));
+  const double pt = L.Px;
+  double e[6];
+  if (L.top == ABX_HEX_POINTY) {
+    const double x1 = 0., y1 = pt / (2. * L.c6), x2 = pt / 2., y2 = pt / (2. * L.s6);
+    e[0] = hex_line(rt, u, x1, y1, x2, y1);
+    e[1] = hex_line(rt, u, x2, y2, x2, -y2);
+    e[2] = hex_line(rt, u, x2, -y2, x1, -y1);
+    e[3] = hex_line(rt, u, x1, -y1, -x2, -y2);
+    e[4] = hex_line(rt, u, -x2, -y2, -x2, y2);
+    e[5] = hex_line(rt, u, -x2, y2, x1, y1);
+  } else {
+    const double x1 = pt / (2. * L.s6), y1 = pt / 2., x2 = pt / (2. * L.c6), y2 = 0.;
+    e[0] = hex_line(rt, u, x1, y1, x2, y1);
+    e[1] = hex_line(rt, u, x2, y2, x1, -y1);
+    e[2] = hex_line(rt, u, x1, -y1, -x1, -y1);
+    e[3] = hex_line(rt, u, -x1, -y1, -x2, -y2);
+    e[4] = hex_line(rt, u, -x2, -y2, -x1, y1);
+    e[5] = hex_line(rt, u, -x1, y1, x1, y1);
+  }
+  const double dzl = (-L.Pz * 0.5 - rt.z) / u.z;
+  const double dzu = (L.Pz * 0.5 - rt.z) / u.z;
+  const double tol = 100 * SURFACE_COINCIDENT;
+  double d = INF;
+  for (int k = 0; k < 6; k++)
+    if (e[k] > 0. && e[k] < d) d = e[k];
+  if (dzl > 0. && dzl < d && std::abs(-L.Pz * 0.5 - rt.z) > tol) d = dzl;
+  if (dzu > 0. && dzu < d && std::abs(L.Pz * 0.5 - rt.z) > tol) d = dzu;
+  return d;
+}
+
+// The element of a hex tile, -1 outside the rings or the layers
+// (src/hex_lattice.cpp:143-206, 245-258).
+static int hex_element(const Universe& L, const int t[3]) {
+  if (hex_ring(t[0], t[1]) >= L.rings) return -1;
+  if (t[2] < 0 || t[2] >= (int32_t)L.Nz) return -1;
+  const size_t li = (size_t)((uint32_t)t[2] * (L.width * L.width) +
+                             ((uint32_t)t[1] + L.mid) * L.width + ((uint32_t)t[0] + L.mid));
+  return L.elems[li];
+}
+
+// Lattice::get_tile / distance_to_tile_boundary for either lattice kind,
+// from a lattice pad's r_local (the frame the lattice was entered in).
+static void lattice_pad_tile(const Universe& L, const V3& r, const V3& u, int t[3]) {
+  if (L.type == ABX_HEX_LATTICE)
+    hex_get_tile(L, r, t);
+  else
+    lattice_get_tile(L, r, u, t);
+}
+static double lattice_pad_distance(const Universe& L, const V3& r, const V3& u,
+                                   const int tile[3]) {
+  return L.type == ABX_HEX_LATTICE ? hex_tile_distance(L, r, u, tile)
+                                   : lattice_tile_distance(L, r, u, tile);
+}
+
 // Universe::get_cell with the lily-pad stack: CellUniverse
 // (src/cell_universe.cpp:72-109) and RectLattice (src/rect_lattice.cpp:128-207).
 // Returns the material cell index or -1.
@@ -724,6 +890,24 @@ static int universe_get_cell(const Problem& P, int ui, std::vector<Pad>& stack,
         return universe_get_cell(P, c.fill, stack, r, u, on_surf);
       }
     }
+    return -1;
+  }
+  if (U.type == ABX_HEX_LATTICE) {
+    // HexLattice::get_cell (src/hex_lattice.cpp:143-217): the tile of the
+    // origin-relative point; the pad keeps r, the element's frame is the
+    // origin-relative point less the tile centre, and the outer universe
+    // (if any) is entered at r.
+    const V3 ro{r.x - U.Xl, r.y - U.Yl, r.z - U.Zl};
+    int t[3];
+    hex_get_tile(U, ro, t);
+    const int e = hex_element(U, t);
+    if (e >= 0) {
+      const V3 c = hex_tile_center(U, t[0], t[1], t[2]);
+      stack.push_back({Pad::Lattice, ui, r, {t[0], t[1], t[2]}, false});
+      return universe_get_cell(P, e, stack, sub(ro, c), u, on_surf);
+    }
+    stack.push_back({Pad::Lattice, ui, r, {t[0], t[1], t[2]}, U.outer >= 0});
+    if (U.outer >= 0) return universe_get_cell(P, U.outer, stack, r, u, on_surf);
     return -1;
   }
   int t[3];
@@ -758,7 +942,7 @@ static BoundaryInfo universe_boundary_condition(const Problem& P, int ui,
                                                 const V3& r, const V3& u,
                                                 int32_t on_surf) {
   const Universe& U = P.universes[(size_t)ui];
-  if (U.type == ABX_RECT_LATTICE) {
+  if (U.type != ABX_CELL_UNIVERSE) {
     if (U.has_bc) return universe_boundary_condition(P, U.outer, r, u, on_surf);
     return {INF, -1, ABX_VACUUM, 0};
   }
@@ -842,7 +1026,7 @@ struct Tracker {
         }
       } else if (pad.type == Pad::Lattice) {
         int t[3];
-        lattice_get_tile(P->universes[(size_t)pad.idx], pad.r_local, u_, t);
+        lattice_pad_tile(P->universes[(size_t)pad.idx], pad.r_local, u_, t);
         if (pad.tile[0] != t[0] || pad.tile[1] != t[1] || pad.tile[2] != t[2]) {
           first_bad = i;
           break;
@@ -926,7 +1110,7 @@ struct Tracker {
     for (const Pad& pad : tree) {
       if (pad.type == Pad::Lattice) {
         const Universe& L = P->universes[(size_t)pad.idx];
-        double d = lattice_tile_distance(L, pad.r_local, u_, pad.tile);
+        double d = lattice_pad_distance(L, pad.r_local, u_, pad.tile);
         if (d < dist && std::abs(d - dist) > BOUNDRY_TOL) {
           dist = d;
           btype = ABX_NORMAL;
@@ -1461,6 +1645,11 @@ static void try_resurrect(Ctx& c, Particle& p, Tracker& t, int& mi) {
 
 // DeltaTracker::transport loop body (src/delta_tracker.cpp:86-229) and
 // CarterTracker (src/carter_tracker.cpp:104-260)
+// Flights per history (with its resurrected secondaries) before it is
+// reported lost instead of looping for ever; the library's kernels stop at
+// the same count.
+constexpr long kMaxFlights = 100000000;
+
 static void run_history_dt(Ctx& c, Particle& p, bool carter) {
   const Problem& P = *c.P;
   Tracker t(&P, p.s.r, p.s.u);
@@ -1469,7 +1658,9 @@ static void run_history_dt(Ctx& c, Particle& p, bool carter) {
     c.diag.lost++;
   }
   int mi = t.material();
+  long steps = 0;
   while (p.alive) {
+    if (++steps > kMaxFlights) fatal(ABX_E_LOST, "history exceeds the flight limit");
     bool had_collision = false;
     bool crossed = false;
     int g = energy_grid_index(P, p.s.E);
@@ -1550,7 +1741,9 @@ static void run_history_st(Ctx& c, Particle& p) {
     c.diag.lost++;
   }
   int mi = t.material();
+  long steps = 0;
   while (p.alive) {
+    if (++steps > kMaxFlights) fatal(ABX_E_LOST, "history exceeds the flight limit");
     bool had_collision = false;
     double d_coll = RNG::exponential(p.rng, mat_Et(P, mi, p.s.E));
     c.diag.flights++;
@@ -1730,6 +1923,36 @@ static void build(Problem& P, const abx_problem_desc* d) {
     if (u.type == ABX_CELL_UNIVERSE) {
       U.cells.assign(d->universe_cells + u.cells_offset,
                      d->universe_cells + u.cells_offset + u.n_cells);
+    } else if (u.type == ABX_HEX_LATTICE) {
+      // HexLattice constructor and set_elements (src/hex_lattice.cpp:30-56,
+      // 219-243); the trigonometric constants as in
+      // include/geometry/hex_lattice.hpp:60-63
+      U.rings = (uint32_t)u.shape[0];
+      U.Nz = (uint32_t)u.shape[1];
+      U.Px = u.pitch[0];
+      U.Pz = u.pitch[1];
+      U.Xl = u.origin[0];
+      U.Yl = u.origin[1];
+      U.Zl = u.origin[2];
+      U.top = u.hex_top;
+      U.c6 = std::cos(PI / 6.0);
+      U.s6 = std::sin(PI / 6.0);
+      U.c3 = std::cos(PI / 3.0);
+      U.s3 = std::sin(PI / 3.0);
+      U.width = 2 * (U.rings - 1) + 1;
+      U.mid = U.width / 2;
+      U.Nx = U.Ny = U.width;
+      U.elems.assign((size_t)U.width * U.width * U.Nz, -1);
+      size_t k = 0;
+      for (uint32_t az = 0; az < U.Nz; az++)
+        for (uint32_t ar = 0; ar < U.width; ar++)
+          for (uint32_t aq = 0; aq < U.width; aq++) {
+            const int32_t q = (int32_t)(aq - U.mid), r = (int32_t)(ar - U.mid);
+            if (hex_ring(q, r) < U.rings)
+              U.elems[(size_t)az * U.width * U.width + ar * U.width + aq] =
+                  d->lattice_elems[u.elems_offset + (int64_t)k++];
+          }
+      U.outer = u.outer;
     } else {
       U.Nx = (uint32_t)u.shape[0];
       U.Ny = (uint32_t)u.shape[1];
@@ -1760,7 +1983,7 @@ static void build(Problem& P, const abx_problem_desc* d) {
         if (P.cells[(size_t)ci].vacuum_or_reflective) U.has_bc = true;
   for (int pass = 0; pass < 8; pass++)
     for (auto& U : P.universes)
-      if (U.type == ABX_RECT_LATTICE && U.outer >= 0)
+      if (U.type != ABX_CELL_UNIVERSE && U.outer >= 0)
         U.has_bc = P.universes[(size_t)U.outer].has_bc;
   P.root = d->root_universe;
 

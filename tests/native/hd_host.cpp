@@ -45,7 +45,8 @@ struct HostSink {
     b.family_id = p.fam;
     bank->push_back(b);
   }
-  void trace_rec(const Part& p, const Trk& t, int ev, const DevProblem& P) {
+  template <class TK>
+  void trace_rec(const Part& p, const TK& t, int ev, const DevProblem& P) {
     if (!trace_buf) return;
     if (n_rec < trace_cap) {
       abx_trace_rec r{};
@@ -64,7 +65,8 @@ struct HostSink {
     n_rec++;
   }
   const DevProblem* Pp = nullptr;
-  void trace(const Part& p, const Trk& t, int ev) { trace_rec(p, t, ev, *Pp); }
+  template <class TK>
+  void trace(const Part& p, const TK& t, int ev) { trace_rec(p, t, ev, *Pp); }
   void error(int code, uint64_t h) {
     if (!err) { err = code; err_hist = h; }
   }
@@ -115,7 +117,7 @@ void hd_destroy(hd_problem* h) { delete h; }
 
 struct HostCursor {
   std::vector<unsigned char> buf;
-  Trk t;
+  TrkT<true> t;
   explicit HostCursor(const DevProblem& P) : buf((size_t)trk_bytes(P.trk_D, P.trk_L) + 16) {
     trk_bind(t, buf.data(), 1, 0, P.trk_D, P.trk_L);
   }

@@ -15,7 +15,9 @@ def _mix():
     out = [("cube_dt", problems.fixed_source_cube(3000, "delta"), 3),
            ("cube_st", problems.fixed_source_cube(2000, "surface"), 2),
            ("mfs_carter_cancel", problems.subcritical_slab(2000, tracking="carter"), 3),
-           ("mfs_delta", problems.subcritical_slab(1500, tracking="delta", cancel=False), 2)]
+           ("mfs_delta", problems.subcritical_slab(1500, tracking="delta", cancel=False), 2),
+           ("hex_fs", problems.hex_assembly(1500, tracking="surface", top="flat",
+                                            mode=abi.FIXED_SOURCE)[0], 2)]
     for k in range(6):
         m, _ = problems.slab_problem(rng, nhist=int(rng.integers(500, 4000)))
         if k % 2:
