@@ -58,6 +58,8 @@ def lib():
         L.abx_run.argtypes = [P, C.c_int, C.c_int, C.c_void_p, C.POINTER(abi.RunStats)]
         L.abx_run_continue.argtypes = [P, C.c_int, C.c_int, C.c_void_p,
                                        C.POINTER(abi.RunStats)]
+        L.abx_request_stop.argtypes = [P]
+        L.abx_set_max_time.argtypes = [P, C.c_double]
         L.abx_cancel_bank.argtypes = [P, C.c_void_p, C.c_size_t]
         L.abx_bank_size.argtypes = [P, C.POINTER(C.c_size_t)]
         L.abx_bank_fetch.argtypes = [P, C.c_void_p]
@@ -186,17 +188,28 @@ class Problem:
         return avg, err
 
     def run(self, ngenerations, nignored=0):
+        """abx_run; k_col per generation that ran (fewer than asked after a
+        stop request or the time budget)."""
         st = abi.RunStats()
         kh = np.zeros(ngenerations)
         _check(lib().abx_run(self.h, ngenerations, nignored, kh.ctypes.data, C.byref(st)))
-        return st, kh
+        return st, kh[:st.generations_run]
 
     def run_continue(self, ngenerations, nignored=0):
         st = abi.RunStats()
         kh = np.zeros(ngenerations)
         _check(lib().abx_run_continue(self.h, ngenerations, nignored, kh.ctypes.data,
                                       C.byref(st)))
-        return st, kh
+        return st, kh[:st.generations_run]
+
+    def request_stop(self):
+        """abx_request_stop: the run in progress (or the next) stops at its
+        next generation boundary; callable from another thread."""
+        _check(lib().abx_request_stop(self.h))
+
+    def set_max_time(self, seconds):
+        """abx_set_max_time: wall-clock budget of each run (<= 0: none)."""
+        _check(lib().abx_set_max_time(self.h, float(seconds)))
 
     def run_replicas(self, replicas, ngenerations, nignored=0, seeds=None, from_bank=False):
         """abx_run_replicas: `replicas` chains of this k-eigenvalue problem
@@ -214,7 +227,7 @@ class Problem:
                                       1 if from_bank else 0, ngenerations, nignored,
                                       kh.ctypes.data, C.byref(st)))
         self._replicas = replicas
-        return st, kh
+        return st, kh[:, :st.generations_run]
 
     def run_replicas_continue(self, ngenerations, nignored=0):
         """abx_run_replicas_continue: more generations of the last replica run."""
@@ -222,7 +235,7 @@ class Problem:
         kh = np.zeros((getattr(self, "_replicas", 1), ngenerations))
         _check(lib().abx_run_replicas_continue(self.h, ngenerations, nignored, kh.ctypes.data,
                                                C.byref(st)))
-        return st, kh
+        return st, kh[:, :st.generations_run]
 
     def fission_fetch(self, cap):
         """The ordered fission sites of the last transport (abx_fission_fetch),

@@ -153,7 +153,7 @@ class RunStats(C.Structure):
                 ("ktot_avg", C.c_double), ("ktot_err", C.c_double),
                 ("leak_avg", C.c_double), ("leak_err", C.c_double),
                 ("mig_avg", C.c_double), ("mig_err", C.c_double),
-                ("active_generations", C.c_int32), ("pad_", C.c_int32),
+                ("active_generations", C.c_int32), ("generations_run", C.c_int32),
                 ("transport_seconds", C.c_double), ("kernel_seconds", C.c_double),
                 ("kernel_launches", C.c_int64), ("kernel_items", C.c_int64),
                 ("histories", C.c_int64), ("diag", Diag)]

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -105,6 +106,11 @@ struct abx_problem {
   bool use_graphs = true;   // k-eigenvalue runs replay captured generations
   uint64_t run_hist = 0;  // histories_counter
   double run_kcol = 1.0;  // Tallies::kcol() of the previous generation
+  // stopping between generations (Simulation::signaled, settings::max_time)
+  std::atomic<int> stop_requested{0};
+  double max_time = 0.0;  // seconds of wall clock per run, <= 0: none
+  std::chrono::steady_clock::time_point run_t0{};
+  int run_gens = 0;  // generations completed since the run started
   // per-kernel timing of the transport kernels (HIP events on p->stream)
   bool timing = false;
   double kernel_ms = 0.0;
@@ -800,6 +806,8 @@ int abx_tally_stats(abx_problem* p, int tally, double* avg, double* std_err) {
 
 static void run_reset(abx_problem* p) {
   const HostTables& T = p->T;
+  p->run_t0 = std::chrono::steady_clock::now();
+  p->run_gens = 0;
   p->gen = 0;
   for (int q = 0; q < 6; q++) {
     p->s_avg[q] = p->s_var[q] = 0.0;
@@ -815,6 +823,22 @@ static void run_reset(abx_problem* p) {
   p->run_kcol = 1.0;
   p->bank_n = 0;
   p->rep_chains = 0;
+}
+
+// Whether a run stops at this generation boundary: a stop was requested
+// (the reference's signal handler and premature_kill, src/main.cpp:47-75,
+// src/power_iterator.cpp:412-417, 695-716), or the time left on the run's
+// budget is under two mean generation times (PowerIterator::out_of_time,
+// src/power_iterator.cpp:719-737).  `done` generations of the run have
+// completed and `in_flight` more are queued on the device; the request is
+// consumed.
+static bool stop_now(abx_problem* p, int done, int in_flight) {
+  if (p->stop_requested.exchange(0) != 0) return true;
+  if (p->max_time <= 0.0 || done < 1) return false;
+  const double used =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - p->run_t0).count();
+  const double t_avg = used / (double)done;
+  return p->max_time - used - (double)in_flight * t_avg < 2.0 * t_avg;
 }
 
 // Tallies' running means and standard errors (src/tallies.cpp:268-280).
@@ -1005,7 +1029,16 @@ static bool run_generations_graph(abx_problem* p, int ngenerations, int nignored
     }
     bool converged = nignored == 0;
     int par = 0;
+    int queued = ngenerations;  // generations launched (fewer after a stop)
     for (int gi = 0; gi < ngenerations; gi++) {
+      if (gi >= 2) {
+        // two generations ahead: gi - 2 has ended once gi - 1 started
+        HIPC(hipEventSynchronize(p->gen_events[2 * (gi - 1)]));
+        if (stop_now(p, p->run_gens + gi - 1, 1)) {
+          queued = gi;
+          break;
+        }
+      }
       const int conv = converged ? 1 : 0;
       if (!ex[par][conv][0]) capture(par, conv);
       HIPC(hipEventRecord(p->gen_events[2 * gi], s));
@@ -1015,7 +1048,7 @@ static bool run_generations_graph(abx_problem* p, int ngenerations, int nignored
       par ^= 1;
       if (gi + 1 == nignored) converged = true;
     }
-    HIPC(hipEventRecord(p->gen_events[2 * ngenerations], s));
+    HIPC(hipEventRecord(p->gen_events[2 * queued], s));
     HIPC(hipMemcpyAsync(&h, g, sizeof(GenDev), hipMemcpyDeviceToHost, s));
     std::vector<double> hist((size_t)6 * ngenerations);
     HIPC(hipMemcpyAsync(hist.data(), p->khist_dev.p, hist.size() * sizeof(double),
@@ -1025,12 +1058,12 @@ static bool run_generations_graph(abx_problem* p, int ngenerations, int nignored
     if (par) std::swap(p->bank_a, p->bank_b);
     float total = 0.f;
     double kms = 0.0;
-    for (int gi = 0; gi < ngenerations; gi++) {
+    for (int gi = 0; gi < queued; gi++) {
       float x = 0.f;
       HIPC(hipEventElapsedTime(&x, p->gen_events[2 * gi], p->gen_events[2 * gi + 1]));
       kms += x;
     }
-    HIPC(hipEventElapsedTime(&total, p->gen_events[0], p->gen_events[2 * ngenerations]));
+    HIPC(hipEventElapsedTime(&total, p->gen_events[0], p->gen_events[2 * queued]));
     cleanup();
     if (h.err) {
       if (h.err == -8) fail(ABX_E_INVALID, "No fission neutrons were produced.");
@@ -1042,12 +1075,13 @@ static bool run_generations_graph(abx_problem* p, int ngenerations, int nignored
       p->s_avg[q] = h.s_avg[q];
       p->s_var[q] = h.s_var[q];
     }
-    // the generations that ran (a device error stops the run early)
-    const int ran = std::min(ngenerations, (int)h.hist_idx);
+    // the generations that ran (a stop or a device error ends the run early)
+    const int ran = std::min(queued, (int)h.hist_idx);
     for (int q = 0; q < 6; q++)
       p->gen_vals[q].insert(p->gen_vals[q].end(), hist.begin() + (size_t)q * ngenerations,
                             hist.begin() + (size_t)q * ngenerations + ran);
-    if (kcol_hist) std::copy(hist.begin(), hist.begin() + ngenerations, kcol_hist);
+    if (kcol_hist) std::copy(hist.begin(), hist.begin() + ran, kcol_hist);
+    p->run_gens += ran;
     p->last_sites = (int64_t)h.n_sites;
     p->bank_n = (size_t)h.n;
     p->run_hist = h.run_hist;
@@ -1066,7 +1100,8 @@ static bool run_generations_graph(abx_problem* p, int ngenerations, int nignored
     d.secondaries = h.diag[7];
     stats->transport_seconds = total * 1e-3;
     stats->kernel_seconds = kms * 1e-3;
-    stats->kernel_launches = ngenerations;
+    stats->kernel_launches = queued;
+    stats->generations_run = ran;
     stats->kernel_items = (int64_t)(d.flights + d.collisions);
     stats->histories = (int64_t)d.histories;
   } catch (...) {
@@ -1305,7 +1340,15 @@ static void run_replicas(abx_problem* p, int R, const uint64_t* seeds, bool from
     }
     bool converged = nignored == 0;
     int par = 0;
+    int queued = ngenerations;  // generations launched (fewer after a stop)
     for (int gi = 0; gi < ngenerations; gi++) {
+      if (gi >= 2) {
+        HIPC(hipEventSynchronize(p->gen_events[2 * (gi - 1)]));
+        if (stop_now(p, p->run_gens + gi - 1, 1)) {
+          queued = gi;
+          break;
+        }
+      }
       const int conv = converged ? 1 : 0;
       if (!ex[par][conv][0]) capture(par, conv);
       HIPC(hipEventRecord(p->gen_events[2 * gi], s));
@@ -1315,7 +1358,7 @@ static void run_replicas(abx_problem* p, int R, const uint64_t* seeds, bool from
       par ^= 1;
       if (gi + 1 == nignored) converged = true;
     }
-    HIPC(hipEventRecord(p->gen_events[2 * ngenerations], s));
+    HIPC(hipEventRecord(p->gen_events[2 * queued], s));
     HIPC(hipMemcpyAsync(&h, g, sizeof(GenRep), hipMemcpyDeviceToHost, s));
     std::vector<double> hist((size_t)(6 + R) * ngenerations);
     HIPC(hipMemcpyAsync(hist.data(), hist_dev, hist.size() * sizeof(double),
@@ -1323,12 +1366,12 @@ static void run_replicas(abx_problem* p, int R, const uint64_t* seeds, bool from
     HIPC(hipStreamSynchronize(s));
     float total = 0.f;
     double kms = 0.0;
-    for (int gi = 0; gi < ngenerations; gi++) {
+    for (int gi = 0; gi < queued; gi++) {
       float x = 0.f;
       HIPC(hipEventElapsedTime(&x, p->gen_events[2 * gi], p->gen_events[2 * gi + 1]));
       kms += x;
     }
-    HIPC(hipEventElapsedTime(&total, p->gen_events[0], p->gen_events[2 * ngenerations]));
+    HIPC(hipEventElapsedTime(&total, p->gen_events[0], p->gen_events[2 * queued]));
     cleanup();
     if (h.err) {
       if (h.err == -8) fail(ABX_E_INVALID, "No fission neutrons were produced.");
@@ -1340,7 +1383,8 @@ static void run_replicas(abx_problem* p, int R, const uint64_t* seeds, bool from
       p->s_avg[q] = h.s_avg[q];
       p->s_var[q] = h.s_var[q];
     }
-    const int ran = std::min(ngenerations, (int)h.hist_idx);
+    const int ran = std::min(queued, (int)h.hist_idx);
+    p->run_gens += ran;
     for (int q = 0; q < 6; q++)
       p->gen_vals[q].insert(p->gen_vals[q].end(), hist.begin() + (size_t)q * ngenerations,
                             hist.begin() + (size_t)q * ngenerations + ran);
@@ -1367,7 +1411,8 @@ static void run_replicas(abx_problem* p, int R, const uint64_t* seeds, bool from
     d.secondaries = h.diag[7];
     stats->transport_seconds = total * 1e-3;
     stats->kernel_seconds = kms * 1e-3;
-    stats->kernel_launches = ngenerations;
+    stats->kernel_launches = queued;
+    stats->generations_run = ran;
     stats->kernel_items = (int64_t)(d.flights + d.collisions);
     stats->histories = (int64_t)d.histories;
   } catch (...) {
@@ -1412,6 +1457,7 @@ static void run_generations(abx_problem* p, int ngenerations, int nignored, doub
     p->timing = false;
     return;
   }
+  int ran = 0;
   for (int g = 1; g <= ngenerations; g++) {
     abx_diag dg{};
     abx_scores sc{};
@@ -1495,12 +1541,16 @@ static void run_generations(abx_problem* p, int ngenerations, int nignored, doub
     }
     clear_generation(p);
     add_diag(stats->diag, dg);
+    p->run_gens++;
+    ran = g;
+    if (g < ngenerations && stop_now(p, p->run_gens, 0)) break;
   }
   HIPC(hipStreamSynchronize(p->stream));
   p->timing = false;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   fill_statistics(p, stats);
+  stats->generations_run = ran;
   stats->transport_seconds = ms_total * 1e-3;
   stats->kernel_seconds = p->kernel_ms * 1e-3;
   stats->kernel_launches = p->kernel_launches;
@@ -1508,6 +1558,19 @@ static void run_generations(abx_problem* p, int ngenerations, int nignored, doub
                             ? (int64_t)(stats->diag.flights + stats->diag.collisions)
                             : p->kernel_items;
   stats->histories = histories;
+}
+
+int abx_request_stop(abx_problem* p) {
+  if (!p) return ABX_E_INVALID;
+  p->stop_requested.store(1);
+  return ABX_OK;
+}
+
+int abx_set_max_time(abx_problem* p, double seconds) {
+  ABX_TRY({
+    if (!p || !(seconds == seconds)) fail(ABX_E_INVALID, "max time");
+    p->max_time = seconds;
+  });
 }
 
 int abx_run(abx_problem* p, int ngenerations, int nignored, double* kcol_hist,

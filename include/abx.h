@@ -428,7 +428,8 @@ typedef struct abx_run_stats {
   double leak_avg, leak_err;
   double mig_avg, mig_err;
   int32_t active_generations;
-  int32_t pad_;
+  int32_t generations_run;  /* generations this call ran: fewer than asked
+                             * after abx_request_stop or the time budget */
   double transport_seconds; /* GPU time of the transport phase */
   double kernel_seconds;    /* GPU time of the dominant transport kernel */
   int64_t kernel_launches;  /* its launches */
@@ -451,6 +452,23 @@ int abx_run(abx_problem* p, int ngenerations, int nignored,
 int abx_run_continue(abx_problem* p, int ngenerations, int nignored,
                      double* kcol_hist, abx_run_stats* stats);
 int abx_tally_stats(abx_problem* p, int tally, double* avg, double* std_err);
+
+/* Stopping a run between generations.  abx_request_stop may be called from
+ * any thread: the run in progress (abx_run, abx_run_continue,
+ * abx_run_replicas, abx_run_replicas_continue), or else the next one,
+ * stops at its next generation boundary, as the reference does on SIGINT /
+ * SIGTERM after the generation's k and tallies are recorded
+ * (src/main.cpp:47-75, src/power_iterator.cpp:412-417, 695-716; there is no
+ * prompt).  abx_set_max_time gives each run a wall-clock budget in seconds
+ * (settings::max_time; <= 0: none): the run stops when the time left is
+ * under two mean generation times (PowerIterator::out_of_time,
+ * src/power_iterator.cpp:719-737).  The graph-replayed drivers keep one
+ * generation queued ahead, which then still completes.  A stopped run's
+ * statistics, tallies and bank are those of a run of
+ * abx_run_stats.generations_run generations, and kcol_hist holds that many
+ * values. */
+int abx_request_stop(abx_problem* p);
+int abx_set_max_time(abx_problem* p, double seconds);
 
 /* PowerIterator::run (src/power_iterator.cpp:305-429) for `replicas`
  * (1..16) independent chains of one k-eigenvalue problem that differ only
