@@ -362,7 +362,7 @@ TransportOut transport_device(abx_problem* p, const abx_particle* d_in, int64_t 
     if (p->engine == ABX_ENGINE_HISTORY) {
       if (p->timing) HIPC(hipEventRecord(p->ke0, p->stream));
       p->pool_next.reserve(sizeof(unsigned long long));
-      HIPC(history_transport(p->P, p->T.has_hex, K, p->stream, sec_cap > 0,
+      HIPC(history_transport(p->P, p->T.ext_geometry, K, p->stream, sec_cap > 0,
                              p->pool_next.as<unsigned long long>(), p->tuning));
       if (p->timing) {
         HIPC(hipEventRecord(p->ke1, p->stream));
@@ -933,16 +933,16 @@ static bool run_generations_graph(abx_problem* p, int ngenerations, int nignored
   bool stage = false;
   int64_t grid = 0;
   // (carter tracking never gets here: it carries the secondary stack)
-  const bool hex = p->T.has_hex;
-  HIPC(dispatch_variant(trk, false, false, split, hex, [&](auto TT, auto, auto SE, auto SP, auto HX) {
+  const bool ext = p->T.ext_geometry;
+  HIPC(dispatch_variant(trk, false, false, split, ext, [&](auto TT, auto, auto SE, auto SP, auto EX) {
     constexpr int TK = decltype(TT)::value;
-    constexpr bool SEC = decltype(SE)::value, SPL = decltype(SP)::value, HEX = decltype(HX)::value;
-    stage = tu.stage_tables && stage_pays(k_history<TK, false, SEC, SPL, true, false, HEX>,
-                                          k_history<TK, false, SEC, SPL, false, false, HEX>, bs,
+    constexpr bool SEC = decltype(SE)::value, SPL = decltype(SP)::value, EXT = decltype(EX)::value;
+    stage = tu.stage_tables && stage_pays(k_history<TK, false, SEC, SPL, true, false, EXT>,
+                                          k_history<TK, false, SEC, SPL, false, false, EXT>, bs,
                                           cur, (size_t)p->P.blob_bytes);
-    grid = stage ? resident_blocks(k_history<TK, false, SEC, SPL, true, false, HEX>, bs,
+    grid = stage ? resident_blocks(k_history<TK, false, SEC, SPL, true, false, EXT>, bs,
                                    cur + (size_t)p->P.blob_bytes)
-                 : resident_blocks(k_history<TK, false, SEC, SPL, false, false, HEX>, bs, cur);
+                 : resident_blocks(k_history<TK, false, SEC, SPL, false, false, EXT>, bs, cur);
     return hipSuccess;
   }));
   const size_t lds = stage ? cur + (size_t)p->P.blob_bytes : cur;
@@ -965,16 +965,16 @@ static bool run_generations_graph(abx_problem* p, int ngenerations, int nignored
     Kc.converged = conv;
     HIPC(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(k_gen_begin, dim3(1), dim3(64), 0, s, g);
-    HIPC(dispatch_variant(trk, false, false, split, hex,
-                          [&](auto TT, auto, auto SE, auto SP, auto HX) {
+    HIPC(dispatch_variant(trk, false, false, split, ext,
+                          [&](auto TT, auto, auto SE, auto SP, auto EX) {
       constexpr int TK = decltype(TT)::value;
       constexpr bool SEC = decltype(SE)::value, SPL = decltype(SP)::value;
-      constexpr bool HEX = decltype(HX)::value;
+      constexpr bool EXT = decltype(EX)::value;
       if (stage)
-        hipLaunchKernelGGL((k_history<TK, false, SEC, SPL, true, false, HEX>),
+        hipLaunchKernelGGL((k_history<TK, false, SEC, SPL, true, false, EXT>),
                            dim3((unsigned)grid), dim3(bs), lds, s, p->P, Kc, &g->next);
       else
-        hipLaunchKernelGGL((k_history<TK, false, SEC, SPL, false, false, HEX>),
+        hipLaunchKernelGGL((k_history<TK, false, SEC, SPL, false, false, EXT>),
                            dim3((unsigned)grid), dim3(bs), lds, s, p->P, Kc, &g->next);
       return hipSuccess;
     }));
@@ -1121,19 +1121,19 @@ static bool run_generations_graph(abx_problem* p, int ngenerations, int nignored
 // The replica-mode kernel variants: surface or delta tracking, FP64 or
 // split precision (no trace, no secondary stack).
 extern "C++" {
-template <bool HEX, class Fn>
+template <bool EXT, class Fn>
 static hipError_t rep_dispatch_t(int trk, bool split, Fn&& fn) {
-  const BoolC<HEX> hx{};
+  const BoolC<EXT> hx{};
   if (trk == 0)
     return split ? fn(IntC<0>{}, BoolC<true>{}, hx) : fn(IntC<0>{}, BoolC<false>{}, hx);
   if (trk == 1)
     return split ? fn(IntC<1>{}, BoolC<true>{}, hx) : fn(IntC<1>{}, BoolC<false>{}, hx);
   return hipErrorInvalidValue;
 }
-// fn(TRK, SPLIT, HEX) over the replica variants
+// fn(TRK, SPLIT, EXT) over the replica variants
 template <class Fn>
-static hipError_t rep_dispatch(int trk, bool split, bool hex, Fn&& fn) {
-  return hex ? rep_dispatch_t<true>(trk, split, fn) : rep_dispatch_t<false>(trk, split, fn);
+static hipError_t rep_dispatch(int trk, bool split, bool ext, Fn&& fn) {
+  return ext ? rep_dispatch_t<true>(trk, split, fn) : rep_dispatch_t<false>(trk, split, fn);
 }
 }
 
@@ -1245,16 +1245,16 @@ static void run_replicas(abx_problem* p, int R, const uint64_t* seeds, bool from
   const bool split = T.st.precision == ABX_SPLIT;
   bool stage = false;
   int64_t grid = 0;
-  const bool hex = p->T.has_hex;
-  HIPC(rep_dispatch(trk, split, hex, [&](auto TT, auto SP, auto HX) {
+  const bool ext = p->T.ext_geometry;
+  HIPC(rep_dispatch(trk, split, ext, [&](auto TT, auto SP, auto EX) {
     constexpr int TK = decltype(TT)::value;
-    constexpr bool SPL = decltype(SP)::value, HEX = decltype(HX)::value;
-    stage = tu.stage_tables && stage_pays(k_history<TK, false, false, SPL, true, true, HEX>,
-                                          k_history<TK, false, false, SPL, false, true, HEX>, bs,
+    constexpr bool SPL = decltype(SP)::value, EXT = decltype(EX)::value;
+    stage = tu.stage_tables && stage_pays(k_history<TK, false, false, SPL, true, true, EXT>,
+                                          k_history<TK, false, false, SPL, false, true, EXT>, bs,
                                           cur, (size_t)p->P.blob_bytes);
-    grid = stage ? resident_blocks(k_history<TK, false, false, SPL, true, true, HEX>, bs,
+    grid = stage ? resident_blocks(k_history<TK, false, false, SPL, true, true, EXT>, bs,
                                    cur + (size_t)p->P.blob_bytes)
-                 : resident_blocks(k_history<TK, false, false, SPL, false, true, HEX>, bs, cur);
+                 : resident_blocks(k_history<TK, false, false, SPL, false, true, EXT>, bs, cur);
     return hipSuccess;
   }));
   const size_t lds = stage ? cur + (size_t)p->P.blob_bytes : cur;
@@ -1278,14 +1278,14 @@ static void run_replicas(abx_problem* p, int R, const uint64_t* seeds, bool from
     Kc.converged = conv;
     HIPC(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(k_rep_begin, dim3(1), dim3(256), 0, s, g);
-    HIPC(rep_dispatch(trk, split, hex, [&](auto TT, auto SP, auto HX) {
+    HIPC(rep_dispatch(trk, split, ext, [&](auto TT, auto SP, auto EX) {
       constexpr int TK = decltype(TT)::value;
-      constexpr bool SPL = decltype(SP)::value, HEX = decltype(HX)::value;
+      constexpr bool SPL = decltype(SP)::value, EXT = decltype(EX)::value;
       if (stage)
-        hipLaunchKernelGGL((k_history<TK, false, false, SPL, true, true, HEX>),
+        hipLaunchKernelGGL((k_history<TK, false, false, SPL, true, true, EXT>),
                            dim3((unsigned)grid), dim3(bs), lds, s, p->P, Kc, &g->next);
       else
-        hipLaunchKernelGGL((k_history<TK, false, false, SPL, false, true, HEX>),
+        hipLaunchKernelGGL((k_history<TK, false, false, SPL, false, true, EXT>),
                            dim3((unsigned)grid), dim3(bs), lds, s, p->P, Kc, &g->next);
       return hipSuccess;
     }));
@@ -1778,29 +1778,29 @@ bool batch_stage_fits(const abx_problem* p) {
   const bool sec = needs_secondaries(T), split = T.st.precision == ABX_SPLIT;
   const size_t cur = cursor_lds_bytes(256, p->P.trk_D, p->P.trk_L);
   bool ok = false;
-  (void)dispatch_variant(trk, false, sec || trk == 2, split, T.has_hex,
-                         [&](auto TT, auto, auto SE, auto SP, auto HX) {
+  (void)dispatch_variant(trk, false, sec || trk == 2, split, T.ext_geometry,
+                         [&](auto TT, auto, auto SE, auto SP, auto EX) {
     constexpr int TK = decltype(TT)::value;
     constexpr bool SEC = decltype(SE)::value, SPL = decltype(SP)::value;
-    constexpr bool HEX = decltype(HX)::value;
-    ok = stage_pays(k_batch_chunks<TK, SEC, SPL, true, HEX>,
-                    k_batch_chunks<TK, SEC, SPL, false, HEX>, 256, cur, (size_t)p->P.blob_bytes);
+    constexpr bool EXT = decltype(EX)::value;
+    ok = stage_pays(k_batch_chunks<TK, SEC, SPL, true, EXT>,
+                    k_batch_chunks<TK, SEC, SPL, false, EXT>, 256, cur, (size_t)p->P.blob_bytes);
     return hipSuccess;
   });
   return ok;
 }
 
 // Launch group of a problem in the batch kernel: tracking mode, secondaries,
-// precision, hexagonal lattices, and whether its tables are worth staging in
+// precision, extended geometry, and whether its tables are worth staging in
 // LDS (problems whose tables are launched apart from those whose tables are
 // not, so that the former run the staged variant).
 int variant_key(abx_problem* p) {
   const HostTables& T = p->T;
   const int sec = needs_secondaries(T) ? 1 : 0;
   const int split = T.st.precision == ABX_SPLIT ? 1 : 0;
-  const int hex = T.has_hex ? 1 : 0;
+  const int ext = T.ext_geometry ? 1 : 0;
   if (p->batch_stage < 0) p->batch_stage = batch_stage_fits(p) ? 1 : 0;
-  return (((T.st.tracking * 2 + sec) * 2 + split) * 2 + hex) * 2 + p->batch_stage;
+  return (((T.st.tracking * 2 + sec) * 2 + split) * 2 + ext) * 2 + p->batch_stage;
 }
 
 // Chunks of `chunk` histories over the problems of one launch (item indices
@@ -1823,7 +1823,7 @@ int64_t chunk_length(int64_t total, int per_cu = 8) {
   return std::max<int64_t>(256, std::min<int64_t>(8192, total / std::max(1, per_cu * cus)));
 }
 
-template <int TRK, bool SEC, bool SPLIT, bool STAGE, bool HEX>
+template <int TRK, bool SEC, bool SPLIT, bool STAGE, bool EXT>
 hipError_t launch_chunks(const DevProblem* probs, const KCtx* ctxs, const BatchChunk* chunks,
                          int n_chunks, int D, int L, size_t blob_max, unsigned* next,
                          hipStream_t s) {
@@ -1832,8 +1832,8 @@ hipError_t launch_chunks(const DevProblem* probs, const KCtx* ctxs, const BatchC
   hipError_t e;
   if ((e = hipMemsetAsync(next, 0, sizeof(unsigned), s)) != hipSuccess) return e;
   const int64_t blocks = std::min<int64_t>(
-      n_chunks, resident_blocks(k_batch_chunks<TRK, SEC, SPLIT, STAGE, HEX>, bs, lds));
-  hipLaunchKernelGGL((k_batch_chunks<TRK, SEC, SPLIT, STAGE, HEX>), dim3((unsigned)blocks),
+      n_chunks, resident_blocks(k_batch_chunks<TRK, SEC, SPLIT, STAGE, EXT>, bs, lds));
+  hipLaunchKernelGGL((k_batch_chunks<TRK, SEC, SPLIT, STAGE, EXT>), dim3((unsigned)blocks),
                      dim3(bs), lds, s, probs, ctxs, chunks, n_chunks, D, L, next);
   return hipGetLastError();
 }
@@ -1844,18 +1844,18 @@ hipError_t launch_chunks_variant(int key, const DevProblem* probs, const KCtx* c
                                  const BatchChunk* chunks, int n_chunks, int D, int L,
                                  size_t blob_max, unsigned* next, hipStream_t s) {
   const int trk = key / 16;
-  const bool sec = (key / 8) & 1, split = (key / 4) & 1, hex = (key / 2) & 1, fits = key & 1;
+  const bool sec = (key / 8) & 1, split = (key / 4) & 1, ext = (key / 2) & 1, fits = key & 1;
   const size_t cur = cursor_lds_bytes(256, D, L);
-  return dispatch_variant(trk, false, sec || trk == 2, split, hex,
-                          [&](auto T, auto, auto SE, auto SP, auto HX) {
+  return dispatch_variant(trk, false, sec || trk == 2, split, ext,
+                          [&](auto T, auto, auto SE, auto SP, auto EX) {
     constexpr int TK = decltype(T)::value;
     constexpr bool SEC = decltype(SE)::value, SPL = decltype(SP)::value;
-    constexpr bool HEX = decltype(HX)::value;
-    if (fits && stage_pays(k_batch_chunks<TK, SEC, SPL, true, HEX>,
-                           k_batch_chunks<TK, SEC, SPL, false, HEX>, 256, cur, blob_max))
-      return launch_chunks<TK, SEC, SPL, true, HEX>(probs, ctxs, chunks, n_chunks, D, L, blob_max,
+    constexpr bool EXT = decltype(EX)::value;
+    if (fits && stage_pays(k_batch_chunks<TK, SEC, SPL, true, EXT>,
+                           k_batch_chunks<TK, SEC, SPL, false, EXT>, 256, cur, blob_max))
+      return launch_chunks<TK, SEC, SPL, true, EXT>(probs, ctxs, chunks, n_chunks, D, L, blob_max,
                                                     next, s);
-    return launch_chunks<TK, SEC, SPL, false, HEX>(probs, ctxs, chunks, n_chunks, D, L, blob_max,
+    return launch_chunks<TK, SEC, SPL, false, EXT>(probs, ctxs, chunks, n_chunks, D, L, blob_max,
                                                    next, s);
   });
 }
@@ -2459,7 +2459,9 @@ extern "C" int abx_transport_noalloc(abx_problem* p, const abx_particle* d_bank,
     K.counters = d_counters;
     K.err = d_err;
     K.err_hist = d_err_hist;
-    if (nn > 0) HIPC(history_transport(p->P, p->T.has_hex, K, p->stream, sec_cap > 0, d_next, p->tuning));
+    if (nn > 0)
+      HIPC(history_transport(p->P, p->T.ext_geometry, K, p->stream, sec_cap > 0, d_next,
+                             p->tuning));
     double sc[8];
     unsigned long long cn[8], nst[2] = {0, 0};
     int32_t err = 0;

@@ -561,22 +561,22 @@ static bool stage_pays(KS staged, KU unstaged, int bs, size_t cur, size_t blob) 
 }
 
 // `next` is a device counter (zeroed here) from which lanes take histories.
-template <int TRK, bool TRACE, bool SEC, bool SPLIT, bool HEX>
+template <int TRK, bool TRACE, bool SEC, bool SPLIT, bool EXT>
 static hipError_t history_transport_t(const DevProblem& P, const KCtx& K, hipStream_t s,
                                       unsigned long long* next, const abx_tuning& tu) {
   const int bs = tu.block_size;
   // Stage the tables in LDS when they fit next to the cursors.
   const size_t cur = cursor_lds_bytes(bs, P.trk_D, P.trk_L);
   const bool stage = tu.stage_tables &&
-                     stage_pays(k_history<TRK, TRACE, SEC, SPLIT, true, false, HEX>,
-                                k_history<TRK, TRACE, SEC, SPLIT, false, false, HEX>, bs, cur,
+                     stage_pays(k_history<TRK, TRACE, SEC, SPLIT, true, false, EXT>,
+                                k_history<TRK, TRACE, SEC, SPLIT, false, false, EXT>, bs, cur,
                                 (size_t)P.blob_bytes);
   const size_t lds = stage ? cur + (size_t)P.blob_bytes : cur;
   hipError_t e;
   if ((e = hipMemsetAsync(next, 0, sizeof(unsigned long long), s)) != hipSuccess) return e;
   const int64_t want = (K.n + bs - 1) / bs;
-  int64_t grid = stage ? resident_blocks(k_history<TRK, TRACE, SEC, SPLIT, true, false, HEX>, bs, lds)
-                       : resident_blocks(k_history<TRK, TRACE, SEC, SPLIT, false, false, HEX>, bs, lds);
+  int64_t grid = stage ? resident_blocks(k_history<TRK, TRACE, SEC, SPLIT, true, false, EXT>, bs, lds)
+                       : resident_blocks(k_history<TRK, TRACE, SEC, SPLIT, false, false, EXT>, bs, lds);
   if (tu.blocks_per_cu > 0) {
     int dev = 0, cus = 0;
     if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
@@ -586,10 +586,10 @@ static hipError_t history_transport_t(const DevProblem& P, const KCtx& K, hipStr
   }
   const int64_t blocks = std::min<int64_t>(want, grid);
   if (stage)
-    hipLaunchKernelGGL((k_history<TRK, TRACE, SEC, SPLIT, true, false, HEX>), dim3((unsigned)blocks), dim3(bs),
+    hipLaunchKernelGGL((k_history<TRK, TRACE, SEC, SPLIT, true, false, EXT>), dim3((unsigned)blocks), dim3(bs),
                        lds, s, P, K, next);
   else
-    hipLaunchKernelGGL((k_history<TRK, TRACE, SEC, SPLIT, false, false, HEX>), dim3((unsigned)blocks), dim3(bs),
+    hipLaunchKernelGGL((k_history<TRK, TRACE, SEC, SPLIT, false, false, EXT>), dim3((unsigned)blocks), dim3(bs),
                        lds, s, P, K, next);
   return hipGetLastError();
 }
@@ -631,10 +631,11 @@ static hipError_t dispatch_variant_t(int trk, bool trace, bool sec, bool split, 
              : fn(IntC<1>{}, BoolC<false>{}, BoolC<false>{}, BoolC<false>{});
 }
 
-// ... x hexagonal lattices: fn(T, TR, SE, SP, HX).
+// ... x extended geometry (hexagonal lattices, cylinders off the coordinate
+// axes): fn(T, TR, SE, SP, EX).
 template <class Fn>
-static hipError_t dispatch_variant(int trk, bool trace, bool sec, bool split, bool hex, Fn&& fn) {
-  if (hex)
+static hipError_t dispatch_variant(int trk, bool trace, bool sec, bool split, bool ext, Fn&& fn) {
+  if (ext)
     return dispatch_variant_t(trk, trace, sec, split, [&](auto T, auto TR, auto SE, auto SP) {
       return fn(T, TR, SE, SP, BoolC<true>{});
     });
@@ -646,7 +647,7 @@ static hipError_t dispatch_variant(int trk, bool trace, bool sec, bool split, bo
 static inline hipError_t event_transport(const DevProblem& P, const KCtx& K, EventBuffers& B,
                                          hipStream_t s, bool sec, bool timing, double* kernel_ms,
                                          int64_t* launches) {
-  // the event kernels always carry the hexagonal-lattice navigation
+  // the event kernels always carry the extended geometry
   return dispatch_variant_t(P.tracking, K.trace != nullptr, sec, P.precision == ABX_SPLIT,
                           [&](auto T, auto TR, auto SE, auto SP) {
                             return event_transport_t<decltype(T)::value, decltype(TR)::value,
@@ -655,15 +656,15 @@ static inline hipError_t event_transport(const DevProblem& P, const KCtx& K, Eve
                           });
 }
 
-// hex: the problem has hexagonal lattices (its tables' has_hex)
-static inline hipError_t history_transport(const DevProblem& P, bool hex, const KCtx& K,
+// ext: the problem needs the extended-geometry variants (its tables' ext_geometry)
+static inline hipError_t history_transport(const DevProblem& P, bool ext, const KCtx& K,
                                            hipStream_t s, bool sec, unsigned long long* next,
                                            const abx_tuning& tu) {
-  return dispatch_variant(P.tracking, K.trace != nullptr, sec, P.precision == ABX_SPLIT, hex,
-                          [&](auto T, auto TR, auto SE, auto SP, auto HX) {
+  return dispatch_variant(P.tracking, K.trace != nullptr, sec, P.precision == ABX_SPLIT, ext,
+                          [&](auto T, auto TR, auto SE, auto SP, auto EX) {
                             return history_transport_t<decltype(T)::value, decltype(TR)::value,
                                                        decltype(SE)::value, decltype(SP)::value,
-                                                       decltype(HX)::value>(P, K, s, next, tu);
+                                                       decltype(EX)::value>(P, K, s, next, tu);
                           });
 }
 

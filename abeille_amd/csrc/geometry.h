@@ -42,7 +42,19 @@ ABX_HD V3 vdir(double x, double y, double z) {
 ABX_HD double dabs(double x) { return fabs(x); }
 
 // ---------------------------------------------------------------- surfaces
+// EXT builds also evaluate Cylinder (src/cylinder.cpp:59-117) off the
+// coordinate axes, type ABX_CYL: the quadric alpha x^2 + beta y^2 +
+// gamma z^2 = R^2 about (x0, y0, z0), its p = {x0, y0, z0, R} and alpha,
+// beta, gamma in p[0..2] of a companion entry pad_[0] entries further on
+// in the same table (appended after the deck's surfaces).
+ABX_HD const DSurf& surf_aux(const DSurf& s) { return (&s)[s.pad_[0]]; }
+
+template <bool EXT>
 ABX_HD V3 surf_norm(const DSurf& s, const V3& r) {
+  if (EXT && s.type == ABX_CYL) {
+    const DSurf& q = surf_aux(s);
+    return vdir(q.p[0] * (r.x - s.p[0]), q.p[1] * (r.y - s.p[1]), q.p[2] * (r.z - s.p[2]));
+  }
   switch (s.type) {
     case 0: return vdir(1., 0., 0.);
     case 1: return vdir(0., 1., 0.);
@@ -56,12 +68,19 @@ ABX_HD V3 surf_norm(const DSurf& s, const V3& r) {
 }
 
 // A point on the surface (|eval| <= 1e-12) takes the sign of u . n.
+template <bool EXT>
 ABX_COLD int surf_sign_on(const DSurf& s, V3 r, V3 u) {
-  return vdot(u, surf_norm(s, r)) > 0. ? 1 : -1;
+  return vdot(u, surf_norm<EXT>(s, r)) > 0. ? 1 : -1;
 }
 
+template <bool EXT>
 ABX_HD int surf_sign(const DSurf& s, const V3& r, const V3& u) {
   double eval;
+  if (EXT && s.type == ABX_CYL) {
+    const DSurf& q = surf_aux(s);
+    const double x = r.x - s.p[0], y = r.y - s.p[1], z = r.z - s.p[2];
+    eval = q.p[0] * x * x + q.p[1] * y * y + q.p[2] * z * z - s.p[3] * s.p[3];
+  } else {
   switch (s.type) {
     case 0: eval = r.x - s.p[0]; break;
     case 1: eval = r.y - s.p[0]; break;
@@ -88,9 +107,10 @@ ABX_HD int surf_sign(const DSurf& s, const V3& r, const V3& u) {
       break;
     }
   }
+  }
   if (eval > ABX_SURFACE_COINCIDENT) return 1;
   if (eval < -ABX_SURFACE_COINCIDENT) return -1;
-  return surf_sign_on(s, r, u);
+  return surf_sign_on<EXT>(s, r, u);
 }
 
 ABX_HD double plane_axis_distance(double x0, double r0, double u0, bool on) {
@@ -112,7 +132,17 @@ ABX_HD double quad_distance(double a, double k, double c, bool on) {
   return d < 0. ? ABX_INF : d;
 }
 
+template <bool EXT>
 ABX_NOINL double surf_distance(const DSurf& s, const V3& r, const V3& u, bool on) {
+  if (EXT && s.type == ABX_CYL) {
+    const DSurf& q = surf_aux(s);
+    const double a = q.p[0] * u.x * u.x + q.p[1] * u.y * u.y + q.p[2] * u.z * u.z;
+    if (a == 0.) return ABX_INF;
+    const double x = r.x - s.p[0], y = r.y - s.p[1], z = r.z - s.p[2];
+    const double k = q.p[0] * x * u.x + q.p[1] * y * u.y + q.p[2] * z * u.z;
+    const double c = q.p[0] * x * x + q.p[1] * y * y + q.p[2] * z * z - s.p[3] * s.p[3];
+    return quad_distance(a, k, c, on);
+  }
   switch (s.type) {
     case 0: return plane_axis_distance(s.p[0], r.x, u.x, on);
     case 1: return plane_axis_distance(s.p[0], r.y, u.y, on);
@@ -162,6 +192,7 @@ ABX_NOINL double surf_distance(const DSurf& s, const V3& r, const V3& u, bool on
 // ------------------------------------------------------------------- cells
 ABX_HD int iabs(int x) { return x < 0 ? -x : x; }
 
+template <bool EXT>
 ABX_HD bool cell_inside(const DevProblem& P, int ci, const V3& r, const V3& u,
                         int32_t on_surf) {
   const DCell& c = P.cell[ci];
@@ -171,7 +202,7 @@ ABX_HD bool cell_inside(const DevProblem& P, int ci, const V3& r, const V3& u,
       const int32_t token = rpn[k];
       if (token == on_surf) continue;
       if (-token == on_surf) return false;
-      const int sign = surf_sign(P.surf[iabs(token) - 1], r, u);
+      const int sign = surf_sign<EXT>(P.surf[iabs(token) - 1], r, u);
       if ((sign > 0 && token < 0) || (sign < 0 && token > 0)) return false;
     }
     return true;
@@ -199,7 +230,7 @@ ABX_HD bool cell_inside(const DevProblem& P, int ci, const V3& r, const V3& u,
       else if (-token == on_surf)
         v = false;
       else {
-        const int sign = surf_sign(P.surf[iabs(token) - 1], r, u);
+        const int sign = surf_sign<EXT>(P.surf[iabs(token) - 1], r, u);
         v = (sign > 0 && token > 0) || (sign < 0 && token < 0);
       }
       st = (st & ~(1ull << i)) | ((uint64_t)(v ? 1u : 0u) << i);
@@ -215,8 +246,9 @@ struct DistTok {
 };
 
 // Cell::distance_to_boundary / distance_to_boundary_condition
+template <bool EXT>
 ABX_NOINL DistTok cell_distance(const DevProblem& P, int ci, const V3& r, const V3& u,
-                             int32_t on_surf, bool bc_only) {
+                                int32_t on_surf, bool bc_only) {
   const DCell& c = P.cell[ci];
   DistTok res{ABX_INF, 0};
   if (bc_only && !c.has_bc) return res;
@@ -227,7 +259,7 @@ ABX_NOINL DistTok cell_distance(const DevProblem& P, int ci, const V3& r, const 
     const bool coincident = iabs(token) == iabs(on_surf);
     const DSurf& s = P.surf[iabs(token) - 1];
     if (bc_only && s.bc == 2) continue;
-    const double d = surf_distance(s, r, u, coincident);
+    const double d = surf_distance<EXT>(s, r, u, coincident);
     if (d < res.d) {
       if (dabs(d - res.d) / res.d >= 1e-14) {
         res.d = d;
@@ -429,9 +461,9 @@ ABX_HD double hex_tile_distance(const DUniv& L, double rx, double ry, double rz,
 
 // Lattice::get_tile / distance_to_tile_boundary of either kind, from a
 // lattice pad's r_local (the frame the lattice was entered in).
-template <bool HEX>
+template <bool EXT>
 ABX_HD void lat_pad_tile(const DUniv& L, const V3& r, const V3& u, int t[3]) {
-  if (HEX && L.type == ABX_HEX_LATTICE) {
+  if (EXT && L.type == ABX_HEX_LATTICE) {
     const HexTile h = hex_tile(L, nullptr, r.x, r.y, r.z);
     t[0] = h.q;
     t[1] = h.r;
@@ -440,9 +472,9 @@ ABX_HD void lat_pad_tile(const DUniv& L, const V3& r, const V3& u, int t[3]) {
     get_tile(L, r, u, t);
   }
 }
-template <bool HEX>
+template <bool EXT>
 ABX_HD double lat_pad_distance(const DUniv& L, const V3& rl, const V3& u, const int32_t t[3]) {
-  if (HEX && L.type == ABX_HEX_LATTICE)
+  if (EXT && L.type == ABX_HEX_LATTICE)
     return hex_tile_distance(L, rl.x, rl.y, rl.z, u.x, u.y, u.z, t[0], t[1], t[2]);
   return tile_distance(L, rl, u, t);
 }
@@ -464,12 +496,13 @@ ABX_HD bool tile_eq(const int32_t a[3], const int b[3]) {
 // The cursor proper.  Pads and frames live outside the struct (per-thread
 // slices of LDS on the GPU, a small local buffer on the host) at a stride,
 // so that runtime nesting depth costs neither registers nor scratch.
-// HEX: whether the navigation built on this cursor handles hexagonal
-// lattices; kernels for problems without them are built without that code
-// (the host launches the HEX variants for every problem that has one).
-template <bool HEX>
+// EXT: whether the navigation built on this cursor handles the extended
+// geometry - hexagonal lattices and cylinders off the coordinate axes;
+// kernels for problems without them are built without that code (the host
+// launches the EXT variants for every problem that has one).
+template <bool EXT>
 struct TrkT {
-  static constexpr bool kHex = HEX;
+  static constexpr bool kExt = EXT;
   int32_t np;     // pads in use
   int32_t cell;   // material cell index, -1 = lost
   int32_t token;  // surface token the particle sits on
@@ -541,7 +574,7 @@ ABX_NOINL int trk_descend(const DevProblem& P, T& t, int ui, int lvl, V3 r, cons
       int found = -1;
       for (int k = 0; k < U.ncells; k++) {
         const int ci = P.ucells[U.cells_off + k];
-        if (cell_inside(P, ci, r, u, t.token)) {
+        if (cell_inside<T::kExt>(P, ci, r, u, t.token)) {
           found = ci;
           break;
         }
@@ -556,7 +589,7 @@ ABX_NOINL int trk_descend(const DevProblem& P, T& t, int ui, int lvl, V3 r, cons
     int tl[3];
     int elem = -1;
     bool hex = false;
-    if (T::kHex && U.type == ABX_HEX_LATTICE) {
+    if (T::kExt && U.type == ABX_HEX_LATTICE) {
       // HexLattice::get_cell (src/hex_lattice.cpp:143-217): the tile of the
       // origin-relative point; the pad and the outer universe keep r
       const HexTile h = hex_tile(U, P.elems, r.x - U.Xl, r.y - U.Yl, r.z - U.Zl);
@@ -578,7 +611,7 @@ ABX_NOINL int trk_descend(const DevProblem& P, T& t, int ui, int lvl, V3 r, cons
       }
       // the element's frame: (origin-relative point) - tile centre for a
       // hexagonal lattice
-      if (T::kHex && hex)
+      if (T::kExt && hex)
         r = vsub(vsub(r, V3{U.Xl, U.Yl, U.Zl}), hex_tile_center(U, tl[0], tl[1], tl[2]));
       else
         r = vsub(r, tile_center(U, tl[0], tl[1], tl[2]));
@@ -642,13 +675,13 @@ ABX_HD void trk_get_current(const DevProblem& P, T& t, const V3& u) {
     const int32_t w = t.pad(i);
     const int ty = pad_type(w);
     if (ty == PAD_C) {
-      if (!cell_inside(P, pad_idx(w), t.lv(pad_lvl(w)), u, t.token)) {
+      if (!cell_inside<T::kExt>(P, pad_idx(w), t.lv(pad_lvl(w)), u, t.token)) {
         first_bad = i;
         break;
       }
     } else if (ty == PAD_L) {
       int tl[3];
-      lat_pad_tile<T::kHex>(P.univ[pad_idx(w)], t.lv(pad_lvl(w)), u, tl);
+      lat_pad_tile<T::kExt>(P.univ[pad_idx(w)], t.lv(pad_lvl(w)), u, tl);
       if (!t.tile_is(i, tl)) {
         first_bad = i;
         break;
@@ -698,12 +731,12 @@ struct Bound {
 };
 
 // Universe::get_boundary_condition (cell universe / lattice via outer)
-template <bool HEX>
+template <bool EXT>
 ABX_NOINL Bound univ_bc(const DevProblem& P, int ui, const V3& r, const V3& u, int32_t on) {
   Bound b{ABX_INF, -1, 0, 0};
   for (int guard = 0; guard < 16; guard++) {
     const DUniv& U = P.univ[ui];
-    if (HEX ? U.type != ABX_CELL_UNIVERSE : U.type == ABX_RECT_LATTICE) {
+    if (EXT ? U.type != ABX_CELL_UNIVERSE : U.type == ABX_RECT_LATTICE) {
       if (!U.has_bc) return b;
       ui = U.outer;
       continue;
@@ -712,7 +745,7 @@ ABX_NOINL Bound univ_bc(const DevProblem& P, int ui, const V3& r, const V3& u, i
       for (int k = 0; k < U.ncells; k++) {
         const int ci = P.ucells[U.cells_off + k];
         if (!P.cell[ci].has_bc) continue;
-        const DistTok dt = cell_distance(P, ci, r, u, on, true);
+        const DistTok dt = cell_distance<EXT>(P, ci, r, u, on, true);
         if (dt.d < b.d && dabs(dt.d - b.d) > ABX_BOUNDRY_TOL) {
           const int32_t tt = iabs(dt.tok);
           if (!tt) continue;
@@ -720,7 +753,7 @@ ABX_NOINL Bound univ_bc(const DevProblem& P, int ui, const V3& r, const V3& u, i
           b.d = dt.d;
           b.sidx = tt - 1;
           b.btype = P.surf[b.sidx].bc;
-          if (surf_sign(P.surf[b.sidx], r, u) < 0) b.tok = -b.tok;
+          if (surf_sign<EXT>(P.surf[b.sidx], r, u) < 0) b.tok = -b.tok;
         }
       }
     }
@@ -732,7 +765,7 @@ ABX_NOINL Bound univ_bc(const DevProblem& P, int ui, const V3& r, const V3& u, i
 // Tracker::get_boundary_condition
 template <class T>
 ABX_NOINL Bound trk_bc(const DevProblem& P, const T& t, const V3& u) {
-  if (t.cell < 0) return univ_bc<T::kHex>(P, P.root, t.lv(0), u, t.token);
+  if (t.cell < 0) return univ_bc<T::kExt>(P, P.root, t.lv(0), u, t.token);
   Bound b{ABX_INF, -1, 0, 0};
   for (int i = 0; i < t.np; i++) {
     const int32_t w = t.pad(i);
@@ -740,7 +773,7 @@ ABX_NOINL Bound trk_bc(const DevProblem& P, const T& t, const V3& u) {
     if (pad_type(w) == PAD_C) {
       const int ci = pad_idx(w);
       if (!P.cell[ci].has_bc) continue;
-      const DistTok dt = cell_distance(P, ci, rl, u, t.token, true);
+      const DistTok dt = cell_distance<T::kExt>(P, ci, rl, u, t.token, true);
       if (dt.d < b.d && dabs(dt.d - b.d) > ABX_BOUNDRY_TOL) {
         const int32_t tt = iabs(dt.tok);
         if (!tt) continue;
@@ -748,12 +781,12 @@ ABX_NOINL Bound trk_bc(const DevProblem& P, const T& t, const V3& u) {
         b.tok = tt;
         b.sidx = tt - 1;
         b.btype = P.surf[b.sidx].bc;
-        if (surf_sign(P.surf[b.sidx], rl, u) < 0) b.tok = -b.tok;
+        if (surf_sign<T::kExt>(P.surf[b.sidx], rl, u) < 0) b.tok = -b.tok;
       }
     } else {
       const int ui = pad_idx(w);
       if (P.univ[ui].has_bc) {
-        const Bound ub = univ_bc<T::kHex>(P, ui, rl, u, t.token);
+        const Bound ub = univ_bc<T::kExt>(P, ui, rl, u, t.token);
         if (ub.d < b.d && dabs(ub.d - b.d) > ABX_BOUNDRY_TOL) b = ub;
       }
     }
@@ -776,7 +809,7 @@ ABX_NOINL Bound trk_nearest(const DevProblem& P, const T& t, const V3& u) {
     const int ty = pad_type(w);
     if (ty == PAD_L) {
       const int32_t tl[3] = {t.tile(i, 0), t.tile(i, 1), t.tile(i, 2)};
-      const double d = lat_pad_distance<T::kHex>(P.univ[pad_idx(w)], rl, u, tl);
+      const double d = lat_pad_distance<T::kExt>(P.univ[pad_idx(w)], rl, u, tl);
       if (d < b.d && dabs(d - b.d) > ABX_BOUNDRY_TOL) {
         b.d = d;
         b.btype = 2;
@@ -784,13 +817,13 @@ ABX_NOINL Bound trk_nearest(const DevProblem& P, const T& t, const V3& u) {
         b.tok = 0;
       }
     } else if (ty == PAD_C) {
-      const DistTok dt = cell_distance(P, pad_idx(w), rl, u, t.token, false);
+      const DistTok dt = cell_distance<T::kExt>(P, pad_idx(w), rl, u, t.token, false);
       if (dt.d < b.d && dabs(dt.d - b.d) > ABX_BOUNDRY_TOL) {
         b.d = dt.d;
         b.tok = iabs(dt.tok);
         b.sidx = b.tok ? b.tok - 1 : -1;
         b.btype = b.sidx >= 0 ? P.surf[b.sidx].bc : 2;
-        if (b.sidx >= 0 && surf_sign(P.surf[b.sidx], rl, u) < 0) b.tok = -b.tok;
+        if (b.sidx >= 0 && surf_sign<T::kExt>(P.surf[b.sidx], rl, u) < 0) b.tok = -b.tok;
       }
     }
   }

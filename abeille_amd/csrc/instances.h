@@ -1,7 +1,7 @@
 // instances.h - the compiled variants of the transport kernels.
 //
 // Every (tracking, trace, secondary stack, precision, LDS staging,
-// hexagonal lattices) variant of the big transport kernels is instantiated
+// extended geometry) variant of the big transport kernels is instantiated
 // once, in one of the parts of
 // instances.hip (each part is its own translation unit, compiled in
 // parallel).  Everywhere else the variants are `extern`, so the host code of
@@ -25,22 +25,22 @@
 #define ABX_BATCH_T1(X) X(1, false, false) X(1, true, false) X(1, false, true) X(1, true, true)
 #define ABX_BATCH_T2(X) X(2, true, false) X(2, true, true)
 
-#define ABX_HIST_DECL(EXT, T, TR, SE, SP, HX)                                                 \
-  EXT template __global__ void k_history<T, TR, SE, SP, true, false, HX>(DevProblem, KCtx,    \
+#define ABX_HIST_DECL(EXT, T, TR, SE, SP, EX)                                                 \
+  EXT template __global__ void k_history<T, TR, SE, SP, true, false, EX>(DevProblem, KCtx,    \
                                                                          unsigned long long*); \
-  EXT template __global__ void k_history<T, TR, SE, SP, false, false, HX>(DevProblem, KCtx,   \
+  EXT template __global__ void k_history<T, TR, SE, SP, false, false, EX>(DevProblem, KCtx,   \
                                                                           unsigned long long*);
 // replica mode: k-eigenvalue chains (no trace, no secondary stack)
-#define ABX_REP_DECL(EXT, T, SP, HX)                                                          \
-  EXT template __global__ void k_history<T, false, false, SP, true, true, HX>(                \
+#define ABX_REP_DECL(EXT, T, SP, EX)                                                          \
+  EXT template __global__ void k_history<T, false, false, SP, true, true, EX>(                \
       DevProblem, KCtx, unsigned long long*);                                                 \
-  EXT template __global__ void k_history<T, false, false, SP, false, true, HX>(               \
+  EXT template __global__ void k_history<T, false, false, SP, false, true, EX>(               \
       DevProblem, KCtx, unsigned long long*);
 #define ABX_REP_ALL(X) X(0, false) X(0, true) X(1, false) X(1, true)
-#define ABX_BATCH_DECL(EXT, T, SE, SP, HX)                                                    \
-  EXT template __global__ void k_batch_chunks<T, SE, SP, true, HX>(                           \
+#define ABX_BATCH_DECL(EXT, T, SE, SP, EX)                                                    \
+  EXT template __global__ void k_batch_chunks<T, SE, SP, true, EX>(                           \
       const DevProblem*, const KCtx*, const BatchChunk*, int, int, int, unsigned*);           \
-  EXT template __global__ void k_batch_chunks<T, SE, SP, false, HX>(                          \
+  EXT template __global__ void k_batch_chunks<T, SE, SP, false, EX>(                          \
       const DevProblem*, const KCtx*, const BatchChunk*, int, int, int, unsigned*);
 #define ABX_EVENT_DECL(EXT, T, TR, SE, SP)                                                    \
   EXT template __global__ void k_ev_flight<T, TR, SE, SP>(DevProblem, KCtx, EvState,          \

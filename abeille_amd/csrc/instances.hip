@@ -34,7 +34,7 @@ ABX_VARIANTS_T0(ABX_IE)
 ABX_VARIANTS_T1(ABX_IE)
 #elif ABX_INST_PART == 7
 ABX_VARIANTS_T2(ABX_IE)
-// hexagonal-lattice variants
+// extended-geometry variants (hexagonal lattices, off-axis cylinders)
 #elif ABX_INST_PART == 8
 ABX_VARIANTS_T0(ABX_IHX)
 #elif ABX_INST_PART == 9

@@ -347,10 +347,11 @@ __device__ __forceinline__ DevProblem stage_tables(const DevProblem& P) {
 // so that each variant's loads carry their address space.
 // REP: replica mode (K.rep_start and friends set), a variant of its own so
 // that the plain kernels carry none of its code.
-// HEX: the problem has hexagonal lattices; likewise its own variant, so
-// that the kernels of other problems carry none of that navigation.
+// EXT: the problem has hexagonal lattices or cylinders off the coordinate
+// axes; likewise its own variant, so that the kernels of other problems
+// carry none of that geometry.
 template <int TRK, bool TRACE, bool SEC, bool SPLIT, bool STAGE, bool REP = false,
-          bool HEX = false>
+          bool EXT = false>
 __global__ void ABX_TKERNEL k_history(DevProblem P0, KCtx K, unsigned long long* next) {
   const DevProblem P = STAGE ? stage_tables(P0) : P0;
   const int64_t n_hist = K.n_dev ? (int64_t)*K.n_dev : K.n;
@@ -380,7 +381,7 @@ __global__ void ABX_TKERNEL k_history(DevProblem P0, KCtx K, unsigned long long*
       __hip_atomic_store(s.srange + 1, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
     }
   }
-  TrkT<HEX> t;
+  TrkT<EXT> t;
   bind_lds_cursor(t, P);
   Part p;
   p.alive = 0;
@@ -494,7 +495,7 @@ struct BatchChunk {
 // registers and the table reads are LDS reads.  A block waits at a chunk's
 // end for its longest history; chunks of several thousand histories keep
 // that wait small.
-template <int TRK, bool SEC, bool SPLIT, bool STAGE, bool HEX = false>
+template <int TRK, bool SEC, bool SPLIT, bool STAGE, bool EXT = false>
 __global__ void ABX_TKERNEL k_batch_chunks(const DevProblem* __restrict__ probs,
                                            const KCtx* __restrict__ ctxs,
                                            const BatchChunk* __restrict__ chunks, int n_chunks,
@@ -502,7 +503,7 @@ __global__ void ABX_TKERNEL k_batch_chunks(const DevProblem* __restrict__ probs,
   __shared__ int s_chunk;
   __shared__ unsigned long long s_next;
   const int lane = threadIdx.x & 63;
-  TrkT<HEX> t;
+  TrkT<EXT> t;
   trk_bind(t, abx_lds, blockDim.x, threadIdx.x, D, L);
   unsigned char* tables = abx_lds + cursor_lds_bytes(blockDim.x, D, L);
   int staged = -1;

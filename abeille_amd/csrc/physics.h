@@ -533,7 +533,7 @@ ABX_HD void reflect(const DevProblem& P, TK& t, Part& p, const Bound& b) {
   }
   const DSurf& S = P.surf[b.sidx];
   const V3 r_on = vadd(p.r, vscale(p.u, b.d));
-  const V3 n = surf_norm(S, r_on);
+  const V3 n = surf_norm<TK::kExt>(S, r_on);
   const V3 nd = vsub(p.u, vscale(n, 2. * vdot(p.u, n)));
   p.r = r_on;
   p.u = vdir(nd.x, nd.y, nd.z);

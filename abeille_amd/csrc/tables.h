@@ -52,7 +52,7 @@ struct HostTables {
   int64_t tally_bins = 0;
   std::vector<abx_source> sources;
   int32_t max_pads = 1, max_levels = 1;  // cursor depth bound over all descents
-  bool has_hex = false;                   // any hexagonal lattice (the kernels' HEX variants)
+  bool ext_geometry = false;  // hexagonal lattices or off-axis cylinders: the EXT kernels
   abx_settings st{};
   double min_energy = 0., max_energy = 100000.;
 };
@@ -81,6 +81,7 @@ static inline void build_tables(const abx_problem_desc* d, HostTables& T) {
 
   // surfaces
   tables_require(d->n_surfaces >= 1, "no surfaces");
+  std::vector<std::pair<int, DSurf>> aux;  // companions of off-axis cylinders
   for (int i = 0; i < d->n_surfaces; i++) {
     const abx_surface& a = d->surfaces[i];
     tables_require(a.type >= 0 && a.type <= 8, "surface type");
@@ -95,8 +96,11 @@ static inline void build_tables(const abx_problem_desc* d, HostTables& T) {
       // gamma are exactly 1 and the third exactly 0, and its sign, distance
       // and normal evaluate bit for bit as the x/y/z-cylinder's (the zero
       // terms add nothing, the others add in either order), so it is
-      // stored as one.  Off the axes the reference's quadric is not a
-      // circular cylinder; the device path does not carry it.
+      // stored as one.  Off the axes (the reference's quadric is then not a
+      // circular cylinder, but it is what the reference evaluates) it stays
+      // ABX_CYL with p = {x0, y0, z0, R} and alpha, beta, gamma in a
+      // companion entry appended after the deck's surfaces; only the
+      // extended-geometry kernels evaluate it.
       double u0 = a.p[3], v0 = a.p[4], w0 = a.p[5];
       const double mag = sqrt(u0 * u0 + v0 * v0 + w0 * w0);
       tables_require(mag != 0., "Direction vector of Cylinder can not have a mangitude of 0.");
@@ -114,11 +118,23 @@ static inline void build_tables(const abx_problem_desc* d, HostTables& T) {
         S.type = ABX_ZCYL;
         S.p[0] = a.p[0], S.p[1] = a.p[1], S.p[2] = R, S.p[3] = 0.;
       } else {
-        tables_require(false, "Cylinder: only an axis along x, y or z is supported on the GPU path");
+        S.p[0] = a.p[0], S.p[1] = a.p[1], S.p[2] = a.p[2], S.p[3] = R;
+        DSurf Q{};
+        Q.type = ABX_CYL;
+        Q.bc = ABX_NORMAL;
+        Q.p[0] = 1. - u0 * u0;
+        Q.p[1] = 1. - v0 * v0;
+        Q.p[2] = 1. - w0 * w0;
+        aux.push_back({i, Q});
+        T.ext_geometry = true;
       }
     }
     S.bc = a.boundary;
     T.surf.push_back(S);
+  }
+  for (const auto& q : aux) {
+    T.surf[q.first].pad_[0] = (int32_t)T.surf.size() - q.first;
+    T.surf.push_back(q.second);
   }
   // cells
   for (int i = 0; i < d->n_cells; i++) {
@@ -198,7 +214,7 @@ static inline void build_tables(const abx_problem_desc* d, HostTables& T) {
       tables_require(a.shape[0] <= 4096 && a.shape[1] <= (1 << 20), "hex lattice size");
       const uint32_t rings = (uint32_t)a.shape[0], width = 2 * (rings - 1) + 1;
       const uint32_t mid = width / 2;
-      T.has_hex = true;
+      T.ext_geometry = true;
       U.rings = (int32_t)rings;
       U.top = a.hex_top;
       U.Nx = U.Ny = (int32_t)width;

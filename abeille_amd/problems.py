@@ -99,7 +99,7 @@ def cylinder_cube(nparticles=25000, tracking="surface", axis=(0., 2., 0.), radiu
                   centre=(0.5, 3.0, -0.5)):
     """The 20 cm vacuum cube with a general Cylinder (src/cylinder.cpp) of
     another, more absorbing material through it; `axis` need not be
-    normalised (the GPU path takes coordinate axes only).  Point source at
+    normalised.  Point source at
     the centre, inside the cylinder; collision-estimator flux on 20^3 x 2."""
     m = Model()
     s = [m.surface("xplane", [-10.], "vacuum"), m.surface("xplane", [10.], "vacuum"),
@@ -238,12 +238,16 @@ def hex_assembly(nparticles=20000, tracking="surface", top="pointy", rings=4, nz
     u_fuel, u_gt = _pin_universes(m, fuel, mod, gt, cyl)
     p, pz = 1.26, 10.0
     hx, hz = rings * p + 0.5, nz * pz / 2.0
+    # the lattice's layers sit at 2 * origin[2] +- hz (HexLattice::get_tile
+    # counts them from the origin-relative point, src/hex_lattice.cpp:
+    # 303-307); the box spans them and the nominal ones
+    zlo, zhi = min(origin[2], 2 * origin[2]) - hz, max(origin[2], 2 * origin[2]) + hz
     b = [m.surface("xplane", [origin[0] - hx], "vacuum", id=11),
          m.surface("xplane", [origin[0] + hx], "vacuum", id=12),
          m.surface("yplane", [origin[1] - hx], "reflective", id=13),
          m.surface("yplane", [origin[1] + hx], "reflective", id=14),
-         m.surface("zplane", [origin[2] - hz], "reflective", id=15),
-         m.surface("zplane", [origin[2] + hz], "reflective", id=16)]
+         m.surface("zplane", [zlo], "reflective", id=15),
+         m.surface("zplane", [zhi], "reflective", id=16)]
     cbox = m.cell(f"+{b[0]} & -{b[1]} & +{b[2]} & -{b[3]} & +{b[4]} & -{b[5]}",
                   material=mod, id=105)
     u_out = m.cell_universe([cbox], id=3)
@@ -261,13 +265,11 @@ def hex_assembly(nparticles=20000, tracking="surface", top="pointy", rings=4, nz
             layout[(q, r, iz)] = u
             elems.append(u)
     m.root = m.hex_lattice(rings, nz, p, pz, list(origin), elems, top=top, outer=u_out, id=10)
-    m.box_source([origin[0] - hx, origin[1] - hx, origin[2] - hz],
-                 [origin[0] + hx, origin[1] + hx, origin[2] + hz], energy=0.5,
-                 fissile_only=True)
+    m.box_source([origin[0] - hx, origin[1] - hx, zlo], [origin[0] + hx, origin[1] + hx, zhi],
+                 energy=0.5, fissile_only=True)
     if tally:
-        m.mesh_tally([origin[0] - hx, origin[1] - hx, origin[2] - hz],
-                     [origin[0] + hx, origin[1] + hx, origin[2] + hz], [24, 24, 2],
-                     np.arange(G + 1.0), estimator="track-length")
+        m.mesh_tally([origin[0] - hx, origin[1] - hx, zlo], [origin[0] + hx, origin[1] + hx, zhi],
+                     [24, 24, 2], np.arange(G + 1.0), estimator="track-length")
     m.set(mode=mode, nparticles=nparticles, energy_bounds=np.arange(G + 1.0),
           tracking=_tracking(tracking))
     return m, layout

@@ -29,6 +29,11 @@ CASES = {
                                                centre=(0., 0.5, -0.5)),
     "cyl_z_st": lambda: problems.cylinder_cube(600, tracking="surface", axis=(0., 0., -1.),
                                                centre=(0.5, -0.5, 2.)),
+    "cyl_oblique_st": lambda: problems.cylinder_cube(600, tracking="surface", axis=(1., 1., 2.)),
+    "cyl_oblique_dt": lambda: problems.cylinder_cube(600, tracking="delta", axis=(-2., 1., 0.5)),
+    "hex_flat_st": lambda: problems.hex_assembly(600, tracking="surface", top="flat")[0],
+    "hex_pointy_dt": lambda: problems.hex_assembly(600, tracking="delta", top="pointy",
+                                                   origin=(0.3, -0.2, 1.0))[0],
 }
 
 

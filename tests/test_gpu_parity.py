@@ -27,14 +27,13 @@ CASES = {
     "slab_mfs_delta": lambda: problems.subcritical_slab(2000, tracking="delta", cancel=False),
     "slab_p2_scatter": lambda: problems.anisotropic_slab(2000, tracking="surface"),
     "cube_cylinder_st": lambda: problems.cylinder_cube(2000, tracking="surface"),
+    # a general Cylinder off the coordinate axes: the reference's quadric
+    # alpha x^2 + beta y^2 + gamma z^2 = R^2, on the extended-geometry kernels
+    "cube_cylinder_oblique_st": lambda: problems.cylinder_cube(2000, tracking="surface",
+                                                               axis=(1., 1., 2.)),
+    "cube_cylinder_oblique_dt": lambda: problems.cylinder_cube(2000, tracking="delta",
+                                                               axis=(-2., 1., 0.5)),
 }
-
-
-def test_oblique_cylinder_rejected():
-    """A general Cylinder off the coordinate axes is the reference's
-    non-circular quadric; the device path refuses it at creation."""
-    with pytest.raises(Exception, match="Cylinder"):
-        Problem(problems.cylinder_cube(10, axis=(1., 1., 2.)))
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
