@@ -54,9 +54,9 @@ enum abx_surface_type {
                      The reference's surface is the quadric
                      a x^2 + b y^2 + c z^2 = R^2 with a = 1 - u^2, b = 1 - v^2,
                      c = 1 - w^2 of the normalised axis (u, v, w): a circular
-                     cylinder when the axis is a coordinate axis, which is the
-                     case abx_create accepts (stored as the x/y/z-cylinder it
-                     equals bit for bit); other axes give ABX_E_INVALID. */
+                     cylinder when the axis is a coordinate axis (stored as the
+                     x/y/z-cylinder it equals bit for bit), the same quadric
+                     evaluated as the reference does for any other axis. */
 };
 
 /* boundary types: include/geometry/surfaces/surface.hpp:34 (same order) */
