@@ -61,6 +61,10 @@ def lib():
         L.abx_request_stop.argtypes = [P]
         L.abx_set_max_time.argtypes = [P, C.c_double]
         L.abx_cancel_bank.argtypes = [P, C.c_void_p, C.c_size_t]
+        L.abx_parent_fetch.argtypes = [P, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]
+        L.abx_exact_cancel_bank.argtypes = [P, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
+                                            C.c_size_t, C.POINTER(C.c_size_t)]
+        L.abx_rng_reset.argtypes = [P]
         L.abx_bank_size.argtypes = [P, C.POINTER(C.c_size_t)]
         L.abx_bank_fetch.argtypes = [P, C.c_void_p]
         L.abx_bank_load.argtypes = [P, C.c_void_p, C.c_size_t]
@@ -257,6 +261,37 @@ class Problem:
         b = np.ascontiguousarray(bank, dtype=abi.BANKED_DTYPE).copy()
         _check(lib().abx_cancel_bank(self.h, b.ctypes.data, len(b)))
         return b
+
+    def parents(self, cap=None):
+        """abx_parent_fetch: the parent records of the last transport's
+        fission sites, in their order."""
+        n = C.c_size_t(0)
+        if cap is None:  # ask for the count first (ABX_E_CAPACITY = -5 then)
+            rc = lib().abx_parent_fetch(self.h, None, 0, C.byref(n))
+            if rc not in (0, -5):
+                _check(rc)
+            cap = n.value
+        out = np.zeros(max(1, cap), abi.PARENT_DTYPE)
+        _check(lib().abx_parent_fetch(self.h, out.ctypes.data, cap, C.byref(n)))
+        return out[: n.value]
+
+    def exact_cancel(self, bank, parents, cap=None):
+        """abx_exact_cancel_bank: the bank with its cancelled weights, then
+        the uniform particles."""
+        b = np.ascontiguousarray(bank, dtype=abi.BANKED_DTYPE)
+        q = np.ascontiguousarray(parents, dtype=abi.PARENT_DTYPE)
+        if len(q) != len(b):
+            raise ValueError("one parent record per site")
+        cap = cap if cap is not None else 4 * len(b) + 1024
+        out = np.zeros(max(1, cap), abi.BANKED_DTYPE)
+        n = C.c_size_t(0)
+        _check(lib().abx_exact_cancel_bank(self.h, b.ctypes.data, q.ctypes.data, len(b),
+                                           out.ctypes.data, cap, C.byref(n)))
+        return out[: n.value]
+
+    def rng_reset(self):
+        """abx_rng_reset: settings::rng back to the seed."""
+        _check(lib().abx_rng_reset(self.h))
 
     def bank(self):
         n = C.c_size_t()

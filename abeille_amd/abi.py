@@ -106,6 +106,19 @@ class CancelMesh(C.Structure):
                 ("n_ebounds", C.c_int32), ("ebounds", PD)]
 
 
+EXACT_MG, BASIC_EXACT_MG = 1, 2
+BETA_ZERO, BETA_MINIMUM, BETA_AVERAGE_F, BETA_AVERAGE_G = 0, 1, 2, 3
+
+
+class ExactCancel(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("shape", C.c_int32 * 3),
+                ("low", C.c_double * 3), ("high", C.c_double * 3),
+                ("n_samples", C.c_int32), ("chi_matrix", C.c_int32),
+                ("use_virtual_collisions", C.c_int32), ("beta", C.c_int32),
+                ("sobol", C.c_int32), ("n_group_bins", C.c_int32),
+                ("group_bin_offsets", PI32), ("group_bins", PI32)]
+
+
 class ProblemDesc(C.Structure):
     _fields_ = [("n_surfaces", C.c_int32), ("surfaces", C.POINTER(Surface)),
                 ("n_cells", C.c_int32), ("cells", C.POINTER(Cell)),
@@ -117,7 +130,8 @@ class ProblemDesc(C.Structure):
                 ("n_materials", C.c_int32), ("materials", C.POINTER(Material)),
                 ("n_sources", C.c_int32), ("sources", C.POINTER(Source)),
                 ("n_tallies", C.c_int32), ("tallies", C.POINTER(MeshTally)),
-                ("settings", Settings), ("cancel", C.POINTER(CancelMesh))]
+                ("settings", Settings), ("cancel", C.POINTER(CancelMesh)),
+                ("exact_cancel", C.POINTER(ExactCancel))]
 
 
 class Particle(C.Structure):
@@ -173,6 +187,8 @@ PARTICLE_DTYPE = np.dtype([("r", "<f8", 3), ("u", "<f8", 3), ("E", "<f8"), ("wgt
 BANKED_DTYPE = np.dtype([("r", "<f8", 3), ("u", "<f8", 3), ("E", "<f8"), ("wgt", "<f8"),
                          ("wgt2", "<f8"), ("parent_history_id", "<u8"),
                          ("parent_daughter_id", "<u8"), ("family_id", "<u8")])
+PARENT_DTYPE = np.dtype([("r1", "<f8", 3), ("u1", "<f8", 3), ("E1", "<f8"), ("E3", "<f8"),
+                         ("Esmp", "<f8"), ("prev_virtual", "<i4"), ("pad_", "<i4")])
 TRACE_DTYPE = np.dtype([("r", "<f8", 3), ("u", "<f8", 3), ("wgt", "<f8"),
                         ("history_id", "<u8"), ("secondary_id", "<i4"), ("event", "<i4"),
                         ("cell", "<i4"), ("material", "<i4"), ("group", "<i4"),

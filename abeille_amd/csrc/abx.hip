@@ -20,6 +20,7 @@
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include "event.h"
+#include "exact_kernels.h"
 #include "kernels.h"
 #include "tables.h"
 
@@ -98,6 +99,13 @@ struct abx_problem {
   int batch_stage = -1;  // batch kernel: tables staged in LDS pay off (-1 = not yet known)
   DevCancel cm{};
   DBuf ckeys, ckeys2, ctemp;
+  // exact cancellation: configuration, the parent record of each staged and
+  // ordered fission site, the pipeline's buffers, and settings::rng
+  int exact = 0;
+  DevExact dex{};
+  DBuf stage_pinfo, ordered_pinfo, ex_head, ex_offs, ex_bins, ex_smp, ex_fv, ex_fi, ex_cnt,
+      ex_state;
+  uint64_t grng = 0;
   // driver state
   DBuf bank_a, bank_b;  // abx_particle banks
   size_t bank_n = 0;
@@ -145,7 +153,8 @@ struct abx_problem {
                    &stage_daughter, &n_stage, &daughters, &offsets, &block_sums, &block_sums2,
                    &ordered, &scores, &counters, &err, &err_hist, &trace, &n_rec, &wsum,
                    &bank_a, &bank_b, &ckeys, &ckeys2, &ctemp, &pool_next, &blob, &gdev,
-                   &khist_dev};
+                   &khist_dev, &stage_pinfo, &ordered_pinfo, &ex_head, &ex_offs, &ex_bins,
+                   &ex_smp, &ex_fv, &ex_fi, &ex_cnt, &ex_state};
     for (auto e : gen_events) (void)hipEventDestroy(e);
     for (DBuf* b : all) b->release();
     ev.release();
@@ -255,6 +264,10 @@ KCtx make_kctx(abx_problem* p, const abx_particle* d_in, int64_t n, double kcol,
   K.n_hole = cs.n_hole;
   K.stage_cap = stage_cap;
   K.daughters = banks ? p->daughters.as<uint32_t>() : nullptr;
+  if (banks && p->exact) {
+    p->stage_pinfo.reserve((size_t)stage_cap * sizeof(abx_parent_info));
+    K.stage_pinfo = p->stage_pinfo.as<abx_parent_info>();
+  }
   K.scores = cs.scores;
   K.counters = cs.counters;
   K.err = cs.err;
@@ -285,8 +298,10 @@ TransportOut collect_scalars(const double* sc, const unsigned long long* cn, int
 
 // Orders the staged fission sites of a call over n parents into p->ordered
 // (parent order, then push order); returns their number.  n_staged staging
-// slots were handed out, n_holes of them hold no site.
-int64_t order_fission_bank(abx_problem* p, int64_t n, uint64_t n_staged, uint64_t n_holes) {
+// slots were handed out, n_holes of them hold no site.  With `pinfo` the
+// sites' parent records go from p->stage_pinfo to p->ordered_pinfo alike.
+int64_t order_fission_bank(abx_problem* p, int64_t n, uint64_t n_staged, uint64_t n_holes,
+                           bool pinfo = false) {
   const int64_t stage_cap = 4 * n + 4096;
   if ((int64_t)n_staged > stage_cap) fail(ABX_E_CAPACITY, "fission staging overflow");
   p->offsets.reserve((size_t)std::max<int64_t>(1, n) * sizeof(uint64_t));
@@ -294,6 +309,7 @@ int64_t order_fission_bank(abx_problem* p, int64_t n, uint64_t n_staged, uint64_
   if (n_holes > n_staged || total != n_staged - n_holes)
     fail(ABX_E_INVALID, "fission site count mismatch");
   p->ordered.reserve((size_t)std::max<uint64_t>(1, total) * sizeof(abx_banked));
+  if (pinfo) p->ordered_pinfo.reserve((size_t)std::max<uint64_t>(1, total) * sizeof(abx_parent_info));
   if (total) {
     p->err.reserve(sizeof(int32_t));
     HIPC(hipMemsetAsync(p->err.p, 0, sizeof(int32_t), p->stream));
@@ -301,7 +317,9 @@ int64_t order_fission_bank(abx_problem* p, int64_t n, uint64_t n_staged, uint64_
                        0, p->stream, p->stage.as<abx_banked>(), p->stage_parent.as<uint32_t>(),
                        p->stage_daughter.as<uint32_t>(), p->offsets.as<uint64_t>(),
                        (int64_t)n_staged, n, total, p->ordered.as<abx_banked>(),
-                       p->err.as<int32_t>());
+                       p->err.as<int32_t>(),
+                       pinfo ? (const abx_parent_info*)p->stage_pinfo.p : nullptr,
+                       pinfo ? p->ordered_pinfo.as<abx_parent_info>() : nullptr);
     HIPC(hipGetLastError());
     int32_t err = 0;
     HIPC(hipMemcpyAsync(&err, p->err.p, sizeof(int32_t), hipMemcpyDeviceToHost, p->stream));
@@ -329,6 +347,8 @@ TransportOut transport_device(abx_problem* p, const abx_particle* d_in, int64_t 
   const HostTables& T = p->T;
   if (d_trace && T.st.precision == ABX_SPLIT)
     fail(ABX_E_INVALID, "event traces are recorded in FP64 precision only");
+  if (p->exact && p->engine != ABX_ENGINE_HISTORY)
+    fail(ABX_E_INVALID, "exact cancellation needs the history engine (parent records)");
   p->scores.reserve(8 * sizeof(double));
   p->counters.reserve(8 * sizeof(unsigned long long));
   p->err.reserve(sizeof(int32_t));
@@ -362,7 +382,7 @@ TransportOut transport_device(abx_problem* p, const abx_particle* d_in, int64_t 
     if (p->engine == ABX_ENGINE_HISTORY) {
       if (p->timing) HIPC(hipEventRecord(p->ke0, p->stream));
       p->pool_next.reserve(sizeof(unsigned long long));
-      HIPC(history_transport(p->P, p->T.ext_geometry, K, p->stream, sec_cap > 0,
+      HIPC(history_transport(p->P, p->T.extended, K, p->stream, sec_cap > 0,
                              p->pool_next.as<unsigned long long>(), p->tuning));
       if (p->timing) {
         HIPC(hipEventRecord(p->ke1, p->stream));
@@ -395,7 +415,7 @@ TransportOut transport_device(abx_problem* p, const abx_particle* d_in, int64_t 
                       p->stream));
   HIPC(hipStreamSynchronize(p->stream));
   TransportOut res = collect_scalars(sc, cn, err, err_hist);
-  if (banks) res.n_sites = order_fission_bank(p, n, nst[0], nst[1]);
+  if (banks) res.n_sites = order_fission_bank(p, n, nst[0], nst[1], K.stage_pinfo != nullptr);
   add_scores(p->gen_scores, res.scores);
   return res;
 }
@@ -591,6 +611,33 @@ int abx_create(const abx_problem_desc* desc, abx_problem** out) {
       if ((int64_t)c.shape[0] * c.shape[1] * c.shape[2] * p->cm.ne >= 0xffffffffll)
         fail(ABX_E_INVALID, "cancellation mesh too large");
     }
+    if (T.exact) {  // validated by build_tables
+      p->exact = T.exact;
+      DevExact& E = p->dex;
+      E.kind = T.exact;
+      E.beta = T.ex.beta;
+      E.n_samples = T.ex.n_samples;
+      E.chi_matrix = T.ex.chi_matrix != 0;
+      E.use_virtual = T.ex.use_virtual_collisions != 0;
+      E.sobol = T.ex.sobol != 0;
+      for (int k = 0; k < 4; k++) E.S[k] = T.ex_shape[k];
+      E.n_gbins = T.exact == ABX_EXACT_MG ? T.ex.n_group_bins : 0;
+      E.nmat = std::max(1, T.nmat);
+      E.G = T.G;
+      for (int k = 0; k < 3; k++) {
+        E.low[k] = T.ex.low[k];
+        E.hi[k] = T.ex.high[k];
+        E.pitch[k] = T.ex_pitch[k];
+      }
+      E.gb_off = p->upload(T.ex_gb_off);
+      E.gb = p->upload(T.ex_gb);
+      E.g2bin = p->upload(T.ex_g2bin);
+      E.chi = p->upload(E.chi_matrix ? T.chi_norm : std::vector<double>{});
+      std::vector<uint64_t> sm((size_t)kSobolDims * kSobolBits);
+      sobol_build(sm.data());
+      E.sobolM = p->upload(sm);
+    }
+    p->grng = pcg_seed(T.st.seed);  // settings::rng (src/settings.cpp:116-118)
     *out = p;
     return ABX_OK;
   } catch (const AbxError& e) {
@@ -823,6 +870,7 @@ static void run_reset(abx_problem* p) {
   p->run_kcol = 1.0;
   p->bank_n = 0;
   p->rep_chains = 0;
+  p->grng = pcg_seed(T.st.seed);
 }
 
 // Whether a run stops at this generation boundary: a stop was requested
@@ -872,6 +920,102 @@ static void grow_keep(DBuf& b, size_t bytes, size_t keep, hipStream_t s) {
   b = n;
 }
 
+// Exact / basic-exact cancellation (perform_regional_cancellation,
+// src/power_iterator.cpp:751-777) of the n ordered fission sites in
+// p->ordered with their parent records in p->ordered_pinfo: weights change
+// in place and the uniform particles are appended; returns the new bank
+// size.  settings::rng (p->grng) advances as the reference's does.
+static int64_t exact_cancel_device(abx_problem* p, int64_t n) {
+  const DevExact& E = p->dex;
+  if (E.kind == ABX_BASIC_EXACT_MG && E.beta == ABX_BETA_ZERO) return n;
+  if (n <= 0) return n;
+  if (n > 0xffffffffll) fail(ABX_E_INVALID, "exact cancellation: more than 2^32 sites");
+  hipStream_t s = p->stream;
+  const size_t lds = (size_t)256 * trk_bytes(p->P.trk_D, p->P.trk_L);
+  abx_banked* bank = p->ordered.as<abx_banked>();
+  const abx_parent_info* par = p->ordered_pinfo.as<abx_parent_info>();
+  require_records(p->ordered, n, sizeof(abx_banked), "cancelled bank");
+  require_records(p->ordered_pinfo, n, sizeof(abx_parent_info), "parent records");
+  p->ckeys.reserve((size_t)n * sizeof(uint64_t));
+  p->ckeys2.reserve((size_t)n * sizeof(uint64_t));
+  p->ex_head.reserve((size_t)n * sizeof(uint32_t));
+  p->ex_offs.reserve((size_t)n * sizeof(uint64_t));
+  HIPC(hipMemsetAsync(p->err.p, 0, sizeof(int32_t), s));
+  hipLaunchKernelGGL(k_ex_keys, dim3((unsigned)blocks_for(n, 256)), dim3(256), lds, s, p->P, E,
+                     (const abx_banked*)bank, par, n, p->ckeys.as<uint64_t>(), p->err.as<int32_t>());
+  HIPC(hipGetLastError());
+  size_t temp = 0;
+  HIPC(rocprim::radix_sort_keys(nullptr, temp, p->ckeys.as<uint64_t>(), p->ckeys2.as<uint64_t>(),
+                                (size_t)n, 0, 64, s));
+  p->ctemp.reserve(std::max<size_t>(temp, 16));
+  HIPC(rocprim::radix_sort_keys(p->ctemp.p, temp, p->ckeys.as<uint64_t>(), p->ckeys2.as<uint64_t>(),
+                                (size_t)n, 0, 64, s));
+  const uint64_t* keys = p->ckeys2.as<uint64_t>();
+  hipLaunchKernelGGL(k_ex_heads, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0, s, keys, n,
+                     p->ex_head.as<uint32_t>());
+  HIPC(hipGetLastError());
+  const int64_t nb = (int64_t)device_scan(p, p->ex_head.as<uint32_t>(), p->ex_offs.as<uint64_t>(), n);
+  int32_t err = 0;
+  HIPC(hipMemcpyAsync(&err, p->err.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  HIPC(hipStreamSynchronize(s));
+  if (err) fail(err, "exact cancellation: a fission site lies in no cell");
+  if (nb == 0) return n;
+  p->ex_bins.reserve((size_t)nb * sizeof(ExBinD));
+  ExBinD* bins = p->ex_bins.as<ExBinD>();
+  hipLaunchKernelGGL(k_ex_bins, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0, s, keys, n,
+                     (const uint32_t*)p->ex_head.p, (const uint64_t*)p->ex_offs.p, bins);
+  HIPC(hipGetLastError());
+  hipLaunchKernelGGL(k_ex_bin_init, dim3((unsigned)blocks_for(nb * 64, 256)), dim3(256), 0, s, E,
+                     (const abx_banked*)bank, keys, bins, nb);
+  HIPC(hipGetLastError());
+  // basic: the per-region streams skip n_samples * 100 (x2 for average-g)
+  // draws per site of every earlier region (:475-489)
+  const bool basic = E.kind == ABX_BASIC_EXACT_MG;
+  const uint64_t max_rn = (uint64_t)E.n_samples * 100u * (E.beta == ABX_BETA_AVERAGE_G ? 2u : 1u);
+  p->ex_smp.reserve((size_t)nb * (size_t)E.n_samples * sizeof(ExSample));
+  hipLaunchKernelGGL(k_ex_samples, dim3((unsigned)blocks_for(nb, 256)), dim3(256), lds, s, p->P, E,
+                     bins, nb, p->grng, max_rn, p->ex_smp.as<ExSample>());
+  HIPC(hipGetLastError());
+  p->ex_fv.reserve((size_t)n * sizeof(double));
+  p->ex_fi.reserve((size_t)n * sizeof(double));
+  hipLaunchKernelGGL(k_ex_f, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0, s, p->P, E,
+                     (const abx_banked*)bank, par, keys, n, (const uint32_t*)p->ex_head.p,
+                     (const uint64_t*)p->ex_offs.p, bins, (const ExSample*)p->ex_smp.p,
+                     p->ex_fv.as<double>(), p->ex_fi.as<double>());
+  HIPC(hipGetLastError());
+  hipLaunchKernelGGL(k_ex_cancel, dim3((unsigned)blocks_for(nb * 64, 256)), dim3(256), 0, s, p->P,
+                     E, bank, par, keys, bins, nb, (const double*)p->ex_fv.p,
+                     (const double*)p->ex_fi.p);
+  HIPC(hipGetLastError());
+  if (basic && E.beta != ABX_BETA_MINIMUM) {
+    ExBinD last{};
+    HIPC(hipMemcpyAsync(&last, bins + (nb - 1), sizeof(ExBinD), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    p->grng = pcg_advance(p->grng, (uint64_t)last.end * max_rn);
+  }
+  // get_new_particles
+  p->ex_cnt.reserve((size_t)nb * sizeof(uint32_t));
+  hipLaunchKernelGGL(k_ex_counts, dim3((unsigned)blocks_for(nb, 256)), dim3(256), 0, s,
+                     (const ExBinD*)bins, nb, p->ex_cnt.as<uint32_t>());
+  HIPC(hipGetLastError());
+  const uint64_t nnew = device_scan(p, p->ex_cnt.as<uint32_t>(), p->ex_offs.as<uint64_t>(), nb);
+  if (nnew == 0) return n;
+  grow_keep(p->ordered, (size_t)(n + (int64_t)nnew) * sizeof(abx_banked), (size_t)n * sizeof(abx_banked), s);
+  p->ex_state.reserve(sizeof(uint64_t));
+  HIPC(hipMemsetAsync(p->err.p, 0, sizeof(int32_t), s));
+  hipLaunchKernelGGL(k_ex_new, dim3(1), dim3(256), lds, s, p->P, E, (const ExBinD*)bins, nb,
+                     (const uint64_t*)p->ex_offs.p, p->grng, p->ordered.as<abx_banked>() + n,
+                     p->ex_state.as<uint64_t>(), p->err.as<int32_t>());
+  HIPC(hipGetLastError());
+  uint64_t g = 0;
+  HIPC(hipMemcpyAsync(&g, p->ex_state.p, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  HIPC(hipMemcpyAsync(&err, p->err.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  HIPC(hipStreamSynchronize(s));
+  if (err) fail(err, "Couldn't sample position for uniform particle.");
+  p->grng = g;
+  return n + (int64_t)nnew;
+}
+
 // PowerIterator::run with every generation a replay of captured graphs:
 // graph 1 = zero the generation's scalars + the persistent transport kernel
 // (bank and k read from GenDev), graph 2 = device scan of the daughter
@@ -884,7 +1028,7 @@ static bool run_generations_graph(abx_problem* p, int ngenerations, int nignored
                                   double* kcol_hist, abx_run_stats* stats) {
   const HostTables& T = p->T;
   if (!p->use_graphs || T.st.mode != ABX_K_EIGENVALUE || p->engine != ABX_ENGINE_HISTORY ||
-      p->cancel || !p->run_started)
+      p->cancel || p->exact || !p->run_started)
     return false;
   const int64_t N = T.st.nparticles;
   const double W = (double)N;
@@ -933,7 +1077,7 @@ static bool run_generations_graph(abx_problem* p, int ngenerations, int nignored
   bool stage = false;
   int64_t grid = 0;
   // (carter tracking never gets here: it carries the secondary stack)
-  const bool ext = p->T.ext_geometry;
+  const bool ext = p->T.extended;
   HIPC(dispatch_variant(trk, false, false, split, ext, [&](auto TT, auto, auto SE, auto SP, auto EX) {
     constexpr int TK = decltype(TT)::value;
     constexpr bool SEC = decltype(SE)::value, SPL = decltype(SP)::value, EXT = decltype(EX)::value;
@@ -1143,6 +1287,7 @@ static void run_replicas(abx_problem* p, int R, const uint64_t* seeds, bool from
   const HostTables& T = p->T;
   if (T.st.mode != ABX_K_EIGENVALUE) fail(ABX_E_INVALID, "replicas are k-eigenvalue chains");
   if (p->cancel) fail(ABX_E_INVALID, "replicas: approximate cancellation is not supported");
+  if (p->exact) fail(ABX_E_INVALID, "replicas: exact cancellation is not supported");
   if (needs_secondaries(T)) fail(ABX_E_INVALID, "replicas: carter tracking is not supported");
   // the bank every chain starts from (from_bank): the problem's current one
   const uint64_t n0 = from_bank ? (uint64_t)p->bank_n : 0;
@@ -1245,7 +1390,7 @@ static void run_replicas(abx_problem* p, int R, const uint64_t* seeds, bool from
   const bool split = T.st.precision == ABX_SPLIT;
   bool stage = false;
   int64_t grid = 0;
-  const bool ext = p->T.ext_geometry;
+  const bool ext = p->T.extended;
   HIPC(rep_dispatch(trk, split, ext, [&](auto TT, auto SP, auto EX) {
     constexpr int TK = decltype(TT)::value;
     constexpr bool SPL = decltype(SP)::value, EXT = decltype(EX)::value;
@@ -1518,6 +1663,7 @@ static void run_generations(abx_problem* p, int ngenerations, int nignored, doub
       // perform_regional_cancellation, then normalize_weights: W = W_pos -
       // W_neg, w *= nparticles / W (src/power_iterator.cpp:354-359)
       if (p->cancel) cancel_device(p, p->ordered.as<abx_banked>(), r.n_sites);
+      if (p->exact) r.n_sites = exact_cancel_device(p, r.n_sites);
       p->wsum.reserve(2 * sizeof(double));
       HIPC(hipMemsetAsync(p->wsum.p, 0, 2 * sizeof(double), p->stream));
       hipLaunchKernelGGL(k_weight_sums, dim3((unsigned)blocks_for(r.n_sites, 256)), dim3(256), 0,
@@ -1632,6 +1778,56 @@ int abx_cancel_bank(abx_problem* p, abx_banked* bank, size_t n) {
     cancel_device(p, p->ordered.as<abx_banked>(), (int64_t)n);
     if (n) HIPC(hipMemcpyAsync(bank, p->ordered.p, n * sizeof(abx_banked), hipMemcpyDeviceToHost, p->stream));
     HIPC(hipStreamSynchronize(p->stream));
+  });
+}
+
+int abx_parent_fetch(abx_problem* p, abx_parent_info* out, size_t cap, size_t* n) {
+  ABX_TRY({
+    if (!p || !n || (!out && cap)) fail(ABX_E_INVALID, "null argument");
+    if (!p->exact) fail(ABX_E_INVALID, "no exact cancelator in the description");
+    const size_t m = (size_t)p->last_sites;
+    *n = m;
+    if (m > cap) fail(ABX_E_CAPACITY, "parent buffer holds " + std::to_string(cap) + " records, " +
+                                          std::to_string(m) + " needed");
+    if (m) {
+      require_records(p->ordered_pinfo, (int64_t)m, sizeof(abx_parent_info), "parent records");
+      HIPC(hipMemcpyAsync(out, p->ordered_pinfo.p, m * sizeof(abx_parent_info), hipMemcpyDeviceToHost,
+                          p->stream));
+    }
+    HIPC(hipStreamSynchronize(p->stream));
+  });
+}
+
+int abx_exact_cancel_bank(abx_problem* p, const abx_banked* bank, const abx_parent_info* parents,
+                          size_t n, abx_banked* out, size_t cap, size_t* n_out) {
+  ABX_TRY({
+    if (!p || !n_out || (n && (!bank || !parents)) || (!out && cap))
+      fail(ABX_E_INVALID, "null argument");
+    if (!p->exact) fail(ABX_E_INVALID, "no exact cancelator in the description");
+    if (n > 0xffffffffull) fail(ABX_E_INVALID, "bank larger than 2^32");
+    p->ordered.reserve(std::max<size_t>(1, n) * sizeof(abx_banked));
+    p->ordered_pinfo.reserve(std::max<size_t>(1, n) * sizeof(abx_parent_info));
+    if (n) {
+      HIPC(hipMemcpyAsync(p->ordered.p, bank, n * sizeof(abx_banked), hipMemcpyHostToDevice, p->stream));
+      HIPC(hipMemcpyAsync(p->ordered_pinfo.p, parents, n * sizeof(abx_parent_info),
+                          hipMemcpyHostToDevice, p->stream));
+    }
+    p->last_sites = 0;  // `ordered` no longer holds the last transport's sites
+    const int64_t m = exact_cancel_device(p, (int64_t)n);
+    *n_out = (size_t)m;
+    if ((size_t)m > cap)
+      fail(ABX_E_CAPACITY, "out holds " + std::to_string(cap) + " records, " + std::to_string(m) +
+                               " needed");
+    if (m) HIPC(hipMemcpyAsync(out, p->ordered.p, (size_t)m * sizeof(abx_banked), hipMemcpyDeviceToHost,
+                               p->stream));
+    HIPC(hipStreamSynchronize(p->stream));
+  });
+}
+
+int abx_rng_reset(abx_problem* p) {
+  ABX_TRY({
+    if (!p) fail(ABX_E_INVALID, "null argument");
+    p->grng = pcg_seed(p->T.st.seed);
   });
 }
 
@@ -1778,7 +1974,7 @@ bool batch_stage_fits(const abx_problem* p) {
   const bool sec = needs_secondaries(T), split = T.st.precision == ABX_SPLIT;
   const size_t cur = cursor_lds_bytes(256, p->P.trk_D, p->P.trk_L);
   bool ok = false;
-  (void)dispatch_variant(trk, false, sec || trk == 2, split, T.ext_geometry,
+  (void)dispatch_variant(trk, false, sec || trk == 2, split, T.extended,
                          [&](auto TT, auto, auto SE, auto SP, auto EX) {
     constexpr int TK = decltype(TT)::value;
     constexpr bool SEC = decltype(SE)::value, SPL = decltype(SP)::value;
@@ -1798,7 +1994,7 @@ int variant_key(abx_problem* p) {
   const HostTables& T = p->T;
   const int sec = needs_secondaries(T) ? 1 : 0;
   const int split = T.st.precision == ABX_SPLIT ? 1 : 0;
-  const int ext = T.ext_geometry ? 1 : 0;
+  const int ext = T.extended ? 1 : 0;
   if (p->batch_stage < 0) p->batch_stage = batch_stage_fits(p) ? 1 : 0;
   return (((T.st.tracking * 2 + sec) * 2 + split) * 2 + ext) * 2 + p->batch_stage;
 }
@@ -2460,7 +2656,7 @@ extern "C" int abx_transport_noalloc(abx_problem* p, const abx_particle* d_bank,
     K.err = d_err;
     K.err_hist = d_err_hist;
     if (nn > 0)
-      HIPC(history_transport(p->P, p->T.ext_geometry, K, p->stream, sec_cap > 0, d_next,
+      HIPC(history_transport(p->P, p->T.extended, K, p->stream, sec_cap > 0, d_next,
                              p->tuning));
     double sc[8];
     unsigned long long cn[8], nst[2] = {0, 0};
@@ -2501,7 +2697,8 @@ extern "C" int abx_transport_noalloc(abx_problem* p, const abx_particle* d_bank,
                                std::to_string(total) + " were banked");
     hipLaunchKernelGGL(k_order_bank, dim3((unsigned)blocks_for((int64_t)nst[0], 256)), dim3(256), 0,
                        p->stream, K.stage, K.stage_parent, K.stage_daughter, d_offsets,
-                       (int64_t)nst[0], nn, (uint64_t)total, d_fission_out, d_err);
+                       (int64_t)nst[0], nn, (uint64_t)total, d_fission_out, d_err,
+                       (const abx_parent_info*)nullptr, (abx_parent_info*)nullptr);
     HIPC(hipGetLastError());
     HIPC(hipMemcpyAsync(&err, d_err, sizeof(err), hipMemcpyDeviceToHost, p->stream));
     HIPC(hipStreamSynchronize(p->stream));

@@ -8,6 +8,8 @@
 // -ffp-contract=off so that no multiply-add is fused.
 //
 // Methods (published, re-derived here; coefficients are plain Taylor terms):
+//  * exp: Cody-Waite reduction by ln 2 and the rational form
+//    1 + 2r / (2 - c) of fdlibm with the Bernoulli series of r coth(r/2).
 //  * log: argument reduction x = 2^k * m, m in [sqrt(2)/2, sqrt(2)); with
 //    f = m - 1 and s = f / (2 + f), log(m) = 2 atanh(s) evaluated as
 //    f - hfsq + s (hfsq + R(s^2)), hfsq = f^2/2, R(z) = sum_{n>=1} 2 z^n/(2n+1)
@@ -99,6 +101,44 @@ DM_HD double dm_log(double x) {
   const double hfsq = 0.5 * f * f;
   const double dk = (double)k;
   return dk * ln2_hi - ((hfsq - (s * (hfsq + R) + dk * ln2_lo)) - f);
+}
+
+// Exponential.  Reduction x = k ln2 + r, |r| <= ln2 / 2 (Cody-Waite, ln2 in
+// the same head / tail split as dm_log), then the rational form
+// exp(r) = 1 + 2r / (2 - c) with c = r - r^2 P(r^2), where r^2 P is the
+// series of r coth(r/2) - 2 (2 B_2n / (2n)!: 1/6, -1/360, 1/15120,
+// -1/604800, 1/23950080, -691/653837184000, 1/37362124800; the next term is
+// below 1e-19 for |r| <= ln2 / 2), kept in the head / tail split of r so that the
+// rounding of r itself does not enter (the Sun fdlibm formulation); 2^k is
+// applied through the exponent bits, in two steps near the underflow.
+// Returns +inf above the overflow threshold and +0 below the underflow one.
+DM_HD double dm_exp(double x) {
+  const double ln2_hi = 6.93147180369123816490e-01; /* 0x3fe62e42fee00000 */
+  const double ln2_lo = 1.90821492927058770002e-10; /* 0x3dea39ef35793c76 */
+  const double inv_ln2 = 1.44269504088896338700e+00;
+  if (x != x) return x;
+  if (x > 709.782712893383973096) return 1.0 / 0.0;
+  if (x < -745.13321910194110842) return 0.0;
+  const double t = x * inv_ln2;
+  const double fk = t >= 0.0 ? (double)(int64_t)(t + 0.5) : -(double)(int64_t)(0.5 - t);
+  const int k = (int)fk;
+  const double hi = x - fk * ln2_hi;
+  const double lo = fk * ln2_lo;
+  const double r = hi - lo;
+  const double z = r * r;
+  const double c =
+      r - z * (1.0 / 6.0 +
+               z * (-1.0 / 360.0 +
+                    z * (1.0 / 15120.0 +
+                         z * (-1.0 / 604800.0 +
+                              z * (1.0 / 23950080.0 +
+                                   z * (-691.0 / 653837184000.0 + z * (1.0 / 37362124800.0)))))));
+  const double y = 1.0 - ((lo - (r * c) / (2.0 - c)) - hi);
+  if (k > 1023) return (y * dm_from_bits((uint64_t)(1023 + 1023) << 52)) * 2.0;
+  if (k >= -1021) return y * dm_from_bits((uint64_t)(k + 1023) << 52);
+  // subnormal result: scale by 2^(k + 1000), then by 2^-1000
+  return (y * dm_from_bits((uint64_t)(k + 1000 + 1023) << 52)) *
+         dm_from_bits((uint64_t)(-1000 + 1023) << 52);
 }
 
 // sin and cos of r, |r| <= pi/4 (Taylor kernels).

@@ -656,7 +656,7 @@ static inline hipError_t event_transport(const DevProblem& P, const KCtx& K, Eve
                           });
 }
 
-// ext: the problem needs the extended-geometry variants (its tables' ext_geometry)
+// ext: the problem needs the extended-geometry variants (its tables' extended)
 static inline hipError_t history_transport(const DevProblem& P, bool ext, const KCtx& K,
                                            hipStream_t s, bool sec, unsigned long long* next,
                                            const abx_tuning& tu) {

@@ -46,6 +46,9 @@ struct KCtx {
   const double* rep_kcol;
   double* rep_scores;
   int32_t nrep;
+  // extended kernels: each staged fission site's parent record, beside
+  // `stage` (null: not kept)
+  abx_parent_info* stage_pinfo;
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -117,9 +120,12 @@ constexpr uint32_t kHole = 0xFFFFFFFFu;
 // Global-memory side of the physics for one lane.  TRACE compiles the
 // per-event trace records in; SEC the per-history secondary stack; SPLIT
 // selects the split-precision physics.
-template <bool TRACE, bool SEC, bool SPLIT = false>
+// EXT: keeps the previous-collision state and writes each fission site's
+// parent record (K->stage_pinfo) for exact cancellation.
+template <bool TRACE, bool SEC, bool SPLIT = false, bool EXT = false>
 struct GpuSink {
   static constexpr bool split = SPLIT;
+  static constexpr bool ext = EXT;
   const KCtx* K;
   const DevProblem* P;
   int64_t slot;   // history index in the input bank
@@ -240,6 +246,24 @@ struct GpuSink {
     K->stage[i] = b;
     K->stage_parent[i] = (uint32_t)slot;
     K->stage_daughter[i] = p.daughters;
+    if constexpr (EXT) {
+      // BankedParticle's cancellation fields (src/transporter.cpp:440-456)
+      if (K->stage_pinfo) {
+        abx_parent_info q;
+        q.r1[0] = p.pr.x;
+        q.r1[1] = p.pr.y;
+        q.r1[2] = p.pr.z;
+        q.u1[0] = p.pu.x;
+        q.u1[1] = p.pu.y;
+        q.u1[2] = p.pu.z;
+        q.E1 = p.pE;
+        q.E3 = p.E;
+        q.Esmp = p.Esmp;
+        q.prev_virtual = p.pvirt;
+        q.pad_ = 0;
+        K->stage_pinfo[i] = q;
+      }
+    }
   }
   template <class TK>
   __device__ void trace(const Part& p, const TK& t, int ev) {
@@ -299,6 +323,13 @@ __device__ __forceinline__ void load_part(const DevProblem& P, const abx_particl
   p.daughters = 0;
   p.sec_id = 0;
   p.alive = 1;
+  // a new Particle's previous state (particle.hpp:231-239)
+  p.pr = V3{0., 0., 0.};
+  p.pu = V3{1., 0., 0.};
+  p.pE = 0.;
+  p.Esmp = 0.;
+  p.pvirt = 0;
+  p.refl = 0;
 }
 
 // Dynamic LDS: every lane's cursor pads / tiles / frames, lane-interleaved.
@@ -369,7 +400,7 @@ __global__ void ABX_TKERNEL k_history(DevProblem P0, KCtx K, unsigned long long*
   }
   const int lane = threadIdx.x & 63;
   Scores6 sc{0, 0, 0, 0, 0, 0};
-  GpuSink<TRACE, SEC, SPLIT> s;
+  GpuSink<TRACE, SEC, SPLIT, EXT> s;
   s.init(&K, &P, 0);
   s.sslot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   // per-wave fission staging ranges (blocks of at most 1024 lanes)
@@ -507,7 +538,7 @@ __global__ void ABX_TKERNEL k_batch_chunks(const DevProblem* __restrict__ probs,
   trk_bind(t, abx_lds, blockDim.x, threadIdx.x, D, L);
   unsigned char* tables = abx_lds + cursor_lds_bytes(blockDim.x, D, L);
   int staged = -1;
-  GpuSink<false, SEC, SPLIT> s;
+  GpuSink<false, SEC, SPLIT, EXT> s;
   s.init(nullptr, nullptr, 0);
   s.sslot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (;;) {
@@ -722,7 +753,8 @@ __global__ void k_add_block_offsets(uint64_t* out, const uint64_t* block_off, in
 // ABX_E_INVALID in *err instead of writing.
 __global__ void k_order_bank(const abx_banked* stage, const uint32_t* parent,
                              const uint32_t* daughter, const uint64_t* offsets, int64_t m,
-                             int64_t n, uint64_t total, abx_banked* out, int32_t* err) {
+                             int64_t n, uint64_t total, abx_banked* out, int32_t* err,
+                             const abx_parent_info* pin, abx_parent_info* pout) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= m) return;
   const uint32_t q = parent[k];
@@ -737,6 +769,7 @@ __global__ void k_order_bank(const abx_banked* stage, const uint32_t* parent,
     return;
   }
   out[dst] = stage[k];
+  if (pout) pout[dst] = pin[k];  // the site's parent record (exact cancellation)
 }
 
 // MeshTally::record_generation (src/mesh_tally.cpp:121-150), then zero the

@@ -42,7 +42,27 @@ struct Part {
   int32_t sec_id;
   int32_t g;           // energy group of E
   int32_t alive;
+  // Particle::previous_{position,direction,energy}, Esmp_, reflected and
+  // previous_collision_virtual_ (include/simulation/particle.hpp:231-239):
+  // kept by the extended kernels only (Sink::ext), for the parent records
+  // of exact cancellation
+  V3 pr, pu;
+  double pE, Esmp;
+  int32_t pvirt, refl;
 };
+
+// Particle::move (particle.hpp:125-133) in the extended kernels: the
+// previous position follows every move except the first after a reflection.
+template <bool EXT>
+ABX_HD void part_move(Part& p, double d) {
+  if constexpr (EXT) {
+    if (!p.refl)
+      p.pr = p.r;
+    else
+      p.refl = 0;
+  }
+  p.r = vadd(p.r, vscale(p.u, d));
+}
 
 struct SecRec {
   double r[3], u[3], E, w, w2;
@@ -505,6 +525,10 @@ ABX_HD void collide(const DevProblem& P, Sink& s, Part& p, int m, double kcol,
     const double E_out = P.emid[ei];
     const double mu = sample_mu(P, p, m, p.g, ei);
     const double phi = 2. * ABX_PI * p_rand(p);
+    if constexpr (Sink::ext) {  // Particle::set_direction / set_energy
+      p.pu = p.u;
+      p.pE = p.E;
+    }
     p.u = rotate<SPLIT>(p.u, mu, phi);
     p.E = E_out;
     p.g = energy_index(P, E_out);
@@ -525,7 +549,10 @@ ABX_HD void leak(Part& p, const Bound& b, Scores6& sc) {
 
 // Tracker::do_reflection; the particle's position before the reflection
 // only feeds Particle::previous_r, which this path does not keep.
-template <class TK>
+// EXT: also the previous position the reference computes for exact
+// cancellation (tracker.hpp:328-356): the point as far behind the surface,
+// along the new direction, as the last sampled position was before it.
+template <bool EXT, class TK>
 ABX_HD void reflect(const DevProblem& P, TK& t, Part& p, const Bound& b) {
   if (b.sidx < 0) {
     t.err = -3;
@@ -535,8 +562,16 @@ ABX_HD void reflect(const DevProblem& P, TK& t, Part& p, const Bound& b) {
   const V3 r_on = vadd(p.r, vscale(p.u, b.d));
   const V3 n = surf_norm<TK::kExt>(S, r_on);
   const V3 nd = vsub(p.u, vscale(n, 2. * vdot(p.u, n)));
+  const V3 u_new = vdir(nd.x, nd.y, nd.z);
+  if constexpr (EXT) {
+    const V3 r_pre = p.refl ? p.pr : p.r;
+    const double d = b.d + vnorm(vsub(p.r, r_pre));
+    p.pr = vsub(r_on, vscale(u_new, d));
+    p.pu = p.u;
+    p.refl = 1;
+  }
   p.r = r_on;
-  p.u = vdir(nd.x, nd.y, nd.z);
+  p.u = u_new;
   t.token = 0;
   t.set_lv(0, p.r);
   trk_restart(P, t, p.u);
@@ -631,7 +666,7 @@ ABX_HD bool step_flight(const DevProblem& P, Sink& s, Part& p, TK& t, int& m, bo
         leak<Sink>(p, b, sc);
         ev = 4;
       } else if (b.btype == 1) {
-        reflect(P, t, p, b);
+        reflect<Sink::ext>(P, t, p, b);
         if (t.cell < 0 || t.err) {
           s.error(-3, p.hid);
           p.alive = 0;
@@ -642,7 +677,7 @@ ABX_HD bool step_flight(const DevProblem& P, Sink& s, Part& p, TK& t, int& m, bo
         trk_move(t, p.u, b.d);
         t.token = -b.tok;
         trk_get_current(P, t, p.u);
-        p.r = vadd(p.r, vscale(p.u, b.d));
+        part_move<Sink::ext>(p, b.d);
         if (t.cell < 0 || t.err) {
           s.error(-3, p.hid);
           p.alive = 0;
@@ -652,7 +687,7 @@ ABX_HD bool step_flight(const DevProblem& P, Sink& s, Part& p, TK& t, int& m, bo
       }
       return false;
     }
-    p.r = vadd(p.r, vscale(p.u, d_coll));
+    part_move<Sink::ext>(p, d_coll);
     trk_move(t, p.u, d_coll);
     return true;
   }
@@ -660,6 +695,7 @@ ABX_HD bool step_flight(const DevProblem& P, Sink& s, Part& p, TK& t, int& m, bo
   // src/carter_tracker.cpp:124-216)
   const bool carter = TRK == 2;
   const double Emaj = (carter ? P.sampling[p.g] : P.majorant[p.g]) + 0.;
+  if constexpr (Sink::ext) p.Esmp = Emaj;  // Particle::set_Esmp
   const double d_coll = flight_distance<Sink::split>(p, Emaj);
   s.count_flight();
   trk_move(t, p.u, d_coll);
@@ -687,7 +723,7 @@ ABX_HD bool step_flight(const DevProblem& P, Sink& s, Part& p, TK& t, int& m, bo
       leak<Sink>(p, b, sc);
       ev = 4;
     } else if (b.btype == 1) {
-      reflect(P, t, p, b);
+      reflect<Sink::ext>(P, t, p, b);
       if (t.cell < 0 || t.err) {
         s.error(-3, p.hid);
         p.alive = 0;
@@ -701,7 +737,7 @@ ABX_HD bool step_flight(const DevProblem& P, Sink& s, Part& p, TK& t, int& m, bo
     return false;
   }
   ev = 0;
-  p.r = vadd(p.r, vscale(p.u, d_coll));
+  part_move<Sink::ext>(p, d_coll);
   m = P.cell[t.cell].fill;
   const double Et = P.Et[m * P.G + p.g];
   bool had_collision = false;
@@ -745,6 +781,15 @@ ABX_HD void step_collision(const DevProblem& P, Sink& s, Part& p, int m, double 
 // whether the slot still carries a live particle.
 template <int TRK, class Sink, class TK>
 ABX_HD bool step_finish(const DevProblem& P, Sink& s, Part& p, TK& t, int& m, int ev) {
+  if constexpr (Sink::ext && TRK != 0) {
+    // set_previous_collision_real / _virtual (src/delta_tracker.cpp:187-195,
+    // src/carter_tracker.cpp:209-217): real after a collision, virtual
+    // after any other event the particle survives (reflections included)
+    if (ev == 1 || ev == 5)
+      p.pvirt = 0;
+    else if (p.alive)
+      p.pvirt = 1;
+  }
   if (TRK == 2) split(s, p, P);
   s.trace(p, t, ev);
   if (!p.alive) try_resurrect(P, s, p, t, m);

@@ -52,8 +52,18 @@ struct HostTables {
   int64_t tally_bins = 0;
   std::vector<abx_source> sources;
   int32_t max_pads = 1, max_levels = 1;  // cursor depth bound over all descents
-  bool ext_geometry = false;  // hexagonal lattices or off-axis cylinders: the EXT kernels
+  bool extended = false;  // hexagonal lattices, off-axis cylinders, exact cancellation
   abx_settings st{};
+  std::vector<double> chi_norm;  // [nmat][G][G] fission spectra, rows normalised
+  // exact cancellation (ExactMGCancelator / BasicExactMGCancelator): the
+  // configuration, the regions' pitch and shape (energy regions last, 1
+  // without group bins), the group bins as offsets + groups and the bin of
+  // each group (the number of bins for a group in none)
+  int32_t exact = 0;  // abx_exact_kind, 0 = none
+  abx_exact_cancel ex{};
+  double ex_pitch[3] = {0., 0., 0.};
+  int32_t ex_shape[4] = {1, 1, 1, 1};
+  std::vector<int32_t> ex_gb_off, ex_gb, ex_g2bin;
   double min_energy = 0., max_energy = 100000.;
 };
 
@@ -64,6 +74,64 @@ struct TableError : std::runtime_error {
 
 static inline void tables_require(bool ok, const std::string& m) {
   if (!ok) throw TableError(ABX_E_INVALID, m);
+}
+
+// make_cancelator (src/cancelator.cpp:32-78) and the cancelators'
+// constructors (src/exact_mg_cancelator.cpp:41-118,
+// src/basic_exact_mg_cancelator.cpp:43-66): validation, pitch, group bins.
+static inline void build_exact(const abx_problem_desc* d, HostTables& T) {
+  const abx_exact_cancel& e = *d->exact_cancel;
+  tables_require(e.kind == ABX_EXACT_MG || e.kind == ABX_BASIC_EXACT_MG, "Unkown cancelator type.");
+  tables_require(d->cancel == nullptr, "one cancelator per problem");
+  tables_require(T.st.tracking != ABX_SURFACE_TRACKING,
+                 "exect cancelators may not be used with surface-tracking.");
+  T.exact = e.kind;
+  T.ex = e;
+  tables_require(e.low[0] < e.high[0] && e.low[1] < e.high[1] && e.low[2] < e.high[2],
+                 "r_low >= r_hi");
+  tables_require(e.n_samples > 0, "n-samples must be greater than zero.");
+  for (int k = 0; k < 3; k++) {
+    tables_require(e.shape[k] >= 1, "cancelator shape");
+    T.ex_shape[k] = e.shape[k];
+    T.ex_pitch[k] = (e.high[k] - e.low[k]) / (double)e.shape[k];
+  }
+  const int G = T.G;
+  T.ex_gb_off.assign(1, 0);
+  T.ex_gb.clear();
+  if (e.kind == ABX_EXACT_MG) {
+    tables_require(e.n_group_bins >= 0, "group bins");
+    tables_require(e.n_group_bins == 0 || (e.group_bin_offsets && e.group_bins), "group bins");
+    std::vector<int> seen((size_t)G, 0);
+    T.ex_g2bin.assign((size_t)G, e.n_group_bins);
+    for (int b = 0; b < e.n_group_bins; b++) {
+      tables_require(e.group_bin_offsets[b] <= e.group_bin_offsets[b + 1], "group bins");
+      for (int q = e.group_bin_offsets[b]; q < e.group_bin_offsets[b + 1]; q++) {
+        const int g = e.group_bins[q];
+        tables_require(g >= 0 && g < G, "grp >= settings::ngroups");
+        tables_require(seen[(size_t)g]++ == 0, "Group is included more than once in group_bins.");
+        T.ex_gb.push_back(g);
+        T.ex_g2bin[(size_t)g] = b;
+      }
+      tables_require(e.group_bin_offsets[b + 1] > e.group_bin_offsets[b] || !e.chi_matrix,
+                     "empty group bin");
+      T.ex_gb_off.push_back((int32_t)T.ex_gb.size());
+    }
+    tables_require(!e.chi_matrix || e.n_group_bins > 0,
+                   "Chi matrix is used, but no group_bins provided.");
+    T.ex_shape[3] = e.n_group_bins > 0 ? e.n_group_bins : 1;
+  } else {
+    tables_require(e.beta >= ABX_BETA_ZERO && e.beta <= ABX_BETA_AVERAGE_G,
+                   "Unkown beta entry for basic exact MG cancelator.");
+    T.ex_g2bin.assign((size_t)G, 0);
+    T.ex_shape[3] = 1;
+  }
+  // region numbers (the mesh widened by one face each way, times the
+  // materials) and site positions share one 64-bit sort key, 32 bits each
+  const double regions = (double)(T.ex_shape[0] + 1) * (double)(T.ex_shape[1] + 1) *
+                         (double)(T.ex_shape[2] + 1) * (double)(T.ex_shape[3] + 1) *
+                         (double)std::max(1, T.nmat);
+  tables_require(regions < 4294967295.0, "exact cancellation mesh too large");
+  T.extended = true;  // the parent records come from the extended kernels
 }
 
 static inline void build_tables(const abx_problem_desc* d, HostTables& T) {
@@ -126,7 +194,7 @@ static inline void build_tables(const abx_problem_desc* d, HostTables& T) {
         Q.p[1] = 1. - v0 * v0;
         Q.p[2] = 1. - w0 * w0;
         aux.push_back({i, Q});
-        T.ext_geometry = true;
+        T.extended = true;
       }
     }
     S.bc = a.boundary;
@@ -214,7 +282,7 @@ static inline void build_tables(const abx_problem_desc* d, HostTables& T) {
       tables_require(a.shape[0] <= 4096 && a.shape[1] <= (1 << 20), "hex lattice size");
       const uint32_t rings = (uint32_t)a.shape[0], width = 2 * (rings - 1) + 1;
       const uint32_t mid = width / 2;
-      T.ext_geometry = true;
+      T.extended = true;
       U.rings = (int32_t)rings;
       U.top = a.hex_top;
       U.Nx = U.Ny = (int32_t)width;
@@ -387,6 +455,7 @@ static inline void build_tables(const abx_problem_desc* d, HostTables& T) {
       for (int o = 0; o < G; o++) chi_i += chi[(size_t)i * G + o];
       for (int o = 0; o < G; o++) chi[(size_t)i * G + o] /= chi_i;
     }
+    T.chi_norm.insert(T.chi_norm.end(), chi.begin(), chi.end());
     // scatter: Es row sums, Ps rows normalised (make_scatter_xs)
     std::vector<double> Ps((size_t)G * G), Es(G);
     for (int k = 0; k < G * G; k++) {
@@ -508,6 +577,7 @@ static inline void build_tables(const abx_problem_desc* d, HostTables& T) {
   }
   if (T.tally_gbin.empty()) T.tally_gbin.push_back(-1);
   if (T.tally_eb.empty()) T.tally_eb.push_back(0.);
+  if (d->exact_cancel) build_exact(d, T);
 }
 
 // DevProblem view of host tables (pointers into the vectors, or into device

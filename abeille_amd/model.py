@@ -35,6 +35,7 @@ class Model:
         self.tallies = []       # dict
         self.root = None
         self.cancel = None
+        self.exact = None
         self.settings = dict(mode=abi.K_EIGENVALUE, tracking=abi.SURFACE_TRACKING,
                              seed=abi.DEFAULT_SEED, stride=abi.DEFAULT_STRIDE,
                              wgt_cutoff=0.25, wgt_survival=1.0, wgt_split=2.0,
@@ -147,6 +148,36 @@ class Model:
                            ebounds=None if energy_bounds is None
                            else np.asarray(energy_bounds, np.float64))
 
+    def exact_cancellation(self, low, high, shape, kind="exact", n_samples=10, group_bins=None,
+                           beta="average-g", sobol=True, chi_matrix=None,
+                           use_virtual_collisions=None):
+        """ExactMGCancelator ("exact", src/exact_mg_cancelator.cpp) or
+        BasicExactMGCancelator ("basic", src/basic_exact_mg_cancelator.cpp)
+        regional cancellation of k-eigenvalue generations (delta or carter
+        tracking).  `chi_matrix` and `use_virtual_collisions` are the
+        reference's settings::chi_matrix / use_virtual_collisions; by default
+        they follow from the materials: a chi whose rows differ is a matrix,
+        and a zero in-group scattering entry forbids virtual collisions
+        (src/mg_nuclide.cpp:639, 713, 808-810).  The deck reader passes the
+        deck's own answers."""
+        if kind not in ("exact", "basic"):
+            raise ValueError(f"Unkown cancelator type \"{kind}\".")
+        betas = {"zero": abi.BETA_ZERO, "minimum": abi.BETA_MINIMUM,
+                 "average-f": abi.BETA_AVERAGE_F, "average-g": abi.BETA_AVERAGE_G}
+        if beta not in betas:
+            raise ValueError(f"Unkown beta entry \"{beta}\" for basic exact MG cancelator.")
+        if chi_matrix is None:
+            chi_matrix = any(m["chi"] is not None and not np.all(m["chi"] == m["chi"][0])
+                             for m in self.materials)
+        if use_virtual_collisions is None:
+            use_virtual_collisions = all(np.all(np.diag(m["scatter"]) != 0.)
+                                         for m in self.materials)
+        self.exact = dict(kind=kind, low=list(low), high=list(high), shape=list(shape),
+                          n_samples=int(n_samples),
+                          group_bins=[list(map(int, g)) for g in (group_bins or [])],
+                          beta=beta, sobol=bool(sobol), chi_matrix=bool(chi_matrix),
+                          use_virtual_collisions=bool(use_virtual_collisions))
+
     def set(self, **kw):
         for k, v in kw.items():
             if k not in self.settings:
@@ -178,9 +209,12 @@ class Model:
         cancel = None
         if self.cancel is not None:
             cancel = dict(self.cancel, ebounds=arr(self.cancel["ebounds"]))
-        return dict(surfaces=[list(s) for s in self.surfaces], cells=self.cells,
-                    universes=self.universes, materials=mats, sources=self.sources,
-                    tallies=tallies, root=self.root, cancel=cancel, settings=st)
+        d = dict(surfaces=[list(s) for s in self.surfaces], cells=self.cells,
+                 universes=self.universes, materials=mats, sources=self.sources,
+                 tallies=tallies, root=self.root, cancel=cancel, settings=st)
+        if self.exact is not None:
+            d["exact_cancel"] = dict(self.exact)
+        return d
 
     @classmethod
     def from_dict(cls, d):
@@ -208,6 +242,8 @@ class Model:
             c["ebounds"] = None if c["ebounds"] is None else np.asarray(c["ebounds"], np.float64)
             m.cancel = c
         m.settings.update(d["settings"])
+        if d.get("exact_cancel"):
+            m.exact = dict(d["exact_cancel"])
         return m
 
     # ---- flattening ----------------------------------------------------
@@ -370,7 +406,29 @@ class Model:
             cm.n_ebounds = 0 if ceb is None else len(ceb)
             cm.ebounds = abi.ptr(ceb)
             d.cancel = C.pointer(cm)
+        ex = None
+        if self.exact is not None:
+            e = self.exact
+            ex = abi.ExactCancel()
+            ex.kind = abi.EXACT_MG if e["kind"] == "exact" else abi.BASIC_EXACT_MG
+            for k in range(3):
+                ex.shape[k] = e["shape"][k]
+                ex.low[k] = e["low"][k]
+                ex.high[k] = e["high"][k]
+            ex.n_samples = e["n_samples"]
+            ex.chi_matrix = 1 if e["chi_matrix"] else 0
+            ex.use_virtual_collisions = 1 if e["use_virtual_collisions"] else 0
+            ex.beta = {"zero": abi.BETA_ZERO, "minimum": abi.BETA_MINIMUM,
+                       "average-f": abi.BETA_AVERAGE_F, "average-g": abi.BETA_AVERAGE_G}[e["beta"]]
+            ex.sobol = 1 if e["sobol"] else 0
+            offs = np.cumsum([0] + [len(g) for g in e["group_bins"]]).astype(np.int32)
+            grp = np.asarray([x for g in e["group_bins"] for x in g] or [0], np.int32)
+            ex.n_group_bins = len(e["group_bins"])
+            ex.group_bin_offsets = abi.ptr(offs, C.c_int32)
+            ex.group_bins = abi.ptr(grp, C.c_int32)
+            d.exact_cancel = C.pointer(ex)
+            keep += [offs, grp]
         keep += [surfs, cells, rpn_arr, unis, ucell_arr, elem_arr, mats, srcs, tals, eb,
-                 ratio, d, cm]
+                 ratio, d, cm, ex]
         self._keep = keep
         return d

@@ -444,3 +444,41 @@ def anisotropic_slab(nparticles=20000, tracking="surface", moments=(0.3, 0.1)):
     m, _ = la13511("PUa-1-0-SL", nparticles, tracking=tracking)
     m.materials[0]["angles"] = [linearize_legendre(list(moments))]
     return m
+
+
+def exact_slab(nparticles=4000, tracking="carter", ratio=0.85, moments=((0.25, 0.05), (0.1, 0.0))):
+    """A three-group k-eigenvalue slab for exact regional cancellation:
+    fuel | moderator | fuel across x (2.5, 3 and 2.5 cm, vacuum at x = +-4),
+    reflective at y, z = +-5.  The fuel's fission spectrum depends on the
+    incident group (a chi matrix); the moderator scatters anisotropically,
+    with linearised Legendre tables per (g_in, g_out) pair (P1, P2 moments
+    `moments[0]` in group, `moments[1]` out of group).  Carter tracking on
+    `ratio` x the majorant gives negative weights."""
+    from .deck import linearize_legendre
+    G = 3
+    m = Model()
+    sa_f = np.array([0.08, 0.12, 0.22])
+    sc_f = np.array([[0.15, 0.07, 0.01], [0.0, 0.20, 0.06], [0.0, 0.01, 0.30]])
+    m.material(1, total=sa_f + sc_f.sum(axis=1), absorption=sa_f, scatter=sc_f,
+               fission=np.array([0.07, 0.11, 0.20]), nu=np.full(G, 2.5),
+               chi=np.array([[0.8, 0.2, 0.0], [0.7, 0.25, 0.05], [0.6, 0.3, 0.1]]))
+    sa_m = np.array([0.10, 0.15, 0.25])
+    sc_m = np.array([[0.20, 0.12, 0.02], [0.0, 0.30, 0.15], [0.0, 0.05, 0.55]])
+    m.material(2, total=sa_m + sc_m.sum(axis=1), absorption=sa_m, scatter=sc_m)
+    m.materials[1]["angles"] = [linearize_legendre(list(moments[0] if gi == go else moments[1]))
+                                for gi in range(G) for go in range(G)]
+    xs = [-4.0, -1.5, 1.5, 4.0]
+    sx = [m.surface("xplane", [x], "vacuum" if i in (0, 3) else "normal")
+          for i, x in enumerate(xs)]
+    sy = [m.surface("yplane", [-5.], "reflective"), m.surface("yplane", [5.], "reflective")]
+    sz = [m.surface("zplane", [-5.], "reflective"), m.surface("zplane", [5.], "reflective")]
+    box = f"+{sy[0]} & -{sy[1]} & +{sz[0]} & -{sz[1]}"
+    cells = [m.cell(f"+{sx[i]} & -{sx[i + 1]} & {box}", material=(2 if i == 1 else 1), id=i + 1)
+             for i in range(3)]
+    m.root = m.cell_universe(cells, id=1)
+    m.point_source([-2.5, 0., 0.], energy=2.5)
+    m.set(mode=abi.K_EIGENVALUE, nparticles=nparticles, energy_bounds=[0., 1., 2., 3.],
+          tracking=_tracking(tracking))
+    if tracking == "carter":
+        m.set(sample_xs_ratio=[ratio] * G)
+    return m

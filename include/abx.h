@@ -224,6 +224,51 @@ typedef struct abx_cancel_mesh {
   const double* ebounds;
 } abx_cancel_mesh;
 
+/* ExactMGCancelator (src/exact_mg_cancelator.cpp) and BasicExactMGCancelator
+ * (src/basic_exact_mg_cancelator.cpp): regional cancellation of each
+ * k-eigenvalue generation's fission sites (src/power_iterator.cpp:354-356,
+ * 750-777) on a mesh of cancellation regions, split by material, with the
+ * transition density from each site's parent's previous collision
+ * (delta or carter tracking only, as src/cancelator.cpp:43-73 requires).
+ * The uniform particles they create draw from the run's global pcg32
+ * (settings::rng, seeded with settings.seed when a run starts). */
+enum abx_exact_kind { ABX_EXACT_MG = 1, ABX_BASIC_EXACT_MG = 2 };
+/* BasicExactMGCancelator::BetaMode (include/simulation/basic_exact_mg_cancelator.hpp:38) */
+enum abx_beta_mode {
+  ABX_BETA_ZERO = 0,
+  ABX_BETA_MINIMUM = 1,
+  ABX_BETA_AVERAGE_F = 2,
+  ABX_BETA_AVERAGE_G = 3
+};
+
+typedef struct abx_exact_cancel {
+  int32_t kind;       /* abx_exact_kind */
+  int32_t shape[3];   /* regions in x, y, z */
+  double low[3];
+  double high[3];
+  int32_t n_samples;  /* points per region for <f> and <1/f> ("n-samples", 10) */
+  int32_t chi_matrix; /* settings::chi_matrix: a fission spectrum given as a matrix */
+  int32_t use_virtual_collisions; /* settings::use_virtual_collisions (exact) */
+  int32_t beta;       /* abx_beta_mode (basic) */
+  int32_t sobol;      /* basic: points from the Sobol sequence (1) or the rng (0) */
+  int32_t n_group_bins;             /* exact: energy regions ("group-bins") */
+  const int32_t* group_bin_offsets; /* [n_group_bins + 1] into group_bins */
+  const int32_t* group_bins;        /* the groups of each energy region */
+} abx_exact_cancel;
+
+/* A fission site's parent information for exact cancellation
+ * (BankedParticle::parents_previous_* and Esmp_parent,
+ * include/simulation/particle.hpp:51-57), kept beside the ordered bank. */
+typedef struct abx_parent_info {
+  double r1[3];         /* parents_previous_position */
+  double u1[3];         /* parents_previous_direction */
+  double E1;            /* parents_previous_previous_energy */
+  double E3;            /* parents_previous_energy */
+  double Esmp;          /* Esmp_parent */
+  int32_t prev_virtual; /* parents_previous_was_virtual */
+  int32_t pad_;
+} abx_parent_info;
+
 typedef struct abx_problem_desc {
   int32_t n_surfaces;
   const abx_surface* surfaces;
@@ -246,6 +291,7 @@ typedef struct abx_problem_desc {
   const abx_mesh_tally* tallies;
   abx_settings settings;
   const abx_cancel_mesh* cancel; /* NULL = no cancellation */
+  const abx_exact_cancel* exact_cancel; /* NULL = none (not with `cancel`) */
 } abx_problem_desc;
 
 /* ---- particles at the boundary ----------------------------------------- */
@@ -413,6 +459,25 @@ int abx_tally_record(abx_problem* p, double multiplier);
  * GPU; `bank` (host memory) is updated in place.  Needs a cancellation mesh
  * in the description. */
 int abx_cancel_bank(abx_problem* p, abx_banked* bank, size_t n);
+
+/* The parent record of each fission site of the last abx_transport, in the
+ * order of its fission_out (BankedParticle::parents_previous_* and
+ * Esmp_parent, filled in Transporter::transport, src/transporter.cpp:440-456).
+ * Needs an exact cancelator in the description. */
+int abx_parent_fetch(abx_problem* p, abx_parent_info* out, size_t cap, size_t* n);
+
+/* PowerIterator::perform_regional_cancellation (src/power_iterator.cpp:751-777)
+ * with the exact or basic-exact cancelator, on the GPU: every site of `bank`
+ * (with its parent record) is added, cancellation is performed and the
+ * uniform particles from get_new_particles are appended.  out receives the
+ * n sites with their new weights and then the uniform particles; *n_out is
+ * their total.  Draws from the problem's settings::rng. */
+int abx_exact_cancel_bank(abx_problem* p, const abx_banked* bank, const abx_parent_info* parents,
+                          size_t n, abx_banked* out, size_t cap, size_t* n_out);
+
+/* settings::rng back to settings.seed (settings::initialize_global_rng,
+ * src/settings.cpp:116-118); abx_create and every abx_run do this. */
+int abx_rng_reset(abx_problem* p);
 
 /* ---- device-resident drivers (the reference's Simulation loops) ---------- */
 

@@ -34,6 +34,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <memory>
 #include <random>
 #include <string>
@@ -58,6 +59,7 @@ static bool g_portable = true;
 
 static double SIN(double x) { return g_portable ? dm_sin(x) : std::sin(x); }
 static double COS(double x) { return g_portable ? dm_cos(x) : std::cos(x); }
+static double EXP(double x) { return g_portable ? dm_exp(x) : std::exp(x); }
 
 struct Fatal {
   int code;
@@ -486,6 +488,7 @@ struct Particle {  // include/simulation/particle.hpp:68-242
   V3 r_birth;
   std::vector<ParticleState> secondaries;
   std::vector<abx_banked> fission;
+  std::vector<abx_parent_info> fission_parent;  // beside `fission`
 
   void move(double d) {
     if (!reflected) {
@@ -569,6 +572,15 @@ struct Problem {
   uint32_t c_shape[4] = {1, 1, 1, 1};
   double c_dx = 0, c_dy = 0, c_dz = 0;
   std::vector<double> c_edges;
+  // ExactMGCancelator / BasicExactMGCancelator, and settings::rng
+  int exact = 0;  // abx_exact_kind, 0 = none
+  abx_exact_cancel ex{};
+  std::vector<std::vector<int>> ex_groups;
+  V3 ex_low{0, 0, 0}, ex_hi{0, 0, 0};
+  double ex_pitch[3] = {0, 0, 0};
+  uint32_t ex_shape[4] = {1, 1, 1, 1};
+  Pcg32 grng;
+  std::vector<abx_parent_info> last_parents;  // beside the last transport's bank
   int universe_by_id(uint32_t id) const {
     for (size_t i = 0; i < universes.size(); i++)
       if (universes[i].id == id) return (int)i;
@@ -1574,6 +1586,19 @@ static void make_fission_neutrons(Ctx& c, Particle& p, const MicroXS& x,
       b.parent_daughter_id = p.daughter_counter++;
       b.family_id = p.family_id;
       p.fission.push_back(b);
+      // the cancellation fields of the banked particle (:440-456)
+      abx_parent_info q{};
+      q.r1[0] = p.prev_r.x;
+      q.r1[1] = p.prev_r.y;
+      q.r1[2] = p.prev_r.z;
+      q.u1[0] = p.prev_u.x;
+      q.u1[1] = p.prev_u.y;
+      q.u1[2] = p.prev_u.z;
+      q.E1 = p.prev_E;
+      q.E3 = p.s.E;
+      q.Esmp = p.Esmp;
+      q.prev_virtual = p.prev_virtual ? 1 : 0;
+      p.fission_parent.push_back(q);
     }
   }
 }
@@ -2101,6 +2126,50 @@ static void build(Problem& P, const abx_problem_desc* d) {
       P.c_shape[3] = (uint32_t)(P.c_edges.size() - 1);
     }
   }
+  if (d->exact_cancel) {
+    // make_cancelator (src/cancelator.cpp:32-78) and the cancelators'
+    // constructors (src/exact_mg_cancelator.cpp:41-118,
+    // src/basic_exact_mg_cancelator.cpp:43-66)
+    const abx_exact_cancel& e = *d->exact_cancel;
+    if (e.kind != ABX_EXACT_MG && e.kind != ABX_BASIC_EXACT_MG)
+      fatal(ABX_E_INVALID, "Unkown cancelator type.");
+    if (d->cancel) fatal(ABX_E_INVALID, "one cancelator per problem");
+    if (P.st.tracking == ABX_SURFACE_TRACKING)
+      fatal(ABX_E_INVALID, "exect cancelators may not be used with surface-tracking.");
+    P.exact = e.kind;
+    P.ex = e;
+    P.ex_low = {e.low[0], e.low[1], e.low[2]};
+    P.ex_hi = {e.high[0], e.high[1], e.high[2]};
+    if (P.ex_low.x >= P.ex_hi.x || P.ex_low.y >= P.ex_hi.y || P.ex_low.z >= P.ex_hi.z)
+      fatal(ABX_E_INVALID, "r_low >= r_hi");
+    if (e.n_samples <= 0) fatal(ABX_E_INVALID, "n-samples must be greater than zero.");
+    for (int k = 0; k < 3; k++) {
+      if (e.shape[k] < 1) fatal(ABX_E_INVALID, "cancelator shape");
+      P.ex_shape[k] = (uint32_t)e.shape[k];
+    }
+    P.ex_pitch[0] = (P.ex_hi.x - P.ex_low.x) / (double)P.ex_shape[0];
+    P.ex_pitch[1] = (P.ex_hi.y - P.ex_low.y) / (double)P.ex_shape[1];
+    P.ex_pitch[2] = (P.ex_hi.z - P.ex_low.z) / (double)P.ex_shape[2];
+    if (e.kind == ABX_EXACT_MG) {
+      std::vector<int> seen((size_t)G, 0);
+      for (int b = 0; b < e.n_group_bins; b++) {
+        std::vector<int> grp(e.group_bins + e.group_bin_offsets[b],
+                             e.group_bins + e.group_bin_offsets[b + 1]);
+        for (int g : grp) {
+          if (g < 0 || g >= G) fatal(ABX_E_INVALID, "grp >= settings::ngroups");
+          if (seen[(size_t)g]++) fatal(ABX_E_INVALID, "Group is included more than once in group_bins.");
+        }
+        // a uniform particle of an empty energy region would index no group
+        if (grp.empty() && e.chi_matrix) fatal(ABX_E_INVALID, "empty group bin");
+        P.ex_groups.push_back(grp);
+      }
+      if (e.chi_matrix && P.ex_groups.empty())
+        fatal(ABX_E_INVALID, "Chi matrix is used, but no group_bins provided.");
+      P.ex_shape[3] = P.ex_groups.empty() ? 1u : (uint32_t)P.ex_groups.size();
+    } else if (e.beta < ABX_BETA_ZERO || e.beta > ABX_BETA_AVERAGE_G) {
+      fatal(ABX_E_INVALID, "Unkown beta entry for basic exact MG cancelator.");
+    }
+  }
   for (int i = 0; i < d->n_tallies; i++) {
     const abx_mesh_tally& a = d->tallies[i];
     MeshTally T;
@@ -2149,6 +2218,7 @@ static void transport(Problem& P, const abx_particle* bank, size_t n,
                       abx_diag& diag, abx_trace_rec* trace, int cap,
                       int32_t* n_rec) {
   std::vector<std::vector<abx_banked>> fis(n);
+  std::vector<std::vector<abx_parent_info>> par(n);
   std::string err;
   int errcode = 0;
 #ifdef _OPENMP
@@ -2181,6 +2251,7 @@ static void transport(Problem& P, const abx_particle* bank, size_t n,
           run_history_dt(c, p, P.st.tracking == ABX_CARTER_TRACKING);
         c.diag.histories++;
         fis[i] = std::move(p.fission);
+        par[i] = std::move(p.fission_parent);
         if (trace) {
           std::copy(tr.begin(), tr.end(), trace + i * (size_t)cap);
           n_rec[i] = c.n_rec;
@@ -2223,6 +2294,8 @@ static void transport(Problem& P, const abx_particle* bank, size_t n,
     fatal(errcode, err);
   }
   for (auto& v : fis) fission_out.insert(fission_out.end(), v.begin(), v.end());
+  P.last_parents.clear();
+  for (auto& v : par) P.last_parents.insert(P.last_parents.end(), v.begin(), v.end());
 }
 
 static void tally_clear(Problem& P) {
@@ -2281,6 +2354,466 @@ static void add_diag(abx_diag& a, const abx_diag& b) {
   a.tally_updates += b.tally_updates;
   a.lost += b.lost;
   a.secondaries += b.secondaries;
+}
+
+//==========================================================================
+// Exact MG cancellation: ExactMGCancelator (src/exact_mg_cancelator.cpp)
+// and BasicExactMGCancelator (src/basic_exact_mg_cancelator.cpp).
+//
+// The reference keeps its regions in nested std::unordered_maps keyed by the
+// region key (i, j, k, e) and by Material*, so the order in which regions
+// draw from settings::rng depends on the library's hashing and on heap
+// addresses.  The restatement visits them in a fixed order - (i, j, k, e)
+// lexicographically, then material index - and the library's device
+// implementation visits them in the same order; everything else follows the
+// reference step by step.
+
+// sobol::sample (vendor/sobol/include/sobol/sobol.hpp): 52-bit generator
+// matrices of the first four dimensions of the Joe-Kuo sequence
+// (new-joe-kuo-6.21201), built from its primitive polynomials and initial
+// direction numbers: dimension 0 is the van der Corput sequence, then
+// (s, a, m) = (1, 0, {1}), (2, 1, {1, 3}), (3, 1, {1, 3, 1}).
+static const uint64_t* sobol_matrices() {
+  static uint64_t M[4 * 52];
+  static bool built = false;
+  if (built) return M;
+  for (int k = 0; k < 52; k++) M[k] = 1ull << (51 - k);
+  const int S[3] = {1, 2, 3}, A[3] = {0, 1, 1};
+  const int Mi[3][3] = {{1, 0, 0}, {1, 3, 0}, {1, 3, 1}};
+  for (int d = 0; d < 3; d++) {
+    uint64_t* V = M + (d + 1) * 52;
+    const int s = S[d], a = A[d];
+    for (int k = 0; k < s; k++) V[k] = (uint64_t)Mi[d][k] << (51 - k);
+    for (int k = s; k < 52; k++) {
+      uint64_t v = V[k - s] ^ (V[k - s] >> s);
+      for (int j = 1; j < s; j++)
+        if ((a >> (s - 1 - j)) & 1) v ^= V[k - j];
+      V[k] = v;
+    }
+  }
+  built = true;
+  return M;
+}
+
+static double sobol_sample(uint64_t index, unsigned dim) {
+  const uint64_t* M = sobol_matrices() + dim * 52;
+  uint64_t r = 0;
+  for (unsigned i = 0; index; index >>= 1, ++i)
+    if (index & 1) r ^= M[i];
+  return (double)r * (1.0 / (double)(1ull << 52));
+}
+
+// settings::group (src/settings.cpp:95-104): bounds inclusive at both ends
+static size_t settings_group(const Problem& P, double E) {
+  if (P.ebounds.size() <= 1) return 0;
+  for (size_t g = 0; g < P.ebounds.size() - 1; g++)
+    if (P.ebounds[g] <= E && E <= P.ebounds[g + 1]) return g;
+  return 0;
+}
+
+// MGAngleDistribution::pdf (include/materials/mg_angle_distribution.hpp:62-75)
+static double angle_pdf(const AngleDist& a, double mu) {
+  if (mu < a.mu.front()) return a.pdf.front();
+  if (mu > a.mu.back()) return a.pdf.back();
+  auto it = std::lower_bound(a.mu.begin(), a.mu.end(), mu);
+  size_t l = (size_t)std::distance(a.mu.begin(), it);
+  if (mu == *it) return a.pdf[l];
+  l--;
+  const double m = (a.pdf[l + 1] - a.pdf[l]) / (a.mu[l + 1] - a.mu[l]);
+  return m * (mu - a.mu[l]) + a.pdf[l];
+}
+
+// get_material (src/exact_mg_cancelator.cpp:154-170): the material index at
+// r, located with u = (1, 0, 0); -1 outside every cell
+static int material_at(const Problem& P, const V3& r) {
+  Tracker t(&P, r, {1., 0., 0.});
+  if (t.cell < 0) return -1;
+  return t.material();
+}
+
+struct ExBin {  // CancelBin (include/simulation/exact_mg_cancelator.hpp:54-68)
+  double uniform_wgt = 0., uniform_wgt2 = 0., W = 0., W2 = 0.;
+  double sum_c = 0., sum_c_wgt = 0., sum_c_wgt2 = 0.;
+  uint64_t rng_seed_advance = 0;
+  bool can_cancel = true;
+  std::vector<size_t> particles;  // bank indices, in bank order
+  std::vector<double> f, f_inv;   // Averages
+};
+struct ExKey {
+  size_t i, j, k, e;
+};
+// keyed like the reference's maps: Key equality on (i, j, k, e), then the
+// material (a site on the mesh's upper faces has i, j or k equal to the
+// shape, and a group in no group bin has e equal to their number)
+using ExBins =
+    std::map<std::tuple<size_t, size_t, size_t, size_t, int>, std::pair<ExKey, ExBin>>;
+
+// ExactMGCancelator::get_f (:220-241)
+static double exact_get_f(const Problem& P, const Material& m, const V3& r1, const V3& u1,
+                          size_t g1, size_t g3, const V3& r4, size_t g4, double Esmp) {
+  const double d = norm(sub(r4, r1));
+  const V3 u = direction(r4.x - r1.x, r4.y - r1.y, r4.z - r1.z);
+  const double mu = dot(u, u1);
+  const double pdf_mu = angle_pdf(m.angle[g1][g3], mu);
+  const double pdf_chi = P.ex.chi_matrix ? m.chi[g3][g4] : 1.;
+  return (pdf_mu * pdf_chi / (d * d)) * EXP(-Esmp * d);
+}
+
+// BasicExactMGCancelator::get_f (:188-216)
+static double basic_get_f(const V3& r, const V3& rp, double Esmp) {
+  const double d = std::sqrt(std::pow(r.x - rp.x, 2.) + std::pow(r.y - rp.y, 2.) +
+                             std::pow(r.z - rp.z, 2.));
+  return (1. / (d * d)) * EXP(-Esmp * d);
+}
+
+static V3 v3(const double* a) { return {a[0], a[1], a[2]}; }
+
+// ExactMGCancelator::get_key (:121-152); nullopt -> false
+static bool exact_get_key(const Problem& P, const V3& r, size_t g, ExKey& key) {
+  if (r.x < P.ex_low.x || r.x > P.ex_hi.x || r.y < P.ex_low.y || r.y > P.ex_hi.y ||
+      r.z < P.ex_low.z || r.z > P.ex_hi.z)
+    return false;
+  key.i = (size_t)std::floor((r.x - P.ex_low.x) / P.ex_pitch[0]);
+  key.j = (size_t)std::floor((r.y - P.ex_low.y) / P.ex_pitch[1]);
+  key.k = (size_t)std::floor((r.z - P.ex_low.z) / P.ex_pitch[2]);
+  bool found = false;
+  size_t e = 0;
+  for (e = 0; e < P.ex_groups.size(); e++) {
+    for (size_t i = 0; i < P.ex_groups[e].size(); i++)
+      if ((int)g == P.ex_groups[e][i]) {
+        found = true;
+        break;
+      }
+    if (found) break;
+  }
+  if (!found && P.ex.chi_matrix) return false;
+  key.e = e;
+  return true;
+}
+
+// Sample a point of a region in a material: sobol index `si` or the rng.
+static bool region_point(const Problem& P, const ExKey& key, int mat, uint64_t* si, Pcg32* g,
+                         V3& r) {
+  const double Xl = P.ex_low.x + (double)key.i * P.ex_pitch[0];
+  const double Yl = P.ex_low.y + (double)key.j * P.ex_pitch[1];
+  const double Zl = P.ex_low.z + (double)key.k * P.ex_pitch[2];
+  for (uint32_t tries = 0; tries < 100; tries++) {  // N_MAX_POS
+    double x, y, z;
+    if (si) {
+      x = Xl + sobol_sample(*si, 0) * P.ex_pitch[0];
+      y = Yl + sobol_sample(*si, 1) * P.ex_pitch[1];
+      z = Zl + sobol_sample(*si, 2) * P.ex_pitch[2];
+      (*si)++;
+    } else {
+      x = Xl + RNG::rand(*g) * P.ex_pitch[0];
+      y = Yl + RNG::rand(*g) * P.ex_pitch[1];
+      z = Zl + RNG::rand(*g) * P.ex_pitch[2];
+    }
+    r = {x, y, z};
+    if (material_at(P, r) == mat) return true;
+  }
+  return false;
+}
+
+// Distribute the bank into regions (add_particle, exact :172-209 / basic
+// :68-113).
+static ExBins exact_bins(const Problem& P, const std::vector<abx_banked>& bank,
+                         const std::vector<abx_parent_info>& par) {
+  ExBins bins;
+  for (size_t n = 0; n < bank.size(); n++) {
+    const abx_banked& b = bank[n];
+    const V3 r = v3(b.r);
+    ExKey key{0, 0, 0, 0};
+    if (P.exact == ABX_EXACT_MG) {
+      if (!exact_get_key(P, r, settings_group(P, b.E), key)) continue;
+      if (!P.ex.use_virtual_collisions && par[n].prev_virtual) continue;
+    } else {
+      const int i = (int)std::floor((r.x - P.ex_low.x) / P.ex_pitch[0]);
+      const int j = (int)std::floor((r.y - P.ex_low.y) / P.ex_pitch[1]);
+      const int k = (int)std::floor((r.z - P.ex_low.z) / P.ex_pitch[2]);
+      if (i < 0 || j < 0 || k < 0) continue;
+      if (i >= (int)P.ex_shape[0] || j >= (int)P.ex_shape[1] || k >= (int)P.ex_shape[2]) continue;
+      key = {(size_t)i, (size_t)j, (size_t)k, 0};
+    }
+    // get_material (:154-170) has no material outside every cell; the bin
+    // would then dereference none.  A fission site is always in a cell.
+    const int mat = material_at(P, r);
+    if (mat < 0) fatal(ABX_E_INVALID, "exact cancellation: fission site outside every cell");
+    auto& kb = bins[{key.i, key.j, key.k, key.e, mat}];
+    kb.first = key;
+    kb.second.particles.push_back(n);
+    kb.second.W += b.wgt;
+    kb.second.W2 += b.wgt2;
+  }
+  return bins;
+}
+
+static void mixed_signs(const std::vector<abx_banked>& bank, const ExBin& bin, bool& m1,
+                        bool& m2) {
+  bool p1 = false, n1 = false, p2 = false, n2 = false;
+  for (size_t n : bin.particles) {
+    const abx_banked& b = bank[n];
+    if (b.wgt > 0.)
+      p1 = true;
+    else if (b.wgt < 0.)
+      n1 = true;
+    if (b.wgt2 > 0.)
+      p2 = true;
+    else if (b.wgt2 < 0.)
+      n2 = true;
+    if (p1 && n1 && p2 && n2) break;
+  }
+  m1 = p1 && n1;
+  m2 = p2 && n2;
+}
+
+// C(f, f_inv) and the bin sums of compute_averages (:343-364)
+static void bin_sums(const std::vector<abx_banked>& bank, ExBin& bin) {
+  auto C = [](double f, double f_inv) { return 1. / (2. * f * f_inv - 1.); };
+  double sum_c = 0.;
+  for (size_t i = 0; i < bin.particles.size(); i++) sum_c += C(bin.f[i], bin.f_inv[i]);
+  double sum_c_wgt = 0., sum_c_wgt2 = 0.;
+  for (size_t i = 0; i < bin.particles.size(); i++) {
+    sum_c_wgt += C(bin.f[i], bin.f_inv[i]) * bank[bin.particles[i]].wgt;
+    sum_c_wgt2 += C(bin.f[i], bin.f_inv[i]) * bank[bin.particles[i]].wgt2;
+  }
+  bin.sum_c = sum_c;
+  bin.sum_c_wgt = sum_c_wgt;
+  bin.sum_c_wgt2 = sum_c_wgt2;
+}
+
+// ExactMGCancelator::compute_averages (:294-365)
+static void exact_averages(const Problem& P, const std::vector<abx_banked>& bank,
+                           const std::vector<abx_parent_info>& par, const ExKey& key, int mat,
+                           ExBin& bin) {
+  bin.f.assign(bin.particles.size(), 0.);
+  bin.f_inv.assign(bin.particles.size(), 0.);
+  std::vector<std::pair<V3, size_t>> smp;
+  uint64_t si = 0;
+  for (uint32_t j = 0; j < (uint32_t)P.ex.n_samples; j++) {
+    V3 r;
+    if (!region_point(P, key, mat, &si, nullptr, r)) {
+      bin.can_cancel = false;
+      return;
+    }
+    size_t g = 0;
+    if (P.ex.chi_matrix) {  // sample_point (:282-289): the 4th coordinate of i - 1
+      const double xi = sobol_sample(si - 1, 3);
+      const size_t gi = (size_t)std::floor(xi * (double)P.ex_groups[key.e].size());
+      g = (size_t)P.ex_groups[key.e][gi];
+    }
+    smp.push_back({r, g});
+  }
+  const Material& m = P.materials[(size_t)mat];
+  for (size_t i = 0; i < bin.particles.size(); i++) {
+    const abx_parent_info& q = par[bin.particles[i]];
+    const size_t g1 = settings_group(P, q.E1), g3 = settings_group(P, q.E3);
+    double sf = 0., sfi = 0.;
+    for (const auto& rs : smp) {
+      const double f = exact_get_f(P, m, v3(q.r1), v3(q.u1), g1, g3, rs.first, rs.second, q.Esmp);
+      if (f == 0.) {
+        bin.can_cancel = false;
+        return;
+      }
+      sf += f;
+      sfi += 1. / f;
+    }
+    bin.f[i] = sf / (double)P.ex.n_samples;
+    bin.f_inv[i] = sfi / (double)P.ex.n_samples;
+  }
+  bin_sums(bank, bin);
+}
+
+// BasicExactMGCancelator::get_averages(_sobol) (:243-407)
+static void basic_averages(const Problem& P, const std::vector<abx_banked>& bank,
+                           const std::vector<abx_parent_info>& par, const ExKey& key, int mat,
+                           ExBin& bin, Pcg32& g) {
+  bin.f.assign(bin.particles.size(), 0.);
+  bin.f_inv.assign(bin.particles.size(), 0.);
+  std::vector<V3> smp;
+  uint64_t si = 0;
+  for (uint32_t j = 0; j < (uint32_t)P.ex.n_samples; j++) {
+    V3 r;
+    if (!region_point(P, key, mat, P.ex.sobol ? &si : nullptr, &g, r)) {
+      bin.can_cancel = false;
+      return;
+    }
+    smp.push_back(r);
+  }
+  for (size_t i = 0; i < bin.particles.size(); i++) {
+    const abx_parent_info& q = par[bin.particles[i]];
+    double sf = 0., sfi = 0.;
+    for (const V3& r : smp) {
+      const double f = basic_get_f(r, v3(q.r1), q.Esmp);
+      sf += f;
+      sfi += 1. / f;
+    }
+    bin.f[i] = sf / (double)P.ex.n_samples;
+    bin.f_inv[i] = sfi / (double)P.ex.n_samples;
+  }
+  if (P.ex.beta == ABX_BETA_AVERAGE_G) bin_sums(bank, bin);
+}
+
+// BasicExactMGCancelator::get_min_f (:218-241)
+static double basic_min_f(const Problem& P, const ExKey& key, const V3& rp, double Esmp) {
+  const double Xl = P.ex_low.x + (double)key.i * P.ex_pitch[0], Xh = Xl + P.ex_pitch[0];
+  const double Yl = P.ex_low.y + (double)key.j * P.ex_pitch[1], Yh = Yl + P.ex_pitch[1];
+  const double Zl = P.ex_low.z + (double)key.k * P.ex_pitch[2], Zh = Zl + P.ex_pitch[2];
+  const double f1 = basic_get_f({Xh, Yh, Zh}, rp, Esmp), f2 = basic_get_f({Xh, Yh, Zl}, rp, Esmp);
+  const double f3 = basic_get_f({Xh, Yl, Zh}, rp, Esmp), f4 = basic_get_f({Xh, Yl, Zl}, rp, Esmp);
+  const double f5 = basic_get_f({Xl, Yh, Zh}, rp, Esmp), f6 = basic_get_f({Xl, Yh, Zl}, rp, Esmp);
+  const double f7 = basic_get_f({Xl, Yl, Zh}, rp, Esmp), f8 = basic_get_f({Xl, Yl, Zl}, rp, Esmp);
+  return std::min(std::min(std::min(f1, f2), std::min(f3, f4)),
+                  std::min(std::min(f5, f6), std::min(f7, f8)));
+}
+
+// cancel_bin of either cancelator (exact :367-403, basic :438-468)
+static void cancel_region(const Problem& P, std::vector<abx_banked>& bank,
+                          const std::vector<abx_parent_info>& par, const ExKey& key, int mat,
+                          ExBin& bin, bool first) {
+  const bool basic = P.exact == ABX_BASIC_EXACT_MG;
+  for (size_t i = 0; i < bin.particles.size(); i++) {
+    abx_banked& b = bank[bin.particles[i]];
+    const abx_parent_info& q = par[bin.particles[i]];
+    const double wgt = first ? b.wgt : b.wgt2;
+    if (basic && wgt == 0.) return;
+    // get_beta
+    double B = 0.;
+    if (bin.can_cancel) {
+      if (!basic || P.ex.beta == ABX_BETA_AVERAGE_G) {
+        const double S = (first ? bin.sum_c_wgt : bin.sum_c_wgt2) / (1. + bin.sum_c);
+        const double f = bin.f[i], f_inv = bin.f_inv[i];
+        B = f * (1. / (2. * f * f_inv - 1.)) * (1. - (S / wgt));
+      } else if (P.ex.beta == ABX_BETA_MINIMUM) {
+        B = basic_min_f(P, key, v3(q.r1), q.Esmp);
+      } else if (P.ex.beta == ABX_BETA_AVERAGE_F) {
+        const double N = (double)bin.particles.size();
+        const double W = first ? bin.W : bin.W2;
+        B = bin.f[i] * (1. - ((W) / ((N + 1.) * wgt)));
+      }
+    }
+    double f;
+    if (basic) {
+      f = basic_get_f(v3(b.r), v3(q.r1), q.Esmp);
+    } else {
+      f = exact_get_f(P, P.materials[(size_t)mat], v3(q.r1), v3(q.u1), settings_group(P, q.E1), settings_group(P, q.E3),
+                      v3(b.r), settings_group(P, b.E), q.Esmp);
+    }
+    const double P_p = (f - B) / f;
+    const double P_u = B / f;
+    if (std::isinf(P_u) || std::isinf(P_p) || std::isnan(P_u) || std::isnan(P_p)) return;
+    if (first) {
+      bin.uniform_wgt += b.wgt * P_u;
+      b.wgt *= P_p;
+    } else {
+      bin.uniform_wgt2 += b.wgt2 * P_u;
+      b.wgt2 *= P_p;
+    }
+  }
+}
+
+// perform_cancellation + get_new_particles (exact :405-605, basic
+// :471-611) with settings::rng; the new particles are appended to the bank.
+static void exact_cancellation(Problem& P, std::vector<abx_banked>& bank,
+                               const std::vector<abx_parent_info>& par) {
+  if (P.exact == ABX_BASIC_EXACT_MG && P.ex.beta == ABX_BETA_ZERO) return;
+  ExBins bins = exact_bins(P, bank, par);
+  if (bins.empty()) return;
+  const bool basic = P.exact == ABX_BASIC_EXACT_MG;
+  // basic: each region's rng stream is settings::rng advanced past the
+  // draws every earlier region may take (:475-489)
+  uint64_t seed_advance = 0;
+  if (basic && P.ex.beta != ABX_BETA_MINIMUM) {
+    const uint64_t max_rn = (uint64_t)P.ex.n_samples * 100u *
+                            (P.ex.beta == ABX_BETA_AVERAGE_G ? 2u : 1u);
+    for (auto& kv : bins) {
+      kv.second.second.rng_seed_advance = seed_advance;
+      seed_advance += kv.second.second.particles.size() * max_rn;
+    }
+  }
+  for (auto& kv : bins) {
+    const ExKey& key = kv.second.first;
+    ExBin& bin = kv.second.second;
+    const int mat = std::get<4>(kv.first);
+    Pcg32 local = P.grng;
+    local.advance(bin.rng_seed_advance);
+    if (bin.particles.size() > 1) {
+      bool m1, m2;
+      mixed_signs(bank, bin, m1, m2);
+      if (m1 || m2) {
+        if (!basic)
+          exact_averages(P, bank, par, key, mat, bin);
+        else if (P.ex.beta == ABX_BETA_AVERAGE_F || P.ex.beta == ABX_BETA_AVERAGE_G)
+          basic_averages(P, bank, par, key, mat, bin, local);
+      }
+      if (m1) cancel_region(P, bank, par, key, mat, bin, true);
+      if (m2) cancel_region(P, bank, par, key, mat, bin, false);
+      bin.particles.clear();
+      bin.f.clear();
+      bin.f_inv.clear();
+      bin.sum_c = bin.sum_c_wgt = bin.sum_c_wgt2 = 0.;
+    }
+  }
+  if (basic && P.ex.beta != ABX_BETA_MINIMUM) P.grng.advance(seed_advance);
+  // get_new_particles
+  std::vector<abx_banked> uni;
+  for (auto& kv : bins) {
+    const ExKey& key = kv.second.first;
+    ExBin& bin = kv.second.second;
+    const int mat = std::get<4>(kv.first);
+    const uint32_t N =
+        (uint32_t)std::ceil(std::max(std::abs(bin.uniform_wgt), std::abs(bin.uniform_wgt2)));
+    if (N > 0) {
+      const double w = bin.uniform_wgt / N, w2 = bin.uniform_wgt2 / N;
+      const Material& m = P.materials[(size_t)mat];
+      for (uint32_t i = 0; i < N; i++) {
+        V3 r;
+        if (!region_point(P, key, mat, nullptr, &P.grng, r))
+          fatal(ABX_E_INVALID, "Couldn't sample position for uniform particle.");
+        abx_banked u{};
+        u.r[0] = r.x;
+        u.r[1] = r.y;
+        u.r[2] = r.z;
+        if (!basic) {
+          size_t e_index = 0;
+          if (P.ex.chi_matrix) {
+            const double xi_E = RNG::rand(P.grng);
+            const size_t gi = (size_t)std::floor(xi_E * (double)P.ex_groups[key.e].size());
+            e_index = (size_t)P.ex_groups[key.e][gi];
+          } else {
+            e_index = (size_t)RNG::discrete(P.grng, m.chi[0].data(), m.chi[0].data() + m.chi[0].size());
+          }
+          u.E = 0.5 * (P.ebounds[e_index] + P.ebounds[e_index + 1]);
+          // Direction u_smp(2 rand - 1, 2 pi rand) (:573): the order of the
+          // two draws is the compiler's; GCC, the reference's usual
+          // compiler, evaluates the arguments right to left - phi first
+          const double phi = 2. * PI * RNG::rand(P.grng);
+          const double mu = 2. * RNG::rand(P.grng) - 1.;
+          const V3 d = direction_mu_phi(mu, phi);
+          u.u[0] = d.x;
+          u.u[1] = d.y;
+          u.u[2] = d.z;
+        } else {
+          // MGNuclide::sample_fission(0, (0, 0, 1), 0, 0, rng) (src/mg_nuclide.cpp:504-544)
+          const auto& chi = m.chi[0];
+          const size_t ei = (size_t)RNG::discrete(P.grng, chi.data(), chi.data() + chi.size());
+          u.E = 0.5 * (P.ebounds[ei] + P.ebounds[ei + 1]);
+          const double mu = 2. * RNG::rand(P.grng) - 1.;
+          const double phi = 2. * PI * RNG::rand(P.grng);
+          const V3 d = rotate_direction({0., 0., 1.}, mu, phi);
+          u.u[0] = d.x;
+          u.u[1] = d.y;
+          u.u[2] = d.z;
+          (void)RNG::rand(P.grng);  // the delayed-neutron test, P_delayed = 0
+        }
+        u.wgt = w;
+        u.wgt2 = w2;
+        uni.push_back(u);
+      }
+    }
+    bin.uniform_wgt = bin.uniform_wgt2 = 0.;
+  }
+  bank.insert(bank.end(), uni.begin(), uni.end());
 }
 
 // ApproximateMeshCancelator::add_particle + perform_cancellation
@@ -2350,7 +2883,10 @@ static void run(Problem& P, int ngen, int nignored, int nthreads,
   bool converged = P.st.mode != ABX_K_EIGENVALUE || nignored == 0;
   S = abx_run_stats{};
   double seconds = 0;
-  // a run starts its statistics afresh (as abx_run does)
+  // a run starts its statistics afresh (as abx_run does), and
+  // settings::rng from the seed (settings::initialize_global_rng,
+  // src/settings.cpp:116-118)
+  P.grng.seed(P.st.seed);
   P.gen = 0;
   P.k_col_avg = P.k_col_var = P.k_abs_avg = P.k_abs_var = P.k_trk_avg = P.k_trk_var = 0;
   P.k_tot_avg = P.k_tot_var = P.leak_avg = P.leak_var = P.mig_avg = P.mig_var = 0;
@@ -2415,6 +2951,7 @@ static void run(Problem& P, int ngen, int nignored, int nthreads,
       if (kcol_hist) kcol_hist[g - 1] = kc;
       push_gen_values(P, sc, W);
       if (P.cancel) approximate_cancellation(P, next);
+      if (P.exact) exact_cancellation(P, next, P.last_parents);
       // normalize_weights (src/power_iterator.cpp:538-586)
       double W_pos = 0., W_neg = 0.;
       for (auto& b : next) {
@@ -2485,6 +3022,7 @@ extern "C" {
 const char* orc_last_error(void) { return g_last.c_str(); }
 
 double orc_log(int portable, double x) { return portable ? dm_log(x) : std::log(x); }
+double orc_exp(int portable, double x) { return portable ? dm_exp(x) : std::exp(x); }
 double orc_sin(int portable, double x) { return portable ? dm_sin(x) : std::sin(x); }
 double orc_cos(int portable, double x) { return portable ? dm_cos(x) : std::cos(x); }
 
@@ -2508,6 +3046,7 @@ int orc_create(const abx_problem_desc* desc, int portable_math,
     p->P.portable = portable_math != 0;
     g_portable = p->P.portable;
     build(p->P, desc);
+    p->P.grng.seed(p->P.st.seed);
     *out = p;
     return ABX_OK;
   } catch (const Fatal& f) {
@@ -2586,6 +3125,33 @@ int orc_tally_record(orc_problem* p, double multiplier) {
     }
   });
 }
+
+int orc_parents_fetch(orc_problem* p, abx_parent_info* out, size_t cap, size_t* n) {
+  ORC_TRY({
+    *n = p->P.last_parents.size();
+    if (*n > cap) fatal(ABX_E_CAPACITY, "parent buffer too small");
+    std::copy(p->P.last_parents.begin(), p->P.last_parents.end(), out);
+  });
+}
+
+int orc_exact_cancel(orc_problem* p, const abx_banked* bank, const abx_parent_info* par, size_t n,
+                     abx_banked* out, size_t cap, size_t* n_out) {
+  ORC_TRY({
+    if (!p->P.exact) fatal(ABX_E_INVALID, "no exact cancelator in this problem");
+    std::vector<abx_banked> b(bank, bank + n);
+    std::vector<abx_parent_info> q(par, par + n);
+    exact_cancellation(p->P, b, q);
+    *n_out = b.size();
+    if (b.size() > cap) fatal(ABX_E_CAPACITY, "cancelled bank buffer too small");
+    std::copy(b.begin(), b.end(), out);
+  });
+}
+
+int orc_rng_reset(orc_problem* p) { ORC_TRY(p->P.grng.seed(p->P.st.seed)); }
+
+uint64_t orc_sobol_matrix(unsigned dim, unsigned k) { return sobol_matrices()[dim * 52 + k]; }
+
+double orc_sobol(uint64_t index, unsigned dim) { return sobol_sample(index, dim); }
 
 int orc_run(orc_problem* p, int ngenerations, int nignored, int nthreads,
             double* kcol_hist, abx_run_stats* stats) {

@@ -67,7 +67,17 @@ int orc_cancel(orc_problem* p, abx_banked* bank, size_t n);
 int orc_majorant(orc_problem* p, double* maj);
 
 /* Math used by the restatement. */
+/* parents of the last transport's fission sites (exact cancellation) */
+int orc_parents_fetch(orc_problem* p, abx_parent_info* out, size_t cap, size_t* n);
+/* one exact / basic-exact cancellation pass over a bank with its parents:
+ * out = the bank with cancelled weights, then the uniform particles */
+int orc_exact_cancel(orc_problem* p, const abx_banked* bank, const abx_parent_info* par, size_t n,
+                     abx_banked* out, size_t cap, size_t* n_out);
+int orc_rng_reset(orc_problem* p); /* settings::rng back to the seed */
+uint64_t orc_sobol_matrix(unsigned dim, unsigned k);
+double orc_sobol(uint64_t index, unsigned dim);
 double orc_log(int portable, double x);
+double orc_exp(int portable, double x);
 double orc_sin(int portable, double x);
 double orc_cos(int portable, double x);
 

@@ -53,7 +53,15 @@ def lib():
                                            C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_majorant.argtypes = [P, C.c_void_p]
         L.orc_cancel.argtypes = [P, C.c_void_p, C.c_size_t]
-        for f in ("orc_log", "orc_sin", "orc_cos"):
+        L.orc_parents_fetch.argtypes = [P, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]
+        L.orc_exact_cancel.argtypes = [P, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
+                                       C.c_size_t, C.POINTER(C.c_size_t)]
+        L.orc_rng_reset.argtypes = [P]
+        L.orc_sobol_matrix.argtypes = [C.c_uint, C.c_uint]
+        L.orc_sobol_matrix.restype = C.c_uint64
+        L.orc_sobol.argtypes = [C.c_uint64, C.c_uint]
+        L.orc_sobol.restype = C.c_double
+        for f in ("orc_log", "orc_exp", "orc_sin", "orc_cos"):
             getattr(L, f).argtypes = [C.c_int, C.c_double]
             getattr(L, f).restype = C.c_double
         _lib = L
@@ -169,6 +177,30 @@ class Oracle:
         _check(lib().orc_nearest_boundary(self.h, r.ctypes.data, u.ctypes.data, n,
                                           d.ctypes.data, s.ctypes.data, b.ctypes.data))
         return d, s, b
+
+    def parents(self):
+        """Parent information of the last transport's fission sites."""
+        n = C.c_size_t()
+        lib().orc_parents_fetch(self.h, None, 0, C.byref(n))  # the size
+        out = np.zeros(max(1, n.value), abi.PARENT_DTYPE)
+        _check(lib().orc_parents_fetch(self.h, out.ctypes.data, len(out), C.byref(n)))
+        return out[:n.value].copy()
+
+    def exact_cancel(self, bank, parents):
+        """One exact / basic-exact cancellation pass: the bank with cancelled
+        weights followed by the uniform particles (settings::rng advances)."""
+        b = np.ascontiguousarray(bank, dtype=abi.BANKED_DTYPE)
+        q = np.ascontiguousarray(parents, dtype=abi.PARENT_DTYPE)
+        assert len(b) == len(q)
+        cap = 4 * len(b) + 4096
+        out = np.zeros(cap, abi.BANKED_DTYPE)
+        n = C.c_size_t()
+        _check(lib().orc_exact_cancel(self.h, b.ctypes.data, q.ctypes.data, len(b),
+                                      out.ctypes.data, cap, C.byref(n)))
+        return out[:n.value].copy()
+
+    def rng_reset(self):
+        _check(lib().orc_rng_reset(self.h))
 
     def cancel(self, bank):
         b = np.ascontiguousarray(bank, dtype=abi.BANKED_DTYPE).copy()

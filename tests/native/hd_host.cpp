@@ -7,13 +7,16 @@
 #include <string>
 #include <vector>
 
+#include "../../abeille_amd/csrc/exact.h"
 #include "../../abeille_amd/csrc/source.h"
 #include "../../abeille_amd/csrc/tables.h"
 
 struct HostSink {
   static constexpr bool split = false;
+  static constexpr bool ext = true;  // keeps the parent records
   std::vector<SecRec> stack;
   std::vector<abx_banked>* bank;
+  std::vector<abx_parent_info>* parents;
   std::vector<abx_trace_rec>* trace_buf;
   int trace_cap = 0;
   int32_t n_rec = 0;
@@ -44,6 +47,11 @@ struct HostSink {
     b.parent_daughter_id = p.daughters;
     b.family_id = p.fam;
     bank->push_back(b);
+    abx_parent_info q{};
+    q.r1[0] = p.pr.x; q.r1[1] = p.pr.y; q.r1[2] = p.pr.z;
+    q.u1[0] = p.pu.x; q.u1[1] = p.pu.y; q.u1[2] = p.pu.z;
+    q.E1 = p.pE; q.E3 = p.E; q.Esmp = p.Esmp; q.prev_virtual = p.pvirt;
+    parents->push_back(q);
   }
   template <class TK>
   void trace_rec(const Part& p, const TK& t, int ev, const DevProblem& P) {
@@ -84,6 +92,7 @@ struct hd_problem {
   DevProblem P;
   std::vector<double> src_cdf;
   DevSources S;
+  std::vector<abx_parent_info> parents;  // one per banked site of the last transport
 };
 
 static std::string g_err;
@@ -137,12 +146,14 @@ int hd_transport(hd_problem* h, const abx_particle* bank, size_t n, double kcol,
                  abx_scores* scores, abx_diag* diag, abx_trace_rec* trace,
                  int trace_cap, int32_t* n_rec) {
   std::vector<abx_banked> fb;
+  h->parents.clear();
   Scores6 sc{0, 0, 0, 0, 0, 0};
   abx_diag dg{};
   for (size_t i = 0; i < n; i++) {
     HostSink s;
     s.Pp = &h->P;
     s.bank = &fb;
+    s.parents = &h->parents;
     std::vector<abx_trace_rec> tr;
     s.trace_buf = trace ? &tr : nullptr;
     s.trace_cap = trace_cap;
@@ -159,6 +170,7 @@ int hd_transport(hd_problem* h, const abx_particle* bank, size_t n, double kcol,
     p.hid = a.history_id;
     p.fam = a.family_id;
     p.alive = 1;
+    p.pu = V3{1., 0., 0.};
     HostCursor cur(h->P);
     switch (h->P.tracking) {
       case 0: run_history<0>(h->P, s, p, cur.t, kcol, converged != 0, sc); break;
@@ -188,6 +200,13 @@ int hd_transport(hd_problem* h, const abx_particle* bank, size_t n, double kcol,
   scores->k_col = sc.k_col; scores->k_abs = sc.k_abs; scores->k_trk = sc.k_trk;
   scores->k_tot = sc.k_tot; scores->leakage = sc.leak; scores->mig = sc.mig;
   *diag = dg;
+  return 0;
+}
+
+int hd_parents_fetch(hd_problem* h, abx_parent_info* out, size_t cap, size_t* n) {
+  *n = h->parents.size();
+  if (h->parents.size() > cap) return ABX_E_CAPACITY;
+  std::copy(h->parents.begin(), h->parents.end(), out);
   return 0;
 }
 
@@ -235,5 +254,15 @@ uint32_t hd_pcg_seeded_stream(uint64_t state_seed, uint64_t stream, int n, uint3
   return 0;
 }
 double hd_log(double x) { return dm_log(x); }
+double hd_sobol(uint64_t index, int dim) {
+  static uint64_t M[kSobolDims * kSobolBits];
+  static bool built = false;
+  if (!built) {
+    sobol_build(M);
+    built = true;
+  }
+  return sobol_at(M, index, dim);
+}
+double hd_exp(double x) { return dm_exp(x); }
 void hd_sincos(double x, double* s, double* c) { dm_sincos(x, s, c); }
 }

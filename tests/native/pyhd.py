@@ -1,6 +1,7 @@
 """ctypes wrapper of tests/native/libhdhost.so (device transport code built
 for the host; tests only)."""
 import ctypes as C
+import fcntl
 import os
 import subprocess
 
@@ -16,7 +17,12 @@ _lib = None
 def lib():
     global _lib
     if _lib is None:
-        subprocess.run(["make", "-s", "-C", HERE], check=True)
+        # one build at a time when test workers run in parallel
+        os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
+        with open(os.path.join(HERE, "_build", ".lock"), "w") as lk:
+            fcntl.flock(lk, fcntl.LOCK_EX)
+            subprocess.run(["make", "-s", "-C", HERE], check=True)
+            fcntl.flock(lk, fcntl.LOCK_UN)
         L = C.CDLL(LIB)
         P = C.c_void_p
         L.hd_last_error.restype = C.c_char_p
@@ -29,6 +35,9 @@ def lib():
                                    C.POINTER(abi.Scores), C.POINTER(abi.Diag),
                                    C.c_void_p, C.c_int, C.c_void_p]
         L.hd_tally_fetch.argtypes = [P, C.c_void_p]
+        L.hd_parents_fetch.argtypes = [P, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]
+        L.hd_sobol.argtypes = [C.c_uint64, C.c_int]
+        L.hd_sobol.restype = C.c_double
         _lib = L
     return _lib
 
@@ -46,6 +55,16 @@ class HostDevice:
     def __del__(self):
         if getattr(self, "h", None):
             lib().hd_destroy(self.h)
+
+    def parents(self):
+        """Parent records of the last transport's fission sites."""
+        n = C.c_size_t()
+        lib().hd_parents_fetch(self.h, None, 0, C.byref(n))
+        out = np.zeros(max(1, n.value), abi.PARENT_DTYPE)
+        rc = lib().hd_parents_fetch(self.h, out.ctypes.data, len(out), C.byref(n))
+        if rc:
+            raise RuntimeError(f"hd_parents_fetch rc={rc}")
+        return out[: n.value].copy()
 
     def sample_sources(self, first, n):
         out = np.zeros(n, abi.PARTICLE_DTYPE)
