@@ -6,9 +6,9 @@ moments linearised like src/legendre_distribution.cpp:105-155), `surfaces`,
 `cells` (region strings), `universes` (cell lists, `rectlinear` or
 `hexagonal` lattices), `root-universe`, `sources` (point / box, isotropic /
 mono-directional, mono-energetic), `tallies` (mesh tallies with
-collision / track-length estimators), `settings`, `sampling-xs-ratio` and an
-approximate `cancelator`.  Keys outside that subset (continuous energy,
-noise, exact cancelators, ...) raise ValueError.
+collision / track-length estimators), `settings`, `sampling-xs-ratio` and a
+`cancelator` (approximate, exact or basic-exact, src/cancelator.cpp:32-78).
+Keys outside that subset (continuous energy, noise, ...) raise ValueError.
 """
 import math
 
@@ -73,6 +73,9 @@ def _vec(node, key, G, name):
 
 
 def _material(m, node, G):
+    """Adds the material; returns (chi given as a matrix, virtual collisions
+    allowed), the flags MGNuclide's reader sets for exact cancellation
+    (src/mg_nuclide.cpp:639, 713, 808-810)."""
     mid = node["id"]
     total = _vec(node, "total", G, mid)
     absorption = _vec(node, "absorption", G, mid)
@@ -94,6 +97,9 @@ def _material(m, node, G):
         kw["delayed_probabilities"] = dg["probabilities"]
         kw["delayed_constants"] = dg["constants"]
     m.material(mid, total=total, absorption=absorption, scatter=scatter, **kw)
+    fissile = "fission" in node and any(float(v) > 0. for v in kw["fission"])
+    chi_matrix = fissile and "chi" in node and len(node["chi"]) == G
+    virtual_ok = all(float(scatter[g][g]) != 0. for g in range(G))
     moments = {}
     for l in range(1, 6):
         key = f"P{l}"
@@ -107,6 +113,61 @@ def _material(m, node, G):
                           for l in range(1, max(moments) + 1)]
                 angles.append(linearize_legendre(coeffs))
         m.materials[-1]["angles"] = angles
+        for g in range(G):  # in-group pdf at mu = 1
+            mu, pdf, _ = angles[g * G + g]
+            if float(np.interp(1.0, mu, pdf)) == 0.:
+                virtual_ok = False
+    return chi_matrix, virtual_ok
+
+
+def _triple(c, key, what):
+    v = c.get(key)
+    if not isinstance(v, (list, tuple)) or len(v) != 3:
+        raise ValueError(f"No valid {key} entry for {what}.")
+    return [float(x) for x in v]
+
+
+def _cancelator(m, c, tracking, chi_matrix, virtual_ok):
+    """make_cancelator (src/cancelator.cpp:32-78) and the exact cancelators'
+    readers (src/exact_mg_cancelator.cpp:595-680,
+    src/basic_exact_mg_cancelator.cpp:625-710)."""
+    if not c or not isinstance(c.get("type"), str):
+        raise ValueError("Invalid type entry for cancelator.")
+    t = c["type"]
+    if t == "approximate":
+        m.cancellation_mesh(c["low"], c["hi"], c["shape"], c.get("energy-bounds"))
+        return
+    if t not in ("exact", "basic-exact"):
+        raise ValueError(f"Unkown cancelator type \"{t}\".")
+    if tracking == abi.SURFACE_TRACKING:
+        raise ValueError(("basic-exect" if t == "basic-exact" else "exect") +
+                         " cancelators may not be used with surface-tracking.")
+    what = "basic exact MG cancelator"
+    low, hi = _triple(c, "low", what), _triple(c, "hi", what)
+    shape = c.get("shape")
+    if not isinstance(shape, (list, tuple)) or len(shape) != 3:
+        raise ValueError(f"No valid shape entry for {what}.")
+    n_samples = c.get("n-samples", 10)
+    if not isinstance(n_samples, int):
+        raise ValueError("Invalid n-samples entry for cancelator.")
+    if n_samples == 0:
+        raise ValueError("n-samples must be greater than zero.")
+    if t == "exact":
+        gb = c.get("group-bins")
+        gb = [[int(g) for g in b] for b in gb] if isinstance(gb, list) else None
+        m.exact_cancellation(low, hi, [int(x) for x in shape], kind="exact",
+                             n_samples=n_samples, group_bins=gb, chi_matrix=chi_matrix,
+                             use_virtual_collisions=virtual_ok)
+        return
+    beta = c.get("beta")
+    if not isinstance(beta, str):
+        raise ValueError(f"No valid beta entry for {what}.")
+    sobol = c.get("sobol", True)
+    if not isinstance(sobol, bool):
+        raise ValueError("Invalid sobol entry for cancelator.")
+    m.exact_cancellation(low, hi, [int(x) for x in shape], kind="basic", n_samples=n_samples,
+                         beta=beta, sobol=sobol, chi_matrix=chi_matrix,
+                         use_virtual_collisions=virtual_ok)
 
 
 SURFACE_PARAMS = {
@@ -129,8 +190,11 @@ def model_from_deck(deck):
     G = int(st["ngroups"])
     ebounds = [float(x) for x in st["energy-bounds"]]
     m = Model()
+    chi_matrix, virtual_ok = False, True
     for node in deck["materials"]:
-        _material(m, node, G)
+        cm, vo = _material(m, node, G)
+        chi_matrix = chi_matrix or cm
+        virtual_ok = virtual_ok and vo
     for node in deck["surfaces"]:
         t = node["type"]
         if t not in SURFACE_PARAMS:
@@ -203,10 +267,7 @@ def model_from_deck(deck):
             raise ValueError("Must provide positive \"sampling-xs-ratio\" for carter tracking.")
         m.set(sample_xs_ratio=[float(v) for v in ratio])
     if st.get("cancellation", False):
-        c = deck.get("cancelator")
-        if not c or c.get("type") != "approximate":
-            raise ValueError("only the approximate cancelator is supported")
-        m.cancellation_mesh(c["low"], c["hi"], c["shape"], c.get("energy-bounds"))
+        _cancelator(m, deck.get("cancelator"), tracking, chi_matrix, virtual_ok)
     m.generations = int(st.get("ngenerations", 1))
     m.ignored = int(st.get("nignored", 0))
     return m

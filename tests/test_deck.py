@@ -77,3 +77,78 @@ def test_general_cylinder_surface_in_a_deck():
     m.set(nparticles=4000)
     st, _ = Oracle(m).run(40, 10)
     assert abs(st.kcol_avg - 1.0) < 4 * st.kcol_err
+
+
+def _exact_deck(cancelator, transport="carter-tracking", chi=None, p1=None):
+    """A one-group slab deck (the PUa-1-0-SL material) with a cancelator."""
+    mat = {"id": 1, "total": [0.32640], "absorption": [0.101184], "scatter": [[0.225216]],
+           "fission": [0.081600], "nu": [3.24], "chi": chi or [[1.0]]}
+    if p1 is not None:
+        mat["P1"] = [[p1]]
+    return {
+        "materials": [mat],
+        "surfaces": [{"id": 1, "type": "xplane", "x0": -1.853722, "boundary": "vacuum"},
+                     {"id": 2, "type": "xplane", "x0": 1.853722, "boundary": "vacuum"},
+                     {"id": 3, "type": "yplane", "y0": -5., "boundary": "reflective"},
+                     {"id": 4, "type": "yplane", "y0": 5., "boundary": "reflective"},
+                     {"id": 5, "type": "zplane", "z0": -5., "boundary": "reflective"},
+                     {"id": 6, "type": "zplane", "z0": 5., "boundary": "reflective"}],
+        "cells": [{"id": 1, "region": "+1 & -2 & +3 & -4 & +5 & -6", "material": 1}],
+        "universes": [{"id": 1, "cells": [1]}],
+        "root-universe": 1,
+        "sources": [{"spatial": {"type": "point", "position": [0., 0., 0.]},
+                     "direction": {"type": "isotropic"},
+                     "energy": {"type": "mono-energetic", "energy": 0.5}, "weight": 1.}],
+        "sampling-xs-ratio": [0.9],
+        "cancelator": cancelator,
+        "settings": {"nparticles": 500, "ngenerations": 4, "nignored": 1, "ngroups": 1,
+                     "energy-bounds": [0., 1.], "transport": transport,
+                     "simulation": "k-eigenvalue", "cancellation": True},
+    }
+
+
+def test_exact_cancelators_in_a_deck():
+    """`type: exact` and `type: basic-exact` (src/cancelator.cpp:32-78):
+    group-bins, n-samples (default 10), beta, sobol (default true); a chi
+    given as a list of ngroups rows is a matrix (here 1 x 1,
+    src/mg_nuclide.cpp:808-810), so the exact cancelator needs group-bins."""
+    box = {"low": [-1.853722, -5., -5.], "hi": [1.853722, 5., 5.], "shape": [4, 1, 1]}
+    m = model_from_deck(_exact_deck(dict(box, type="exact", **{"group-bins": [[0]]})))
+    assert m.exact["kind"] == "exact" and m.exact["n_samples"] == 10
+    assert m.exact["group_bins"] == [[0]] and m.exact["chi_matrix"]
+    assert m.exact["use_virtual_collisions"]
+    Oracle(m).run(3, 1)
+    m = model_from_deck(_exact_deck(dict(box, type="basic-exact", beta="minimum", sobol=False,
+                                         **{"n-samples": 4}), transport="delta-tracking"))
+    assert (m.exact["kind"], m.exact["beta"], m.exact["sobol"], m.exact["n_samples"]) == \
+        ("basic", "minimum", False, 4)
+    Oracle(m).run(3, 1)
+    # P1 = -1/3: the in-group pdf 1/2 - mu/2 vanishes at mu = 1 (mg_nuclide.cpp:713)
+    m = model_from_deck(_exact_deck(dict(box, type="exact", **{"group-bins": [[0]]}),
+                                    p1=-1.0 / 3.0))
+    assert not m.exact["use_virtual_collisions"]
+    with pytest.raises(ValueError, match="surface-tracking"):
+        model_from_deck(_exact_deck(dict(box, type="basic-exact", beta="zero"),
+                                    transport="surface-tracking"))
+    with pytest.raises(ValueError, match="beta"):
+        model_from_deck(_exact_deck(dict(box, type="basic-exact")))
+    with pytest.raises(ValueError, match="Unkown cancelator type"):
+        model_from_deck(_exact_deck(dict(box, type="mesh")))
+    m = model_from_deck(_exact_deck(dict(box, type="exact")))
+    with pytest.raises(Exception, match="Chi matrix is used"):
+        Oracle(m)
+
+
+@pytest.mark.skipif(not os.path.isdir(REF), reason="reference decks not present")
+def test_c5g7_basic_exact_cancelator_block():
+    """The C5G7 deck's cancelator block (basic-exact, average-g, sobol) once
+    cancellation is switched on; its `nsamples` key is not the reader's
+    `n-samples`, so the default of 10 points stays."""
+    import yaml
+    with open(os.path.join(REF, "c5g7.yaml")) as f:
+        deck = yaml.safe_load(f)
+    deck["settings"]["cancellation"] = True
+    m = model_from_deck(deck)
+    e = m.exact
+    assert (e["kind"], e["beta"], e["sobol"], e["n_samples"]) == ("basic", "average-g", True, 10)
+    assert e["shape"] == [170, 170, 765] and e["low"] == [-32.13, -32.13, -107.11]
