@@ -78,6 +78,9 @@ def lib():
         L.abx_batch_kernel_seconds.argtypes = [P, C.POINTER(C.c_double)]
         L.abx_batch_set_step_histories.argtypes = [P, C.c_int64]
         L.abx_batch_set_params.argtypes = [P, C.c_int64, C.c_int64]
+        L.abx_batch_set_block_size.argtypes = [P, C.c_int]
+        L.abx_batch_set_graphs.argtypes = [P, C.c_int]
+        L.abx_batch_graph_stats.argtypes = [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.abx_batch_need.argtypes = [P, P, C.c_int, C.POINTER(C.c_size_t)]
         L.abx_problem_stats.argtypes = [P, C.POINTER(abi.RunStats)]
         L.abx_fission_fetch.argtypes = [P, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]
@@ -349,12 +352,24 @@ class Batch:
         if step_histories:
             self.set_step_histories(step_histories)
 
-    def set_params(self, chunk=0, multi_hist=0):
+    def set_params(self, chunk=0, multi_hist=0, block=256):
         """Batch-kernel chunk length (histories; 0 = about 8 chunks per CU,
-        -f = about f per CU) and the cap on a problem's side-by-side batches
-        (0 = default)."""
+        -f = about f per CU), the cap on a problem's side-by-side batches
+        (0 = default) and the kernel's lanes per block (64, 128, 256)."""
         _check(lib().abx_batch_set_params(self.h, C.c_int64(int(chunk)),
                                           C.c_int64(int(multi_hist))))
+        _check(lib().abx_batch_set_block_size(self.h, int(block)))
+
+    def set_graphs(self, on):
+        """Replay the transport of steps that repeat the previous one as a
+        captured graph (default on)."""
+        _check(lib().abx_batch_set_graphs(self.h, 1 if on else 0))
+
+    def graph_stats(self):
+        """(graphs captured, graph replays) so far."""
+        c, r = C.c_int64(), C.c_int64()
+        _check(lib().abx_batch_graph_stats(self.h, C.byref(c), C.byref(r)))
+        return c.value, r.value
 
     def set_step_histories(self, n):
         """Cap the histories of one step over all problems (0 = no cap); a

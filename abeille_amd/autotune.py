@@ -76,8 +76,8 @@ def autotune(problem, model, sample=200_000, repeats=2, cache=DEFAULT_CACHE,
 
 # chunk lengths as chunks per CU (negative), so that a choice made on a
 # sample carries over to the full set of problems
-BATCH_CANDIDATES = [dict(chunk=c, multi_hist=m) for c in (0, -2, -4, -16)
-                    for m in (1 << 20, 1 << 23)]
+BATCH_CANDIDATES = [dict(chunk=c, multi_hist=m, block=bs) for c in (0, -4, -16)
+                    for m in (1 << 20, 1 << 23) for bs in (128, 256)]
 
 
 def autotune_batch(batch, sample, nbatches=1, candidates=BATCH_CANDIDATES, repeats=1):

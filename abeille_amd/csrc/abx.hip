@@ -2019,14 +2019,12 @@ int64_t chunk_length(int64_t total, int per_cu = 8) {
   return std::max<int64_t>(256, std::min<int64_t>(8192, total / std::max(1, per_cu * cus)));
 }
 
+// The chunk counter `next` was zeroed by the caller.
 template <int TRK, bool SEC, bool SPLIT, bool STAGE, bool EXT>
 hipError_t launch_chunks(const DevProblem* probs, const KCtx* ctxs, const BatchChunk* chunks,
-                         int n_chunks, int D, int L, size_t blob_max, unsigned* next,
+                         int n_chunks, int D, int L, size_t blob_max, unsigned* next, int bs,
                          hipStream_t s) {
-  const int bs = 256;
   const size_t lds = cursor_lds_bytes(bs, D, L) + (STAGE ? blob_max : 0);
-  hipError_t e;
-  if ((e = hipMemsetAsync(next, 0, sizeof(unsigned), s)) != hipSuccess) return e;
   const int64_t blocks = std::min<int64_t>(
       n_chunks, resident_blocks(k_batch_chunks<TRK, SEC, SPLIT, STAGE, EXT>, bs, lds));
   hipLaunchKernelGGL((k_batch_chunks<TRK, SEC, SPLIT, STAGE, EXT>), dim3((unsigned)blocks),
@@ -2034,25 +2032,26 @@ hipError_t launch_chunks(const DevProblem* probs, const KCtx* ctxs, const BatchC
   return hipGetLastError();
 }
 
-// One launch over the chunks of problems of one kernel variant; the tables
-// are staged in LDS when the largest blob fits next to the cursors.
+// One launch over the chunks of problems of one kernel variant, blocks of
+// bs lanes; the tables are staged in LDS when the largest blob fits next to
+// the cursors.
 hipError_t launch_chunks_variant(int key, const DevProblem* probs, const KCtx* ctxs,
                                  const BatchChunk* chunks, int n_chunks, int D, int L,
-                                 size_t blob_max, unsigned* next, hipStream_t s) {
+                                 size_t blob_max, unsigned* next, int bs, hipStream_t s) {
   const int trk = key / 16;
   const bool sec = (key / 8) & 1, split = (key / 4) & 1, ext = (key / 2) & 1, fits = key & 1;
-  const size_t cur = cursor_lds_bytes(256, D, L);
+  const size_t cur = cursor_lds_bytes(bs, D, L);
   return dispatch_variant(trk, false, sec || trk == 2, split, ext,
                           [&](auto T, auto, auto SE, auto SP, auto EX) {
     constexpr int TK = decltype(T)::value;
     constexpr bool SEC = decltype(SE)::value, SPL = decltype(SP)::value;
     constexpr bool EXT = decltype(EX)::value;
     if (fits && stage_pays(k_batch_chunks<TK, SEC, SPL, true, EXT>,
-                           k_batch_chunks<TK, SEC, SPL, false, EXT>, 256, cur, blob_max))
+                           k_batch_chunks<TK, SEC, SPL, false, EXT>, bs, cur, blob_max))
       return launch_chunks<TK, SEC, SPL, true, EXT>(probs, ctxs, chunks, n_chunks, D, L, blob_max,
-                                                    next, s);
+                                                    next, bs, s);
     return launch_chunks<TK, SEC, SPL, false, EXT>(probs, ctxs, chunks, n_chunks, D, L, blob_max,
-                                                   next, s);
+                                                   next, bs, s);
   });
 }
 
@@ -2099,7 +2098,16 @@ struct abx_batch {
   static constexpr int NS = 4;
   hipStream_t aux[NS] = {};
   hipEvent_t ready = nullptr, done[NS] = {};
+  int block = 256;  // lanes per block of the batch kernel (64, 128 or 256)
+  // A step whose launches repeat the previous step's (same problems, sizes,
+  // buffers and chunks; only the source histories' ids move on) replays the
+  // transport captured as a graph instead of launching it again.
+  bool graphs = true;
+  std::vector<unsigned char> last_sig;  // launch signature of the previous step
+  hipGraphExec_t gexec = nullptr;       // its captured transport, or none
+  int64_t captures = 0, replays = 0;
   ~abx_batch() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
     DBuf* all[] = {&probs, &ctxs, &offsets, &scal, &next, &sec, &chunks, &sjob_d, &ssrc_d, &rjob_d};
     for (DBuf* b : all) b->release();
     if (e0) (void)hipEventDestroy(e0);
@@ -2310,23 +2318,76 @@ void batch_step(abx_batch* b, std::vector<abx_problem*>& finished) {
   }
   // one chunk counter per group (groups run concurrently)
   b->next.reserve(std::max<size_t>(8, groups.size()) * sizeof(unsigned long long));
-  HIPC(hipEventRecord(b->e0, b->stream));
-  HIPC(hipEventRecord(b->ready, b->stream));
+  // the transport: counters zeroed, then one launch per group on the side
+  // streams, joined back into the batch stream
   const int used = (int)std::min<size_t>(NS, groups.size());
-  for (int k = 0; k < used; k++) HIPC(hipStreamWaitEvent(b->aux[k], b->ready, 0));
-  for (size_t gi = 0; gi < groups.size(); gi++) {
-    const Group& gr = groups[gi];
-    if (gr.nc > 0)
-      HIPC(launch_chunks_variant(gr.key, b->probs.as<DevProblem>() + gr.g0,
-                                 b->ctxs.as<KCtx>() + gr.g0, b->chunks.as<BatchChunk>() + gr.c0,
-                                 (int)gr.nc, gr.D, gr.L, gr.blob_max,
-                                 reinterpret_cast<unsigned*>(b->next.as<unsigned long long>() + gi),
-                                 b->aux[gi % NS]));
+  auto enqueue = [&]() -> hipError_t {
+    hipError_t e;
+    const int ng = (int)std::max<size_t>(8, groups.size());
+    hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)blocks_for(ng, 256)), dim3(256), 0, b->stream,
+                       b->next.as<unsigned long long>(), ng);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = hipEventRecord(b->ready, b->stream)) != hipSuccess) return e;
+    for (int k = 0; k < used; k++)
+      if ((e = hipStreamWaitEvent(b->aux[k], b->ready, 0)) != hipSuccess) return e;
+    for (size_t gi = 0; gi < groups.size(); gi++) {
+      const Group& gr = groups[gi];
+      if (gr.nc > 0 &&
+          (e = launch_chunks_variant(gr.key, b->probs.as<DevProblem>() + gr.g0,
+                                     b->ctxs.as<KCtx>() + gr.g0, b->chunks.as<BatchChunk>() + gr.c0,
+                                     (int)gr.nc, gr.D, gr.L, gr.blob_max,
+                                     reinterpret_cast<unsigned*>(b->next.as<unsigned long long>() + gi),
+                                     b->block, b->aux[gi % NS])) != hipSuccess)
+        return e;
+    }
+    for (int k = 0; k < used; k++) {
+      if ((e = hipEventRecord(b->done[k], b->aux[k])) != hipSuccess) return e;
+      if ((e = hipStreamWaitEvent(b->stream, b->done[k], 0)) != hipSuccess) return e;
+    }
+    return hipSuccess;
+  };
+  // the step's launch signature: everything the enqueued launches read
+  // from the host (the sources' history ids are sampled outside)
+  std::vector<unsigned char> sig;
+  auto put = [&sig](const void* d, size_t n) {
+    const unsigned char* c = (const unsigned char*)d;
+    sig.insert(sig.end(), c, c + n);
+  };
+  put(&ni, sizeof ni);
+  put(&b->block, sizeof b->block);
+  for (const Group& gr : groups) put(&gr, sizeof gr);
+  put(b->hp.data(), (size_t)ni * sizeof(DevProblem));
+  put(b->hk.data(), (size_t)ni * sizeof(KCtx));
+  put(b->all_chunks.data(), b->all_chunks.size() * sizeof(BatchChunk));
+  const void* bufs[] = {b->probs.p, b->ctxs.p, b->chunks.p, b->next.p, b->sec.p};
+  put(bufs, sizeof bufs);
+  const bool repeat = b->graphs && sig == b->last_sig;
+  if (!repeat && b->gexec) {
+    HIPC(hipGraphExecDestroy(b->gexec));
+    b->gexec = nullptr;
   }
-  for (int k = 0; k < used; k++) {
-    HIPC(hipEventRecord(b->done[k], b->aux[k]));
-    HIPC(hipStreamWaitEvent(b->stream, b->done[k], 0));
+  HIPC(hipEventRecord(b->e0, b->stream));
+  if (repeat && !b->gexec) {
+    hipGraph_t g = nullptr;
+    HIPC(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const hipError_t ce = enqueue();
+    const hipError_t ee = hipStreamEndCapture(b->stream, &g);
+    if (ce != hipSuccess || ee != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      HIPC(ce != hipSuccess ? ce : ee);
+    }
+    const hipError_t ie = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPC(ie);
+    b->captures++;
   }
+  if (repeat) {
+    HIPC(hipGraphLaunch(b->gexec, b->stream));
+    b->replays++;
+  } else {
+    HIPC(enqueue());
+  }
+  b->last_sig.swap(sig);
   HIPC(hipEventRecord(b->e1, b->stream));
   b->h_scal.resize(ni);
   HIPC(hipMemcpyAsync(b->h_scal.data(), b->scal.p, (size_t)ni * sizeof(BatchScalars),
@@ -2501,6 +2562,30 @@ int abx_batch_set_params(abx_batch* b, int64_t chunk, int64_t multi_hist) {
     if (!b->items.empty()) fail(ABX_E_INVALID, "set batch parameters before attaching problems");
     b->chunk = chunk;
     b->multi_hist = multi_hist > 0 ? multi_hist : (int64_t(1) << 23);
+  });
+}
+
+int abx_batch_set_block_size(abx_batch* b, int block_size) {
+  ABX_TRY({
+    if (!b) fail(ABX_E_INVALID, "null argument");
+    if (block_size != 64 && block_size != 128 && block_size != 256)
+      fail(ABX_E_INVALID, "block_size must be 64, 128 or 256");
+    b->block = block_size;
+  });
+}
+
+int abx_batch_set_graphs(abx_batch* b, int on) {
+  ABX_TRY({
+    if (!b) fail(ABX_E_INVALID, "null argument");
+    b->graphs = on != 0;
+  });
+}
+
+int abx_batch_graph_stats(abx_batch* b, int64_t* captures, int64_t* replays) {
+  ABX_TRY({
+    if (!b || !captures || !replays) fail(ABX_E_INVALID, "null argument");
+    *captures = b->captures;
+    *replays = b->replays;
   });
 }
 

@@ -788,6 +788,13 @@ __global__ void k_record_generation(double* gen, double* avg, double* var, int64
   if (clear) gen[i] = 0.0;
 }
 
+// Zeroes n counters: a kernel, not hipMemsetAsync, so that it can sit in a
+// captured graph.
+__global__ void k_zero_u64(unsigned long long* a, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0ull;
+}
+
 __global__ void k_fill(double* a, int64_t n, double v) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = v;

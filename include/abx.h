@@ -667,6 +667,17 @@ int abx_batch_set_step_histories(abx_batch* b, int64_t histories);  /* before an
  * counterpart: its batches run one after another (FixedSource::run,
  * src/fixed_source.cpp:81-173). */
 int abx_batch_set_params(abx_batch* b, int64_t chunk, int64_t multi_hist);  /* before any add */
+/* Lanes per block of the batch kernel: 64, 128 or 256 (default), for the
+ * autotuner.  No counterpart in the reference. */
+int abx_batch_set_block_size(abx_batch* b, int block_size);
+/* A step whose launches repeat the previous step's - the same problems,
+ * sizes, device buffers and chunks, as the steps of a fixed-source problem
+ * with many batches do - replays the previous step's transport as a
+ * captured hipGraph (on = 1, the default) instead of launching it again.
+ * Results do not depend on it.  No counterpart in the reference. */
+int abx_batch_set_graphs(abx_batch* b, int on);
+/* Transport graphs captured and replayed by the batch so far. */
+int abx_batch_graph_stats(abx_batch* b, int64_t* captures, int64_t* replays);
 /* GPU time of the batch's transport launches so far. */
 int abx_batch_kernel_seconds(abx_batch* b, double* seconds);
 /* Statistics of a problem's batch-engine run (or of its last abx_run). */

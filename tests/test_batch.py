@@ -134,3 +134,42 @@ def test_batch_parameters_only_before_attaching():
         b.set_step_histories(4096)
     while len(b):
         b.step()
+
+
+@pytest.mark.parametrize("block", [64, 128])
+@pytest.mark.parametrize("graphs", [True, False], ids=["graphs", "eager"])
+def test_batch_block_size_and_graph_replay(block, graphs):
+    """Steps that repeat the previous step's launches replay its captured
+    transport graph; the results are those of the problems alone, with any
+    block size of the batch kernel."""
+    print(f"[batch] block={block} graphs={graphs}", flush=True)
+    cases = [("cube_dt", problems.fixed_source_cube(2500, "delta"), 6),
+             ("cube_st", problems.fixed_source_cube(1500, "surface"), 5)]
+    b = Batch()
+    b.set_params(chunk=512, multi_hist=1, block=block)  # one batch per problem and step
+    b.set_graphs(graphs)
+    probs = {}
+    for name, m, nb in cases:
+        probs[name] = Problem(m)
+        assert b.add(probs[name], nb)
+    steps = 0
+    while len(b):
+        b.step()
+        steps += 1
+        assert steps < 100
+    captures, replays = b.graph_stats()
+    if graphs:
+        assert captures >= 1 and replays >= 3, (captures, replays)
+    else:
+        assert captures == replays == 0
+    for name, m, nb in cases:
+        sa, ta = _alone(m, nb)
+        sb = probs[name].stats()
+        assert sb.active_generations == nb
+        assert sb.diag.collisions == sa.diag.collisions, name
+        assert sb.histories == sa.histories, name
+        assert sb.leak_avg == pytest.approx(sa.leak_avg, rel=1e-9, abs=1e-14), name
+        for t, (aa, ea) in enumerate(ta):
+            ab, eb = probs[name].tally_stats(t)
+            np.testing.assert_allclose(ab, aa, rtol=1e-8, atol=1e-14, err_msg=name)
+            np.testing.assert_allclose(eb, ea, rtol=1e-6, atol=1e-12, err_msg=name)
