@@ -65,6 +65,8 @@ def lib():
         L.abx_exact_cancel_bank.argtypes = [P, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
                                             C.c_size_t, C.POINTER(C.c_size_t)]
         L.abx_rng_reset.argtypes = [P]
+        L.abx_rng_get_state.argtypes = [P, C.POINTER(C.c_uint64)]
+        L.abx_rng_set_state.argtypes = [P, C.c_uint64]
         L.abx_bank_size.argtypes = [P, C.POINTER(C.c_size_t)]
         L.abx_bank_fetch.argtypes = [P, C.c_void_p]
         L.abx_bank_load.argtypes = [P, C.c_void_p, C.c_size_t]
@@ -295,6 +297,15 @@ class Problem:
     def rng_reset(self):
         """abx_rng_reset: settings::rng back to the seed."""
         _check(lib().abx_rng_reset(self.h))
+
+    def rng_state(self):
+        """settings::rng's pcg32 state (abx_rng_get_state)."""
+        x = C.c_uint64()
+        _check(lib().abx_rng_get_state(self.h, C.byref(x)))
+        return x.value
+
+    def set_rng_state(self, state):
+        _check(lib().abx_rng_set_state(self.h, C.c_uint64(int(state))))
 
     def bank(self):
         n = C.c_size_t()

@@ -1831,6 +1831,20 @@ int abx_rng_reset(abx_problem* p) {
   });
 }
 
+int abx_rng_get_state(abx_problem* p, uint64_t* state) {
+  ABX_TRY({
+    if (!p || !state) fail(ABX_E_INVALID, "null argument");
+    *state = p->grng;
+  });
+}
+
+int abx_rng_set_state(abx_problem* p, uint64_t state) {
+  ABX_TRY({
+    if (!p) fail(ABX_E_INVALID, "null argument");
+    p->grng = state;
+  });
+}
+
 int abx_bank_size(abx_problem* p, size_t* n) {
   ABX_TRY({
     if (!p || !n) fail(ABX_E_INVALID, "null argument");

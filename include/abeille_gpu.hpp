@@ -5,9 +5,11 @@
 //   Transporter          include/simulation/transporter.hpp:39-87
 //   Tallies              include/simulation/tallies.hpp:38-186
 //   Particle / BankedParticle  include/simulation/particle.hpp:38-242
+//   Cancelator           include/simulation/cancelator.hpp:36-45
 // so a driver written against them (FixedSource::run, PowerIterator::run)
 // runs unchanged with GpuTransporter in place of DeltaTracker /
-// SurfaceTracker / CarterTracker.  Fatal conditions throw abx::Error (the
+// SurfaceTracker / CarterTracker, and GpuExactCancelator in place of
+// ExactMGCancelator / BasicExactMGCancelator.  Fatal conditions throw abx::Error (the
 // reference calls fatal_error(), which exits).
 #ifndef ABEILLE_GPU_HPP
 #define ABEILLE_GPU_HPP
@@ -35,11 +37,29 @@ struct Vec3 {
   double x = 0, y = 0, z = 0;
 };
 
-// BankedParticle (particle.hpp:38-66) without the exact-cancellation fields.
+// BankedParticle (particle.hpp:38-66).  The exact-cancellation fields are
+// filled when the problem has an exact cancelator.
 struct BankedParticle {
   Vec3 r, u;
   double E = 0, wgt = 0, wgt2 = 0;
   uint64_t parent_history_id = 0, parent_daughter_id = 0, family_id = 0;
+  bool parents_previous_was_virtual = false;
+  Vec3 parents_previous_position;
+  Vec3 parents_previous_direction{1., 0., 0.};
+  double parents_previous_previous_energy = 0;
+  double parents_previous_energy = 0;
+  double Esmp_parent = 0.;
+};
+
+// pcg32 (pcg-cpp, default stream) as far as the drivers use it: seeded like
+// settings::rng, its state handed to the device where the cancellation draws.
+struct pcg32 {
+  uint64_t state = 0;
+  explicit pcg32(uint64_t seed = 0) { this->seed(seed); }
+  void seed(uint64_t s) {
+    const uint64_t M = 6364136223846793005ULL, I = 1442695040888963407ULL;
+    state = (s + I) * M + I;
+  }
 };
 
 // Particle as handed to transport(): state plus its pcg32 stream.
@@ -86,14 +106,19 @@ class Particle {
 // parse_input_file() builds in the reference.
 class Problem {
  public:
-  explicit Problem(const abx_problem_desc& d) { check(abx_create(&d, &p_)); }
+  explicit Problem(const abx_problem_desc& d) : exact_(d.exact_cancel != nullptr) {
+    check(abx_create(&d, &p_));
+  }
   ~Problem() { abx_destroy(p_); }
   Problem(const Problem&) = delete;
   Problem& operator=(const Problem&) = delete;
   abx_problem* get() const { return p_; }
+  // the description names an exact cancelator: sites carry parent records
+  bool exact_cancellation() const { return exact_; }
 
  private:
   abx_problem* p_ = nullptr;
+  bool exact_ = false;
 };
 
 // settings::converged (src/settings.cpp): set by the driver once the
@@ -278,9 +303,125 @@ class GpuTransporter : public Transporter {
       res[i].parent_daughter_id = b.parent_daughter_id;
       res[i].family_id = b.family_id;
     }
+    if (tallies->problem().exact_cancellation() && n_out) {
+      // BankedParticle's cancellation fields (src/transporter.cpp:440-456)
+      std::vector<abx_parent_info> par(n_out);
+      size_t np = 0;
+      check(abx_parent_fetch(prob, par.data(), par.size(), &np));
+      for (size_t i = 0; i < n_out; i++) {
+        const abx_parent_info& q = par[i];
+        res[i].parents_previous_position = {q.r1[0], q.r1[1], q.r1[2]};
+        res[i].parents_previous_direction = {q.u1[0], q.u1[1], q.u1[2]};
+        res[i].parents_previous_previous_energy = q.E1;
+        res[i].parents_previous_energy = q.E3;
+        res[i].Esmp_parent = q.Esmp;
+        res[i].parents_previous_was_virtual = q.prev_virtual != 0;
+      }
+    }
     bank.clear();
     return res;
   }
+};
+
+// Cancelator (cancelator.hpp:36-45).
+class Cancelator {
+ public:
+  virtual ~Cancelator() = default;
+  virtual bool add_particle(BankedParticle& p) = 0;
+  virtual void perform_cancellation(pcg32& rng) = 0;
+  virtual std::vector<BankedParticle> get_new_particles(pcg32& rng) = 0;
+  virtual void clear() = 0;
+};
+
+// The exact or basic-exact cancelator the problem's description names, on
+// the device.  add_particle keeps a pointer to the particle, as the
+// reference's bins do; perform_cancellation runs the whole pass (regions,
+// weights, uniform particles) from rng's state and writes the new weights
+// through those pointers; get_new_particles hands over the uniform
+// particles.  rng ends where the reference's settings::rng ends after both
+// calls.  A particle outside every region is still kept and returns true
+// (the device decides region membership; the reference's add_particle
+// returns false for one outside the mesh, which only the message of
+// perform_regional_cancellation counts).
+class GpuExactCancelator : public Cancelator {
+ public:
+  explicit GpuExactCancelator(std::shared_ptr<Problem> prob) : prob_(std::move(prob)) {
+    if (!prob_->exact_cancellation())
+      throw Error(ABX_E_INVALID, "the problem's description names no exact cancelator");
+  }
+  bool add_particle(BankedParticle& p) override {
+    parts_.push_back(&p);
+    return true;
+  }
+  void perform_cancellation(pcg32& rng) override {
+    const size_t n = parts_.size();
+    std::vector<abx_banked> b(n);
+    std::vector<abx_parent_info> q(n);
+    for (size_t i = 0; i < n; i++) {
+      const BankedParticle& p = *parts_[i];
+      abx_banked& a = b[i];
+      a.r[0] = p.r.x; a.r[1] = p.r.y; a.r[2] = p.r.z;
+      a.u[0] = p.u.x; a.u[1] = p.u.y; a.u[2] = p.u.z;
+      a.E = p.E; a.wgt = p.wgt; a.wgt2 = p.wgt2;
+      a.parent_history_id = p.parent_history_id;
+      a.parent_daughter_id = p.parent_daughter_id;
+      a.family_id = p.family_id;
+      abx_parent_info& r = q[i];
+      r.r1[0] = p.parents_previous_position.x;
+      r.r1[1] = p.parents_previous_position.y;
+      r.r1[2] = p.parents_previous_position.z;
+      r.u1[0] = p.parents_previous_direction.x;
+      r.u1[1] = p.parents_previous_direction.y;
+      r.u1[2] = p.parents_previous_direction.z;
+      r.E1 = p.parents_previous_previous_energy;
+      r.E3 = p.parents_previous_energy;
+      r.Esmp = p.Esmp_parent;
+      r.prev_virtual = p.parents_previous_was_virtual ? 1 : 0;
+      r.pad_ = 0;
+    }
+    abx_problem* pr = prob_->get();
+    check(abx_rng_set_state(pr, rng.state));
+    std::vector<abx_banked> out(2 * n + 1024);
+    size_t n_out = 0;
+    int rc = abx_exact_cancel_bank(pr, b.data(), q.data(), n, out.data(), out.size(), &n_out);
+    if (rc == ABX_E_CAPACITY && n_out > out.size()) {
+      // more uniform particles than the guess: run the pass again from
+      // the same rng state with room for them
+      out.resize(n_out);
+      check(abx_rng_set_state(pr, rng.state));
+      rc = abx_exact_cancel_bank(pr, b.data(), q.data(), n, out.data(), out.size(), &n_out);
+    }
+    check(rc);
+    check(abx_rng_get_state(pr, &rng.state));
+    for (size_t i = 0; i < n; i++) {
+      parts_[i]->wgt = out[i].wgt;
+      parts_[i]->wgt2 = out[i].wgt2;
+    }
+    uniform_.clear();
+    for (size_t i = n; i < n_out; i++) {
+      BankedParticle u;
+      u.r = {out[i].r[0], out[i].r[1], out[i].r[2]};
+      u.u = {out[i].u[0], out[i].u[1], out[i].u[2]};
+      u.E = out[i].E;
+      u.wgt = out[i].wgt;
+      u.wgt2 = out[i].wgt2;
+      uniform_.push_back(u);
+    }
+  }
+  std::vector<BankedParticle> get_new_particles(pcg32&) override {
+    std::vector<BankedParticle> u;
+    u.swap(uniform_);
+    return u;
+  }
+  void clear() override {
+    parts_.clear();
+    uniform_.clear();
+  }
+
+ private:
+  std::shared_ptr<Problem> prob_;
+  std::vector<BankedParticle*> parts_;
+  std::vector<BankedParticle> uniform_;
 };
 
 }  // namespace abx

@@ -478,6 +478,12 @@ int abx_exact_cancel_bank(abx_problem* p, const abx_banked* bank, const abx_pare
 /* settings::rng back to settings.seed (settings::initialize_global_rng,
  * src/settings.cpp:116-118); abx_create and every abx_run do this. */
 int abx_rng_reset(abx_problem* p);
+/* The problem's settings::rng state (pcg32 on its default stream), read and
+ * written, so that a host driver's own pcg32 object can be the one the
+ * cancellation draws from (perform_cancellation(settings::rng) and
+ * get_new_particles(settings::rng), src/power_iterator.cpp:766-771). */
+int abx_rng_get_state(abx_problem* p, uint64_t* state);
+int abx_rng_set_state(abx_problem* p, uint64_t state);
 
 /* ---- device-resident drivers (the reference's Simulation loops) ---------- */
 

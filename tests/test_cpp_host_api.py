@@ -55,3 +55,43 @@ def test_transport_signature_is_the_references(tmp_path):
                    '  return a;\n}\n')
     subprocess.run(["g++", "-std=c++17", "-fsyntax-only", str(src), "-I",
                     os.path.join(ROOT, "include")], check=True)
+
+
+EXACT_SRC = os.path.join(ROOT, "tests", "cpp", "exact_power_iteration.cpp")
+
+
+def build_exact(tmp_path):
+    exe = str(tmp_path / "exact_power_iteration")
+    lib = os.path.join(ROOT, "abeille_amd")
+    subprocess.run(["g++", "-std=c++17", "-O2", EXACT_SRC, "-I", os.path.join(ROOT, "include"),
+                    "-L", lib, "-labx", f"-Wl,-rpath,{lib}", "-o", exe], check=True)
+    return exe
+
+
+def test_cpp_exact_driver_compiles(tmp_path):
+    assert os.path.exists(build_exact(tmp_path))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["exact", "basic"])
+def test_cpp_power_iteration_with_exact_cancellation(tmp_path, kind):
+    """PowerIterator::run with perform_regional_cancellation written against
+    the mirror (GpuTransporter, GpuExactCancelator, the driver's own
+    settings::rng) gives the k per generation of abx_run's device loop."""
+    print(f"[cpp] exact driver kind={kind}", flush=True)
+    exe = build_exact(tmp_path)
+    out = subprocess.run([exe, kind, "4000", "6", "2"], capture_output=True, text=True,
+                         timeout=300)
+    assert out.returncode == 0, out.stderr
+    kgen, krun, sizes = [], [], []
+    for ln in out.stdout.splitlines():
+        f = ln.split()
+        if f[0] == "kgen":
+            kgen.append(float(f[2]))
+            sizes.append(int(f[3]))
+        elif f[0] == "krun":
+            krun.append(float(f[2]))
+    assert len(kgen) == len(krun) == 6
+    assert max(sizes) > 0
+    for a, b in zip(kgen, krun):
+        assert a == pytest.approx(b, rel=1e-9)

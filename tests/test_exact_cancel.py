@@ -305,3 +305,20 @@ def test_gpu_exact_cancellation_refusals():
         g.run(2, 1)
     with pytest.raises(Exception, match="replicas"):
         Problem(m).run_replicas(2, 2, 1)
+
+
+@pytest.mark.gpu
+def test_gpu_rng_state_round_trip():
+    """abx_rng_get_state / abx_rng_set_state: a pass repeated from a saved
+    settings::rng state repeats its uniform particles and its end state."""
+    m = exact_model("basic_avg_g_rng")
+    o = Oracle(m)
+    g = Problem(m)
+    fis, par = generation(o, 2000)
+    s0 = g.rng_state()
+    a = g.exact_cancel(fis, par)
+    s1 = g.rng_state()
+    assert s1 != s0
+    g.set_rng_state(s0)
+    b = g.exact_cancel(fis, par)
+    assert a.tobytes() == b.tobytes() and g.rng_state() == s1
