@@ -962,11 +962,23 @@ static int64_t exact_cancel_device(abx_problem* p, int64_t n) {
   if (nb == 0) return n;
   p->ex_bins.reserve((size_t)nb * sizeof(ExBinD));
   ExBinD* bins = p->ex_bins.as<ExBinD>();
+  const uint32_t* head = p->ex_head.as<uint32_t>();
+  const uint64_t* offs = p->ex_offs.as<uint64_t>();
   hipLaunchKernelGGL(k_ex_bins, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0, s, keys, n,
-                     (const uint32_t*)p->ex_head.p, (const uint64_t*)p->ex_offs.p, bins);
+                     head, offs, bins);
   HIPC(hipGetLastError());
-  hipLaunchKernelGGL(k_ex_bin_init, dim3((unsigned)blocks_for(nb * 64, 256)), dim3(256), 0, s, E,
-                     (const abx_banked*)bank, keys, bins, nb);
+  // the per-site arrays, in sorted order
+  p->ex_fv.reserve((size_t)n * 11 * sizeof(double));
+  ExSiteArrays A;
+  double* a0 = p->ex_fv.as<double>();
+  double** fields[] = {&A.w, &A.w2, &A.fv, &A.fi, &A.cc, &A.cw, &A.cw2, &A.kp1, &A.up1, &A.kp2, &A.up2};
+  for (int k = 0; k < 11; k++) *fields[k] = a0 + (size_t)k * (size_t)n;
+  // site kernels: 256 lanes a block; region kernels: one 64-lane block a region
+  const dim3 gs((unsigned)blocks_for(n, 256)), b256(256), gr((unsigned)nb), b64(64);
+  hipLaunchKernelGGL(k_ex_site, gs, b256, 0, s, (const abx_banked*)bank, keys, n, head, offs, bins,
+                     A);
+  HIPC(hipGetLastError());
+  hipLaunchKernelGGL(k_ex_region, gr, b64, 0, s, E, bins, nb, A);
   HIPC(hipGetLastError());
   // basic: the per-region streams skip n_samples * 100 (x2 for average-g)
   // draws per site of every earlier region (:475-489)
@@ -976,16 +988,17 @@ static int64_t exact_cancel_device(abx_problem* p, int64_t n) {
   hipLaunchKernelGGL(k_ex_samples, dim3((unsigned)blocks_for(nb, 256)), dim3(256), lds, s, p->P, E,
                      bins, nb, p->grng, max_rn, p->ex_smp.as<ExSample>());
   HIPC(hipGetLastError());
-  p->ex_fv.reserve((size_t)n * sizeof(double));
-  p->ex_fi.reserve((size_t)n * sizeof(double));
-  hipLaunchKernelGGL(k_ex_f, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0, s, p->P, E,
-                     (const abx_banked*)bank, par, keys, n, (const uint32_t*)p->ex_head.p,
-                     (const uint64_t*)p->ex_offs.p, bins, (const ExSample*)p->ex_smp.p,
-                     p->ex_fv.as<double>(), p->ex_fi.as<double>());
+  hipLaunchKernelGGL(k_ex_f, gs, b256, 0, s, p->P, E, par, keys, n, head, offs, bins,
+                     (const ExSample*)p->ex_smp.p, A);
   HIPC(hipGetLastError());
-  hipLaunchKernelGGL(k_ex_cancel, dim3((unsigned)blocks_for(nb * 64, 256)), dim3(256), 0, s, p->P,
-                     E, bank, par, keys, bins, nb, (const double*)p->ex_fv.p,
-                     (const double*)p->ex_fi.p);
+  hipLaunchKernelGGL(k_ex_sums, gr, b64, 0, s, E, bins, nb, A);
+  HIPC(hipGetLastError());
+  hipLaunchKernelGGL(k_ex_p, gs, b256, 0, s, p->P, E, (const abx_banked*)bank, par, keys, n, head,
+                     offs, bins, A);
+  HIPC(hipGetLastError());
+  hipLaunchKernelGGL(k_ex_uniform, gr, b64, 0, s, bins, nb, A);
+  HIPC(hipGetLastError());
+  hipLaunchKernelGGL(k_ex_apply, gs, b256, 0, s, bank, keys, n, head, offs, (const ExBinD*)bins, A);
   HIPC(hipGetLastError());
   if (basic && E.beta != ABX_BETA_MINIMUM) {
     ExBinD last{};
@@ -1666,7 +1679,7 @@ static void run_generations(abx_problem* p, int ngenerations, int nignored, doub
       if (p->exact) r.n_sites = exact_cancel_device(p, r.n_sites);
       p->wsum.reserve(2 * sizeof(double));
       HIPC(hipMemsetAsync(p->wsum.p, 0, 2 * sizeof(double), p->stream));
-      hipLaunchKernelGGL(k_weight_sums, dim3((unsigned)blocks_for(r.n_sites, 256)), dim3(256), 0,
+      hipLaunchKernelGGL(k_weight_sums, dim3((unsigned)std::min<int64_t>(1024, blocks_for(r.n_sites, 256))), dim3(256), 0,
                          p->stream, p->ordered.as<abx_banked>(), r.n_sites, p->wsum.as<double>());
       HIPC(hipGetLastError());
       double ws[2];
@@ -2540,6 +2553,9 @@ int abx_batch_add(abx_batch* b, abx_problem* p, int nbatches) {
     if (b->budget && b->used + need > b->budget)
       fail(ABX_E_BUDGET, "problem does not fit the batch's device budget");
     run_reset(p);
+    // run_reset clears the tally arrays on the problem's own stream; the
+    // batch launches on its streams, which are not ordered after it
+    HIPC(hipStreamSynchronize(p->stream));
     p->run_diag = abx_diag{};
     p->run_histories = 0;
     b->used += need;

@@ -2,22 +2,28 @@
 // (src/exact_mg_cancelator.cpp, src/basic_exact_mg_cancelator.cpp) on a
 // device-resident fission bank and its parent records.
 //
-// Pipeline (the host drives it, abx.hip):
-//   k_ex_keys      one lane per site: region number (exact.h) << 32 | site
+// Pipeline (the host drives it, abx.hip); "site" kernels run one lane per
+// site in sorted order, "region" kernels one wave per region:
+//   k_ex_keys      site: region number (exact.h) << 32 | bank index
 //   radix sort     regions in order, each region's sites in bank order
 //   k_ex_heads     region starts; an exclusive scan numbers the regions
 //   k_ex_bins      one record per region: its range of sorted positions
-//   k_ex_bin_init  one wave per region: signs present, W and W2 in bank order
+//   k_ex_site      site: its weights in sorted order, the signs present
+//   k_ex_region    region: mixed signs, W and W2 (basic average-f)
 //   k_ex_samples   one lane per region: the n_samples points (Sobol or pcg32)
-//   k_ex_f         one lane per site: <f> and <1/f> over the points
-//   k_ex_cancel    one wave per region: the C sums, then cancel_bin for wgt
-//                  and wgt2 with their uniform weights, in bank order
+//   k_ex_f         site: <f>, <1/f> and the C terms
+//   k_ex_sums      region: sum_c, sum_c_wgt, sum_c_wgt2
+//   k_ex_p         site: beta, its own f, the kept and uniform shares of
+//                  wgt and wgt2, and where a region's pass stops
+//   k_ex_uniform   region: the uniform weights up to the stops
+//   k_ex_apply     site: the new weights
 //   k_ex_new       one block: the uniform particles, in region order, from
 //                  the one global pcg32 stream
-// Every floating-point sum the reference forms over a region's particles is
-// formed here in the same order (a wave reads 64 values, then adds them one
-// by one), so the weights and new particles are bit-identical to a serial
-// pass over the regions in order.
+// The reference forms its per-region sums in bank order, one addition after
+// another; the region kernels do the same additions in the same order (a
+// wave loads 512 consecutive values, then adds them one at a time), so the
+// weights and new particles are bit-identical to a serial pass.  Everything
+// else is independent per site and runs in parallel.
 #pragma once
 
 #include "exact.h"
@@ -26,14 +32,17 @@
 struct ExBinD {
   int64_t start, end;  // sorted positions [start, end)
   uint64_t reg;        // region number
-  int32_t m1, m2;      // wgt / wgt2 of both signs present
+  uint32_t sign;       // bit 0/1: some wgt > 0 / < 0, bit 2/3: the same for wgt2
+  int32_t m1, m2;      // wgt / wgt2 of both signs present (cancel_bin runs)
   int32_t avg;         // <f> and <1/f> are computed for it
   int32_t can_cancel;
+  int32_t pad_;
+  int64_t stop1, stop2;  // first sorted position whose P_p / P_u ends the pass
   double W, W2;
   double sum_c, sum_c_wgt, sum_c_wgt2;
   double uw, uw2;      // uniform weights
   uint32_t n_new;      // uniform particles
-  int32_t pad_;
+  int32_t pad2_;
 };
 
 struct ExSample {
@@ -42,13 +51,61 @@ struct ExSample {
   int32_t pad_;
 };
 
+// Per-site arrays, in sorted order.
+struct ExSiteArrays {
+  double *w, *w2;          // weights
+  double *fv, *fi;         // <f>, <1/f>
+  double *cc, *cw, *cw2;   // C, C * wgt, C * wgt2
+  double *kp1, *up1;       // wgt * P_p, wgt * P_u
+  double *kp2, *up2;       // the same for wgt2
+};
+
 __device__ __forceinline__ uint64_t ex_reg_of(uint64_t key) { return key >> 32; }
 __device__ __forceinline__ uint32_t ex_site_of(uint64_t key) { return (uint32_t)key; }
 
-// Adds lanes 0..cnt-1 of v to acc, in lane order (all lanes get the sum).
-__device__ __forceinline__ double ex_wave_add(double acc, double v, int cnt) {
-  for (int l = 0; l < cnt; l++) acc += __shfl(v, l, 64);
-  return acc;
+constexpr int kExPer = 8;                 // values per lane and block
+constexpr int kExBlock = 64 * kExPer;     // values a wave stages at a time
+
+// Sums a[k][s0 .. s1) for k < K in index order, one addition after
+// another (every lane of the wave gets the sums).  Region kernels run one
+// wave per block.  The wave stages 512 consecutive values of each array in
+// LDS (coalesced, the next block's loads in flight meanwhile) and every lane
+// walks them from LDS with broadcast reads; independent sums interleave.
+template <int K>
+__device__ void ex_seq_sums(const double* const* a, int64_t s0, int64_t s1, double* acc) {
+  __shared__ double buf[K][kExBlock];
+  const int lane = threadIdx.x & 63;
+  double v[K][kExPer];
+  auto fetch = [&](int64_t c) {
+#pragma unroll
+    for (int k = 0; k < K; k++)
+#pragma unroll
+      for (int j = 0; j < kExPer; j++) {
+        const int64_t t = c + j * 64 + lane;
+        v[k][j] = t < s1 ? a[k][t] : 0.;
+      }
+  };
+  if (s0 < s1) fetch(s0);
+  for (int64_t c = s0; c < s1; c += kExBlock) {
+    __syncthreads();  // the previous block's walk is over
+#pragma unroll
+    for (int k = 0; k < K; k++)
+#pragma unroll
+      for (int j = 0; j < kExPer; j++) buf[k][j * 64 + lane] = v[k][j];
+    __syncthreads();
+    if (c + kExBlock < s1) fetch(c + kExBlock);
+    const int nblk = (int)(s1 - c < kExBlock ? s1 - c : kExBlock);
+    int i = 0;
+    for (; i + 8 <= nblk; i += 8) {
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+#pragma unroll
+        for (int k = 0; k < K; k++) acc[k] += buf[k][i + u];
+    }
+    for (; i < nblk; i++)
+#pragma unroll
+      for (int k = 0; k < K; k++) acc[k] += buf[k][i];
+  }
 }
 
 __global__ void k_ex_keys(DevProblem P, DevExact E, const abx_banked* __restrict__ bank,
@@ -79,6 +136,7 @@ __global__ void k_ex_heads(const uint64_t* __restrict__ keys, int64_t n,
   head[t] = (r != kExNone && (t == 0 || ex_reg_of(keys[t - 1]) != r)) ? 1u : 0u;
 }
 
+// Region records: range and number, everything else cleared.
 __global__ void k_ex_bins(const uint64_t* __restrict__ keys, int64_t n,
                           const uint32_t* __restrict__ head, const uint64_t* __restrict__ offs,
                           ExBinD* __restrict__ bins) {
@@ -88,57 +146,88 @@ __global__ void k_ex_bins(const uint64_t* __restrict__ keys, int64_t n,
   if (r == kExNone) return;
   const uint64_t b = offs[t] + head[t] - 1u;
   if (head[t]) {
-    bins[b].start = t;
-    bins[b].reg = r;
+    ExBinD& B = bins[b];
+    B.start = t;
+    B.reg = r;
+    B.sign = 0u;
+    B.m1 = B.m2 = B.avg = 0;
+    B.can_cancel = 1;
+    B.W = B.W2 = B.sum_c = B.sum_c_wgt = B.sum_c_wgt2 = B.uw = B.uw2 = 0.;
+    B.n_new = 0u;
   }
   if (t == n - 1 || ex_reg_of(keys[t + 1]) != r) bins[b].end = t + 1;
 }
 
-// add_particle's sums and mixed-sign tests of each region
-__global__ void k_ex_bin_init(DevExact E, const abx_banked* __restrict__ bank,
-                              const uint64_t* __restrict__ keys, ExBinD* __restrict__ bins,
-                              int64_t nb) {
+__device__ __forceinline__ uint64_t ex_bin_of(const uint32_t* head, const uint64_t* offs,
+                                              int64_t t) {
+  return offs[t] + head[t] - 1u;
+}
+
+// The sites' weights in sorted order and the signs of each region.
+__global__ void k_ex_site(const abx_banked* __restrict__ bank, const uint64_t* __restrict__ keys,
+                          int64_t n, const uint32_t* __restrict__ head,
+                          const uint64_t* __restrict__ offs, ExBinD* __restrict__ bins,
+                          ExSiteArrays A) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
-  const int64_t b = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  if (b >= nb) return;
-  const int64_t s0 = bins[b].start, s1 = bins[b].end;
-  bool p1 = false, n1 = false, p2 = false, n2 = false;
-  double W = 0., W2 = 0.;
-  for (int64_t c = s0; c < s1; c += 64) {
-    const int64_t t = c + lane;
-    const int cnt = (int)(s1 - c < 64 ? s1 - c : 64);
-    double w = 0., w2 = 0.;
-    if (t < s1) {
-      const abx_banked& q = bank[ex_site_of(keys[t])];
-      w = q.wgt;
-      w2 = q.wgt2;
+  uint32_t s = 0u;
+  uint64_t bin = ~0ull;  // no region
+  if (t < n) {
+    const uint64_t key = keys[t];
+    if (ex_reg_of(key) != kExNone) {
+      const abx_banked& q = bank[ex_site_of(key)];
+      const double w = q.wgt, w2 = q.wgt2;
+      A.w[t] = w;
+      A.w2[t] = w2;
       if (w > 0.)
-        p1 = true;
+        s |= 1u;
       else if (w < 0.)
-        n1 = true;
+        s |= 2u;
       if (w2 > 0.)
-        p2 = true;
+        s |= 4u;
       else if (w2 < 0.)
-        n2 = true;
+        s |= 8u;
+      bin = ex_bin_of(head, offs, t);
     }
-    W = ex_wave_add(W, w, cnt);
-    W2 = ex_wave_add(W2, w2, cnt);
   }
-  const bool m1 = __ballot(p1) != 0 && __ballot(n1) != 0;
-  const bool m2 = __ballot(p2) != 0 && __ballot(n2) != 0;
-  if (lane == 0) {
-    ExBinD& B = bins[b];
-    B.m1 = m1;
-    B.m2 = m2;
-    const bool needs = E.kind == ABX_EXACT_MG || E.beta == ABX_BETA_AVERAGE_F ||
-                       E.beta == ABX_BETA_AVERAGE_G;
-    B.avg = (s1 - s0 > 1) && (m1 || m2) && needs;
-    B.can_cancel = 1;
-    B.W = W;
-    B.W2 = W2;
-    B.sum_c = B.sum_c_wgt = B.sum_c_wgt2 = 0.;
-    B.uw = B.uw2 = 0.;
-    B.n_new = 0;
+  // sites are sorted by region: OR the wave's runs of one region together
+  // (inclusive segmented scan), one atomic per run
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t os = __shfl_up(s, (unsigned)d, 64);
+    const uint64_t ob = __shfl_up(bin, (unsigned)d, 64);
+    if (lane >= d && ob == bin) s |= os;
+  }
+  const uint64_t nb_next = __shfl_down(bin, 1u, 64);
+  const bool last = lane == 63 || nb_next != bin;
+  if (bin != ~0ull && last && s) atomicOr(&bins[bin].sign, s);
+}
+
+// add_particle's W, W2 (in bank order) and the cancel_bin / averages
+// conditions of perform_cancellation (exact :405-517, basic :471-562).
+__global__ void __launch_bounds__(64) k_ex_region(DevExact E, ExBinD* __restrict__ bins,
+                                                 int64_t nb, ExSiteArrays A) {
+  const int64_t b = blockIdx.x;  // one wave per block and region
+  if (b >= nb) return;
+  ExBinD& B = bins[b];
+  const int64_t s0 = B.start, s1 = B.end;
+  const uint32_t sg = B.sign;
+  const bool m1 = (sg & 3u) == 3u, m2 = (sg & 12u) == 12u;
+  double acc[2] = {0., 0.};
+  const bool basic = E.kind == ABX_BASIC_EXACT_MG;
+  if (basic && E.beta == ABX_BETA_AVERAGE_F) {  // the only reader of W and W2
+    const double* a[2] = {A.w, A.w2};
+    ex_seq_sums<2>(a, s0, s1, acc);
+  }
+  if (threadIdx.x == 0) {
+    const bool many = s1 - s0 > 1;
+    B.m1 = many && m1;
+    B.m2 = many && m2;
+    const bool needs = !basic || E.beta == ABX_BETA_AVERAGE_F || E.beta == ABX_BETA_AVERAGE_G;
+    B.avg = many && (m1 || m2) && needs;
+    B.W = acc[0];
+    B.W2 = acc[1];
+    B.stop1 = s1;
+    B.stop2 = s1;
   }
 }
 
@@ -201,19 +290,18 @@ __global__ void k_ex_samples(DevProblem P, DevExact E, ExBinD* __restrict__ bins
 }
 
 // <f> and <1/f> of each site of a region with averages (compute_averages
-// :329-341; get_averages :262-283).  The exact cancelator gives up on the
-// region at the first f of zero.
-__global__ void k_ex_f(DevProblem P, DevExact E, const abx_banked* __restrict__ bank,
-                       const abx_parent_info* __restrict__ par, const uint64_t* __restrict__ keys,
-                       int64_t n, const uint32_t* __restrict__ head,
-                       const uint64_t* __restrict__ offs, ExBinD* __restrict__ bins,
-                       const ExSample* __restrict__ smp, double* __restrict__ fv,
-                       double* __restrict__ fi) {
+// :329-341; get_averages :262-283) and its C terms (:343-364).  The exact
+// cancelator gives up on the region at the first f of zero.
+__global__ void k_ex_f(DevProblem P, DevExact E, const abx_parent_info* __restrict__ par,
+                       const uint64_t* __restrict__ keys, int64_t n,
+                       const uint32_t* __restrict__ head, const uint64_t* __restrict__ offs,
+                       ExBinD* __restrict__ bins, const ExSample* __restrict__ smp,
+                       ExSiteArrays A) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const uint64_t key = keys[t];
   if (ex_reg_of(key) == kExNone) return;
-  const uint64_t b = offs[t] + head[t] - 1u;
+  const uint64_t b = ex_bin_of(head, offs, t);
   ExBinD& B = bins[b];
   if (!B.avg || !__hip_atomic_load(&B.can_cancel, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
     return;
@@ -242,116 +330,150 @@ __global__ void k_ex_f(DevProblem P, DevExact E, const abx_banked* __restrict__ 
       sfi += 1. / f;
     }
   }
-  fv[t] = sf / (double)E.n_samples;
-  fi[t] = sfi / (double)E.n_samples;
+  const double fv = sf / (double)E.n_samples, fi = sfi / (double)E.n_samples;
+  A.fv[t] = fv;
+  A.fi[t] = fi;
+  const double c = ex_c(fv, fi);
+  A.cc[t] = c;
+  A.cw[t] = c * A.w[t];
+  A.cw2[t] = c * A.w2[t];
 }
 
-// The C sums (compute_averages :343-364), then cancel_bin for wgt and for
-// wgt2 (exact :367-403, basic :438-468): beta, the site's own f, the kept
-// fraction P_p and the uniform share P_u.  A non-finite P_p or P_u, or (basic)
-// a zero weight, ends the region's pass, as the reference's `return` does.
-__global__ void k_ex_cancel(DevProblem P, DevExact E, abx_banked* __restrict__ bank,
-                            const abx_parent_info* __restrict__ par,
-                            const uint64_t* __restrict__ keys, ExBinD* __restrict__ bins,
-                            int64_t nb, const double* __restrict__ fv,
-                            const double* __restrict__ fi) {
-  const int lane = threadIdx.x & 63;
-  const int64_t b = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+// sum_c, sum_c_wgt and sum_c_wgt2 of compute_averages (:343-364), in bank
+// order, for the regions whose beta uses them.
+// sum_c_wgt (sum_c_wgt2) is only read by the wgt (wgt2) pass.
+__global__ void __launch_bounds__(64) k_ex_sums(DevExact E, ExBinD* __restrict__ bins, int64_t nb,
+                                               ExSiteArrays A) {
+  const int64_t b = blockIdx.x;
   if (b >= nb) return;
-  const ExBinD B = bins[b];
-  const int64_t s0 = B.start, s1 = B.end;
-  if (s1 - s0 <= 1) return;
+  ExBinD& B = bins[b];
+  const bool uses = E.kind == ABX_EXACT_MG || E.beta == ABX_BETA_AVERAGE_G;
+  if (!(B.avg && B.can_cancel && uses)) return;
+  double acc[3] = {0., 0., 0.};
+  if (B.m1 && B.m2) {
+    const double* a[3] = {A.cc, A.cw, A.cw2};
+    ex_seq_sums<3>(a, B.start, B.end, acc);
+  } else {
+    const double* a[2] = {A.cc, B.m1 ? A.cw : A.cw2};
+    ex_seq_sums<2>(a, B.start, B.end, acc);
+    if (!B.m1) acc[2] = acc[1];
+  }
+  if (threadIdx.x == 0) {
+    B.sum_c = acc[0];
+    B.sum_c_wgt = B.m1 ? acc[1] : 0.;
+    B.sum_c_wgt2 = B.m2 ? acc[2] : 0.;
+  }
+}
+
+// cancel_bin for wgt and wgt2 (exact :367-403, basic :438-468), per site:
+// beta (get_beta), the site's own f, the kept share wgt * P_p and the
+// uniform share wgt * P_u.  A non-finite P_p or P_u, or (basic) a zero
+// weight, ends the region's pass there, as the reference's `return` does:
+// the earliest such position is the region's stop.
+__global__ void k_ex_p(DevProblem P, DevExact E, const abx_banked* __restrict__ bank,
+                       const abx_parent_info* __restrict__ par, const uint64_t* __restrict__ keys,
+                       int64_t n, const uint32_t* __restrict__ head,
+                       const uint64_t* __restrict__ offs, ExBinD* __restrict__ bins,
+                       ExSiteArrays A) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const uint64_t key = keys[t];
+  if (ex_reg_of(key) == kExNone) return;
+  ExBinD& B = bins[ex_bin_of(head, offs, t)];
+  if (!B.m1 && !B.m2) return;
   const bool basic = E.kind == ABX_BASIC_EXACT_MG;
   const bool can = B.can_cancel != 0;
-  double sum_c = 0., scw = 0., scw2 = 0.;
-  if (B.avg && can && (!basic || E.beta == ABX_BETA_AVERAGE_G)) {
-    for (int64_t c = s0; c < s1; c += 64) {
-      const int64_t t = c + lane;
-      const int cnt = (int)(s1 - c < 64 ? s1 - c : 64);
-      double cc = 0., cw = 0., cw2 = 0.;
-      if (t < s1) {
-        const abx_banked& q = bank[ex_site_of(keys[t])];
-        cc = ex_c(fv[t], fi[t]);
-        cw = cc * q.wgt;
-        cw2 = cc * q.wgt2;
-      }
-      sum_c = ex_wave_add(sum_c, cc, cnt);
-      scw = ex_wave_add(scw, cw, cnt);
-      scw2 = ex_wave_add(scw2, cw2, cnt);
-    }
+  const uint32_t site = ex_site_of(key);
+  const abx_banked s = bank[site];
+  const abx_parent_info q = par[site];
+  const V3 r1{q.r1[0], q.r1[1], q.r1[2]};
+  const V3 r4{s.r[0], s.r[1], s.r[2]};
+  double f;  // get_f at the site itself
+  if (basic) {
+    f = ex_basic_f(r4, r1, q.Esmp);
+  } else {
+    const V3 u1{q.u1[0], q.u1[1], q.u1[2]};
+    f = ex_exact_f(E, P, ex_decode(E, B.reg).mat, r1, u1, ex_group(P, q.E1), ex_group(P, q.E3),
+                   r4, ex_group(P, s.E), q.Esmp);
   }
-  const ExKeyD K = ex_decode(E, B.reg);
-  double uw[2] = {0., 0.};
+  double Bmin = 0.;
+  if (basic && can && E.beta == ABX_BETA_MINIMUM) Bmin = ex_basic_min_f(E, ex_decode(E, B.reg), r1, q.Esmp);
   for (int pass = 0; pass < 2; pass++) {
     const bool first = pass == 0;
     if (!(first ? B.m1 : B.m2)) continue;
-    double acc = 0.;
-    for (int64_t c = s0; c < s1; c += 64) {
-      const int64_t t = c + lane;
-      const int cnt = (int)(s1 - c < 64 ? s1 - c : 64);
-      bool stop = false;
-      double wgt = 0., Pp = 0., Pu = 0.;
-      uint32_t site = 0;
-      if (t < s1) {
-        site = ex_site_of(keys[t]);
-        const abx_banked s = bank[site];
-        const abx_parent_info q = par[site];
-        const V3 r1{q.r1[0], q.r1[1], q.r1[2]};
-        const V3 r4{s.r[0], s.r[1], s.r[2]};
-        wgt = first ? s.wgt : s.wgt2;
-        if (basic && wgt == 0.) {
-          stop = true;
-        } else {
-          double Bt = 0.;  // get_beta
-          if (can) {
-            if (!basic || E.beta == ABX_BETA_AVERAGE_G) {
-              const double S = (first ? scw : scw2) / (1. + sum_c);
-              const double f = fv[t], f_inv = fi[t];
-              Bt = f * (1. / (2. * f * f_inv - 1.)) * (1. - (S / wgt));
-            } else if (E.beta == ABX_BETA_MINIMUM) {
-              Bt = ex_basic_min_f(E, K, r1, q.Esmp);
-            } else if (E.beta == ABX_BETA_AVERAGE_F) {
-              const double N = (double)(s1 - s0);
-              const double W = first ? B.W : B.W2;
-              Bt = fv[t] * (1. - ((W) / ((N + 1.) * wgt)));
-            }
-          }
-          double f;
-          if (basic) {
-            f = ex_basic_f(r4, r1, q.Esmp);
-          } else {
-            const V3 u1{q.u1[0], q.u1[1], q.u1[2]};
-            f = ex_exact_f(E, P, K.mat, r1, u1, ex_group(P, q.E1), ex_group(P, q.E3), r4,
-                           ex_group(P, s.E), q.Esmp);
-          }
-          Pp = (f - Bt) / f;
-          Pu = Bt / f;
-          if (isinf(Pu) || isinf(Pp) || isnan(Pu) || isnan(Pp)) stop = true;
+    const double wgt = first ? s.wgt : s.wgt2;
+    bool stop = false;
+    double Pp = 0., Pu = 0.;
+    if (basic && wgt == 0.) {
+      stop = true;
+    } else {
+      double Bt = 0.;  // get_beta
+      if (can) {
+        if (!basic || E.beta == ABX_BETA_AVERAGE_G) {
+          const double S = (first ? B.sum_c_wgt : B.sum_c_wgt2) / (1. + B.sum_c);
+          const double fv = A.fv[t], f_inv = A.fi[t];
+          Bt = fv * (1. / (2. * fv * f_inv - 1.)) * (1. - (S / wgt));
+        } else if (E.beta == ABX_BETA_MINIMUM) {
+          Bt = Bmin;
+        } else if (E.beta == ABX_BETA_AVERAGE_F) {
+          const double N = (double)(B.end - B.start);
+          const double W = first ? B.W : B.W2;
+          Bt = A.fv[t] * (1. - ((W) / ((N + 1.) * wgt)));
         }
       }
-      const uint64_t smask = __ballot(t < s1 && stop);
-      const int upto = smask ? (int)__ffsll((long long)smask) - 1 : cnt;
-      acc = ex_wave_add(acc, wgt * Pu, upto);
-      if (lane < upto) {
-        if (first)
-          bank[site].wgt = wgt * Pp;
-        else
-          bank[site].wgt2 = wgt * Pp;
-      }
-      if (smask) break;
+      Pp = (f - Bt) / f;
+      Pu = Bt / f;
+      if (isinf(Pu) || isinf(Pp) || isnan(Pu) || isnan(Pp)) stop = true;
     }
-    uw[pass] = acc;
+    if (stop) {
+      atomicMin((unsigned long long*)(first ? &B.stop1 : &B.stop2), (unsigned long long)t);
+    } else if (first) {
+      A.kp1[t] = wgt * Pp;
+      A.up1[t] = wgt * Pu;
+    } else {
+      A.kp2[t] = wgt * Pp;
+      A.up2[t] = wgt * Pu;
+    }
   }
-  if (lane == 0) {
-    ExBinD& O = bins[b];
-    O.sum_c = sum_c;
-    O.sum_c_wgt = scw;
-    O.sum_c_wgt2 = scw2;
-    O.uw = uw[0];
-    O.uw2 = uw[1];
-    const double a = fabs(uw[0]), a2 = fabs(uw[1]);
-    O.n_new = (uint32_t)ceil(a < a2 ? a2 : a);
+}
+
+// The uniform weights (cancel_bin's uniform_wgt sums, in bank order up to
+// each pass's stop) and the number of uniform particles
+// (get_new_particles: ceil(max(|uniform_wgt|, |uniform_wgt2|))).
+__global__ void __launch_bounds__(64) k_ex_uniform(ExBinD* __restrict__ bins, int64_t nb,
+                                                  ExSiteArrays A) {
+  const int64_t b = blockIdx.x;
+  if (b >= nb) return;
+  ExBinD& B = bins[b];
+  double u1 = 0., u2 = 0.;
+  if (B.m1) {
+    const double* a[1] = {A.up1};
+    ex_seq_sums<1>(a, B.start, B.stop1, &u1);
   }
+  if (B.m2) {
+    const double* a[1] = {A.up2};
+    ex_seq_sums<1>(a, B.start, B.stop2, &u2);
+  }
+  if (threadIdx.x == 0) {
+    B.uw = u1;
+    B.uw2 = u2;
+    const double x = fabs(u1), y = fabs(u2);
+    B.n_new = (uint32_t)ceil(x < y ? y : x);
+  }
+}
+
+__global__ void k_ex_apply(abx_banked* __restrict__ bank, const uint64_t* __restrict__ keys,
+                           int64_t n, const uint32_t* __restrict__ head,
+                           const uint64_t* __restrict__ offs, const ExBinD* __restrict__ bins,
+                           ExSiteArrays A) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const uint64_t key = keys[t];
+  if (ex_reg_of(key) == kExNone) return;
+  const ExBinD& B = bins[ex_bin_of(head, offs, t)];
+  abx_banked& s = bank[ex_site_of(key)];
+  if (B.m1 && t < B.stop1) s.wgt = A.kp1[t];
+  if (B.m2 && t < B.stop2) s.wgt2 = A.kp2[t];
 }
 
 __global__ void k_ex_counts(const ExBinD* __restrict__ bins, int64_t nb, uint32_t* __restrict__ c) {
@@ -362,23 +484,30 @@ __global__ void k_ex_counts(const ExBinD* __restrict__ bins, int64_t nb, uint32_
 // get_new_particles (exact :519-605, basic :564-611): each region's uniform
 // particles in region order from the one pcg32 stream, each at the first of
 // at most 100 candidate points (three numbers each) in the region's
-// material.  One block: its lanes take the next particles assuming each
-// accepts its first candidate (14 or 12 pcg32 steps apart); the particles
-// before the first lane whose candidate fails are final.  That particle's
-// later candidates are then tried side by side, the first that lands in the
-// material wins, and the stream continues after it.
+// material, followed by its energy and direction draws (3 numbers, or 4
+// for the basic cancelator).  One block.  A round fixes the stream's state
+// o at the next unresolved particle and tests, side by side, the candidate
+// that would start at each of the next kExWindow numbers; one lane then
+// walks that table as the reference's loop consumes the stream (a rejected
+// candidate moves on three numbers; an accepted one ends the particle,
+// whose energy and direction take the next numbers) and the particles it
+// resolved are written side by side.  The window holds 100 candidates of
+// the round's first particle, so every round resolves at least one.
+constexpr int kExWindow = 512;
+
 __global__ void __launch_bounds__(256) k_ex_new(DevProblem P, DevExact E,
                                                 const ExBinD* __restrict__ bins, int64_t nb,
                                                 const uint64_t* __restrict__ first_new,
                                                 uint64_t grng, abx_banked* __restrict__ out,
                                                 uint64_t* __restrict__ grng_out, int32_t* err) {
-  __shared__ int s_min;
+  __shared__ unsigned char s_acc[kExWindow];
+  __shared__ int s_cand[kExWindow / 6 + 1];  // candidate slot of each resolved particle
+  __shared__ int s_np, s_next, s_fail;
   TrkT<true> t;
   bind_lds_cursor(t, P);
   const int tid = threadIdx.x, nt = blockDim.x;
-  const uint64_t tail = ex_tail_steps(E);
-  const uint64_t pstep = 6u + tail;  // a particle whose first candidate is accepted
-  uint64_t o = grng;                 // every lane keeps the same copy
+  const int tail = (int)(ex_tail_steps(E) / 2u);  // numbers after the position
+  uint64_t o = grng;  // every lane keeps the same copy
   for (int64_t b = 0; b < nb; b++) {
     const ExBinD B = bins[b];
     const uint32_t N = B.n_new;
@@ -388,55 +517,42 @@ __global__ void __launch_bounds__(256) k_ex_new(DevProblem P, DevExact E,
     abx_banked* dst = out + first_new[b];
     uint32_t j0 = 0;
     while (j0 < N) {
-      const uint32_t cnt = N - j0 < (uint32_t)nt ? N - j0 : (uint32_t)nt;
-      uint64_t s = 0;
-      V3 r{0., 0., 0.};
-      bool acc = true;
-      if ((uint32_t)tid < cnt) {
-        s = pcg_advance(o, (uint64_t)tid * pstep);
+      for (int k = tid; k < kExWindow; k += nt) {
+        uint64_t s = pcg_advance(o, 2u * (uint64_t)k);
         const double x0 = rng_rand(s), x1 = rng_rand(s), x2 = rng_rand(s);
-        r = ex_point(E, K, x0, x1, x2);
-        acc = ex_material(P, t, r) == K.mat;
+        s_acc[k] = ex_material(P, t, ex_point(E, K, x0, x1, x2)) == K.mat ? 1 : 0;
       }
-      if (tid == 0) s_min = nt;
       __syncthreads();
-      if (!acc) atomicMin(&s_min, tid);
-      __syncthreads();
-      const uint32_t firstbad = (uint32_t)s_min;
-      __syncthreads();
-      const uint32_t done = firstbad < cnt ? firstbad : cnt;
-      if ((uint32_t)tid < done) {
-        abx_banked u{};
-        u.r[0] = r.x;
-        u.r[1] = r.y;
-        u.r[2] = r.z;
-        ex_new_tail(E, P, K, s, u);
-        u.wgt = w;
-        u.wgt2 = w2;
-        dst[j0 + tid] = u;
+      if (tid == 0) {
+        int q = 0, np = 0, fail = 0;
+        while (j0 + (uint32_t)np < N && q < kExWindow) {
+          int k = q, tries = 0;
+          while (tries < 100 && k < kExWindow && !s_acc[k]) {
+            k += 3;
+            tries++;
+          }
+          if (tries == 100) {
+            fail = 1;
+            break;
+          }
+          if (k >= kExWindow) break;  // its candidates run past the window: next round
+          s_cand[np++] = k;
+          q = k + 3 + tail;
+        }
+        s_np = np;
+        s_next = q;
+        s_fail = fail;
       }
-      o = pcg_advance(o, (uint64_t)done * pstep);
-      j0 += done;
-      if (done == cnt) continue;
-      // particle j0 failed its first candidate: candidates 1..99 side by side
-      bool hit = false;
-      if (tid >= 1 && tid < 100) {
-        s = pcg_advance(o, (uint64_t)tid * 6u);
-        const double x0 = rng_rand(s), x1 = rng_rand(s), x2 = rng_rand(s);
-        r = ex_point(E, K, x0, x1, x2);
-        hit = ex_material(P, t, r) == K.mat;
-      }
-      if (tid == 0) s_min = 100;
       __syncthreads();
-      if (hit) atomicMin(&s_min, tid);
-      __syncthreads();
-      const int k = s_min;
-      __syncthreads();
-      if (k >= 100) {  // "Couldn't sample position for uniform particle."
+      const int np = s_np, next = s_next;
+      if (s_fail && np == 0) {  // "Couldn't sample position for uniform particle."
         if (tid == 0) atomicCAS(err, 0, ABX_E_INVALID);
         return;
       }
-      if (tid == k) {
+      for (int i = tid; i < np; i += nt) {
+        uint64_t s = pcg_advance(o, 2u * (uint64_t)s_cand[i]);
+        const double x0 = rng_rand(s), x1 = rng_rand(s), x2 = rng_rand(s);
+        const V3 r = ex_point(E, K, x0, x1, x2);
         abx_banked u{};
         u.r[0] = r.x;
         u.r[1] = r.y;
@@ -444,10 +560,11 @@ __global__ void __launch_bounds__(256) k_ex_new(DevProblem P, DevExact E,
         ex_new_tail(E, P, K, s, u);
         u.wgt = w;
         u.wgt2 = w2;
-        dst[j0] = u;
+        dst[j0 + (uint32_t)i] = u;
       }
-      o = pcg_advance(o, (uint64_t)(k + 1) * 6u + tail);
-      j0 += 1;
+      o = pcg_advance(o, 2u * (uint64_t)next);
+      j0 += (uint32_t)np;
+      __syncthreads();  // s_acc and s_cand are rewritten by the next round
     }
   }
   if (tid == 0) *grng_out = o;

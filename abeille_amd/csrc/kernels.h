@@ -801,21 +801,35 @@ __global__ void k_fill(double* a, int64_t n, double v) {
 }
 
 // PowerIterator::normalize_weights partial sums (positive / negative weight)
+// Grid-stride over the bank, summed per block (waves, then LDS) and added
+// once per block: per-wave atomics on the two sums serialise at the memory.
 __global__ void k_weight_sums(const abx_banked* b, int64_t n, double* sums) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double part[2][16];
   double wp = 0., wn = 0.;
-  if (i < n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
     const double w = b[i].wgt;
     if (w > 0.)
-      wp = w;
+      wp += w;
     else
-      wn = -w;
+      wn -= w;
   }
   wp = wave_sum(wp);
   wn = wave_sum(wn);
+  const int wv = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
   if ((threadIdx.x & 63) == 0) {
-    if (wp != 0.) unsafeAtomicAdd(sums, wp);
-    if (wn != 0.) unsafeAtomicAdd(sums + 1, wn);
+    part[0][wv] = wp;
+    part[1][wv] = wn;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0., c = 0.;
+    for (int k = 0; k < nw; k++) {
+      a += part[0][k];
+      c += part[1][k];
+    }
+    if (a != 0.) unsafeAtomicAdd(sums, a);
+    if (c != 0.) unsafeAtomicAdd(sums + 1, c);
   }
 }
 

@@ -23,7 +23,9 @@ counter profile) and a CPU baseline (the CPU restatement under OpenMP on this
 host's cores, on a bounded sample of the same workload).
 
 Other workloads: --workload lattice|core|cube (the plain path alone),
-batch (512 slab problems), serve (the full served stream),
+batch (512 slab problems), serve (the full served stream), exact (exact
+regional cancellation of one generation's fission sites, GPU against the
+CPU restatement, and the device loop with it),
 --compare-precision (FP64 against split precision, per-bin deviations).
 """
 import argparse
@@ -399,6 +401,63 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_exact(args):
+    """Exact regional cancellation (ExactMGCancelator) of one k-eigenvalue
+    generation: the three-group fuel | moderator | fuel slab under carter
+    tracking (0.85 x the majorant, signed weights), --particles histories,
+    8 x 2 x 2 regions x 2 energy regions, 10 Sobol points per region.  The
+    same sites and parent records go through the GPU pass (abx_exact_cancel_bank:
+    host bank in and out, so PCIe included) and the CPU restatement (one
+    core); then the device-resident loop runs `--steps` generations with the
+    cancellation in it."""
+    from abeille_amd import Problem, problems, synchronize
+    from oracle.pyoracle import Oracle
+    n = int(args.particles)
+    m = problems.exact_slab(n)
+    m.exact_cancellation([-4., -5., -5.], [4., 5., 5.], [8, 2, 2], kind="exact",
+                         group_bins=[[0], [1, 2]])
+    g = Problem(m)
+    src = g.sample_sources(0, n)
+    r = g.transport(src, kcol=1.0, cap=4 * n + 4096)
+    fis, par = r["fission"], g.parents()
+    g.exact_cancel(fis, par)  # warm-up
+    reps = max(1, args.steps // 3)
+    synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g.exact_cancel(fis, par)
+    synchronize()
+    gpu_s = (time.perf_counter() - t0) / reps
+    g.rng_reset()  # the compared pass starts from settings::rng's seed, as the oracle's
+    out = g.exact_cancel(fis, par)
+    cpu = None
+    if not args.no_cpu:
+        o = Oracle(m)
+        t0 = time.perf_counter()
+        ref = o.exact_cancel(fis, par)
+        cpu_s = time.perf_counter() - t0
+        cpu = {"seconds": cpu_s, "cores": 1, "kind": "port",
+               "sites_per_s": len(fis) / cpu_s, "same_bytes_as_gpu": ref.tobytes() == out.tobytes()}
+    # the device loop with the cancellation in every generation
+    p = Problem(m)
+    t0 = time.perf_counter()
+    st, kh = p.run(args.steps, min(2, args.steps - 1))
+    synchronize()
+    loop_s = time.perf_counter() - t0
+    res = _common(args, len(fis) / gpu_s, "sites/s", 1e3 * gpu_s, "f64",
+                  {"workload": f"exact regional cancellation of one generation of the "
+                               f"3-group slab (carter 0.85), {n:.0e} histories, 8x2x2x2 "
+                               "regions, 10 Sobol points"},
+                  {"sites": int(len(fis)), "negative_sites": int((fis["wgt"] < 0).sum()),
+                   "uniform_particles": int(len(out) - len(fis)),
+                   "gpu_pass_s_pcie_inclusive": gpu_s, "cpu_baseline": cpu,
+                   "device_loop": {"generations": args.steps, "wall_s": loop_s,
+                                   "transport_s": st.transport_seconds,
+                                   "histories": int(st.histories),
+                                   "kcol_avg": st.kcol_avg}})
+    print(json.dumps(res), flush=True)
+
+
 def serve_legs(args, n_requests):
     """The served stream in short (the headline's latency legs): Poisson
     arrivals of `n_requests` requests of the served-stream generator, served
@@ -537,7 +596,7 @@ def main():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="headline",
-                    choices=["headline", "lattice", "core", "cube", "batch", "serve"])
+                    choices=["headline", "lattice", "core", "cube", "batch", "serve", "exact"])
     ap.add_argument("--chains", type=int, default=8,
                     help="headline: chains of the batched k-eigenvalue form (1..16)")
     ap.add_argument("--compare-precision", action="store_true",
@@ -590,6 +649,8 @@ def main():
         return bench_batch(args)
     if args.workload == "serve":
         return bench_serve(args)
+    if args.workload == "exact":
+        return bench_exact(args)
     from abeille_amd import Problem, abi, synchronize
     nparticles = int(args.particles)
     m, desc = workload(args.workload, nparticles, args.tracking, args.precision)
