@@ -233,9 +233,12 @@ ABX_HD double ex_c(double f, double f_inv) { return 1. / (2. * f * f_inv - 1.); 
 // pcg32 steps one uniform particle of get_new_particles takes after its
 // accepted position (three numbers): the exact cancelator draws the energy
 // and the direction's two angles; the basic one calls sample_fission, which
-// draws the energy, two angles and the delayed-neutron test.
+// draws the energy, two angles and the delayed-neutron test.  The energy
+// takes no number when it is a discrete draw over a single group
+// (std::discrete_distribution of one weight draws nothing).
 ABX_HD uint64_t ex_tail_steps(const DevExact& E) {
-  return E.kind == ABX_EXACT_MG ? 6u : 8u;
+  const uint64_t energy = ((E.kind == ABX_EXACT_MG && E.chi_matrix) || E.G >= 2) ? 1u : 0u;
+  return 2u * (energy + (E.kind == ABX_EXACT_MG ? 2u : 3u));
 }
 
 // Energy and direction of a uniform particle (get_new_particles, exact
@@ -271,6 +274,6 @@ ABX_HD void ex_new_tail(const DevExact& E, const DevProblem& P, const ExKeyD& K,
     u.u[0] = d.x;
     u.u[1] = d.y;
     u.u[2] = d.z;
-    (void)rng_rand(s);  // the delayed-neutron test (no delayed groups in the bank's E)
+    (void)rng_rand(s);  // the delayed-neutron test, rand() < Pdelayed = 0
   }
 }

@@ -484,8 +484,8 @@ __global__ void k_ex_counts(const ExBinD* __restrict__ bins, int64_t nb, uint32_
 // get_new_particles (exact :519-605, basic :564-611): each region's uniform
 // particles in region order from the one pcg32 stream, each at the first of
 // at most 100 candidate points (three numbers each) in the region's
-// material, followed by its energy and direction draws (3 numbers, or 4
-// for the basic cancelator).  One block.  A round fixes the stream's state
+// material, followed by its energy and direction draws (ex_tail_steps).
+// One block.  A round fixes the stream's state
 // o at the next unresolved particle and tests, side by side, the candidate
 // that would start at each of the next kExWindow numbers; one lane then
 // walks that table as the reference's loop consumes the stream (a rejected
@@ -501,7 +501,10 @@ __global__ void __launch_bounds__(256) k_ex_new(DevProblem P, DevExact E,
                                                 uint64_t grng, abx_banked* __restrict__ out,
                                                 uint64_t* __restrict__ grng_out, int32_t* err) {
   __shared__ unsigned char s_acc[kExWindow];
-  __shared__ int s_cand[kExWindow / 6 + 1];  // candidate slot of each resolved particle
+  // candidate slot of each resolved particle: a particle takes at least five
+  // numbers (three for its position, two or more after it)
+  constexpr int kCand = kExWindow / 5 + 1;
+  __shared__ int s_cand[kCand];
   __shared__ int s_np, s_next, s_fail;
   TrkT<true> t;
   bind_lds_cursor(t, P);
@@ -525,7 +528,7 @@ __global__ void __launch_bounds__(256) k_ex_new(DevProblem P, DevExact E,
       __syncthreads();
       if (tid == 0) {
         int q = 0, np = 0, fail = 0;
-        while (j0 + (uint32_t)np < N && q < kExWindow) {
+        while (j0 + (uint32_t)np < N && q < kExWindow && np < kCand) {
           int k = q, tries = 0;
           while (tries < 100 && k < kExWindow && !s_acc[k]) {
             k += 3;

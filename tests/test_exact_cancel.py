@@ -322,3 +322,39 @@ def test_gpu_rng_state_round_trip():
     g.set_rng_state(s0)
     b = g.exact_cancel(fis, par)
     assert a.tobytes() == b.tobytes() and g.rng_state() == s1
+
+
+def _one_group(kind, beta):
+    m, _ = problems.la13511("PUa-1-0-SL", 3000, tracking="carter")
+    m.set(sample_xs_ratio=[0.85])
+    m.exact_cancellation([-1.853722, -5., -5.], [1.853722, 5., 5.], [8, 1, 1], kind=kind,
+                         beta=beta)
+    return m
+
+
+def test_one_group_uniform_particles_draw_no_energy():
+    """With one group the uniform particles' energy is a discrete draw over
+    one weight, which takes no number: the pass still conserves weight and
+    every uniform particle sits at the group's midpoint."""
+    for kind, beta in (("exact", "average-g"), ("basic", "average-g")):
+        m = _one_group(kind, beta)
+        o = Oracle(m)
+        fis, par = generation(o, 3000)
+        out = o.exact_cancel(fis, par)
+        assert len(out) > len(fis)
+        assert out["wgt"].sum() == pytest.approx(fis["wgt"].sum(), abs=1e-10 * np.abs(fis["wgt"]).sum())
+        assert np.all(out["E"][len(fis):] == 0.5)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["exact", "basic"])
+def test_gpu_one_group_cancellation_matches_restatement(kind):
+    print(f"[exact] one group kind={kind}", flush=True)
+    m = _one_group(kind, "average-g")
+    o = Oracle(m)
+    g = Problem(m)
+    fis, par = generation(o, 3000)
+    for rep in range(2):
+        a = o.exact_cancel(fis, par)
+        b = g.exact_cancel(fis, par, cap=len(a) + 16)
+        assert a.tobytes() == b.tobytes(), f"call {rep}"
