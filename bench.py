@@ -597,7 +597,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="headline",
                     choices=["headline", "lattice", "core", "cube", "batch", "serve", "exact"])
-    ap.add_argument("--chains", type=int, default=8,
+    ap.add_argument("--chains", type=int, default=12,
                     help="headline: chains of the batched k-eigenvalue form (1..16)")
     ap.add_argument("--compare-precision", action="store_true",
                     help="run the workload in FP64 and in split precision and report the "
