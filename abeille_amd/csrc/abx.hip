@@ -2126,11 +2126,11 @@ struct abx_batch {
   hipStream_t aux[NS] = {};
   hipEvent_t ready = nullptr, done[NS] = {};
   int block = 256;  // lanes per block of the batch kernel (64, 128 or 256)
-  // A step whose launches repeat the previous step's (same problems, sizes,
-  // buffers and chunks; only the source histories' ids move on) replays the
-  // transport captured as a graph instead of launching it again.
+  // A step whose launches have the previous step's shape (kernel variants,
+  // chunk counts, block, device arrays) replays the transport captured as a
+  // graph instead of launching it again.
   bool graphs = true;
-  std::vector<unsigned char> last_sig;  // launch signature of the previous step
+  std::vector<int64_t> last_sig;  // launch signature of the previous step
   hipGraphExec_t gexec = nullptr;       // its captured transport, or none
   int64_t captures = 0, replays = 0;
   ~abx_batch() {
@@ -2373,21 +2373,18 @@ void batch_step(abx_batch* b, std::vector<abx_problem*>& finished) {
     }
     return hipSuccess;
   };
-  // the step's launch signature: everything the enqueued launches read
-  // from the host (the sources' history ids are sampled outside)
-  std::vector<unsigned char> sig;
-  auto put = [&sig](const void* d, size_t n) {
-    const unsigned char* c = (const unsigned char*)d;
-    sig.insert(sig.end(), c, c + n);
-  };
-  put(&ni, sizeof ni);
-  put(&b->block, sizeof b->block);
-  for (const Group& gr : groups) put(&gr, sizeof gr);
-  put(b->hp.data(), (size_t)ni * sizeof(DevProblem));
-  put(b->hk.data(), (size_t)ni * sizeof(KCtx));
-  put(b->all_chunks.data(), b->all_chunks.size() * sizeof(BatchChunk));
-  const void* bufs[] = {b->probs.p, b->ctxs.p, b->chunks.p, b->next.p, b->sec.p};
-  put(bufs, sizeof bufs);
+  // The step's launch signature: what the captured launches fix - the
+  // kernel variant, grid, block, LDS and chunk count of every group and the
+  // device arrays they read.  The problems, contexts and chunks themselves
+  // are uploaded into those arrays before every launch or replay.
+  std::vector<int64_t> sig;
+  sig.push_back(b->block);
+  sig.push_back((int64_t)groups.size());
+  for (const Group& gr : groups)
+    sig.insert(sig.end(), {(int64_t)gr.key, (int64_t)gr.g0, (int64_t)gr.cnt, (int64_t)gr.D,
+                           (int64_t)gr.L, (int64_t)gr.blob_max, gr.c0, gr.nc});
+  for (const void* q : {b->probs.p, b->ctxs.p, b->chunks.p, b->next.p, b->sec.p})
+    sig.push_back((int64_t)(uintptr_t)q);
   const bool repeat = b->graphs && sig == b->last_sig;
   if (!repeat && b->gexec) {
     HIPC(hipGraphExecDestroy(b->gexec));

@@ -676,11 +676,13 @@ int abx_batch_set_params(abx_batch* b, int64_t chunk, int64_t multi_hist);  /* b
 /* Lanes per block of the batch kernel: 64, 128 or 256 (default), for the
  * autotuner.  No counterpart in the reference. */
 int abx_batch_set_block_size(abx_batch* b, int block_size);
-/* A step whose launches repeat the previous step's - the same problems,
- * sizes, device buffers and chunks, as the steps of a fixed-source problem
- * with many batches do - replays the previous step's transport as a
- * captured hipGraph (on = 1, the default) instead of launching it again.
- * Results do not depend on it.  No counterpart in the reference. */
+/* A step whose launches have the previous step's shape - the same kernel
+ * variants, chunk counts per variant and device arrays, as the steps of a
+ * fixed-source problem with many batches do - replays the previous step's
+ * transport as a captured hipGraph (on = 1, the default) instead of
+ * launching it again; the problems, contexts and chunks are uploaded into
+ * those arrays before every step.  Results do not depend on it.  No
+ * counterpart in the reference. */
 int abx_batch_set_graphs(abx_batch* b, int on);
 /* Transport graphs captured and replayed by the batch so far. */
 int abx_batch_graph_stats(abx_batch* b, int64_t* captures, int64_t* replays);
