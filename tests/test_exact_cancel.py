@@ -358,3 +358,48 @@ def test_gpu_one_group_cancellation_matches_restatement(kind):
         a = o.exact_cancel(fis, par)
         b = g.exact_cancel(fis, par, cap=len(a) + 16)
         assert a.tobytes() == b.tobytes(), f"call {rep}"
+
+
+def _hex_exact(n):
+    m, _ = problems.hex_assembly(n, tracking="carter", top="flat", tally=False)
+    m.set(sample_xs_ratio=[0.9] * 7)
+    m.exact_cancellation([-5.54, -5.54, -10.], [5.54, 5.54, 10.], [4, 4, 2], kind="exact",
+                         group_bins=[[0, 1, 2], [3, 4, 5, 6]])
+    return m
+
+
+def test_device_parents_on_host_match_restatement_hex():
+    """Parent records through a hexagonal lattice (carter tracking, lattice
+    relocations, reflective faces) from the host build of the device code."""
+    m = _hex_exact(300)
+    o = Oracle(m)
+    h = HostDevice(m)
+    src = o.sample_sources(0, 300)
+    ro = o.transport(src, kcol=1.0, nthreads=1, cap=40000)
+    rh = h.transport(src, kcol=1.0, cap=40000)
+    assert ro["fission"].tobytes() == rh["fission"].tobytes()
+    assert o.parents().tobytes() == h.parents().tobytes()
+
+
+@pytest.mark.gpu
+def test_gpu_exact_cancellation_hex_lattice():
+    """Exact cancellation over a hexagonal assembly (regions split by fuel,
+    moderator and guide tube, located through the lattice): parent records,
+    one cancellation pass and a short run against the restatement."""
+    print("[exact] hex assembly", flush=True)
+    m = _hex_exact(3000)
+    o = Oracle(m)
+    g = Problem(m)
+    src = o.sample_sources(0, 3000)
+    ro = o.transport(src, kcol=1.0, nthreads=0, cap=200000)
+    rg = g.transport(src, kcol=1.0, cap=200000)
+    assert ro["fission"].tobytes() == rg["fission"].tobytes()
+    par = o.parents()
+    assert par.tobytes() == g.parents().tobytes()
+    a = o.exact_cancel(ro["fission"], par)
+    b = g.exact_cancel(ro["fission"], par, cap=len(a) + 16)
+    assert len(a) > len(ro["fission"])
+    assert a.tobytes() == b.tobytes()
+    so, ko = Oracle(m).run(4, 1, nthreads=1)
+    sg, kg = Problem(m).run(4, 1)
+    np.testing.assert_allclose(kg, ko, rtol=1e-9)
