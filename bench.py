@@ -22,7 +22,7 @@ per launch / HIP-event launch time, HBM-bound; issue rate from the committed
 counter profile) and a CPU baseline (the CPU restatement under OpenMP on this
 host's cores, on a bounded sample of the same workload).
 
-Other workloads: --workload lattice|core|cube (the plain path alone),
+Other workloads: --workload lattice|core|cube|hex (the plain path alone),
 batch (512 slab problems), serve (the full served stream), exact (exact
 regional cancellation of one generation's fission sites, GPU against the
 CPU restatement, and the device loop with it),
@@ -66,6 +66,10 @@ def workload(name, nparticles, tracking, precision):
         m = problems.fixed_source_cube(nparticles, tracking=tracking)
         desc = (f"20 cm vacuum cube fixed source, 2 groups, {tracking} tracking, "
                 f"{nparticles:.0e} histories/batch, collision tally 20^3x2")
+    elif name == "hex":
+        m, _ = problems.hex_assembly(nparticles, tracking=tracking, top="pointy", rings=9, nz=2)
+        desc = (f"hexagonal assembly k-eigenvalue (9 rings, 217 pins, 2 layers), 7 groups, "
+                f"{tracking} tracking, {nparticles:.0e} particles/generation, TLE 24x24x2x7")
     else:
         raise SystemExit(f"unknown workload {name}")
     if precision == "split":
@@ -596,7 +600,8 @@ def main():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="headline",
-                    choices=["headline", "lattice", "core", "cube", "batch", "serve", "exact"])
+                    choices=["headline", "lattice", "core", "cube", "hex", "batch", "serve",
+                             "exact"])
     ap.add_argument("--chains", type=int, default=12,
                     help="headline: chains of the batched k-eigenvalue form (1..16)")
     ap.add_argument("--compare-precision", action="store_true",
