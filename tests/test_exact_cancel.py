@@ -403,3 +403,30 @@ def test_gpu_exact_cancellation_hex_lattice():
     so, ko = Oracle(m).run(4, 1, nthreads=1)
     sg, kg = Problem(m).run(4, 1)
     np.testing.assert_allclose(kg, ko, rtol=1e-9)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["basic_avg_g", "basic_min", "basic_avg_f", "exact"])
+def test_gpu_cancellation_stops_match_restatement(name):
+    """A zero weight ends the basic cancelator's pass through a region
+    (cancel_bin returns, basic :438-468) and a non-finite P_p or P_u ends
+    either cancelator's: the second weights here have zeros sprinkled among
+    both signs, so several regions stop part-way."""
+    print(f"[exact] stops config={name}", flush=True)
+    m = exact_model(name)
+    o = Oracle(m)
+    g = Problem(m)
+    fis, par = generation(o, 3000)
+    fis = fis.copy()
+    rng = np.random.default_rng(17)
+    w2 = rng.normal(0.0, 1.0, len(fis))
+    w2[rng.random(len(fis)) < 0.03] = 0.0
+    fis["wgt2"] = w2
+    a = o.exact_cancel(fis, par)
+    b = g.exact_cancel(fis, par, cap=len(a) + 16)
+    assert a.tobytes() == b.tobytes()
+    if name.startswith("basic"):
+        # some regions kept sites after a zero untouched: not every
+        # mixed-sign region changed all its second weights
+        n = len(fis)
+        assert np.any(a["wgt2"][:n] == fis["wgt2"]) and np.any(a["wgt2"][:n] != fis["wgt2"])
