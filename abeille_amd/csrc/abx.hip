@@ -98,7 +98,7 @@ struct abx_problem {
   bool cancel = false;
   int batch_stage = -1;  // batch kernel: tables staged in LDS pay off (-1 = not yet known)
   DevCancel cm{};
-  DBuf ckeys, ckeys2, ctemp;
+  DBuf ckeys, ckeys2, ctemp, cw;  // sort keys, sort scratch, weights in sorted order
   // exact cancellation: configuration, the parent record of each staged and
   // ordered fission site, the pipeline's buffers, and settings::rng
   int exact = 0;
@@ -152,7 +152,7 @@ struct abx_problem {
     DBuf* all[] = {&tally_gen, &tally_avg, &tally_var, &tally_more, &in, &sec, &stage, &stage_parent,
                    &stage_daughter, &n_stage, &daughters, &offsets, &block_sums, &block_sums2,
                    &ordered, &scores, &counters, &err, &err_hist, &trace, &n_rec, &wsum,
-                   &bank_a, &bank_b, &ckeys, &ckeys2, &ctemp, &pool_next, &blob, &gdev,
+                   &bank_a, &bank_b, &ckeys, &ckeys2, &ctemp, &cw, &pool_next, &blob, &gdev,
                    &khist_dev, &stage_pinfo, &ordered_pinfo, &ex_head, &ex_offs, &ex_bins,
                    &ex_smp, &ex_fv, &ex_fi, &ex_cnt, &ex_state};
     for (auto e : gen_events) (void)hipEventDestroy(e);
@@ -207,8 +207,15 @@ void cancel_device(abx_problem* p, abx_banked* d_bank, int64_t n) {
   p->ctemp.reserve(std::max<size_t>(temp, 16));
   HIPC(rocprim::radix_sort_keys(p->ctemp.p, temp, p->ckeys.as<uint64_t>(), p->ckeys2.as<uint64_t>(),
                                 (size_t)n, 0, 64, p->stream));
-  hipLaunchKernelGGL(k_cancel_apply, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0, p->stream,
-                     p->ckeys2.as<uint64_t>(), n, d_bank);
+  const uint64_t* keys = p->ckeys2.as<uint64_t>();
+  p->cw.reserve((size_t)n * 2 * sizeof(double));
+  double* w = p->cw.as<double>();
+  hipLaunchKernelGGL(k_cancel_gather, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0, p->stream,
+                     (const abx_banked*)d_bank, keys, n, w, w + n);
+  HIPC(hipGetLastError());
+  const int64_t chunks = (n + kCancelChunk - 1) / kCancelChunk;
+  hipLaunchKernelGGL(k_cancel_bins, dim3((unsigned)std::min<int64_t>(chunks, 8192)), dim3(64), 0,
+                     p->stream, keys, n, (const double*)w, (const double*)(w + n), d_bank);
   HIPC(hipGetLastError());
 }
 

@@ -63,51 +63,6 @@ struct ExSiteArrays {
 __device__ __forceinline__ uint64_t ex_reg_of(uint64_t key) { return key >> 32; }
 __device__ __forceinline__ uint32_t ex_site_of(uint64_t key) { return (uint32_t)key; }
 
-constexpr int kExPer = 8;                 // values per lane and block
-constexpr int kExBlock = 64 * kExPer;     // values a wave stages at a time
-
-// Sums a[k][s0 .. s1) for k < K in index order, one addition after
-// another (every lane of the wave gets the sums).  Region kernels run one
-// wave per block.  The wave stages 512 consecutive values of each array in
-// LDS (coalesced, the next block's loads in flight meanwhile) and every lane
-// walks them from LDS with broadcast reads; independent sums interleave.
-template <int K>
-__device__ void ex_seq_sums(const double* const* a, int64_t s0, int64_t s1, double* acc) {
-  __shared__ double buf[K][kExBlock];
-  const int lane = threadIdx.x & 63;
-  double v[K][kExPer];
-  auto fetch = [&](int64_t c) {
-#pragma unroll
-    for (int k = 0; k < K; k++)
-#pragma unroll
-      for (int j = 0; j < kExPer; j++) {
-        const int64_t t = c + j * 64 + lane;
-        v[k][j] = t < s1 ? a[k][t] : 0.;
-      }
-  };
-  if (s0 < s1) fetch(s0);
-  for (int64_t c = s0; c < s1; c += kExBlock) {
-    __syncthreads();  // the previous block's walk is over
-#pragma unroll
-    for (int k = 0; k < K; k++)
-#pragma unroll
-      for (int j = 0; j < kExPer; j++) buf[k][j * 64 + lane] = v[k][j];
-    __syncthreads();
-    if (c + kExBlock < s1) fetch(c + kExBlock);
-    const int nblk = (int)(s1 - c < kExBlock ? s1 - c : kExBlock);
-    int i = 0;
-    for (; i + 8 <= nblk; i += 8) {
-#pragma unroll
-      for (int u = 0; u < 8; u++)
-#pragma unroll
-        for (int k = 0; k < K; k++) acc[k] += buf[k][i + u];
-    }
-    for (; i < nblk; i++)
-#pragma unroll
-      for (int k = 0; k < K; k++) acc[k] += buf[k][i];
-  }
-}
-
 __global__ void k_ex_keys(DevProblem P, DevExact E, const abx_banked* __restrict__ bank,
                           const abx_parent_info* __restrict__ par, int64_t n,
                           uint64_t* __restrict__ keys, int32_t* err) {

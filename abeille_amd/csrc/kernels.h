@@ -858,11 +858,58 @@ __global__ void k_rebank(const abx_banked* b, int64_t n, double w_per_part, uint
   out[i] = p;
 }
 
+constexpr int kExPer = 8;                 // values per lane and block
+constexpr int kExBlock = 64 * kExPer;     // values a wave stages at a time
+
+// Sums a[k][s0 .. s1) for k < K in index order, one addition after
+// another (every lane of the wave gets the sums).  Region kernels run one
+// wave per block.  The wave stages 512 consecutive values of each array in
+// LDS (coalesced, the next block's loads in flight meanwhile) and every lane
+// walks them from LDS with broadcast reads; independent sums interleave.
+template <int K>
+__device__ void ex_seq_sums(const double* const* a, int64_t s0, int64_t s1, double* acc) {
+  __shared__ double buf[K][kExBlock];
+  const int lane = threadIdx.x & 63;
+  double v[K][kExPer];
+  auto fetch = [&](int64_t c) {
+#pragma unroll
+    for (int k = 0; k < K; k++)
+#pragma unroll
+      for (int j = 0; j < kExPer; j++) {
+        const int64_t t = c + j * 64 + lane;
+        v[k][j] = t < s1 ? a[k][t] : 0.;
+      }
+  };
+  if (s0 < s1) fetch(s0);
+  for (int64_t c = s0; c < s1; c += kExBlock) {
+    __syncthreads();  // the previous block's walk is over
+#pragma unroll
+    for (int k = 0; k < K; k++)
+#pragma unroll
+      for (int j = 0; j < kExPer; j++) buf[k][j * 64 + lane] = v[k][j];
+    __syncthreads();
+    if (c + kExBlock < s1) fetch(c + kExBlock);
+    const int nblk = (int)(s1 - c < kExBlock ? s1 - c : kExBlock);
+    int i = 0;
+    for (; i + 8 <= nblk; i += 8) {
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+#pragma unroll
+        for (int k = 0; k < K; k++) acc[k] += buf[k][i + u];
+    }
+    for (; i < nblk; i++)
+#pragma unroll
+      for (int k = 0; k < K; k++) acc[k] += buf[k][i];
+  }
+}
+
 // ---------------------------------------------------------- cancellation
 // ApproximateMeshCancelator (src/approximate_mesh_cancelator.cpp:97-190) on a
-// device bank: key = (bin << 32) | index, sorted, then one lane per bin sums
-// the bin in bank order and, if the bin is mixed-sign with more than one
-// particle, writes the mean back.
+// device bank: key = (bin << 32) | index, sorted (each bin's sites in bank
+// order), the weights gathered into sorted order, then one wave per bin
+// finds the bin's signs and, if the bin holds more than one particle and
+// both signs of wgt (wgt2), sums the weights in bank order, one addition
+// after another as the reference does, and writes the mean back.
 struct DevCancel {
   double lx, ly, lz, dx, dy, dz;
   int32_t nx, ny, nz, ne;  // ne = number of energy bins (1 without edges)
@@ -893,37 +940,83 @@ __global__ void k_cancel_keys(const abx_banked* b, int64_t n, DevCancel C, uint6
   keys[t] = (key << 32) | (uint64_t)t;
 }
 
-__global__ void k_cancel_apply(const uint64_t* keys, int64_t n, abx_banked* b) {
+// Sorted position t: the site's weights in sorted order.
+__global__ void k_cancel_gather(const abx_banked* __restrict__ b, const uint64_t* __restrict__ keys,
+                                int64_t n, double* __restrict__ w, double* __restrict__ w2) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  const uint64_t bin = keys[t] >> 32;
-  if (bin == 0xffffffffull) return;
-  if (t > 0 && (keys[t - 1] >> 32) == bin) return;  // not the first of its bin
-  int64_t e = t;
-  bool p1 = false, n1 = false, p2 = false, n2 = false;
-  double s1 = 0., s2 = 0.;
-  while (e < n && (keys[e] >> 32) == bin) {
-    const abx_banked& q = b[keys[e] & 0xffffffffull];
-    if (q.wgt > 0.)
-      p1 = true;
-    else if (q.wgt < 0.)
-      n1 = true;
-    s1 += q.wgt;
-    if (q.wgt2 > 0.)
-      p2 = true;
-    else if (q.wgt2 < 0.)
-      n2 = true;
-    s2 += q.wgt2;
-    e++;
-  }
-  const int64_t cnt = e - t;
-  if (cnt <= 1) return;
-  const double N = (double)cnt;
-  const double a1 = s1 / N, a2 = s2 / N;
-  for (int64_t f = t; f < e; f++) {
-    abx_banked& q = b[keys[f] & 0xffffffffull];
-    if (p1 && n1) q.wgt = a1;
-    if (p2 && n2) q.wgt2 = a2;
+  const uint64_t key = keys[t];
+  if ((key >> 32) == 0xffffffffull) return;  // outside the mesh: never read
+  const abx_banked& q = b[key & 0xffffffffull];
+  w[t] = q.wgt;
+  w2[t] = q.wgt2;
+}
+
+constexpr int kCancelChunk = 512;  // sorted positions whose bin heads a block takes
+
+// One wave per block.  A block takes the bins that start in its chunks of
+// sorted positions (grid-stride), finds each bin's end, its signs, and, for
+// a mixed bin of more than one site, its means, then writes them to the
+// bin's sites.  Every bin is read and written by one block only, and the
+// bank is not read here, so blocks never race.
+__global__ void __launch_bounds__(64) k_cancel_bins(const uint64_t* __restrict__ keys, int64_t n,
+                                                   const double* __restrict__ w,
+                                                   const double* __restrict__ w2,
+                                                   abx_banked* __restrict__ b) {
+  const int lane = threadIdx.x;
+  for (int64_t c0 = (int64_t)blockIdx.x * kCancelChunk; c0 < n;
+       c0 += (int64_t)gridDim.x * kCancelChunk) {
+    const int64_t c1 = c0 + kCancelChunk < n ? c0 + kCancelChunk : n;
+    for (int64_t base = c0; base < c1; base += 64) {
+      const int64_t t = base + lane;
+      bool head = false;
+      if (t < c1) {
+        const uint64_t bin = keys[t] >> 32;
+        head = bin != 0xffffffffull && (t == 0 || (keys[t - 1] >> 32) != bin);
+      }
+      uint64_t heads = __ballot(head);
+      while (heads) {
+        const int64_t h = base + (__ffsll((unsigned long long)heads) - 1);
+        heads &= heads - 1;
+        const uint64_t bin = keys[h] >> 32;
+        int64_t e = h + 1;  // the bin's end: the first later position of another bin
+        for (;;) {
+          const int64_t u = e + lane;
+          const uint64_t out = __ballot(!(u < n && (keys[u] >> 32) == bin));
+          if (out) {
+            e += __ffsll((unsigned long long)out) - 1;
+            break;
+          }
+          e += 64;
+        }
+        if (e - h <= 1) continue;
+        uint32_t s = 0u;
+        for (int64_t u = h + lane; u < e; u += 64) {
+          const double x = w[u], y = w2[u];
+          if (x > 0.)
+            s |= 1u;
+          else if (x < 0.)
+            s |= 2u;
+          if (y > 0.)
+            s |= 4u;
+          else if (y < 0.)
+            s |= 8u;
+        }
+        const bool m1 = __ballot(s & 1u) != 0 && __ballot(s & 2u) != 0;
+        const bool m2 = __ballot(s & 4u) != 0 && __ballot(s & 8u) != 0;
+        if (!m1 && !m2) continue;
+        double acc[2] = {0., 0.};
+        const double* a[2] = {w, w2};
+        ex_seq_sums<2>(a, h, e, acc);
+        const double N = (double)(e - h);
+        const double a1 = acc[0] / N, a2 = acc[1] / N;
+        for (int64_t u = h + lane; u < e; u += 64) {
+          abx_banked& q = b[keys[u] & 0xffffffffull];
+          if (m1) q.wgt = a1;
+          if (m2) q.wgt2 = a2;
+        }
+      }
+    }
   }
 }
 

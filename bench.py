@@ -462,6 +462,55 @@ def bench_exact(args):
     print(json.dumps(res), flush=True)
 
 
+def bench_cancel(args):
+    """Approximate mesh cancellation (ApproximateMeshCancelator) of a
+    synthetic bank of 2e6 sites (positions uniform in [-12, 12]^3, part of
+    them outside the [-10, 10]^3 mesh; signed weights; wgt2 nonzero for 30%)
+    on three meshes: one bin, 4 x 4 x 4 x 2 energy bins, 20 x 20 x 20.  Host
+    bank in and out (abx_cancel_bank), so the pass time includes PCIe; the
+    kernels' own durations come from a kernel trace of this command.  The
+    CPU restatement (one core) cancels the same bank on every mesh."""
+    import numpy as np
+    from abeille_amd import Problem, abi, problems, synchronize
+    n = 2_000_000
+    rng = np.random.default_rng(5)
+    b = np.zeros(n, abi.BANKED_DTYPE)
+    b["r"] = rng.uniform(-12, 12, (n, 3))
+    b["E"] = rng.choice([0.5, 1.5], n)
+    b["wgt"] = rng.choice([-1.0, 1.0], n) * rng.uniform(0.5, 1.5, n)
+    b["wgt2"] = np.where(rng.random(n) < 0.3, rng.normal(size=n), 0.0)
+    b["parent_history_id"] = np.arange(n)
+    meshes = {"1x1x1": ([1, 1, 1], None), "4x4x4x2": ([4, 4, 4], [0.0, 1.0, 2.0]),
+              "20x20x20": ([20, 20, 20], None)}
+    legs = {}
+    for name, (shape, eb) in meshes.items():
+        m = problems.fixed_source_cube(100)
+        m.cancellation_mesh([-10, -10, -10], [10, 10, 10], shape, energy_bounds=eb)
+        g = Problem(m)
+        out = g.cancel_bank(b)  # warm-up
+        reps = max(1, args.steps)
+        synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            out = g.cancel_bank(b)
+        synchronize()
+        leg = {"gpu_pass_s_pcie_inclusive": (time.perf_counter() - t0) / reps}
+        if not args.no_cpu:
+            from oracle.pyoracle import Oracle
+            t0 = time.perf_counter()
+            ref = Oracle(m).cancel(b)
+            leg["cpu_s"] = time.perf_counter() - t0
+            leg["same_bytes_as_gpu"] = ref.tobytes() == out.tobytes()
+        legs[name] = leg
+    gpu_s = legs["4x4x4x2"]["gpu_pass_s_pcie_inclusive"]
+    res = _common(args, n / gpu_s, "sites/s", 1e3 * gpu_s, "f64",
+                  {"workload": "approximate mesh cancellation of 2e6 banked sites, "
+                               "4x4x4 mesh x 2 energy bins (value); 1x1x1 and 20x20x20 "
+                               "in meshes"},
+                  {"meshes": legs, "cpu_baseline": None})
+    print(json.dumps(res), flush=True)
+
+
 def serve_legs(args, n_requests):
     """The served stream in short (the headline's latency legs): Poisson
     arrivals of `n_requests` requests of the served-stream generator, served
@@ -601,7 +650,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="headline",
                     choices=["headline", "lattice", "core", "cube", "hex", "batch", "serve",
-                             "exact"])
+                             "exact", "cancel"])
     ap.add_argument("--chains", type=int, default=12,
                     help="headline: chains of the batched k-eigenvalue form (1..16)")
     ap.add_argument("--compare-precision", action="store_true",
@@ -656,6 +705,8 @@ def main():
         return bench_serve(args)
     if args.workload == "exact":
         return bench_exact(args)
+    if args.workload == "cancel":
+        return bench_cancel(args)
     from abeille_amd import Problem, abi, synchronize
     nparticles = int(args.particles)
     m, desc = workload(args.workload, nparticles, args.tracking, args.precision)

@@ -105,6 +105,34 @@ def test_cancellation_bit_exact():
     assert not np.array_equal(a_o["wgt"], b["wgt"])
 
 
+@pytest.mark.parametrize("case", ["one_bin", "fine", "same_sign", "tiny", "ragged"])
+def test_cancellation_bit_exact_shapes(case):
+    """The wave-per-bin cancellation against the CPU restatement on bins
+    that span many blocks of sorted positions (one bin for the whole mesh),
+    bins of one site (a fine mesh), bins of one sign (nothing changes),
+    banks smaller than a wave, and a ragged size."""
+    print(f"[cancel] case={case}", flush=True)
+    from test_cancellation import make_bank
+    m = problems.fixed_source_cube(100)
+    shape = {"one_bin": [1, 1, 1], "fine": [60, 60, 60]}.get(case, [3, 5, 2])
+    m.cancellation_mesh([-10, -10, -10], [10, 10, 10], shape)
+    o = Oracle(m)
+    g = Problem(m)
+    n = {"one_bin": 150000, "fine": 120000, "tiny": 37, "ragged": 70001}.get(case, 50000)
+    b = make_bank(n, np.random.default_rng(23))
+    if case == "same_sign":
+        b["wgt"] = np.abs(b["wgt"])
+        b["wgt2"] = -np.abs(b["wgt2"])
+    for _ in range(2):  # the second call reuses the buffers of the first
+        a_o = o.cancel(b)
+        a_g = g.cancel_bank(b)
+        assert a_o.tobytes() == a_g.tobytes()
+    if case == "same_sign":
+        assert a_g.tobytes() == b.tobytes()
+    elif case != "tiny":
+        assert not np.array_equal(a_g["wgt"], b["wgt"])
+
+
 RUNS = {
     "lattice_dt": (lambda: problems.pin_lattice_2d(4000, tracking="delta"), 4, 1),
     "slab_PUa_1_0_SL": (lambda: problems.la13511("PUa-1-0-SL", 4000)[0], 4, 2),
