@@ -193,6 +193,17 @@ uint64_t device_scan(abx_problem* p, const uint32_t* in, uint64_t* out, int64_t 
   return acc;
 }
 
+// Sort keys are (bin << 32) | bank index, written in index order, with bins
+// below nbins and 0xffffffff for no bin.  Sorting on bits [32, end) with
+// 2^(end - 32) > nbins orders the bins and puts no-bin last (its low bits
+// are all ones, above every bin); the radix sort is stable, so each bin's
+// sites stay in bank order without sorting the index bits.
+static int bin_key_end_bit(uint64_t nbins) {
+  int b = 1;
+  while (b < 32 && (1ull << b) <= nbins) b++;
+  return 32 + b;
+}
+
 // Approximate mesh cancellation of a device bank (in place).
 void cancel_device(abx_problem* p, abx_banked* d_bank, int64_t n) {
   if (n <= 1) return;
@@ -201,12 +212,14 @@ void cancel_device(abx_problem* p, abx_banked* d_bank, int64_t n) {
   hipLaunchKernelGGL(k_cancel_keys, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0, p->stream,
                      d_bank, n, p->cm, p->ckeys.as<uint64_t>());
   HIPC(hipGetLastError());
+  const int end_bit = bin_key_end_bit((uint64_t)p->cm.nx * (uint64_t)p->cm.ny * (uint64_t)p->cm.nz *
+                                      (uint64_t)p->cm.ne);
   size_t temp = 0;
   HIPC(rocprim::radix_sort_keys(nullptr, temp, p->ckeys.as<uint64_t>(), p->ckeys2.as<uint64_t>(),
-                                (size_t)n, 0, 64, p->stream));
+                                (size_t)n, 32, end_bit, p->stream));
   p->ctemp.reserve(std::max<size_t>(temp, 16));
   HIPC(rocprim::radix_sort_keys(p->ctemp.p, temp, p->ckeys.as<uint64_t>(), p->ckeys2.as<uint64_t>(),
-                                (size_t)n, 0, 64, p->stream));
+                                (size_t)n, 32, end_bit, p->stream));
   const uint64_t* keys = p->ckeys2.as<uint64_t>();
   p->cw.reserve((size_t)n * 2 * sizeof(double));
   double* w = p->cw.as<double>();
@@ -951,12 +964,15 @@ static int64_t exact_cancel_device(abx_problem* p, int64_t n) {
   hipLaunchKernelGGL(k_ex_keys, dim3((unsigned)blocks_for(n, 256)), dim3(256), lds, s, p->P, E,
                      (const abx_banked*)bank, par, n, p->ckeys.as<uint64_t>(), p->err.as<int32_t>());
   HIPC(hipGetLastError());
+  const int end_bit = bin_key_end_bit((uint64_t)(E.S[0] + 1) * (uint64_t)(E.S[1] + 1) *
+                                      (uint64_t)(E.S[2] + 1) * (uint64_t)(E.S[3] + 1) *
+                                      (uint64_t)E.nmat);
   size_t temp = 0;
   HIPC(rocprim::radix_sort_keys(nullptr, temp, p->ckeys.as<uint64_t>(), p->ckeys2.as<uint64_t>(),
-                                (size_t)n, 0, 64, s));
+                                (size_t)n, 32, end_bit, s));
   p->ctemp.reserve(std::max<size_t>(temp, 16));
   HIPC(rocprim::radix_sort_keys(p->ctemp.p, temp, p->ckeys.as<uint64_t>(), p->ckeys2.as<uint64_t>(),
-                                (size_t)n, 0, 64, s));
+                                (size_t)n, 32, end_bit, s));
   const uint64_t* keys = p->ckeys2.as<uint64_t>();
   hipLaunchKernelGGL(k_ex_heads, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0, s, keys, n,
                      p->ex_head.as<uint32_t>());
